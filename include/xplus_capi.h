@@ -1,0 +1,219 @@
+/*
+ * xplus_capi.h -- C-ABI of the MI355X Jacobi-PCG library (libxplus_amd.so).
+ *
+ * Plain C types only: pointers, sizes, ints, doubles.  No HIP or torch types
+ * cross this boundary (a stream is passed as `void*` holding a hipStream_t,
+ * NULL = the library's own stream).
+ *
+ * Three layers:
+ *   1. The reference's kernel call surface, same names and signatures.
+ *   2. Device-resident matrices and stream-ordered operations (xp_*).
+ *   3. The solver: graph-captured Jacobi-PCG, batched systems, FP64
+ *      refinement, status and trace readback.
+ * Plus host helpers (dataset loader, generators) used by the drivers.
+ *
+ * Unless stated otherwise an xp_* function returns 0 on success and a
+ * negative code on failure; xp_last_error() then returns the message.
+ */
+#ifndef XPLUS_CAPI_H
+#define XPLUS_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== *
+ * 1. Drop-in kernel call surface.
+ *
+ * Replaces archived/include/cg_kernels.hpp:9-54 (bodies
+ * archived/kernels/cg_kernels.cpp:43-158): host `double*` buffers owned by
+ * the caller, synchronous (each call has finished when it returns), no
+ * return code, a silent no-op for n < 0.  Differences, by design:
+ *   - no n <= 1024 (kMaxN) limit (archived/kernels/cg_kernels.cpp:9-12);
+ *   - the work runs on the GPU in FP64 (exact FP64 values, FP64 vectors),
+ *     reductions in a fixed parallel order instead of a sequential loop;
+ *   - a malformed CSR (which the reference would read out of bounds) makes
+ *     the call a no-op and records the reason for xp_last_error().
+ * The SpMV caches the device copy of the last matrix seen, keyed by the
+ * three array pointers, n, nnz and a 64-bit hash of the arrays' contents.
+ * ===================================================================== */
+
+/* y = A x, CSR.  archived/include/cg_kernels.hpp:11-16 */
+void spmv_csr_kernel(const int* row_ptr, const int* col_idx, const double* values,
+                     const double* x, double* y, int n);
+/* Same contract; the reference's local version is CSR as well
+ * (archived/kernels/cg_kernels.cpp:58-71).  cg_kernels.hpp:18-23 */
+void spmv_blocked_kernel(const int* row_ptr, const int* col_idx, const double* values,
+                         const double* x, double* y, int n);
+/* r = b - ax; z = m_inv r; p = z; metrics = {r^T z, r^T r}.  cg_kernels.hpp:25-32 */
+void init_pcg_kernel(const double* b, const double* ax, const double* m_inv, double* r,
+                     double* z, double* p, double* metrics, int n);
+/* out[0] = a^T b.  cg_kernels.hpp:34-37 */
+void dot_kernel(const double* a, const double* b, double* out, int n);
+/* x += alpha p; r -= alpha ap; z = m_inv r; metrics = {r^T z, r^T r}.  cg_kernels.hpp:39-47 */
+void update_xrz_kernel(double* x, const double* p, double* r, const double* ap,
+                       const double* m_inv, double* z, double* metrics, double alpha, int n);
+/* p = z + beta p.  cg_kernels.hpp:49-52 */
+void update_p_kernel(const double* z, double* p, double beta, int n);
+
+/* Drops the cached device matrix of the drop-in SpMV (also done
+ * automatically when the content hash changes). */
+void xp_legacy_reset(void);
+
+/* ===================================================================== *
+ * Errors and device.
+ * ===================================================================== */
+/* Copies the calling thread's last error message; returns its length. */
+int xp_last_error(char* buf, int len);
+/* Number of visible GPUs (0 without one). */
+int xp_device_count(void);
+
+/* ===================================================================== *
+ * 2. Device-resident SELL-C-sigma matrices.
+ *
+ * The matrix is converted once (host) and kept on the device in "solve
+ * order": rows sorted by length inside windows of `sigma` rows, applied as
+ * a symmetric permutation and padded to n_pad rows.  Vectors passed to the
+ * stream-ordered operations below are device pointers in solve order of
+ * length n_pad; xp_to_solve_order / xp_from_solve_order convert.
+ * ===================================================================== */
+typedef struct xp_matrix xp_matrix;
+
+typedef struct {
+    int sigma;   /* sort window in rows, multiple of 64; 0 = library default */
+    int keep_lo; /* keep the FP32 low-part stream (FP64-accurate SpMV, refinement) */
+    int keep64;  /* keep exact FP64 values (drop-in FP64 SpMV) */
+    int grid;    /* workgroups of the streaming kernels; 0 = library default */
+} xp_matrix_opts;
+
+typedef struct {
+    int n, n_pad, nslices, nsys, sigma, idx16, grid;
+    long long nnz;          /* entries of the CSR input */
+    long long stored;       /* stored SELL entries incl. padding */
+    long long bytes_spmv;   /* HBM bytes one FP32 SpMV streams (values, columns, x, y) */
+} xp_matrix_info;
+
+/* One system (batch of 1).  Host CSR arrays; values are FP64. */
+int xp_matrix_create(int n, const int* row_ptr, const int* col_idx, const double* values,
+                     const xp_matrix_opts* opts, xp_matrix** out);
+/* A batch of independent systems solved together (block-diagonal).  The
+ * arrays are the systems' CSR arrays concatenated: row_ptrs holds
+ * sum(ns[k] + 1) entries (each system's own row_ptr, starting at 0),
+ * col_idx/values sum(nnz_k) entries with system-local column indices. */
+int xp_matrix_create_batch(int nsys, const int* ns, const int* row_ptrs, const int* col_idx,
+                           const double* values, const xp_matrix_opts* opts, xp_matrix** out);
+void xp_matrix_destroy(xp_matrix* m);
+int xp_matrix_get_info(const xp_matrix* m, xp_matrix_info* out);
+/* perm[i] = natural (global) row of solve row i, -1 for padding; n_pad entries. */
+int xp_matrix_permutation(const xp_matrix* m, int* perm);
+/* Changes the workgroup count of the streaming kernels (autotuning). */
+int xp_matrix_set_grid(xp_matrix* m, int grid);
+
+/* y = A x, FP32 values and vectors, FP32 accumulation. */
+int xp_spmv_f32(xp_matrix* m, const float* x, float* y, void* stream);
+/* y = A x in FP64: exact values if keep64, else hi+lo pairs (needs keep_lo). */
+int xp_spmv_f64(xp_matrix* m, const double* x, double* y, void* stream);
+/* natural (n, FP64) -> solve order (n_pad, FP32), and back. */
+int xp_to_solve_order(xp_matrix* m, const double* nat, float* solve, void* stream);
+int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* stream);
+/* Host-pointer convenience: y = A x in FP64 on the GPU for a CSR given once. */
+int xp_csr_spmv_f64(int n, const int* row_ptr, const int* col_idx, const double* values,
+                    const double* x, double* y);
+
+/* ===================================================================== *
+ * 3. Solver.
+ *
+ * Jacobi-PCG with the semantics of the reference's on-chip loop
+ * (kernels/cuper_pcg_control_kernel.cpp:1069-1234) and golden
+ * (src/CgSolverGolden.hpp:42-124): stop when r^T r <= tau on the
+ * recursively updated residual; breakdown on non-finite or |.| <= 1e-30
+ * pAp / rz and non-finite alpha / beta.  FP32 vectors and SpMV, FP64
+ * reductions and scalars.  One iteration = 3 kernels, captured in a HIP
+ * graph of `graph_iters` iterations; the host checks the state once per
+ * graph launch.
+ * ===================================================================== */
+typedef struct xp_pcg xp_pcg;
+
+enum {
+    XP_CONVERGED = 0,
+    XP_MAX_ITER = 1,
+    XP_BREAKDOWN = 2,
+    XP_STAGNATION = 3,
+    XP_RUNNING = -1
+};
+
+typedef struct {
+    double tau;        /* absolute bound on r^T r; used when rel_tol <= 0 */
+    double rel_tol;    /* if > 0: tau_k = (rel_tol * ||b_k||)^2 per system */
+    int max_iters;     /* 0 -> max(4 n_k, 1000) (src/CgSolverGolden.hpp:47) */
+    int graph_iters;   /* iterations per graph launch; 0 = auto */
+    int use_graph;     /* 0: launch kernels directly (debugging) */
+    int trace_cap;     /* trace rows kept per system */
+    int stall_limit;   /* 0 = off; else stop with XP_STAGNATION after this many
+                          iterations without halving the best r^T r */
+} xp_solve_opts;
+
+typedef struct {
+    int status;        /* XP_* */
+    int iterations;
+    double rz, rr, pap, alpha, beta;
+    double rr0;        /* r^T r after init */
+    double tau;
+} xp_status;
+
+int xp_pcg_create(xp_matrix* m, const xp_solve_opts* opts, xp_pcg** out);
+void xp_pcg_destroy(xp_pcg* s);
+/* Uploads b and x0 (host, natural order, FP64, sum of n_k entries; x0 may
+ * be NULL for zero) and derives tau per system.  Not part of a timed solve. */
+int xp_pcg_set_rhs(xp_pcg* s, const double* b, const double* x0);
+/* Runs init + iterations until every system stops.  Device-resident. */
+int xp_pcg_run(xp_pcg* s);
+/* Runs init + exactly `iters` iterations (graph-captured as one launch),
+ * stopping early only where the state machine stops a system. */
+int xp_pcg_run_fixed(xp_pcg* s, int iters);
+/* Mixed-precision refinement: outer FP64 residual b - A x (hi+lo values)
+ * and FP64 solution update on the GPU, inner FP32 PCG to
+ * inner_rel * ||r_outer||, until ||b - A x|| <= outer_rel * ||b|| per system.
+ * Needs keep_lo.  Writes the outer true relative residuals to rel_res. */
+int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_outer,
+                       double* rel_res, int* outer_steps, int* total_iters);
+int xp_pcg_get_status(xp_pcg* s, int sys, xp_status* out);
+/* Solution in natural order, FP64 (sum of n_k entries). */
+int xp_pcg_get_x(xp_pcg* s, double* x);
+/* Trace rows {alpha, beta, rz, rr} of system `sys`; returns rows copied. */
+int xp_pcg_get_trace(xp_pcg* s, int sys, double* rows, int cap);
+/* FP64 true residual ||b - A x||_2 per system on the GPU. */
+int xp_pcg_true_residual(xp_pcg* s, double* out_norms);
+/* Device time of the last xp_pcg_run* (ms, HIP events) and launches made. */
+int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches);
+
+/* ===================================================================== *
+ * Host helpers: loader and generators (no GPU needed).
+ * ===================================================================== */
+typedef struct xp_csr xp_csr;
+/* CsrDataset::load (src/CsrDataset.hpp:19-67): same files, same checks,
+ * same messages (via xp_last_error). */
+int xp_dataset_load(const char* dir, xp_csr** out);
+int xp_dataset_save(const xp_csr* a, const char* dir);
+/* Generators (seeded, deterministic).  b ~ U[-1,1], x0 = 0. */
+int xp_gen_diffusion2d(int nx, int ny, double sigma, unsigned long long seed, xp_csr** out);
+int xp_gen_diffusion3d(int k, double sigma, unsigned long long seed, xp_csr** out);
+int xp_gen_fem2d(int m, double sigma, int n_incl, double contrast, double jitter,
+                 unsigned long long seed, xp_csr** out);
+int xp_csr_from_arrays(int n, const int* row_ptr, const int* col_idx, const double* values,
+                       const double* b, const double* x0, xp_csr** out);
+int xp_csr_info(const xp_csr* a, int* n, int* nnz);
+int xp_csr_copy(const xp_csr* a, int* row_ptr, int* col_idx, double* values, double* b, double* x0);
+void xp_csr_free(xp_csr* a);
+/* Host-side SELL build for inspection and CPU tests (no GPU): returns the
+ * arrays of the format described in the library's xp_types.hpp. */
+int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const double* values,
+                       int sigma, int* n_pad, int* nslices, long long* stored, int* idx16);
+int xp_sell_copy_host(int* perm, int* slice_ptr, int* slice_base, float* val,
+                      unsigned short* col16, int* col32);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPLUS_CAPI_H */

@@ -1,0 +1,672 @@
+// kernels.hip -- hand-written gfx950 (CDNA4) kernels of the Jacobi-PCG hot path.
+//
+// Replaces the reference's five fused operations (archived/kernels/
+// cg_kernels.cpp:43-158, HLS twins archived/kernels/*_kernel.cpp) and the
+// on-chip PCG loop of kernels/cuper_pcg_control_kernel.cpp:1121-1222.
+//
+// Design points:
+//  * SpMV is HBM-bound integer/gather work with no dense contraction, so no
+//    MFMA: one 64-lane wavefront per 64-row SELL slice, one row per lane,
+//    coalesced 4-byte value and 2-byte column streams, diagonal column first
+//    (its column index is the row itself and is never stored or loaded).
+//  * The pAp dot of a PCG step is fused into the SpMV epilogue (the x value
+//    of the diagonal slot *is* p[row]); the init step r = b - Ax, z, p, rz, rr
+//    is fused into the SpMV of x0 (as write_cuper_outputs<Init> does on the
+//    FPGA, kernels/cuper_pcg_control_kernel.cpp:463-555).
+//  * Reductions: FP64 products of the FP32 operands (exact), summed per lane,
+//    per wavefront with cross-lane shuffles, per workgroup through LDS, then
+//    across workgroups by the last workgroup to arrive (agent-scope atomic
+//    ticket), which sums the per-workgroup partials in workgroup order.  The
+//    order is fixed, so results are bitwise reproducible run to run.
+//  * The scalar part of a PCG step (alpha, beta, convergence and breakdown
+//    tests, trace) runs in that last workgroup, so a whole iteration is three
+//    kernels with no host round trip and can be captured in a HIP graph.  A
+//    system that has stopped turns every later kernel of its rows into a
+//    no-op, so the iteration count stays exact when a graph runs past it.
+//  * Workgroup -> rows mapping is XCD-aware: physical workgroup b runs on XCD
+//    b % 8, so logical block (b % 8) * (grid / 8) + b / 8 gives each XCD one
+//    contiguous range of rows and keeps the x entries they share in one L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace xp {
+
+namespace {
+
+constexpr double kBreakdownEps = 1.0e-30;  // include/cg_common.hpp:13 kBreakdownEps
+
+__device__ __forceinline__ int logical_block() {
+    const int g = static_cast<int>(gridDim.x);
+    const int b = static_cast<int>(blockIdx.x);
+    if (g % kNumXcd != 0) return b;
+    return (b % kNumXcd) * (g / kNumXcd) + b / kNumXcd;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sums NV values over the workgroup (kBlock threads).  Every thread gets the
+// totals.  Fixed combination order.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double red[kWaves][NV];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[wave][k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) t += red[w][k];
+        v[k] = t;
+    }
+    __syncthreads();
+}
+
+// Publishes this workgroup's NV partials and returns true in the last
+// workgroup of `sys` to arrive, whose threads then hold the system totals
+// (partials summed in logical-block order).  Every thread must call it.
+template <int NV>
+__device__ bool reduce_across_blocks(const Launch& L, int lb, int sys, int slot, double (&v)[NV]) {
+    __shared__ int is_last;
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) L.partials[static_cast<size_t>(lb) * 2 + k] = v[k];
+        __threadfence();  // release the partials at agent scope (all XCDs)
+        const int2 range = L.sys_blocks[sys];
+        unsigned* ctr = L.counters + static_cast<size_t>(sys) * kCounterSlots + slot;
+        const unsigned ticket = atomicAdd(ctr, 1u);
+        const bool last = ticket == static_cast<unsigned>(range.y - range.x - 1);
+        if (last) atomicExch(ctr, 0u);  // re-arm for the next launch
+        is_last = last ? 1 : 0;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();  // acquire: see every block's partials
+    const int2 range = L.sys_blocks[sys];
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = range.x + static_cast<int>(threadIdx.x); b < range.y; b += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            acc[k] += __hip_atomic_load(&L.partials[static_cast<size_t>(b) * 2 + k], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<NV>(acc);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = acc[k];
+    return true;
+}
+
+__device__ __forceinline__ bool bad(double v) { return !isfinite(v); }
+
+// One SELL row: y = sum_j val * x[col] in FP32, diagonal first.  xdiag
+// receives x[row].
+template <bool IDX16>
+__device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int lane,
+                                              const float* __restrict__ x, float& xdiag) {
+    const int j0 = A.slice_ptr[slice];
+    const int j1 = A.slice_ptr[slice + 1];
+    const int row = slice * kC + lane;
+    size_t e = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
+    xdiag = x[row];
+    float acc = A.val[e] * xdiag;
+    const int base = A.slice_base[slice];
+#pragma unroll 4
+    for (int j = j0 + 1; j < j1; ++j) {
+        e += kC;
+        const int c = IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e];
+        acc = fmaf(A.val[e], x[c], acc);
+    }
+    return acc;
+}
+
+template <bool IDX16, int VMODE>  // VMODE 0: val64; 1: val + val_lo
+__device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int lane,
+                                               const double* __restrict__ x) {
+    const int j0 = A.slice_ptr[slice];
+    const int j1 = A.slice_ptr[slice + 1];
+    const int row = slice * kC + lane;
+    size_t e = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
+    auto value = [&](size_t k) -> double {
+        if (VMODE == 0) return A.val64[k];
+        return static_cast<double>(A.val[k]) + static_cast<double>(A.val_lo[k]);
+    };
+    double acc = value(e) * x[row];
+    const int base = A.slice_base[slice];
+#pragma unroll 4
+    for (int j = j0 + 1; j < j1; ++j) {
+        e += kC;
+        const int c = IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e];
+        acc = fma(value(e), x[c], acc);
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// PCG init: r = b - A x, z = minv r, p = z; rz, rr.  Mirrors init_pcg_kernel
+// (archived/kernels/cg_kernels.cpp:73-103) fused with the x0 SpMV.
+template <bool IDX16>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(Launch L, SellView A, SysState* st,
+                                                     const float* __restrict__ x,
+                                                     const float* __restrict__ b,
+                                                     const float* __restrict__ minv,
+                                                     float* __restrict__ r, float* __restrict__ z,
+                                                     float* __restrict__ p) {
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    double v[2] = {0.0, 0.0};
+    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
+        const int slice = ch * kWaves + wave;
+        const int row = slice * kC + lane;
+        float xd;
+        const float ax = sell_row_f32<IDX16>(A, slice, lane, x, xd);
+        const float rv = b[row] - ax;
+        const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
+        r[row] = rv;
+        z[row] = zv;
+        p[row] = zv;
+        v[0] += static_cast<double>(rv) * static_cast<double>(zv);
+        v[1] += static_cast<double>(rv) * static_cast<double>(rv);
+    }
+    if (w.sys < 0) return;
+    if (!reduce_across_blocks<2>(L, lb, w.sys, 0, v)) return;
+    if (threadIdx.x != 0) return;
+    SysState& s = st[w.sys];
+    s.rz = v[0];
+    s.rr = v[1];
+    s.rr0 = v[1];
+    s.best_rr = v[1];
+    s.stall = 0;
+    s.iters = 0;
+    s.pap = 0.0;
+    s.alpha = 0.0;
+    s.beta = 0.0;
+    // Loop-top tests of the on-chip PCG loop
+    // (kernels/cuper_pcg_control_kernel.cpp:1123-1130).
+    if (s.rr <= s.tau) {
+        s.status = kStatusConverged;
+        s.active = 0;
+    } else if (bad(s.rz) || bad(s.rr) || fabs(s.rz) <= kBreakdownEps) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else if (s.max_iters <= 0) {
+        s.status = kStatusMaxIter;
+        s.active = 0;
+    } else {
+        s.status = kStatusRunning;
+        s.active = 1;
+    }
+}
+
+// ap = A p, pAp, alpha.
+template <bool IDX16>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(Launch L, SellView A, SysState* st,
+                                                     const float* __restrict__ p,
+                                                     float* __restrict__ ap) {
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    if (w.sys < 0 || !st[w.sys].active) return;
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    double v[1] = {0.0};
+    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
+        const int slice = ch * kWaves + wave;
+        float pd;
+        const float y = sell_row_f32<IDX16>(A, slice, lane, p, pd);
+        ap[slice * kC + lane] = y;
+        v[0] += static_cast<double>(pd) * static_cast<double>(y);
+    }
+    if (!reduce_across_blocks<1>(L, lb, w.sys, 1, v)) return;
+    if (threadIdx.x != 0) return;
+    SysState& s = st[w.sys];
+    const double pap = v[0];
+    s.pap = pap;
+    // Breakdown tests of run_local_multi_kernel_solver
+    // (archived/host/multi_kernel_solver.hpp:170-180).
+    if (bad(pap) || fabs(pap) <= kBreakdownEps) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+        return;
+    }
+    const double alpha = s.rz / pap;
+    if (bad(alpha)) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+        return;
+    }
+    s.alpha = alpha;
+}
+
+// x += alpha p; r -= alpha ap; z = minv r; rz_new, rr_new; beta; trace.
+// update_xrz_kernel (archived/kernels/cg_kernels.cpp:118-147) with the
+// scalar tail of the golden loop (src/CgSolverGolden.hpp:100-118).
+__global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, SysState* st, TraceRow* trace,
+                                                           int trace_stride, float* __restrict__ x,
+                                                           const float* __restrict__ p,
+                                                           float* __restrict__ r,
+                                                           const float* __restrict__ ap,
+                                                           const float* __restrict__ minv,
+                                                           float* __restrict__ z) {
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    if (w.sys < 0 || !st[w.sys].active) return;
+    const double alpha = st[w.sys].alpha;
+    double v[2] = {0.0, 0.0};
+    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
+        const int row = ch * kChunkRows + static_cast<int>(threadIdx.x);
+        const double pv = p[row];
+        const float xv = static_cast<float>(static_cast<double>(x[row]) + alpha * pv);
+        const float rv = static_cast<float>(static_cast<double>(r[row]) - alpha * static_cast<double>(ap[row]));
+        const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
+        x[row] = xv;
+        r[row] = rv;
+        z[row] = zv;
+        v[0] += static_cast<double>(rv) * static_cast<double>(zv);
+        v[1] += static_cast<double>(rv) * static_cast<double>(rv);
+    }
+    if (!reduce_across_blocks<2>(L, lb, w.sys, 2, v)) return;
+    if (threadIdx.x != 0) return;
+    SysState& s = st[w.sys];
+    const double rz_old = s.rz;
+    const double rz = v[0];
+    const double rr = v[1];
+    const double beta = rz / rz_old;
+    if (trace != nullptr && s.iters < s.trace_cap) {
+        TraceRow& t = trace[static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
+                            static_cast<size_t>(s.iters)];
+        t.alpha = s.alpha;
+        t.beta = beta;
+        t.rz = rz;
+        t.rr = rr;
+    }
+    s.iters += 1;
+    s.rz = rz;
+    s.rr = rr;
+    s.beta = beta;
+    // Post-update tests in the order of the on-chip loop
+    // (kernels/cuper_pcg_control_kernel.cpp:1193-1214), then the loop bound
+    // and the next loop-top rz test (:1127-1130).
+    if (bad(rz) || bad(rr)) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else if (rr <= s.tau) {
+        s.status = kStatusConverged;
+        s.active = 0;
+    } else if (fabs(rz_old) <= kBreakdownEps || bad(beta)) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else if (s.iters >= s.max_iters) {
+        s.status = kStatusMaxIter;
+        s.active = 0;
+    } else if (fabs(rz) <= kBreakdownEps) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else if (s.stall_limit > 0) {
+        // Stagnation watch (used by preconditioners that may fail): stop
+        // when r^T r has not dropped below 0.5 x its best for stall_limit
+        // iterations.
+        if (rr < 0.5 * s.best_rr) {
+            s.best_rr = rr;
+            s.stall = 0;
+        } else if (++s.stall >= s.stall_limit) {
+            s.status = kStatusStagnation;
+            s.active = 0;
+        }
+    }
+}
+
+// p = z + beta p (update_p_kernel, archived/kernels/cg_kernels.cpp:149-158).
+__global__ __launch_bounds__(kBlock) void k_pcg_update_p(Launch L, const SysState* st,
+                                                         const float* __restrict__ z,
+                                                         float* __restrict__ p) {
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    if (w.sys < 0 || !st[w.sys].active) return;
+    const double beta = st[w.sys].beta;
+    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
+        const int row = ch * kChunkRows + static_cast<int>(threadIdx.x);
+        p[row] = static_cast<float>(static_cast<double>(z[row]) + beta * static_cast<double>(p[row]));
+    }
+}
+
+template <bool IDX16>
+__global__ __launch_bounds__(kBlock) void k_spmv_f32(Launch L, SellView A,
+                                                     const float* __restrict__ x,
+                                                     float* __restrict__ y) {
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
+        const int slice = ch * kWaves + wave;
+        float xd;
+        y[slice * kC + lane] = sell_row_f32<IDX16>(A, slice, lane, x, xd);
+    }
+}
+
+template <bool IDX16, int VMODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_f64(Launch L, SellView A,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
+        const int slice = ch * kWaves + wave;
+        y[slice * kC + lane] = sell_row_f64<IDX16, VMODE>(A, slice, lane, x);
+    }
+}
+
+template <bool IDX16, int VMODE>
+__global__ __launch_bounds__(kBlock) void k_residual_f64(Launch L, SellView A,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ b,
+                                                         double* __restrict__ r,
+                                                         float* __restrict__ r32,
+                                                         double* __restrict__ out_rr) {
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    double v[1] = {0.0};
+    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
+        const int slice = ch * kWaves + wave;
+        const int row = slice * kC + lane;
+        const double rv = b[row] - sell_row_f64<IDX16, VMODE>(A, slice, lane, x);
+        r[row] = rv;
+        if (r32 != nullptr) r32[row] = static_cast<float>(rv);
+        v[0] += rv * rv;
+    }
+    if (w.sys < 0) return;
+    if (!reduce_across_blocks<1>(L, lb, w.sys, 3, v)) return;
+    if (threadIdx.x == 0) out_rr[w.sys] = v[0];
+}
+
+__global__ void k_axpy_f64_f32(int n, const float* __restrict__ d, double* __restrict__ x) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i < n) x[i] += static_cast<double>(d[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Natural-order FP64 kernels behind the drop-in C-ABI.  Same formulas as
+// archived/kernels/cg_kernels.cpp:73-158; sums in a fixed parallel order.
+
+struct Flat {
+    double* partials;
+    unsigned* counter;
+};
+
+// Sums NV values over the whole grid; returns true (in every thread of the
+// last block) with the totals in v.
+template <int NV>
+__device__ bool flat_reduce(Flat F, double (&v)[NV]) {
+    __shared__ int is_last;
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) F.partials[static_cast<size_t>(blockIdx.x) * 2 + k] = v[k];
+        __threadfence();
+        const unsigned ticket = atomicAdd(F.counter, 1u);
+        const bool last = ticket == gridDim.x - 1;
+        if (last) atomicExch(F.counter, 0u);
+        is_last = last ? 1 : 0;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = static_cast<int>(threadIdx.x); b < static_cast<int>(gridDim.x); b += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            acc[k] += __hip_atomic_load(&F.partials[static_cast<size_t>(b) * 2 + k], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<NV>(acc);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = acc[k];
+    return true;
+}
+
+__device__ __forceinline__ void flat_range(int n, int& i0, int& i1) {
+    const int per = (n + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x);
+    i0 = min(n, static_cast<int>(blockIdx.x) * per);
+    i1 = min(n, i0 + per);
+}
+
+__global__ __launch_bounds__(kBlock) void k_init_pcg_f64(int n, const double* __restrict__ b,
+                                                         const double* __restrict__ ax,
+                                                         const double* __restrict__ m_inv,
+                                                         double* __restrict__ r, double* __restrict__ z,
+                                                         double* __restrict__ p, double* out, Flat F) {
+    int i0, i1;
+    flat_range(n, i0, i1);
+    double v[2] = {0.0, 0.0};
+    for (int i = i0 + static_cast<int>(threadIdx.x); i < i1; i += kBlock) {
+        const double rv = b[i] - ax[i];
+        const double zv = m_inv[i] * rv;
+        r[i] = rv;
+        z[i] = zv;
+        p[i] = zv;
+        v[0] += rv * zv;
+        v[1] += rv * rv;
+    }
+    if (flat_reduce<2>(F, v) && threadIdx.x == 0) {
+        out[0] = v[0];
+        out[1] = v[1];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_f64(int n, const double* __restrict__ a,
+                                                    const double* __restrict__ b, double* out, Flat F) {
+    int i0, i1;
+    flat_range(n, i0, i1);
+    double v[1] = {0.0};
+    for (int i = i0 + static_cast<int>(threadIdx.x); i < i1; i += kBlock) v[0] += a[i] * b[i];
+    if (flat_reduce<1>(F, v) && threadIdx.x == 0) out[0] = v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_xrz_f64(int n, double* __restrict__ x,
+                                                           const double* __restrict__ p,
+                                                           double* __restrict__ r,
+                                                           const double* __restrict__ ap,
+                                                           const double* __restrict__ m_inv,
+                                                           double* __restrict__ z, double alpha,
+                                                           double* out, Flat F) {
+    int i0, i1;
+    flat_range(n, i0, i1);
+    double v[2] = {0.0, 0.0};
+    for (int i = i0 + static_cast<int>(threadIdx.x); i < i1; i += kBlock) {
+        x[i] += alpha * p[i];
+        const double rv = r[i] - alpha * ap[i];
+        const double zv = m_inv[i] * rv;
+        r[i] = rv;
+        z[i] = zv;
+        v[0] += rv * zv;
+        v[1] += rv * rv;
+    }
+    if (flat_reduce<2>(F, v) && threadIdx.x == 0) {
+        out[0] = v[0];
+        out[1] = v[1];
+    }
+}
+
+__global__ void k_update_p_f64(int n, const double* __restrict__ z, double* __restrict__ p, double beta) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+template <typename S, typename D>
+__global__ void k_gather(int n_pad, const int* __restrict__ perm, const S* __restrict__ src,
+                         D* __restrict__ dst) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i >= n_pad) return;
+    const int g = perm[i];
+    dst[i] = g >= 0 ? static_cast<D>(src[g]) : static_cast<D>(0);
+}
+
+template <typename S, typename D>
+__global__ void k_scatter(int n_pad, const int* __restrict__ perm, const S* __restrict__ src,
+                          D* __restrict__ dst) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i >= n_pad) return;
+    const int g = perm[i];
+    if (g >= 0) dst[g] = static_cast<D>(src[i]);
+}
+
+__global__ void k_convert_f2d(int n, const float* __restrict__ src, double* __restrict__ dst) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i < n) dst[i] = static_cast<double>(src[i]);
+}
+
+template <typename T>
+__global__ void k_fill(size_t n, T v, T* __restrict__ dst) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+int blocks_for(size_t n, int bs) { return static_cast<int>((n + static_cast<size_t>(bs) - 1) / static_cast<size_t>(bs)); }
+
+}  // namespace
+
+void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
+                     const float* b, const float* minv, float* r, float* z, float* p) {
+    if (A.idx16)
+        hipLaunchKernelGGL(k_pcg_init<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, x, b, minv, r, z, p);
+    else
+        hipLaunchKernelGGL(k_pcg_init<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, x, b, minv, r, z, p);
+}
+
+void launch_pcg_spmv(const Launch& L, const SellView& A, SysState* st, const float* p, float* ap) {
+    if (A.idx16)
+        hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
+    else
+        hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
+}
+
+void launch_pcg_update_xrz(const Launch& L, SysState* st, TraceRow* trace, int trace_stride,
+                           float* x, const float* p, float* r, const float* ap,
+                           const float* minv, float* z) {
+    hipLaunchKernelGGL(k_pcg_update_xrz, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, trace,
+                       trace_stride, x, p, r, ap, minv, z);
+}
+
+void launch_pcg_update_p(const Launch& L, const SysState* st, const float* z, float* p) {
+    hipLaunchKernelGGL(k_pcg_update_p, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, z, p);
+}
+
+void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y) {
+    if (A.idx16)
+        hipLaunchKernelGGL(k_spmv_f32<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+    else
+        hipLaunchKernelGGL(k_spmv_f32<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+}
+
+void launch_spmv_f64(const Launch& L, const SellView& A, bool use_lo, const double* x, double* y) {
+    if (A.idx16) {
+        if (use_lo)
+            hipLaunchKernelGGL((k_spmv_f64<true, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+        else
+            hipLaunchKernelGGL((k_spmv_f64<true, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+    } else {
+        if (use_lo)
+            hipLaunchKernelGGL((k_spmv_f64<false, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+        else
+            hipLaunchKernelGGL((k_spmv_f64<false, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+    }
+}
+
+void launch_residual_f64(const Launch& L, const SellView& A, bool use_lo, const double* x,
+                         const double* b, double* r, float* r32, double* out_rr) {
+    if (A.idx16) {
+        if (use_lo)
+            hipLaunchKernelGGL((k_residual_f64<true, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+        else
+            hipLaunchKernelGGL((k_residual_f64<true, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+    } else {
+        if (use_lo)
+            hipLaunchKernelGGL((k_residual_f64<false, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+        else
+            hipLaunchKernelGGL((k_residual_f64<false, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+    }
+}
+
+void launch_axpy_f64_f32(int n_pad, const float* d, double* x, hipStream_t s) {
+    if (n_pad <= 0) return;
+    hipLaunchKernelGGL(k_axpy_f64_f32, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad, d, x);
+}
+
+void launch_init_pcg_f64(int n, const double* b, const double* ax, const double* m_inv,
+                         double* r, double* z, double* p, double* out, double* partials,
+                         unsigned* counter, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_pcg_f64, dim3(grid), dim3(kBlock), 0, s, n, b, ax, m_inv, r, z, p, out,
+                       Flat{partials, counter});
+}
+
+void launch_dot_f64(int n, const double* a, const double* b, double* out, double* partials,
+                    unsigned* counter, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_f64, dim3(grid), dim3(kBlock), 0, s, n, a, b, out, Flat{partials, counter});
+}
+
+void launch_update_xrz_f64(int n, double* x, const double* p, double* r, const double* ap,
+                           const double* m_inv, double* z, double alpha, double* out,
+                           double* partials, unsigned* counter, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_xrz_f64, dim3(grid), dim3(kBlock), 0, s, n, x, p, r, ap, m_inv, z, alpha,
+                       out, Flat{partials, counter});
+}
+
+void launch_update_p_f64(int n, const double* z, double* p, double beta, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_update_p_f64, dim3(blocks_for(static_cast<size_t>(n), 256)), dim3(256), 0, s, n, z, p, beta);
+}
+
+#define XP_PERM_LAUNCH(name, kern, S, D)                                                          \
+    void name(int n_pad, const int* perm, const S* src, D* dst, hipStream_t s) {                  \
+        if (n_pad <= 0) return;                                                                   \
+        hipLaunchKernelGGL((kern<S, D>), dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, \
+                           n_pad, perm, src, dst);                                                \
+    }
+XP_PERM_LAUNCH(launch_gather_d2f, k_gather, double, float)
+XP_PERM_LAUNCH(launch_gather_d2d, k_gather, double, double)
+XP_PERM_LAUNCH(launch_gather_f2f, k_gather, float, float)
+XP_PERM_LAUNCH(launch_scatter_f2d, k_scatter, float, double)
+XP_PERM_LAUNCH(launch_scatter_d2d, k_scatter, double, double)
+XP_PERM_LAUNCH(launch_scatter_f2f, k_scatter, float, float)
+#undef XP_PERM_LAUNCH
+
+void launch_convert_f2d(int n, const float* src, double* dst, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_convert_f2d, dim3(blocks_for(static_cast<size_t>(n), 256)), dim3(256), 0, s, n, src, dst);
+}
+
+void launch_fill_f32(size_t n, float v, float* dst, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill<float>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, v, dst);
+}
+
+void launch_fill_u32(size_t n, unsigned v, unsigned* dst, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill<unsigned>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, v, dst);
+}
+
+}  // namespace xp

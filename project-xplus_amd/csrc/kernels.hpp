@@ -1,0 +1,77 @@
+// kernels.hpp -- launchers for the gfx950 kernels in kernels.hip.
+//
+// Every launcher is stream-ordered and allocates nothing; buffers belong to
+// the caller (the runtime in runtime.hip).  All pointers are device pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "xp_types.hpp"
+
+namespace xp {
+
+// Shared launch context of the row-streaming kernels: the work map (one
+// WorkItem per logical workgroup), the per-system workgroup ranges used by
+// the reductions, and the reduction scratch.
+struct Launch {
+    int grid = 0;                       // workgroups, a multiple of kNumXcd
+    const WorkItem* work = nullptr;     // [grid]
+    const int2* sys_blocks = nullptr;   // [nsys] logical block range per system
+    unsigned* counters = nullptr;       // [nsys * kCounterSlots], zero at rest
+    double* partials = nullptr;         // [grid * 2]
+    hipStream_t stream = nullptr;
+};
+constexpr int kCounterSlots = 4;        // one arrival counter per reduction kind
+
+// ---- FP32 solve-order PCG step kernels (state machine in SysState) ----
+// init: r = b - A x; z = minv r; p = z; rz, rr; state set running/converged/breakdown.
+void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
+                     const float* b, const float* minv, float* r, float* z, float* p);
+// ap = A p; pAp; alpha = rz / pAp.
+void launch_pcg_spmv(const Launch& L, const SellView& A, SysState* st, const float* p, float* ap);
+// x += alpha p; r -= alpha ap; z = minv r; rz, rr; beta; trace; convergence.
+void launch_pcg_update_xrz(const Launch& L, SysState* st, TraceRow* trace, int trace_stride,
+                           float* x, const float* p, float* r, const float* ap,
+                           const float* minv, float* z);
+// p = z + beta p.
+void launch_pcg_update_p(const Launch& L, const SysState* st, const float* z, float* p);
+
+// ---- plain SpMV (no reduction) ----
+void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y);
+// FP64 SpMV with exact FP64 values (val64) or hi+lo pairs (val + val_lo).
+void launch_spmv_f64(const Launch& L, const SellView& A, bool use_lo, const double* x, double* y);
+// r = b - A x in FP64 (hi+lo values if use_lo, else exact val64),
+// out_rr[sys] = r^T r.  Also writes the FP32 copy of r into r32 (may be null).
+void launch_residual_f64(const Launch& L, const SellView& A, bool use_lo, const double* x,
+                         const double* b, double* r, float* r32, double* out_rr);
+// x64 += (double) d32 over the solve-order rows.
+void launch_axpy_f64_f32(int n_pad, const float* d, double* x, hipStream_t s);
+
+// ---- natural-order FP64 element kernels behind the drop-in C-ABI ----
+// Each one reduces deterministically (fixed per-block order, last block
+// sums the block partials in block order) into out[0..1].
+void launch_init_pcg_f64(int n, const double* b, const double* ax, const double* m_inv,
+                         double* r, double* z, double* p, double* out, double* partials,
+                         unsigned* counter, int grid, hipStream_t s);
+void launch_dot_f64(int n, const double* a, const double* b, double* out, double* partials,
+                    unsigned* counter, int grid, hipStream_t s);
+void launch_update_xrz_f64(int n, double* x, const double* p, double* r, const double* ap,
+                           const double* m_inv, double* z, double alpha, double* out,
+                           double* partials, unsigned* counter, int grid, hipStream_t s);
+void launch_update_p_f64(int n, const double* z, double* p, double beta, hipStream_t s);
+
+// ---- permutation between natural and solve order ----
+// dst[i] = src[perm[i]] (0 where perm[i] < 0), i < n_pad, with conversion.
+void launch_gather_d2f(int n_pad, const int* perm, const double* src, float* dst, hipStream_t s);
+void launch_gather_d2d(int n_pad, const int* perm, const double* src, double* dst, hipStream_t s);
+void launch_gather_f2f(int n_pad, const int* perm, const float* src, float* dst, hipStream_t s);
+// dst[perm[i]] = src[i] for perm[i] >= 0.
+void launch_scatter_f2d(int n_pad, const int* perm, const float* src, double* dst, hipStream_t s);
+void launch_scatter_d2d(int n_pad, const int* perm, const double* src, double* dst, hipStream_t s);
+void launch_scatter_f2f(int n_pad, const int* perm, const float* src, float* dst, hipStream_t s);
+void launch_convert_f2d(int n, const float* src, double* dst, hipStream_t s);
+// Fills a float or int buffer (graph-safe replacement for a memset).
+void launch_fill_f32(size_t n, float v, float* dst, hipStream_t s);
+void launch_fill_u32(size_t n, unsigned v, unsigned* dst, hipStream_t s);
+
+}  // namespace xp

@@ -1,0 +1,1148 @@
+// runtime.hip -- host runtime and C-ABI of libxplus_amd.so.
+//
+// Owns device memory, the library stream, the SELL matrices, the
+// graph-captured PCG iteration and the drop-in kernel call surface of
+// archived/include/cg_kernels.hpp:9-54.  Fails loudly (error code +
+// xp_last_error, or a no-op with the error recorded for the void drop-in
+// functions) when no GPU is present: there is no CPU fallback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/xplus_capi.h"
+#include "csr_dataset.hpp"
+#include "generators.hpp"
+#include "kernels.hpp"
+#include "sell_build.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPX(expr) hip_check((expr), #expr)
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    } catch (...) {
+        set_error("unknown error");
+        return -1;
+    }
+}
+
+// ---- device context --------------------------------------------------------
+struct Device {
+    int cus = 0;
+    hipStream_t stream = nullptr;
+};
+
+Device& device() {
+    static std::once_flag once;
+    static Device dev;
+    static std::string init_error;
+    std::call_once(once, [] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+            init_error = "no GPU visible: libxplus_amd runs on an MI355X (gfx950) only";
+            return;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+            init_error = "hipGetDeviceProperties failed";
+            return;
+        }
+        dev.cus = prop.multiProcessorCount;
+        if (hipStreamCreateWithFlags(&dev.stream, hipStreamNonBlocking) != hipSuccess) {
+            init_error = "hipStreamCreate failed";
+            return;
+        }
+    });
+    if (!init_error.empty()) throw std::runtime_error(init_error);
+    return dev;
+}
+
+hipStream_t pick_stream(void* s) { return s != nullptr ? static_cast<hipStream_t>(s) : device().stream; }
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        HIPX(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) { if (count > n) alloc(std::max(count, n + n / 2)); }
+    void release() {
+        if (p != nullptr) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        if (count) HIPX(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* dst, size_t count, hipStream_t s) const {
+        if (count) HIPX(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+template <typename T>
+DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
+    DevBuf<T> d(v.size());
+    d.upload(v.data(), v.size(), s);
+    return d;
+}
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+// ---- matrices ----------------------------------------------------------------
+struct xp_matrix {
+    int n = 0, n_pad = 0, nslices = 0, nsys = 0, sigma = 0, idx16 = 0;
+    long long nnz = 0, stored = 0;
+    bool has_lo = false, has64 = false;
+    std::vector<int> perm;            // solve -> natural
+    std::vector<int> sys_row_begin;   // solve order
+    std::vector<int> sys_nat_begin;   // natural order
+    std::vector<double> diag;         // natural order, first diagonal entry (0 if missing)
+    std::vector<int> diag_state;      // 0 ok, 1 missing, 2 zero (|d| <= DBL_MIN)
+    DevBuf<int> d_slice_ptr, d_slice_base, d_perm;
+    DevBuf<float> d_val, d_val_lo;
+    DevBuf<double> d_val64;
+    DevBuf<uint16_t> d_col16;
+    DevBuf<int> d_col32;
+    // launch machinery
+    int grid = 0;
+    DevBuf<xp::WorkItem> d_work;
+    DevBuf<int2> d_sys_blocks;
+    DevBuf<unsigned> d_counters;
+    DevBuf<double> d_partials;
+
+    xp::SellView view() const {
+        xp::SellView v{};
+        v.n_pad = n_pad;
+        v.nslices = nslices;
+        v.nsys = nsys;
+        v.idx16 = idx16;
+        v.slice_ptr = d_slice_ptr.p;
+        v.slice_base = d_slice_base.p;
+        v.val = d_val.p;
+        v.val_lo = d_val_lo.p;
+        v.val64 = d_val64.p;
+        v.col16 = d_col16.p;
+        v.col32 = d_col32.p;
+        return v;
+    }
+    xp::Launch launch(hipStream_t s) const {
+        xp::Launch L;
+        L.grid = grid;
+        L.work = d_work.p;
+        L.sys_blocks = d_sys_blocks.p;
+        L.counters = d_counters.p;
+        L.partials = d_partials.p;
+        L.stream = s;
+        return L;
+    }
+    long long bytes_spmv() const {
+        // values (4 B) for every stored entry, 2 or 4 B of column index for the
+        // off-diagonal ones, x read and y written once per row.
+        const long long offdiag = stored - static_cast<long long>(n_pad);
+        return 4LL * stored + (idx16 ? 2LL : 4LL) * offdiag + 8LL * n_pad;
+    }
+};
+
+namespace {
+
+// Splits each system's chunks over workgroups: about `target` workgroups in
+// total, at least one per system, a system's workgroups contiguous in
+// logical order, total rounded up to a multiple of the XCD count.
+void build_work_map(xp_matrix& m, int target) {
+    const hipStream_t s = device().stream;
+    const int total_chunks = m.n_pad / xp::kChunkRows;
+    std::vector<int> g(static_cast<size_t>(m.nsys));
+    for (int k = 0; k < m.nsys; ++k) {
+        const int ch = (m.sys_row_begin[static_cast<size_t>(k) + 1] - m.sys_row_begin[static_cast<size_t>(k)]) / xp::kChunkRows;
+        const double share = static_cast<double>(target) * ch / std::max(1, total_chunks);
+        g[static_cast<size_t>(k)] = std::max(1, std::min(ch, static_cast<int>(std::lround(share))));
+    }
+    std::vector<xp::WorkItem> work;
+    std::vector<int2> sb(static_cast<size_t>(m.nsys));
+    for (int k = 0; k < m.nsys; ++k) {
+        const int c0 = m.sys_row_begin[static_cast<size_t>(k)] / xp::kChunkRows;
+        const int c1 = m.sys_row_begin[static_cast<size_t>(k) + 1] / xp::kChunkRows;
+        const int gk = g[static_cast<size_t>(k)];
+        const int ch = c1 - c0;
+        sb[static_cast<size_t>(k)] = make_int2(static_cast<int>(work.size()), static_cast<int>(work.size()) + gk);
+        for (int q = 0; q < gk; ++q) {
+            const int a = c0 + static_cast<int>(static_cast<long long>(ch) * q / gk);
+            const int b = c0 + static_cast<int>(static_cast<long long>(ch) * (q + 1) / gk);
+            work.push_back(xp::WorkItem{k, a, b, 0});
+        }
+    }
+    const int grid = round_up(static_cast<int>(work.size()), xp::kNumXcd);
+    while (static_cast<int>(work.size()) < grid) work.push_back(xp::WorkItem{-1, 0, 0, 0});
+    m.grid = grid;
+    m.d_work = to_device(work, s);
+    m.d_sys_blocks = to_device(sb, s);
+    m.d_counters.alloc(static_cast<size_t>(m.nsys) * xp::kCounterSlots);
+    HIPX(hipMemsetAsync(m.d_counters.p, 0, m.d_counters.n * sizeof(unsigned), s));
+    m.d_partials.alloc(static_cast<size_t>(grid) * 2);
+    HIPX(hipStreamSynchronize(s));
+}
+
+int default_grid(const xp_matrix& m) {
+    const int chunks = m.n_pad / xp::kChunkRows;
+    return std::min(chunks, std::max(8, device().cus * 8));
+}
+
+std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, const xp_matrix_opts* o) {
+    const hipStream_t s = device().stream;
+    const int sigma = (o != nullptr && o->sigma > 0) ? o->sigma : 256;
+    const bool keep_lo = o != nullptr && o->keep_lo;
+    const bool keep64 = o != nullptr && o->keep64;
+    xp::SellHost h = xp::build_sell(systems, sigma, keep_lo, keep64);
+    auto m = std::make_unique<xp_matrix>();
+    m->n = h.n;
+    m->n_pad = h.n_pad;
+    m->nslices = h.nslices;
+    m->nsys = h.nsys;
+    m->sigma = h.sigma;
+    m->idx16 = h.idx16;
+    m->nnz = h.nnz;
+    m->stored = h.stored;
+    m->has_lo = keep_lo;
+    m->has64 = keep64;
+    m->perm = h.perm;
+    m->sys_row_begin = h.sys_row_begin;
+    m->sys_nat_begin = h.sys_nat_begin;
+    // Diagonal per natural row, with the reference's two failure cases kept
+    // for the solver (src/CsrDataset.hpp:92-116).
+    m->diag.assign(static_cast<size_t>(h.n), 0.0);
+    m->diag_state.assign(static_cast<size_t>(h.n), 0);
+    for (int k = 0; k < h.nsys; ++k) {
+        const xp::CsrRef& a = systems[static_cast<size_t>(k)];
+        for (int r = 0; r < a.n; ++r) {
+            const size_t g = static_cast<size_t>(h.sys_nat_begin[static_cast<size_t>(k)] + r);
+            int state = 1;
+            for (int o2 = a.row_ptr[r]; o2 < a.row_ptr[r + 1]; ++o2)
+                if (a.col_idx[o2] == r) {
+                    m->diag[g] = a.values[o2];
+                    state = std::fabs(a.values[o2]) <= std::numeric_limits<double>::min() ? 2 : 0;
+                    break;
+                }
+            m->diag_state[g] = state;
+        }
+    }
+    m->d_slice_ptr = to_device(h.slice_ptr, s);
+    m->d_slice_base = to_device(h.slice_base, s);
+    m->d_perm = to_device(h.perm, s);
+    m->d_val = to_device(h.val, s);
+    if (keep_lo) m->d_val_lo = to_device(h.val_lo, s);
+    if (keep64) m->d_val64 = to_device(h.val64, s);
+    if (h.idx16)
+        m->d_col16 = to_device(h.col16, s);
+    else
+        m->d_col32 = to_device(h.col32, s);
+    HIPX(hipStreamSynchronize(s));
+    const int grid = (o != nullptr && o->grid > 0) ? o->grid : default_grid(*m);
+    build_work_map(*m, grid);
+    return m;
+}
+
+}  // namespace
+
+// ---- solver ------------------------------------------------------------------
+struct xp_pcg {
+    xp_matrix* m = nullptr;
+    xp_solve_opts opts{};
+    hipStream_t stream = nullptr;
+    DevBuf<float> x, r, z, p, ap, b, minv;
+    DevBuf<xp::SysState> st;
+    DevBuf<xp::TraceRow> trace;
+    int trace_stride = 0;
+    std::vector<xp::SysState> st_init;   // per-system config (tau, max_iters, ...)
+    xp::SysState* st_host = nullptr;     // pinned readback
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_iters = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int last_launches = 0;
+    bool have_rhs = false;
+    // natural-order FP64 copies of b and x0 (host) for refinement / residuals
+    std::vector<double> b_nat, x0_nat;
+    // refinement buffers (allocated on first use)
+    DevBuf<double> x64, b64, r64, rr_out, tmp_nat;
+    DevBuf<float> r32;
+    bool x64_valid = false;           // x64 holds the solution of the last (refined) run
+    const float* rhs = nullptr;       // RHS of the init kernel; null = b
+
+    ~xp_pcg() {
+        if (exec != nullptr) (void)hipGraphExecDestroy(exec);
+        if (graph != nullptr) (void)hipGraphDestroy(graph);
+        if (ev0 != nullptr) (void)hipEventDestroy(ev0);
+        if (ev1 != nullptr) (void)hipEventDestroy(ev1);
+        if (st_host != nullptr) (void)hipHostFree(st_host);
+        if (stream != nullptr) (void)hipStreamDestroy(stream);
+    }
+
+    void enqueue_iteration() {
+        const xp::Launch L = m->launch(stream);
+        const xp::SellView A = m->view();
+        xp::launch_pcg_spmv(L, A, st.p, p.p, ap.p);
+        xp::launch_pcg_update_xrz(L, st.p, trace.p, trace_stride, x.p, p.p, r.p, ap.p, minv.p, z.p);
+        xp::launch_pcg_update_p(L, st.p, z.p, p.p);
+    }
+
+    void build_graph(int iters) {
+        if (exec != nullptr) { HIPX(hipGraphExecDestroy(exec)); exec = nullptr; }
+        if (graph != nullptr) { HIPX(hipGraphDestroy(graph)); graph = nullptr; }
+        HIPX(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < iters; ++k) enqueue_iteration();
+        HIPX(hipStreamEndCapture(stream, &graph));
+        HIPX(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        graph_iters = iters;
+    }
+
+    void reset_state() {
+        st.upload(st_init.data(), st_init.size(), stream);
+    }
+
+    void enqueue_init() {
+        xp::launch_pcg_init(m->launch(stream), m->view(), st.p, x.p, rhs != nullptr ? rhs : b.p, minv.p, r.p,
+                            z.p, p.p);
+    }
+
+    bool any_active() {
+        HIPX(hipMemcpyAsync(st_host, st.p, st.n * sizeof(xp::SysState), hipMemcpyDeviceToHost, stream));
+        HIPX(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < st.n; ++k)
+            if (st_host[k].active) return true;
+        return false;
+    }
+
+    // Runs init + iterations until every system stops or `cap` iterations
+    // were issued; times the whole thing with HIP events on the solver stream.
+    void run_loop(long long cap) {
+        HIPX(hipEventRecord(ev0, stream));
+        run_inner(cap);
+        HIPX(hipEventRecord(ev1, stream));
+        HIPX(hipEventSynchronize(ev1));
+        float ms = 0.0f;
+        HIPX(hipEventElapsedTime(&ms, ev0, ev1));
+        last_ms = ms;
+        x64_valid = false;
+    }
+
+    void run_inner(long long cap) {
+        last_launches = 0;
+        enqueue_init();
+        long long issued = 0;
+        if (any_active()) {
+            while (issued < cap) {
+                if (opts.use_graph) {
+                    HIPX(hipGraphLaunch(exec, stream));
+                    issued += graph_iters;
+                } else {
+                    for (int k = 0; k < graph_iters; ++k) enqueue_iteration();
+                    issued += graph_iters;
+                }
+                ++last_launches;
+                if (!any_active()) break;
+            }
+        }
+    }
+};
+
+namespace {
+
+std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
+    if (m == nullptr) throw std::runtime_error("null matrix");
+    // The reference refuses a bad Jacobi diagonal before iterating
+    // (src/CsrDataset.hpp:107-112, archived/host/multi_kernel_solver.hpp:63-75).
+    for (size_t g = 0; g < m->diag_state.size(); ++g) {
+        // report in system-local row numbers, as the reference would for that system
+        int k = 0;
+        while (k + 1 < m->nsys && static_cast<int>(g) >= m->sys_nat_begin[static_cast<size_t>(k) + 1]) ++k;
+        const int local = static_cast<int>(g) - m->sys_nat_begin[static_cast<size_t>(k)];
+        if (m->diag_state[g] == 1) throw std::runtime_error("missing diagonal entry at row " + std::to_string(local));
+        if (m->diag_state[g] == 2) throw std::runtime_error("zero diagonal entry at row " + std::to_string(local));
+        if (std::fabs(m->diag[g]) <= 1.0e-30)
+            throw std::runtime_error("zero diagonal entry while building Jacobi inverse");
+    }
+    auto s = std::make_unique<xp_pcg>();
+    s->m = m;
+    xp_solve_opts d{};
+    d.tau = 1.0e-10;  // include/cg_common.hpp:53 SolverConfig::tau
+    d.use_graph = 1;
+    d.trace_cap = 0;
+    if (o != nullptr) d = *o;
+    if (d.graph_iters <= 0) d.graph_iters = m->n_pad <= 65536 ? 64 : 32;
+    s->opts = d;
+    HIPX(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPX(hipEventCreate(&s->ev0));
+    HIPX(hipEventCreate(&s->ev1));
+    const size_t np = static_cast<size_t>(m->n_pad);
+    s->x.alloc(np); s->r.alloc(np); s->z.alloc(np); s->p.alloc(np); s->ap.alloc(np); s->b.alloc(np);
+    // minv in solve order (padding rows: 1)
+    std::vector<float> minv(np, 1.0f);
+    for (size_t i = 0; i < np; ++i) {
+        const int g = m->perm[i];
+        if (g >= 0) minv[i] = static_cast<float>(1.0 / m->diag[static_cast<size_t>(g)]);
+    }
+    s->minv = to_device(minv, s->stream);
+    s->st.alloc(static_cast<size_t>(m->nsys));
+    s->st_init.assign(static_cast<size_t>(m->nsys), xp::SysState{});
+    s->trace_stride = std::max(0, d.trace_cap);
+    if (s->trace_stride > 0) s->trace.alloc(static_cast<size_t>(m->nsys) * static_cast<size_t>(s->trace_stride));
+    for (int k = 0; k < m->nsys; ++k) {
+        xp::SysState& t = s->st_init[static_cast<size_t>(k)];
+        const int nk = m->sys_nat_begin[static_cast<size_t>(k) + 1] - m->sys_nat_begin[static_cast<size_t>(k)];
+        t.tau = d.tau;
+        t.max_iters = d.max_iters > 0 ? d.max_iters : std::max(4 * nk, 1000);
+        t.status = xp::kStatusRunning;
+        t.stall_limit = d.stall_limit;
+        t.trace_cap = s->trace_stride;
+    }
+    HIPX(hipHostMalloc(reinterpret_cast<void**>(&s->st_host), static_cast<size_t>(m->nsys) * sizeof(xp::SysState)));
+    s->build_graph(d.graph_iters);
+    HIPX(hipStreamSynchronize(s->stream));
+    return s;
+}
+
+void set_rhs(xp_pcg& s, const double* b, const double* x0) {
+    xp_matrix& m = *s.m;
+    const size_t n = static_cast<size_t>(m.n);
+    s.b_nat.assign(b, b + n);
+    if (x0 != nullptr)
+        s.x0_nat.assign(x0, x0 + n);
+    else
+        s.x0_nat.assign(n, 0.0);
+    // tau per system (absolute, or relative to ||b_k||)
+    for (int k = 0; k < m.nsys; ++k) {
+        xp::SysState& t = s.st_init[static_cast<size_t>(k)];
+        if (s.opts.rel_tol > 0.0) {
+            double bb = 0.0;
+            for (int g = m.sys_nat_begin[static_cast<size_t>(k)]; g < m.sys_nat_begin[static_cast<size_t>(k) + 1]; ++g)
+                bb += b[g] * b[g];
+            t.tau = s.opts.rel_tol * s.opts.rel_tol * bb;
+        } else {
+            t.tau = s.opts.tau;
+        }
+    }
+    DevBuf<double> tmp(std::max<size_t>(n, 1));
+    tmp.upload(b, n, s.stream);
+    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s.b.p, s.stream);
+    tmp.upload(s.x0_nat.data(), n, s.stream);
+    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s.x.p, s.stream);
+    HIPX(hipStreamSynchronize(s.stream));
+    s.have_rhs = true;
+}
+
+}  // namespace
+
+// ---- drop-in kernel call surface -----------------------------------------------
+namespace {
+
+uint64_t mix_hash(uint64_t h, const void* data, size_t bytes) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    const uint64_t k1 = 0x9E3779B185EBCA87ull, k2 = 0xC2B2AE3D27D4EB4Full;
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h ^= w * k2;
+        h = ((h << 31) | (h >> 33)) * k1;
+    }
+    for (; i < bytes; ++i) h = (h ^ p[i]) * 0x100000001B3ull;
+    return h ^ bytes;
+}
+
+struct Legacy {
+    std::mutex mu;
+    // cached matrix
+    const int* rp = nullptr;
+    const int* ci = nullptr;
+    const double* va = nullptr;
+    int n = -1;
+    int nnz = -1;
+    uint64_t hash = 0;
+    std::unique_ptr<xp_matrix> mat;
+    // natural-order work buffers
+    DevBuf<double> buf[8];
+    DevBuf<double> partials;
+    DevBuf<unsigned> counter;
+    DevBuf<double> out;
+
+    void ensure_flat() {
+        const hipStream_t s = device().stream;
+        if (partials.n == 0) {
+            partials.alloc(2 * 1024);
+            counter.alloc(1);
+            HIPX(hipMemsetAsync(counter.p, 0, sizeof(unsigned), s));
+            out.alloc(2);
+        }
+    }
+    int flat_grid(int n) const { return std::max(1, std::min(1024, (n + 255) / 256)); }
+
+    xp_matrix& matrix(const int* row_ptr, const int* col_idx, const double* values, int n_) {
+        const int nnz_ = row_ptr[n_];
+        if (nnz_ < 0) throw std::runtime_error("row_ptr[n] must be non-negative");
+        uint64_t h = mix_hash(0x12345678ull, row_ptr, sizeof(int) * (static_cast<size_t>(n_) + 1));
+        h = mix_hash(h, col_idx, sizeof(int) * static_cast<size_t>(nnz_));
+        h = mix_hash(h, values, sizeof(double) * static_cast<size_t>(nnz_));
+        if (!mat || rp != row_ptr || ci != col_idx || va != values || n != n_ || nnz != nnz_ || hash != h) {
+            mat.reset();
+            xp::CsrRef a{n_, row_ptr, col_idx, values};
+            xp_matrix_opts o{};
+            o.sigma = 256;
+            o.keep64 = 1;
+            mat = make_matrix({a}, &o);
+            rp = row_ptr; ci = col_idx; va = values; n = n_; nnz = nnz_; hash = h;
+        }
+        return *mat;
+    }
+
+    void spmv(const int* row_ptr, const int* col_idx, const double* values, const double* x, double* y, int n_) {
+        const hipStream_t s = device().stream;
+        xp_matrix& m = matrix(row_ptr, col_idx, values, n_);
+        const size_t np = static_cast<size_t>(m.n_pad);
+        for (int k = 0; k < 4; ++k) buf[k].ensure(std::max<size_t>(np, 1));
+        buf[0].upload(x, static_cast<size_t>(n_), s);
+        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, buf[0].p, buf[1].p, s);
+        xp::launch_spmv_f64(m.launch(s), m.view(), false, buf[1].p, buf[2].p);
+        xp::launch_scatter_d2d(m.n_pad, m.d_perm.p, buf[2].p, buf[3].p, s);
+        buf[3].download(y, static_cast<size_t>(n_), s);
+        HIPX(hipStreamSynchronize(s));
+    }
+};
+
+Legacy& legacy() {
+    static Legacy L;
+    return L;
+}
+
+template <typename F>
+void legacy_call(F&& f) {
+    Legacy& L = legacy();
+    std::lock_guard<std::mutex> lock(L.mu);
+    guarded([&] { f(L); });
+}
+
+}  // namespace
+
+extern "C" {
+
+void spmv_csr_kernel(const int* row_ptr, const int* col_idx, const double* values, const double* x,
+                     double* y, int n) {
+    if (n <= 0) return;  // n < 0: the reference's silent no-op; n == 0: nothing to do
+    legacy_call([&](Legacy& L) { L.spmv(row_ptr, col_idx, values, x, y, n); });
+}
+
+void spmv_blocked_kernel(const int* row_ptr, const int* col_idx, const double* values, const double* x,
+                         double* y, int n) {
+    if (n <= 0) return;
+    legacy_call([&](Legacy& L) { L.spmv(row_ptr, col_idx, values, x, y, n); });
+}
+
+void init_pcg_kernel(const double* b, const double* ax, const double* m_inv, double* r, double* z,
+                     double* p, double* metrics, int n) {
+    if (n < 0) return;
+    legacy_call([&](Legacy& L) {
+        const hipStream_t s = device().stream;
+        L.ensure_flat();
+        const size_t nn = std::max<size_t>(static_cast<size_t>(n), 1);
+        for (int k = 0; k < 6; ++k) L.buf[k].ensure(nn);
+        L.buf[0].upload(b, static_cast<size_t>(n), s);
+        L.buf[1].upload(ax, static_cast<size_t>(n), s);
+        L.buf[2].upload(m_inv, static_cast<size_t>(n), s);
+        xp::launch_init_pcg_f64(n, L.buf[0].p, L.buf[1].p, L.buf[2].p, L.buf[3].p, L.buf[4].p, L.buf[5].p,
+                                L.out.p, L.partials.p, L.counter.p, L.flat_grid(n), s);
+        L.buf[3].download(r, static_cast<size_t>(n), s);
+        L.buf[4].download(z, static_cast<size_t>(n), s);
+        L.buf[5].download(p, static_cast<size_t>(n), s);
+        L.out.download(metrics, 2, s);
+        HIPX(hipStreamSynchronize(s));
+    });
+}
+
+void dot_kernel(const double* a, const double* b, double* out, int n) {
+    if (n < 0) return;
+    legacy_call([&](Legacy& L) {
+        const hipStream_t s = device().stream;
+        L.ensure_flat();
+        const size_t nn = std::max<size_t>(static_cast<size_t>(n), 1);
+        L.buf[0].ensure(nn);
+        L.buf[1].ensure(nn);
+        L.buf[0].upload(a, static_cast<size_t>(n), s);
+        L.buf[1].upload(b, static_cast<size_t>(n), s);
+        xp::launch_dot_f64(n, L.buf[0].p, L.buf[1].p, L.out.p, L.partials.p, L.counter.p, L.flat_grid(n), s);
+        L.out.download(out, 1, s);
+        HIPX(hipStreamSynchronize(s));
+    });
+}
+
+void update_xrz_kernel(double* x, const double* p, double* r, const double* ap, const double* m_inv,
+                       double* z, double* metrics, double alpha, int n) {
+    if (n < 0) return;
+    legacy_call([&](Legacy& L) {
+        const hipStream_t s = device().stream;
+        L.ensure_flat();
+        const size_t nn = std::max<size_t>(static_cast<size_t>(n), 1);
+        for (int k = 0; k < 6; ++k) L.buf[k].ensure(nn);
+        const size_t cnt = static_cast<size_t>(n);
+        L.buf[0].upload(x, cnt, s);
+        L.buf[1].upload(p, cnt, s);
+        L.buf[2].upload(r, cnt, s);
+        L.buf[3].upload(ap, cnt, s);
+        L.buf[4].upload(m_inv, cnt, s);
+        xp::launch_update_xrz_f64(n, L.buf[0].p, L.buf[1].p, L.buf[2].p, L.buf[3].p, L.buf[4].p, L.buf[5].p,
+                                  alpha, L.out.p, L.partials.p, L.counter.p, L.flat_grid(n), s);
+        L.buf[0].download(x, cnt, s);
+        L.buf[2].download(r, cnt, s);
+        L.buf[5].download(z, cnt, s);
+        L.out.download(metrics, 2, s);
+        HIPX(hipStreamSynchronize(s));
+    });
+}
+
+void update_p_kernel(const double* z, double* p, double beta, int n) {
+    if (n < 0) return;
+    legacy_call([&](Legacy& L) {
+        const hipStream_t s = device().stream;
+        const size_t nn = std::max<size_t>(static_cast<size_t>(n), 1);
+        L.buf[0].ensure(nn);
+        L.buf[1].ensure(nn);
+        L.buf[0].upload(z, static_cast<size_t>(n), s);
+        L.buf[1].upload(p, static_cast<size_t>(n), s);
+        xp::launch_update_p_f64(n, L.buf[0].p, L.buf[1].p, beta, s);
+        L.buf[1].download(p, static_cast<size_t>(n), s);
+        HIPX(hipStreamSynchronize(s));
+    });
+}
+
+void xp_legacy_reset(void) {
+    Legacy& L = legacy();
+    std::lock_guard<std::mutex> lock(L.mu);
+    L.mat.reset();
+    L.rp = nullptr;
+}
+
+int xp_last_error(char* buf, int len) {
+    if (buf != nullptr && len > 0) {
+        std::snprintf(buf, static_cast<size_t>(len), "%s", g_last_error.c_str());
+    }
+    return static_cast<int>(g_last_error.size());
+}
+
+int xp_device_count(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    return count;
+}
+
+// ---- matrices ----
+int xp_matrix_create(int n, const int* row_ptr, const int* col_idx, const double* values,
+                     const xp_matrix_opts* opts, xp_matrix** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null output pointer");
+        *out = nullptr;
+        if (n <= 0) throw std::runtime_error("n must be positive");
+        xp::CsrRef a{n, row_ptr, col_idx, values};
+        *out = make_matrix({a}, opts).release();
+    });
+}
+
+int xp_matrix_create_batch(int nsys, const int* ns, const int* row_ptrs, const int* col_idx,
+                           const double* values, const xp_matrix_opts* opts, xp_matrix** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null output pointer");
+        *out = nullptr;
+        if (nsys <= 0) throw std::runtime_error("nsys must be positive");
+        std::vector<xp::CsrRef> sys(static_cast<size_t>(nsys));
+        size_t rp_off = 0, nz_off = 0;
+        for (int k = 0; k < nsys; ++k) {
+            const int nk = ns[k];
+            if (nk <= 0) throw std::runtime_error("every system needs n > 0");
+            const int* rp = row_ptrs + rp_off;
+            sys[static_cast<size_t>(k)] = xp::CsrRef{nk, rp, col_idx + nz_off, values + nz_off};
+            rp_off += static_cast<size_t>(nk) + 1;
+            nz_off += static_cast<size_t>(rp[nk]);
+        }
+        *out = make_matrix(sys, opts).release();
+    });
+}
+
+void xp_matrix_destroy(xp_matrix* m) { delete m; }
+
+int xp_matrix_get_info(const xp_matrix* m, xp_matrix_info* out) {
+    return guarded([&] {
+        if (m == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        out->n = m->n;
+        out->n_pad = m->n_pad;
+        out->nslices = m->nslices;
+        out->nsys = m->nsys;
+        out->sigma = m->sigma;
+        out->idx16 = m->idx16;
+        out->grid = m->grid;
+        out->nnz = m->nnz;
+        out->stored = m->stored;
+        out->bytes_spmv = m->bytes_spmv();
+    });
+}
+
+int xp_matrix_permutation(const xp_matrix* m, int* perm) {
+    return guarded([&] {
+        if (m == nullptr || perm == nullptr) throw std::runtime_error("null argument");
+        std::memcpy(perm, m->perm.data(), m->perm.size() * sizeof(int));
+    });
+}
+
+int xp_matrix_set_grid(xp_matrix* m, int grid) {
+    return guarded([&] {
+        if (m == nullptr || grid <= 0) throw std::runtime_error("bad argument");
+        build_work_map(*m, grid);
+    });
+}
+
+int xp_spmv_f32(xp_matrix* m, const float* x, float* y, void* stream) {
+    return guarded([&] {
+        if (m == nullptr) throw std::runtime_error("null matrix");
+        xp::launch_spmv_f32(m->launch(pick_stream(stream)), m->view(), x, y);
+        HIPX(hipGetLastError());
+    });
+}
+
+int xp_spmv_f64(xp_matrix* m, const double* x, double* y, void* stream) {
+    return guarded([&] {
+        if (m == nullptr) throw std::runtime_error("null matrix");
+        if (!m->has64 && !m->has_lo) throw std::runtime_error("xp_spmv_f64 needs keep64 or keep_lo");
+        xp::launch_spmv_f64(m->launch(pick_stream(stream)), m->view(), !m->has64, x, y);
+        HIPX(hipGetLastError());
+    });
+}
+
+int xp_to_solve_order(xp_matrix* m, const double* nat, float* solve, void* stream) {
+    return guarded([&] {
+        if (m == nullptr) throw std::runtime_error("null matrix");
+        xp::launch_gather_d2f(m->n_pad, m->d_perm.p, nat, solve, pick_stream(stream));
+    });
+}
+
+int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* stream) {
+    return guarded([&] {
+        if (m == nullptr) throw std::runtime_error("null matrix");
+        xp::launch_scatter_f2d(m->n_pad, m->d_perm.p, solve, nat, pick_stream(stream));
+    });
+}
+
+int xp_csr_spmv_f64(int n, const int* row_ptr, const int* col_idx, const double* values,
+                    const double* x, double* y) {
+    if (n <= 0) return 0;
+    Legacy& L = legacy();
+    std::lock_guard<std::mutex> lock(L.mu);
+    return guarded([&] { L.spmv(row_ptr, col_idx, values, x, y, n); });
+}
+
+// ---- solver ----
+int xp_pcg_create(xp_matrix* m, const xp_solve_opts* opts, xp_pcg** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null output pointer");
+        *out = nullptr;
+        *out = make_pcg(m, opts).release();
+    });
+}
+
+void xp_pcg_destroy(xp_pcg* s) { delete s; }
+
+int xp_pcg_set_rhs(xp_pcg* s, const double* b, const double* x0) {
+    return guarded([&] {
+        if (s == nullptr || b == nullptr) throw std::runtime_error("null argument");
+        set_rhs(*s, b, x0);
+    });
+}
+
+int xp_pcg_run(xp_pcg* s) {
+    return guarded([&] {
+        if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run: call xp_pcg_set_rhs first");
+        // x holds x0 (solve order) until the run starts; restore it so that
+        // repeated runs start from the same point.
+        DevBuf<double> tmp(std::max<size_t>(static_cast<size_t>(s->m->n), 1));
+        tmp.upload(s->x0_nat.data(), static_cast<size_t>(s->m->n), s->stream);
+        xp::launch_gather_d2f(s->m->n_pad, s->m->d_perm.p, tmp.p, s->x.p, s->stream);
+        s->reset_state();
+        HIPX(hipStreamSynchronize(s->stream));
+        long long cap = 0;
+        for (const auto& t : s->st_init) cap = std::max<long long>(cap, t.max_iters);
+        s->run_loop(cap + s->graph_iters);
+    });
+}
+
+int xp_pcg_run_fixed(xp_pcg* s, int iters) {
+    return guarded([&] {
+        if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run_fixed: call xp_pcg_set_rhs first");
+        if (iters < 0) throw std::runtime_error("iters must be >= 0");
+        DevBuf<double> tmp(std::max<size_t>(static_cast<size_t>(s->m->n), 1));
+        tmp.upload(s->x0_nat.data(), static_cast<size_t>(s->m->n), s->stream);
+        xp::launch_gather_d2f(s->m->n_pad, s->m->d_perm.p, tmp.p, s->x.p, s->stream);
+        std::vector<xp::SysState> init = s->st_init;
+        for (auto& t : init) t.max_iters = std::min(t.max_iters, std::max(iters, 0));
+        s->st.upload(init.data(), init.size(), s->stream);
+        HIPX(hipStreamSynchronize(s->stream));
+        s->run_loop(iters);
+    });
+}
+
+int xp_pcg_get_status(xp_pcg* s, int sys, xp_status* out) {
+    return guarded([&] {
+        if (s == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        if (sys < 0 || sys >= s->m->nsys) throw std::runtime_error("system index out of range");
+        xp::SysState t;
+        HIPX(hipMemcpyAsync(&t, s->st.p + sys, sizeof t, hipMemcpyDeviceToHost, s->stream));
+        HIPX(hipStreamSynchronize(s->stream));
+        out->status = t.status;
+        out->iterations = t.iters;
+        out->rz = t.rz;
+        out->rr = t.rr;
+        out->pap = t.pap;
+        out->alpha = t.alpha;
+        out->beta = t.beta;
+        out->rr0 = t.rr0;
+        out->tau = t.tau;
+    });
+}
+
+int xp_pcg_get_x(xp_pcg* s, double* x) {
+    return guarded([&] {
+        if (s == nullptr || x == nullptr) throw std::runtime_error("null argument");
+        const size_t n = static_cast<size_t>(s->m->n);
+        DevBuf<double> tmp(std::max<size_t>(n, 1));
+        if (s->x64_valid)
+            xp::launch_scatter_d2d(s->m->n_pad, s->m->d_perm.p, s->x64.p, tmp.p, s->stream);
+        else
+            xp::launch_scatter_f2d(s->m->n_pad, s->m->d_perm.p, s->x.p, tmp.p, s->stream);
+        tmp.download(x, n, s->stream);
+        HIPX(hipStreamSynchronize(s->stream));
+    });
+}
+
+int xp_pcg_get_trace(xp_pcg* s, int sys, double* rows, int cap) {
+    int copied = 0;
+    const int rc = guarded([&] {
+        if (s == nullptr || rows == nullptr) throw std::runtime_error("null argument");
+        if (sys < 0 || sys >= s->m->nsys) throw std::runtime_error("system index out of range");
+        xp::SysState t;
+        HIPX(hipMemcpyAsync(&t, s->st.p + sys, sizeof t, hipMemcpyDeviceToHost, s->stream));
+        HIPX(hipStreamSynchronize(s->stream));
+        const int avail = std::min(t.iters, s->trace_stride);
+        copied = std::max(0, std::min(cap, avail));
+        if (copied > 0) {
+            HIPX(hipMemcpyAsync(rows, s->trace.p + static_cast<size_t>(sys) * static_cast<size_t>(s->trace_stride),
+                                static_cast<size_t>(copied) * sizeof(xp::TraceRow), hipMemcpyDeviceToHost, s->stream));
+            HIPX(hipStreamSynchronize(s->stream));
+        }
+    });
+    return rc != 0 ? rc : copied;
+}
+
+int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches) {
+    return guarded([&] {
+        if (s == nullptr) throw std::runtime_error("null argument");
+        if (ms != nullptr) *ms = s->last_ms;
+        if (graph_launches != nullptr) *graph_launches = s->last_launches;
+    });
+}
+
+}  // extern "C"
+
+// ---- FP64 residuals and refinement ----------------------------------------------
+namespace {
+
+void ensure_f64(xp_pcg& s) {
+    const size_t np = static_cast<size_t>(s.m->n_pad);
+    if (s.x64.n != np) s.x64.alloc(np);
+    if (s.b64.n != np) s.b64.alloc(np);
+    if (s.r64.n != np) s.r64.alloc(np);
+    if (s.r32.n != np) s.r32.alloc(np);
+    if (s.rr_out.n != static_cast<size_t>(s.m->nsys)) s.rr_out.alloc(static_cast<size_t>(s.m->nsys));
+    const size_t nn = std::max<size_t>(static_cast<size_t>(s.m->n), 1);
+    if (s.tmp_nat.n < nn) s.tmp_nat.alloc(nn);
+}
+
+// ||b - A x|| per system, x in solve order FP64; r32 (optional) receives
+// the FP32 copy of the residual.
+std::vector<double> residual_norms(xp_pcg& s, const double* x64, float* r32) {
+    xp_matrix& m = *s.m;
+    if (!m.has_lo && !m.has64) throw std::runtime_error("FP64 residuals need keep_lo or keep64");
+    xp::launch_residual_f64(m.launch(s.stream), m.view(), !m.has64, x64, s.b64.p, s.r64.p, r32, s.rr_out.p);
+    std::vector<double> rr(static_cast<size_t>(m.nsys));
+    s.rr_out.download(rr.data(), rr.size(), s.stream);
+    HIPX(hipStreamSynchronize(s.stream));
+    for (auto& v : rr) v = std::sqrt(std::max(v, 0.0));
+    return rr;
+}
+
+void upload_b64(xp_pcg& s) {
+    s.tmp_nat.upload(s.b_nat.data(), s.b_nat.size(), s.stream);
+    xp::launch_gather_d2d(s.m->n_pad, s.m->d_perm.p, s.tmp_nat.p, s.b64.p, s.stream);
+}
+
+std::vector<double> system_bnorms(const xp_pcg& s) {
+    const xp_matrix& m = *s.m;
+    std::vector<double> out(static_cast<size_t>(m.nsys), 0.0);
+    for (int k = 0; k < m.nsys; ++k) {
+        double acc = 0.0;
+        for (int g = m.sys_nat_begin[static_cast<size_t>(k)]; g < m.sys_nat_begin[static_cast<size_t>(k) + 1]; ++g)
+            acc += s.b_nat[static_cast<size_t>(g)] * s.b_nat[static_cast<size_t>(g)];
+        out[static_cast<size_t>(k)] = std::sqrt(acc);
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xp_pcg_true_residual(xp_pcg* s, double* out_norms) {
+    return guarded([&] {
+        if (s == nullptr || out_norms == nullptr || !s->have_rhs) throw std::runtime_error("bad argument");
+        ensure_f64(*s);
+        upload_b64(*s);
+        const double* xs = s->x64.p;
+        DevBuf<double> conv;
+        if (!s->x64_valid) {
+            conv.alloc(static_cast<size_t>(s->m->n_pad));
+            xp::launch_convert_f2d(s->m->n_pad, s->x.p, conv.p, s->stream);
+            xs = conv.p;
+        }
+        const std::vector<double> rn = residual_norms(*s, xs, nullptr);
+        std::memcpy(out_norms, rn.data(), rn.size() * sizeof(double));
+    });
+}
+
+int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_outer,
+                       double* rel_res, int* outer_steps, int* total_iters) {
+    return guarded([&] {
+        if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run_refined: call xp_pcg_set_rhs first");
+        if (!s->m->has_lo && !s->m->has64) throw std::runtime_error("refinement needs a matrix built with keep_lo");
+        if (!(outer_rel > 0.0) || !(inner_rel > 0.0) || max_outer <= 0) throw std::runtime_error("bad tolerances");
+        xp_pcg& S = *s;
+        xp_matrix& m = *S.m;
+        ensure_f64(S);
+        const std::vector<double> bn = system_bnorms(S);
+        HIPX(hipEventRecord(S.ev0, S.stream));
+        upload_b64(S);
+        S.tmp_nat.upload(S.x0_nat.data(), S.x0_nat.size(), S.stream);
+        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, S.tmp_nat.p, S.x64.p, S.stream);
+        std::vector<double> rn = residual_norms(S, S.x64.p, S.r32.p);
+        std::vector<long long> iters(static_cast<size_t>(m.nsys), 0);
+        int outer = 0;
+        int launches = 0;
+        long long cap = 0;
+        for (const auto& t : S.st_init) cap = std::max<long long>(cap, t.max_iters);
+        auto done = [&](int k) { return rn[static_cast<size_t>(k)] <= outer_rel * bn[static_cast<size_t>(k)]; };
+        while (outer < max_outer) {
+            bool any = false;
+            for (int k = 0; k < m.nsys; ++k) any = any || !done(k);
+            if (!any) break;
+            std::vector<xp::SysState> init = S.st_init;
+            for (int k = 0; k < m.nsys; ++k) {
+                const double t = inner_rel * rn[static_cast<size_t>(k)];
+                init[static_cast<size_t>(k)].tau = done(k) ? std::numeric_limits<double>::max() : t * t;
+            }
+            S.st.upload(init.data(), init.size(), S.stream);
+            xp::launch_fill_f32(static_cast<size_t>(m.n_pad), 0.0f, S.x.p, S.stream);
+            S.rhs = S.r32.p;
+            S.run_inner(cap);
+            S.rhs = nullptr;
+            launches += S.last_launches;
+            HIPX(hipMemcpyAsync(S.st_host, S.st.p, S.st.n * sizeof(xp::SysState), hipMemcpyDeviceToHost, S.stream));
+            HIPX(hipStreamSynchronize(S.stream));
+            for (int k = 0; k < m.nsys; ++k) iters[static_cast<size_t>(k)] += S.st_host[k].iters;
+            xp::launch_axpy_f64_f32(m.n_pad, S.x.p, S.x64.p, S.stream);
+            rn = residual_norms(S, S.x64.p, S.r32.p);
+            ++outer;
+        }
+        HIPX(hipEventRecord(S.ev1, S.stream));
+        HIPX(hipEventSynchronize(S.ev1));
+        float ms = 0.0f;
+        HIPX(hipEventElapsedTime(&ms, S.ev0, S.ev1));
+        S.last_ms = ms;
+        S.last_launches = launches;
+        S.x64_valid = true;
+        if (rel_res != nullptr)
+            for (int k = 0; k < m.nsys; ++k)
+                rel_res[k] = rn[static_cast<size_t>(k)] / std::max(bn[static_cast<size_t>(k)], 1.0e-300);
+        if (outer_steps != nullptr) *outer_steps = outer;
+        if (total_iters != nullptr) {
+            long long mx = 0;
+            for (auto v : iters) mx = std::max(mx, v);
+            *total_iters = static_cast<int>(mx);
+        }
+    });
+}
+
+}  // extern "C"
+
+// ---- host helpers: dataset, generators, SELL inspection ----------------------------
+struct xp_csr {
+    xp::HostCsr a;
+};
+
+namespace {
+thread_local xp::SellHost g_sell_host;
+}
+
+extern "C" {
+
+int xp_dataset_load(const char* dir, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr || dir == nullptr) throw std::runtime_error("null argument");
+        *out = nullptr;
+        xp::CsrDataset d = xp::CsrDataset::load(dir);
+        auto h = std::make_unique<xp_csr>();
+        h->a.n = d.n();
+        h->a.row_ptr = d.row_ptr();
+        h->a.col_idx = d.col_idx();
+        h->a.values = d.values();
+        h->a.b = d.b();
+        h->a.x0 = d.x0();
+        *out = h.release();
+    });
+}
+
+int xp_dataset_save(const xp_csr* a, const char* dir) {
+    return guarded([&] {
+        if (a == nullptr || dir == nullptr) throw std::runtime_error("null argument");
+        xp::CsrDataset d = xp::CsrDataset::from_arrays(a->a.row_ptr, a->a.col_idx, a->a.values, a->a.b, a->a.x0);
+        d.save(dir);
+    });
+}
+
+int xp_gen_diffusion2d(int nx, int ny, double sigma, unsigned long long seed, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null argument");
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::gen_diffusion2d(nx, ny, sigma, seed);
+        *out = h.release();
+    });
+}
+
+int xp_gen_diffusion3d(int k, double sigma, unsigned long long seed, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null argument");
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::gen_diffusion3d(k, sigma, seed);
+        *out = h.release();
+    });
+}
+
+int xp_gen_fem2d(int m, double sigma, int n_incl, double contrast, double jitter, unsigned long long seed,
+                 xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null argument");
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::gen_fem2d(m, sigma, n_incl, contrast, jitter, seed);
+        *out = h.release();
+    });
+}
+
+int xp_csr_from_arrays(int n, const int* row_ptr, const int* col_idx, const double* values, const double* b,
+                       const double* x0, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr || row_ptr == nullptr) throw std::runtime_error("null argument");
+        if (n < 0) throw std::runtime_error("n must be non-negative");
+        auto h = std::make_unique<xp_csr>();
+        const int nnz = row_ptr[n];
+        h->a.n = n;
+        h->a.row_ptr.assign(row_ptr, row_ptr + n + 1);
+        h->a.col_idx.assign(col_idx, col_idx + nnz);
+        h->a.values.assign(values, values + nnz);
+        if (b != nullptr) h->a.b.assign(b, b + n); else h->a.b.assign(static_cast<size_t>(n), 0.0);
+        if (x0 != nullptr) h->a.x0.assign(x0, x0 + n); else h->a.x0.assign(static_cast<size_t>(n), 0.0);
+        *out = h.release();
+    });
+}
+
+int xp_csr_info(const xp_csr* a, int* n, int* nnz) {
+    return guarded([&] {
+        if (a == nullptr) throw std::runtime_error("null argument");
+        if (n != nullptr) *n = a->a.n;
+        if (nnz != nullptr) *nnz = static_cast<int>(a->a.col_idx.size());
+    });
+}
+
+int xp_csr_copy(const xp_csr* a, int* row_ptr, int* col_idx, double* values, double* b, double* x0) {
+    return guarded([&] {
+        if (a == nullptr) throw std::runtime_error("null argument");
+        const auto& h = a->a;
+        if (row_ptr) std::memcpy(row_ptr, h.row_ptr.data(), h.row_ptr.size() * sizeof(int));
+        if (col_idx) std::memcpy(col_idx, h.col_idx.data(), h.col_idx.size() * sizeof(int));
+        if (values) std::memcpy(values, h.values.data(), h.values.size() * sizeof(double));
+        if (b) std::memcpy(b, h.b.data(), h.b.size() * sizeof(double));
+        if (x0) std::memcpy(x0, h.x0.data(), h.x0.size() * sizeof(double));
+    });
+}
+
+void xp_csr_free(xp_csr* a) { delete a; }
+
+int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const double* values, int sigma,
+                       int* n_pad, int* nslices, long long* stored, int* idx16) {
+    return guarded([&] {
+        xp::CsrRef a{n, row_ptr, col_idx, values};
+        g_sell_host = xp::build_sell({a}, sigma, false, false);
+        if (n_pad) *n_pad = g_sell_host.n_pad;
+        if (nslices) *nslices = g_sell_host.nslices;
+        if (stored) *stored = g_sell_host.stored;
+        if (idx16) *idx16 = g_sell_host.idx16;
+    });
+}
+
+int xp_sell_copy_host(int* perm, int* slice_ptr, int* slice_base, float* val, unsigned short* col16, int* col32) {
+    return guarded([&] {
+        const auto& h = g_sell_host;
+        if (perm) std::memcpy(perm, h.perm.data(), h.perm.size() * sizeof(int));
+        if (slice_ptr) std::memcpy(slice_ptr, h.slice_ptr.data(), h.slice_ptr.size() * sizeof(int));
+        if (slice_base) std::memcpy(slice_base, h.slice_base.data(), h.slice_base.size() * sizeof(int));
+        if (val) std::memcpy(val, h.val.data(), h.val.size() * sizeof(float));
+        if (col16 && h.idx16) std::memcpy(col16, h.col16.data(), h.col16.size() * sizeof(uint16_t));
+        if (col32 && !h.idx16) std::memcpy(col32, h.col32.data(), h.col32.size() * sizeof(int));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// xp_types.hpp -- data layout shared by the host builder and the gfx950 kernels.
+//
+// Matrix format: SELL-C-sigma with C = 64 (one wavefront lane per row, one
+// wavefront per slice), rows sorted by length inside windows of sigma rows.
+// The sort is applied to the whole system as a symmetric permutation, so the
+// solver runs in "solve order": every vector is contiguous in that order and
+// the SpMV writes y coalesced.  Vectors are padded to n_pad (a multiple of
+// the work-chunk height); padding rows are identity rows with b = x0 = 0, so
+// they stay exactly zero and add nothing to any dot product.
+//
+// Inside slice s (rows 64s .. 64s+63), column j of the slice occupies the 64
+// consecutive entries [(slice_ptr[s] + j) * 64, +64): lane l holds entry j of
+// row 64s+l.  Column j = 0 is the diagonal of every row (its column index is
+// the row itself and is never stored or loaded); columns j >= 1 hold the
+// off-diagonal entries, padded with value 0 and delta 0.  Off-diagonal column
+// indices are stored as 16-bit deltas from a per-slice base column when every
+// slice's column span fits, else as absolute 32-bit indices.
+#pragma once
+
+#include <cstdint>
+
+namespace xp {
+
+constexpr int kC = 64;             // slice height == wavefront width on CDNA
+constexpr int kWaves = 4;          // wavefronts per workgroup in the streaming kernels
+constexpr int kBlock = kC * kWaves;  // 256 threads, one row per thread per chunk
+constexpr int kChunkRows = kBlock;   // rows per work chunk
+constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// One logical workgroup's share of the rows: chunks [chunk_begin, chunk_end)
+// of system `sys`.  A workgroup never straddles two systems.
+struct WorkItem {
+    int sys;
+    int chunk_begin;
+    int chunk_end;
+    int pad_;
+};
+
+// Per-system PCG state machine, resident on the device.  Written only by the
+// last workgroup of a reduction (see kernels.hip), read by every kernel of
+// the next step.  Layout is plain-old-data and mirrored by xp_status_t.
+struct SysState {
+    double rz;
+    double rr;
+    double pap;
+    double alpha;
+    double beta;
+    double tau;       // absolute bound on r^T r
+    double rr0;       // r^T r after init
+    double best_rr;   // smallest rr seen (stagnation watch)
+    int iters;
+    int max_iters;
+    int status;       // 0 converged, 1 max_iter, 2 breakdown, 3 stagnation, -1 running
+    int active;       // 1 while iterating
+    int stall;        // iterations since best_rr last improved by the stagnation factor
+    int stall_limit;  // 0 = stagnation watch off
+    int trace_cap;
+    int pad_;
+};
+
+enum : int {
+    kStatusConverged = 0,
+    kStatusMaxIter = 1,
+    kStatusBreakdown = 2,
+    kStatusStagnation = 3,
+    kStatusRunning = -1,
+};
+
+// One trace row per completed iteration (the reference's
+// GoldenIterationTrace minus the derived residual = sqrt(rr)).
+struct TraceRow {
+    double alpha;
+    double beta;
+    double rz;
+    double rr;
+};
+
+// Device view of a SELL matrix (all pointers are device pointers).
+struct SellView {
+    int n_pad;
+    int nslices;
+    int nsys;
+    int idx16;               // 1: col16 deltas from slice_base; 0: col32 absolute
+    const int* slice_ptr;    // [nslices + 1], in units of 64-entry columns
+    const int* slice_base;   // [nslices]
+    const float* val;        // [slice_ptr[nslices] * 64], diagonal first
+    const float* val_lo;     // optional: v - (double)val, rounded to float
+    const double* val64;     // optional: exact FP64 values
+    const uint16_t* col16;
+    const int* col32;
+};
+
+}  // namespace xp

@@ -1,0 +1,413 @@
+"""Python binding (ctypes) of libxplus_amd.so, the MI355X Jacobi-PCG library.
+
+The library is C++/HIP; this module only marshals numpy arrays across the
+C-ABI declared in include/xplus_capi.h.  Importing it loads the built
+library and raises ImportError if it is missing -- there is no Python or CPU
+fallback for any computation.
+
+Layout of the API (mirrors the C-ABI):
+  * drop-in kernel surface: spmv_csr_kernel, spmv_blocked_kernel,
+    init_pcg_kernel, dot_kernel, update_xrz_kernel, update_p_kernel
+    (archived/include/cg_kernels.hpp:9-54), on numpy float64 arrays;
+  * Csr: host CSR + b + x0 (loader CsrDataset::load, generators);
+  * Matrix: device-resident SELL-C-sigma matrix;
+  * Pcg: graph-captured Jacobi-PCG on one system or a batch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "lib" / "libxplus_amd.so"
+CLI_PATH = PKG_DIR / "lib" / "xplus_local"
+
+if not LIB_PATH.exists():
+    raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C project-xplus_amd)")
+
+_lib = C.CDLL(str(LIB_PATH))
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+CONVERGED, MAX_ITER, BREAKDOWN, STAGNATION, RUNNING = 0, 1, 2, 3, -1
+STATUS_NAMES = {0: "converged", 1: "max_iter", 2: "breakdown", 3: "stagnation", -1: "running"}
+
+
+class MatrixOpts(C.Structure):
+    _fields_ = [("sigma", C.c_int), ("keep_lo", C.c_int), ("keep64", C.c_int), ("grid", C.c_int)]
+
+
+class MatrixInfo(C.Structure):
+    _fields_ = [("n", C.c_int), ("n_pad", C.c_int), ("nslices", C.c_int), ("nsys", C.c_int),
+                ("sigma", C.c_int), ("idx16", C.c_int), ("grid", C.c_int),
+                ("nnz", C.c_longlong), ("stored", C.c_longlong), ("bytes_spmv", C.c_longlong)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tau", C.c_double), ("rel_tol", C.c_double), ("max_iters", C.c_int),
+                ("graph_iters", C.c_int), ("use_graph", C.c_int), ("trace_cap", C.c_int),
+                ("stall_limit", C.c_int)]
+
+
+class Status(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("rz", C.c_double), ("rr", C.c_double),
+                ("pap", C.c_double), ("alpha", C.c_double), ("beta", C.c_double), ("rr0", C.c_double),
+                ("tau", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _sig(name, restype, argtypes):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+# drop-in surface (void returns, silent no-op for n < 0)
+_spmv_csr = _sig("spmv_csr_kernel", None, [_i32p, _i32p, _f64p, _f64p, _f64p, C.c_int])
+_spmv_blocked = _sig("spmv_blocked_kernel", None, [_i32p, _i32p, _f64p, _f64p, _f64p, C.c_int])
+_init_pcg = _sig("init_pcg_kernel", None, [_f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int])
+_dot = _sig("dot_kernel", None, [_f64p, _f64p, _f64p, C.c_int])
+_update_xrz = _sig("update_xrz_kernel", None,
+                   [_f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_double, C.c_int])
+_update_p = _sig("update_p_kernel", None, [_f64p, _f64p, C.c_double, C.c_int])
+_legacy_reset = _sig("xp_legacy_reset", None, [])
+_last_error = _sig("xp_last_error", C.c_int, [C.c_char_p, C.c_int])
+device_count = _sig("xp_device_count", C.c_int, [])
+
+_m_create = _sig("xp_matrix_create", C.c_int,
+                 [C.c_int, _i32p, _i32p, _f64p, C.POINTER(MatrixOpts), C.POINTER(_vp)])
+_m_create_batch = _sig("xp_matrix_create_batch", C.c_int,
+                       [C.c_int, _i32p, _i32p, _i32p, _f64p, C.POINTER(MatrixOpts), C.POINTER(_vp)])
+_m_destroy = _sig("xp_matrix_destroy", None, [_vp])
+_m_info = _sig("xp_matrix_get_info", C.c_int, [_vp, C.POINTER(MatrixInfo)])
+_m_perm = _sig("xp_matrix_permutation", C.c_int, [_vp, _i32p])
+_m_set_grid = _sig("xp_matrix_set_grid", C.c_int, [_vp, C.c_int])
+_spmv_f32 = _sig("xp_spmv_f32", C.c_int, [_vp, _vp, _vp, _vp])
+_spmv_f64 = _sig("xp_spmv_f64", C.c_int, [_vp, _vp, _vp, _vp])
+_to_solve = _sig("xp_to_solve_order", C.c_int, [_vp, _vp, _vp, _vp])
+_from_solve = _sig("xp_from_solve_order", C.c_int, [_vp, _vp, _vp, _vp])
+_csr_spmv_f64 = _sig("xp_csr_spmv_f64", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p])
+
+_p_create = _sig("xp_pcg_create", C.c_int, [_vp, C.POINTER(SolveOpts), C.POINTER(_vp)])
+_p_destroy = _sig("xp_pcg_destroy", None, [_vp])
+_p_set_rhs = _sig("xp_pcg_set_rhs", C.c_int, [_vp, _f64p, _vp])
+_p_run = _sig("xp_pcg_run", C.c_int, [_vp])
+_p_run_fixed = _sig("xp_pcg_run_fixed", C.c_int, [_vp, C.c_int])
+_p_run_refined = _sig("xp_pcg_run_refined", C.c_int,
+                      [_vp, C.c_double, C.c_double, C.c_int, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
+_p_status = _sig("xp_pcg_get_status", C.c_int, [_vp, C.c_int, C.POINTER(Status)])
+_p_get_x = _sig("xp_pcg_get_x", C.c_int, [_vp, _f64p])
+_p_trace = _sig("xp_pcg_get_trace", C.c_int, [_vp, C.c_int, _f64p, C.c_int])
+_p_true_res = _sig("xp_pcg_true_residual", C.c_int, [_vp, _f64p])
+_p_timing = _sig("xp_pcg_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int)])
+
+_ds_load = _sig("xp_dataset_load", C.c_int, [C.c_char_p, C.POINTER(_vp)])
+_ds_save = _sig("xp_dataset_save", C.c_int, [_vp, C.c_char_p])
+_gen2d = _sig("xp_gen_diffusion2d", C.c_int, [C.c_int, C.c_int, C.c_double, C.c_ulonglong, C.POINTER(_vp)])
+_gen3d = _sig("xp_gen_diffusion3d", C.c_int, [C.c_int, C.c_double, C.c_ulonglong, C.POINTER(_vp)])
+_genfem = _sig("xp_gen_fem2d", C.c_int,
+               [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.c_ulonglong, C.POINTER(_vp)])
+_csr_from = _sig("xp_csr_from_arrays", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.POINTER(_vp)])
+_csr_info = _sig("xp_csr_info", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)])
+_csr_copy = _sig("xp_csr_copy", C.c_int, [_vp, _i32p, _i32p, _f64p, _f64p, _f64p])
+_csr_free = _sig("xp_csr_free", None, [_vp])
+_sell_build = _sig("xp_sell_build_host", C.c_int,
+                   [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                    C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
+_sell_copy = _sig("xp_sell_copy_host", C.c_int, [_i32p, _i32p, _i32p, _f32p, _u16p, _i32p])
+
+
+class XplusError(RuntimeError):
+    pass
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(4096)
+    _last_error(buf, 4096)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise XplusError(last_error())
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- drop-in surface
+def spmv_csr_kernel(row_ptr, col_idx, values, x, y, n: int) -> None:
+    _spmv_csr(row_ptr, col_idx, values, x, y, n)
+
+
+def spmv_blocked_kernel(row_ptr, col_idx, values, x, y, n: int) -> None:
+    _spmv_blocked(row_ptr, col_idx, values, x, y, n)
+
+
+def init_pcg_kernel(b, ax, m_inv, r, z, p, metrics, n: int) -> None:
+    _init_pcg(b, ax, m_inv, r, z, p, metrics, n)
+
+
+def dot_kernel(a, b, out, n: int) -> None:
+    _dot(a, b, out, n)
+
+
+def update_xrz_kernel(x, p, r, ap, m_inv, z, metrics, alpha: float, n: int) -> None:
+    _update_xrz(x, p, r, ap, m_inv, z, metrics, alpha, n)
+
+
+def update_p_kernel(z, p, beta: float, n: int) -> None:
+    _update_p(z, p, beta, n)
+
+
+def legacy_reset() -> None:
+    _legacy_reset()
+
+
+# ---------------------------------------------------------------- host CSR
+class Csr:
+    """Host CSR matrix with b and x0 (numpy), as CsrDataset holds them."""
+
+    def __init__(self, row_ptr, col_idx, values, b=None, x0=None):
+        self.row_ptr = _i32(row_ptr)
+        self.col_idx = _i32(col_idx)
+        self.values = _f64(values)
+        self.n = len(self.row_ptr) - 1
+        self.b = _f64(b) if b is not None else np.zeros(self.n)
+        self.x0 = _f64(x0) if x0 is not None else np.zeros(self.n)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1])
+
+    @classmethod
+    def _from_handle(cls, h: _vp) -> "Csr":
+        try:
+            n, nnz = C.c_int(), C.c_int()
+            _check(_csr_info(h, C.byref(n), C.byref(nnz)))
+            rp = np.empty(n.value + 1, np.int32)
+            ci = np.empty(max(nnz.value, 1), np.int32)
+            va = np.empty(max(nnz.value, 1), np.float64)
+            b = np.empty(max(n.value, 1), np.float64)
+            x0 = np.empty(max(n.value, 1), np.float64)
+            _check(_csr_copy(h, rp, ci, va, b, x0))
+            return cls(rp, ci[:nnz.value], va[:nnz.value], b[:n.value], x0[:n.value])
+        finally:
+            _csr_free(h)
+
+    def _handle(self) -> _vp:
+        h = _vp()
+        _check(_csr_from(self.n, self.row_ptr, self.col_idx, self.values, self.b, self.x0, C.byref(h)))
+        return h
+
+    def save(self, directory) -> None:
+        h = self._handle()
+        try:
+            _check(_ds_save(h, str(directory).encode()))
+        finally:
+            _csr_free(h)
+
+    def to_dense(self) -> np.ndarray:
+        a = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            for o in range(self.row_ptr[r], self.row_ptr[r + 1]):
+                a[r, self.col_idx[o]] += self.values[o]
+        return a
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.values, self.col_idx, self.row_ptr), shape=(self.n, self.n))
+
+
+def load_dataset(directory) -> Csr:
+    """CsrDataset::load semantics (src/CsrDataset.hpp:19-67); raises XplusError
+    carrying the reference's message on invalid input."""
+    h = _vp()
+    _check(_ds_load(str(directory).encode(), C.byref(h)))
+    return Csr._from_handle(h)
+
+
+def gen_diffusion2d(nx: int, ny: int, sigma: float = 1.0, seed: int = 0) -> Csr:
+    h = _vp()
+    _check(_gen2d(nx, ny, sigma, seed, C.byref(h)))
+    return Csr._from_handle(h)
+
+
+def gen_diffusion3d(k: int, sigma: float = 1.5, seed: int = 0) -> Csr:
+    h = _vp()
+    _check(_gen3d(k, sigma, seed, C.byref(h)))
+    return Csr._from_handle(h)
+
+
+def gen_fem2d(m: int, sigma: float = 1.0, n_incl: int = 20, contrast: float = 1e3,
+              jitter: float = 0.3, seed: int = 0) -> Csr:
+    h = _vp()
+    _check(_genfem(m, sigma, n_incl, contrast, jitter, seed, C.byref(h)))
+    return Csr._from_handle(h)
+
+
+def sell_build_host(a: Csr, sigma: int = 256) -> dict:
+    """Host-side SELL build (no GPU): returns the arrays of the format."""
+    n_pad, nsl, idx16 = C.c_int(), C.c_int(), C.c_int()
+    stored = C.c_longlong()
+    _check(_sell_build(a.n, a.row_ptr, a.col_idx, a.values, sigma, C.byref(n_pad), C.byref(nsl),
+                       C.byref(stored), C.byref(idx16)))
+    perm = np.empty(n_pad.value, np.int32)
+    sp = np.empty(nsl.value + 1, np.int32)
+    sb = np.empty(nsl.value, np.int32)
+    val = np.empty(stored.value, np.float32)
+    c16 = np.empty(stored.value if idx16.value else 1, np.uint16)
+    c32 = np.empty(1 if idx16.value else stored.value, np.int32)
+    _check(_sell_copy(perm, sp, sb, val, c16, c32))
+    return {"n_pad": n_pad.value, "nslices": nsl.value, "stored": stored.value, "idx16": idx16.value,
+            "perm": perm, "slice_ptr": sp, "slice_base": sb, "val": val,
+            "col16": c16 if idx16.value else None, "col32": None if idx16.value else c32}
+
+
+# ---------------------------------------------------------------- device matrix
+class Matrix:
+    """Device-resident SELL-C-sigma matrix (one system or a batch)."""
+
+    def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0):
+        if isinstance(systems, Csr):
+            systems = [systems]
+        self.systems = list(systems)
+        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid)
+        h = _vp()
+        if len(self.systems) == 1:
+            a = self.systems[0]
+            _check(_m_create(a.n, a.row_ptr, a.col_idx, a.values, C.byref(opts), C.byref(h)))
+        else:
+            ns = _i32([a.n for a in self.systems])
+            rp = _i32(np.concatenate([a.row_ptr for a in self.systems]))
+            ci = _i32(np.concatenate([a.col_idx for a in self.systems]))
+            va = _f64(np.concatenate([a.values for a in self.systems]))
+            _check(_m_create_batch(len(self.systems), ns, rp, ci, va, C.byref(opts), C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _m_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self) -> dict:
+        i = MatrixInfo()
+        _check(_m_info(self.h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    def permutation(self) -> np.ndarray:
+        perm = np.empty(self.info()["n_pad"], np.int32)
+        _check(_m_perm(self.h, perm))
+        return perm
+
+    def set_grid(self, grid: int) -> None:
+        _check(_m_set_grid(self.h, grid))
+
+    # device-pointer operations (ints are device addresses, e.g. tensor.data_ptr())
+    def spmv_f32(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
+        _check(_spmv_f32(self.h, x_ptr, y_ptr, stream or None))
+
+    def spmv_f64(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
+        _check(_spmv_f64(self.h, x_ptr, y_ptr, stream or None))
+
+    def to_solve_order(self, nat_ptr: int, solve_ptr: int, stream: int = 0) -> None:
+        _check(_to_solve(self.h, nat_ptr, solve_ptr, stream or None))
+
+    def from_solve_order(self, solve_ptr: int, nat_ptr: int, stream: int = 0) -> None:
+        _check(_from_solve(self.h, solve_ptr, nat_ptr, stream or None))
+
+
+def csr_spmv_f64(a: Csr, x) -> np.ndarray:
+    y = np.zeros(a.n)
+    _check(_csr_spmv_f64(a.n, a.row_ptr, a.col_idx, a.values, _f64(x), y))
+    return y
+
+
+# ---------------------------------------------------------------- solver
+class Pcg:
+    """Graph-captured Jacobi-PCG over a Matrix (one or many systems)."""
+
+    def __init__(self, matrix: Matrix, tau: float = 1e-10, rel_tol: float = 0.0, max_iters: int = 0,
+                 graph_iters: int = 0, use_graph: bool = True, trace_cap: int = 0, stall_limit: int = 0):
+        self.matrix = matrix
+        self.opts = SolveOpts(tau, rel_tol, max_iters, graph_iters, int(use_graph), trace_cap, stall_limit)
+        h = _vp()
+        _check(_p_create(matrix.h, C.byref(self.opts), C.byref(h)))
+        self.h = h
+        self.nsys = len(matrix.systems)
+        self.n = sum(a.n for a in matrix.systems)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _p_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_rhs(self, b, x0=None) -> None:
+        b = _f64(b)
+        if x0 is None:
+            _check(_p_set_rhs(self.h, b, None))
+        else:
+            x0 = _f64(x0)
+            _check(_p_set_rhs(self.h, b, x0.ctypes.data_as(_vp)))
+
+    def run(self) -> None:
+        _check(_p_run(self.h))
+
+    def run_fixed(self, iters: int) -> None:
+        _check(_p_run_fixed(self.h, iters))
+
+    def run_refined(self, outer_rel: float = 1e-10, inner_rel: float = 1e-5, max_outer: int = 20):
+        rel = np.zeros(self.nsys)
+        outer, total = C.c_int(), C.c_int()
+        _check(_p_run_refined(self.h, outer_rel, inner_rel, max_outer, rel, C.byref(outer), C.byref(total)))
+        return rel, outer.value, total.value
+
+    def status(self, sys: int = 0) -> dict:
+        s = Status()
+        _check(_p_status(self.h, sys, C.byref(s)))
+        return s.as_dict()
+
+    def x(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        _check(_p_get_x(self.h, out))
+        return out
+
+    def trace(self, sys: int = 0) -> np.ndarray:
+        """Rows {alpha, beta, rz, rr} per completed iteration."""
+        cap = max(self.opts.trace_cap, 1)
+        rows = np.zeros(cap * 4)
+        got = _p_trace(self.h, sys, rows, cap)
+        if got < 0:
+            raise XplusError(last_error())
+        return rows[: got * 4].reshape(got, 4)
+
+    def true_residual(self) -> np.ndarray:
+        out = np.zeros(self.nsys)
+        _check(_p_true_res(self.h, out))
+        return out
+
+    def last_timing(self):
+        ms, launches = C.c_double(), C.c_int()
+        _check(_p_timing(self.h, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
