@@ -1,0 +1,171 @@
+"""Benchmark: time to relative residual 1e-6 of Jacobi-PCG on one MI355X, plus
+the standalone SpMV's HBM roofline, on the thermal-FEM workload
+(P1 FEM stiffness, 1100 x 1100 node grid, n = 1,205,604, ~8.4M nonzeros,
+log-normal conductivity sigma = 1 with 20 inclusions at 1e3 contrast,
+b ~ U[-1,1], x0 = 0, synthetic, seed 20263203).
+
+A step is one complete solve from x0 to ||r||/||b|| <= 1e-6 with the matrix,
+b and x0 resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid M] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "project-xplus_amd"))
+
+METRIC = "time-to-1e-6 residual: learned-precond vs plain Jacobi-PCG; SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_roofline(X, m, a, reps: int = 200) -> dict:
+    """Times the standalone FP32 SELL SpMV (kernel k_spmv_f32) with HIP events
+    recorded on the stream the kernel is launched on."""
+    import torch
+    info = m.info()
+    stream = torch.cuda.Stream()
+    x = torch.rand(info["n_pad"], dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    sp = stream.cuda_stream
+    torch.cuda.synchronize()
+    for _ in range(20):
+        m.spmv_f32(x.data_ptr(), y.data_ptr(), sp)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+        for _ in range(reps):
+            m.spmv_f32(x.data_ptr(), y.data_ptr(), sp)
+        e1.record(stream)
+    e1.synchronize()
+    us = e0.elapsed_time(e1) * 1000.0 / reps
+    alg_bytes = 8 * a.nnz + 12 * a.n                 # FP32 value + int32 column per nonzero, row ptr, x, y
+    achieved = alg_bytes / (us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv_f32<true>",
+            "avg_us": round(us, 3), "alg_bytes": alg_bytes, "stream_bytes": info["bytes_spmv"]}
+
+
+def cpu_baseline(a, gpu_iters: int, target_s: float = 15.0) -> dict:
+    """The oracle's FP64 golden Jacobi-PCG (single thread, -O2), timed on a bounded
+    sample of the same solve (its first K iterations); projected to the
+    iteration count of the full solve."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    tau = 1e-300
+    k = 20
+    t0 = time.perf_counter()
+    O.golden(a, tau=tau, max_iters=k, trace_cap=1)
+    dt = time.perf_counter() - t0
+    k2 = max(k, min(2000, int(k * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    O.golden(a, tau=tau, max_iters=k2, trace_cap=1)
+    dt2 = time.perf_counter() - t0
+    per_iter_ms = dt2 * 1000.0 / k2
+    return {"value": round(per_iter_ms * gpu_iters, 1), "unit": "ms (projected time to 1e-6)", "cores": 1,
+            "kind": "port", "sample": f"first {k2} iterations of the same solve in {dt2:.2f} s "
+                                      f"({per_iter_ms:.2f} ms/iteration) x {gpu_iters} iterations",
+            "host": os.uname().machine}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--mesh", type=int, default=1100)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the solver shards as described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    import xplus_amd as X
+
+    if X.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    seed = 20261203 + 1000 * 2
+    t0 = time.time()
+    a = X.gen_fem2d(args.mesh, 1.0, 20, 1e3, 0.3, seed)
+    log(f"generated FEM mesh {args.mesh}: n={a.n} nnz={a.nnz} in {time.time() - t0:.1f}s")
+    m = X.Matrix(a, sigma=256, keep_lo=True, grid=args.grid)
+    info = m.info()
+    log(f"matrix: {info}")
+
+    s = X.Pcg(m, rel_tol=1e-6)
+    s.set_rhs(a.b, a.x0)
+    for w in range(args.warmup):
+        s.run()
+        log(f"warmup {w}: {s.status()['iterations']} iterations, {s.last_timing()[0]:.2f} ms")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = []
+    for k in range(args.steps):
+        s.run()
+        dev_ms.append(s.last_timing()[0])
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    st = s.status()
+    true_rel = float(s.true_residual()[0] / np.linalg.norm(a.b))
+    log(f"solve: {st['iterations']} iterations status={X.STATUS_NAMES[st['status']]} "
+        f"wall {wall_ms:.2f} ms/solve, device {np.mean(dev_ms):.2f} ms, true rel residual {true_rel:.3e}")
+    if st["status"] != X.CONVERGED:
+        print(f"solve did not converge: {st}", file=sys.stderr)
+        return 1
+
+    roof = spmv_roofline(X, m, a)
+    log(f"spmv: {roof}")
+    cpu = None if args.no_cpu else cpu_baseline(a, st["iterations"])
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(wall_ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"thermal-FEM P1 {args.mesh}x{args.mesh} nodes, n={a.n}, nnz={a.nnz}, "
+                               f"plain Jacobi-PCG to rel 1e-6",
+                   "n": a.n, "nnz": a.nnz, "method": "plain Jacobi-PCG (FP32 vectors/SpMV, FP64 reductions)",
+                   "iterations": st["iterations"], "grid": info["grid"], "sigma": info["sigma"]},
+        "plain_ms": round(wall_ms, 3),
+        "plain_device_ms": round(float(np.mean(dev_ms)), 3),
+        "learned_ms": None,
+        "speedup_learned_vs_plain": None,
+        "ms_per_iteration": round(wall_ms / st["iterations"], 5),
+        "true_rel_residual": true_rel,
+        "spmv_hbm_gbs": roof["achieved"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -291,7 +291,7 @@ struct xp_pcg {
     xp_matrix* m = nullptr;
     xp_solve_opts opts{};
     hipStream_t stream = nullptr;
-    DevBuf<float> x, r, z, p, ap, b, minv;
+    DevBuf<float> x, r, z, p, ap, b, minv, x0s;  // x0s: x0 in solve order, restored before each run
     DevBuf<xp::SysState> st;
     DevBuf<xp::TraceRow> trace;
     int trace_stride = 0;
@@ -469,7 +469,8 @@ void set_rhs(xp_pcg& s, const double* b, const double* x0) {
     tmp.upload(b, n, s.stream);
     xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s.b.p, s.stream);
     tmp.upload(s.x0_nat.data(), n, s.stream);
-    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s.x.p, s.stream);
+    if (s.x0s.n != static_cast<size_t>(m.n_pad)) s.x0s.alloc(static_cast<size_t>(m.n_pad));
+    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s.x0s.p, s.stream);
     HIPX(hipStreamSynchronize(s.stream));
     s.have_rhs = true;
 }
@@ -800,13 +801,9 @@ int xp_pcg_set_rhs(xp_pcg* s, const double* b, const double* x0) {
 int xp_pcg_run(xp_pcg* s) {
     return guarded([&] {
         if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run: call xp_pcg_set_rhs first");
-        // x holds x0 (solve order) until the run starts; restore it so that
-        // repeated runs start from the same point.
-        DevBuf<double> tmp(std::max<size_t>(static_cast<size_t>(s->m->n), 1));
-        tmp.upload(s->x0_nat.data(), static_cast<size_t>(s->m->n), s->stream);
-        xp::launch_gather_d2f(s->m->n_pad, s->m->d_perm.p, tmp.p, s->x.p, s->stream);
+        // Every run starts from x0 and the configured state.
+        HIPX(hipMemcpyAsync(s->x.p, s->x0s.p, s->x0s.n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
         s->reset_state();
-        HIPX(hipStreamSynchronize(s->stream));
         long long cap = 0;
         for (const auto& t : s->st_init) cap = std::max<long long>(cap, t.max_iters);
         s->run_loop(cap + s->graph_iters);
@@ -817,13 +814,10 @@ int xp_pcg_run_fixed(xp_pcg* s, int iters) {
     return guarded([&] {
         if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run_fixed: call xp_pcg_set_rhs first");
         if (iters < 0) throw std::runtime_error("iters must be >= 0");
-        DevBuf<double> tmp(std::max<size_t>(static_cast<size_t>(s->m->n), 1));
-        tmp.upload(s->x0_nat.data(), static_cast<size_t>(s->m->n), s->stream);
-        xp::launch_gather_d2f(s->m->n_pad, s->m->d_perm.p, tmp.p, s->x.p, s->stream);
+        HIPX(hipMemcpyAsync(s->x.p, s->x0s.p, s->x0s.n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
         std::vector<xp::SysState> init = s->st_init;
         for (auto& t : init) t.max_iters = std::min(t.max_iters, std::max(iters, 0));
         s->st.upload(init.data(), init.size(), s->stream);
-        HIPX(hipStreamSynchronize(s->stream));
         s->run_loop(iters);
     });
 }

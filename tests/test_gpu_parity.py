@@ -1,0 +1,336 @@
+"""GPU parity: the HIP path through the C-ABI against the oracle.
+
+Bars (written in each test):
+  * drop-in FP64 operations: elementwise results equal to the oracle up to
+    the 1-ulp difference of a fused multiply-add, reductions within
+    n * eps * sum|a_i b_i| (parallel order), the multi-kernel solve within
+    the reference's own max_abs_diff <= 1e-9 (archived/host/main.cpp:140);
+  * FP32 SpMV: |y - A32 x32| <= (k+1) u sum|a32 x32| per row (u = 2^-24);
+  * FP32 PCG against the FP64 golden: iteration count within +-1, x within
+    relative 1e-5, per-iteration rr trace within relative 1e-3.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import xplus_amd as X
+
+pytestmark = pytest.mark.gpu
+N512 = O.ROOT / "tests" / "golden" / "n512"
+EPS = np.finfo(np.float64).eps
+U32 = 2.0 ** -24
+
+
+def n512():
+    d = O.load(N512)
+    return X.Csr(d["row_ptr"], d["col_idx"], d["values"], d["b"], d["x0"])
+
+
+def c1(seed=20261204):
+    return X.gen_diffusion2d(32, 32, 1.0, seed)
+
+
+def close_fma(a, b):
+    """Equal up to the one rounding a fused multiply-add saves."""
+    np.testing.assert_allclose(a, b, rtol=4 * EPS, atol=1e-300)
+
+
+# ------------------------------------------------------------------ drop-in surface
+@pytest.mark.parametrize("n", [700, 5000, 100_003])
+def test_dropin_elementwise_ops(gpu, n):
+    print(f"dropin ops n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    b, ax, mi, x, p, r, ap = (rng.standard_normal(n) for _ in range(7))
+    ro, zo, po, mo = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(2)
+    rg, zg, pg, mg = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(2)
+    O.oracle.orc_k_init_pcg(b, ax, mi, ro, zo, po, mo, n)
+    X.init_pcg_kernel(b, ax, mi, rg, zg, pg, mg, n)
+    np.testing.assert_array_equal(rg, ro)
+    close_fma(zg, zo)
+    np.testing.assert_array_equal(pg, zg)
+    np.testing.assert_allclose(mg, mo, rtol=n * EPS)
+    do, dg = np.zeros(1), np.zeros(1)
+    O.oracle.orc_k_dot(x, p, do, n)
+    X.dot_kernel(x, p, dg, n)
+    assert abs(dg[0] - do[0]) <= n * EPS * np.abs(x * p).sum()
+    xo, xg, ro, rg = x.copy(), x.copy(), r.copy(), r.copy()
+    zo, zg = np.zeros(n), np.zeros(n)
+    O.oracle.orc_k_update_xrz(xo, p, ro, ap, mi, zo, mo, 0.731, n)
+    X.update_xrz_kernel(xg, p, rg, ap, mi, zg, mg, 0.731, n)
+    close_fma(xg, xo)
+    close_fma(rg, ro)
+    np.testing.assert_allclose(zg, zo, rtol=8 * EPS, atol=1e-300)
+    np.testing.assert_allclose(mg, mo, rtol=n * EPS * 4)
+    po, pg = p.copy(), p.copy()
+    O.oracle.orc_k_update_p(zo, po, -0.25, n)
+    X.update_p_kernel(zo, pg, -0.25, n)
+    close_fma(pg, po)
+    # negative n: silent no-op, nothing written
+    before = pg.copy()
+    X.update_p_kernel(zo, pg, 3.0, -5)
+    np.testing.assert_array_equal(pg, before)
+
+
+@pytest.mark.parametrize("which", ["n512", "c1", "fem"])
+def test_dropin_spmv(gpu, which):
+    a = {"n512": n512, "c1": c1, "fem": lambda: X.gen_fem2d(302, 1.0, 20, 1e3, 0.3, 5)}[which]()
+    print(f"dropin spmv {which} n={a.n}", flush=True)
+    x = np.random.default_rng(1).standard_normal(a.n)
+    yo = O.spmv(a, x)
+    for fn in (X.spmv_csr_kernel, X.spmv_blocked_kernel):
+        yg = np.zeros(a.n)
+        fn(a.row_ptr, a.col_idx, a.values, x, yg, a.n)
+        absrow = O.spmv(X.Csr(a.row_ptr, a.col_idx, np.abs(a.values)), np.abs(x))
+        kmax = np.diff(a.row_ptr).max()
+        assert np.all(np.abs(yg - yo) <= (kmax + 1) * EPS * absrow)
+    # the cache notices an in-place edit of the values
+    a.values[0] *= 2.0
+    yg = np.zeros(a.n)
+    X.spmv_csr_kernel(a.row_ptr, a.col_idx, a.values, x, yg, a.n)
+    np.testing.assert_allclose(yg, O.spmv(a, x), rtol=1e-12, atol=1e-12)
+
+
+def run_dropin_multi_kernel(a, tau, max_iters):
+    """run_local_multi_kernel_solver's loop (archived/host/multi_kernel_solver.hpp:100-234)
+    driven through the library's six entry points."""
+    n = a.n
+    diag = np.array([a.values[a.row_ptr[r] + np.where(a.col_idx[a.row_ptr[r]:a.row_ptr[r + 1]] == r)[0][0]]
+                     for r in range(n)])
+    minv = 1.0 / diag
+    x = a.x0.copy()
+    r, z, p, y = (np.zeros(n) for _ in range(4))
+    met, dot = np.zeros(2), np.zeros(1)
+    X.spmv_csr_kernel(a.row_ptr, a.col_idx, a.values, x, y, n)
+    X.init_pcg_kernel(a.b, y, minv, r, z, p, met, n)
+    rz, rr = met
+    it = 0
+    emi = max_iters if max_iters > 0 else max(4 * n, 1000)
+    while it < emi and rr > tau:
+        X.spmv_csr_kernel(a.row_ptr, a.col_idx, a.values, p, y, n)
+        X.dot_kernel(p, y, dot, n)
+        alpha = rz / dot[0]
+        X.update_xrz_kernel(x, p, r, y, minv, z, met, alpha, n)
+        beta = met[0] / rz
+        X.update_p_kernel(z, p, beta, n)
+        rz, rr = met
+        it += 1
+    return x, it
+
+
+@pytest.mark.parametrize("which", ["n512", "c1"])
+def test_dropin_multi_kernel_solve_matches_golden(gpu, which):
+    a = n512() if which == "n512" else c1()
+    tau = 1e-8 if which == "n512" else 1e-12 * float(a.b @ a.b)
+    print(f"dropin multi-kernel {which}", flush=True)
+    x, it = run_dropin_multi_kernel(a, tau, 0)
+    g = O.golden(a, tau=tau)
+    assert it == g["iterations"]
+    assert np.max(np.abs(x - g["x"])) <= 1e-9          # archived/host/main.cpp:140
+
+
+def test_reference_main_linked_against_library(gpu):
+    """The reference's own archived/host/main.cpp, compiled unchanged and linked
+    against libxplus_amd.so instead of its CPU kernels, passes its own check."""
+    import subprocess
+    if not O.REF_ON_AMD_BIN.exists():
+        pytest.skip("oracle/_ref/xplus_local_on_amd not built")
+    out = subprocess.run([str(O.REF_ON_AMD_BIN), str(N512), "--tau", "1e-8", "--max-iters", "100"],
+                         capture_output=True, text=True, timeout=120)
+    print(out.stdout[-600:])
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "[done] iter=41" in out.stdout
+
+
+def test_cli_drop_in(gpu):
+    import subprocess
+    out = subprocess.run([str(X.CLI_PATH), str(N512), "--tau", "1e-8", "--max-iters", "100", "--device-solver"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "[done] iter=41" in out.stdout and "status=converged" in out.stdout
+    assert "[device] iter=" in out.stdout
+
+
+# ------------------------------------------------------------------ device FP32 SpMV
+@pytest.mark.parametrize("sigma", [64, 256])
+def test_spmv_f32_rounding_bound(gpu, sigma):
+    import torch
+    a = X.gen_fem2d(302, 1.0, 20, 1e3, 0.3, 11)
+    m = X.Matrix(a, sigma=sigma, keep_lo=True)
+    info = m.info()
+    print(f"spmv f32 n={a.n} info={info}", flush=True)
+    perm = m.permutation()
+    x_nat = np.random.default_rng(2).standard_normal(a.n)
+    xs = torch.zeros(info["n_pad"], dtype=torch.float32, device="cuda")
+    ys = torch.zeros_like(xs)
+    xn = torch.from_numpy(x_nat).cuda()
+    m.to_solve_order(xn.data_ptr(), xs.data_ptr())
+    m.spmv_f32(xs.data_ptr(), ys.data_ptr())
+    torch.cuda.synchronize()
+    y = np.zeros(a.n)
+    y[perm[perm >= 0]] = ys.cpu().numpy()[perm >= 0]
+    a32 = X.Csr(a.row_ptr, a.col_idx, a.values.astype(np.float32).astype(np.float64))
+    x32 = x_nat.astype(np.float32).astype(np.float64)
+    ref = O.spmv(a32, x32)
+    absrow = O.spmv(X.Csr(a.row_ptr, a.col_idx, np.abs(a32.values)), np.abs(x32))
+    k = np.diff(a.row_ptr).max() + 1
+    assert np.all(np.abs(y - ref) <= k * U32 * absrow)
+    # FP64 SpMV with hi+lo values: ~2^-48 relative per entry
+    x64s = torch.zeros(info["n_pad"], dtype=torch.float64, device="cuda")
+    y64s = torch.zeros_like(x64s)
+    x64s[: a.n] = 0
+    xperm = np.zeros(info["n_pad"])
+    xperm[perm >= 0] = x_nat[perm[perm >= 0]]
+    x64s.copy_(torch.from_numpy(xperm))
+    m.spmv_f64(x64s.data_ptr(), y64s.data_ptr())
+    torch.cuda.synchronize()
+    y64 = np.zeros(a.n)
+    y64[perm[perm >= 0]] = y64s.cpu().numpy()[perm >= 0]
+    absrow64 = O.spmv(X.Csr(a.row_ptr, a.col_idx, np.abs(a.values)), np.abs(x_nat))
+    assert np.all(np.abs(y64 - O.spmv(a, x_nat)) <= (k * EPS + 2.0 ** -46) * absrow64)
+
+
+# ------------------------------------------------------------------ device PCG vs golden
+def check_against_golden(a, tau, trace_tol=1e-3, x_tol=1e-5, **kw):
+    g = O.golden(a, tau=tau)
+    m = X.Matrix(a, keep_lo=True, **kw)
+    s = X.Pcg(m, tau=tau, trace_cap=g["iterations"] + 8)
+    s.set_rhs(a.b, a.x0)
+    s.run()
+    st = s.status()
+    print(f"n={a.n} golden iters={g['iterations']} gpu iters={st['iterations']} status={st['status']}", flush=True)
+    assert st["status"] == X.CONVERGED
+    assert abs(st["iterations"] - g["iterations"]) <= 1
+    x = s.x()
+    relx = np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"])
+    assert relx <= x_tol, relx
+    tr = s.trace()
+    k = min(len(tr), len(g["trace"]))
+    gt = g["trace"][:k]
+    np.testing.assert_allclose(tr[:k, 3], gt[:, 4], rtol=trace_tol)   # rr
+    np.testing.assert_allclose(tr[:k, 0], gt[:, 1], rtol=trace_tol)   # alpha
+    np.testing.assert_allclose(tr[:k, 1], gt[:, 2], rtol=trace_tol)   # beta
+    return s, g
+
+
+@pytest.mark.parametrize("seed", [20261204, 1, 2])
+def test_pcg_c1_matches_golden(gpu, seed):
+    a = c1(seed)
+    check_against_golden(a, 1e-12 * float(a.b @ a.b))
+
+
+def test_pcg_n512_known_answers(gpu):
+    a = n512()
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, tau=1e300)
+    s.set_rhs(a.b, a.x0)
+    s.run()
+    assert s.status()["rr0"] == pytest.approx(3.427876635047e+01, rel=1e-6)
+    s = X.Pcg(m, tau=1e-30, max_iters=1, trace_cap=4)
+    s.set_rhs(a.b, a.x0)
+    s.run()
+    assert s.trace()[0, 3] == pytest.approx(9.407299557997e+01, rel=1e-6)
+    assert s.true_residual()[0] == pytest.approx(9.699123342347, rel=1e-6)
+    check_against_golden(a, 1e-8)
+
+
+def test_pcg_fem_fixed_iterations(gpu):
+    """Config-2 family at n = 90K, fixed K = 60 iterations: x_K and the trace."""
+    a = X.gen_fem2d(302, 1.0, 20, 1e3, 0.3, 20263203)
+    K = 60
+    g = O.golden(a, tau=1e-300, max_iters=K)
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, tau=1e-300, trace_cap=K)
+    s.set_rhs(a.b, a.x0)
+    s.run_fixed(K)
+    st = s.status()
+    assert st["iterations"] == K and st["status"] == X.MAX_ITER
+    x = s.x()
+    assert np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"]) <= 1e-5
+    np.testing.assert_allclose(s.trace()[:, 3], g["trace"][:, 4], rtol=1e-3)
+
+
+def test_pcg_graph_and_direct_launch_agree_bitwise(gpu):
+    a = c1(7)
+    tau = 1e-12 * float(a.b @ a.b)
+    m = X.Matrix(a)
+    outs = []
+    for use_graph in (True, False, True):
+        s = X.Pcg(m, tau=tau, use_graph=use_graph, graph_iters=16)
+        s.set_rhs(a.b)
+        s.run()
+        outs.append((s.status()["iterations"], s.x()))
+    assert outs[0][0] == outs[1][0] == outs[2][0]
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    np.testing.assert_array_equal(outs[0][1], outs[2][1])
+
+
+def test_pcg_breakdown_on_nan_rhs(gpu):
+    a = c1(3)
+    b = a.b.copy()
+    b[17] = np.nan
+    s = X.Pcg(X.Matrix(a), tau=1e-10)
+    s.set_rhs(b)
+    s.run()
+    st = s.status()
+    assert st["status"] == X.BREAKDOWN and st["iterations"] == 0
+
+
+def test_pcg_max_iter_status(gpu):
+    a = c1(4)
+    s = X.Pcg(X.Matrix(a), tau=1e-300, max_iters=5)
+    s.set_rhs(a.b)
+    s.run()
+    st = s.status()
+    assert st["status"] == X.MAX_ITER and st["iterations"] == 5
+
+
+def test_pcg_rejects_bad_diagonal(gpu):
+    a = X.Csr([0, 1, 2], [1, 0], [1.0, 1.0], [1.0, 1.0])
+    with pytest.raises(X.XplusError, match="missing diagonal entry at row 0"):
+        X.Pcg(X.Matrix(a))
+    a = X.Csr([0, 1, 2], [0, 1], [1.0, 0.0], [1.0, 1.0])
+    with pytest.raises(X.XplusError, match="zero diagonal entry at row 1"):
+        X.Pcg(X.Matrix(a))
+
+
+# ------------------------------------------------------------------ refinement
+def test_refined_reaches_1e10(gpu):
+    a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, 20263303)
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, rel_tol=1e-6)
+    s.set_rhs(a.b)
+    rel, outer, total = s.run_refined(1e-10, 1e-5, 20)
+    print(f"refined: rel={rel} outer={outer} inner_total={total}", flush=True)
+    assert rel[0] <= 1e-10 and outer >= 2
+    x = s.x()
+    # independent FP64 check of the true residual and the error against FP64 PCG
+    res = O.residual_norm(a, x) / np.linalg.norm(a.b)
+    assert res <= 2e-10
+    g = O.golden(a, tau=(1e-12 * np.linalg.norm(a.b)) ** 2)
+    err = np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"])
+    print(f"error vs FP64 oracle: {err:.3e}", flush=True)
+    assert err <= 1e-6
+
+
+# ------------------------------------------------------------------ batched
+def test_batched_matches_individual_and_golden(gpu):
+    systems = [X.gen_diffusion3d(k, 1.5, 20264203 + i) for i, k in enumerate((16, 19, 23, 17))]
+    rel = 1e-6
+    m = X.Matrix(systems, keep_lo=True)
+    s = X.Pcg(m, rel_tol=rel, trace_cap=0)
+    s.set_rhs(np.concatenate([a.b for a in systems]))
+    s.run()
+    xb = s.x()
+    off = 0
+    for i, a in enumerate(systems):
+        st = s.status(i)
+        tau = (rel * np.linalg.norm(a.b)) ** 2
+        g = O.golden(a, tau=tau, trace_cap=1)
+        print(f"sys {i} n={a.n} batched iters={st['iterations']} golden={g['iterations']}", flush=True)
+        assert st["status"] == X.CONVERGED
+        assert abs(st["iterations"] - g["iterations"]) <= 1
+        xi = xb[off:off + a.n]
+        assert np.linalg.norm(xi - g["x"]) / np.linalg.norm(g["x"]) <= 1e-5
+        off += a.n
