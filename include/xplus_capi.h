@@ -120,6 +120,20 @@ int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* str
 int xp_csr_spmv_f64(int n, const int* row_ptr, const int* col_idx, const double* values,
                     const double* x, double* y);
 
+/* Device memory and timing helpers for drivers that hold no other GPU
+ * runtime (tests, benchmarks): plain hipMalloc/hipMemcpy on the library's
+ * stream. */
+int xp_device_alloc(unsigned long long bytes, void** out);
+int xp_device_free(void* ptr);
+int xp_memcpy_htod(void* dst, const void* src, unsigned long long bytes);
+int xp_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes);
+int xp_device_synchronize(void);
+/* Times `reps` back-to-back launches of the FP32 SpMV (kernel k_spmv_f32)
+ * with HIP events recorded on the stream the kernels are launched on (the
+ * library stream), after `warmup` untimed launches; x is filled with a
+ * fixed pattern.  avg_us receives the mean time per launch. */
+int xp_spmv_f32_time(xp_matrix* m, int warmup, int reps, double* avg_us);
+
 /* ===================================================================== *
  * 3. Solver.
  *

@@ -13,16 +13,22 @@
 //    of the diagonal slot *is* p[row]); the init step r = b - Ax, z, p, rz, rr
 //    is fused into the SpMV of x0 (as write_cuper_outputs<Init> does on the
 //    FPGA, kernels/cuper_pcg_control_kernel.cpp:463-555).
-//  * Reductions: FP64 products of the FP32 operands (exact), summed per lane,
-//    per wavefront with cross-lane shuffles, per workgroup through LDS, then
-//    across workgroups by the last workgroup to arrive (agent-scope atomic
-//    ticket), which sums the per-workgroup partials in workgroup order.  The
-//    order is fixed, so results are bitwise reproducible run to run.
+//  * Reductions without atomics: FP64 products of the FP32 operands (exact),
+//    summed per lane, per wavefront with cross-lane shuffles, per workgroup
+//    through LDS, and written as one partial per workgroup.  The kernel that
+//    needs the total (the next one in stream order) has every workgroup sum
+//    the few hundred partials of its system itself, in a fixed order, so all
+//    workgroups agree bit for bit and the result is reproducible run to run.
+//    Kernel boundaries are the only synchronisation: no fences, no tickets
+//    (a 2048-way atomic ticket cost ~45 us per kernel on MI355X).
 //  * The scalar part of a PCG step (alpha, beta, convergence and breakdown
-//    tests, trace) runs in that last workgroup, so a whole iteration is three
-//    kernels with no host round trip and can be captured in a HIP graph.  A
-//    system that has stopped turns every later kernel of its rows into a
-//    no-op, so the iteration count stays exact when a graph runs past it.
+//    tests, trace) is evaluated redundantly by every workgroup from those
+//    sums; the first workgroup of a system writes the next state slot (two
+//    slots, alternating per iteration, so no kernel reads what it writes).
+//    A whole iteration is three kernels with no host round trip and is
+//    captured in a HIP graph.  A system that has stopped turns every later
+//    kernel of its rows into a no-op (the state is carried forward), so the
+//    iteration count stays exact when a graph runs past it.
 //  * Workgroup -> rows mapping is XCD-aware: physical workgroup b runs on XCD
 //    b % 8, so logical block (b % 8) * (grid / 8) + b / 8 gives each XCD one
 //    contiguous range of rows and keeps the x entries they share in one L2.
@@ -76,41 +82,23 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Publishes this workgroup's NV partials and returns true in the last
-// workgroup of `sys` to arrive, whose threads then hold the system totals
-// (partials summed in logical-block order).  Every thread must call it.
+// Sum of NV partial slots over the logical blocks [range.x, range.y) of a
+// system, written by an earlier kernel.  Every thread receives the totals;
+// every workgroup computes them in the same order.
 template <int NV>
-__device__ bool reduce_across_blocks(const Launch& L, int lb, int sys, int slot, double (&v)[NV]) {
-    __shared__ int is_last;
-    block_sum<NV>(v);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) L.partials[static_cast<size_t>(lb) * 2 + k] = v[k];
-        __threadfence();  // release the partials at agent scope (all XCDs)
-        const int2 range = L.sys_blocks[sys];
-        unsigned* ctr = L.counters + static_cast<size_t>(sys) * kCounterSlots + slot;
-        const unsigned ticket = atomicAdd(ctr, 1u);
-        const bool last = ticket == static_cast<unsigned>(range.y - range.x - 1);
-        if (last) atomicExch(ctr, 0u);  // re-arm for the next launch
-        is_last = last ? 1 : 0;
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();  // acquire: see every block's partials
+__device__ __forceinline__ void sum_partials(const Launch& L, int sys, const int (&slots)[NV], double (&out)[NV]) {
     const int2 range = L.sys_blocks[sys];
-    double acc[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int k = 0; k < NV; ++k) out[k] = 0.0;
     for (int b = range.x + static_cast<int>(threadIdx.x); b < range.y; b += kBlock) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(&L.partials[static_cast<size_t>(b) * 2 + k], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < NV; ++k) out[k] += L.partials[static_cast<size_t>(slots[k]) * L.grid + b];
     }
-    block_sum<NV>(acc);
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = acc[k];
-    return true;
+    block_sum<NV>(out);
+}
+
+__device__ __forceinline__ void put_partial(const Launch& L, int slot, int lb, double v) {
+    L.partials[static_cast<size_t>(slot) * L.grid + lb] = v;
 }
 
 __device__ __forceinline__ bool bad(double v) { return !isfinite(v); }
@@ -158,26 +146,29 @@ __device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int
     return acc;
 }
 
+#define XP_WAVE_LANE                                                  \
+    const int wave = static_cast<int>(threadIdx.x) >> 6;              \
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+
 // ---------------------------------------------------------------------------
-// PCG init: r = b - A x, z = minv r, p = z; rz, rr.  Mirrors init_pcg_kernel
-// (archived/kernels/cg_kernels.cpp:73-103) fused with the x0 SpMV.
+// PCG init: r = b - A x, z = minv r, p = z; partials of rz, rr.  Mirrors
+// init_pcg_kernel (archived/kernels/cg_kernels.cpp:73-103) fused with the
+// x0 SpMV.
 template <bool IDX16>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(Launch L, SellView A, SysState* st,
-                                                     const float* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_pcg_init(Launch L, SellView A, const float* __restrict__ x,
                                                      const float* __restrict__ b,
                                                      const float* __restrict__ minv,
                                                      float* __restrict__ r, float* __restrict__ z,
                                                      float* __restrict__ p) {
     const int lb = logical_block();
     const WorkItem w = L.work[lb];
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int lane = static_cast<int>(threadIdx.x) & 63;
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
     double v[2] = {0.0, 0.0};
-    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
-        const int slice = ch * kWaves + wave;
-        const int row = slice * kC + lane;
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
         float xd;
-        const float ax = sell_row_f32<IDX16>(A, slice, lane, x, xd);
+        const float ax = sell_row_f32<IDX16>(A, sl, lane, x, xd);
         const float rv = b[row] - ax;
         const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
         r[row] = rv;
@@ -186,21 +177,31 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Launch L, SellView A, SysSt
         v[0] += static_cast<double>(rv) * static_cast<double>(zv);
         v[1] += static_cast<double>(rv) * static_cast<double>(rv);
     }
-    if (w.sys < 0) return;
-    if (!reduce_across_blocks<2>(L, lb, w.sys, 0, v)) return;
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        put_partial(L, kSlotRz, lb, v[0]);
+        put_partial(L, kSlotRr, lb, v[1]);
+    }
+}
+
+// One workgroup per system: the state after init, with the loop-top tests of
+// the on-chip PCG loop (kernels/cuper_pcg_control_kernel.cpp:1123-1130).
+// S holds each system's configuration (tau, max_iters, ...) on entry.
+__global__ __launch_bounds__(kBlock) void k_pcg_init_state(Launch L, SysState* S) {
+    const int sys = static_cast<int>(blockIdx.x);
+    double t[2];
+    sum_partials<2>(L, sys, {kSlotRz, kSlotRr}, t);
     if (threadIdx.x != 0) return;
-    SysState& s = st[w.sys];
-    s.rz = v[0];
-    s.rr = v[1];
-    s.rr0 = v[1];
-    s.best_rr = v[1];
+    SysState& s = S[sys];
+    s.rz = t[0];
+    s.rr = t[1];
+    s.rr0 = t[1];
+    s.best_rr = t[1];
     s.stall = 0;
     s.iters = 0;
     s.pap = 0.0;
     s.alpha = 0.0;
     s.beta = 0.0;
-    // Loop-top tests of the on-chip PCG loop
-    // (kernels/cuper_pcg_control_kernel.cpp:1123-1130).
     if (s.rr <= s.tau) {
         s.status = kStatusConverged;
         s.active = 0;
@@ -216,50 +217,30 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Launch L, SellView A, SysSt
     }
 }
 
-// ap = A p, pAp, alpha.
+// ap = A p and the partials of pAp.
 template <bool IDX16>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(Launch L, SellView A, SysState* st,
-                                                     const float* __restrict__ p,
-                                                     float* __restrict__ ap) {
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(Launch L, SellView A, const SysState* S,
+                                                     const float* __restrict__ p, float* __restrict__ ap) {
     const int lb = logical_block();
     const WorkItem w = L.work[lb];
-    if (w.sys < 0 || !st[w.sys].active) return;
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int lane = static_cast<int>(threadIdx.x) & 63;
+    if (w.sys < 0 || !S[w.sys].active) return;
+    XP_WAVE_LANE
     double v[1] = {0.0};
-    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
-        const int slice = ch * kWaves + wave;
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         float pd;
-        const float y = sell_row_f32<IDX16>(A, slice, lane, p, pd);
-        ap[slice * kC + lane] = y;
+        const float y = sell_row_f32<IDX16>(A, sl, lane, p, pd);
+        ap[sl * kC + lane] = y;
         v[0] += static_cast<double>(pd) * static_cast<double>(y);
     }
-    if (!reduce_across_blocks<1>(L, lb, w.sys, 1, v)) return;
-    if (threadIdx.x != 0) return;
-    SysState& s = st[w.sys];
-    const double pap = v[0];
-    s.pap = pap;
-    // Breakdown tests of run_local_multi_kernel_solver
-    // (archived/host/multi_kernel_solver.hpp:170-180).
-    if (bad(pap) || fabs(pap) <= kBreakdownEps) {
-        s.status = kStatusBreakdown;
-        s.active = 0;
-        return;
-    }
-    const double alpha = s.rz / pap;
-    if (bad(alpha)) {
-        s.status = kStatusBreakdown;
-        s.active = 0;
-        return;
-    }
-    s.alpha = alpha;
+    block_sum<1>(v);
+    if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
 }
 
-// x += alpha p; r -= alpha ap; z = minv r; rz_new, rr_new; beta; trace.
-// update_xrz_kernel (archived/kernels/cg_kernels.cpp:118-147) with the
-// scalar tail of the golden loop (src/CgSolverGolden.hpp:100-118).
-__global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, SysState* st, TraceRow* trace,
-                                                           int trace_stride, float* __restrict__ x,
+// alpha = rz / pAp; x += alpha p; r -= alpha ap; z = minv r; partials of
+// rz_new, rr_new.  update_xrz_kernel (archived/kernels/cg_kernels.cpp:118-147).
+// A breakdown (bad pAp or alpha) leaves the vectors untouched; the next
+// kernel records it.
+__global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, const SysState* S, float* __restrict__ x,
                                                            const float* __restrict__ p,
                                                            float* __restrict__ r,
                                                            const float* __restrict__ ap,
@@ -267,11 +248,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, SysState* s
                                                            float* __restrict__ z) {
     const int lb = logical_block();
     const WorkItem w = L.work[lb];
-    if (w.sys < 0 || !st[w.sys].active) return;
-    const double alpha = st[w.sys].alpha;
+    if (w.sys < 0 || !S[w.sys].active) return;
+    double t[1];
+    sum_partials<1>(L, w.sys, {kSlotPap}, t);
+    const double pap = t[0];
+    const double alpha = S[w.sys].rz / pap;
+    if (bad(pap) || fabs(pap) <= kBreakdownEps || bad(alpha)) return;
     double v[2] = {0.0, 0.0};
-    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
-        const int row = ch * kChunkRows + static_cast<int>(threadIdx.x);
+    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock) {
         const double pv = p[row];
         const float xv = static_cast<float>(static_cast<double>(x[row]) + alpha * pv);
         const float rv = static_cast<float>(static_cast<double>(r[row]) - alpha * static_cast<double>(ap[row]));
@@ -282,123 +266,141 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, SysState* s
         v[0] += static_cast<double>(rv) * static_cast<double>(zv);
         v[1] += static_cast<double>(rv) * static_cast<double>(rv);
     }
-    if (!reduce_across_blocks<2>(L, lb, w.sys, 2, v)) return;
-    if (threadIdx.x != 0) return;
-    SysState& s = st[w.sys];
-    const double rz_old = s.rz;
-    const double rz = v[0];
-    const double rr = v[1];
-    const double beta = rz / rz_old;
-    if (trace != nullptr && s.iters < s.trace_cap) {
-        TraceRow& t = trace[static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
-                            static_cast<size_t>(s.iters)];
-        t.alpha = s.alpha;
-        t.beta = beta;
-        t.rz = rz;
-        t.rr = rr;
-    }
-    s.iters += 1;
-    s.rz = rz;
-    s.rr = rr;
-    s.beta = beta;
-    // Post-update tests in the order of the on-chip loop
-    // (kernels/cuper_pcg_control_kernel.cpp:1193-1214), then the loop bound
-    // and the next loop-top rz test (:1127-1130).
-    if (bad(rz) || bad(rr)) {
-        s.status = kStatusBreakdown;
-        s.active = 0;
-    } else if (rr <= s.tau) {
-        s.status = kStatusConverged;
-        s.active = 0;
-    } else if (fabs(rz_old) <= kBreakdownEps || bad(beta)) {
-        s.status = kStatusBreakdown;
-        s.active = 0;
-    } else if (s.iters >= s.max_iters) {
-        s.status = kStatusMaxIter;
-        s.active = 0;
-    } else if (fabs(rz) <= kBreakdownEps) {
-        s.status = kStatusBreakdown;
-        s.active = 0;
-    } else if (s.stall_limit > 0) {
-        // Stagnation watch (used by preconditioners that may fail): stop
-        // when r^T r has not dropped below 0.5 x its best for stall_limit
-        // iterations.
-        if (rr < 0.5 * s.best_rr) {
-            s.best_rr = rr;
-            s.stall = 0;
-        } else if (++s.stall >= s.stall_limit) {
-            s.status = kStatusStagnation;
-            s.active = 0;
-        }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        put_partial(L, kSlotRz, lb, v[0]);
+        put_partial(L, kSlotRr, lb, v[1]);
     }
 }
 
-// p = z + beta p (update_p_kernel, archived/kernels/cg_kernels.cpp:149-158).
-__global__ __launch_bounds__(kBlock) void k_pcg_update_p(Launch L, const SysState* st,
-                                                         const float* __restrict__ z,
-                                                         float* __restrict__ p) {
+// The scalar tail of the iteration and p = z + beta p
+// (update_p_kernel, archived/kernels/cg_kernels.cpp:149-158; golden loop
+// src/CgSolverGolden.hpp:100-118).  Reads state slot Sc, writes slot Sn.
+__global__ __launch_bounds__(kBlock) void k_pcg_update_p(Launch L, const SysState* Sc, SysState* Sn,
+                                                         TraceRow* trace, int trace_stride,
+                                                         const float* __restrict__ z, float* __restrict__ p) {
     const int lb = logical_block();
     const WorkItem w = L.work[lb];
-    if (w.sys < 0 || !st[w.sys].active) return;
-    const double beta = st[w.sys].beta;
-    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
-        const int row = ch * kChunkRows + static_cast<int>(threadIdx.x);
-        p[row] = static_cast<float>(static_cast<double>(z[row]) + beta * static_cast<double>(p[row]));
+    if (w.sys < 0) return;
+    const bool leader = lb == L.sys_blocks[w.sys].x && threadIdx.x == 0;
+    SysState s = Sc[w.sys];
+    if (!s.active) {
+        if (leader) Sn[w.sys] = s;  // carry the stopped state forward
+        return;
     }
+    double t[3];
+    sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);
+    const double pap = t[0];
+    const double alpha = s.rz / pap;
+    s.pap = pap;
+    // Breakdown tests of run_local_multi_kernel_solver
+    // (archived/host/multi_kernel_solver.hpp:170-180).
+    if (bad(pap) || fabs(pap) <= kBreakdownEps || bad(alpha)) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else {
+        const double rz_old = s.rz;
+        const double rz = t[1];
+        const double rr = t[2];
+        const double beta = rz / rz_old;
+        if (leader && trace != nullptr && s.iters < s.trace_cap) {
+            TraceRow& row = trace[static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
+                                  static_cast<size_t>(s.iters)];
+            row.alpha = alpha;
+            row.beta = beta;
+            row.rz = rz;
+            row.rr = rr;
+        }
+        s.iters += 1;
+        s.rz = rz;
+        s.rr = rr;
+        s.alpha = alpha;
+        s.beta = beta;
+        // Post-update tests in the order of the on-chip loop
+        // (kernels/cuper_pcg_control_kernel.cpp:1193-1214), then the loop
+        // bound and the next loop-top rz test (:1127-1130).
+        if (bad(rz) || bad(rr)) {
+            s.status = kStatusBreakdown;
+            s.active = 0;
+        } else if (rr <= s.tau) {
+            s.status = kStatusConverged;
+            s.active = 0;
+        } else if (fabs(rz_old) <= kBreakdownEps || bad(beta)) {
+            s.status = kStatusBreakdown;
+            s.active = 0;
+        } else if (s.iters >= s.max_iters) {
+            s.status = kStatusMaxIter;
+            s.active = 0;
+        } else if (fabs(rz) <= kBreakdownEps) {
+            s.status = kStatusBreakdown;
+            s.active = 0;
+        } else if (s.stall_limit > 0) {
+            // Stagnation watch (for preconditioners that may fail): stop when
+            // r^T r has not dropped below half its best for stall_limit iterations.
+            if (rr < 0.5 * s.best_rr) {
+                s.best_rr = rr;
+                s.stall = 0;
+            } else if (++s.stall >= s.stall_limit) {
+                s.status = kStatusStagnation;
+                s.active = 0;
+            }
+        }
+    }
+    if (s.active) {
+        const double beta = s.beta;
+        for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock)
+            p[row] = static_cast<float>(static_cast<double>(z[row]) + beta * static_cast<double>(p[row]));
+    }
+    if (leader) Sn[w.sys] = s;
 }
 
 template <bool IDX16>
-__global__ __launch_bounds__(kBlock) void k_spmv_f32(Launch L, SellView A,
-                                                     const float* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_spmv_f32(Launch L, SellView A, const float* __restrict__ x,
                                                      float* __restrict__ y) {
-    const int lb = logical_block();
-    const WorkItem w = L.work[lb];
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
-        const int slice = ch * kWaves + wave;
+    const WorkItem w = L.work[logical_block()];
+    XP_WAVE_LANE
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         float xd;
-        y[slice * kC + lane] = sell_row_f32<IDX16>(A, slice, lane, x, xd);
+        y[sl * kC + lane] = sell_row_f32<IDX16>(A, sl, lane, x, xd);
     }
 }
 
 template <bool IDX16, int VMODE>
-__global__ __launch_bounds__(kBlock) void k_spmv_f64(Launch L, SellView A,
-                                                     const double* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_spmv_f64(Launch L, SellView A, const double* __restrict__ x,
                                                      double* __restrict__ y) {
-    const int lb = logical_block();
-    const WorkItem w = L.work[lb];
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
-        const int slice = ch * kWaves + wave;
-        y[slice * kC + lane] = sell_row_f64<IDX16, VMODE>(A, slice, lane, x);
-    }
+    const WorkItem w = L.work[logical_block()];
+    XP_WAVE_LANE
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves)
+        y[sl * kC + lane] = sell_row_f64<IDX16, VMODE>(A, sl, lane, x);
 }
 
+// r = b - A x in FP64 and the partials of r^T r.
 template <bool IDX16, int VMODE>
-__global__ __launch_bounds__(kBlock) void k_residual_f64(Launch L, SellView A,
-                                                         const double* __restrict__ x,
-                                                         const double* __restrict__ b,
-                                                         double* __restrict__ r,
-                                                         float* __restrict__ r32,
-                                                         double* __restrict__ out_rr) {
+__global__ __launch_bounds__(kBlock) void k_residual_f64(Launch L, SellView A, const double* __restrict__ x,
+                                                         const double* __restrict__ b, double* __restrict__ r,
+                                                         float* __restrict__ r32) {
     const int lb = logical_block();
     const WorkItem w = L.work[lb];
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int lane = static_cast<int>(threadIdx.x) & 63;
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
     double v[1] = {0.0};
-    for (int ch = w.chunk_begin; ch < w.chunk_end; ++ch) {
-        const int slice = ch * kWaves + wave;
-        const int row = slice * kC + lane;
-        const double rv = b[row] - sell_row_f64<IDX16, VMODE>(A, slice, lane, x);
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
+        const double rv = b[row] - sell_row_f64<IDX16, VMODE>(A, sl, lane, x);
         r[row] = rv;
         if (r32 != nullptr) r32[row] = static_cast<float>(rv);
         v[0] += rv * rv;
     }
-    if (w.sys < 0) return;
-    if (!reduce_across_blocks<1>(L, lb, w.sys, 3, v)) return;
-    if (threadIdx.x == 0) out_rr[w.sys] = v[0];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) put_partial(L, kSlotAux, lb, v[0]);
+}
+
+// One workgroup per system: out[sys] = sum of the kSlotAux partials.
+__global__ __launch_bounds__(kBlock) void k_finish_aux(Launch L, double* out) {
+    const int sys = static_cast<int>(blockIdx.x);
+    double t[1];
+    sum_partials<1>(L, sys, {kSlotAux}, t);
+    if (threadIdx.x == 0) out[sys] = t[0];
 }
 
 __global__ void k_axpy_f64_f32(int n, const float* __restrict__ d, double* __restrict__ x) {
@@ -408,45 +410,9 @@ __global__ void k_axpy_f64_f32(int n, const float* __restrict__ d, double* __res
 
 // ---------------------------------------------------------------------------
 // Natural-order FP64 kernels behind the drop-in C-ABI.  Same formulas as
-// archived/kernels/cg_kernels.cpp:73-158; sums in a fixed parallel order.
-
-struct Flat {
-    double* partials;
-    unsigned* counter;
-};
-
-// Sums NV values over the whole grid; returns true (in every thread of the
-// last block) with the totals in v.
-template <int NV>
-__device__ bool flat_reduce(Flat F, double (&v)[NV]) {
-    __shared__ int is_last;
-    block_sum<NV>(v);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) F.partials[static_cast<size_t>(blockIdx.x) * 2 + k] = v[k];
-        __threadfence();
-        const unsigned ticket = atomicAdd(F.counter, 1u);
-        const bool last = ticket == gridDim.x - 1;
-        if (last) atomicExch(F.counter, 0u);
-        is_last = last ? 1 : 0;
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int b = static_cast<int>(threadIdx.x); b < static_cast<int>(gridDim.x); b += kBlock) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(&F.partials[static_cast<size_t>(b) * 2 + k], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<NV>(acc);
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = acc[k];
-    return true;
-}
+// archived/kernels/cg_kernels.cpp:73-158.  Each writes one partial per
+// workgroup (partials[k * grid + block]); k_flat_finish then sums them in
+// block order into out[0..NV-1].
 
 __device__ __forceinline__ void flat_range(int n, int& i0, int& i1) {
     const int per = (n + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x);
@@ -454,11 +420,36 @@ __device__ __forceinline__ void flat_range(int n, int& i0, int& i1) {
     i1 = min(n, i0 + per);
 }
 
+template <int NV>
+__device__ __forceinline__ void flat_put(double* partials, double (&v)[NV]) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[static_cast<size_t>(k) * gridDim.x + blockIdx.x] = v[k];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_flat_finish(const double* partials, int nblocks, double* out) {
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = static_cast<int>(threadIdx.x); b < nblocks; b += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += partials[static_cast<size_t>(k) * nblocks + b];
+    }
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] = acc[k];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_init_pcg_f64(int n, const double* __restrict__ b,
                                                          const double* __restrict__ ax,
                                                          const double* __restrict__ m_inv,
                                                          double* __restrict__ r, double* __restrict__ z,
-                                                         double* __restrict__ p, double* out, Flat F) {
+                                                         double* __restrict__ p, double* partials) {
     int i0, i1;
     flat_range(n, i0, i1);
     double v[2] = {0.0, 0.0};
@@ -471,19 +462,16 @@ __global__ __launch_bounds__(kBlock) void k_init_pcg_f64(int n, const double* __
         v[0] += rv * zv;
         v[1] += rv * rv;
     }
-    if (flat_reduce<2>(F, v) && threadIdx.x == 0) {
-        out[0] = v[0];
-        out[1] = v[1];
-    }
+    flat_put<2>(partials, v);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dot_f64(int n, const double* __restrict__ a,
-                                                    const double* __restrict__ b, double* out, Flat F) {
+                                                    const double* __restrict__ b, double* partials) {
     int i0, i1;
     flat_range(n, i0, i1);
     double v[1] = {0.0};
     for (int i = i0 + static_cast<int>(threadIdx.x); i < i1; i += kBlock) v[0] += a[i] * b[i];
-    if (flat_reduce<1>(F, v) && threadIdx.x == 0) out[0] = v[0];
+    flat_put<1>(partials, v);
 }
 
 __global__ __launch_bounds__(kBlock) void k_update_xrz_f64(int n, double* __restrict__ x,
@@ -492,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xrz_f64(int n, double* __rest
                                                            const double* __restrict__ ap,
                                                            const double* __restrict__ m_inv,
                                                            double* __restrict__ z, double alpha,
-                                                           double* out, Flat F) {
+                                                           double* partials) {
     int i0, i1;
     flat_range(n, i0, i1);
     double v[2] = {0.0, 0.0};
@@ -505,10 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xrz_f64(int n, double* __rest
         v[0] += rv * zv;
         v[1] += rv * rv;
     }
-    if (flat_reduce<2>(F, v) && threadIdx.x == 0) {
-        out[0] = v[0];
-        out[1] = v[1];
-    }
+    flat_put<2>(partials, v);
 }
 
 __global__ void k_update_p_f64(int n, const double* __restrict__ z, double* __restrict__ p, double beta) {
@@ -552,27 +537,28 @@ int blocks_for(size_t n, int bs) { return static_cast<int>((n + static_cast<size
 void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
                      const float* b, const float* minv, float* r, float* z, float* p) {
     if (A.idx16)
-        hipLaunchKernelGGL(k_pcg_init<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, x, b, minv, r, z, p);
+        hipLaunchKernelGGL(k_pcg_init<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, r, z, p);
     else
-        hipLaunchKernelGGL(k_pcg_init<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, x, b, minv, r, z, p);
+        hipLaunchKernelGGL(k_pcg_init<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, r, z, p);
+    hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
 }
 
-void launch_pcg_spmv(const Launch& L, const SellView& A, SysState* st, const float* p, float* ap) {
+void launch_pcg_spmv(const Launch& L, const SellView& A, const SysState* st, const float* p, float* ap) {
     if (A.idx16)
         hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
     else
         hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
 }
 
-void launch_pcg_update_xrz(const Launch& L, SysState* st, TraceRow* trace, int trace_stride,
-                           float* x, const float* p, float* r, const float* ap,
-                           const float* minv, float* z) {
-    hipLaunchKernelGGL(k_pcg_update_xrz, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, trace,
-                       trace_stride, x, p, r, ap, minv, z);
+void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
+                           const float* ap, const float* minv, float* z) {
+    hipLaunchKernelGGL(k_pcg_update_xrz, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, x, p, r, ap, minv, z);
 }
 
-void launch_pcg_update_p(const Launch& L, const SysState* st, const float* z, float* p) {
-    hipLaunchKernelGGL(k_pcg_update_p, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, z, p);
+void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
+                         int trace_stride, const float* z, float* p) {
+    hipLaunchKernelGGL(k_pcg_update_p, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st_cur, st_next, trace,
+                       trace_stride, z, p);
 }
 
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y) {
@@ -600,15 +586,16 @@ void launch_residual_f64(const Launch& L, const SellView& A, bool use_lo, const 
                          const double* b, double* r, float* r32, double* out_rr) {
     if (A.idx16) {
         if (use_lo)
-            hipLaunchKernelGGL((k_residual_f64<true, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+            hipLaunchKernelGGL((k_residual_f64<true, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32);
         else
-            hipLaunchKernelGGL((k_residual_f64<true, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+            hipLaunchKernelGGL((k_residual_f64<true, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32);
     } else {
         if (use_lo)
-            hipLaunchKernelGGL((k_residual_f64<false, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+            hipLaunchKernelGGL((k_residual_f64<false, 1>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32);
         else
-            hipLaunchKernelGGL((k_residual_f64<false, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32, out_rr);
+            hipLaunchKernelGGL((k_residual_f64<false, 0>), dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, r, r32);
     }
+    hipLaunchKernelGGL(k_finish_aux, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, out_rr);
 }
 
 void launch_axpy_f64_f32(int n_pad, const float* d, double* x, hipStream_t s) {
@@ -617,22 +604,22 @@ void launch_axpy_f64_f32(int n_pad, const float* d, double* x, hipStream_t s) {
 }
 
 void launch_init_pcg_f64(int n, const double* b, const double* ax, const double* m_inv,
-                         double* r, double* z, double* p, double* out, double* partials,
-                         unsigned* counter, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_init_pcg_f64, dim3(grid), dim3(kBlock), 0, s, n, b, ax, m_inv, r, z, p, out,
-                       Flat{partials, counter});
+                         double* r, double* z, double* p, double* out, double* partials, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_pcg_f64, dim3(grid), dim3(kBlock), 0, s, n, b, ax, m_inv, r, z, p, partials);
+    hipLaunchKernelGGL(k_flat_finish<2>, dim3(1), dim3(kBlock), 0, s, partials, grid, out);
 }
 
-void launch_dot_f64(int n, const double* a, const double* b, double* out, double* partials,
-                    unsigned* counter, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot_f64, dim3(grid), dim3(kBlock), 0, s, n, a, b, out, Flat{partials, counter});
+void launch_dot_f64(int n, const double* a, const double* b, double* out, double* partials, int grid,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_f64, dim3(grid), dim3(kBlock), 0, s, n, a, b, partials);
+    hipLaunchKernelGGL(k_flat_finish<1>, dim3(1), dim3(kBlock), 0, s, partials, grid, out);
 }
 
 void launch_update_xrz_f64(int n, double* x, const double* p, double* r, const double* ap,
                            const double* m_inv, double* z, double alpha, double* out,
-                           double* partials, unsigned* counter, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_update_xrz_f64, dim3(grid), dim3(kBlock), 0, s, n, x, p, r, ap, m_inv, z, alpha,
-                       out, Flat{partials, counter});
+                           double* partials, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_update_xrz_f64, dim3(grid), dim3(kBlock), 0, s, n, x, p, r, ap, m_inv, z, alpha, partials);
+    hipLaunchKernelGGL(k_flat_finish<2>, dim3(1), dim3(kBlock), 0, s, partials, grid, out);
 }
 
 void launch_update_p_f64(int n, const double* z, double* p, double beta, hipStream_t s) {

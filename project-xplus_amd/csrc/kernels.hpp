@@ -11,30 +11,29 @@
 namespace xp {
 
 // Shared launch context of the row-streaming kernels: the work map (one
-// WorkItem per logical workgroup), the per-system workgroup ranges used by
-// the reductions, and the reduction scratch.
+// WorkItem per logical workgroup), the per-system workgroup ranges and the
+// per-workgroup reduction partials (kNumSlots x grid doubles).
 struct Launch {
     int grid = 0;                       // workgroups, a multiple of kNumXcd
     const WorkItem* work = nullptr;     // [grid]
     const int2* sys_blocks = nullptr;   // [nsys] logical block range per system
-    unsigned* counters = nullptr;       // [nsys * kCounterSlots], zero at rest
-    double* partials = nullptr;         // [grid * 2]
+    double* partials = nullptr;         // [kNumSlots * grid]
     hipStream_t stream = nullptr;
 };
-constexpr int kCounterSlots = 4;        // one arrival counter per reduction kind
 
 // ---- FP32 solve-order PCG step kernels (state machine in SysState) ----
-// init: r = b - A x; z = minv r; p = z; rz, rr; state set running/converged/breakdown.
+// init: r = b - A x; z = minv r; p = z; then the state after init in st[sys]
+// (which must hold the configuration: tau, max_iters, trace_cap, stall_limit).
 void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
                      const float* b, const float* minv, float* r, float* z, float* p);
-// ap = A p; pAp; alpha = rz / pAp.
-void launch_pcg_spmv(const Launch& L, const SellView& A, SysState* st, const float* p, float* ap);
-// x += alpha p; r -= alpha ap; z = minv r; rz, rr; beta; trace; convergence.
-void launch_pcg_update_xrz(const Launch& L, SysState* st, TraceRow* trace, int trace_stride,
-                           float* x, const float* p, float* r, const float* ap,
-                           const float* minv, float* z);
-// p = z + beta p.
-void launch_pcg_update_p(const Launch& L, const SysState* st, const float* z, float* p);
+// ap = A p and the pAp partials.
+void launch_pcg_spmv(const Launch& L, const SellView& A, const SysState* st, const float* p, float* ap);
+// alpha; x += alpha p; r -= alpha ap; z = minv r; rz, rr partials.
+void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
+                           const float* ap, const float* minv, float* z);
+// scalar tail (beta, tests, trace) and p = z + beta p; reads st_cur, writes st_next.
+void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
+                         int trace_stride, const float* z, float* p);
 
 // ---- plain SpMV (no reduction) ----
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y);
@@ -48,16 +47,16 @@ void launch_residual_f64(const Launch& L, const SellView& A, bool use_lo, const 
 void launch_axpy_f64_f32(int n_pad, const float* d, double* x, hipStream_t s);
 
 // ---- natural-order FP64 element kernels behind the drop-in C-ABI ----
-// Each one reduces deterministically (fixed per-block order, last block
-// sums the block partials in block order) into out[0..1].
+// Each writes per-workgroup partials (partials: >= 2 * grid doubles) and a
+// one-workgroup kernel sums them in workgroup order into out[0..1].
 void launch_init_pcg_f64(int n, const double* b, const double* ax, const double* m_inv,
-                         double* r, double* z, double* p, double* out, double* partials,
-                         unsigned* counter, int grid, hipStream_t s);
-void launch_dot_f64(int n, const double* a, const double* b, double* out, double* partials,
-                    unsigned* counter, int grid, hipStream_t s);
+                         double* r, double* z, double* p, double* out, double* partials, int grid,
+                         hipStream_t s);
+void launch_dot_f64(int n, const double* a, const double* b, double* out, double* partials, int grid,
+                    hipStream_t s);
 void launch_update_xrz_f64(int n, double* x, const double* p, double* r, const double* ap,
                            const double* m_inv, double* z, double alpha, double* out,
-                           double* partials, unsigned* counter, int grid, hipStream_t s);
+                           double* partials, int grid, hipStream_t s);
 void launch_update_p_f64(int n, const double* z, double* p, double beta, hipStream_t s);
 
 // ---- permutation between natural and solve order ----

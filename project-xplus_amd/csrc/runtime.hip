@@ -150,7 +150,6 @@ struct xp_matrix {
     int grid = 0;
     DevBuf<xp::WorkItem> d_work;
     DevBuf<int2> d_sys_blocks;
-    DevBuf<unsigned> d_counters;
     DevBuf<double> d_partials;
 
     xp::SellView view() const {
@@ -173,7 +172,6 @@ struct xp_matrix {
         L.grid = grid;
         L.work = d_work.p;
         L.sys_blocks = d_sys_blocks.p;
-        L.counters = d_counters.p;
         L.partials = d_partials.p;
         L.stream = s;
         return L;
@@ -188,29 +186,29 @@ struct xp_matrix {
 
 namespace {
 
-// Splits each system's chunks over workgroups: about `target` workgroups in
+// Splits each system's slices over workgroups: about `target` workgroups in
 // total, at least one per system, a system's workgroups contiguous in
 // logical order, total rounded up to a multiple of the XCD count.
 void build_work_map(xp_matrix& m, int target) {
     const hipStream_t s = device().stream;
-    const int total_chunks = m.n_pad / xp::kChunkRows;
+    const int total = m.nslices;
     std::vector<int> g(static_cast<size_t>(m.nsys));
     for (int k = 0; k < m.nsys; ++k) {
-        const int ch = (m.sys_row_begin[static_cast<size_t>(k) + 1] - m.sys_row_begin[static_cast<size_t>(k)]) / xp::kChunkRows;
-        const double share = static_cast<double>(target) * ch / std::max(1, total_chunks);
-        g[static_cast<size_t>(k)] = std::max(1, std::min(ch, static_cast<int>(std::lround(share))));
+        const int sl = (m.sys_row_begin[static_cast<size_t>(k) + 1] - m.sys_row_begin[static_cast<size_t>(k)]) / xp::kC;
+        const double share = static_cast<double>(target) * sl / std::max(1, total);
+        g[static_cast<size_t>(k)] = std::max(1, std::min(sl, static_cast<int>(std::lround(share))));
     }
     std::vector<xp::WorkItem> work;
     std::vector<int2> sb(static_cast<size_t>(m.nsys));
     for (int k = 0; k < m.nsys; ++k) {
-        const int c0 = m.sys_row_begin[static_cast<size_t>(k)] / xp::kChunkRows;
-        const int c1 = m.sys_row_begin[static_cast<size_t>(k) + 1] / xp::kChunkRows;
+        const int s0 = m.sys_row_begin[static_cast<size_t>(k)] / xp::kC;
+        const int s1 = m.sys_row_begin[static_cast<size_t>(k) + 1] / xp::kC;
         const int gk = g[static_cast<size_t>(k)];
-        const int ch = c1 - c0;
+        const int sl = s1 - s0;
         sb[static_cast<size_t>(k)] = make_int2(static_cast<int>(work.size()), static_cast<int>(work.size()) + gk);
         for (int q = 0; q < gk; ++q) {
-            const int a = c0 + static_cast<int>(static_cast<long long>(ch) * q / gk);
-            const int b = c0 + static_cast<int>(static_cast<long long>(ch) * (q + 1) / gk);
+            const int a = s0 + static_cast<int>(static_cast<long long>(sl) * q / gk);
+            const int b = s0 + static_cast<int>(static_cast<long long>(sl) * (q + 1) / gk);
             work.push_back(xp::WorkItem{k, a, b, 0});
         }
     }
@@ -219,15 +217,15 @@ void build_work_map(xp_matrix& m, int target) {
     m.grid = grid;
     m.d_work = to_device(work, s);
     m.d_sys_blocks = to_device(sb, s);
-    m.d_counters.alloc(static_cast<size_t>(m.nsys) * xp::kCounterSlots);
-    HIPX(hipMemsetAsync(m.d_counters.p, 0, m.d_counters.n * sizeof(unsigned), s));
-    m.d_partials.alloc(static_cast<size_t>(grid) * 2);
+    m.d_partials.alloc(static_cast<size_t>(grid) * xp::kNumSlots);
+    HIPX(hipMemsetAsync(m.d_partials.p, 0, m.d_partials.n * sizeof(double), s));
     HIPX(hipStreamSynchronize(s));
 }
 
+// Two 1024-thread workgroups per CU: full occupancy in one wave of
+// workgroups, and few enough partials for every workgroup to re-sum them.
 int default_grid(const xp_matrix& m) {
-    const int chunks = m.n_pad / xp::kChunkRows;
-    return std::min(chunks, std::max(8, device().cus * 8));
+    return std::min(m.nslices, std::max(8, device().cus * 2));
 }
 
 std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, const xp_matrix_opts* o) {
@@ -321,19 +319,25 @@ struct xp_pcg {
         if (stream != nullptr) (void)hipStreamDestroy(stream);
     }
 
-    void enqueue_iteration() {
+    // Iteration k reads state slot k % 2 and writes slot (k + 1) % 2; the
+    // graph holds an even number of iterations, so slot 0 is current
+    // between launches.
+    void enqueue_iteration(int k) {
         const xp::Launch L = m->launch(stream);
         const xp::SellView A = m->view();
-        xp::launch_pcg_spmv(L, A, st.p, p.p, ap.p);
-        xp::launch_pcg_update_xrz(L, st.p, trace.p, trace_stride, x.p, p.p, r.p, ap.p, minv.p, z.p);
-        xp::launch_pcg_update_p(L, st.p, z.p, p.p);
+        const size_t ns = static_cast<size_t>(m->nsys);
+        xp::SysState* cur = st.p + (k % 2) * ns;
+        xp::SysState* nxt = st.p + ((k + 1) % 2) * ns;
+        xp::launch_pcg_spmv(L, A, cur, p.p, ap.p);
+        xp::launch_pcg_update_xrz(L, cur, x.p, p.p, r.p, ap.p, minv.p, z.p);
+        xp::launch_pcg_update_p(L, cur, nxt, trace.p, trace_stride, z.p, p.p);
     }
 
     void build_graph(int iters) {
         if (exec != nullptr) { HIPX(hipGraphExecDestroy(exec)); exec = nullptr; }
         if (graph != nullptr) { HIPX(hipGraphDestroy(graph)); graph = nullptr; }
         HIPX(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < iters; ++k) enqueue_iteration();
+        for (int k = 0; k < iters; ++k) enqueue_iteration(k);
         HIPX(hipStreamEndCapture(stream, &graph));
         HIPX(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         graph_iters = iters;
@@ -349,9 +353,10 @@ struct xp_pcg {
     }
 
     bool any_active() {
-        HIPX(hipMemcpyAsync(st_host, st.p, st.n * sizeof(xp::SysState), hipMemcpyDeviceToHost, stream));
+        const size_t ns = static_cast<size_t>(m->nsys);
+        HIPX(hipMemcpyAsync(st_host, st.p, ns * sizeof(xp::SysState), hipMemcpyDeviceToHost, stream));
         HIPX(hipStreamSynchronize(stream));
-        for (size_t k = 0; k < st.n; ++k)
+        for (size_t k = 0; k < ns; ++k)
             if (st_host[k].active) return true;
         return false;
     }
@@ -379,7 +384,7 @@ struct xp_pcg {
                     HIPX(hipGraphLaunch(exec, stream));
                     issued += graph_iters;
                 } else {
-                    for (int k = 0; k < graph_iters; ++k) enqueue_iteration();
+                    for (int k = 0; k < graph_iters; ++k) enqueue_iteration(k);
                     issued += graph_iters;
                 }
                 ++last_launches;
@@ -413,6 +418,7 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
     d.trace_cap = 0;
     if (o != nullptr) d = *o;
     if (d.graph_iters <= 0) d.graph_iters = m->n_pad <= 65536 ? 64 : 32;
+    d.graph_iters += d.graph_iters % 2;  // even: the state slots alternate
     s->opts = d;
     HIPX(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIPX(hipEventCreate(&s->ev0));
@@ -426,7 +432,7 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
         if (g >= 0) minv[i] = static_cast<float>(1.0 / m->diag[static_cast<size_t>(g)]);
     }
     s->minv = to_device(minv, s->stream);
-    s->st.alloc(static_cast<size_t>(m->nsys));
+    s->st.alloc(2 * static_cast<size_t>(m->nsys));
     s->st_init.assign(static_cast<size_t>(m->nsys), xp::SysState{});
     s->trace_stride = std::max(0, d.trace_cap);
     if (s->trace_stride > 0) s->trace.alloc(static_cast<size_t>(m->nsys) * static_cast<size_t>(s->trace_stride));
@@ -507,19 +513,15 @@ struct Legacy {
     // natural-order work buffers
     DevBuf<double> buf[8];
     DevBuf<double> partials;
-    DevBuf<unsigned> counter;
     DevBuf<double> out;
 
     void ensure_flat() {
-        const hipStream_t s = device().stream;
         if (partials.n == 0) {
             partials.alloc(2 * 1024);
-            counter.alloc(1);
-            HIPX(hipMemsetAsync(counter.p, 0, sizeof(unsigned), s));
             out.alloc(2);
         }
     }
-    int flat_grid(int n) const { return std::max(1, std::min(1024, (n + 255) / 256)); }
+    int flat_grid(int n) const { return std::max(1, std::min(1024, (n + 1023) / 1024)); }
 
     xp_matrix& matrix(const int* row_ptr, const int* col_idx, const double* values, int n_) {
         const int nnz_ = row_ptr[n_];
@@ -593,7 +595,7 @@ void init_pcg_kernel(const double* b, const double* ax, const double* m_inv, dou
         L.buf[1].upload(ax, static_cast<size_t>(n), s);
         L.buf[2].upload(m_inv, static_cast<size_t>(n), s);
         xp::launch_init_pcg_f64(n, L.buf[0].p, L.buf[1].p, L.buf[2].p, L.buf[3].p, L.buf[4].p, L.buf[5].p,
-                                L.out.p, L.partials.p, L.counter.p, L.flat_grid(n), s);
+                                L.out.p, L.partials.p, L.flat_grid(n), s);
         L.buf[3].download(r, static_cast<size_t>(n), s);
         L.buf[4].download(z, static_cast<size_t>(n), s);
         L.buf[5].download(p, static_cast<size_t>(n), s);
@@ -612,7 +614,7 @@ void dot_kernel(const double* a, const double* b, double* out, int n) {
         L.buf[1].ensure(nn);
         L.buf[0].upload(a, static_cast<size_t>(n), s);
         L.buf[1].upload(b, static_cast<size_t>(n), s);
-        xp::launch_dot_f64(n, L.buf[0].p, L.buf[1].p, L.out.p, L.partials.p, L.counter.p, L.flat_grid(n), s);
+        xp::launch_dot_f64(n, L.buf[0].p, L.buf[1].p, L.out.p, L.partials.p, L.flat_grid(n), s);
         L.out.download(out, 1, s);
         HIPX(hipStreamSynchronize(s));
     });
@@ -633,7 +635,7 @@ void update_xrz_kernel(double* x, const double* p, double* r, const double* ap, 
         L.buf[3].upload(ap, cnt, s);
         L.buf[4].upload(m_inv, cnt, s);
         xp::launch_update_xrz_f64(n, L.buf[0].p, L.buf[1].p, L.buf[2].p, L.buf[3].p, L.buf[4].p, L.buf[5].p,
-                                  alpha, L.out.p, L.partials.p, L.counter.p, L.flat_grid(n), s);
+                                  alpha, L.out.p, L.partials.p, L.flat_grid(n), s);
         L.buf[0].download(x, cnt, s);
         L.buf[2].download(r, cnt, s);
         L.buf[5].download(z, cnt, s);
@@ -769,6 +771,67 @@ int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* str
     return guarded([&] {
         if (m == nullptr) throw std::runtime_error("null matrix");
         xp::launch_scatter_f2d(m->n_pad, m->d_perm.p, solve, nat, pick_stream(stream));
+    });
+}
+
+int xp_device_alloc(unsigned long long bytes, void** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null output pointer");
+        device();
+        *out = nullptr;
+        if (bytes == 0) return;
+        HIPX(hipMalloc(out, static_cast<size_t>(bytes)));
+    });
+}
+
+int xp_device_free(void* ptr) {
+    return guarded([&] {
+        if (ptr != nullptr) HIPX(hipFree(ptr));
+    });
+}
+
+int xp_memcpy_htod(void* dst, const void* src, unsigned long long bytes) {
+    return guarded([&] {
+        const hipStream_t s = device().stream;
+        HIPX(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice, s));
+        HIPX(hipStreamSynchronize(s));
+    });
+}
+
+int xp_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes) {
+    return guarded([&] {
+        const hipStream_t s = device().stream;
+        HIPX(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+    });
+}
+
+int xp_device_synchronize(void) {
+    return guarded([&] { HIPX(hipStreamSynchronize(device().stream)); HIPX(hipDeviceSynchronize()); });
+}
+
+int xp_spmv_f32_time(xp_matrix* m, int warmup, int reps, double* avg_us) {
+    return guarded([&] {
+        if (m == nullptr || avg_us == nullptr || reps <= 0) throw std::runtime_error("bad argument");
+        const hipStream_t s = device().stream;
+        const size_t np = static_cast<size_t>(m->n_pad);
+        std::vector<float> xh(np);
+        for (size_t i = 0; i < np; ++i) xh[i] = 1.0f + static_cast<float>(i % 7) * 0.125f;
+        DevBuf<float> x(np), y(np);
+        x.upload(xh.data(), np, s);
+        for (int k = 0; k < warmup; ++k) xp::launch_spmv_f32(m->launch(s), m->view(), x.p, y.p);
+        hipEvent_t e0, e1;
+        HIPX(hipEventCreate(&e0));
+        HIPX(hipEventCreate(&e1));
+        HIPX(hipEventRecord(e0, s));
+        for (int k = 0; k < reps; ++k) xp::launch_spmv_f32(m->launch(s), m->view(), x.p, y.p);
+        HIPX(hipEventRecord(e1, s));
+        HIPX(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIPX(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_us = 1000.0 * ms / reps;
     });
 }
 
@@ -985,7 +1048,8 @@ int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_ou
             S.run_inner(cap);
             S.rhs = nullptr;
             launches += S.last_launches;
-            HIPX(hipMemcpyAsync(S.st_host, S.st.p, S.st.n * sizeof(xp::SysState), hipMemcpyDeviceToHost, S.stream));
+            HIPX(hipMemcpyAsync(S.st_host, S.st.p, static_cast<size_t>(m.nsys) * sizeof(xp::SysState),
+                                hipMemcpyDeviceToHost, S.stream));
             HIPX(hipStreamSynchronize(S.stream));
             for (int k = 0; k < m.nsys; ++k) iters[static_cast<size_t>(k)] += S.st_host[k].iters;
             xp::launch_axpy_f64_f32(m.n_pad, S.x.p, S.x64.p, S.stream);

@@ -28,8 +28,8 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
         if (a.n < 0) throw std::runtime_error("build_sell: negative n");
         if (a.n > 0 && (a.row_ptr == nullptr || a.col_idx == nullptr || a.values == nullptr))
             throw std::runtime_error("build_sell: null CSR array");
-        // Every system gets at least one chunk so that it owns a workgroup.
-        const int padded = std::max(round_up(a.n, kChunkRows), kChunkRows);
+        // Every system gets at least one slice so that it owns a workgroup.
+        const int padded = std::max(round_up(a.n, kC), kC);
         s.sys_row_begin[k + 1] = s.sys_row_begin[k] + padded;
         s.sys_nat_begin[k + 1] = s.sys_nat_begin[k] + a.n;
     }

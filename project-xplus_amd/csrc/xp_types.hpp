@@ -21,20 +21,23 @@
 
 namespace xp {
 
-constexpr int kC = 64;             // slice height == wavefront width on CDNA
-constexpr int kWaves = 4;          // wavefronts per workgroup in the streaming kernels
-constexpr int kBlock = kC * kWaves;  // 256 threads, one row per thread per chunk
-constexpr int kChunkRows = kBlock;   // rows per work chunk
+constexpr int kC = 64;               // slice height == wavefront width on CDNA
+constexpr int kWaves = 16;           // wavefronts per workgroup in the streaming kernels
+constexpr int kBlock = kC * kWaves;  // 1024 threads
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, workgroups dispatched round-robin
 
-// One logical workgroup's share of the rows: chunks [chunk_begin, chunk_end)
-// of system `sys`.  A workgroup never straddles two systems.
+// One logical workgroup's share of the rows: slices [slice_begin, slice_end)
+// of system `sys` (rows 64*slice_begin .. 64*slice_end-1).  A workgroup
+// never straddles two systems; sys < 0 marks an idle workgroup.
 struct WorkItem {
     int sys;
-    int chunk_begin;
-    int chunk_end;
+    int slice_begin;
+    int slice_end;
     int pad_;
 };
+
+// Reduction slots of the per-workgroup partial sums (partials[slot * grid + block]).
+enum : int { kSlotPap = 0, kSlotRz = 1, kSlotRr = 2, kSlotAux = 3, kNumSlots = 4 };
 
 // Per-system PCG state machine, resident on the device.  Written only by the
 // last workgroup of a reduction (see kernels.hip), read by every kernel of

@@ -97,6 +97,12 @@ _spmv_f64 = _sig("xp_spmv_f64", C.c_int, [_vp, _vp, _vp, _vp])
 _to_solve = _sig("xp_to_solve_order", C.c_int, [_vp, _vp, _vp, _vp])
 _from_solve = _sig("xp_from_solve_order", C.c_int, [_vp, _vp, _vp, _vp])
 _csr_spmv_f64 = _sig("xp_csr_spmv_f64", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p])
+_dev_alloc = _sig("xp_device_alloc", C.c_int, [C.c_ulonglong, C.POINTER(_vp)])
+_dev_free = _sig("xp_device_free", C.c_int, [_vp])
+_htod = _sig("xp_memcpy_htod", C.c_int, [_vp, _vp, C.c_ulonglong])
+_dtoh = _sig("xp_memcpy_dtoh", C.c_int, [_vp, _vp, C.c_ulonglong])
+_dev_sync = _sig("xp_device_synchronize", C.c_int, [])
+_spmv_time = _sig("xp_spmv_f32_time", C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)])
 
 _p_create = _sig("xp_pcg_create", C.c_int, [_vp, C.POINTER(SolveOpts), C.POINTER(_vp)])
 _p_destroy = _sig("xp_pcg_destroy", None, [_vp])
@@ -279,6 +285,52 @@ def sell_build_host(a: Csr, sigma: int = 256) -> dict:
             "col16": c16 if idx16.value else None, "col32": None if idx16.value else c32}
 
 
+# ---------------------------------------------------------------- device memory
+class DeviceArray:
+    """A device buffer owned through the library (hipMalloc), with numpy
+    upload/download.  `ptr` is the device address for the xp_* calls."""
+
+    def __init__(self, count: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        h = _vp()
+        _check(_dev_alloc(self.count * self.dtype.itemsize, C.byref(h)))
+        self.h = h
+
+    @property
+    def ptr(self) -> int:
+        return self.h.value or 0
+
+    @classmethod
+    def from_numpy(cls, a) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size <= self.count
+        _check(_htod(self.h, a.ctypes.data_as(_vp), a.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.count, self.dtype)
+        _check(_dtoh(out.ctypes.data_as(_vp), self.h, out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "h", None) and self.h.value:
+            _dev_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def synchronize() -> None:
+    _check(_dev_sync())
+
+
 # ---------------------------------------------------------------- device matrix
 class Matrix:
     """Device-resident SELL-C-sigma matrix (one system or a batch)."""
@@ -327,6 +379,12 @@ class Matrix:
 
     def spmv_f64(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
         _check(_spmv_f64(self.h, x_ptr, y_ptr, stream or None))
+
+    def spmv_f32_time(self, warmup: int = 20, reps: int = 200) -> float:
+        """Mean microseconds per FP32 SpMV launch (HIP events on the launch stream)."""
+        us = C.c_double()
+        _check(_spmv_time(self.h, warmup, reps, C.byref(us)))
+        return us.value
 
     def to_solve_order(self, nat_ptr: int, solve_ptr: int, stream: int = 0) -> None:
         _check(_to_solve(self.h, nat_ptr, solve_ptr, stream or None))
