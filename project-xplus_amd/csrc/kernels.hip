@@ -59,10 +59,12 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Sums NV values over the workgroup (kBlock threads).  Every thread gets the
-// totals.  Fixed combination order.
+// totals.  Fixed combination order: lanes by butterfly, then the kWaves
+// wavefront sums in wave order by lanes 0..NV-1 of wave 0.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __shared__ double red[kWaves][NV];
+    __shared__ double tot[NV];
     const int wave = static_cast<int>(threadIdx.x) >> 6;
     const int lane = static_cast<int>(threadIdx.x) & 63;
 #pragma unroll
@@ -72,14 +74,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
         for (int k = 0; k < NV; ++k) red[wave][k] = v[k];
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
+    if (static_cast<int>(threadIdx.x) < NV) {
         double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) t += red[w][k];
-        v[k] = t;
+        for (int w = 0; w < kWaves; ++w) t += red[w][threadIdx.x];
+        tot[threadIdx.x] = t;
     }
     __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = tot[k];
 }
 
 // Sum of NV partial slots over the logical blocks [range.x, range.y) of a
@@ -104,19 +106,47 @@ __device__ __forceinline__ void put_partial(const Launch& L, int slot, int lb, d
 __device__ __forceinline__ bool bad(double v) { return !isfinite(v); }
 
 // One SELL row: y = sum_j val * x[col] in FP32, diagonal first.  xdiag
-// receives x[row].
+// receives x[row].  Slices up to kFastW columns wide (every stencil matrix
+// here) take a fully unrolled path that issues all value and column loads of
+// the slice, then all x gathers, before the first FMA, so each wavefront has
+// ~3 w loads in flight; the width test is wave-uniform (no divergence).
+constexpr int kFastW = 10;
+
 template <bool IDX16>
 __device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int lane,
                                               const float* __restrict__ x, float& xdiag) {
     const int j0 = A.slice_ptr[slice];
-    const int j1 = A.slice_ptr[slice + 1];
+    const int w = A.slice_ptr[slice + 1] - j0;
     const int row = slice * kC + lane;
-    size_t e = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
+    const size_t e0 = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
+    const int base = A.slice_base[slice];
+    if (w <= kFastW) {
+        float v[kFastW], xv[kFastW];
+        int c[kFastW];
+#pragma unroll
+        for (int k = 0; k < kFastW; ++k) {
+            if (k < w) {
+                v[k] = A.val[e0 + static_cast<size_t>(k) * kC];
+                c[k] = k == 0 ? row
+                              : (IDX16 ? base + static_cast<int>(A.col16[e0 + static_cast<size_t>(k) * kC])
+                                       : A.col32[e0 + static_cast<size_t>(k) * kC]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kFastW; ++k)
+            if (k < w) xv[k] = x[c[k]];
+        xdiag = xv[0];
+        float acc = v[0] * xv[0];
+#pragma unroll
+        for (int k = 1; k < kFastW; ++k)
+            if (k < w) acc = fmaf(v[k], xv[k], acc);
+        return acc;
+    }
+    size_t e = e0;
     xdiag = x[row];
     float acc = A.val[e] * xdiag;
-    const int base = A.slice_base[slice];
 #pragma unroll 4
-    for (int j = j0 + 1; j < j1; ++j) {
+    for (int j = 1; j < w; ++j) {
         e += kC;
         const int c = IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e];
         acc = fmaf(A.val[e], x[c], acc);
@@ -155,7 +185,7 @@ __device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int
 // init_pcg_kernel (archived/kernels/cg_kernels.cpp:73-103) fused with the
 // x0 SpMV.
 template <bool IDX16>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(Launch L, SellView A, const float* __restrict__ x,
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_init(Launch L, SellView A, const float* __restrict__ x,
                                                      const float* __restrict__ b,
                                                      const float* __restrict__ minv,
                                                      float* __restrict__ r, float* __restrict__ z,
@@ -187,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Launch L, SellView A, const
 // One workgroup per system: the state after init, with the loop-top tests of
 // the on-chip PCG loop (kernels/cuper_pcg_control_kernel.cpp:1123-1130).
 // S holds each system's configuration (tau, max_iters, ...) on entry.
-__global__ __launch_bounds__(kBlock) void k_pcg_init_state(Launch L, SysState* S) {
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_init_state(Launch L, SysState* S) {
     const int sys = static_cast<int>(blockIdx.x);
     double t[2];
     sum_partials<2>(L, sys, {kSlotRz, kSlotRr}, t);
@@ -219,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_state(Launch L, SysState* S
 
 // ap = A p and the partials of pAp.
 template <bool IDX16>
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(Launch L, SellView A, const SysState* S,
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv(Launch L, SellView A, const SysState* S,
                                                      const float* __restrict__ p, float* __restrict__ ap) {
     const int lb = logical_block();
     const WorkItem w = L.work[lb];
@@ -236,11 +266,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(Launch L, SellView A, const
     if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
 }
 
+// Rows handled per thread in one pass of the vector kernels: every load of a
+// pass is issued before the reduction prologue, so the prologue's latency
+// (summing the partials of the previous kernel) overlaps the HBM reads.
+constexpr int kRowsPerThread = 4;
+
 // alpha = rz / pAp; x += alpha p; r -= alpha ap; z = minv r; partials of
 // rz_new, rr_new.  update_xrz_kernel (archived/kernels/cg_kernels.cpp:118-147).
 // A breakdown (bad pAp or alpha) leaves the vectors untouched; the next
 // kernel records it.
-__global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, const SysState* S, float* __restrict__ x,
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const SysState* S, float* __restrict__ x,
                                                            const float* __restrict__ p,
                                                            float* __restrict__ r,
                                                            const float* __restrict__ ap,
@@ -249,22 +284,45 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, const SysSt
     const int lb = logical_block();
     const WorkItem w = L.work[lb];
     if (w.sys < 0 || !S[w.sys].active) return;
-    double t[1];
-    sum_partials<1>(L, w.sys, {kSlotPap}, t);
-    const double pap = t[0];
-    const double alpha = S[w.sys].rz / pap;
-    if (bad(pap) || fabs(pap) <= kBreakdownEps || bad(alpha)) return;
+    const int r0 = w.slice_begin * kC, r1 = w.slice_end * kC;
+    const double rz_old = S[w.sys].rz;
     double v[2] = {0.0, 0.0};
-    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock) {
-        const double pv = p[row];
-        const float xv = static_cast<float>(static_cast<double>(x[row]) + alpha * pv);
-        const float rv = static_cast<float>(static_cast<double>(r[row]) - alpha * static_cast<double>(ap[row]));
-        const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
-        x[row] = xv;
-        r[row] = rv;
-        z[row] = zv;
-        v[0] += static_cast<double>(rv) * static_cast<double>(zv);
-        v[1] += static_cast<double>(rv) * static_cast<double>(rv);
+    bool first = true;
+    double alpha = 0.0;
+    for (int base = r0; base < r1; base += kRowsPerThread * kBlock) {
+        float xv[kRowsPerThread], pv[kRowsPerThread], rv[kRowsPerThread], av[kRowsPerThread], mv[kRowsPerThread];
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
+            if (row < r1) {
+                xv[k] = x[row];
+                pv[k] = p[row];
+                rv[k] = r[row];
+                av[k] = ap[row];
+                mv[k] = minv[row];
+            }
+        }
+        if (first) {
+            first = false;
+            double t[1];
+            sum_partials<1>(L, w.sys, {kSlotPap}, t);
+            alpha = rz_old / t[0];
+            if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;
+        }
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
+            if (row < r1) {
+                const float xn = static_cast<float>(static_cast<double>(xv[k]) + alpha * static_cast<double>(pv[k]));
+                const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * static_cast<double>(av[k]));
+                const float zn = static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
+                x[row] = xn;
+                r[row] = rn;
+                z[row] = zn;
+                v[0] += static_cast<double>(rn) * static_cast<double>(zn);
+                v[1] += static_cast<double>(rn) * static_cast<double>(rn);
+            }
+        }
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
@@ -273,89 +331,120 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_xrz(Launch L, const SysSt
     }
 }
 
-// The scalar tail of the iteration and p = z + beta p
-// (update_p_kernel, archived/kernels/cg_kernels.cpp:149-158; golden loop
-// src/CgSolverGolden.hpp:100-118).  Reads state slot Sc, writes slot Sn.
-__global__ __launch_bounds__(kBlock) void k_pcg_update_p(Launch L, const SysState* Sc, SysState* Sn,
-                                                         TraceRow* trace, int trace_stride,
-                                                         const float* __restrict__ z, float* __restrict__ p) {
-    const int lb = logical_block();
-    const WorkItem w = L.work[lb];
-    if (w.sys < 0) return;
-    const bool leader = lb == L.sys_blocks[w.sys].x && threadIdx.x == 0;
-    SysState s = Sc[w.sys];
-    if (!s.active) {
-        if (leader) Sn[w.sys] = s;  // carry the stopped state forward
-        return;
-    }
-    double t[3];
-    sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);
+// Evaluates the scalar tail of an iteration from the partials: the state
+// after the iteration, in `s`; identical in every thread of every workgroup
+// of the system.  Golden loop src/CgSolverGolden.hpp:100-118; breakdown
+// tests of archived/host/multi_kernel_solver.hpp:170-203; order of the
+// post-update tests as the on-chip loop (kernels/cuper_pcg_control_kernel.cpp:
+// 1193-1214), then the loop bound and the next loop-top rz test (:1127-1130).
+__device__ __forceinline__ void pcg_scalar_step(SysState& s, const double (&t)[3], TraceRow* trace_row) {
     const double pap = t[0];
     const double alpha = s.rz / pap;
     s.pap = pap;
-    // Breakdown tests of run_local_multi_kernel_solver
-    // (archived/host/multi_kernel_solver.hpp:170-180).
     if (bad(pap) || fabs(pap) <= kBreakdownEps || bad(alpha)) {
         s.status = kStatusBreakdown;
         s.active = 0;
-    } else {
-        const double rz_old = s.rz;
-        const double rz = t[1];
-        const double rr = t[2];
-        const double beta = rz / rz_old;
-        if (leader && trace != nullptr && s.iters < s.trace_cap) {
-            TraceRow& row = trace[static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
-                                  static_cast<size_t>(s.iters)];
-            row.alpha = alpha;
-            row.beta = beta;
-            row.rz = rz;
-            row.rr = rr;
+        return;
+    }
+    const double rz_old = s.rz;
+    const double rz = t[1];
+    const double rr = t[2];
+    const double beta = rz / rz_old;
+    if (trace_row != nullptr) {
+        trace_row->alpha = alpha;
+        trace_row->beta = beta;
+        trace_row->rz = rz;
+        trace_row->rr = rr;
+    }
+    s.iters += 1;
+    s.rz = rz;
+    s.rr = rr;
+    s.alpha = alpha;
+    s.beta = beta;
+    if (bad(rz) || bad(rr)) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else if (rr <= s.tau) {
+        s.status = kStatusConverged;
+        s.active = 0;
+    } else if (fabs(rz_old) <= kBreakdownEps || bad(beta)) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else if (s.iters >= s.max_iters) {
+        s.status = kStatusMaxIter;
+        s.active = 0;
+    } else if (fabs(rz) <= kBreakdownEps) {
+        s.status = kStatusBreakdown;
+        s.active = 0;
+    } else if (s.stall_limit > 0) {
+        // Stagnation watch (for preconditioners that may fail): stop when
+        // r^T r has not dropped below half its best for stall_limit iterations.
+        if (rr < 0.5 * s.best_rr) {
+            s.best_rr = rr;
+            s.stall = 0;
+        } else if (++s.stall >= s.stall_limit) {
+            s.status = kStatusStagnation;
+            s.active = 0;
         }
-        s.iters += 1;
-        s.rz = rz;
-        s.rr = rr;
-        s.alpha = alpha;
-        s.beta = beta;
-        // Post-update tests in the order of the on-chip loop
-        // (kernels/cuper_pcg_control_kernel.cpp:1193-1214), then the loop
-        // bound and the next loop-top rz test (:1127-1130).
-        if (bad(rz) || bad(rr)) {
-            s.status = kStatusBreakdown;
-            s.active = 0;
-        } else if (rr <= s.tau) {
-            s.status = kStatusConverged;
-            s.active = 0;
-        } else if (fabs(rz_old) <= kBreakdownEps || bad(beta)) {
-            s.status = kStatusBreakdown;
-            s.active = 0;
-        } else if (s.iters >= s.max_iters) {
-            s.status = kStatusMaxIter;
-            s.active = 0;
-        } else if (fabs(rz) <= kBreakdownEps) {
-            s.status = kStatusBreakdown;
-            s.active = 0;
-        } else if (s.stall_limit > 0) {
-            // Stagnation watch (for preconditioners that may fail): stop when
-            // r^T r has not dropped below half its best for stall_limit iterations.
-            if (rr < 0.5 * s.best_rr) {
-                s.best_rr = rr;
-                s.stall = 0;
-            } else if (++s.stall >= s.stall_limit) {
-                s.status = kStatusStagnation;
-                s.active = 0;
+    }
+}
+
+// The scalar tail of the iteration and p = z + beta p
+// (update_p_kernel, archived/kernels/cg_kernels.cpp:149-158).  Reads state
+// slot Sc, writes slot Sn.
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_update_p(Launch L, const SysState* Sc, SysState* Sn,
+                                                            TraceRow* trace, int trace_stride,
+                                                            const float* __restrict__ z, float* __restrict__ p) {
+    __shared__ SysState s_sh;  // the scalar step runs on LDS: no registers held across it
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    if (w.sys < 0) return;
+    const bool leader_block = lb == L.sys_blocks[w.sys].x;
+    if (!Sc[w.sys].active) {
+        if (leader_block && threadIdx.x == 0) Sn[w.sys] = Sc[w.sys];  // carry the stopped state forward
+        return;
+    }
+    const int r0 = w.slice_begin * kC, r1 = w.slice_end * kC;
+    bool first = true;
+    for (int base = r0; base < r1; base += kRowsPerThread * kBlock) {
+        float zv[kRowsPerThread], pv[kRowsPerThread];
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
+            if (row < r1) {
+                zv[k] = z[row];
+                pv[k] = p[row];
             }
         }
+        if (first) {
+            first = false;
+            double t[3];
+            sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);
+            // One thread evaluates the scalar step; the others only need beta
+            // and whether the system is still running.
+            if (threadIdx.x == 0) {
+                s_sh = Sc[w.sys];
+                TraceRow* tr = nullptr;
+                if (leader_block && trace != nullptr && s_sh.iters < s_sh.trace_cap)
+                    tr = trace + static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
+                         static_cast<size_t>(s_sh.iters);
+                pcg_scalar_step(s_sh, t, tr);
+                if (leader_block) Sn[w.sys] = s_sh;
+            }
+            __syncthreads();
+            if (!s_sh.active) return;
+        }
+        const double beta = s_sh.beta;
+#pragma unroll
+        for (int k = 0; k < kRowsPerThread; ++k) {
+            const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
+            if (row < r1) p[row] = static_cast<float>(static_cast<double>(zv[k]) + beta * static_cast<double>(pv[k]));
+        }
     }
-    if (s.active) {
-        const double beta = s.beta;
-        for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock)
-            p[row] = static_cast<float>(static_cast<double>(z[row]) + beta * static_cast<double>(p[row]));
-    }
-    if (leader) Sn[w.sys] = s;
 }
 
 template <bool IDX16>
-__global__ __launch_bounds__(kBlock) void k_spmv_f32(Launch L, SellView A, const float* __restrict__ x,
+__global__ __launch_bounds__(kBlock, 8) void k_spmv_f32(Launch L, SellView A, const float* __restrict__ x,
                                                      float* __restrict__ y) {
     const WorkItem w = L.work[logical_block()];
     XP_WAVE_LANE
@@ -366,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f32(Launch L, SellView A, const
 }
 
 template <bool IDX16, int VMODE>
-__global__ __launch_bounds__(kBlock) void k_spmv_f64(Launch L, SellView A, const double* __restrict__ x,
+__global__ __launch_bounds__(kBlock, 8) void k_spmv_f64(Launch L, SellView A, const double* __restrict__ x,
                                                      double* __restrict__ y) {
     const WorkItem w = L.work[logical_block()];
     XP_WAVE_LANE
@@ -376,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_f64(Launch L, SellView A, const
 
 // r = b - A x in FP64 and the partials of r^T r.
 template <bool IDX16, int VMODE>
-__global__ __launch_bounds__(kBlock) void k_residual_f64(Launch L, SellView A, const double* __restrict__ x,
+__global__ __launch_bounds__(kBlock, 8) void k_residual_f64(Launch L, SellView A, const double* __restrict__ x,
                                                          const double* __restrict__ b, double* __restrict__ r,
                                                          float* __restrict__ r32) {
     const int lb = logical_block();
@@ -396,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_f64(Launch L, SellView A, c
 }
 
 // One workgroup per system: out[sys] = sum of the kSlotAux partials.
-__global__ __launch_bounds__(kBlock) void k_finish_aux(Launch L, double* out) {
+__global__ __launch_bounds__(kBlock, 8) void k_finish_aux(Launch L, double* out) {
     const int sys = static_cast<int>(blockIdx.x);
     double t[1];
     sum_partials<1>(L, sys, {kSlotAux}, t);
@@ -430,7 +519,7 @@ __device__ __forceinline__ void flat_put(double* partials, double (&v)[NV]) {
 }
 
 template <int NV>
-__global__ __launch_bounds__(kBlock) void k_flat_finish(const double* partials, int nblocks, double* out) {
+__global__ __launch_bounds__(kBlock, 8) void k_flat_finish(const double* partials, int nblocks, double* out) {
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
@@ -445,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void k_flat_finish(const double* partials, 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_init_pcg_f64(int n, const double* __restrict__ b,
+__global__ __launch_bounds__(kBlock, 8) void k_init_pcg_f64(int n, const double* __restrict__ b,
                                                          const double* __restrict__ ax,
                                                          const double* __restrict__ m_inv,
                                                          double* __restrict__ r, double* __restrict__ z,
@@ -465,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_init_pcg_f64(int n, const double* __
     flat_put<2>(partials, v);
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot_f64(int n, const double* __restrict__ a,
+__global__ __launch_bounds__(kBlock, 8) void k_dot_f64(int n, const double* __restrict__ a,
                                                     const double* __restrict__ b, double* partials) {
     int i0, i1;
     flat_range(n, i0, i1);
@@ -474,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_dot_f64(int n, const double* __restr
     flat_put<1>(partials, v);
 }
 
-__global__ __launch_bounds__(kBlock) void k_update_xrz_f64(int n, double* __restrict__ x,
+__global__ __launch_bounds__(kBlock, 8) void k_update_xrz_f64(int n, double* __restrict__ x,
                                                            const double* __restrict__ p,
                                                            double* __restrict__ r,
                                                            const double* __restrict__ ap,

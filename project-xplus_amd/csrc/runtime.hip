@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -417,6 +418,11 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
     d.use_graph = 1;
     d.trace_cap = 0;
     if (o != nullptr) d = *o;
+    // XPLUS_NO_GRAPH=1 launches the same kernels directly instead of through
+    // a captured graph (profilers that do not follow graph launches).
+    if (const char* e = std::getenv("XPLUS_NO_GRAPH")) {
+        if (e[0] == '1') d.use_graph = 0;
+    }
     if (d.graph_iters <= 0) d.graph_iters = m->n_pad <= 65536 ? 64 : 32;
     d.graph_iters += d.graph_iters % 2;  // even: the state slots alternate
     s->opts = d;
@@ -446,7 +452,8 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
         t.trace_cap = s->trace_stride;
     }
     HIPX(hipHostMalloc(reinterpret_cast<void**>(&s->st_host), static_cast<size_t>(m->nsys) * sizeof(xp::SysState)));
-    s->build_graph(d.graph_iters);
+    if (d.use_graph) s->build_graph(d.graph_iters);
+    s->graph_iters = d.graph_iters;
     HIPX(hipStreamSynchronize(s->stream));
     return s;
 }
