@@ -78,6 +78,7 @@ def main() -> int:
     ap.add_argument("--mesh", type=int, default=1100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner-rel", type=float, default=1e-2)
+    ap.add_argument("--no-tune-sigma", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -94,12 +95,21 @@ def main() -> int:
     a = X.gen_fem2d(args.mesh, 1.0, 20, 1e3, 0.3, seed)
     log(f"generated FEM mesh {args.mesh}: n={a.n} nnz={a.nnz} in {time.time() - t0:.1f}s")
     m = X.Matrix(a, sigma=256, keep_lo=True, grid=args.grid)
-    info = m.info()
-    log(f"matrix: {info}")
     bnorm = float(np.linalg.norm(a.b))
+    # autotuner: sigma x SpMV inner loop x workgroups, for both iteration forms
+    t0 = time.perf_counter()
+    tune_f = m.autotune(fused=True, tune_sigma=not args.no_tune_sigma, iters=64)
+    tune_u = m.autotune(fused=False, tune_sigma=False, iters=64)
+    fused = tune_f["us_per_iter"] < tune_u["us_per_iter"]
+    if fused:
+        m.autotune(fused=True, tune_sigma=not args.no_tune_sigma, iters=64)  # cached: re-applies the fused choice
+    tune_ms = (time.perf_counter() - t0) * 1000.0
+    info = m.info()
+    log(f"autotune ({tune_ms:.0f} ms): fused={tune_f} unfused={tune_u} -> fused={fused}")
+    log(f"matrix: {info}")
 
     # plain FP32 Jacobi-PCG to recursive relative residual 1e-6
-    s = X.Pcg(m, rel_tol=1e-6)
+    s = X.Pcg(m, rel_tol=1e-6, fused=fused)
     s.set_rhs(a.b, a.x0)
     for w in range(args.warmup):
         s.run()
@@ -115,7 +125,7 @@ def main() -> int:
         f"{plain_ms:.2f} ms/solve (device {s.last_timing()[0]:.2f} ms), true rel residual {plain_true:.3e}")
 
     # refined: FP32 inner Jacobi-PCG, FP64 outer residual, to TRUE relative residual 1e-6
-    r = X.Pcg(m, rel_tol=1e-6)
+    r = X.Pcg(m, rel_tol=1e-6, fused=fused)
     r.set_rhs(a.b, a.x0)
     for w in range(args.warmup):
         r.run_refined(1e-6, args.inner_rel, 50)
@@ -157,7 +167,8 @@ def main() -> int:
                    "n": a.n, "nnz": a.nnz,
                    "method": "FP32 Jacobi-PCG inner (FP64 reductions) + FP64 outer residual/update",
                    "inner_rel": args.inner_rel, "outer_steps": outer, "inner_iterations": inner_total,
-                   "grid": info["grid"], "sigma": info["sigma"]},
+                   "grid": info["grid"], "sigma": info["sigma"], "fused_iteration": fused,
+                   "autotune": {"fused": tune_f, "unfused": tune_u, "ms": round(tune_ms, 1)}},
         "refined_ms": round(ref_ms, 3),
         "true_rel_residual": float(rel[0]),
         "plain_ms": round(plain_ms, 3),

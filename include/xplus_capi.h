@@ -84,6 +84,7 @@ typedef struct {
     int keep_lo; /* keep the FP32 low-part stream (FP64-accurate SpMV, refinement) */
     int keep64;  /* keep exact FP64 values (drop-in FP64 SpMV) */
     int grid;    /* workgroups of the streaming kernels; 0 = library default */
+    int spmv_variant; /* SpMV inner loop: 0 rolled loop, 1 fixed-width unrolled */
 } xp_matrix_opts;
 
 typedef struct {
@@ -108,6 +109,21 @@ int xp_matrix_get_info(const xp_matrix* m, xp_matrix_info* out);
 int xp_matrix_permutation(const xp_matrix* m, int* perm);
 /* Changes the workgroup count of the streaming kernels (autotuning). */
 int xp_matrix_set_grid(xp_matrix* m, int grid);
+int xp_matrix_set_variant(xp_matrix* m, int spmv_variant);
+
+/* Autotuner: times `iters` Jacobi-PCG iterations (the real graph-captured
+ * iteration, 3-kernel or fused per `fused`) for every candidate of SpMV
+ * inner loop x workgroup count (x sort window sigma when tune_sigma != 0,
+ * which re-sorts the matrix and so must run before any solver is created on
+ * it), keeps the fastest and returns it.  Results are cached per matrix
+ * signature (n, nnz, content hash) for the life of the process. */
+typedef struct {
+    int sigma, spmv_variant, grid;
+    double us_per_iter;      /* best */
+    double us_per_iter_default;
+    int candidates;
+} xp_tune_report;
+int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int iters, xp_tune_report* out);
 
 /* y = A x, FP32 values and vectors, FP32 accumulation. */
 int xp_spmv_f32(xp_matrix* m, const float* x, float* y, void* stream);
@@ -165,6 +181,8 @@ typedef struct {
     int trace_cap;     /* trace rows kept per system */
     int stall_limit;   /* 0 = off; else stop with XP_STAGNATION after this many
                           iterations without halving the best r^T r */
+    int fused;         /* 1: two kernels per iteration (p update folded into the
+                          SpMV via A p = A z + beta A p_old); 0: three kernels */
 } xp_solve_opts;
 
 typedef struct {

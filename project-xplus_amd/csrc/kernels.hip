@@ -112,7 +112,7 @@ __device__ __forceinline__ bool bad(double v) { return !isfinite(v); }
 // ~3 w loads in flight; the width test is wave-uniform (no divergence).
 constexpr int kFastW = 10;
 
-template <bool IDX16>
+template <bool IDX16, int V>
 __device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int lane,
                                               const float* __restrict__ x, float& xdiag) {
     const int j0 = A.slice_ptr[slice];
@@ -120,7 +120,7 @@ __device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int 
     const int row = slice * kC + lane;
     const size_t e0 = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
     const int base = A.slice_base[slice];
-    if (w <= kFastW) {
+    if (V == 1 && w <= kFastW) {
         float v[kFastW], xv[kFastW];
         int c[kFastW];
 #pragma unroll
@@ -184,7 +184,7 @@ __device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int
 // PCG init: r = b - A x, z = minv r, p = z; partials of rz, rr.  Mirrors
 // init_pcg_kernel (archived/kernels/cg_kernels.cpp:73-103) fused with the
 // x0 SpMV.
-template <bool IDX16>
+template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_init(Launch L, SellView A, const float* __restrict__ x,
                                                      const float* __restrict__ b,
                                                      const float* __restrict__ minv,
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_init(Launch L, SellView A, co
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         const int row = sl * kC + lane;
         float xd;
-        const float ax = sell_row_f32<IDX16>(A, sl, lane, x, xd);
+        const float ax = sell_row_f32<IDX16, V>(A, sl, lane, x, xd);
         const float rv = b[row] - ax;
         const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
         r[row] = rv;
@@ -232,6 +232,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_init_state(Launch L, SysState
     s.pap = 0.0;
     s.alpha = 0.0;
     s.beta = 0.0;
+    s.fresh = 1;
     if (s.rr <= s.tau) {
         s.status = kStatusConverged;
         s.active = 0;
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_init_state(Launch L, SysState
 }
 
 // ap = A p and the partials of pAp.
-template <bool IDX16>
+template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv(Launch L, SellView A, const SysState* S,
                                                      const float* __restrict__ p, float* __restrict__ ap) {
     const int lb = logical_block();
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv(Launch L, SellView A, co
     double v[1] = {0.0};
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         float pd;
-        const float y = sell_row_f32<IDX16>(A, sl, lane, p, pd);
+        const float y = sell_row_f32<IDX16, V>(A, sl, lane, p, pd);
         ap[sl * kC + lane] = y;
         v[0] += static_cast<double>(pd) * static_cast<double>(y);
     }
@@ -443,14 +444,78 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_p(Launch L, const SysS
     }
 }
 
-template <bool IDX16>
+// Fused iteration, first kernel: finishes the previous iteration (scalar
+// tail from the rz/rr partials, as k_pcg_update_p does) and forms the new
+// direction without a separate pass, using A p_new = A z + beta A p_old:
+//   p = z + beta p;  ap = A z + beta ap;  partials of pAp.
+// Right after init (state.fresh) p = z already and ap = A z.  Reads slot Sc,
+// writes slot Sn; the k_pcg_update_xrz that follows reads Sn.
+template <bool IDX16, int V>
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView A, const SysState* Sc,
+                                                              SysState* Sn, TraceRow* trace, int trace_stride,
+                                                              const float* __restrict__ z, float* __restrict__ p,
+                                                              float* __restrict__ ap) {
+    __shared__ SysState s_sh;
+    const int lb = logical_block();
+    const WorkItem w = L.work[lb];
+    if (w.sys < 0) return;
+    const bool leader_block = lb == L.sys_blocks[w.sys].x;
+    if (!Sc[w.sys].active) {
+        if (leader_block && threadIdx.x == 0) Sn[w.sys] = Sc[w.sys];
+        return;
+    }
+    const bool fresh = Sc[w.sys].fresh != 0;
+    double beta = 0.0;
+    if (!fresh) {
+        double t[3];
+        sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);
+        if (threadIdx.x == 0) {
+            s_sh = Sc[w.sys];
+            TraceRow* tr = nullptr;
+            if (leader_block && trace != nullptr && s_sh.iters < s_sh.trace_cap)
+                tr = trace + static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
+                     static_cast<size_t>(s_sh.iters);
+            pcg_scalar_step(s_sh, t, tr);
+            if (leader_block) Sn[w.sys] = s_sh;
+        }
+        __syncthreads();
+        if (!s_sh.active) return;
+        beta = s_sh.beta;
+    } else if (leader_block && threadIdx.x == 0) {
+        SysState s0 = Sc[w.sys];
+        s0.fresh = 0;
+        Sn[w.sys] = s0;
+    }
+    XP_WAVE_LANE
+    double v[1] = {0.0};
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
+        float zd;
+        const float az = sell_row_f32<IDX16, V>(A, sl, lane, z, zd);
+        float pn, an;
+        if (fresh) {
+            pn = zd;
+            an = az;
+        } else {
+            pn = static_cast<float>(static_cast<double>(zd) + beta * static_cast<double>(p[row]));
+            an = static_cast<float>(static_cast<double>(az) + beta * static_cast<double>(ap[row]));
+            p[row] = pn;
+        }
+        ap[row] = an;
+        v[0] += static_cast<double>(pn) * static_cast<double>(an);
+    }
+    block_sum<1>(v);
+    if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
+}
+
+template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_spmv_f32(Launch L, SellView A, const float* __restrict__ x,
                                                      float* __restrict__ y) {
     const WorkItem w = L.work[logical_block()];
     XP_WAVE_LANE
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         float xd;
-        y[sl * kC + lane] = sell_row_f32<IDX16>(A, sl, lane, x, xd);
+        y[sl * kC + lane] = sell_row_f32<IDX16, V>(A, sl, lane, x, xd);
     }
 }
 
@@ -623,20 +688,35 @@ int blocks_for(size_t n, int bs) { return static_cast<int>((n + static_cast<size
 
 }  // namespace
 
+#define XP_DISPATCH_SELL(A, KERNEL, ...)                                                            \
+    do {                                                                                        \
+        if ((A).idx16) {                                                                        \
+            if ((A).variant == 1)                                                               \
+                hipLaunchKernelGGL((KERNEL<true, 1>), __VA_ARGS__);                             \
+            else                                                                                \
+                hipLaunchKernelGGL((KERNEL<true, 0>), __VA_ARGS__);                             \
+        } else {                                                                                \
+            if ((A).variant == 1)                                                               \
+                hipLaunchKernelGGL((KERNEL<false, 1>), __VA_ARGS__);                            \
+            else                                                                                \
+                hipLaunchKernelGGL((KERNEL<false, 0>), __VA_ARGS__);                            \
+        }                                                                                       \
+    } while (0)
+
 void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
                      const float* b, const float* minv, float* r, float* z, float* p) {
-    if (A.idx16)
-        hipLaunchKernelGGL(k_pcg_init<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, r, z, p);
-    else
-        hipLaunchKernelGGL(k_pcg_init<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, r, z, p);
+    XP_DISPATCH_SELL(A, k_pcg_init, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, r, z, p);
     hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
 }
 
 void launch_pcg_spmv(const Launch& L, const SellView& A, const SysState* st, const float* p, float* ap) {
-    if (A.idx16)
-        hipLaunchKernelGGL(k_pcg_spmv<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
-    else
-        hipLaunchKernelGGL(k_pcg_spmv<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
+    XP_DISPATCH_SELL(A, k_pcg_spmv, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
+}
+
+void launch_pcg_spmv_fused(const Launch& L, const SellView& A, const SysState* st_cur, SysState* st_next,
+                           TraceRow* trace, int trace_stride, const float* z, float* p, float* ap) {
+    XP_DISPATCH_SELL(A, k_pcg_spmv_fused, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st_cur, st_next, trace,
+                     trace_stride, z, p, ap);
 }
 
 void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
@@ -651,10 +731,7 @@ void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_n
 }
 
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y) {
-    if (A.idx16)
-        hipLaunchKernelGGL(k_spmv_f32<true>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
-    else
-        hipLaunchKernelGGL(k_spmv_f32<false>, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+    XP_DISPATCH_SELL(A, k_spmv_f32, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
 }
 
 void launch_spmv_f64(const Launch& L, const SellView& A, bool use_lo, const double* x, double* y) {

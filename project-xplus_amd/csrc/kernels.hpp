@@ -28,6 +28,10 @@ void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const flo
                      const float* b, const float* minv, float* r, float* z, float* p);
 // ap = A p and the pAp partials.
 void launch_pcg_spmv(const Launch& L, const SellView& A, const SysState* st, const float* p, float* ap);
+// Fused iteration, first kernel: scalar tail of the previous iteration,
+// p = z + beta p, ap = A z + beta ap, pAp partials; reads st_cur, writes st_next.
+void launch_pcg_spmv_fused(const Launch& L, const SellView& A, const SysState* st_cur, SysState* st_next,
+                           TraceRow* trace, int trace_stride, const float* z, float* p, float* ap);
 // alpha; x += alpha p; r -= alpha ap; z = minv r; rz, rr partials.
 void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
                            const float* ap, const float* minv, float* z);

@@ -134,7 +134,11 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // ---- matrices ----------------------------------------------------------------
 struct xp_matrix {
-    int n = 0, n_pad = 0, nslices = 0, nsys = 0, sigma = 0, idx16 = 0;
+    int n = 0, n_pad = 0, nslices = 0, nsys = 0, sigma = 0, idx16 = 0, variant = 0;
+    int solvers = 0;                  // live xp_pcg objects built on this matrix
+    bool keep_lo = false, keep64 = false;
+    std::vector<int> h_row_ptr, h_col_idx, h_ns;  // host copy of the CSR batch (for re-sorting)
+    std::vector<double> h_values;
     long long nnz = 0, stored = 0;
     bool has_lo = false, has64 = false;
     std::vector<int> perm;            // solve -> natural
@@ -159,6 +163,7 @@ struct xp_matrix {
         v.nslices = nslices;
         v.nsys = nsys;
         v.idx16 = idx16;
+        v.variant = variant;
         v.slice_ptr = d_slice_ptr.p;
         v.slice_base = d_slice_base.p;
         v.val = d_val.p;
@@ -229,30 +234,35 @@ int default_grid(const xp_matrix& m) {
     return std::min(m.nslices, std::max(8, device().cus * 2));
 }
 
-std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, const xp_matrix_opts* o) {
+// (Re)builds the SELL arrays of m from its host CSR copy with window sigma.
+void load_sell(xp_matrix& m, int sigma) {
     const hipStream_t s = device().stream;
-    const int sigma = (o != nullptr && o->sigma > 0) ? o->sigma : 256;
-    const bool keep_lo = o != nullptr && o->keep_lo;
-    const bool keep64 = o != nullptr && o->keep64;
-    xp::SellHost h = xp::build_sell(systems, sigma, keep_lo, keep64);
-    auto m = std::make_unique<xp_matrix>();
-    m->n = h.n;
-    m->n_pad = h.n_pad;
-    m->nslices = h.nslices;
-    m->nsys = h.nsys;
-    m->sigma = h.sigma;
-    m->idx16 = h.idx16;
-    m->nnz = h.nnz;
-    m->stored = h.stored;
-    m->has_lo = keep_lo;
-    m->has64 = keep64;
-    m->perm = h.perm;
-    m->sys_row_begin = h.sys_row_begin;
-    m->sys_nat_begin = h.sys_nat_begin;
+    std::vector<xp::CsrRef> systems;
+    size_t rp_off = 0, nz_off = 0;
+    for (int nk : m.h_ns) {
+        const int* rp = m.h_row_ptr.data() + rp_off;
+        systems.push_back(xp::CsrRef{nk, rp, m.h_col_idx.data() + nz_off, m.h_values.data() + nz_off});
+        rp_off += static_cast<size_t>(nk) + 1;
+        nz_off += static_cast<size_t>(rp[nk]);
+    }
+    xp::SellHost h = xp::build_sell(systems, sigma, m.keep_lo, m.keep64);
+    m.n = h.n;
+    m.n_pad = h.n_pad;
+    m.nslices = h.nslices;
+    m.nsys = h.nsys;
+    m.sigma = h.sigma;
+    m.idx16 = h.idx16;
+    m.nnz = h.nnz;
+    m.stored = h.stored;
+    m.has_lo = m.keep_lo;
+    m.has64 = m.keep64;
+    m.perm = h.perm;
+    m.sys_row_begin = h.sys_row_begin;
+    m.sys_nat_begin = h.sys_nat_begin;
     // Diagonal per natural row, with the reference's two failure cases kept
     // for the solver (src/CsrDataset.hpp:92-116).
-    m->diag.assign(static_cast<size_t>(h.n), 0.0);
-    m->diag_state.assign(static_cast<size_t>(h.n), 0);
+    m.diag.assign(static_cast<size_t>(h.n), 0.0);
+    m.diag_state.assign(static_cast<size_t>(h.n), 0);
     for (int k = 0; k < h.nsys; ++k) {
         const xp::CsrRef& a = systems[static_cast<size_t>(k)];
         for (int r = 0; r < a.n; ++r) {
@@ -260,24 +270,48 @@ std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, c
             int state = 1;
             for (int o2 = a.row_ptr[r]; o2 < a.row_ptr[r + 1]; ++o2)
                 if (a.col_idx[o2] == r) {
-                    m->diag[g] = a.values[o2];
+                    m.diag[g] = a.values[o2];
                     state = std::fabs(a.values[o2]) <= std::numeric_limits<double>::min() ? 2 : 0;
                     break;
                 }
-            m->diag_state[g] = state;
+            m.diag_state[g] = state;
         }
     }
-    m->d_slice_ptr = to_device(h.slice_ptr, s);
-    m->d_slice_base = to_device(h.slice_base, s);
-    m->d_perm = to_device(h.perm, s);
-    m->d_val = to_device(h.val, s);
-    if (keep_lo) m->d_val_lo = to_device(h.val_lo, s);
-    if (keep64) m->d_val64 = to_device(h.val64, s);
+    m.d_slice_ptr = to_device(h.slice_ptr, s);
+    m.d_slice_base = to_device(h.slice_base, s);
+    m.d_perm = to_device(h.perm, s);
+    m.d_val = to_device(h.val, s);
+    m.d_val_lo.release();
+    m.d_val64.release();
+    m.d_col16.release();
+    m.d_col32.release();
+    if (m.keep_lo) m.d_val_lo = to_device(h.val_lo, s);
+    if (m.keep64) m.d_val64 = to_device(h.val64, s);
     if (h.idx16)
-        m->d_col16 = to_device(h.col16, s);
+        m.d_col16 = to_device(h.col16, s);
     else
-        m->d_col32 = to_device(h.col32, s);
+        m.d_col32 = to_device(h.col32, s);
     HIPX(hipStreamSynchronize(s));
+}
+
+std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, const xp_matrix_opts* o) {
+    auto m = std::make_unique<xp_matrix>();
+    m->keep_lo = o != nullptr && o->keep_lo;
+    m->keep64 = o != nullptr && o->keep64;
+    m->variant = (o != nullptr && o->spmv_variant == 1) ? 1 : 0;
+    for (const auto& a : systems) {
+        if (a.n < 0) throw std::runtime_error("build_sell: negative n");
+        if (a.n > 0 && (a.row_ptr == nullptr || a.col_idx == nullptr || a.values == nullptr))
+            throw std::runtime_error("build_sell: null CSR array");
+        m->h_ns.push_back(a.n);
+        const int nnz = a.n > 0 ? a.row_ptr[a.n] : 0;
+        if (a.n > 0 && (a.row_ptr[0] != 0 || nnz < 0)) throw std::runtime_error("row_ptr[0] must be 0");
+        m->h_row_ptr.insert(m->h_row_ptr.end(), a.row_ptr, a.row_ptr + a.n + 1);
+        m->h_col_idx.insert(m->h_col_idx.end(), a.col_idx, a.col_idx + nnz);
+        m->h_values.insert(m->h_values.end(), a.values, a.values + nnz);
+    }
+    device();
+    load_sell(*m, (o != nullptr && o->sigma > 0) ? o->sigma : 256);
     const int grid = (o != nullptr && o->grid > 0) ? o->grid : default_grid(*m);
     build_work_map(*m, grid);
     return m;
@@ -312,6 +346,7 @@ struct xp_pcg {
     const float* rhs = nullptr;       // RHS of the init kernel; null = b
 
     ~xp_pcg() {
+        if (m != nullptr) --m->solvers;
         if (exec != nullptr) (void)hipGraphExecDestroy(exec);
         if (graph != nullptr) (void)hipGraphDestroy(graph);
         if (ev0 != nullptr) (void)hipEventDestroy(ev0);
@@ -329,9 +364,14 @@ struct xp_pcg {
         const size_t ns = static_cast<size_t>(m->nsys);
         xp::SysState* cur = st.p + (k % 2) * ns;
         xp::SysState* nxt = st.p + ((k + 1) % 2) * ns;
-        xp::launch_pcg_spmv(L, A, cur, p.p, ap.p);
-        xp::launch_pcg_update_xrz(L, cur, x.p, p.p, r.p, ap.p, minv.p, z.p);
-        xp::launch_pcg_update_p(L, cur, nxt, trace.p, trace_stride, z.p, p.p);
+        if (opts.fused) {
+            xp::launch_pcg_spmv_fused(L, A, cur, nxt, trace.p, trace_stride, z.p, p.p, ap.p);
+            xp::launch_pcg_update_xrz(L, nxt, x.p, p.p, r.p, ap.p, minv.p, z.p);
+        } else {
+            xp::launch_pcg_spmv(L, A, cur, p.p, ap.p);
+            xp::launch_pcg_update_xrz(L, cur, x.p, p.p, r.p, ap.p, minv.p, z.p);
+            xp::launch_pcg_update_p(L, cur, nxt, trace.p, trace_stride, z.p, p.p);
+        }
     }
 
     void build_graph(int iters) {
@@ -413,6 +453,7 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
     }
     auto s = std::make_unique<xp_pcg>();
     s->m = m;
+    ++m->solvers;
     xp_solve_opts d{};
     d.tau = 1.0e-10;  // include/cg_common.hpp:53 SolverConfig::tau
     d.use_graph = 1;
@@ -747,6 +788,13 @@ int xp_matrix_set_grid(xp_matrix* m, int grid) {
     return guarded([&] {
         if (m == nullptr || grid <= 0) throw std::runtime_error("bad argument");
         build_work_map(*m, grid);
+    });
+}
+
+int xp_matrix_set_variant(xp_matrix* m, int spmv_variant) {
+    return guarded([&] {
+        if (m == nullptr || spmv_variant < 0 || spmv_variant > 1) throw std::runtime_error("bad argument");
+        m->variant = spmv_variant;
     });
 }
 
@@ -1211,3 +1259,100 @@ int xp_sell_copy_host(int* perm, int* slice_ptr, int* slice_base, float* val, un
 }
 
 }  // extern "C"
+
+// ---- autotuner -------------------------------------------------------------------
+namespace {
+
+struct TuneChoice {
+    int sigma, variant, grid;
+    double us, us_default;
+    int candidates;
+};
+
+std::mutex g_tune_mu;
+std::vector<std::pair<std::vector<uint64_t>, TuneChoice>> g_tune_cache;
+
+// Mean time per iteration of `iters` graph-captured PCG iterations on the
+// current layout of m (after one untimed run).
+double time_iterations(xp_matrix* m, int fused, int iters) {
+    xp_solve_opts o{};
+    o.tau = 1.0e-300;
+    o.max_iters = iters;
+    o.graph_iters = iters;
+    o.use_graph = 1;
+    o.fused = fused;
+    auto s = make_pcg(m, &o);
+    std::vector<double> b(static_cast<size_t>(m->n));
+    for (size_t i = 0; i < b.size(); ++i) b[i] = 1.0 + static_cast<double>(i % 7) * 0.125;
+    set_rhs(*s, b.data(), nullptr);
+    double best = 1e300;
+    for (int rep = 0; rep < 3; ++rep) {
+        HIPX(hipMemcpyAsync(s->x.p, s->x0s.p, s->x0s.n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+        s->reset_state();
+        s->run_loop(iters);
+        if (rep > 0) best = std::min(best, s->last_ms * 1000.0 / iters);
+    }
+    return best;
+}
+
+}  // namespace
+
+extern "C" int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int iters, xp_tune_report* out) {
+    return guarded([&] {
+        if (m == nullptr) throw std::runtime_error("null matrix");
+        if (tune_sigma && m->solvers > 0)
+            throw std::runtime_error("autotune with tune_sigma re-sorts the matrix: call it before creating solvers");
+        if (iters <= 0) iters = 64;
+        iters += iters % 2;
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        uint64_t h = mix_hash(0x7475ull, m->h_row_ptr.data(), m->h_row_ptr.size() * sizeof(int));
+        h = mix_hash(h, m->h_col_idx.data(), m->h_col_idx.size() * sizeof(int));
+        h = mix_hash(h, m->h_values.data(), m->h_values.size() * sizeof(double));
+        const std::vector<uint64_t> key = {static_cast<uint64_t>(m->n), static_cast<uint64_t>(m->nnz), h,
+                                           static_cast<uint64_t>(fused), static_cast<uint64_t>(tune_sigma),
+                                           static_cast<uint64_t>(m->nsys)};
+        TuneChoice best{m->sigma, m->variant, m->grid, 1e300, 0.0, 0};
+        bool cached = false;
+        for (const auto& kv : g_tune_cache)
+            if (kv.first == key) { best = kv.second; cached = true; }
+        if (!cached) {
+            std::vector<int> sigmas = {m->sigma};
+            if (tune_sigma) sigmas = {64, 256, 1024, 4096};
+            const int sigma0 = m->sigma;
+            for (int sigma : sigmas) {
+                if (sigma != m->sigma) load_sell(*m, sigma);
+                std::vector<int> grids = {256, 512, 768, 1024, 2048, default_grid(*m)};
+                std::sort(grids.begin(), grids.end());
+                grids.erase(std::unique(grids.begin(), grids.end()), grids.end());
+                for (int v = 0; v <= 1; ++v) {
+                    m->variant = v;
+                    for (int g : grids) {
+                        if (g > m->nslices) continue;
+                        build_work_map(*m, g);
+                        const double us = time_iterations(m, fused, iters);
+                        ++best.candidates;
+                        if (sigma == sigma0 && v == 0 && g == default_grid(*m)) best.us_default = us;
+                        if (us < best.us) {
+                            best.us = us;
+                            best.sigma = sigma;
+                            best.variant = v;
+                            best.grid = m->grid;
+                        }
+                    }
+                }
+            }
+            g_tune_cache.emplace_back(key, best);
+        }
+        if (best.sigma != m->sigma) load_sell(*m, best.sigma);
+        m->variant = best.variant;
+        build_work_map(*m, best.grid);
+        if (out != nullptr) {
+            out->sigma = best.sigma;
+            out->spmv_variant = best.variant;
+            out->grid = m->grid;
+            out->us_per_iter = best.us;
+            out->us_per_iter_default = best.us_default;
+            out->candidates = best.candidates;
+        }
+    });
+}

@@ -58,7 +58,7 @@ struct SysState {
     int stall;        // iterations since best_rr last improved by the stagnation factor
     int stall_limit;  // 0 = stagnation watch off
     int trace_cap;
-    int pad_;
+    int fresh;        // 1 right after init (fused iteration: no previous step to finish)
 };
 
 enum : int {
@@ -84,6 +84,7 @@ struct SellView {
     int nslices;
     int nsys;
     int idx16;               // 1: col16 deltas from slice_base; 0: col32 absolute
+    int variant;             // SpMV inner loop: 0 rolled loop, 1 fixed-width unrolled
     const int* slice_ptr;    // [nslices + 1], in units of 64-entry columns
     const int* slice_base;   // [nslices]
     const float* val;        // [slice_ptr[nslices] * 64], diagonal first

@@ -41,7 +41,8 @@ STATUS_NAMES = {0: "converged", 1: "max_iter", 2: "breakdown", 3: "stagnation", 
 
 
 class MatrixOpts(C.Structure):
-    _fields_ = [("sigma", C.c_int), ("keep_lo", C.c_int), ("keep64", C.c_int), ("grid", C.c_int)]
+    _fields_ = [("sigma", C.c_int), ("keep_lo", C.c_int), ("keep64", C.c_int), ("grid", C.c_int),
+                ("spmv_variant", C.c_int)]
 
 
 class MatrixInfo(C.Structure):
@@ -53,7 +54,12 @@ class MatrixInfo(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("tau", C.c_double), ("rel_tol", C.c_double), ("max_iters", C.c_int),
                 ("graph_iters", C.c_int), ("use_graph", C.c_int), ("trace_cap", C.c_int),
-                ("stall_limit", C.c_int)]
+                ("stall_limit", C.c_int), ("fused", C.c_int)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("sigma", C.c_int), ("spmv_variant", C.c_int), ("grid", C.c_int),
+                ("us_per_iter", C.c_double), ("us_per_iter_default", C.c_double), ("candidates", C.c_int)]
 
 
 class Status(C.Structure):
@@ -92,6 +98,8 @@ _m_destroy = _sig("xp_matrix_destroy", None, [_vp])
 _m_info = _sig("xp_matrix_get_info", C.c_int, [_vp, C.POINTER(MatrixInfo)])
 _m_perm = _sig("xp_matrix_permutation", C.c_int, [_vp, _i32p])
 _m_set_grid = _sig("xp_matrix_set_grid", C.c_int, [_vp, C.c_int])
+_m_set_variant = _sig("xp_matrix_set_variant", C.c_int, [_vp, C.c_int])
+_m_autotune = _sig("xp_matrix_autotune", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(TuneReport)])
 _spmv_f32 = _sig("xp_spmv_f32", C.c_int, [_vp, _vp, _vp, _vp])
 _spmv_f64 = _sig("xp_spmv_f64", C.c_int, [_vp, _vp, _vp, _vp])
 _to_solve = _sig("xp_to_solve_order", C.c_int, [_vp, _vp, _vp, _vp])
@@ -335,11 +343,12 @@ def synchronize() -> None:
 class Matrix:
     """Device-resident SELL-C-sigma matrix (one system or a batch)."""
 
-    def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0):
+    def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0,
+                 spmv_variant: int = 0):
         if isinstance(systems, Csr):
             systems = [systems]
         self.systems = list(systems)
-        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid)
+        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant)
         h = _vp()
         if len(self.systems) == 1:
             a = self.systems[0]
@@ -373,6 +382,16 @@ class Matrix:
     def set_grid(self, grid: int) -> None:
         _check(_m_set_grid(self.h, grid))
 
+    def set_variant(self, v: int) -> None:
+        _check(_m_set_variant(self.h, v))
+
+    def autotune(self, fused: bool = False, tune_sigma: bool = False, iters: int = 64) -> dict:
+        """Picks SpMV inner loop x workgroup count (x sigma) by timing real PCG
+        iterations; tune_sigma re-sorts the matrix, so call it before any Pcg."""
+        r = TuneReport()
+        _check(_m_autotune(self.h, int(fused), int(tune_sigma), iters, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
     # device-pointer operations (ints are device addresses, e.g. tensor.data_ptr())
     def spmv_f32(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
         _check(_spmv_f32(self.h, x_ptr, y_ptr, stream or None))
@@ -404,9 +423,11 @@ class Pcg:
     """Graph-captured Jacobi-PCG over a Matrix (one or many systems)."""
 
     def __init__(self, matrix: Matrix, tau: float = 1e-10, rel_tol: float = 0.0, max_iters: int = 0,
-                 graph_iters: int = 0, use_graph: bool = True, trace_cap: int = 0, stall_limit: int = 0):
+                 graph_iters: int = 0, use_graph: bool = True, trace_cap: int = 0, stall_limit: int = 0,
+                 fused: bool = False):
         self.matrix = matrix
-        self.opts = SolveOpts(tau, rel_tol, max_iters, graph_iters, int(use_graph), trace_cap, stall_limit)
+        self.opts = SolveOpts(tau, rel_tol, max_iters, graph_iters, int(use_graph), trace_cap, stall_limit,
+                              int(fused))
         h = _vp()
         _check(_p_create(matrix.h, C.byref(self.opts), C.byref(h)))
         self.h = h

@@ -193,10 +193,10 @@ def test_spmv_f32_rounding_bound(gpu, sigma):
 
 
 # ------------------------------------------------------------------ device PCG vs golden
-def check_against_golden(a, tau, trace_tol=1e-3, x_tol=1e-5, **kw):
+def check_against_golden(a, tau, trace_tol=1e-3, x_tol=1e-5, fused=False, **kw):
     g = O.golden(a, tau=tau)
     m = X.Matrix(a, keep_lo=True, **kw)
-    s = X.Pcg(m, tau=tau, trace_cap=g["iterations"] + 8)
+    s = X.Pcg(m, tau=tau, trace_cap=g["iterations"] + 8, fused=fused)
     s.set_rhs(a.b, a.x0)
     s.run()
     st = s.status()
@@ -216,9 +216,19 @@ def check_against_golden(a, tau, trace_tol=1e-3, x_tol=1e-5, **kw):
 
 
 @pytest.mark.parametrize("seed", [20261204, 1, 2])
-def test_pcg_c1_matches_golden(gpu, seed):
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_pcg_c1_matches_golden(gpu, seed, fused, variant):
     a = c1(seed)
-    check_against_golden(a, 1e-12 * float(a.b @ a.b))
+    check_against_golden(a, 1e-12 * float(a.b @ a.b), fused=fused, spmv_variant=variant)
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_pcg_c1_small_grids(gpu, fused):
+    """Work maps with one workgroup and with many: same bars."""
+    a = c1(5)
+    for grid in (1, 3, 16):
+        check_against_golden(a, 1e-12 * float(a.b @ a.b), fused=fused, grid=grid)
 
 
 def test_pcg_n512_known_answers(gpu):
@@ -247,23 +257,46 @@ def test_pcg_fem_fixed_iterations(gpu):
     K = 20
     g = O.golden(a, tau=1e-300, max_iters=K)
     m = X.Matrix(a, keep_lo=True)
-    s = X.Pcg(m, tau=1e-300, trace_cap=K)
-    s.set_rhs(a.b, a.x0)
-    s.run_fixed(K)
-    st = s.status()
-    assert st["iterations"] == K and st["status"] == X.MAX_ITER
-    x = s.x()
-    assert np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"]) <= 1e-5
-    np.testing.assert_allclose(s.trace()[:, 3], g["trace"][:, 4], rtol=1e-3)
+    for fused in (False, True):
+        s = X.Pcg(m, tau=1e-300, trace_cap=K, fused=fused)
+        s.set_rhs(a.b, a.x0)
+        s.run_fixed(K)
+        st = s.status()
+        assert st["iterations"] == K and st["status"] == X.MAX_ITER
+        x = s.x()
+        assert np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"]) <= 1e-5
+        np.testing.assert_allclose(s.trace()[:, 3], g["trace"][:, 4], rtol=1e-3)
 
 
-def test_pcg_graph_and_direct_launch_agree_bitwise(gpu):
+def test_autotune_keeps_results(gpu):
+    """The tuner only changes the layout (sigma, inner loop, workgroups): the
+    solve still meets the golden bars afterwards, and a second call hits the cache."""
+    a = c1(9)
+    m = X.Matrix(a, keep_lo=True)
+    rep = m.autotune(fused=True, tune_sigma=True, iters=16)
+    print("tune:", rep, flush=True)
+    assert rep["candidates"] > 4 and rep["us_per_iter"] > 0
+    rep2 = m.autotune(fused=True, tune_sigma=True, iters=16)
+    assert (rep2["sigma"], rep2["spmv_variant"], rep2["grid"]) == (rep["sigma"], rep["spmv_variant"], rep["grid"])
+    tau = 1e-12 * float(a.b @ a.b)
+    g = O.golden(a, tau=tau)
+    s = X.Pcg(m, tau=tau, fused=True)
+    s.set_rhs(a.b)
+    s.run()
+    assert abs(s.status()["iterations"] - g["iterations"]) <= 1
+    assert np.linalg.norm(s.x() - g["x"]) / np.linalg.norm(g["x"]) <= 1e-5
+    with pytest.raises(X.XplusError, match="before creating solvers"):
+        m.autotune(tune_sigma=True)
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_pcg_graph_and_direct_launch_agree_bitwise(gpu, fused):
     a = c1(7)
     tau = 1e-12 * float(a.b @ a.b)
     m = X.Matrix(a)
     outs = []
     for use_graph in (True, False, True):
-        s = X.Pcg(m, tau=tau, use_graph=use_graph, graph_iters=16)
+        s = X.Pcg(m, tau=tau, use_graph=use_graph, graph_iters=16, fused=fused)
         s.set_rhs(a.b)
         s.run()
         outs.append((s.status()["iterations"], s.x()))
