@@ -52,6 +52,31 @@ __device__ __forceinline__ int logical_block() {
     return (b % kNumXcd) * (g / kNumXcd) + b / kNumXcd;
 }
 
+// The work item of logical block lb.  A single system uses the uniform
+// split computed from the block index (no dependent load); batches read the
+// work map.
+__device__ __forceinline__ WorkItem work_item(const Launch& L, int lb) {
+    if (L.u_blocks > 0) {
+        WorkItem w;
+        w.pad_ = 0;
+        if (lb >= L.u_blocks) {
+            w.sys = -1;
+            w.slice_begin = w.slice_end = 0;
+            return w;
+        }
+        w.sys = 0;
+        w.slice_begin = static_cast<int>(static_cast<long long>(L.u_slices) * lb / L.u_blocks);
+        w.slice_end = static_cast<int>(static_cast<long long>(L.u_slices) * (lb + 1) / L.u_blocks);
+        return w;
+    }
+    return L.work[lb];
+}
+
+__device__ __forceinline__ int2 sys_range(const Launch& L, int sys) {
+    if (L.u_blocks > 0) return make_int2(0, L.u_blocks);
+    return L.sys_blocks[sys];
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -89,7 +114,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 // every workgroup computes them in the same order.
 template <int NV>
 __device__ __forceinline__ void sum_partials(const Launch& L, int sys, const int (&slots)[NV], double (&out)[NV]) {
-    const int2 range = L.sys_blocks[sys];
+    const int2 range = sys_range(L, sys);
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = 0.0;
     for (int b = range.x + static_cast<int>(threadIdx.x); b < range.y; b += kBlock) {
@@ -191,7 +216,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_init(Launch L, SellView A, co
                                                      float* __restrict__ r, float* __restrict__ z,
                                                      float* __restrict__ p) {
     const int lb = logical_block();
-    const WorkItem w = L.work[lb];
+    const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
     XP_WAVE_LANE
     double v[2] = {0.0, 0.0};
@@ -253,7 +278,7 @@ template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv(Launch L, SellView A, const SysState* S,
                                                      const float* __restrict__ p, float* __restrict__ ap) {
     const int lb = logical_block();
-    const WorkItem w = L.work[lb];
+    const WorkItem w = work_item(L, lb);
     if (w.sys < 0 || !S[w.sys].active) return;
     XP_WAVE_LANE
     double v[1] = {0.0};
@@ -283,10 +308,13 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
                                                            const float* __restrict__ minv,
                                                            float* __restrict__ z) {
     const int lb = logical_block();
-    const WorkItem w = L.work[lb];
-    if (w.sys < 0 || !S[w.sys].active) return;
-    const int r0 = w.slice_begin * kC, r1 = w.slice_end * kC;
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    // Everything the first pass needs is requested at once (state, vectors,
+    // partials); the activity test waits for all of it in one round trip.
+    const int active = S[w.sys].active;
     const double rz_old = S[w.sys].rz;
+    const int r0 = w.slice_begin * kC, r1 = w.slice_end * kC;
     double v[2] = {0.0, 0.0};
     bool first = true;
     double alpha = 0.0;
@@ -307,6 +335,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
             first = false;
             double t[1];
             sum_partials<1>(L, w.sys, {kSlotPap}, t);
+            if (!active) return;
             alpha = rz_old / t[0];
             if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;
         }
@@ -398,13 +427,10 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_p(Launch L, const SysS
                                                             const float* __restrict__ z, float* __restrict__ p) {
     __shared__ SysState s_sh;  // the scalar step runs on LDS: no registers held across it
     const int lb = logical_block();
-    const WorkItem w = L.work[lb];
+    const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
-    const bool leader_block = lb == L.sys_blocks[w.sys].x;
-    if (!Sc[w.sys].active) {
-        if (leader_block && threadIdx.x == 0) Sn[w.sys] = Sc[w.sys];  // carry the stopped state forward
-        return;
-    }
+    const bool leader_block = lb == sys_range(L, w.sys).x;
+    if (threadIdx.x == 0) s_sh = Sc[w.sys];  // in flight with the loads below
     const int r0 = w.slice_begin * kC, r1 = w.slice_end * kC;
     bool first = true;
     for (int base = r0; base < r1; base += kRowsPerThread * kBlock) {
@@ -420,11 +446,13 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_p(Launch L, const SysS
         if (first) {
             first = false;
             double t[3];
-            sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);
-            // One thread evaluates the scalar step; the others only need beta
-            // and whether the system is still running.
+            sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);  // ends with a barrier
+            if (!s_sh.active) {
+                if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;  // carry the stopped state forward
+                return;
+            }
+            __syncthreads();
             if (threadIdx.x == 0) {
-                s_sh = Sc[w.sys];
                 TraceRow* tr = nullptr;
                 if (leader_block && trace != nullptr && s_sh.iters < s_sh.trace_cap)
                     tr = trace + static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
@@ -457,35 +485,35 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
                                                               float* __restrict__ ap) {
     __shared__ SysState s_sh;
     const int lb = logical_block();
-    const WorkItem w = L.work[lb];
+    const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
-    const bool leader_block = lb == L.sys_blocks[w.sys].x;
-    if (!Sc[w.sys].active) {
-        if (leader_block && threadIdx.x == 0) Sn[w.sys] = Sc[w.sys];
+    const bool leader_block = lb == sys_range(L, w.sys).x;
+    // Prologue first (state and partials requested together), then a
+    // barrier-free streaming loop.
+    if (threadIdx.x == 0) s_sh = Sc[w.sys];
+    double t[3];
+    sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);  // ends with a barrier
+    if (!s_sh.active) {
+        if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
         return;
     }
-    const bool fresh = Sc[w.sys].fresh != 0;
-    double beta = 0.0;
-    if (!fresh) {
-        double t[3];
-        sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);
-        if (threadIdx.x == 0) {
-            s_sh = Sc[w.sys];
+    const bool fresh = s_sh.fresh != 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!fresh) {
             TraceRow* tr = nullptr;
             if (leader_block && trace != nullptr && s_sh.iters < s_sh.trace_cap)
                 tr = trace + static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
                      static_cast<size_t>(s_sh.iters);
             pcg_scalar_step(s_sh, t, tr);
-            if (leader_block) Sn[w.sys] = s_sh;
+        } else {
+            s_sh.fresh = 0;
         }
-        __syncthreads();
-        if (!s_sh.active) return;
-        beta = s_sh.beta;
-    } else if (leader_block && threadIdx.x == 0) {
-        SysState s0 = Sc[w.sys];
-        s0.fresh = 0;
-        Sn[w.sys] = s0;
+        if (leader_block) Sn[w.sys] = s_sh;
     }
+    __syncthreads();
+    if (!s_sh.active) return;
+    const double beta = s_sh.beta;
     XP_WAVE_LANE
     double v[1] = {0.0};
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
@@ -493,7 +521,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
         float zd;
         const float az = sell_row_f32<IDX16, V>(A, sl, lane, z, zd);
         float pn, an;
-        if (fresh) {
+        if (fresh) {  // right after init p = z already; ap = A z
             pn = zd;
             an = az;
         } else {
@@ -511,7 +539,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
 template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_spmv_f32(Launch L, SellView A, const float* __restrict__ x,
                                                      float* __restrict__ y) {
-    const WorkItem w = L.work[logical_block()];
+    const WorkItem w = work_item(L, logical_block());
     XP_WAVE_LANE
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         float xd;
@@ -522,7 +550,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_spmv_f32(Launch L, SellView A, co
 template <bool IDX16, int VMODE>
 __global__ __launch_bounds__(kBlock, 8) void k_spmv_f64(Launch L, SellView A, const double* __restrict__ x,
                                                      double* __restrict__ y) {
-    const WorkItem w = L.work[logical_block()];
+    const WorkItem w = work_item(L, logical_block());
     XP_WAVE_LANE
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves)
         y[sl * kC + lane] = sell_row_f64<IDX16, VMODE>(A, sl, lane, x);
@@ -534,7 +562,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_residual_f64(Launch L, SellView A
                                                          const double* __restrict__ b, double* __restrict__ r,
                                                          float* __restrict__ r32) {
     const int lb = logical_block();
-    const WorkItem w = L.work[lb];
+    const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
     XP_WAVE_LANE
     double v[1] = {0.0};

@@ -18,6 +18,8 @@ struct Launch {
     const WorkItem* work = nullptr;     // [grid]
     const int2* sys_blocks = nullptr;   // [nsys] logical block range per system
     double* partials = nullptr;         // [kNumSlots * grid]
+    int u_blocks = 0;                   // > 0: one system, uniform split of u_slices over u_blocks
+    int u_slices = 0;
     hipStream_t stream = nullptr;
 };
 

@@ -153,6 +153,7 @@ struct xp_matrix {
     DevBuf<int> d_col32;
     // launch machinery
     int grid = 0;
+    int u_blocks = 0;                 // single system: workgroups with work (uniform split)
     DevBuf<xp::WorkItem> d_work;
     DevBuf<int2> d_sys_blocks;
     DevBuf<double> d_partials;
@@ -179,6 +180,8 @@ struct xp_matrix {
         L.work = d_work.p;
         L.sys_blocks = d_sys_blocks.p;
         L.partials = d_partials.p;
+        L.u_blocks = u_blocks;
+        L.u_slices = nsys == 1 ? nslices : 0;
         L.stream = s;
         return L;
     }
@@ -218,6 +221,9 @@ void build_work_map(xp_matrix& m, int target) {
             work.push_back(xp::WorkItem{k, a, b, 0});
         }
     }
+    // A single system's split is recomputed in the kernels from the block
+    // index (same formula as above), saving a dependent load per kernel.
+    m.u_blocks = m.nsys == 1 ? g[0] : 0;
     const int grid = round_up(static_cast<int>(work.size()), xp::kNumXcd);
     while (static_cast<int>(work.size()) < grid) work.push_back(xp::WorkItem{-1, 0, 0, 0});
     m.grid = grid;
