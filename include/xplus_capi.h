@@ -85,6 +85,7 @@ typedef struct {
     int keep64;  /* keep exact FP64 values (drop-in FP64 SpMV) */
     int grid;    /* workgroups of the streaming kernels; 0 = library default */
     int spmv_variant; /* SpMV inner loop: 0 rolled loop, 1 fixed-width unrolled */
+    int block_rows;   /* 8: keep aligned 8-row groups together (block preconditioners); 0/1: rows */
 } xp_matrix_opts;
 
 typedef struct {
@@ -191,6 +192,8 @@ typedef struct {
     double rz, rr, pap, alpha, beta;
     double rr0;        /* r^T r after init */
     double tau;
+    int fallback;      /* 1: the block preconditioner stagnated or broke down and
+                          the solve was redone with plain Jacobi */
 } xp_status;
 
 int xp_pcg_create(xp_matrix* m, const xp_solve_opts* opts, xp_pcg** out);
@@ -216,8 +219,58 @@ int xp_pcg_get_x(xp_pcg* s, double* x);
 int xp_pcg_get_trace(xp_pcg* s, int sys, double* rows, int cap);
 /* FP64 true residual ||b - A x||_2 per system on the GPU. */
 int xp_pcg_true_residual(xp_pcg* s, double* out_norms);
+/* Preconditioner.  Jacobi (z = r / diag) by default.  A block preconditioner
+ * needs a matrix built with block_rows = 8; blocks are the aligned groups of
+ * 8 natural rows of each system (the last one may be partial), each an 8x8
+ * row-major FP32 matrix M_b applied as z_b = M_b r_b.
+ *   mode 0: Jacobi;  mode 1: exact block Jacobi (M_b = A_bb^-1, FP64 Cholesky
+ *   on the host);  mode 2: user blocks from `blocks` (nblocks x 64 floats,
+ *   natural block order).  With a block preconditioner and stall_limit > 0,
+ *   a stagnating or breaking-down solve is redone with Jacobi (fallback). */
+int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks);
+/* Number of 8-row blocks (natural order) of the matrix: sum ceil(n_k / 8). */
+int xp_matrix_num_blocks(const xp_matrix* m);
 /* Device time of the last xp_pcg_run* (ms, HIP events) and launches made. */
 int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches);
+
+/* ===================================================================== *
+ * Learned 8x8 block-Jacobi preconditioner (new: the reference has none).
+ *
+ * Per aligned 8-row block b of a system: s_b = mean diagonal, Ahat = A_bb/s_b,
+ * features (48) = upper triangle of Ahat (36) + off-block |a| row sums / s_b
+ * (8) + 4 zeros; MLP 48 -> 64 -> 64 -> 48 (ReLU, ReLU, linear), outputs 0..35
+ * = lower triangle of L; block M_b = (L L^T + 1e-3 I) / s_b (SPD by
+ * construction).  Forward and input-gradient GEMMs are BF16 MFMA with FP32
+ * accumulation; weight gradients FP32; Adam.  Training loss (block SPAI):
+ * mean over blocks of ||Mh Ahat - I||_F^2 + ||Mh Ahat_off||_F^2.
+ * Parameters are one flat FP32 vector: W1[48][64], b1[64], W2[64][64],
+ * b2[64], W3[64][48], b3[48] (row-major, input index first).
+ * ===================================================================== */
+typedef struct xp_mlp xp_mlp;
+typedef struct xp_blockset xp_blockset;
+int xp_mlp_create(unsigned long long seed, xp_mlp** out);
+void xp_mlp_destroy(xp_mlp* net);
+int xp_mlp_num_params(void);
+int xp_mlp_get_params(xp_mlp* net, float* out);
+int xp_mlp_set_params(xp_mlp* net, const float* params);
+/* Blocks of a host CSR system (no GPU needed); append concatenates sets. */
+int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, xp_blockset** out);
+int xp_blockset_append(xp_blockset* dst, const xp_blockset* src);
+int xp_blockset_size(const xp_blockset* set);
+/* feat: size x 48, ahat: size x 64, hmat (Ahat^2 + G): size x 64, scale: size. */
+int xp_blockset_copy(const xp_blockset* set, float* feat, float* ahat, float* hmat, float* scale);
+void xp_blockset_free(xp_blockset* set);
+/* Mean loss over samples [first, first+count) and its gradient (flat). */
+int xp_mlp_loss_grad(xp_mlp* net, const xp_blockset* set, int first, int count, double* loss, float* grad);
+/* Network outputs (count x 48) and blocks M_b (size x 64). */
+int xp_mlp_forward(xp_mlp* net, const xp_blockset* set, int first, int count, float* out);
+int xp_mlp_blocks(xp_mlp* net, const xp_blockset* set, float* blocks);
+/* `steps` Adam steps on contiguous batches of `batch` samples; loss per step. */
+int xp_mlp_train(xp_mlp* net, const xp_blockset* set, int steps, int batch, float lr, unsigned long long seed,
+                 double* loss_history);
+/* Builds the blocks of the solver's matrix with the network and installs
+ * them as its preconditioner (xp_pcg_set_preconditioner mode 2). */
+int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net);
 
 /* ===================================================================== *
  * Host helpers: loader and generators (no GPU needed).

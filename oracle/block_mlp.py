@@ -1,0 +1,166 @@
+"""CPU oracle (FP64 numpy) of the learned 8x8 block-Jacobi preconditioner.
+
+TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's checks, never by
+the product).  The reference has no learned preconditioner; this module is
+the independent CPU restatement its GPU kernels are checked against:
+
+  features_from_csr  -- block features, Ahat, H = Ahat^2 + G, scale
+  forward            -- the MLP 48 -> 64 -> 64 -> 48 (ReLU, ReLU, linear)
+  loss_grad          -- mean block-SPAI loss and its analytic gradient
+  blocks_from_out    -- M_b = (L L^T + 1e-3 I) / s_b
+  block_pcg          -- FP64 PCG with a block-diagonal preconditioner
+
+Pinning: loss_grad against central finite differences of loss (FP64,
+tests/test_block_oracle.py); block_pcg against the golden Jacobi-PCG when
+the blocks are diagonal, and against finite termination with exact block
+inverses on block-diagonal matrices; features_from_csr against dense
+closed forms on hand-built blocks.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+N_IN, N_H, N_OUT = 48, 64, 48
+SHAPES = [("W1", (N_IN, N_H)), ("b1", (N_H,)), ("W2", (N_H, N_H)), ("b2", (N_H,)),
+          ("W3", (N_H, N_OUT)), ("b3", (N_OUT,))]
+N_PARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
+EPS_M = 1e-3
+TRIL = [(r, c) for r in range(8) for c in range(r + 1)]
+
+
+def unpack(flat):
+    out, o = {}, 0
+    for name, shp in SHAPES:
+        k = int(np.prod(shp))
+        out[name] = np.asarray(flat[o:o + k], dtype=np.float64).reshape(shp)
+        o += k
+    return out
+
+
+def pack(d):
+    return np.concatenate([np.asarray(d[name], dtype=np.float64).ravel() for name, _ in SHAPES])
+
+
+def features_from_csr(row_ptr, col_idx, values):
+    n = len(row_ptr) - 1
+    nb = (n + 7) // 8
+    feat = np.zeros((nb, N_IN))
+    ahat = np.zeros((nb, 8, 8))
+    hmat = np.zeros((nb, 8, 8))
+    scale = np.ones(nb)
+    for b in range(nb):
+        r0, m = 8 * b, min(8, n - 8 * b)
+        a = np.zeros((8, 8))
+        offabs = np.zeros(8)
+        off = [dict() for _ in range(8)]
+        for i in range(m):
+            for o in range(row_ptr[r0 + i], row_ptr[r0 + i + 1]):
+                c = col_idx[o] - r0
+                if 0 <= c < m:
+                    a[i, c] += values[o]
+                else:
+                    offabs[i] += abs(values[o])
+                    off[i][col_idx[o]] = off[i].get(col_idx[o], 0.0) + values[o]
+        sc = np.trace(a[:m, :m]) / m
+        if not (sc > 0 and np.isfinite(sc)):
+            sc = 1.0
+        for i in range(m, 8):
+            a[i, i] = sc
+        scale[b] = sc
+        g = np.zeros((8, 8))
+        for i in range(8):
+            for j in range(8):
+                g[i, j] = sum(v * off[j].get(k, 0.0) for k, v in off[i].items())
+        ah = a / sc
+        ahat[b] = ah
+        hmat[b] = ah @ ah + g / sc ** 2
+        feat[b, :36] = ah[np.triu_indices(8)]
+        feat[b, 36:44] = offabs / sc
+    return feat, ahat, hmat, scale
+
+
+def forward(params, feat, keep=False):
+    p = unpack(params)
+    z1 = feat @ p["W1"] + p["b1"]
+    h1 = np.maximum(z1, 0.0)
+    z2 = h1 @ p["W2"] + p["b2"]
+    h2 = np.maximum(z2, 0.0)
+    out = h2 @ p["W3"] + p["b3"]
+    if keep:
+        return out, (feat, z1, h1, z2, h2)
+    return out
+
+
+def l_from_out(out):
+    n = out.shape[0]
+    L = np.zeros((n, 8, 8))
+    for t, (r, c) in enumerate(TRIL):
+        L[:, r, c] = out[:, t]
+    return L
+
+
+def loss_grad(params, feat, ahat, hmat):
+    """Mean over samples of tr(M H M^T) - 2 tr(M A) + 8, M = L L^T + eps I."""
+    out, (x, z1, h1, z2, h2) = forward(params, feat, keep=True)
+    n = out.shape[0]
+    L = l_from_out(out)
+    M = L @ np.transpose(L, (0, 2, 1)) + EPS_M * np.eye(8)
+    MH = M @ hmat
+    loss = np.einsum("nij,nij->n", MH, M) - 2.0 * np.einsum("nij,nji->n", M, ahat) + 8.0
+    D = 2.0 * (MH - ahat) / n
+    dL = (D + np.transpose(D, (0, 2, 1))) @ L
+    dout = np.zeros_like(out)
+    for t, (r, c) in enumerate(TRIL):
+        dout[:, t] = dL[:, r, c]
+    p = unpack(params)
+    g = {}
+    g["W3"] = h2.T @ dout
+    g["b3"] = dout.sum(0)
+    dh2 = (dout @ p["W3"].T) * (z2 > 0)
+    g["W2"] = h1.T @ dh2
+    g["b2"] = dh2.sum(0)
+    dh1 = (dh2 @ p["W2"].T) * (z1 > 0)
+    g["W1"] = x.T @ dh1
+    g["b1"] = dh1.sum(0)
+    return float(loss.mean()), pack(g)
+
+
+def blocks_from_out(out, scale):
+    L = l_from_out(out)
+    M = L @ np.transpose(L, (0, 2, 1)) + EPS_M * np.eye(8)
+    return M / scale[:, None, None]
+
+
+def block_pcg(A, b, blocks, tau, max_iters=100000):
+    """FP64 PCG with z_b = M_b r_b on aligned 8-row blocks (blocks: nb x 8 x 8).
+    Returns (x, iterations, rr trace)."""
+    n = A.shape[0]
+    nb = (n + 7) // 8
+
+    def apply(r):
+        rp = np.zeros(nb * 8)
+        rp[:n] = r
+        z = np.einsum("bij,bj->bi", blocks, rp.reshape(nb, 8)).ravel()
+        return z[:n]
+
+    x = np.zeros(n)
+    r = b.copy()
+    z = apply(r)
+    p = z.copy()
+    rz, rr = r @ z, r @ r
+    trace = []
+    it = 0
+    while it < max_iters and rr > tau:
+        ap = A @ p
+        alpha = rz / (p @ ap)
+        x += alpha * p
+        r -= alpha * ap
+        z = apply(r)
+        rzn, rr = r @ z, r @ r
+        trace.append(rr)
+        it += 1
+        if rr <= tau:
+            break
+        p = z + (rzn / rz) * p
+        rz = rzn
+    return x, it, np.array(trace)

@@ -201,6 +201,25 @@ __device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int
     return acc;
 }
 
+// z = M r for the 8x8 block of this lane's row: the 8 rows of a block are 8
+// consecutive lanes (blocks start at multiples of 8 in solve order), so the
+// block's r values are exchanged with cross-lane shuffles.  M is stored per
+// block as 8 rows of 8 floats (row-major), i.e. row `row` at M + 8 * row.
+// Every lane of the wavefront must call it.
+__device__ __forceinline__ float block_apply(const float* __restrict__ M, int row, float r) {
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const float4* mr = reinterpret_cast<const float4*>(M + static_cast<size_t>(row) * 8);
+    const float4 m0 = mr[0], m1 = mr[1];
+    const float mv[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float rj = __shfl(r, (lane & ~7) | j, 64);
+        acc += static_cast<double>(mv[j]) * static_cast<double>(rj);
+    }
+    return static_cast<float>(acc);
+}
+
 #define XP_WAVE_LANE                                                  \
     const int wave = static_cast<int>(threadIdx.x) >> 6;              \
     const int lane = static_cast<int>(threadIdx.x) & 63;
@@ -213,6 +232,7 @@ template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_init(Launch L, SellView A, const float* __restrict__ x,
                                                      const float* __restrict__ b,
                                                      const float* __restrict__ minv,
+                                                     const float* __restrict__ blockM,
                                                      float* __restrict__ r, float* __restrict__ z,
                                                      float* __restrict__ p) {
     const int lb = logical_block();
@@ -225,7 +245,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_init(Launch L, SellView A, co
         float xd;
         const float ax = sell_row_f32<IDX16, V>(A, sl, lane, x, xd);
         const float rv = b[row] - ax;
-        const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
+        const float zv = blockM != nullptr ? block_apply(blockM, row, rv)
+                                           : static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
         r[row] = rv;
         z[row] = zv;
         p[row] = zv;
@@ -306,6 +327,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
                                                            float* __restrict__ r,
                                                            const float* __restrict__ ap,
                                                            const float* __restrict__ minv,
+                                                           const float* __restrict__ blockM,
                                                            float* __restrict__ z) {
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
@@ -328,7 +350,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
                 pv[k] = p[row];
                 rv[k] = r[row];
                 av[k] = ap[row];
-                mv[k] = minv[row];
+                mv[k] = blockM == nullptr ? minv[row] : 0.0f;
             }
         }
         if (first) {
@@ -342,10 +364,12 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
 #pragma unroll
         for (int k = 0; k < kRowsPerThread; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
-            if (row < r1) {
+            if (row < r1) {  // uniform per wavefront: r1 - base is a multiple of 64
                 const float xn = static_cast<float>(static_cast<double>(xv[k]) + alpha * static_cast<double>(pv[k]));
                 const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * static_cast<double>(av[k]));
-                const float zn = static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
+                const float zn = blockM != nullptr
+                                     ? block_apply(blockM, row, rn)
+                                     : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
                 x[row] = xn;
                 r[row] = rn;
                 z[row] = zn;
@@ -732,8 +756,8 @@ int blocks_for(size_t n, int bs) { return static_cast<int>((n + static_cast<size
     } while (0)
 
 void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
-                     const float* b, const float* minv, float* r, float* z, float* p) {
-    XP_DISPATCH_SELL(A, k_pcg_init, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, r, z, p);
+                     const float* b, const float* minv, const float* blockM, float* r, float* z, float* p) {
+    XP_DISPATCH_SELL(A, k_pcg_init, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, blockM, r, z, p);
     hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
 }
 
@@ -748,8 +772,8 @@ void launch_pcg_spmv_fused(const Launch& L, const SellView& A, const SysState* s
 }
 
 void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
-                           const float* ap, const float* minv, float* z) {
-    hipLaunchKernelGGL(k_pcg_update_xrz, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, x, p, r, ap, minv, z);
+                           const float* ap, const float* minv, const float* blockM, float* z) {
+    hipLaunchKernelGGL(k_pcg_update_xrz, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, x, p, r, ap, minv, blockM, z);
 }
 
 void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,

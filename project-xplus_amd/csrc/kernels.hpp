@@ -26,8 +26,9 @@ struct Launch {
 // ---- FP32 solve-order PCG step kernels (state machine in SysState) ----
 // init: r = b - A x; z = minv r; p = z; then the state after init in st[sys]
 // (which must hold the configuration: tau, max_iters, trace_cap, stall_limit).
+// blockM: null for Jacobi (z = minv r), else 8x8 blocks (z = M_b r_b).
 void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
-                     const float* b, const float* minv, float* r, float* z, float* p);
+                     const float* b, const float* minv, const float* blockM, float* r, float* z, float* p);
 // ap = A p and the pAp partials.
 void launch_pcg_spmv(const Launch& L, const SellView& A, const SysState* st, const float* p, float* ap);
 // Fused iteration, first kernel: scalar tail of the previous iteration,
@@ -36,7 +37,7 @@ void launch_pcg_spmv_fused(const Launch& L, const SellView& A, const SysState* s
                            TraceRow* trace, int trace_stride, const float* z, float* p, float* ap);
 // alpha; x += alpha p; r -= alpha ap; z = minv r; rz, rr partials.
 void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
-                           const float* ap, const float* minv, float* z);
+                           const float* ap, const float* minv, const float* blockM, float* z);
 // scalar tail (beta, tests, trace) and p = z + beta p; reads st_cur, writes st_next.
 void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
                          int trace_stride, const float* z, float* p);

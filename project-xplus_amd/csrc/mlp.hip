@@ -1,0 +1,569 @@
+// mlp.hip -- the learned 8x8 block-Jacobi preconditioner: features, a small
+// MLP (48 -> 64 -> 64 -> 48, ReLU) with BF16 MFMA forward and input-gradient
+// GEMMs (FP32 accumulation), FP32 weight gradients, Adam, and the assembly of
+// SPD blocks M_b = (L L^T + 1e-3 I) / s_b for the PCG.
+//
+// Per 8-row block b (aligned natural rows of one system):
+//   s_b   = mean of the block's diagonal entries,
+//   Ahat  = A_bb / s_b (8x8, identity on the rows of a partial last block),
+//   o_i   = sum_{j outside the block} |a_ij| / s_b,
+//   G     = Ahat_bo Ahat_bo^T (8x8, couplings leaving the block),
+//   features (48) = upper triangle of Ahat row by row (36), o (8), 4 zeros.
+// Training objective (block SPAI, closed form per block, averaged):
+//   loss_b = || Mh Ahat - I ||_F^2 + || Mh Ahat_bo ||_F^2
+//          = tr(Mh H Mh^T) - 2 tr(Mh Ahat) + 8,   H = Ahat^2 + G,
+// with Mh = L L^T + eps I, L lower triangular from outputs 0..35 (row-major
+// lower triangle).  dloss/dMh = 2 (Mh H - Ahat); dloss/dL = (dMh + dMh^T) L.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mlp.hpp"
+
+namespace xp {
+
+namespace {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<uint16_t>(u >> 16);  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+// Y[rows x N] = act(X[rows x K] W + bias) for one 16-row tile per wavefront.
+// X is BF16 row-major with leading dimension K; Wt is BF16 "n-major" (N rows
+// of K), so an MFMA B fragment (4 consecutive k at one n) is one 8-byte load.
+// v_mfma_f32_16x16x16_bf16 lane layout: A: row l%16, k 4(l/16)..+3;
+// B: col l%16, k 4(l/16)..+3; D: col l%16, rows 4(l/16)..+3.
+// MODE 0: ReLU (Yf = post-activation FP32, Yb = BF16 copy);
+// MODE 1: identity (Yf only);  MODE 2: masked by ref > 0 (backward through
+// a ReLU: Yf = acc * [ref > 0], Yb = BF16 copy), no bias.
+template <int K, int N, int MODE>
+__global__ __launch_bounds__(256) void k_mlp_gemm(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
+                                                  const float* __restrict__ bias, const float* __restrict__ ref,
+                                                  float* __restrict__ Yf, uint16_t* __restrict__ Yb, int rows) {
+    static_assert(K % 16 == 0 && N % 16 == 0, "tile sizes");
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int row0 = (static_cast<int>(blockIdx.x) * 4 + wave) * 16;
+    if (row0 >= rows) return;  // rows is a multiple of 16: whole tiles only
+    const int li = lane & 15, lk = (lane >> 4) * 4;
+    short4_t a[K / 16];
+#pragma unroll
+    for (int kk = 0; kk < K / 16; ++kk)
+        a[kk] = *reinterpret_cast<const short4_t*>(X + static_cast<size_t>(row0 + li) * K + kk * 16 + lk);
+#pragma unroll
+    for (int nt = 0; nt < N / 16; ++nt) {
+        float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kk = 0; kk < K / 16; ++kk) {
+            const short4_t b = *reinterpret_cast<const short4_t*>(Wt + static_cast<size_t>(nt * 16 + li) * K + kk * 16 + lk);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kk], b, acc, 0, 0, 0);
+        }
+        const int col = nt * 16 + li;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = row0 + lk + q;
+            float y = acc[q];
+            if (MODE != 2) y += bias[col];
+            if (MODE == 0) y = fmaxf(y, 0.0f);
+            if (MODE == 2) y = ref[static_cast<size_t>(row) * N + col] > 0.0f ? y : 0.0f;
+            Yf[static_cast<size_t>(row) * N + col] = y;
+            if (MODE != 1) Yb[static_cast<size_t>(row) * N + col] = f2bf(y);
+        }
+    }
+}
+
+// Loss and dloss/dout per sample (one thread per sample).  out: [rows x 48]
+// network outputs; ahat, hmat: [rows x 64]; dout: [rows x 48] (BF16 copy in
+// dout_b for the MFMA backward).  loss_part[block] = sum of this block's losses.
+__global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ out, const float* __restrict__ ahat,
+                                                  const float* __restrict__ hmat, int count, float inv_count,
+                                                  float* __restrict__ dout, uint16_t* __restrict__ dout_b,
+                                                  double* __restrict__ loss_part) {
+    __shared__ double red[256];
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    double loss = 0.0;
+    if (i < count) {
+        const float* o = out + static_cast<size_t>(i) * 48;
+        const float* A = ahat + static_cast<size_t>(i) * 64;
+        const float* H = hmat + static_cast<size_t>(i) * 64;
+        float L[8][8];
+        int t = 0;
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c < 8; ++c) L[r][c] = c <= r ? o[t++] : 0.0f;
+        float M[8][8];
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c < 8; ++c) {
+                float v = 0.0f;
+                for (int k = 0; k <= min(r, c); ++k) v += L[r][k] * L[c][k];
+                M[r][c] = v + (r == c ? 1.0e-3f : 0.0f);
+            }
+        // dM = 2 (M H - A) (times 1/count for the mean); loss pieces
+        float D[8][8];
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c < 8; ++c) {
+                float mh = 0.0f;
+                for (int k = 0; k < 8; ++k) mh += M[r][k] * H[k * 8 + c];
+                loss += static_cast<double>(mh) * M[r][c] - 2.0 * static_cast<double>(M[r][c]) * A[c * 8 + r];
+                D[r][c] = 2.0f * (mh - A[r * 8 + c]) * inv_count;
+            }
+        loss += 8.0;
+        // dL = (D + D^T) L, lower triangle
+        float* g = dout + static_cast<size_t>(i) * 48;
+        uint16_t* gb = dout_b + static_cast<size_t>(i) * 48;
+        t = 0;
+        for (int r = 0; r < 8; ++r)
+            for (int c = 0; c <= r; ++c) {
+                float v = 0.0f;
+                for (int k = c; k < 8; ++k) v += (D[r][k] + D[k][r]) * L[k][c];
+                g[t] = v;
+                gb[t] = f2bf(v);
+                ++t;
+            }
+        for (; t < 48; ++t) {
+            g[t] = 0.0f;
+            gb[t] = 0;
+        }
+    }
+    red[threadIdx.x] = loss;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// Partial weight gradients over a chunk of kDwRows samples:
+// dW[k][n] = sum X[b][k] dY[b][n] (FP32), db[n] = sum dY[b][n].
+// part: [chunks x (K*N + N)].
+constexpr int kDwRows = 64;
+template <int K, int N>
+__global__ __launch_bounds__(256) void k_mlp_dw(const float* __restrict__ X, const float* __restrict__ dY, int rows,
+                                                float* __restrict__ part) {
+    __shared__ float xs[kDwRows][K];
+    __shared__ float ys[kDwRows][N];
+    const int b0 = static_cast<int>(blockIdx.x) * kDwRows;
+    const int nb = min(kDwRows, rows - b0);
+    for (int e = static_cast<int>(threadIdx.x); e < kDwRows * K; e += 256) {
+        const int r = e / K, c = e % K;
+        xs[r][c] = r < nb ? X[static_cast<size_t>(b0 + r) * K + c] : 0.0f;
+    }
+    for (int e = static_cast<int>(threadIdx.x); e < kDwRows * N; e += 256) {
+        const int r = e / N, c = e % N;
+        ys[r][c] = r < nb ? dY[static_cast<size_t>(b0 + r) * N + c] : 0.0f;
+    }
+    __syncthreads();
+    float* out = part + static_cast<size_t>(blockIdx.x) * (K * N + N);
+    for (int e = static_cast<int>(threadIdx.x); e < K * N + N; e += 256) {
+        float acc = 0.0f;
+        if (e < K * N) {
+            const int k = e / N, n = e % N;
+            for (int r = 0; r < kDwRows; ++r) acc += xs[r][k] * ys[r][n];
+        } else {
+            const int n = e - K * N;
+            for (int r = 0; r < kDwRows; ++r) acc += ys[r][n];
+        }
+        out[e] = acc;
+    }
+}
+
+// grad[e] = sum over chunks of part[chunk][e], in chunk order.
+__global__ void k_mlp_reduce(const float* __restrict__ part, int chunks, int len, float* __restrict__ grad) {
+    const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (e >= len) return;
+    float acc = 0.0f;
+    for (int c = 0; c < chunks; ++c) acc += part[static_cast<size_t>(c) * len + e];
+    grad[e] = acc;
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 copies of a weight matrix W[K][N] (in-major FP32): Wt[n][k] and W[k][n].
+__global__ void k_cast_w(const float* __restrict__ w, int K, int N, uint16_t* __restrict__ wt,
+                         uint16_t* __restrict__ wk) {
+    const int e = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (e >= K * N) return;
+    const int k = e / N, n = e % N;
+    const uint16_t h = f2bf(w[e]);
+    wt[static_cast<size_t>(n) * K + k] = h;
+    wk[e] = h;
+}
+
+__global__ void k_cast_rows(const float* __restrict__ src, size_t n, uint16_t* __restrict__ dst) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) dst[i] = f2bf(src[i]);
+}
+
+// M_b = (L L^T + eps I) / s_b for every sample: blocks [count x 64].
+__global__ void k_mlp_blocks(const float* __restrict__ out, const float* __restrict__ scale, int count,
+                             float* __restrict__ blocks) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= count) return;
+    const float* o = out + static_cast<size_t>(i) * 48;
+    float L[8][8];
+    int t = 0;
+    for (int r = 0; r < 8; ++r)
+        for (int c = 0; c < 8; ++c) L[r][c] = c <= r ? o[t++] : 0.0f;
+    const float inv_s = 1.0f / scale[i];
+    float* mb = blocks + static_cast<size_t>(i) * 64;
+    for (int r = 0; r < 8; ++r)
+        for (int c = 0; c < 8; ++c) {
+            float v = 0.0f;
+            for (int k = 0; k <= min(r, c); ++k) v += L[r][k] * L[c][k];
+            mb[r * 8 + c] = (v + (r == c ? 1.0e-3f : 0.0f)) * inv_s;
+        }
+}
+
+int nblk(size_t n, int b) { return static_cast<int>((n + static_cast<size_t>(b) - 1) / static_cast<size_t>(b)); }
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct Buf {
+    T* p = nullptr;
+    size_t n = 0;
+    void ensure(size_t c) {
+        if (c <= n) return;
+        if (p != nullptr) (void)hipFree(p);
+        p = nullptr;
+        check(hipMalloc(reinterpret_cast<void**>(&p), c * sizeof(T)), "hipMalloc");
+        n = c;
+    }
+    ~Buf() {
+        if (p != nullptr) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+// ---- host side ---------------------------------------------------------------------
+
+BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const double* values) {
+    BlockSet s;
+    const int nb = (n + 7) / 8;
+    s.count = nb;
+    s.feat.assign(static_cast<size_t>(nb) * kMlpIn, 0.0f);
+    s.ahat.assign(static_cast<size_t>(nb) * 64, 0.0f);
+    s.hmat.assign(static_cast<size_t>(nb) * 64, 0.0f);
+    s.scale.assign(static_cast<size_t>(nb), 1.0f);
+    for (int b = 0; b < nb; ++b) {
+        const int r0 = 8 * b, m = std::min(8, n - r0);
+        double a[8][8] = {};
+        double off_abs[8] = {};
+        // off-block entries per row (column, value) for G
+        std::vector<std::pair<int, double>> off[8];
+        for (int i = 0; i < m; ++i)
+            for (int o = row_ptr[r0 + i]; o < row_ptr[r0 + i + 1]; ++o) {
+                const int c = col_idx[o] - r0;
+                if (c >= 0 && c < m)
+                    a[i][c] += values[o];
+                else {
+                    off_abs[i] += std::fabs(values[o]);
+                    off[i].push_back({col_idx[o], values[o]});
+                }
+            }
+        double sdiag = 0.0;
+        for (int i = 0; i < m; ++i) sdiag += a[i][i];
+        double sc = sdiag / m;
+        if (!(sc > 0.0) || !std::isfinite(sc)) sc = 1.0;
+        for (int i = m; i < 8; ++i) a[i][i] = sc;  // partial block: identity after scaling
+        s.scale[static_cast<size_t>(b)] = static_cast<float>(sc);
+        float* f = s.feat.data() + static_cast<size_t>(b) * kMlpIn;
+        float* ah = s.ahat.data() + static_cast<size_t>(b) * 64;
+        float* hm = s.hmat.data() + static_cast<size_t>(b) * 64;
+        int t = 0;
+        for (int i = 0; i < 8; ++i)
+            for (int j = i; j < 8; ++j) f[t++] = static_cast<float>(a[i][j] / sc);
+        for (int i = 0; i < 8; ++i) f[t++] = static_cast<float>(off_abs[i] / sc);
+        for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j) ah[i * 8 + j] = static_cast<float>(a[i][j] / sc);
+        // H = Ahat^2 + G, G_ij = sum_k ahat_ik ahat_jk over off-block columns k
+        for (int i = 0; i < 8; ++i)
+            for (int j = 0; j < 8; ++j) {
+                double v = 0.0;
+                for (int k = 0; k < 8; ++k) v += a[i][k] * a[k][j];
+                v /= sc * sc;
+                for (const auto& e1 : off[i])
+                    for (const auto& e2 : off[j])
+                        if (e1.first == e2.first) v += e1.second * e2.second / (sc * sc);
+                hm[i * 8 + j] = static_cast<float>(v);
+            }
+    }
+    return s;
+}
+
+void BlockSet::append(const BlockSet& o) {
+    count += o.count;
+    feat.insert(feat.end(), o.feat.begin(), o.feat.end());
+    ahat.insert(ahat.end(), o.ahat.begin(), o.ahat.end());
+    hmat.insert(hmat.end(), o.hmat.begin(), o.hmat.end());
+    scale.insert(scale.end(), o.scale.begin(), o.scale.end());
+}
+
+struct Mlp::Dev {
+    Buf<float> w, g, m, v;            // master weights, grads, Adam moments (flat)
+    Buf<uint16_t> w1t, w1k, w2t, w2k, w3t, w3k;
+    Buf<float> xf, h1, h2, out, dout, dh2, dh1, ahat, hmat, scale, part, blocks, dummy;
+    Buf<uint16_t> xb, h1b, h2b, doutb, dh2b, dh1b;
+    Buf<double> loss_part;
+    hipStream_t s = nullptr;
+};
+
+namespace {
+constexpr int kW1 = kMlpIn * kMlpHidden, kB1 = kMlpHidden;
+constexpr int kW2 = kMlpHidden * kMlpHidden, kB2 = kMlpHidden;
+constexpr int kW3 = kMlpHidden * kMlpOut, kB3 = kMlpOut;
+constexpr int kOffW1 = 0, kOffB1 = kW1, kOffW2 = kOffB1 + kB1, kOffB2 = kOffW2 + kW2, kOffW3 = kOffB2 + kB2,
+              kOffB3 = kOffW3 + kW3;
+}  // namespace
+
+int Mlp::num_params() { return kOffB3 + kB3; }
+
+Mlp::Mlp(uint64_t seed) : d_(new Dev) {
+    check(hipStreamCreateWithFlags(&d_->s, hipStreamNonBlocking), "hipStreamCreate");
+    params_.assign(static_cast<size_t>(num_params()), 0.0f);
+    // He-normal initialisation from a SplitMix64 stream (deterministic).
+    uint64_t st = seed ^ 0x6A09E667F3BCC909ull;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    auto normal = [&]() {
+        double u1 = static_cast<double>(next() >> 11) * 0x1.0p-53, u2 = static_cast<double>(next() >> 11) * 0x1.0p-53;
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int i = 0; i < kW1; ++i) params_[static_cast<size_t>(kOffW1 + i)] = static_cast<float>(normal() * std::sqrt(2.0 / 44.0));
+    for (int i = 0; i < kW2; ++i) params_[static_cast<size_t>(kOffW2 + i)] = static_cast<float>(normal() * std::sqrt(2.0 / kMlpHidden));
+    for (int i = 0; i < kW3; ++i) params_[static_cast<size_t>(kOffW3 + i)] = static_cast<float>(normal() * std::sqrt(1.0 / kMlpHidden) * 0.1);
+    // zero the unused input rows (features 44..47) and output columns 36..47
+    for (int k = 44; k < kMlpIn; ++k)
+        for (int n = 0; n < kMlpHidden; ++n) params_[static_cast<size_t>(kOffW1 + k * kMlpHidden + n)] = 0.0f;
+    for (int k = 0; k < kMlpHidden; ++k)
+        for (int n = 36; n < kMlpOut; ++n) params_[static_cast<size_t>(kOffW3 + k * kMlpOut + n)] = 0.0f;
+    // bias of the diagonal of L: start near the identity (Mh ~ I)
+    int t = 0;
+    for (int r = 0; r < 8; ++r)
+        for (int c = 0; c <= r; ++c, ++t)
+            if (r == c) params_[static_cast<size_t>(kOffB3 + t)] = 1.0f;
+    upload_params();
+    const size_t np = static_cast<size_t>(num_params());
+    d_->m.ensure(np);
+    d_->v.ensure(np);
+    check(hipMemsetAsync(d_->m.p, 0, np * sizeof(float), d_->s), "memset");
+    check(hipMemsetAsync(d_->v.p, 0, np * sizeof(float), d_->s), "memset");
+    check(hipStreamSynchronize(d_->s), "sync");
+}
+
+Mlp::~Mlp() {
+    if (d_ && d_->s != nullptr) (void)hipStreamDestroy(d_->s);
+}
+
+void Mlp::set_params(const float* p) {
+    params_.assign(p, p + num_params());
+    upload_params();
+}
+
+void Mlp::upload_params() {
+    Dev& d = *d_;
+    const size_t np = static_cast<size_t>(num_params());
+    d.w.ensure(np);
+    d.g.ensure(np);
+    check(hipMemcpyAsync(d.w.p, params_.data(), np * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
+    cast_weights();
+    check(hipStreamSynchronize(d.s), "sync");
+}
+
+void Mlp::cast_weights() {
+    Dev& d = *d_;
+    d.w1t.ensure(kW1); d.w1k.ensure(kW1);
+    d.w2t.ensure(kW2); d.w2k.ensure(kW2);
+    d.w3t.ensure(kW3); d.w3k.ensure(kW3);
+    hipLaunchKernelGGL(k_cast_w, dim3(nblk(kW1, 256)), dim3(256), 0, d.s, d.w.p + kOffW1, kMlpIn, kMlpHidden, d.w1t.p, d.w1k.p);
+    hipLaunchKernelGGL(k_cast_w, dim3(nblk(kW2, 256)), dim3(256), 0, d.s, d.w.p + kOffW2, kMlpHidden, kMlpHidden, d.w2t.p, d.w2k.p);
+    hipLaunchKernelGGL(k_cast_w, dim3(nblk(kW3, 256)), dim3(256), 0, d.s, d.w.p + kOffW3, kMlpHidden, kMlpOut, d.w3t.p, d.w3k.p);
+}
+
+const std::vector<float>& Mlp::params() {
+    check(hipMemcpyAsync(params_.data(), d_->w.p, params_.size() * sizeof(float), hipMemcpyDeviceToHost, d_->s), "download");
+    check(hipStreamSynchronize(d_->s), "sync");
+    return params_;
+}
+
+// Uploads samples [first, first + count) of data (rows padded to a multiple of 64).
+int Mlp::stage(const BlockSet& data, int first, int count) {
+    Dev& d = *d_;
+    if (first < 0 || count <= 0 || first + count > data.count) throw std::runtime_error("sample range out of bounds");
+    const int rows = (count + 63) / 64 * 64;
+    d.xf.ensure(static_cast<size_t>(rows) * kMlpIn);
+    d.xb.ensure(static_cast<size_t>(rows) * kMlpIn);
+    d.h1.ensure(static_cast<size_t>(rows) * kMlpHidden); d.h1b.ensure(static_cast<size_t>(rows) * kMlpHidden);
+    d.h2.ensure(static_cast<size_t>(rows) * kMlpHidden); d.h2b.ensure(static_cast<size_t>(rows) * kMlpHidden);
+    d.out.ensure(static_cast<size_t>(rows) * kMlpOut);
+    d.dout.ensure(static_cast<size_t>(rows) * kMlpOut); d.doutb.ensure(static_cast<size_t>(rows) * kMlpOut);
+    d.dh2.ensure(static_cast<size_t>(rows) * kMlpHidden); d.dh2b.ensure(static_cast<size_t>(rows) * kMlpHidden);
+    d.dh1.ensure(static_cast<size_t>(rows) * kMlpHidden); d.dh1b.ensure(static_cast<size_t>(rows) * kMlpHidden);
+    d.ahat.ensure(static_cast<size_t>(rows) * 64);
+    d.hmat.ensure(static_cast<size_t>(rows) * 64);
+    d.scale.ensure(static_cast<size_t>(rows));
+    d.loss_part.ensure(static_cast<size_t>(nblk(static_cast<size_t>(rows), 256)));
+    check(hipMemsetAsync(d.xf.p, 0, static_cast<size_t>(rows) * kMlpIn * sizeof(float), d.s), "memset");
+    check(hipMemsetAsync(d.ahat.p, 0, static_cast<size_t>(rows) * 64 * sizeof(float), d.s), "memset");
+    check(hipMemsetAsync(d.hmat.p, 0, static_cast<size_t>(rows) * 64 * sizeof(float), d.s), "memset");
+    check(hipMemcpyAsync(d.xf.p, data.feat.data() + static_cast<size_t>(first) * kMlpIn,
+                         static_cast<size_t>(count) * kMlpIn * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
+    check(hipMemcpyAsync(d.ahat.p, data.ahat.data() + static_cast<size_t>(first) * 64,
+                         static_cast<size_t>(count) * 64 * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
+    check(hipMemcpyAsync(d.hmat.p, data.hmat.data() + static_cast<size_t>(first) * 64,
+                         static_cast<size_t>(count) * 64 * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
+    check(hipMemcpyAsync(d.scale.p, data.scale.data() + first, static_cast<size_t>(count) * sizeof(float),
+                         hipMemcpyHostToDevice, d.s), "upload");
+    hipLaunchKernelGGL(k_cast_rows, dim3(nblk(static_cast<size_t>(rows) * kMlpIn, 256)), dim3(256), 0, d.s, d.xf.p,
+                       static_cast<size_t>(rows) * kMlpIn, d.xb.p);
+    return rows;
+}
+
+void Mlp::forward_staged(int rows) {
+    Dev& d = *d_;
+    const dim3 g(static_cast<unsigned>(rows / 64)), b(256);
+    hipLaunchKernelGGL((k_mlp_gemm<kMlpIn, kMlpHidden, 0>), g, b, 0, d.s, d.xb.p, d.w1t.p, d.w.p + kOffB1, nullptr,
+                       d.h1.p, d.h1b.p, rows);
+    hipLaunchKernelGGL((k_mlp_gemm<kMlpHidden, kMlpHidden, 0>), g, b, 0, d.s, d.h1b.p, d.w2t.p, d.w.p + kOffB2, nullptr,
+                       d.h2.p, d.h2b.p, rows);
+    hipLaunchKernelGGL((k_mlp_gemm<kMlpHidden, kMlpOut, 1>), g, b, 0, d.s, d.h2b.p, d.w3t.p, d.w.p + kOffB3, nullptr,
+                       d.out.p, nullptr, rows);
+}
+
+double Mlp::loss_grad_staged(int rows, int count) {
+    Dev& d = *d_;
+    const int lb = nblk(static_cast<size_t>(count), 256);
+    hipLaunchKernelGGL(k_mlp_loss, dim3(lb), dim3(256), 0, d.s, d.out.p, d.ahat.p, d.hmat.p, count,
+                       1.0f / static_cast<float>(count), d.dout.p, d.doutb.p, d.loss_part.p);
+    // zero the gradient of the padding rows (loss kernel writes rows < count only)
+    if (rows > count) {
+        check(hipMemsetAsync(d.dout.p + static_cast<size_t>(count) * kMlpOut, 0,
+                             static_cast<size_t>(rows - count) * kMlpOut * sizeof(float), d.s), "memset");
+        check(hipMemsetAsync(d.doutb.p + static_cast<size_t>(count) * kMlpOut, 0,
+                             static_cast<size_t>(rows - count) * kMlpOut * sizeof(uint16_t), d.s), "memset");
+    }
+    const dim3 g(static_cast<unsigned>(rows / 64)), b(256);
+    // dH2 = (dOut W3^T) * [H2 > 0];  dH1 = (dH2 W2^T) * [H1 > 0]   (BF16 MFMA)
+    hipLaunchKernelGGL((k_mlp_gemm<kMlpOut, kMlpHidden, 2>), g, b, 0, d.s, d.doutb.p, d.w3k.p, nullptr, d.h2.p,
+                       d.dh2.p, d.dh2b.p, rows);
+    hipLaunchKernelGGL((k_mlp_gemm<kMlpHidden, kMlpHidden, 2>), g, b, 0, d.s, d.dh2b.p, d.w2k.p, nullptr, d.h1.p,
+                       d.dh1.p, d.dh1b.p, rows);
+    // weight gradients (FP32), chunked and reduced in chunk order
+    const int chunks = nblk(static_cast<size_t>(rows), kDwRows);
+    const int l3 = kW3 + kB3, l2 = kW2 + kB2, l1 = kW1 + kB1;
+    d.part.ensure(static_cast<size_t>(chunks) * static_cast<size_t>(std::max({l1, l2, l3})));
+    hipLaunchKernelGGL((k_mlp_dw<kMlpHidden, kMlpOut>), dim3(chunks), b, 0, d.s, d.h2.p, d.dout.p, rows, d.part.p);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(l3, 256)), b, 0, d.s, d.part.p, chunks, l3, d.g.p + kOffW3);
+    hipLaunchKernelGGL((k_mlp_dw<kMlpHidden, kMlpHidden>), dim3(chunks), b, 0, d.s, d.h1.p, d.dh2.p, rows, d.part.p);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(l2, 256)), b, 0, d.s, d.part.p, chunks, l2, d.g.p + kOffW2);
+    hipLaunchKernelGGL((k_mlp_dw<kMlpIn, kMlpHidden>), dim3(chunks), b, 0, d.s, d.xf.p, d.dh1.p, rows, d.part.p);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(l1, 256)), b, 0, d.s, d.part.p, chunks, l1, d.g.p + kOffW1);
+    std::vector<double> lp(static_cast<size_t>(lb));
+    check(hipMemcpyAsync(lp.data(), d.loss_part.p, lp.size() * sizeof(double), hipMemcpyDeviceToHost, d.s), "download");
+    check(hipStreamSynchronize(d.s), "sync");
+    double loss = 0.0;
+    for (double v : lp) loss += v;
+    return loss / count;
+}
+
+double Mlp::loss_grad(const BlockSet& data, int first, int count, float* grad) {
+    const int rows = stage(data, first, count);
+    forward_staged(rows);
+    const double loss = loss_grad_staged(rows, count);
+    if (grad != nullptr) {
+        check(hipMemcpyAsync(grad, d_->g.p, static_cast<size_t>(num_params()) * sizeof(float), hipMemcpyDeviceToHost,
+                             d_->s), "download");
+        check(hipStreamSynchronize(d_->s), "sync");
+    }
+    return loss;
+}
+
+void Mlp::forward(const BlockSet& data, int first, int count, float* out) {
+    const int rows = stage(data, first, count);
+    forward_staged(rows);
+    check(hipMemcpyAsync(out, d_->out.p, static_cast<size_t>(count) * kMlpOut * sizeof(float), hipMemcpyDeviceToHost,
+                         d_->s), "download");
+    check(hipStreamSynchronize(d_->s), "sync");
+}
+
+std::vector<float> Mlp::blocks(const BlockSet& data) {
+    Dev& d = *d_;
+    std::vector<float> out(static_cast<size_t>(data.count) * 64);
+    const int step = 1 << 16;
+    for (int first = 0; first < data.count; first += step) {
+        const int count = std::min(step, data.count - first);
+        const int rows = stage(data, first, count);
+        forward_staged(rows);
+        d.blocks.ensure(static_cast<size_t>(rows) * 64);
+        hipLaunchKernelGGL(k_mlp_blocks, dim3(nblk(static_cast<size_t>(count), 256)), dim3(256), 0, d.s, d.out.p, d.scale.p,
+                           count, d.blocks.p);
+        check(hipMemcpyAsync(out.data() + static_cast<size_t>(first) * 64, d.blocks.p,
+                             static_cast<size_t>(count) * 64 * sizeof(float), hipMemcpyDeviceToHost, d.s), "download");
+    }
+    check(hipStreamSynchronize(d.s), "sync");
+    return out;
+}
+
+std::vector<double> Mlp::train(const BlockSet& data, int steps, int batch, float lr, uint64_t seed) {
+    Dev& d = *d_;
+    std::vector<double> hist;
+    if (data.count <= 0) throw std::runtime_error("empty training set");
+    batch = std::max(64, std::min(batch, data.count));
+    // Deterministic shuffled order of contiguous batches: batch starts drawn
+    // from a SplitMix64 stream over [0, count - batch].
+    uint64_t st = seed ^ 0xBB67AE8584CAA73Bull;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    const int np = num_params();
+    for (int it = 1; it <= steps; ++it) {
+        const int first = data.count == batch ? 0 : static_cast<int>(next() % static_cast<uint64_t>(data.count - batch + 1));
+        const int rows = stage(data, first, batch);
+        forward_staged(rows);
+        hist.push_back(loss_grad_staged(rows, batch));
+        const float c1 = 1.0f - std::pow(0.9f, static_cast<float>(it));
+        const float c2 = 1.0f - std::pow(0.999f, static_cast<float>(it));
+        hipLaunchKernelGGL(k_adam, dim3(nblk(static_cast<size_t>(np), 256)), dim3(256), 0, d.s, d.w.p, d.g.p, d.m.p, d.v.p,
+                           np, lr, 0.9f, 0.999f, 1e-8f, c1, c2);
+        cast_weights();
+    }
+    check(hipStreamSynchronize(d.s), "sync");
+    return hist;
+}
+
+}  // namespace xp

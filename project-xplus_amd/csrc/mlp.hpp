@@ -1,0 +1,50 @@
+// mlp.hpp -- learned 8x8 block-Jacobi preconditioner (features, MLP, training).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace xp {
+
+constexpr int kMlpIn = 48;      // 36 block entries + 8 off-block sums + 4 zero pad
+constexpr int kMlpHidden = 64;
+constexpr int kMlpOut = 48;     // 36 entries of L + 12 unused
+
+// Features and training targets of the aligned 8-row blocks of a matrix.
+struct BlockSet {
+    int count = 0;
+    std::vector<float> feat;   // count x kMlpIn
+    std::vector<float> ahat;   // count x 64: A_bb / s_b
+    std::vector<float> hmat;   // count x 64: Ahat^2 + G
+    std::vector<float> scale;  // count: s_b
+    static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values);
+    void append(const BlockSet& o);
+};
+
+class Mlp {
+  public:
+    explicit Mlp(uint64_t seed);
+    ~Mlp();
+    static int num_params();
+    const std::vector<float>& params();      // downloads the current weights
+    void set_params(const float* p);
+    // Mean loss over samples [first, first+count) and its gradient (flat, like params).
+    double loss_grad(const BlockSet& data, int first, int count, float* grad);
+    void forward(const BlockSet& data, int first, int count, float* out);  // count x kMlpOut
+    std::vector<double> train(const BlockSet& data, int steps, int batch, float lr, uint64_t seed);
+    // M_b = (L L^T + 1e-3 I) / s_b for every block of data (count x 64, row-major).
+    std::vector<float> blocks(const BlockSet& data);
+
+  private:
+    struct Dev;
+    std::unique_ptr<Dev> d_;
+    std::vector<float> params_;
+    void upload_params();
+    void cast_weights();
+    int stage(const BlockSet& data, int first, int count);
+    void forward_staged(int rows);
+    double loss_grad_staged(int rows, int count);
+};
+
+}  // namespace xp

@@ -23,6 +23,7 @@
 #include "csr_dataset.hpp"
 #include "generators.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "sell_build.hpp"
 
 namespace {
@@ -136,6 +137,7 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 struct xp_matrix {
     int n = 0, n_pad = 0, nslices = 0, nsys = 0, sigma = 0, idx16 = 0, variant = 0;
     int solvers = 0;                  // live xp_pcg objects built on this matrix
+    int block_rows = 1;               // 8: aligned 8-row groups kept together
     bool keep_lo = false, keep64 = false;
     std::vector<int> h_row_ptr, h_col_idx, h_ns;  // host copy of the CSR batch (for re-sorting)
     std::vector<double> h_values;
@@ -251,7 +253,7 @@ void load_sell(xp_matrix& m, int sigma) {
         rp_off += static_cast<size_t>(nk) + 1;
         nz_off += static_cast<size_t>(rp[nk]);
     }
-    xp::SellHost h = xp::build_sell(systems, sigma, m.keep_lo, m.keep64);
+    xp::SellHost h = xp::build_sell(systems, sigma, m.keep_lo, m.keep64, m.block_rows);
     m.n = h.n;
     m.n_pad = h.n_pad;
     m.nslices = h.nslices;
@@ -305,6 +307,7 @@ std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, c
     m->keep_lo = o != nullptr && o->keep_lo;
     m->keep64 = o != nullptr && o->keep64;
     m->variant = (o != nullptr && o->spmv_variant == 1) ? 1 : 0;
+    m->block_rows = (o != nullptr && o->block_rows == 8) ? 8 : 1;
     for (const auto& a : systems) {
         if (a.n < 0) throw std::runtime_error("build_sell: negative n");
         if (a.n > 0 && (a.row_ptr == nullptr || a.col_idx == nullptr || a.values == nullptr))
@@ -348,6 +351,9 @@ struct xp_pcg {
     // refinement buffers (allocated on first use)
     DevBuf<double> x64, b64, r64, rr_out, tmp_nat;
     DevBuf<float> r32;
+    DevBuf<float> blockM;             // 8x8 blocks in solve order (block preconditioner)
+    bool use_block = false;
+    int fallback = 0;
     bool x64_valid = false;           // x64 holds the solution of the last (refined) run
     const float* rhs = nullptr;       // RHS of the init kernel; null = b
 
@@ -361,6 +367,8 @@ struct xp_pcg {
         if (stream != nullptr) (void)hipStreamDestroy(stream);
     }
 
+    const float* bm() const { return use_block ? blockM.p : nullptr; }
+
     // Iteration k reads state slot k % 2 and writes slot (k + 1) % 2; the
     // graph holds an even number of iterations, so slot 0 is current
     // between launches.
@@ -372,10 +380,10 @@ struct xp_pcg {
         xp::SysState* nxt = st.p + ((k + 1) % 2) * ns;
         if (opts.fused) {
             xp::launch_pcg_spmv_fused(L, A, cur, nxt, trace.p, trace_stride, z.p, p.p, ap.p);
-            xp::launch_pcg_update_xrz(L, nxt, x.p, p.p, r.p, ap.p, minv.p, z.p);
+            xp::launch_pcg_update_xrz(L, nxt, x.p, p.p, r.p, ap.p, minv.p, bm(), z.p);
         } else {
             xp::launch_pcg_spmv(L, A, cur, p.p, ap.p);
-            xp::launch_pcg_update_xrz(L, cur, x.p, p.p, r.p, ap.p, minv.p, z.p);
+            xp::launch_pcg_update_xrz(L, cur, x.p, p.p, r.p, ap.p, minv.p, bm(), z.p);
             xp::launch_pcg_update_p(L, cur, nxt, trace.p, trace_stride, z.p, p.p);
         }
     }
@@ -395,7 +403,7 @@ struct xp_pcg {
     }
 
     void enqueue_init() {
-        xp::launch_pcg_init(m->launch(stream), m->view(), st.p, x.p, rhs != nullptr ? rhs : b.p, minv.p, r.p,
+        xp::launch_pcg_init(m->launch(stream), m->view(), st.p, x.p, rhs != nullptr ? rhs : b.p, minv.p, bm(), r.p,
                             z.p, p.p);
     }
 
@@ -925,12 +933,35 @@ int xp_pcg_set_rhs(xp_pcg* s, const double* b, const double* x0) {
 int xp_pcg_run(xp_pcg* s) {
     return guarded([&] {
         if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run: call xp_pcg_set_rhs first");
+        long long cap = 0;
+        for (const auto& t : s->st_init) cap = std::max<long long>(cap, t.max_iters);
+        s->fallback = 0;
         // Every run starts from x0 and the configured state.
         HIPX(hipMemcpyAsync(s->x.p, s->x0s.p, s->x0s.n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
         s->reset_state();
-        long long cap = 0;
-        for (const auto& t : s->st_init) cap = std::max<long long>(cap, t.max_iters);
         s->run_loop(cap + s->graph_iters);
+        if (s->use_block && s->opts.stall_limit > 0) {
+            // Stagnation fallback: a block preconditioner that stalls or breaks
+            // down is replaced by plain Jacobi and the solve redone from x0.
+            const size_t ns = static_cast<size_t>(s->m->nsys);
+            HIPX(hipMemcpyAsync(s->st_host, s->st.p, ns * sizeof(xp::SysState), hipMemcpyDeviceToHost, s->stream));
+            HIPX(hipStreamSynchronize(s->stream));
+            bool failed = false;
+            for (size_t k = 0; k < ns; ++k)
+                failed = failed || s->st_host[k].status == xp::kStatusStagnation || s->st_host[k].status == xp::kStatusBreakdown;
+            if (failed) {
+                const double first_ms = s->last_ms;
+                s->use_block = false;
+                if (s->opts.use_graph) s->build_graph(s->opts.graph_iters);
+                HIPX(hipMemcpyAsync(s->x.p, s->x0s.p, s->x0s.n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+                s->reset_state();
+                s->run_loop(cap + s->graph_iters);
+                s->last_ms += first_ms;
+                s->use_block = true;
+                if (s->opts.use_graph) s->build_graph(s->opts.graph_iters);
+                s->fallback = 1;
+            }
+        }
     });
 }
 
@@ -962,6 +993,7 @@ int xp_pcg_get_status(xp_pcg* s, int sys, xp_status* out) {
         out->beta = t.beta;
         out->rr0 = t.rr0;
         out->tau = t.tau;
+        out->fallback = s->fallback;
     });
 }
 
@@ -1362,3 +1394,255 @@ extern "C" int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int i
         }
     });
 }
+
+// ---- block preconditioners -------------------------------------------------------
+namespace {
+
+int num_blocks(const xp_matrix& m) {
+    int nb = 0;
+    for (int nk : m.h_ns) nb += (nk + 7) / 8;
+    return nb;
+}
+
+// Exact inverse of every natural 8-row diagonal block (FP64 Cholesky; a block
+// that is not positive definite falls back to its diagonal inverse).
+std::vector<float> exact_block_inverses(const xp_matrix& m) {
+    std::vector<float> out(static_cast<size_t>(num_blocks(m)) * 64, 0.0f);
+    size_t rp_off = 0, nz_off = 0, bid = 0;
+    for (int nk : m.h_ns) {
+        const int* rp = m.h_row_ptr.data() + rp_off;
+        const int* ci = m.h_col_idx.data() + nz_off;
+        const double* va = m.h_values.data() + nz_off;
+        for (int b0 = 0; b0 < nk; b0 += 8, ++bid) {
+            const int nb = std::min(8, nk - b0);
+            double a[8][8] = {};
+            for (int i = 0; i < nb; ++i)
+                for (int o = rp[b0 + i]; o < rp[b0 + i + 1]; ++o) {
+                    const int c = ci[o] - b0;
+                    if (c >= 0 && c < nb) a[i][c] += va[o];
+                }
+            double l[8][8] = {};
+            bool ok = true;
+            for (int j = 0; j < nb && ok; ++j) {
+                double d = a[j][j];
+                for (int k = 0; k < j; ++k) d -= l[j][k] * l[j][k];
+                if (!(d > 0.0)) { ok = false; break; }
+                l[j][j] = std::sqrt(d);
+                for (int i = j + 1; i < nb; ++i) {
+                    double v = a[i][j];
+                    for (int k = 0; k < j; ++k) v -= l[i][k] * l[j][k];
+                    l[i][j] = v / l[j][j];
+                }
+            }
+            float* mb = out.data() + bid * 64;
+            if (ok) {
+                // inverse column by column: solve L L^T x = e_c
+                for (int c = 0; c < nb; ++c) {
+                    double y[8] = {}, x[8] = {};
+                    for (int i = 0; i < nb; ++i) {
+                        double v = i == c ? 1.0 : 0.0;
+                        for (int k = 0; k < i; ++k) v -= l[i][k] * y[k];
+                        y[i] = v / l[i][i];
+                    }
+                    for (int i = nb - 1; i >= 0; --i) {
+                        double v = y[i];
+                        for (int k = i + 1; k < nb; ++k) v -= l[k][i] * x[k];
+                        x[i] = v / l[i][i];
+                    }
+                    for (int i = 0; i < nb; ++i) mb[i * 8 + c] = static_cast<float>(x[i]);
+                }
+            } else {
+                for (int i = 0; i < nb; ++i) mb[i * 8 + i] = static_cast<float>(1.0 / a[i][i]);
+            }
+        }
+        rp_off += static_cast<size_t>(nk) + 1;
+        nz_off += static_cast<size_t>(rp[nk]);
+    }
+    return out;
+}
+
+// Natural-order blocks -> solve-order block array (padding rows: identity).
+std::vector<float> arrange_blocks(const xp_matrix& m, const float* nat) {
+    std::vector<float> out(static_cast<size_t>(m.n_pad) * 8, 0.0f);
+    std::vector<int> block_first(m.h_ns.size() + 1, 0);
+    for (size_t k = 0; k < m.h_ns.size(); ++k) block_first[k + 1] = block_first[k] + (m.h_ns[k] + 7) / 8;
+    for (int q = 0; q < m.n_pad / 8; ++q) {
+        float* dst = out.data() + static_cast<size_t>(q) * 64;
+        int g = -1;
+        for (int j = 0; j < 8 && g < 0; ++j) g = m.perm[static_cast<size_t>(q) * 8 + j] >= 0 ? m.perm[static_cast<size_t>(q) * 8 + j] - j : -1;
+        if (g < 0) {  // an all-padding block
+            for (int j = 0; j < 8; ++j) dst[j * 8 + j] = 1.0f;
+            continue;
+        }
+        int k = 0;
+        while (k + 1 < m.nsys && g >= m.sys_nat_begin[static_cast<size_t>(k) + 1]) ++k;
+        const int local = g - m.sys_nat_begin[static_cast<size_t>(k)];
+        const int bid = block_first[static_cast<size_t>(k)] + local / 8;
+        const float* src = nat + static_cast<size_t>(bid) * 64;
+        for (int j = 0; j < 8; ++j) {
+            const bool real_j = m.perm[static_cast<size_t>(q) * 8 + j] >= 0;
+            for (int c = 0; c < 8; ++c) {
+                const bool real_c = m.perm[static_cast<size_t>(q) * 8 + c] >= 0;
+                dst[j * 8 + c] = (real_j && real_c) ? src[j * 8 + c] : (j == c ? 1.0f : 0.0f);
+            }
+        }
+    }
+    return out;
+}
+
+void set_preconditioner(xp_pcg& s, int mode, const float* blocks) {
+    xp_matrix& m = *s.m;
+    if (mode == 0) {
+        s.use_block = false;
+    } else {
+        if (m.block_rows != 8) throw std::runtime_error("block preconditioners need a matrix built with block_rows = 8");
+        std::vector<float> nat;
+        const float* src = blocks;
+        if (mode == 1) {
+            nat = exact_block_inverses(m);
+            src = nat.data();
+        } else if (mode != 2 || blocks == nullptr) {
+            throw std::runtime_error("bad preconditioner mode");
+        }
+        const std::vector<float> sol = arrange_blocks(m, src);
+        if (s.blockM.n != sol.size()) s.blockM.alloc(sol.size());
+        s.blockM.upload(sol.data(), sol.size(), s.stream);
+        HIPX(hipStreamSynchronize(s.stream));
+        s.use_block = true;
+    }
+    if (s.opts.use_graph) s.build_graph(s.opts.graph_iters);
+}
+
+}  // namespace
+
+extern "C" int xp_matrix_num_blocks(const xp_matrix* m) {
+    if (m == nullptr) return -1;
+    return num_blocks(*m);
+}
+
+extern "C" int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks) {
+    return guarded([&] {
+        if (s == nullptr) throw std::runtime_error("null solver");
+        set_preconditioner(*s, mode, blocks);
+    });
+}
+
+// ---- learned block preconditioner: C-ABI ----------------------------------------
+struct xp_mlp {
+    xp::Mlp net;
+    explicit xp_mlp(uint64_t seed) : net(seed) {}
+};
+struct xp_blockset {
+    xp::BlockSet set;
+};
+
+extern "C" {
+
+int xp_mlp_create(unsigned long long seed, xp_mlp** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null output pointer");
+        device();
+        *out = new xp_mlp(seed);
+    });
+}
+
+void xp_mlp_destroy(xp_mlp* net) { delete net; }
+
+int xp_mlp_num_params(void) { return xp::Mlp::num_params(); }
+
+int xp_mlp_get_params(xp_mlp* net, float* out) {
+    return guarded([&] {
+        if (net == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        const auto& p = net->net.params();
+        std::memcpy(out, p.data(), p.size() * sizeof(float));
+    });
+}
+
+int xp_mlp_set_params(xp_mlp* net, const float* params) {
+    return guarded([&] {
+        if (net == nullptr || params == nullptr) throw std::runtime_error("null argument");
+        net->net.set_params(params);
+    });
+}
+
+int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, xp_blockset** out) {
+    return guarded([&] {
+        if (out == nullptr || row_ptr == nullptr || n <= 0) throw std::runtime_error("bad argument");
+        auto b = std::make_unique<xp_blockset>();
+        b->set = xp::BlockSet::from_csr(n, row_ptr, col_idx, values);
+        *out = b.release();
+    });
+}
+
+int xp_blockset_append(xp_blockset* dst, const xp_blockset* src) {
+    return guarded([&] {
+        if (dst == nullptr || src == nullptr) throw std::runtime_error("null argument");
+        dst->set.append(src->set);
+    });
+}
+
+int xp_blockset_size(const xp_blockset* set) { return set == nullptr ? -1 : set->set.count; }
+
+int xp_blockset_copy(const xp_blockset* set, float* feat, float* ahat, float* hmat, float* scale) {
+    return guarded([&] {
+        if (set == nullptr) throw std::runtime_error("null argument");
+        const auto& b = set->set;
+        if (feat) std::memcpy(feat, b.feat.data(), b.feat.size() * sizeof(float));
+        if (ahat) std::memcpy(ahat, b.ahat.data(), b.ahat.size() * sizeof(float));
+        if (hmat) std::memcpy(hmat, b.hmat.data(), b.hmat.size() * sizeof(float));
+        if (scale) std::memcpy(scale, b.scale.data(), b.scale.size() * sizeof(float));
+    });
+}
+
+void xp_blockset_free(xp_blockset* set) { delete set; }
+
+int xp_mlp_loss_grad(xp_mlp* net, const xp_blockset* set, int first, int count, double* loss, float* grad) {
+    return guarded([&] {
+        if (net == nullptr || set == nullptr) throw std::runtime_error("null argument");
+        const double l = net->net.loss_grad(set->set, first, count, grad);
+        if (loss != nullptr) *loss = l;
+    });
+}
+
+int xp_mlp_forward(xp_mlp* net, const xp_blockset* set, int first, int count, float* out) {
+    return guarded([&] {
+        if (net == nullptr || set == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        net->net.forward(set->set, first, count, out);
+    });
+}
+
+int xp_mlp_blocks(xp_mlp* net, const xp_blockset* set, float* blocks) {
+    return guarded([&] {
+        if (net == nullptr || set == nullptr || blocks == nullptr) throw std::runtime_error("null argument");
+        const std::vector<float> b = net->net.blocks(set->set);
+        std::memcpy(blocks, b.data(), b.size() * sizeof(float));
+    });
+}
+
+int xp_mlp_train(xp_mlp* net, const xp_blockset* set, int steps, int batch, float lr, unsigned long long seed,
+                 double* loss_history) {
+    return guarded([&] {
+        if (net == nullptr || set == nullptr || steps < 0) throw std::runtime_error("bad argument");
+        const std::vector<double> h = net->net.train(set->set, steps, batch, lr, seed);
+        if (loss_history != nullptr) std::memcpy(loss_history, h.data(), h.size() * sizeof(double));
+    });
+}
+
+int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net) {
+    return guarded([&] {
+        if (s == nullptr || net == nullptr) throw std::runtime_error("null argument");
+        const xp_matrix& m = *s->m;
+        xp::BlockSet all;
+        size_t rp_off = 0, nz_off = 0;
+        for (int nk : m.h_ns) {
+            const int* rp = m.h_row_ptr.data() + rp_off;
+            all.append(xp::BlockSet::from_csr(nk, rp, m.h_col_idx.data() + nz_off, m.h_values.data() + nz_off));
+            rp_off += static_cast<size_t>(nk) + 1;
+            nz_off += static_cast<size_t>(rp[nk]);
+        }
+        const std::vector<float> blocks = net->net.blocks(all);
+        set_preconditioner(*s, 2, blocks.data());
+    });
+}
+
+}  // extern "C"

@@ -13,14 +13,17 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 }  // namespace
 
-SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64) {
+SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64, int block_rows) {
     if (systems.empty()) throw std::runtime_error("build_sell: no systems");
     if (sigma <= 0) sigma = kC;
     if (sigma % kC != 0) throw std::runtime_error("build_sell: sigma must be a multiple of 64");
+    if (block_rows != 1 && block_rows != 8) throw std::runtime_error("build_sell: block_rows must be 1 or 8");
+    const int br = block_rows;
 
     SellHost s;
     s.nsys = static_cast<int>(systems.size());
     s.sigma = sigma;
+    s.block_rows = br;
     s.sys_row_begin.assign(systems.size() + 1, 0);
     s.sys_nat_begin.assign(systems.size() + 1, 0);
     for (size_t k = 0; k < systems.size(); ++k) {
@@ -28,8 +31,9 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
         if (a.n < 0) throw std::runtime_error("build_sell: negative n");
         if (a.n > 0 && (a.row_ptr == nullptr || a.col_idx == nullptr || a.values == nullptr))
             throw std::runtime_error("build_sell: null CSR array");
-        // Every system gets at least one slice so that it owns a workgroup.
-        const int padded = std::max(round_up(a.n, kC), kC);
+        // Every system gets at least one slice so that it owns a workgroup;
+        // with blocks, a partial last block keeps its own 8 solve rows.
+        const int padded = std::max(round_up(round_up(a.n, br), kC), kC);
         s.sys_row_begin[k + 1] = s.sys_row_begin[k] + padded;
         s.sys_nat_begin[k + 1] = s.sys_nat_begin[k] + a.n;
     }
@@ -68,22 +72,33 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
     }
 
     // Permutation: stable sort by descending off-diagonal count inside each
-    // window of sigma rows of each system.
+    // window of sigma rows of each system (units of br rows, keyed by the
+    // longest row of the unit, when block_rows = 8).
     for (int k = 0; k < s.nsys; ++k) {
         const int nat0 = s.sys_nat_begin[static_cast<size_t>(k)];
         const int n_k = s.sys_nat_begin[static_cast<size_t>(k) + 1] - nat0;
         const int solve0 = s.sys_row_begin[static_cast<size_t>(k)];
-        std::vector<int> order(static_cast<size_t>(n_k));
+        const int nu = (n_k + br - 1) / br;  // units
+        std::vector<int> key(static_cast<size_t>(nu), 0);
+        for (int u = 0; u < nu; ++u)
+            for (int j = 0; j < br && u * br + j < n_k; ++j)
+                key[static_cast<size_t>(u)] = std::max(key[static_cast<size_t>(u)], offcnt[static_cast<size_t>(nat0 + u * br + j)]);
+        std::vector<int> order(static_cast<size_t>(nu));
         std::iota(order.begin(), order.end(), 0);
-        for (int w0 = 0; w0 < n_k; w0 += sigma) {
-            const int w1 = std::min(n_k, w0 + sigma);
-            std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
-                return offcnt[static_cast<size_t>(nat0 + a)] > offcnt[static_cast<size_t>(nat0 + b)];
-            });
+        const int wu = sigma / br;  // units per window
+        for (int w0 = 0; w0 < nu; w0 += wu) {
+            const int w1 = std::min(nu, w0 + wu);
+            std::stable_sort(order.begin() + w0, order.begin() + w1,
+                             [&](int a, int b) { return key[static_cast<size_t>(a)] > key[static_cast<size_t>(b)]; });
         }
-        for (int i = 0; i < n_k; ++i) {
-            s.perm[static_cast<size_t>(solve0 + i)] = nat0 + order[static_cast<size_t>(i)];
-            s.iperm[static_cast<size_t>(nat0 + order[static_cast<size_t>(i)])] = solve0 + i;
+        for (int pos = 0; pos < nu; ++pos) {
+            const int u = order[static_cast<size_t>(pos)];
+            for (int j = 0; j < br && u * br + j < n_k; ++j) {
+                const int g = nat0 + u * br + j;
+                const int i = solve0 + pos * br + j;
+                s.perm[static_cast<size_t>(i)] = g;
+                s.iperm[static_cast<size_t>(g)] = i;
+            }
         }
     }
 

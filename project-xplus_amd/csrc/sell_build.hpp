@@ -30,6 +30,7 @@ struct SellHost {
     int nsys = 0;
     int sigma = 0;
     int idx16 = 0;
+    int block_rows = 1;
     long long nnz = 0;            // real stored entries
     long long stored = 0;         // slice_ptr[nslices] * 64, including padding
     std::vector<int> perm;        // solve row -> natural global row, -1 for padding
@@ -50,6 +51,9 @@ struct SellHost {
 // the sort window in rows (a multiple of 64, or 0/64 for no sorting).
 // keep_lo / keep64 request the hi+lo and exact-FP64 value streams.
 // Throws std::runtime_error on malformed input.
-SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64);
+// block_rows = 8 keeps aligned groups of 8 natural rows together as 8
+// consecutive solve rows (sorted as units), for block preconditioners.
+SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64,
+                    int block_rows = 1);
 
 }  // namespace xp

@@ -42,7 +42,7 @@ STATUS_NAMES = {0: "converged", 1: "max_iter", 2: "breakdown", 3: "stagnation", 
 
 class MatrixOpts(C.Structure):
     _fields_ = [("sigma", C.c_int), ("keep_lo", C.c_int), ("keep64", C.c_int), ("grid", C.c_int),
-                ("spmv_variant", C.c_int)]
+                ("spmv_variant", C.c_int), ("block_rows", C.c_int)]
 
 
 class MatrixInfo(C.Structure):
@@ -65,7 +65,7 @@ class TuneReport(C.Structure):
 class Status(C.Structure):
     _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("rz", C.c_double), ("rr", C.c_double),
                 ("pap", C.c_double), ("alpha", C.c_double), ("beta", C.c_double), ("rr0", C.c_double),
-                ("tau", C.c_double)]
+                ("tau", C.c_double), ("fallback", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -124,6 +124,24 @@ _p_get_x = _sig("xp_pcg_get_x", C.c_int, [_vp, _f64p])
 _p_trace = _sig("xp_pcg_get_trace", C.c_int, [_vp, C.c_int, _f64p, C.c_int])
 _p_true_res = _sig("xp_pcg_true_residual", C.c_int, [_vp, _f64p])
 _p_timing = _sig("xp_pcg_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int)])
+
+_p_set_precond = _sig("xp_pcg_set_preconditioner", C.c_int, [_vp, C.c_int, _vp])
+_p_set_mlp = _sig("xp_pcg_set_mlp_preconditioner", C.c_int, [_vp, _vp])
+_m_num_blocks = _sig("xp_matrix_num_blocks", C.c_int, [_vp])
+_mlp_create = _sig("xp_mlp_create", C.c_int, [C.c_ulonglong, C.POINTER(_vp)])
+_mlp_destroy = _sig("xp_mlp_destroy", None, [_vp])
+_mlp_nparams = _sig("xp_mlp_num_params", C.c_int, [])
+_mlp_get = _sig("xp_mlp_get_params", C.c_int, [_vp, _f32p])
+_mlp_set = _sig("xp_mlp_set_params", C.c_int, [_vp, _f32p])
+_bs_from = _sig("xp_blockset_from_csr", C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)])
+_bs_append = _sig("xp_blockset_append", C.c_int, [_vp, _vp])
+_bs_size = _sig("xp_blockset_size", C.c_int, [_vp])
+_bs_copy = _sig("xp_blockset_copy", C.c_int, [_vp, _f32p, _f32p, _f32p, _f32p])
+_bs_free = _sig("xp_blockset_free", None, [_vp])
+_mlp_loss_grad = _sig("xp_mlp_loss_grad", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.POINTER(C.c_double), _f32p])
+_mlp_forward = _sig("xp_mlp_forward", C.c_int, [_vp, _vp, C.c_int, C.c_int, _f32p])
+_mlp_blocks = _sig("xp_mlp_blocks", C.c_int, [_vp, _vp, _f32p])
+_mlp_train = _sig("xp_mlp_train", C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, _f64p])
 
 _ds_load = _sig("xp_dataset_load", C.c_int, [C.c_char_p, C.POINTER(_vp)])
 _ds_save = _sig("xp_dataset_save", C.c_int, [_vp, C.c_char_p])
@@ -344,11 +362,11 @@ class Matrix:
     """Device-resident SELL-C-sigma matrix (one system or a batch)."""
 
     def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0,
-                 spmv_variant: int = 0):
+                 spmv_variant: int = 0, block_rows: int = 1):
         if isinstance(systems, Csr):
             systems = [systems]
         self.systems = list(systems)
-        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant)
+        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant, block_rows)
         h = _vp()
         if len(self.systems) == 1:
             a = self.systems[0]
@@ -368,6 +386,9 @@ class Matrix:
 
     def __del__(self):
         self.close()
+
+    def num_blocks(self) -> int:
+        return _m_num_blocks(self.h)
 
     def info(self) -> dict:
         i = MatrixInfo()
@@ -425,6 +446,8 @@ class Pcg:
     def __init__(self, matrix: Matrix, tau: float = 1e-10, rel_tol: float = 0.0, max_iters: int = 0,
                  graph_iters: int = 0, use_graph: bool = True, trace_cap: int = 0, stall_limit: int = 0,
                  fused: bool = False):
+        """tau: absolute bound on r^T r; rel_tol > 0 overrides it per system as
+        (rel_tol ||b_k||)^2."""
         self.matrix = matrix
         self.opts = SolveOpts(tau, rel_tol, max_iters, graph_iters, int(use_graph), trace_cap, stall_limit,
                               int(fused))
@@ -486,7 +509,114 @@ class Pcg:
         _check(_p_true_res(self.h, out))
         return out
 
+    def set_preconditioner(self, mode: int, blocks=None) -> None:
+        """0 Jacobi, 1 exact 8x8 block Jacobi, 2 given blocks (nblocks x 8 x 8, natural order)."""
+        if blocks is None:
+            _check(_p_set_precond(self.h, mode, None))
+        else:
+            b = np.ascontiguousarray(blocks, dtype=np.float32)
+            _check(_p_set_precond(self.h, mode, b.ctypes.data_as(_vp)))
+
+    def set_mlp_preconditioner(self, net: "Mlp") -> None:
+        _check(_p_set_mlp(self.h, net.h))
+
     def last_timing(self):
         ms, launches = C.c_double(), C.c_int()
         _check(_p_timing(self.h, C.byref(ms), C.byref(launches)))
         return ms.value, launches.value
+
+
+# ---------------------------------------------------------------- learned block preconditioner
+class BlockSet:
+    """Features / training targets of the aligned 8-row blocks (host)."""
+
+    def __init__(self, a: Csr = None):
+        self.h = None
+        if a is not None:
+            h = _vp()
+            _check(_bs_from(a.n, a.row_ptr, a.col_idx, a.values, C.byref(h)))
+            self.h = h
+
+    def append(self, other: "BlockSet") -> None:
+        _check(_bs_append(self.h, other.h))
+
+    def __len__(self) -> int:
+        return _bs_size(self.h)
+
+    def arrays(self):
+        n = len(self)
+        f = np.empty(n * 48, np.float32)
+        a = np.empty(n * 64, np.float32)
+        hm = np.empty(n * 64, np.float32)
+        sc = np.empty(n, np.float32)
+        _check(_bs_copy(self.h, f, a, hm, sc))
+        return f.reshape(n, 48), a.reshape(n, 8, 8), hm.reshape(n, 8, 8), sc
+
+    def close(self):
+        if getattr(self, "h", None):
+            _bs_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def blockset(systems) -> BlockSet:
+    if isinstance(systems, Csr):
+        systems = [systems]
+    out = BlockSet(systems[0])
+    for a in systems[1:]:
+        out.append(BlockSet(a))
+    return out
+
+
+class Mlp:
+    """The block-preconditioner network on the GPU (BF16 MFMA forward)."""
+
+    def __init__(self, seed: int = 0):
+        h = _vp()
+        _check(_mlp_create(seed, C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def num_params() -> int:
+        return _mlp_nparams()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.num_params(), np.float32)
+        _check(_mlp_get(self.h, out))
+        return out
+
+    def set_params(self, p) -> None:
+        _check(_mlp_set(self.h, np.ascontiguousarray(p, dtype=np.float32)))
+
+    def loss_grad(self, bs: BlockSet, first: int = 0, count: int = None):
+        count = len(bs) - first if count is None else count
+        loss = C.c_double()
+        g = np.empty(self.num_params(), np.float32)
+        _check(_mlp_loss_grad(self.h, bs.h, first, count, C.byref(loss), g))
+        return loss.value, g
+
+    def forward(self, bs: BlockSet, first: int = 0, count: int = None) -> np.ndarray:
+        count = len(bs) - first if count is None else count
+        out = np.empty(count * 48, np.float32)
+        _check(_mlp_forward(self.h, bs.h, first, count, out))
+        return out.reshape(count, 48)
+
+    def blocks(self, bs: BlockSet) -> np.ndarray:
+        out = np.empty(len(bs) * 64, np.float32)
+        _check(_mlp_blocks(self.h, bs.h, out))
+        return out.reshape(len(bs), 8, 8)
+
+    def train(self, bs: BlockSet, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        hist = np.zeros(max(steps, 1))
+        _check(_mlp_train(self.h, bs.h, steps, batch, lr, seed, hist))
+        return hist[:steps]
+
+    def close(self):
+        if getattr(self, "h", None):
+            _mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,122 @@
+"""Pins the CPU oracle of the learned block preconditioner (oracle/block_mlp.py)
+and checks the library's host-side block features against it (no GPU)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import xplus_amd as X
+
+sys.path.insert(0, str(O.ROOT / "oracle"))
+import block_mlp as B  # noqa: E402
+
+
+def small_set(seed=0, m=18):
+    a = X.gen_fem2d(m, 1.0, 3, 1e3, 0.3, seed)
+    return a, B.features_from_csr(a.row_ptr, a.col_idx, a.values)
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    p = B.unpack(np.zeros(B.N_PARAMS))
+    p["W1"] = rng.standard_normal(p["W1"].shape) * 0.2
+    p["b1"] = rng.standard_normal(p["b1"].shape) * 0.1 + 0.05
+    p["W2"] = rng.standard_normal(p["W2"].shape) * 0.15
+    p["b2"] = rng.standard_normal(p["b2"].shape) * 0.1 + 0.05
+    p["W3"] = rng.standard_normal(p["W3"].shape) * 0.1
+    p["b3"] = rng.standard_normal(p["b3"].shape) * 0.1
+    return B.pack(p)
+
+
+def test_oracle_gradient_matches_finite_differences():
+    a, (feat, ahat, hmat, scale) = small_set(1)
+    params = random_params(2)
+    loss, g = B.loss_grad(params, feat, ahat, hmat)
+    rng = np.random.default_rng(3)
+    idx = rng.choice(B.N_PARAMS, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(B.N_PARAMS)
+        e[i] = h
+        lp, _ = B.loss_grad(params + e, feat, ahat, hmat)
+        lm, _ = B.loss_grad(params - e, feat, ahat, hmat)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+    assert loss > 0
+
+
+def test_oracle_loss_closed_form():
+    """tr(M H M^T) - 2 tr(M A) + 8 equals ||M A - I||^2 + ||M A_off||^2 written out."""
+    a, (feat, ahat, hmat, scale) = small_set(4)
+    params = random_params(5)
+    out = B.forward(params, feat)
+    L = B.l_from_out(out)
+    M = L @ np.transpose(L, (0, 2, 1)) + 1e-3 * np.eye(8)
+    g = hmat - ahat @ ahat                     # = Ahat_off Ahat_off^T
+    direct = [np.linalg.norm(M[k] @ ahat[k] - np.eye(8)) ** 2 + np.trace(M[k] @ g[k] @ M[k].T)
+              for k in range(len(M))]
+    loss, _ = B.loss_grad(params, feat, ahat, hmat)
+    assert loss == pytest.approx(np.mean(direct), rel=1e-10)
+    # G is PSD (a Gram matrix of the off-block couplings)
+    assert min(np.linalg.eigvalsh(gk).min() for gk in g) > -1e-10
+
+
+def test_library_features_match_oracle():
+    a, (feat, ahat, hmat, scale) = small_set(6, m=27)   # n = 625: a partial last block
+    assert a.n % 8 != 0
+    bs = X.blockset(a)
+    f, ah, hm, sc = bs.arrays()
+    assert len(bs) == (a.n + 7) // 8
+    np.testing.assert_allclose(f, feat, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(ah, ahat, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(hm, hmat, rtol=1e-5, atol=1e-6 * np.abs(hmat).max())
+    np.testing.assert_allclose(sc, scale, rtol=1e-6)
+
+
+def test_features_hand_block():
+    """One 8x8 tridiagonal block with a single off-block neighbour: closed forms."""
+    n = 9
+    rp, ci, va = [0], [], []
+    for r in range(n):
+        for c in (r - 1, r, r + 1):
+            if 0 <= c < n:
+                ci.append(c)
+                va.append(4.0 if c == r else -1.0)
+        rp.append(len(ci))
+    feat, ahat, hmat, scale = B.features_from_csr(np.array(rp), np.array(ci), np.array(va))
+    assert scale[0] == 4.0
+    assert feat[0, 36 + 7] == 0.25 and feat[0, 36:43].sum() == 0.0   # only row 7 couples to row 8
+    g = hmat[0] - ahat[0] @ ahat[0]
+    assert g[7, 7] == pytest.approx(1 / 16) and abs(g).sum() == pytest.approx(1 / 16)
+
+
+def test_block_pcg_diagonal_blocks_is_jacobi():
+    a = X.gen_diffusion2d(16, 16, 1.0, 3)
+    A = a.to_scipy()
+    d = A.diagonal()
+    nb = (a.n + 7) // 8
+    blocks = np.zeros((nb, 8, 8))
+    for i in range(a.n):
+        blocks[i // 8, i % 8, i % 8] = 1.0 / d[i]
+    tau = 1e-20
+    x, it, tr = B.block_pcg(A, a.b, blocks, tau)
+    g = O.golden(a, tau=tau)
+    assert it == g["iterations"]
+    np.testing.assert_allclose(tr, g["trace"][:, 4], rtol=1e-6)   # 1/d multiply vs divide, spmv order
+    np.testing.assert_allclose(x, g["x"], rtol=1e-10, atol=1e-12)
+
+
+def test_block_pcg_exact_inverse_on_block_diagonal_matrix():
+    rng = np.random.default_rng(0)
+    n = 40
+    dense = np.zeros((n, n))
+    for b in range(n // 8):
+        q = rng.standard_normal((8, 8))
+        dense[8 * b:8 * b + 8, 8 * b:8 * b + 8] = q @ q.T + 8 * np.eye(8)
+    inv = np.stack([np.linalg.inv(dense[8 * b:8 * b + 8, 8 * b:8 * b + 8]) for b in range(n // 8)])
+    x, it, _ = B.block_pcg(dense, np.ones(n), inv, 1e-24)
+    assert it == 1
+    np.testing.assert_allclose(dense @ x, np.ones(n), rtol=1e-10)

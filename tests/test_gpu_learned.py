@@ -1,0 +1,162 @@
+"""GPU tests of the learned 8x8 block-Jacobi preconditioner.
+
+Bars:
+  * MFMA forward (BF16 inputs/weights, FP32 accumulation) against the FP64
+    oracle: normwise relative error <= 2e-2 per output;
+  * loss and gradient (BF16 MFMA input-gradients, FP32 weight gradients)
+    against the FP64 oracle's analytic ones: loss within 1e-2 relative,
+    gradient normwise relative error <= 5e-2 per parameter tensor;
+  * block-preconditioned FP32 PCG against the FP64 oracle block PCG with the
+    same blocks: iterations within +-1, x within relative 1e-5 (config-1 size);
+  * the trained network beats plain Jacobi in iterations on held-out
+    matrices (the gain is measured and printed);
+  * a useless block preconditioner triggers the Jacobi fallback.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import xplus_amd as X
+
+sys.path.insert(0, str(O.ROOT / "oracle"))
+import block_mlp as B  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """Round to BF16 (nearest even), as the GPU kernels do."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_mfma_forward_matches_oracle(gpu):
+    a = X.gen_fem2d(40, 1.0, 4, 1e3, 0.3, 3)
+    bs = X.blockset(a)
+    net = X.Mlp(seed=11)
+    params = net.params().astype(np.float64)
+    out = net.forward(bs)
+    feat = bs.arrays()[0].astype(np.float64)
+    ref = B.forward(params, feat)
+    # the oracle on BF16-rounded inputs and weights (hidden layers rounded too)
+    p = B.unpack(params)
+    h1 = np.maximum(bf16(feat) @ bf16(p["W1"]) + p["b1"], 0)
+    h2 = np.maximum(bf16(h1) @ bf16(p["W2"]) + p["b2"], 0)
+    ref_bf = bf16(h2) @ bf16(p["W3"]) + p["b3"]
+    print("forward rel err vs fp64:", np.linalg.norm(out - ref) / np.linalg.norm(ref),
+          "vs bf16 model:", np.linalg.norm(out - ref_bf) / np.linalg.norm(ref_bf), flush=True)
+    assert np.linalg.norm(out - ref) <= 2e-2 * np.linalg.norm(ref)
+    assert np.linalg.norm(out - ref_bf) <= 1e-5 * np.linalg.norm(ref_bf) + 1e-6
+
+
+def test_loss_and_gradient_match_oracle(gpu):
+    a = X.gen_fem2d(34, 1.0, 4, 1e3, 0.3, 5)
+    bs = X.blockset(a)
+    net = X.Mlp(seed=12)
+    # move away from the initialisation so every layer carries gradient
+    net.train(bs, 20, batch=len(bs), lr=3e-3, seed=1)
+    params = net.params().astype(np.float64)
+    loss, grad = net.loss_grad(bs)
+    feat, ahat, hmat, _ = (v.astype(np.float64) for v in bs.arrays())
+    lref, gref = B.loss_grad(params, feat, ahat, hmat)
+    print(f"loss gpu={loss:.6e} oracle={lref:.6e}", flush=True)
+    assert loss == pytest.approx(lref, rel=1e-2)
+    o = 0
+    for name, shp in B.SHAPES:
+        k = int(np.prod(shp))
+        g, gr = grad[o:o + k].astype(np.float64), gref[o:o + k]
+        rel = np.linalg.norm(g - gr) / max(np.linalg.norm(gr), 1e-30)
+        print(f"grad {name}: rel err {rel:.3e} (|g| = {np.linalg.norm(gr):.3e})", flush=True)
+        assert rel <= 5e-2, name
+        o += k
+
+
+def test_training_is_deterministic_and_descends(gpu):
+    sets = X.blockset([X.gen_fem2d(66, 1.0, 6, 1e3, 0.3, 20265003 + i) for i in range(2)])
+    n1, n2 = X.Mlp(seed=3), X.Mlp(seed=3)
+    h1 = n1.train(sets, 60, batch=1024, lr=2e-3, seed=9)
+    h2 = n2.train(sets, 60, batch=1024, lr=2e-3, seed=9)
+    np.testing.assert_array_equal(h1, h2)
+    np.testing.assert_array_equal(n1.params(), n2.params())
+    print("loss", h1[0], "->", h1[-1], flush=True)
+    assert np.mean(h1[-10:]) < 0.7 * np.mean(h1[:5])
+
+
+@pytest.mark.parametrize("which", ["c1", "fem"])
+def test_block_pcg_matches_oracle(gpu, which):
+    a = X.gen_diffusion2d(32, 32, 1.0, 20265001) if which == "c1" else X.gen_fem2d(34, 1.0, 4, 1e3, 0.3, 7)
+    tau = 1e-12 * float(a.b @ a.b)
+    m = X.Matrix(a, block_rows=8, keep_lo=True)
+    # exact block inverses (mode 1) and arbitrary SPD blocks from a network (mode 2)
+    net = X.Mlp(seed=21)
+    net_blocks = net.blocks(X.blockset(a)).astype(np.float64)
+    A = a.to_scipy()
+    nb = (a.n + 7) // 8
+    exact = np.zeros((nb, 8, 8))
+    for b in range(nb):
+        r0, k = 8 * b, min(8, a.n - 8 * b)
+        exact[b, :k, :k] = np.linalg.inv(A[r0:r0 + k, r0:r0 + k].toarray())
+        exact[b, k:, k:] = np.eye(8 - k)
+    for mode, blocks in ((1, exact), (2, net_blocks)):
+        s = X.Pcg(m, tau=tau, trace_cap=5000)
+        s.set_preconditioner(mode, None if mode == 1 else net_blocks.astype(np.float32))
+        s.set_rhs(a.b)
+        s.run()
+        st = s.status()
+        x_ref, it_ref, tr_ref = B.block_pcg(A, a.b, blocks.astype(np.float32).astype(np.float64), tau)
+        print(f"{which} mode {mode}: gpu iters {st['iterations']} oracle {it_ref}", flush=True)
+        assert st["status"] == X.CONVERGED
+        assert abs(st["iterations"] - it_ref) <= 1
+        x = s.x()
+        assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) <= 1e-5
+        k = min(len(tr_ref), len(s.trace()))
+        np.testing.assert_allclose(s.trace()[:k, 3], tr_ref[:k], rtol=1e-3)
+
+
+def test_trained_network_beats_jacobi_on_held_out(gpu):
+    """Train on FEM matrices of one seed split, compare on another."""
+    train = [X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265100 + i) for i in range(3)]
+    test = X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265200)
+    net = X.Mlp(seed=5)
+    hist = net.train(X.blockset(train), 400, batch=4096, lr=2e-3, seed=1)
+    print("train loss", hist[0], "->", hist[-1], flush=True)
+    tau = (1e-6 * np.linalg.norm(test.b)) ** 2
+    results = {}
+    m = X.Matrix(test, block_rows=8, keep_lo=True)
+    for name in ("jacobi", "exact_block", "learned"):
+        s = X.Pcg(m, tau=tau, stall_limit=0)
+        if name == "exact_block":
+            s.set_preconditioner(1)
+        elif name == "learned":
+            s.set_mlp_preconditioner(net)
+        s.set_rhs(test.b)
+        s.run()
+        st = s.status()
+        assert st["status"] == X.CONVERGED, (name, st)
+        results[name] = st["iterations"]
+    print("held-out iterations:", results, "gain learned vs jacobi: %.3f" %
+          (results["jacobi"] / results["learned"]), flush=True)
+    assert results["learned"] < results["jacobi"]
+
+
+def test_useless_blocks_fall_back_to_jacobi(gpu):
+    a = X.gen_fem2d(34, 1.0, 4, 1e3, 0.3, 8)
+    m = X.Matrix(a, block_rows=8)
+    nb = m.num_blocks()
+    rng = np.random.default_rng(0)
+    q = rng.standard_normal((nb, 8, 8))
+    bad = (q @ np.transpose(q, (0, 2, 1)) + 1e-3 * np.eye(8)) * 1e-2   # SPD, unrelated to A
+    s = X.Pcg(m, rel_tol=1e-6, stall_limit=20)
+    s.set_preconditioner(2, bad.astype(np.float32))
+    s.set_rhs(a.b)
+    s.run()
+    st = s.status()
+    print("fallback status", st, flush=True)
+    assert st["status"] == X.CONVERGED
+    assert st["fallback"] == 1
