@@ -236,13 +236,14 @@ int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches);
 /* ===================================================================== *
  * Learned 8x8 block-Jacobi preconditioner (new: the reference has none).
  *
- * Per aligned 8-row block b of a system: s_b = mean diagonal, Ahat = A_bb/s_b,
- * features (48) = upper triangle of Ahat (36) + off-block |a| row sums / s_b
- * (8) + 4 zeros; MLP 48 -> 64 -> 64 -> 48 (ReLU, ReLU, linear), outputs 0..35
- * = lower triangle of L; block M_b = (L L^T + 1e-3 I) / s_b (SPD by
- * construction).  Forward and input-gradient GEMMs are BF16 MFMA with FP32
- * accumulation; weight gradients FP32; Adam.  Training loss (block SPAI):
- * mean over blocks of ||Mh Ahat - I||_F^2 + ||Mh Ahat_off||_F^2.
+ * In the Jacobi-scaled matrix S = D^-1/2 A D^-1/2, per aligned 8-row block b
+ * of a system: Ahat = S_bb; features (48) = upper triangle of Ahat (36) +
+ * off-block |s| row sums (8) + 4 zeros; MLP 48 -> 64 -> 64 -> 48 (ReLU,
+ * ReLU, linear), outputs 0..35 = lower triangle of L; block
+ * M_b = D^-1/2 (L L^T + 1e-3 I) D^-1/2 (SPD by construction; L = I is plain
+ * Jacobi, the initialisation).  Forward and input-gradient GEMMs are BF16
+ * MFMA with FP32 accumulation; weight gradients FP32; Adam.  Training loss
+ * (block SPAI): mean over blocks of ||Mh Ahat - I||_F^2 + 0.5 ||Mh S_off||_F^2.
  * Parameters are one flat FP32 vector: W1[48][64], b1[64], W2[64][64],
  * b2[64], W3[64][48], b3[48] (row-major, input index first).
  * ===================================================================== */
@@ -257,7 +258,7 @@ int xp_mlp_set_params(xp_mlp* net, const float* params);
 int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, xp_blockset** out);
 int xp_blockset_append(xp_blockset* dst, const xp_blockset* src);
 int xp_blockset_size(const xp_blockset* set);
-/* feat: size x 48, ahat: size x 64, hmat (Ahat^2 + G): size x 64, scale: size. */
+/* feat: size x 48, ahat: size x 64, hmat (Ahat^2 + 0.5 G): size x 64, scale (D^-1/2): size x 8. */
 int xp_blockset_copy(const xp_blockset* set, float* feat, float* ahat, float* hmat, float* scale);
 void xp_blockset_free(xp_blockset* set);
 /* Mean loss over samples [first, first+count) and its gradient (flat). */

@@ -4,10 +4,11 @@ TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's checks, never by
 the product).  The reference has no learned preconditioner; this module is
 the independent CPU restatement its GPU kernels are checked against:
 
-  features_from_csr  -- block features, Ahat, H = Ahat^2 + G, scale
+  features_from_csr  -- block features of S = D^-1/2 A D^-1/2, Ahat = S_bb,
+                        H = Ahat^2 + LAMBDA G, D^-1/2 per row
   forward            -- the MLP 48 -> 64 -> 64 -> 48 (ReLU, ReLU, linear)
   loss_grad          -- mean block-SPAI loss and its analytic gradient
-  blocks_from_out    -- M_b = (L L^T + 1e-3 I) / s_b
+  blocks_from_out    -- M_b = D^-1/2 (L L^T + 1e-3 I) D^-1/2
   block_pcg          -- FP64 PCG with a block-diagonal preconditioner
 
 Pinning: loss_grad against central finite differences of loss (FP64,
@@ -25,6 +26,7 @@ SHAPES = [("W1", (N_IN, N_H)), ("b1", (N_H,)), ("W2", (N_H, N_H)), ("b2", (N_H,)
           ("W3", (N_H, N_OUT)), ("b3", (N_OUT,))]
 N_PARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 EPS_M = 1e-3
+LAMBDA = 0.5
 TRIL = [(r, c) for r in range(8) for c in range(r + 1)]
 
 
@@ -44,10 +46,17 @@ def pack(d):
 def features_from_csr(row_ptr, col_idx, values):
     n = len(row_ptr) - 1
     nb = (n + 7) // 8
+    dis = np.ones(n)
+    for r in range(n):
+        for o in range(row_ptr[r], row_ptr[r + 1]):
+            if col_idx[o] == r:
+                if values[o] > 0:
+                    dis[r] = 1.0 / np.sqrt(values[o])
+                break
     feat = np.zeros((nb, N_IN))
     ahat = np.zeros((nb, 8, 8))
     hmat = np.zeros((nb, 8, 8))
-    scale = np.ones(nb)
+    scale = np.ones((nb, 8))
     for b in range(nb):
         r0, m = 8 * b, min(8, n - 8 * b)
         a = np.zeros((8, 8))
@@ -56,26 +65,23 @@ def features_from_csr(row_ptr, col_idx, values):
         for i in range(m):
             for o in range(row_ptr[r0 + i], row_ptr[r0 + i + 1]):
                 c = col_idx[o] - r0
+                v = values[o] * dis[r0 + i] * dis[col_idx[o]]
                 if 0 <= c < m:
-                    a[i, c] += values[o]
+                    a[i, c] += v
                 else:
-                    offabs[i] += abs(values[o])
-                    off[i][col_idx[o]] = off[i].get(col_idx[o], 0.0) + values[o]
-        sc = np.trace(a[:m, :m]) / m
-        if not (sc > 0 and np.isfinite(sc)):
-            sc = 1.0
+                    offabs[i] += abs(v)
+                    off[i][col_idx[o]] = off[i].get(col_idx[o], 0.0) + v
         for i in range(m, 8):
-            a[i, i] = sc
-        scale[b] = sc
+            a[i, i] = 1.0
+        scale[b, :m] = dis[r0:r0 + m]
         g = np.zeros((8, 8))
         for i in range(8):
             for j in range(8):
                 g[i, j] = sum(v * off[j].get(k, 0.0) for k, v in off[i].items())
-        ah = a / sc
-        ahat[b] = ah
-        hmat[b] = ah @ ah + g / sc ** 2
-        feat[b, :36] = ah[np.triu_indices(8)]
-        feat[b, 36:44] = offabs / sc
+        ahat[b] = a
+        hmat[b] = a @ a + LAMBDA * g
+        feat[b, :36] = a[np.triu_indices(8)]
+        feat[b, 36:44] = offabs
     return feat, ahat, hmat, scale
 
 
@@ -128,7 +134,7 @@ def loss_grad(params, feat, ahat, hmat):
 def blocks_from_out(out, scale):
     L = l_from_out(out)
     M = L @ np.transpose(L, (0, 2, 1)) + EPS_M * np.eye(8)
-    return M / scale[:, None, None]
+    return M * scale[:, :, None] * scale[:, None, :]
 
 
 def block_pcg(A, b, blocks, tau, max_iters=100000):

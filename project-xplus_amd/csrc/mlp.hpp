@@ -10,14 +10,15 @@ namespace xp {
 constexpr int kMlpIn = 48;      // 36 block entries + 8 off-block sums + 4 zero pad
 constexpr int kMlpHidden = 64;
 constexpr int kMlpOut = 48;     // 36 entries of L + 12 unused
+constexpr double kMlpLambda = 0.5;  // weight of the off-block term of the training loss
 
 // Features and training targets of the aligned 8-row blocks of a matrix.
 struct BlockSet {
     int count = 0;
     std::vector<float> feat;   // count x kMlpIn
     std::vector<float> ahat;   // count x 64: A_bb / s_b
-    std::vector<float> hmat;   // count x 64: Ahat^2 + G
-    std::vector<float> scale;  // count: s_b
+    std::vector<float> hmat;   // count x 64: Ahat^2 + lambda G
+    std::vector<float> scale;  // count x 8: D^-1/2 of the block's rows (1 on padding rows)
     static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values);
     void append(const BlockSet& o);
 };

@@ -954,7 +954,10 @@ int xp_pcg_run(xp_pcg* s) {
                 s->use_block = false;
                 if (s->opts.use_graph) s->build_graph(s->opts.graph_iters);
                 HIPX(hipMemcpyAsync(s->x.p, s->x0s.p, s->x0s.n * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
-                s->reset_state();
+                // the Jacobi rerun runs without the stagnation watch
+                std::vector<xp::SysState> init = s->st_init;
+                for (auto& t : init) t.stall_limit = 0;
+                s->st.upload(init.data(), init.size(), s->stream);
                 s->run_loop(cap + s->graph_iters);
                 s->last_ms += first_ms;
                 s->use_block = true;

@@ -548,9 +548,9 @@ class BlockSet:
         f = np.empty(n * 48, np.float32)
         a = np.empty(n * 64, np.float32)
         hm = np.empty(n * 64, np.float32)
-        sc = np.empty(n, np.float32)
+        sc = np.empty(n * 8, np.float32)
         _check(_bs_copy(self.h, f, a, hm, sc))
-        return f.reshape(n, 48), a.reshape(n, 8, 8), hm.reshape(n, 8, 8), sc
+        return f.reshape(n, 48), a.reshape(n, 8, 8), hm.reshape(n, 8, 8), sc.reshape(n, 8)
 
     def close(self):
         if getattr(self, "h", None):

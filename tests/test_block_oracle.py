@@ -49,13 +49,13 @@ def test_oracle_gradient_matches_finite_differences():
 
 
 def test_oracle_loss_closed_form():
-    """tr(M H M^T) - 2 tr(M A) + 8 equals ||M A - I||^2 + ||M A_off||^2 written out."""
+    """tr(M H M^T) - 2 tr(M A) + 8 equals ||M A - I||^2 + LAMBDA ||M A_off||^2 written out."""
     a, (feat, ahat, hmat, scale) = small_set(4)
     params = random_params(5)
     out = B.forward(params, feat)
     L = B.l_from_out(out)
     M = L @ np.transpose(L, (0, 2, 1)) + 1e-3 * np.eye(8)
-    g = hmat - ahat @ ahat                     # = Ahat_off Ahat_off^T
+    g = hmat - ahat @ ahat                     # = LAMBDA S_off S_off^T
     direct = [np.linalg.norm(M[k] @ ahat[k] - np.eye(8)) ** 2 + np.trace(M[k] @ g[k] @ M[k].T)
               for k in range(len(M))]
     loss, _ = B.loss_grad(params, feat, ahat, hmat)
@@ -87,9 +87,11 @@ def test_features_hand_block():
                 va.append(4.0 if c == r else -1.0)
         rp.append(len(ci))
     feat, ahat, hmat, scale = B.features_from_csr(np.array(rp), np.array(ci), np.array(va))
-    assert scale[0] == 4.0
+    np.testing.assert_allclose(scale[0], 0.5)                     # D^-1/2 = 1/2
+    np.testing.assert_allclose(np.diag(ahat[0]), 1.0)               # unit diagonal
+    assert ahat[0][0, 1] == -0.25
     assert feat[0, 36 + 7] == 0.25 and feat[0, 36:43].sum() == 0.0   # only row 7 couples to row 8
-    g = hmat[0] - ahat[0] @ ahat[0]
+    g = (hmat[0] - ahat[0] @ ahat[0]) / B.LAMBDA
     assert g[7, 7] == pytest.approx(1 / 16) and abs(g).sum() == pytest.approx(1 / 16)
 
 
