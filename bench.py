@@ -69,6 +69,75 @@ def cpu_baseline(a, gpu_iters: int, target_s: float = 15.0) -> dict:
             "host": os.uname().machine}
 
 
+def train_network(X, args):
+    """Trains the block network on the training split of the FEM family
+    (grids from --train-grids, seeds 20266203 + i; the solved matrix uses
+    seed 20263203, so it is held out).  Outside the timed region."""
+    t0 = time.perf_counter()
+    sets = None
+    grids = [int(g) for g in args.train_grids.split(",") if g]
+    for i, g in enumerate(grids):
+        bs = X.BlockSet(X.gen_fem2d(g, 1.0, 20, 1e3, 0.3, 20266203 + i))
+        if sets is None:
+            sets = bs
+        else:
+            sets.append(bs)
+    feat_s = time.perf_counter() - t0
+    net = X.Mlp(seed=20266203)
+    t0 = time.perf_counter()
+    hist = net.train(sets, args.train_steps, batch=4096, lr=2e-3, seed=1)
+    train_s = time.perf_counter() - t0
+    info = {"grids": grids, "blocks": len(sets), "steps": args.train_steps, "batch": 4096,
+            "loss_first": float(np.mean(hist[:10])), "loss_last": float(np.mean(hist[-50:])),
+            "host_feature_s": round(feat_s, 2), "train_s": round(train_s, 2)}
+    log(f"trained: {info}")
+    return net, info
+
+
+def learned_solve(X, a, args, fused: bool, bnorm: float) -> dict:
+    """Learned 8x8 block-Jacobi refined PCG to true relative residual 1e-6 on
+    the same matrix.  Setup (device features + BF16 MFMA inference + block
+    layout) and solve are timed separately."""
+    net, tinfo = train_network(X, args)
+    ml = X.Matrix(a, sigma=256, keep_lo=True, block_rows=8, grid=args.grid)
+    tune = ml.autotune(fused=fused, tune_sigma=not args.no_tune_sigma, iters=64)
+    s = X.Pcg(ml, rel_tol=1e-6, fused=fused, stall_limit=0)
+    s.set_rhs(a.b, a.x0)
+    X.synchronize()
+    t0 = time.perf_counter()
+    s.set_mlp_preconditioner(net)
+    X.synchronize()
+    setup_ms = (time.perf_counter() - t0) * 1000.0
+    for w in range(args.warmup):
+        s.run_refined(1e-6, args.inner_rel, 50)
+    X.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rel, outer, inner_total = s.run_refined(1e-6, args.inner_rel, 50)
+    X.synchronize()
+    solve_ms = (time.perf_counter() - t0) * 1000.0 / args.steps
+    st = s.status()
+    out = {"setup_ms": round(setup_ms, 3), "solve_ms": round(solve_ms, 3), "total_ms": round(setup_ms + solve_ms, 3),
+           "outer_steps": outer, "inner_iterations": inner_total, "true_rel_residual": float(rel[0]),
+           "fallback": st["fallback"], "tune": tune, "training": tinfo}
+    log(f"learned refined to true 1e-6: {out}")
+    # the network against plain Jacobi in iterations on the same FP32 solve (recursive 1e-6)
+    it = {}
+    for name in ("jacobi", "learned"):
+        q = X.Pcg(ml, rel_tol=1e-6, fused=fused, stall_limit=0)
+        if name == "learned":
+            q.set_mlp_preconditioner(net)
+        q.set_rhs(a.b, a.x0)
+        q.run()
+        it[name] = q.status()["iterations"]
+        q.close()
+    out["fp32_iterations_to_recursive_1e-6"] = it
+    log(f"iterations to recursive 1e-6: {it}")
+    if not rel[0] <= 1e-6:
+        raise RuntimeError(f"learned refined solve missed 1e-6: {rel}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +148,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--inner-rel", type=float, default=1e-2)
     ap.add_argument("--no-tune-sigma", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-grids", default="512,700,900,1100")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -142,12 +214,14 @@ def main() -> int:
         print(f"refined solve missed 1e-6: {rel}", file=sys.stderr)
         return 1
 
+    learned = None if args.no_learned else learned_solve(X, a, args, fused, bnorm)
+
     roof = spmv_roofline(m, a)
     log(f"spmv: {roof}")
     cpu = None if args.no_cpu else cpu_baseline(a, st["iterations"])
     if cpu:
         log(f"cpu baseline: {cpu}")
-    wall_ms = ref_ms
+    wall_ms = learned["total_ms"] if learned else ref_ms
 
     out = {
         "metric": METRIC,
@@ -165,7 +239,9 @@ def main() -> int:
         "config": {"workload": f"thermal-FEM P1 {args.mesh}x{args.mesh} nodes, n={a.n}, nnz={a.nnz}, "
                                f"Jacobi-PCG to true relative residual 1e-6",
                    "n": a.n, "nnz": a.nnz,
-                   "method": "FP32 Jacobi-PCG inner (FP64 reductions) + FP64 outer residual/update",
+                   "method": ("learned 8x8 block-Jacobi (BF16 MFMA MLP) " if learned else "Jacobi ")
+                   + "FP32 PCG inner (FP64 reductions) + FP64 outer residual/update; value includes the "
+                     "learned preconditioner's device setup",
                    "inner_rel": args.inner_rel, "outer_steps": outer, "inner_iterations": inner_total,
                    "grid": info["grid"], "sigma": info["sigma"], "fused_iteration": fused,
                    "autotune": {"fused": tune_f, "unfused": tune_u, "ms": round(tune_ms, 1)}},
@@ -175,8 +251,10 @@ def main() -> int:
         "plain_iterations": st["iterations"],
         "plain_true_rel_residual": plain_true,
         "plain_ms_per_iteration": round(plain_ms / max(st["iterations"], 1), 5),
-        "learned_ms": None,
-        "speedup_learned_vs_plain": None,
+        "learned_ms": learned["total_ms"] if learned else None,
+        "learned": learned,
+        "speedup_learned_vs_plain": round(plain_ms / learned["total_ms"], 3) if learned else None,
+        "speedup_learned_vs_refined_jacobi": round(ref_ms / learned["total_ms"], 3) if learned else None,
         "spmv_hbm_gbs": roof["achieved"],
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -270,8 +270,15 @@ int xp_mlp_blocks(xp_mlp* net, const xp_blockset* set, float* blocks);
 int xp_mlp_train(xp_mlp* net, const xp_blockset* set, int steps, int batch, float lr, unsigned long long seed,
                  double* loss_history);
 /* Builds the blocks of the solver's matrix with the network and installs
- * them as its preconditioner (xp_pcg_set_preconditioner mode 2). */
+ * them as its preconditioner (like xp_pcg_set_preconditioner mode 2).  All
+ * on the device: features from the resident SELL matrix, BF16 MFMA
+ * inference, blocks written in solve order; padding rows stay identity.
+ * The matrix must be built with block_rows = 8. */
 int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net);
+/* The device-computed features of a block_rows = 8 matrix, in natural block
+ * order (xp_matrix_num_blocks of them): feat nblocks x 48, scale nblocks x 8
+ * (the same quantities as xp_blockset_copy's feat and scale). */
+int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale);
 
 /* ===================================================================== *
  * Host helpers: loader and generators (no GPU needed).

@@ -740,6 +740,79 @@ int blocks_for(size_t n, int bs) { return static_cast<int>((n + static_cast<size
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Learned block preconditioner: features of the aligned 8-row solve-order
+// blocks, read straight from the SELL matrix (block_rows = 8 keeps every
+// natural 8-row group together as 8 consecutive solve rows, so a solve-order
+// block is a natural block).  Same definitions as the host features
+// (BlockSet::from_csr): S = D^-1/2 A D^-1/2, the 36 upper entries of S_bb,
+// the 8 off-block sums of |s|, 4 zeros.
+
+__device__ __forceinline__ double sell_value(const SellView& A, size_t e) {
+    if (A.val64 != nullptr) return A.val64[e];
+    return static_cast<double>(A.val[e]) + (A.val_lo != nullptr ? static_cast<double>(A.val_lo[e]) : 0.0);
+}
+
+// dis[row] = 1/sqrt(a_rr) from the stored diagonal (slot 0), 1 where it is
+// not positive (rows without a diagonal store 0 there).
+__global__ void k_inv_sqrt_diag(SellView A, double* __restrict__ dis) {
+    const int row = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (row >= A.n_pad) return;
+    const int sl = row / kC;
+    const double d = sell_value(A, static_cast<size_t>(A.slice_ptr[sl]) * kC + static_cast<size_t>(row % kC));
+    dis[row] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+}
+
+// One thread per solve row: its row of S_bb (register-resident, selected by
+// column without dynamic indexing) and its off-block sum.  Thread r of a
+// block writes the upper-triangle entries (r, c >= r) of the feature row.
+template <bool IDX16>
+__global__ void k_block_features(SellView A, const double* __restrict__ dis, float* __restrict__ feat,
+                                 float* __restrict__ scale) {
+    const int row = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (row >= A.n_pad) return;
+    const int sl = row / kC, lane = row % kC;
+    const int j0 = A.slice_ptr[sl], w = A.slice_ptr[sl + 1] - j0, base = A.slice_base[sl];
+    const double di = dis[row];
+    const int blk0 = row & ~7;
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double off = 0.0;
+    for (int k = 0; k < w; ++k) {
+        const size_t e = static_cast<size_t>(j0 + k) * kC + static_cast<size_t>(lane);
+        const int c = k == 0 ? row : (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]);
+        const double v = sell_value(A, e) * di * dis[c];
+        if ((c & ~7) == blk0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) a[q] += (c & 7) == q ? v : 0.0;
+        } else {
+            off += fabs(v);
+        }
+    }
+    const int r = row & 7;
+    float* f = feat + static_cast<size_t>(row >> 3) * 48;
+    const int o = 8 * r - r * (r - 1) / 2 - r;  // feature index of (r, c) is o + c
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (c >= r) f[o + c] = static_cast<float>(a[c]);
+    f[36 + r] = static_cast<float>(off);
+    if (r < 4) f[44 + r] = 0.0f;
+    scale[row] = static_cast<float>(di);
+}
+
+// Rows and columns of padding rows (perm < 0) become those of the identity,
+// so padding rows stay decoupled (r = z = 0 there).  One thread per solve row
+// rewrites its own block row.
+__global__ void k_mask_padding_blocks(int n_pad, const int* __restrict__ perm, float* __restrict__ blocks) {
+    const int row = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (row >= n_pad) return;
+    const int blk0 = row & ~7, j = row & 7;
+    const bool pad_row = perm[row] < 0;
+    float* b = blocks + static_cast<size_t>(row) * 8;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        if (pad_row || perm[blk0 + c] < 0) b[c] = c == j ? 1.0f : 0.0f;
+}
+
 #define XP_DISPATCH_SELL(A, KERNEL, ...)                                                            \
     do {                                                                                        \
         if ((A).idx16) {                                                                        \
@@ -872,6 +945,20 @@ void launch_fill_f32(size_t n, float v, float* dst, hipStream_t s) {
 void launch_fill_u32(size_t n, unsigned v, unsigned* dst, hipStream_t s) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_fill<unsigned>, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, v, dst);
+}
+
+void launch_block_features(const SellView& A, double* dis, float* feat, float* scale, hipStream_t s) {
+    const int g = static_cast<int>(blocks_for(static_cast<size_t>(A.n_pad), 256));
+    hipLaunchKernelGGL(k_inv_sqrt_diag, dim3(g), dim3(256), 0, s, A, dis);
+    if (A.idx16)
+        hipLaunchKernelGGL(k_block_features<true>, dim3(g), dim3(256), 0, s, A, dis, feat, scale);
+    else
+        hipLaunchKernelGGL(k_block_features<false>, dim3(g), dim3(256), 0, s, A, dis, feat, scale);
+}
+
+void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_mask_padding_blocks, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad,
+                       perm, blocks);
 }
 
 }  // namespace xp

@@ -80,4 +80,12 @@ void launch_convert_f2d(int n, const float* src, double* dst, hipStream_t s);
 void launch_fill_f32(size_t n, float v, float* dst, hipStream_t s);
 void launch_fill_u32(size_t n, unsigned v, unsigned* dst, hipStream_t s);
 
+// ---- learned block preconditioner ----
+// Block features of the solve-order 8-row blocks of a block_rows = 8 matrix:
+// feat [n_pad/8 x 48] (upper S_bb, off-block |s| sums, 4 zeros), scale
+// [n_pad] = D^-1/2 per solve row; dis [n_pad] is FP64 scratch.
+void launch_block_features(const SellView& A, double* dis, float* feat, float* scale, hipStream_t s);
+// Identity rows/columns for the padding rows (perm < 0) of solve-order 8x8 blocks.
+void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s);
+
 }  // namespace xp

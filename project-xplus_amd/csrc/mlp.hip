@@ -245,6 +245,14 @@ __global__ void k_mlp_blocks(const float* __restrict__ out, const float* __restr
         }
 }
 
+// hist[0] = sum of part[0..n) in index order (one thread: n is a few dozen).
+__global__ void k_sum_loss(const double* __restrict__ part, int n, double inv_count, double* __restrict__ hist) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc = 0.0;
+    for (int i = 0; i < n; ++i) acc += part[i];
+    hist[0] = acc * inv_count;
+}
+
 int nblk(size_t n, int b) { return static_cast<int>((n + static_cast<size_t>(b) - 1) / static_cast<size_t>(b)); }
 
 void check(hipError_t e, const char* what) {
@@ -345,7 +353,8 @@ struct Mlp::Dev {
     Buf<uint16_t> w1t, w1k, w2t, w2k, w3t, w3k;
     Buf<float> xf, h1, h2, out, dout, dh2, dh1, ahat, hmat, scale, part, blocks, dummy;
     Buf<uint16_t> xb, h1b, h2b, doutb, dh2b, dh1b;
-    Buf<double> loss_part;
+    Buf<double> loss_part, hist;
+    Buf<float> set_feat, set_ahat, set_hmat;   // device-resident training set (+64 zero rows)
     hipStream_t s = nullptr;
 };
 
@@ -433,11 +442,8 @@ const std::vector<float>& Mlp::params() {
 }
 
 // Uploads samples [first, first + count) of data (rows padded to a multiple of 64).
-int Mlp::stage(const BlockSet& data, int first, int count) {
+void Mlp::ensure_rows(int rows) {
     Dev& d = *d_;
-    if (first < 0 || count <= 0 || first + count > data.count) throw std::runtime_error("sample range out of bounds");
-    const int rows = (count + 63) / 64 * 64;
-    d.xf.ensure(static_cast<size_t>(rows) * kMlpIn);
     d.xb.ensure(static_cast<size_t>(rows) * kMlpIn);
     d.h1.ensure(static_cast<size_t>(rows) * kMlpHidden); d.h1b.ensure(static_cast<size_t>(rows) * kMlpHidden);
     d.h2.ensure(static_cast<size_t>(rows) * kMlpHidden); d.h2b.ensure(static_cast<size_t>(rows) * kMlpHidden);
@@ -445,10 +451,18 @@ int Mlp::stage(const BlockSet& data, int first, int count) {
     d.dout.ensure(static_cast<size_t>(rows) * kMlpOut); d.doutb.ensure(static_cast<size_t>(rows) * kMlpOut);
     d.dh2.ensure(static_cast<size_t>(rows) * kMlpHidden); d.dh2b.ensure(static_cast<size_t>(rows) * kMlpHidden);
     d.dh1.ensure(static_cast<size_t>(rows) * kMlpHidden); d.dh1b.ensure(static_cast<size_t>(rows) * kMlpHidden);
+    d.loss_part.ensure(static_cast<size_t>(nblk(static_cast<size_t>(rows), 256)));
+}
+
+int Mlp::stage(const BlockSet& data, int first, int count) {
+    Dev& d = *d_;
+    if (first < 0 || count <= 0 || first + count > data.count) throw std::runtime_error("sample range out of bounds");
+    const int rows = (count + 63) / 64 * 64;
+    ensure_rows(rows);
+    d.xf.ensure(static_cast<size_t>(rows) * kMlpIn);
     d.ahat.ensure(static_cast<size_t>(rows) * 64);
     d.hmat.ensure(static_cast<size_t>(rows) * 64);
     d.scale.ensure(static_cast<size_t>(rows) * 8);
-    d.loss_part.ensure(static_cast<size_t>(nblk(static_cast<size_t>(rows), 256)));
     check(hipMemsetAsync(d.xf.p, 0, static_cast<size_t>(rows) * kMlpIn * sizeof(float), d.s), "memset");
     check(hipMemsetAsync(d.ahat.p, 0, static_cast<size_t>(rows) * 64 * sizeof(float), d.s), "memset");
     check(hipMemsetAsync(d.hmat.p, 0, static_cast<size_t>(rows) * 64 * sizeof(float), d.s), "memset");
@@ -476,10 +490,10 @@ void Mlp::forward_staged(int rows) {
                        d.out.p, nullptr, rows);
 }
 
-double Mlp::loss_grad_staged(int rows, int count) {
+void Mlp::backward_staged(int rows, int count, const float* xf, const float* ahat, const float* hmat) {
     Dev& d = *d_;
     const int lb = nblk(static_cast<size_t>(count), 256);
-    hipLaunchKernelGGL(k_mlp_loss, dim3(lb), dim3(256), 0, d.s, d.out.p, d.ahat.p, d.hmat.p, count,
+    hipLaunchKernelGGL(k_mlp_loss, dim3(lb), dim3(256), 0, d.s, d.out.p, ahat, hmat, count,
                        1.0f / static_cast<float>(count), d.dout.p, d.doutb.p, d.loss_part.p);
     // zero the gradient of the padding rows (loss kernel writes rows < count only)
     if (rows > count) {
@@ -502,8 +516,14 @@ double Mlp::loss_grad_staged(int rows, int count) {
     hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(l3, 256)), b, 0, d.s, d.part.p, chunks, l3, d.g.p + kOffW3);
     hipLaunchKernelGGL((k_mlp_dw<kMlpHidden, kMlpHidden>), dim3(chunks), b, 0, d.s, d.h1.p, d.dh2.p, rows, d.part.p);
     hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(l2, 256)), b, 0, d.s, d.part.p, chunks, l2, d.g.p + kOffW2);
-    hipLaunchKernelGGL((k_mlp_dw<kMlpIn, kMlpHidden>), dim3(chunks), b, 0, d.s, d.xf.p, d.dh1.p, rows, d.part.p);
+    hipLaunchKernelGGL((k_mlp_dw<kMlpIn, kMlpHidden>), dim3(chunks), b, 0, d.s, xf, d.dh1.p, rows, d.part.p);
     hipLaunchKernelGGL(k_mlp_reduce, dim3(nblk(l1, 256)), b, 0, d.s, d.part.p, chunks, l1, d.g.p + kOffW1);
+}
+
+double Mlp::loss_grad_staged(int rows, int count) {
+    Dev& d = *d_;
+    backward_staged(rows, count, d.xf.p, d.ahat.p, d.hmat.p);
+    const int lb = nblk(static_cast<size_t>(count), 256);
     std::vector<double> lp(static_cast<size_t>(lb));
     check(hipMemcpyAsync(lp.data(), d.loss_part.p, lp.size() * sizeof(double), hipMemcpyDeviceToHost, d.s), "download");
     check(hipStreamSynchronize(d.s), "sync");
@@ -550,11 +570,41 @@ std::vector<float> Mlp::blocks(const BlockSet& data) {
     return out;
 }
 
+void Mlp::blocks_device(const float* feat, const float* scale, int count, float* blocks) {
+    Dev& d = *d_;
+    if (count <= 0) return;
+    const int rows = (count + 63) / 64 * 64;
+    ensure_rows(rows);
+    hipLaunchKernelGGL(k_cast_rows, dim3(nblk(static_cast<size_t>(rows) * kMlpIn, 256)), dim3(256), 0, d.s, feat,
+                       static_cast<size_t>(rows) * kMlpIn, d.xb.p);
+    forward_staged(rows);
+    hipLaunchKernelGGL(k_mlp_blocks, dim3(nblk(static_cast<size_t>(count), 256)), dim3(256), 0, d.s, d.out.p, scale,
+                       count, blocks);
+    check(hipStreamSynchronize(d.s), "sync");
+}
+
+// The training set is uploaded once and stays resident in HBM; each step
+// casts its batch to BF16 in place, runs forward, loss, backward and Adam,
+// and records the batch loss on the device (one download at the end).
 std::vector<double> Mlp::train(const BlockSet& data, int steps, int batch, float lr, uint64_t seed) {
     Dev& d = *d_;
-    std::vector<double> hist;
     if (data.count <= 0) throw std::runtime_error("empty training set");
+    if (steps <= 0) return {};
     batch = std::max(64, std::min(batch, data.count));
+    const size_t cnt = static_cast<size_t>(data.count), pad = cnt + 64;
+    d.set_feat.ensure(pad * kMlpIn);
+    d.set_ahat.ensure(pad * 64);
+    d.set_hmat.ensure(pad * 64);
+    check(hipMemsetAsync(d.set_feat.p + cnt * kMlpIn, 0, 64 * kMlpIn * sizeof(float), d.s), "memset");
+    check(hipMemsetAsync(d.set_ahat.p + cnt * 64, 0, 64 * 64 * sizeof(float), d.s), "memset");
+    check(hipMemsetAsync(d.set_hmat.p + cnt * 64, 0, 64 * 64 * sizeof(float), d.s), "memset");
+    check(hipMemcpyAsync(d.set_feat.p, data.feat.data(), cnt * kMlpIn * sizeof(float), hipMemcpyHostToDevice, d.s),
+          "upload");
+    check(hipMemcpyAsync(d.set_ahat.p, data.ahat.data(), cnt * 64 * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
+    check(hipMemcpyAsync(d.set_hmat.p, data.hmat.data(), cnt * 64 * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
+    d.hist.ensure(static_cast<size_t>(steps));
+    const int rows = (batch + 63) / 64 * 64;
+    ensure_rows(rows);
     // Deterministic shuffled order of contiguous batches: batch starts drawn
     // from a SplitMix64 stream over [0, count - batch].
     uint64_t st = seed ^ 0xBB67AE8584CAA73Bull;
@@ -565,17 +615,24 @@ std::vector<double> Mlp::train(const BlockSet& data, int steps, int batch, float
         return z ^ (z >> 31);
     };
     const int np = num_params();
+    const int lb = nblk(static_cast<size_t>(batch), 256);
     for (int it = 1; it <= steps; ++it) {
-        const int first = data.count == batch ? 0 : static_cast<int>(next() % static_cast<uint64_t>(data.count - batch + 1));
-        const int rows = stage(data, first, batch);
+        const size_t first =
+            data.count == batch ? 0 : static_cast<size_t>(next() % static_cast<uint64_t>(data.count - batch + 1));
+        const float* xf = d.set_feat.p + first * kMlpIn;  // rows past the batch are real samples or the zero pad
+        hipLaunchKernelGGL(k_cast_rows, dim3(nblk(static_cast<size_t>(rows) * kMlpIn, 256)), dim3(256), 0, d.s, xf,
+                           static_cast<size_t>(rows) * kMlpIn, d.xb.p);
         forward_staged(rows);
-        hist.push_back(loss_grad_staged(rows, batch));
+        backward_staged(rows, batch, xf, d.set_ahat.p + first * 64, d.set_hmat.p + first * 64);
+        hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, d.s, d.loss_part.p, lb, 1.0 / batch, d.hist.p + (it - 1));
         const float c1 = 1.0f - std::pow(0.9f, static_cast<float>(it));
         const float c2 = 1.0f - std::pow(0.999f, static_cast<float>(it));
         hipLaunchKernelGGL(k_adam, dim3(nblk(static_cast<size_t>(np), 256)), dim3(256), 0, d.s, d.w.p, d.g.p, d.m.p, d.v.p,
                            np, lr, 0.9f, 0.999f, 1e-8f, c1, c2);
         cast_weights();
     }
+    std::vector<double> hist(static_cast<size_t>(steps));
+    check(hipMemcpyAsync(hist.data(), d.hist.p, hist.size() * sizeof(double), hipMemcpyDeviceToHost, d.s), "download");
     check(hipStreamSynchronize(d.s), "sync");
     return hist;
 }

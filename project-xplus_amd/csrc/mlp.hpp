@@ -36,6 +36,10 @@ class Mlp {
     std::vector<double> train(const BlockSet& data, int steps, int batch, float lr, uint64_t seed);
     // M_b = (L L^T + 1e-3 I) / s_b for every block of data (count x 64, row-major).
     std::vector<float> blocks(const BlockSet& data);
+    // Device-side inference: feat [rows x kMlpIn] (rows = count rounded up to
+    // 64, padding rows finite), scale [count x 8] -> blocks [count x 64].
+    // All device pointers; synchronous.
+    void blocks_device(const float* feat, const float* scale, int count, float* blocks);
 
   private:
     struct Dev;
@@ -43,8 +47,10 @@ class Mlp {
     std::vector<float> params_;
     void upload_params();
     void cast_weights();
+    void ensure_rows(int rows);
     int stage(const BlockSet& data, int first, int count);
     void forward_staged(int rows);
+    void backward_staged(int rows, int count, const float* xf, const float* ahat, const float* hmat);
     double loss_grad_staged(int rows, int count);
 };
 

@@ -1116,6 +1116,7 @@ int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_ou
         if (!(outer_rel > 0.0) || !(inner_rel > 0.0) || max_outer <= 0) throw std::runtime_error("bad tolerances");
         xp_pcg& S = *s;
         xp_matrix& m = *S.m;
+        S.fallback = 0;
         ensure_f64(S);
         const std::vector<double> bn = system_bnorms(S);
         HIPX(hipEventRecord(S.ev0, S.stream));
@@ -1147,10 +1148,28 @@ int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_ou
             HIPX(hipMemcpyAsync(S.st_host, S.st.p, static_cast<size_t>(m.nsys) * sizeof(xp::SysState),
                                 hipMemcpyDeviceToHost, S.stream));
             HIPX(hipStreamSynchronize(S.stream));
-            for (int k = 0; k < m.nsys; ++k) iters[static_cast<size_t>(k)] += S.st_host[k].iters;
+            bool failed = false;
+            for (int k = 0; k < m.nsys; ++k) {
+                iters[static_cast<size_t>(k)] += S.st_host[k].iters;
+                failed = failed || S.st_host[k].status == xp::kStatusStagnation ||
+                         S.st_host[k].status == xp::kStatusBreakdown;
+            }
+            ++outer;
+            if (failed && S.use_block) {
+                // Stagnation fallback inside refinement: the block preconditioner's
+                // correction is discarded (it may hold non-finite values after a
+                // breakdown) and the remaining outer steps use plain Jacobi.
+                S.use_block = false;
+                S.fallback = 1;
+                if (S.opts.use_graph) S.build_graph(S.opts.graph_iters);
+                continue;
+            }
             xp::launch_axpy_f64_f32(m.n_pad, S.x.p, S.x64.p, S.stream);
             rn = residual_norms(S, S.x64.p, S.r32.p);
-            ++outer;
+        }
+        if (S.fallback != 0 && !S.use_block) {
+            S.use_block = true;
+            if (S.opts.use_graph) S.build_graph(S.opts.graph_iters);
         }
         HIPX(hipEventRecord(S.ev1, S.stream));
         HIPX(hipEventSynchronize(S.ev1));
@@ -1464,23 +1483,37 @@ std::vector<float> exact_block_inverses(const xp_matrix& m) {
     return out;
 }
 
+// Natural block id (systems' blocks numbered consecutively) of solve-order
+// block q, or -1 for an all-padding block.
+int natural_block_of(const xp_matrix& m, const std::vector<int>& block_first, int q) {
+    int g = -1;
+    for (int j = 0; j < 8 && g < 0; ++j) {
+        const int pj = m.perm[static_cast<size_t>(q) * 8 + j];
+        g = pj >= 0 ? pj - j : -1;
+    }
+    if (g < 0) return -1;
+    int k = 0;
+    while (k + 1 < m.nsys && g >= m.sys_nat_begin[static_cast<size_t>(k) + 1]) ++k;
+    return block_first[static_cast<size_t>(k)] + (g - m.sys_nat_begin[static_cast<size_t>(k)]) / 8;
+}
+
+std::vector<int> system_block_first(const xp_matrix& m) {
+    std::vector<int> block_first(m.h_ns.size() + 1, 0);
+    for (size_t k = 0; k < m.h_ns.size(); ++k) block_first[k + 1] = block_first[k] + (m.h_ns[k] + 7) / 8;
+    return block_first;
+}
+
 // Natural-order blocks -> solve-order block array (padding rows: identity).
 std::vector<float> arrange_blocks(const xp_matrix& m, const float* nat) {
     std::vector<float> out(static_cast<size_t>(m.n_pad) * 8, 0.0f);
-    std::vector<int> block_first(m.h_ns.size() + 1, 0);
-    for (size_t k = 0; k < m.h_ns.size(); ++k) block_first[k + 1] = block_first[k] + (m.h_ns[k] + 7) / 8;
+    const std::vector<int> block_first = system_block_first(m);
     for (int q = 0; q < m.n_pad / 8; ++q) {
         float* dst = out.data() + static_cast<size_t>(q) * 64;
-        int g = -1;
-        for (int j = 0; j < 8 && g < 0; ++j) g = m.perm[static_cast<size_t>(q) * 8 + j] >= 0 ? m.perm[static_cast<size_t>(q) * 8 + j] - j : -1;
-        if (g < 0) {  // an all-padding block
+        const int bid = natural_block_of(m, block_first, q);
+        if (bid < 0) {  // an all-padding block
             for (int j = 0; j < 8; ++j) dst[j * 8 + j] = 1.0f;
             continue;
         }
-        int k = 0;
-        while (k + 1 < m.nsys && g >= m.sys_nat_begin[static_cast<size_t>(k) + 1]) ++k;
-        const int local = g - m.sys_nat_begin[static_cast<size_t>(k)];
-        const int bid = block_first[static_cast<size_t>(k)] + local / 8;
         const float* src = nat + static_cast<size_t>(bid) * 64;
         for (int j = 0; j < 8; ++j) {
             const bool real_j = m.perm[static_cast<size_t>(q) * 8 + j] >= 0;
@@ -1491,6 +1524,28 @@ std::vector<float> arrange_blocks(const xp_matrix& m, const float* nat) {
         }
     }
     return out;
+}
+
+// Solve-order block features on the device (feat: rows x 48 with rows the
+// block count rounded up to 64, zero padding rows; scale: n_pad).
+struct DeviceFeatures {
+    int blocks = 0, rows = 0;
+    DevBuf<float> feat, scale;
+};
+
+DeviceFeatures device_block_features(const xp_matrix& m, hipStream_t stream) {
+    if (m.block_rows != 8) throw std::runtime_error("block features need a matrix built with block_rows = 8");
+    DeviceFeatures f;
+    f.blocks = m.n_pad / 8;
+    f.rows = (f.blocks + 63) / 64 * 64;
+    f.feat.alloc(static_cast<size_t>(f.rows) * 48);
+    f.scale.alloc(static_cast<size_t>(m.n_pad));
+    DevBuf<double> dis(static_cast<size_t>(m.n_pad));
+    xp::launch_fill_f32(static_cast<size_t>(f.rows - f.blocks) * 48, 0.0f, f.feat.p + static_cast<size_t>(f.blocks) * 48,
+                        stream);
+    xp::launch_block_features(m.view(), dis.p, f.feat.p, f.scale.p, stream);
+    HIPX(hipStreamSynchronize(stream));
+    return f;
 }
 
 void set_preconditioner(xp_pcg& s, int mode, const float* blocks) {
@@ -1634,17 +1689,36 @@ int xp_mlp_train(xp_mlp* net, const xp_blockset* set, int steps, int batch, floa
 int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net) {
     return guarded([&] {
         if (s == nullptr || net == nullptr) throw std::runtime_error("null argument");
-        const xp_matrix& m = *s->m;
-        xp::BlockSet all;
-        size_t rp_off = 0, nz_off = 0;
-        for (int nk : m.h_ns) {
-            const int* rp = m.h_row_ptr.data() + rp_off;
-            all.append(xp::BlockSet::from_csr(nk, rp, m.h_col_idx.data() + nz_off, m.h_values.data() + nz_off));
-            rp_off += static_cast<size_t>(nk) + 1;
-            nz_off += static_cast<size_t>(rp[nk]);
+        xp_matrix& m = *s->m;
+        // features from the resident SELL matrix, BF16 MFMA inference, blocks
+        // written in solve order: nothing crosses PCIe
+        DeviceFeatures f = device_block_features(m, s->stream);
+        const size_t nb = static_cast<size_t>(m.n_pad) * 8;
+        if (s->blockM.n != nb) s->blockM.alloc(nb);
+        net->net.blocks_device(f.feat.p, f.scale.p, f.blocks, s->blockM.p);
+        xp::launch_mask_padding_blocks(m.n_pad, m.d_perm.p, s->blockM.p, s->stream);
+        HIPX(hipStreamSynchronize(s->stream));
+        s->use_block = true;
+        if (s->opts.use_graph) s->build_graph(s->opts.graph_iters);
+    });
+}
+
+int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale) {
+    return guarded([&] {
+        if (m == nullptr || feat == nullptr || scale == nullptr) throw std::runtime_error("null argument");
+        hipStream_t st = device().stream;
+        DeviceFeatures f = device_block_features(*m, st);
+        std::vector<float> hf(static_cast<size_t>(f.blocks) * 48), hs(static_cast<size_t>(m->n_pad));
+        HIPX(hipMemcpyAsync(hf.data(), f.feat.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPX(hipMemcpyAsync(hs.data(), f.scale.p, hs.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+        const std::vector<int> block_first = system_block_first(*m);
+        for (int q = 0; q < f.blocks; ++q) {
+            const int bid = natural_block_of(*m, block_first, q);
+            if (bid < 0) continue;
+            std::memcpy(feat + static_cast<size_t>(bid) * 48, hf.data() + static_cast<size_t>(q) * 48, 48 * sizeof(float));
+            std::memcpy(scale + static_cast<size_t>(bid) * 8, hs.data() + static_cast<size_t>(q) * 8, 8 * sizeof(float));
         }
-        const std::vector<float> blocks = net->net.blocks(all);
-        set_preconditioner(*s, 2, blocks.data());
     });
 }
 

@@ -128,6 +128,7 @@ _p_timing = _sig("xp_pcg_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), C.P
 _p_set_precond = _sig("xp_pcg_set_preconditioner", C.c_int, [_vp, C.c_int, _vp])
 _p_set_mlp = _sig("xp_pcg_set_mlp_preconditioner", C.c_int, [_vp, _vp])
 _m_num_blocks = _sig("xp_matrix_num_blocks", C.c_int, [_vp])
+_m_block_features = _sig("xp_matrix_block_features", C.c_int, [_vp, _f32p, _f32p])
 _mlp_create = _sig("xp_mlp_create", C.c_int, [C.c_ulonglong, C.POINTER(_vp)])
 _mlp_destroy = _sig("xp_mlp_destroy", None, [_vp])
 _mlp_nparams = _sig("xp_mlp_num_params", C.c_int, [])
@@ -389,6 +390,14 @@ class Matrix:
 
     def num_blocks(self) -> int:
         return _m_num_blocks(self.h)
+
+    def block_features(self):
+        """Device-computed block features (natural block order): feat nb x 48, scale nb x 8."""
+        nb = self.num_blocks()
+        f = np.empty(nb * 48, np.float32)
+        sc = np.empty(nb * 8, np.float32)
+        _check(_m_block_features(self.h, f, sc))
+        return f.reshape(nb, 48), sc.reshape(nb, 8)
 
     def info(self) -> dict:
         i = MatrixInfo()

@@ -171,6 +171,57 @@ def test_block_pcg_matches_oracle(gpu, which):
         np.testing.assert_allclose(tr[:k, 3][early], tr_ref[:k][early], rtol=1e-4)
 
 
+@pytest.mark.parametrize("case", ["c1", "fem", "ragged", "batch"])
+def test_device_features_match_oracle(gpu, case):
+    """Features computed on the GPU from the SELL matrix (solve order, mapped
+    back to natural block order) against the oracle's features_from_csr."""
+    if case == "c1":
+        systems = [X.gen_diffusion2d(32, 32, 1.0, 20265001)]
+    elif case == "fem":
+        systems = [X.gen_fem2d(40, 1.0, 4, 1e3, 0.3, 31)]
+    elif case == "ragged":                                  # n = 143: a partial last block
+        systems = [X.gen_diffusion2d(13, 11, 1.0, 7)]
+    else:                                                    # two systems of uneven size
+        systems = [X.gen_diffusion2d(13, 11, 1.0, 8), X.gen_fem2d(30, 1.0, 3, 1e3, 0.3, 9)]
+    m = X.Matrix(systems, sigma=256, block_rows=8, keep_lo=True)
+    feat, scale = m.block_features()
+    ref_f, ref_s = [], []
+    for a in systems:
+        f, _, _, sc = B.features_from_csr(a.row_ptr, a.col_idx, a.values)
+        ref_f.append(f)
+        ref_s.append(sc)
+    ref_f, ref_s = np.concatenate(ref_f), np.concatenate(ref_s)
+    assert feat.shape == ref_f.shape
+    # both are FP32 roundings of FP64 values (hi+lo pairs on the GPU)
+    np.testing.assert_allclose(feat, ref_f, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(scale, ref_s, rtol=1e-6)
+
+
+def test_device_mlp_preconditioner_matches_host_blocks(gpu):
+    """set_mlp_preconditioner (device features + inference + solve-order
+    blocks) against mode 2 with host-built blocks of the same network."""
+    train = X.gen_fem2d(66, 1.0, 6, 1e3, 0.3, 20265300)
+    a = X.gen_fem2d(61, 1.0, 6, 1e3, 0.3, 20265301)          # n = 3481: a partial last block
+    net = X.Mlp(seed=4)
+    net.train(X.blockset(train), 100, batch=512, lr=2e-3, seed=2)
+    m = X.Matrix(a, block_rows=8, keep_lo=True)
+    res = []
+    for how in ("device", "host"):
+        s = X.Pcg(m, rel_tol=1e-6, stall_limit=0)
+        if how == "device":
+            s.set_mlp_preconditioner(net)
+        else:
+            s.set_preconditioner(2, net.blocks(X.blockset(a)))
+        s.set_rhs(a.b)
+        s.run()
+        res.append((s.status(), s.x()))
+    (s1, x1), (s2, x2) = res
+    print("device", s1["iterations"], "host", s2["iterations"], flush=True)
+    assert s1["status"] == s2["status"] == X.CONVERGED
+    assert abs(s1["iterations"] - s2["iterations"]) <= 1
+    assert np.linalg.norm(x1 - x2) <= 1e-5 * np.linalg.norm(x2)
+
+
 def test_trained_network_beats_jacobi_on_held_out(gpu):
     """Train on FEM matrices of one seed split, compare on another."""
     train = [X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265100 + i) for i in range(3)]
