@@ -113,7 +113,8 @@ int xp_matrix_set_grid(xp_matrix* m, int grid);
 int xp_matrix_set_variant(xp_matrix* m, int spmv_variant);
 
 /* Autotuner: times `iters` Jacobi-PCG iterations (the real graph-captured
- * iteration, 3-kernel or fused per `fused`) for every candidate of SpMV
+ * iteration: 3-kernel for fused = 0, fused for 1, the accurate iteration of
+ * xp_solve_opts.accurate for 2, which needs keep_lo) for every candidate of SpMV
  * inner loop x workgroup count (x sort window sigma when tune_sigma != 0,
  * which re-sorts the matrix and so must run before any solver is created on
  * it), keeps the fastest and returns it.  Results are cached per matrix
@@ -184,6 +185,11 @@ typedef struct {
                           iterations without halving the best r^T r */
     int fused;         /* 1: two kernels per iteration (p update folded into the
                           SpMV via A p = A z + beta A p_old); 0: three kernels */
+    int accurate;      /* 1: FP64-accumulated SpMV with hi+lo values (needs
+                          keep_lo or keep64), FP64 p, A p and x, FP32 r and z;
+                          two kernels per iteration.  The recursive residual
+                          then tracks b - A x, so ill-conditioned systems reach
+                          a true 1e-6 without refinement.  0: FP32 iteration. */
 } xp_solve_opts;
 
 typedef struct {
@@ -212,6 +218,17 @@ int xp_pcg_run_fixed(xp_pcg* s, int iters);
  * Needs keep_lo.  Writes the outer true relative residuals to rel_res. */
 int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_outer,
                        double* rel_res, int* outer_steps, int* total_iters);
+/* FP32 PCG with residual replacement (reliable updates) and an FP64
+ * solution: at graph-launch boundaries, when a system's r^T r has fallen by
+ * delta^2 since its last replacement (or it converged on the recursive
+ * residual), x64 += x32 and r, z are replaced by the FP64 true residual of
+ * x64 (hi+lo values) while p is kept, so the Krylov recurrence is not
+ * restarted.  A system stops when ||b - A x64|| <= rel_tol * ||b||.  Needs
+ * keep_lo; runs the fused iteration form; delta in (0, 1), 0.1 typical.
+ * Writes true relative residuals, the number of replacements and the
+ * largest iteration count. */
+int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res, int* replacements,
+                        int* total_iters);
 int xp_pcg_get_status(xp_pcg* s, int sys, xp_status* out);
 /* Solution in natural order, FP64 (sum of n_k entries). */
 int xp_pcg_get_x(xp_pcg* s, double* x);

@@ -201,6 +201,55 @@ __device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int
     return acc;
 }
 
+// A x for an FP32 vector x with FP64 values (VMODE 0: val64; 1: val +
+// val_lo) and FP64 accumulation.  An FP64 value times an FP32 operand rounds
+// once in FP64 and the row sum carries only FP64 rounding, so A x keeps its
+// relative accuracy when the row's terms cancel (smooth x on an
+// ill-conditioned matrix), which an FP32 sum does not.  xdiag = x[row].
+template <bool IDX16, int VMODE>
+__device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int lane, const float* __restrict__ x,
+                                               float& xdiag) {
+    const int j0 = A.slice_ptr[slice];
+    const int w = A.slice_ptr[slice + 1] - j0;
+    const int row = slice * kC + lane;
+    const size_t e0 = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
+    const int base = A.slice_base[slice];
+    auto value = [&](size_t e) -> double {
+        if (VMODE == 0) return A.val64[e];
+        return static_cast<double>(A.val[e]) + static_cast<double>(A.val_lo[e]);
+    };
+    if (w <= kFastW) {  // indices and gathers issued first; values stream into the sum
+        float xv[kFastW];
+        int c[kFastW];
+#pragma unroll
+        for (int k = 0; k < kFastW; ++k) {
+            if (k < w)
+                c[k] = k == 0 ? row
+                              : (IDX16 ? base + static_cast<int>(A.col16[e0 + static_cast<size_t>(k) * kC])
+                                       : A.col32[e0 + static_cast<size_t>(k) * kC]);
+        }
+#pragma unroll
+        for (int k = 0; k < kFastW; ++k)
+            if (k < w) xv[k] = x[c[k]];
+        xdiag = xv[0];
+        double acc = value(e0) * static_cast<double>(xv[0]);
+#pragma unroll
+        for (int k = 1; k < kFastW; ++k)
+            if (k < w) acc = fma(value(e0 + static_cast<size_t>(k) * kC), static_cast<double>(xv[k]), acc);
+        return acc;
+    }
+    size_t e = e0;
+    xdiag = x[row];
+    double acc = value(e) * static_cast<double>(xdiag);
+#pragma unroll 4
+    for (int j = 1; j < w; ++j) {
+        e += kC;
+        const int c = IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e];
+        acc = fma(value(e), static_cast<double>(x[c]), acc);
+    }
+    return acc;
+}
+
 // z = M r for the 8x8 block of this lane's row: the 8 rows of a block are 8
 // consecutive lanes (blocks start at multiples of 8 in solve order), so the
 // block's r values are exchanged with cross-lane shuffles.  M is stored per
@@ -411,6 +460,7 @@ __device__ __forceinline__ void pcg_scalar_step(SysState& s, const double (&t)[3
         trace_row->rr = rr;
     }
     s.iters += 1;
+    s.rz_prev = rz_old;
     s.rz = rz;
     s.rr = rr;
     s.alpha = alpha;
@@ -560,6 +610,137 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
     if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
 }
 
+// ---------------------------------------------------------------------------
+// Accurate iteration (FP32 r and z, FP64 p, A p and x; FP64-accumulated SpMV
+// with hi+lo or FP64 values).  Same state machine, partial slots and scalar
+// step as the FP32 kernels above; only the vector arithmetic differs.  With
+// FP64 p and A p the recurrence A p = A z + beta A p_old stays consistent
+// with p, so the recursive residual tracks b - A x.
+
+// First kernel: scalar tail of the previous iteration, p = z + beta p,
+// ap = A z + beta ap, pAp partials.  Right after init (fresh): p = z, ap = A z.
+template <bool IDX16, int VMODE>
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, SellView A, const SysState* Sc,
+                                                                  SysState* Sn, TraceRow* trace, int trace_stride,
+                                                                  const float* __restrict__ z, double* __restrict__ p,
+                                                                  double* __restrict__ ap) {
+    __shared__ SysState s_sh;
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    const bool leader_block = lb == sys_range(L, w.sys).x;
+    if (threadIdx.x == 0) s_sh = Sc[w.sys];
+    double t[3];
+    sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);  // ends with a barrier
+    if (!s_sh.active) {
+        if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
+        return;
+    }
+    const bool fresh = s_sh.fresh != 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!fresh) {
+            TraceRow* tr = nullptr;
+            if (leader_block && trace != nullptr && s_sh.iters < s_sh.trace_cap)
+                tr = trace + static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
+                     static_cast<size_t>(s_sh.iters);
+            pcg_scalar_step(s_sh, t, tr);
+        } else {
+            s_sh.fresh = 0;
+        }
+        if (leader_block) Sn[w.sys] = s_sh;
+    }
+    __syncthreads();
+    if (!s_sh.active) return;
+    const double beta = s_sh.beta;
+    XP_WAVE_LANE
+    double v[1] = {0.0};
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
+        float zd;
+        const double az = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd);
+        double pn, an;
+        if (fresh) {
+            pn = static_cast<double>(zd);
+            an = az;
+        } else {
+            pn = static_cast<double>(zd) + beta * p[row];
+            an = az + beta * ap[row];
+        }
+        p[row] = pn;
+        ap[row] = an;
+        v[0] += pn * an;
+    }
+    block_sum<1>(v);
+    if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
+}
+
+// alpha = rz / pAp; x += alpha p (FP64); r = fp32(r - alpha ap); z = M r;
+// partials of rz_new, rr_new.  Rows in flight per thread: the FP64 streams
+// double the registers per row.
+constexpr int kRowsAcc = 2;
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, const SysState* S,
+                                                               double* __restrict__ x,
+                                                               const double* __restrict__ p,
+                                                               float* __restrict__ r,
+                                                               const double* __restrict__ ap,
+                                                               const float* __restrict__ minv,
+                                                               const float* __restrict__ blockM,
+                                                               float* __restrict__ z) {
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    const int active = S[w.sys].active;
+    const double rz_old = S[w.sys].rz;
+    const int r0 = w.slice_begin * kC, r1 = w.slice_end * kC;
+    double v[2] = {0.0, 0.0};
+    bool first = true;
+    double alpha = 0.0;
+    for (int base = r0; base < r1; base += kRowsAcc * kBlock) {
+        double xv[kRowsAcc], pv[kRowsAcc], av[kRowsAcc];
+        float rv[kRowsAcc], mv[kRowsAcc];
+#pragma unroll
+        for (int k = 0; k < kRowsAcc; ++k) {
+            const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
+            if (row < r1) {
+                xv[k] = x[row];
+                pv[k] = p[row];
+                rv[k] = r[row];
+                av[k] = ap[row];
+                mv[k] = blockM == nullptr ? minv[row] : 0.0f;
+            }
+        }
+        if (first) {
+            first = false;
+            double t[1];
+            sum_partials<1>(L, w.sys, {kSlotPap}, t);
+            if (!active) return;
+            alpha = rz_old / t[0];
+            if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;
+        }
+#pragma unroll
+        for (int k = 0; k < kRowsAcc; ++k) {
+            const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
+            if (row < r1) {  // uniform per wavefront: r1 - base is a multiple of 64
+                const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * av[k]);
+                const float zn = blockM != nullptr
+                                     ? block_apply(blockM, row, rn)
+                                     : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
+                x[row] = xv[k] + alpha * pv[k];
+                r[row] = rn;
+                z[row] = zn;
+                v[0] += static_cast<double>(rn) * static_cast<double>(zn);
+                v[1] += static_cast<double>(rn) * static_cast<double>(rn);
+            }
+        }
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        put_partial(L, kSlotRz, lb, v[0]);
+        put_partial(L, kSlotRr, lb, v[1]);
+    }
+}
+
 template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_spmv_f32(Launch L, SellView A, const float* __restrict__ x,
                                                      float* __restrict__ y) {
@@ -607,6 +788,63 @@ __global__ __launch_bounds__(kBlock, 8) void k_finish_aux(Launch L, double* out)
     double t[1];
     sum_partials<1>(L, sys, {kSlotAux}, t);
     if (threadIdx.x == 0) out[sys] = t[0];
+}
+
+// Residual replacement (reliable updates): the FP32 recursion's r and z are
+// replaced by the true residual of the FP64 solution, r = fp32(b - A x64)
+// with hi+lo (or exact) values, and z = M r; the rz / rr partials the next
+// scalar step consumes are rewritten from them (as k_pcg_update_xrz writes
+// them) and the Aux partials get the FP64 true r^T r.  p and ap are kept.
+template <bool IDX16, int VMODE>
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_replace(Launch L, SellView A, const double* __restrict__ x,
+                                                        const double* __restrict__ b,
+                                                        const float* __restrict__ minv,
+                                                        const float* __restrict__ blockM, float* __restrict__ r,
+                                                        float* __restrict__ z) {
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {  // wave-uniform: block_apply shuffles
+        const int row = sl * kC + lane;
+        const double rv = b[row] - sell_row_f64<IDX16, VMODE>(A, sl, lane, x);
+        const float r32 = static_cast<float>(rv);
+        const float zv = blockM != nullptr ? block_apply(blockM, row, r32)
+                                           : static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(r32));
+        r[row] = r32;
+        z[row] = zv;
+        v[0] += static_cast<double>(r32) * static_cast<double>(zv);
+        v[1] += static_cast<double>(r32) * static_cast<double>(r32);
+        v[2] += rv * rv;
+    }
+    block_sum<3>(v);
+    if (threadIdx.x == 0) {
+        put_partial(L, kSlotRz, lb, v[0]);
+        put_partial(L, kSlotRr, lb, v[1]);
+        put_partial(L, kSlotAux, lb, v[2]);
+    }
+}
+
+// One workgroup per system: out[3 sys + 0..2] = the replaced rz, rr and the
+// FP64 true r^T r, summed in workgroup order.
+__global__ __launch_bounds__(kBlock, 8) void k_replace_sums(Launch L, double* out) {
+    const int sys = static_cast<int>(blockIdx.x);
+    double t[3];
+    sum_partials<3>(L, sys, {kSlotRz, kSlotRr, kSlotAux}, t);
+    if (threadIdx.x != 0) return;
+    out[3 * sys + 0] = t[0];
+    out[3 * sys + 1] = t[1];
+    out[3 * sys + 2] = t[2];
+}
+
+// x64 += (double) d; d = 0 (folds the FP32 correction into the FP64 solution).
+__global__ void k_fold_f32_into_f64(int n, float* __restrict__ d, double* __restrict__ x) {
+    const int i = static_cast<int>(blockIdx.x) * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    if (i < n) {
+        x[i] += static_cast<double>(d[i]);
+        d[i] = 0.0f;
+    }
 }
 
 __global__ void k_axpy_f64_f32(int n, const float* __restrict__ d, double* __restrict__ x) {
@@ -954,6 +1192,62 @@ void launch_block_features(const SellView& A, double* dis, float* feat, float* s
         hipLaunchKernelGGL(k_block_features<true>, dim3(g), dim3(256), 0, s, A, dis, feat, scale);
     else
         hipLaunchKernelGGL(k_block_features<false>, dim3(g), dim3(256), 0, s, A, dis, feat, scale);
+}
+
+void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const double* x, const double* b,
+                        const float* minv, const float* blockM, float* r, float* z, double* out) {
+    const dim3 g(L.grid), blk(kBlock);
+    if (A.idx16) {
+        if (use_lo)
+            hipLaunchKernelGGL((k_pcg_replace<true, 1>), g, blk, 0, L.stream, L, A, x, b, minv, blockM, r, z);
+        else
+            hipLaunchKernelGGL((k_pcg_replace<true, 0>), g, blk, 0, L.stream, L, A, x, b, minv, blockM, r, z);
+    } else {
+        if (use_lo)
+            hipLaunchKernelGGL((k_pcg_replace<false, 1>), g, blk, 0, L.stream, L, A, x, b, minv, blockM, r, z);
+        else
+            hipLaunchKernelGGL((k_pcg_replace<false, 0>), g, blk, 0, L.stream, L, A, x, b, minv, blockM, r, z);
+    }
+    hipLaunchKernelGGL(k_replace_sums, dim3(A.nsys), blk, 0, L.stream, L, out);
+}
+
+#define XP_DISPATCH_ACC(A, use_lo, KERNEL, ...)                                  \
+    do {                                                                     \
+        if ((A).idx16) {                                                     \
+            if (use_lo)                                                      \
+                hipLaunchKernelGGL((KERNEL<true, 1>), __VA_ARGS__);          \
+            else                                                             \
+                hipLaunchKernelGGL((KERNEL<true, 0>), __VA_ARGS__);          \
+        } else {                                                             \
+            if (use_lo)                                                      \
+                hipLaunchKernelGGL((KERNEL<false, 1>), __VA_ARGS__);         \
+            else                                                             \
+                hipLaunchKernelGGL((KERNEL<false, 0>), __VA_ARGS__);         \
+        }                                                                    \
+    } while (0)
+
+void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
+                         const double* b, const float* minv, const float* blockM, float* r, float* z) {
+    XP_DISPATCH_ACC(A, use_lo, k_pcg_replace, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, blockM, r, z);
+    hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
+}
+
+void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, bool use_lo, const SysState* st_cur,
+                               SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
+                               double* ap) {
+    XP_DISPATCH_ACC(A, use_lo, k_pcg_spmv_fused_acc, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st_cur, st_next,
+                    trace, trace_stride, z, p, ap);
+}
+
+void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,
+                               const double* ap, const float* minv, const float* blockM, float* z) {
+    hipLaunchKernelGGL(k_pcg_update_xrz_acc, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, x, p, r, ap, minv, blockM,
+                       z);
+}
+
+void launch_fold_f32_into_f64(int n_pad, float* d, double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold_f32_into_f64, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad,
+                       d, x);
 }
 
 void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s) {

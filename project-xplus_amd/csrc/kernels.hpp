@@ -42,6 +42,18 @@ void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const 
 void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
                          int trace_stride, const float* z, float* p);
 
+// ---- accurate iteration: FP32 r, z; FP64 p, ap, x; FP64-accumulated SpMV
+// with hi+lo (use_lo) or FP64 values ----
+// init: r = fp32(b - A x) (FP64), z = M r, rz / rr partials, then the state
+// after init; the first fused iteration sets p = z, ap = A z (state.fresh).
+void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
+                         const double* b, const float* minv, const float* blockM, float* r, float* z);
+void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, bool use_lo, const SysState* st_cur,
+                               SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
+                               double* ap);
+void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,
+                               const double* ap, const float* minv, const float* blockM, float* z);
+
 // ---- plain SpMV (no reduction) ----
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y);
 // FP64 SpMV with exact FP64 values (val64) or hi+lo pairs (val + val_lo).
@@ -50,6 +62,13 @@ void launch_spmv_f64(const Launch& L, const SellView& A, bool use_lo, const doub
 // out_rr[sys] = r^T r.  Also writes the FP32 copy of r into r32 (may be null).
 void launch_residual_f64(const Launch& L, const SellView& A, bool use_lo, const double* x,
                          const double* b, double* r, float* r32, double* out_rr);
+// Residual replacement: r = fp32(b - A x) (FP64 SpMV), z = M r, and the rz /
+// rr partials of the next scalar step rewritten from them;
+// out[3 sys + 0..2] = (rz, rr, FP64 true r^T r) of each system.
+void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const double* x, const double* b,
+                        const float* minv, const float* blockM, float* r, float* z, double* out);
+// x64 += (double) d; d = 0.
+void launch_fold_f32_into_f64(int n_pad, float* d, double* x, hipStream_t s);
 // x64 += (double) d32 over the solve-order rows.
 void launch_axpy_f64_f32(int n_pad, const float* d, double* x, hipStream_t s);
 

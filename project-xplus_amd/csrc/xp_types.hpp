@@ -51,6 +51,7 @@ struct SysState {
     double tau;       // absolute bound on r^T r
     double rr0;       // r^T r after init
     double best_rr;   // smallest rr seen (stagnation watch)
+    double rz_prev;   // rz before the last scalar step (residual replacement rewinds to it)
     int iters;
     int max_iters;
     int status;       // 0 converged, 1 max_iter, 2 breakdown, 3 stagnation, -1 running

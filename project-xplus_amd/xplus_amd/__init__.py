@@ -54,7 +54,7 @@ class MatrixInfo(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("tau", C.c_double), ("rel_tol", C.c_double), ("max_iters", C.c_int),
                 ("graph_iters", C.c_int), ("use_graph", C.c_int), ("trace_cap", C.c_int),
-                ("stall_limit", C.c_int), ("fused", C.c_int)]
+                ("stall_limit", C.c_int), ("fused", C.c_int), ("accurate", C.c_int)]
 
 
 class TuneReport(C.Structure):
@@ -119,6 +119,8 @@ _p_run = _sig("xp_pcg_run", C.c_int, [_vp])
 _p_run_fixed = _sig("xp_pcg_run_fixed", C.c_int, [_vp, C.c_int])
 _p_run_refined = _sig("xp_pcg_run_refined", C.c_int,
                       [_vp, C.c_double, C.c_double, C.c_int, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
+_p_run_reliable = _sig("xp_pcg_run_reliable", C.c_int,
+                       [_vp, C.c_double, C.c_double, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
 _p_status = _sig("xp_pcg_get_status", C.c_int, [_vp, C.c_int, C.POINTER(Status)])
 _p_get_x = _sig("xp_pcg_get_x", C.c_int, [_vp, _f64p])
 _p_trace = _sig("xp_pcg_get_trace", C.c_int, [_vp, C.c_int, _f64p, C.c_int])
@@ -415,9 +417,10 @@ class Matrix:
     def set_variant(self, v: int) -> None:
         _check(_m_set_variant(self.h, v))
 
-    def autotune(self, fused: bool = False, tune_sigma: bool = False, iters: int = 64) -> dict:
+    def autotune(self, fused=False, tune_sigma: bool = False, iters: int = 64) -> dict:
         """Picks SpMV inner loop x workgroup count (x sigma) by timing real PCG
-        iterations; tune_sigma re-sorts the matrix, so call it before any Pcg."""
+        iterations (fused: False/True, or 2 for the accurate iteration);
+        tune_sigma re-sorts the matrix, so call it before any Pcg."""
         r = TuneReport()
         _check(_m_autotune(self.h, int(fused), int(tune_sigma), iters, C.byref(r)))
         return {k: getattr(r, k) for k, _ in r._fields_}
@@ -454,12 +457,13 @@ class Pcg:
 
     def __init__(self, matrix: Matrix, tau: float = 1e-10, rel_tol: float = 0.0, max_iters: int = 0,
                  graph_iters: int = 0, use_graph: bool = True, trace_cap: int = 0, stall_limit: int = 0,
-                 fused: bool = False):
+                 fused: bool = False, accurate: bool = False):
         """tau: absolute bound on r^T r; rel_tol > 0 overrides it per system as
-        (rel_tol ||b_k||)^2."""
+        (rel_tol ||b_k||)^2.  accurate: FP64-accumulated SpMV (hi+lo values,
+        the matrix needs keep_lo) with FP64 p, A p and x."""
         self.matrix = matrix
         self.opts = SolveOpts(tau, rel_tol, max_iters, graph_iters, int(use_graph), trace_cap, stall_limit,
-                              int(fused))
+                              int(fused), int(accurate))
         h = _vp()
         _check(_p_create(matrix.h, C.byref(self.opts), C.byref(h)))
         self.h = h
@@ -493,6 +497,14 @@ class Pcg:
         outer, total = C.c_int(), C.c_int()
         _check(_p_run_refined(self.h, outer_rel, inner_rel, max_outer, rel, C.byref(outer), C.byref(total)))
         return rel, outer.value, total.value
+
+    def run_reliable(self, rel_tol: float = 1e-6, delta: float = 0.1):
+        """FP32 PCG with residual replacement to a true relative residual:
+        returns (true relative residuals, replacements, iterations)."""
+        rel = np.zeros(self.nsys)
+        repl, total = C.c_int(), C.c_int()
+        _check(_p_run_reliable(self.h, rel_tol, delta, rel, C.byref(repl), C.byref(total)))
+        return rel, repl.value, total.value
 
     def status(self, sys: int = 0) -> dict:
         s = Status()

@@ -1,0 +1,217 @@
+"""GPU tests of the solves that meet a TRUE residual: FP32 PCG with residual
+replacement (xp_pcg_run_reliable) and the accurate iteration
+(xp_solve_opts.accurate: FP64-accumulated SpMV with hi+lo values, FP64 p,
+A p and x, FP32 r and z).
+
+Bars (written in each test):
+  * the TRUE relative residual ||b - A x|| / ||b||, recomputed by the oracle in
+    FP64 from the returned x, meets the requested tolerance;
+  * on the config-1 matrix: x within relative 1e-5 of the FP64 golden solved
+    to the same true tolerance, iteration count within +-2 of the golden's
+    (replacements perturb the FP32 recursion by rounding-level amounts);
+  * on the ill-conditioned FEM matrix: fewer iterations than restarted
+    refinement.  (Only a few percent fewer: FP32-accumulated SpMVs lose the
+    A-conjugacy of the directions on this matrix, which replacement cannot
+    restore; the accurate iteration below is the remedy);
+  * batches of uneven systems: every system meets its own tolerance;
+  * with block preconditioners the same true-residual bar;
+  * accurate iteration on the config-1 matrix: iterations within +-1 of the
+    FP64 golden, x within relative 1e-5, rr trace within relative 1e-4 while
+    rr > 1e-8 rr0; on the FEM matrix: true residual met with at most 1.2x
+    the golden's iterations (a CPU simulation of this arithmetic needs 1.13x).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import xplus_amd as X
+
+pytestmark = pytest.mark.gpu
+
+
+def true_rel(a, x):
+    return O.residual_norm(a, x) / np.linalg.norm(a.b)
+
+
+def test_reliable_c1_matches_golden(gpu):
+    a = X.gen_diffusion2d(32, 32, 1.0, 20261204)
+    tol = 1e-6
+    g = O.golden(a, tau=(tol * np.linalg.norm(a.b)) ** 2)
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, rel_tol=tol)
+    s.set_rhs(a.b, a.x0)
+    rel, repl, iters = s.run_reliable(tol, 0.1)
+    x = s.x()
+    print(f"c1 reliable: iters {iters} golden {g['iterations']} replacements {repl} rel {rel[0]:.3e}", flush=True)
+    assert rel[0] <= tol
+    assert true_rel(a, x) <= tol * (1 + 1e-9)
+    assert abs(iters - g["iterations"]) <= 2
+    assert np.linalg.norm(x - g["x"]) <= 1e-5 * np.linalg.norm(g["x"])
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_reliable_fem_beats_restarts(gpu, fused):
+    a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, 20263203)
+    tol = 1e-6
+    m = X.Matrix(a, sigma=256, keep_lo=True)
+    plain = X.Pcg(m, rel_tol=tol, fused=fused)
+    plain.set_rhs(a.b, a.x0)
+    plain.run()
+    plain_iters = plain.status()["iterations"]
+    ref = X.Pcg(m, rel_tol=tol, fused=fused)
+    ref.set_rhs(a.b, a.x0)
+    rrel, outer, ref_iters = ref.run_refined(tol, 1e-2, 50)
+    s = X.Pcg(m, rel_tol=tol, fused=fused)
+    s.set_rhs(a.b, a.x0)
+    rel, repl, iters = s.run_reliable(tol, 0.1)
+    x = s.x()
+    print(f"fem fused={fused}: plain recursive {plain_iters}, refined {ref_iters} ({outer} outer), "
+          f"reliable {iters} ({repl} replacements) rel {rel[0]:.3e} (refined {rrel[0]:.3e})", flush=True)
+    assert rel[0] <= tol
+    assert true_rel(a, x) <= tol * (1 + 1e-6)
+    assert iters < ref_iters
+    # the solver still runs its own iteration form afterwards
+    s.run()
+    assert s.status()["iterations"] == plain_iters
+
+
+def test_reliable_batch_uneven(gpu):
+    systems = [X.gen_diffusion3d(k, 1.5, 20264203 + i) for i, k in enumerate([7, 16, 11, 20, 9])]
+    tol = 1e-6
+    m = X.Matrix(systems, sigma=256, keep_lo=True)
+    s = X.Pcg(m, rel_tol=tol)
+    b = np.concatenate([a.b for a in systems])
+    s.set_rhs(b)
+    rel, repl, iters = s.run_reliable(tol, 0.1)
+    x = s.x()
+    o = 0
+    for k, a in enumerate(systems):
+        xr = x[o:o + a.n]
+        o += a.n
+        r = O.residual_norm(a, xr, a.b) / np.linalg.norm(a.b)
+        print(f"system {k}: n={a.n} status {s.status(k)['status']} iters {s.status(k)['iterations']} rel {r:.3e}",
+              flush=True)
+        assert rel[k] <= tol
+        assert r <= tol * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_reliable_block_preconditioners(gpu, mode):
+    a = X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265200)
+    tol = 1e-6
+    m = X.Matrix(a, block_rows=8, keep_lo=True)
+    s = X.Pcg(m, rel_tol=tol, stall_limit=0)
+    if mode == 1:
+        s.set_preconditioner(1)
+    else:
+        net = X.Mlp(seed=5)
+        net.train(X.blockset([X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265100 + i) for i in range(2)]),
+                  300, batch=4096, lr=2e-3, seed=1)
+        s.set_mlp_preconditioner(net)
+    s.set_rhs(a.b)
+    rel, repl, iters = s.run_reliable(tol, 0.1)
+    x = s.x()
+    print(f"block mode {mode}: iters {iters} replacements {repl} rel {rel[0]:.3e}", flush=True)
+    assert rel[0] <= tol
+    assert true_rel(a, x) <= tol * (1 + 1e-6)
+
+
+def test_reliable_converged_start(gpu):
+    """x0 already meeting the tolerance: no iterations, x unchanged."""
+    a = X.gen_diffusion2d(32, 32, 1.0, 20261204)
+    A = a.to_scipy().toarray()
+    x_exact = np.linalg.solve(A, a.b)
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, rel_tol=1e-6)
+    s.set_rhs(a.b, x_exact)
+    rel, repl, iters = s.run_reliable(1e-6, 0.1)
+    assert iters == 0
+    assert rel[0] <= 1e-6
+    np.testing.assert_allclose(s.x(), x_exact, rtol=0, atol=1e-12 * np.abs(x_exact).max())
+
+
+def test_accurate_c1_matches_golden(gpu):
+    a = X.gen_diffusion2d(32, 32, 1.0, 20261204)
+    tol = 1e-6
+    g = O.golden(a, tau=(tol * np.linalg.norm(a.b)) ** 2)
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, rel_tol=tol, accurate=True, trace_cap=4000)
+    s.set_rhs(a.b, a.x0)
+    s.run()
+    st = s.status()
+    x = s.x()
+    print(f"c1 accurate: iters {st['iterations']} golden {g['iterations']}", flush=True)
+    assert st["status"] == X.CONVERGED
+    assert abs(st["iterations"] - g["iterations"]) <= 1
+    assert np.linalg.norm(x - g["x"]) <= 1e-5 * np.linalg.norm(g["x"])
+    tr = s.trace()
+    ref = g["trace"][:, 4]
+    k = min(len(tr), len(ref))
+    keep = ref[:k] > 1e-8 * ref[0]
+    np.testing.assert_allclose(tr[:k, 3][keep], ref[:k][keep], rtol=1e-4)
+
+
+@pytest.mark.parametrize("block", [False, True])
+def test_accurate_fem_true_residual(gpu, block):
+    a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, 20263203)
+    tol = 1e-6
+    g = O.golden(a, tau=(tol * np.linalg.norm(a.b)) ** 2, trace_cap=1)
+    m = X.Matrix(a, sigma=256, keep_lo=True, block_rows=8 if block else 1)
+    s = X.Pcg(m, rel_tol=tol, accurate=True, stall_limit=0)
+    if block:
+        s.set_preconditioner(1)
+    s.set_rhs(a.b, a.x0)
+    s.run()
+    st = s.status()
+    rec_rel = true_rel(a, s.x())
+    rel, repl, iters = s.run_reliable(tol, 0.1)
+    x = s.x()
+    print(f"fem accurate block={block}: run {st['iterations']} iterations (true rel {rec_rel:.3e}), "
+          f"reliable {iters} ({repl} replacements, rel {rel[0]:.3e}); golden {g['iterations']}", flush=True)
+    assert st["status"] == X.CONVERGED
+    assert rec_rel <= 2 * tol           # the recursive residual tracks the true one
+    assert rel[0] <= tol
+    assert true_rel(a, x) <= tol * (1 + 1e-6)
+    if not block:
+        assert iters <= 1.2 * g["iterations"]
+
+
+def test_accurate_batch_uneven(gpu):
+    systems = [X.gen_diffusion3d(k, 1.5, 20264303 + i) for i, k in enumerate([7, 16, 11, 20, 9])]
+    tol = 1e-6
+    m = X.Matrix(systems, sigma=256, keep_lo=True)
+    s = X.Pcg(m, rel_tol=tol, accurate=True)
+    s.set_rhs(np.concatenate([a.b for a in systems]))
+    rel, repl, iters = s.run_reliable(tol, 0.1)
+    x = s.x()
+    o = 0
+    for k, a in enumerate(systems):
+        g = O.golden(a, tau=(tol * np.linalg.norm(a.b)) ** 2, trace_cap=1)
+        xr = x[o:o + a.n]
+        o += a.n
+        it = s.status(k)["iterations"]
+        print(f"system {k}: n={a.n} iters {it} golden {g['iterations']} rel {rel[k]:.3e}", flush=True)
+        assert rel[k] <= tol
+        assert O.residual_norm(a, xr, a.b) / np.linalg.norm(a.b) <= tol * (1 + 1e-6)
+        assert abs(it - g["iterations"]) <= 2
+
+
+def test_solver_survives_workgroup_change(gpu):
+    """A solver's captured graph holds the matrix's work map and partials by
+    address; changing the workgroup count (autotuning) re-captures it."""
+    a = X.gen_fem2d(66, 1.0, 6, 1e3, 0.3, 20264401)
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, rel_tol=1e-6, accurate=True)
+    s.set_rhs(a.b)
+    s.run()
+    st1, x1 = s.status(), s.x()
+    for g in (16, 48, 0):
+        m.set_grid(g if g else m.info()["nslices"])
+        s.run()
+        st2 = s.status()
+        print(f"grid {m.info()['grid']}: iters {st2['iterations']} vs {st1['iterations']}", flush=True)
+        assert st2["status"] == X.CONVERGED
+        assert abs(st2["iterations"] - st1["iterations"]) <= 1
+        assert np.linalg.norm(s.x() - x1) <= 1e-4 * np.linalg.norm(x1)
