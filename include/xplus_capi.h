@@ -85,7 +85,10 @@ typedef struct {
     int keep64;  /* keep exact FP64 values (drop-in FP64 SpMV) */
     int grid;    /* workgroups of the streaming kernels; 0 = library default */
     int spmv_variant; /* SpMV inner loop: 0 rolled loop, 1 fixed-width unrolled */
-    int block_rows;   /* 8: keep aligned 8-row groups together (block preconditioners); 0/1: rows */
+    int block_rows;   /* 8: keep blocks of up to 8 rows together as 8 solve rows
+                         (block preconditioners); 0/1: rows */
+    int blocking;     /* with block_rows = 8: 0 = blocks of 8 consecutive rows;
+                         1 = greedy strength aggregates (xp_strength_blocking) */
 } xp_matrix_opts;
 
 typedef struct {
@@ -145,6 +148,9 @@ int xp_device_alloc(unsigned long long bytes, void** out);
 int xp_device_free(void* ptr);
 int xp_memcpy_htod(void* dst, const void* src, unsigned long long bytes);
 int xp_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes);
+/* Device-to-device copy (the runtime's copy kernel): a transfer of known
+ * size, e.g. to calibrate memory-traffic counters. */
+int xp_memcpy_dtod(void* dst, const void* src, unsigned long long bytes);
 int xp_device_synchronize(void);
 /* Times `reps` back-to-back launches of the FP32 SpMV (kernel k_spmv_f32)
  * with HIP events recorded on the stream the kernels are launched on (the
@@ -224,7 +230,11 @@ int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_ou
  * residual), x64 += x32 and r, z are replaced by the FP64 true residual of
  * x64 (hi+lo values) while p is kept, so the Krylov recurrence is not
  * restarted.  A system stops when ||b - A x64|| <= rel_tol * ||b||.  Needs
- * keep_lo; runs the fused iteration form; delta in (0, 1), 0.1 typical.
+ * keep_lo; runs the fused iteration form; delta in [0, 1): 0.1 suits the
+ * FP32 iteration; 0 replaces only when a system has converged on its
+ * recursive residual, which suits the accurate iteration (its recursive
+ * residual tracks the true one, and each replacement perturbs the Krylov
+ * recurrence).
  * Writes true relative residuals, the number of replacements and the
  * largest iteration count. */
 int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res, int* replacements,
@@ -245,8 +255,29 @@ int xp_pcg_true_residual(xp_pcg* s, double* out_norms);
  *   natural block order).  With a block preconditioner and stall_limit > 0,
  *   a stagnating or breaking-down solve is redone with Jacobi (fallback). */
 int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks);
-/* Number of 8-row blocks (natural order) of the matrix: sum ceil(n_k / 8). */
+/* Number of 8-row blocks of the matrix (all systems): sum ceil(n_k / 8)
+ * for consecutive-row blocks, the aggregate count for strength blocking.
+ * Block-ordered arrays (mode 2 blocks, xp_matrix_block_features) list the
+ * blocks of system 0, then system 1, ... */
 int xp_matrix_num_blocks(const xp_matrix* m);
+/* The rows of every block: rows[8 b + j] = global natural row of slot j of
+ * block b, -1 for a padding slot (real rows first). */
+int xp_matrix_block_rows(const xp_matrix* m, int* rows);
+/* Greedy strength aggregation of one CSR system into blocks of at most 8
+ * rows (no GPU): s_ij = |a_ij| / sqrt(a_ii a_jj); seeds in row order, a
+ * block grows by the unassigned row of largest summed strength to it (ties:
+ * smaller index) up to 8 rows; rows of partial blocks are regrouped among
+ * themselves the same way, the remaining partial blocks packed first-fit;
+ * rows ascending inside a block, blocks ordered by their first row.
+ * slots: capacity 8 n; writes 8 * nblocks entries (-1 = padding). */
+int xp_strength_blocking(int n, const int* row_ptr, const int* col_idx, const double* values, int* slots,
+                         int* nblocks);
+/* Average device time (us) of each kernel of one iteration, over `iters`
+ * directly launched iterations from x0 with the convergence test off, HIP
+ * events recorded on the solver's stream between kernels: us[0] the SpMV
+ * kernel (fused or not), us[1] update_xrz, us[2] update_p (three-kernel
+ * form only).  Leaves the solver's vectors as after those iterations. */
+int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels);
 /* Device time of the last xp_pcg_run* (ms, HIP events) and launches made. */
 int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches);
 
@@ -273,6 +304,9 @@ int xp_mlp_get_params(xp_mlp* net, float* out);
 int xp_mlp_set_params(xp_mlp* net, const float* params);
 /* Blocks of a host CSR system (no GPU needed); append concatenates sets. */
 int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, xp_blockset** out);
+/* The same over the blocks of a blocking (0: consecutive rows, 1: strength). */
+int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, const double* values, int blocking,
+                                 xp_blockset** out);
 int xp_blockset_append(xp_blockset* dst, const xp_blockset* src);
 int xp_blockset_size(const xp_blockset* set);
 /* feat: size x 48, ahat: size x 64, hmat (Ahat^2 + 0.5 G): size x 64, scale (D^-1/2): size x 8. */

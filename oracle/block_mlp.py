@@ -5,7 +5,8 @@ the product).  The reference has no learned preconditioner; this module is
 the independent CPU restatement its GPU kernels are checked against:
 
   features_from_csr  -- block features of S = D^-1/2 A D^-1/2, Ahat = S_bb,
-                        H = Ahat^2 + LAMBDA G, D^-1/2 per row
+                        H = Ahat^2 + LAMBDA G, D^-1/2 per row, for natural
+                        8-row groups or any blocks of at most 8 rows
   forward            -- the MLP 48 -> 64 -> 64 -> 48 (ReLU, ReLU, linear)
   loss_grad          -- mean block-SPAI loss and its analytic gradient
   blocks_from_out    -- M_b = D^-1/2 (L L^T + 1e-3 I) D^-1/2
@@ -43,9 +44,18 @@ def pack(d):
     return np.concatenate([np.asarray(d[name], dtype=np.float64).ravel() for name, _ in SHAPES])
 
 
-def features_from_csr(row_ptr, col_idx, values):
+def natural_groups(n):
+    return [list(range(r, min(r + 8, n))) for r in range(0, n, 8)]
+
+
+def features_from_csr(row_ptr, col_idx, values, groups=None):
+    """Features of the blocks `groups` (lists of at most 8 rows; slot j of a
+    block is its j-th row, padding slots after them), natural 8-row groups
+    when None.  Returns feat (nb x 48), ahat, hmat (nb x 8 x 8), scale (nb x 8)."""
     n = len(row_ptr) - 1
-    nb = (n + 7) // 8
+    if groups is None:
+        groups = natural_groups(n)
+    nb = len(groups)
     dis = np.ones(n)
     for r in range(n):
         for o in range(row_ptr[r], row_ptr[r + 1]):
@@ -53,33 +63,38 @@ def features_from_csr(row_ptr, col_idx, values):
                 if values[o] > 0:
                     dis[r] = 1.0 / np.sqrt(values[o])
                 break
+    where = {}
+    for b, g in enumerate(groups):
+        for j, r in enumerate(g):
+            where[r] = (b, j)
     feat = np.zeros((nb, N_IN))
     ahat = np.zeros((nb, 8, 8))
     hmat = np.zeros((nb, 8, 8))
     scale = np.ones((nb, 8))
-    for b in range(nb):
-        r0, m = 8 * b, min(8, n - 8 * b)
+    for b, g in enumerate(groups):
+        m = len(g)
         a = np.zeros((8, 8))
         offabs = np.zeros(8)
         off = [dict() for _ in range(8)]
-        for i in range(m):
-            for o in range(row_ptr[r0 + i], row_ptr[r0 + i + 1]):
-                c = col_idx[o] - r0
-                v = values[o] * dis[r0 + i] * dis[col_idx[o]]
-                if 0 <= c < m:
-                    a[i, c] += v
+        for i, r in enumerate(g):
+            for o in range(row_ptr[r], row_ptr[r + 1]):
+                c = col_idx[o]
+                v = values[o] * dis[r] * dis[c]
+                bc, jc = where[c]
+                if bc == b:
+                    a[i, jc] += v
                 else:
                     offabs[i] += abs(v)
-                    off[i][col_idx[o]] = off[i].get(col_idx[o], 0.0) + v
+                    off[i][c] = off[i].get(c, 0.0) + v
         for i in range(m, 8):
             a[i, i] = 1.0
-        scale[b, :m] = dis[r0:r0 + m]
-        g = np.zeros((8, 8))
+        scale[b, :m] = dis[list(g)]
+        gm = np.zeros((8, 8))
         for i in range(8):
             for j in range(8):
-                g[i, j] = sum(v * off[j].get(k, 0.0) for k, v in off[i].items())
+                gm[i, j] = sum(v * off[j].get(k, 0.0) for k, v in off[i].items())
         ahat[b] = a
-        hmat[b] = a @ a + LAMBDA * g
+        hmat[b] = a @ a + LAMBDA * gm
         feat[b, :36] = a[np.triu_indices(8)]
         feat[b, 36:44] = offabs
     return feat, ahat, hmat, scale
@@ -137,16 +152,22 @@ def blocks_from_out(out, scale):
     return M * scale[:, :, None] * scale[:, None, :]
 
 
-def block_pcg(A, b, blocks, tau, max_iters=100000):
-    """FP64 PCG with z_b = M_b r_b on aligned 8-row blocks (blocks: nb x 8 x 8).
+def block_pcg(A, b, blocks, tau, max_iters=100000, groups=None):
+    """FP64 PCG with z_b = M_b r_b on the blocks `groups` (natural 8-row
+    groups when None; blocks: nb x 8 x 8 in slot order).
     Returns (x, iterations, rr trace)."""
     n = A.shape[0]
-    nb = (n + 7) // 8
+    if groups is None:
+        groups = natural_groups(n)
+    nb = len(groups)
+    idx = np.full((nb, 8), n)                       # padding slots read / write a dummy entry
+    for k, g in enumerate(groups):
+        idx[k, :len(g)] = g
 
     def apply(r):
-        rp = np.zeros(nb * 8)
-        rp[:n] = r
-        z = np.einsum("bij,bj->bi", blocks, rp.reshape(nb, 8)).ravel()
+        rp = np.append(r, 0.0)[idx]
+        z = np.zeros(n + 1)
+        z[idx] = np.einsum("bij,bj->bi", blocks, rp)
         return z[:n]
 
     x = np.zeros(n)

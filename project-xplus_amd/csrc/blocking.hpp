@@ -1,0 +1,39 @@
+// blocking.hpp -- which rows form the 8-row blocks of the block preconditioner.
+//
+// A block is any set of at most 8 rows of one system: the SELL builder lays
+// each block out as 8 consecutive solve rows (padding rows for smaller
+// blocks), so block-Jacobi is free to group rows by coupling strength rather
+// than by index.  Grouping strongly coupled rows puts more of the matrix
+// inside the blocks: on the thermal-FEM family, exact block inverses need
+// ~30% fewer PCG iterations with strength aggregates than with 8 consecutive
+// rows (1 x 8 node strips).  The reference has no block preconditioner.
+#pragma once
+
+#include <vector>
+
+namespace xp {
+
+// Blocks of one system of n rows: unit u occupies slots [8u, 8u + 8) of
+// `slots`, each a row index in [0, n) or -1 (padding); every row appears in
+// exactly one slot.
+struct Blocking {
+    int n = 0;
+    std::vector<int> slots;
+    int units() const { return static_cast<int>(slots.size() / 8); }
+};
+
+// Rows 8u .. 8u + 7 (the last block padded).
+Blocking natural_blocking(int n);
+
+// Greedy strength aggregation.  s_ij = |a_ij| / sqrt(a_ii a_jj) (a row
+// without a positive diagonal scales by 1).  Seeds are taken in row order;
+// a block grows by the unassigned row with the largest summed strength to
+// the block's rows (ties: the smaller index) until it holds 8 rows or has no
+// unassigned neighbour.  Rows left in blocks of fewer than 8 are then
+// pooled, regrouped by the same rule among themselves, and what is still
+// partial is packed first-fit (blocks in creation order) into blocks of up
+// to 8 rows.  Rows inside a block are in ascending order; blocks are
+// numbered by their smallest row.  Deterministic.
+Blocking strength_blocking(int n, const int* row_ptr, const int* col_idx, const double* values);
+
+}  // namespace xp

@@ -32,6 +32,7 @@
 //  * Workgroup -> rows mapping is XCD-aware: physical workgroup b runs on XCD
 //    b % 8, so logical block (b % 8) * (grid / 8) + b / 8 gives each XCD one
 //    contiguous range of rows and keeps the x entries they share in one L2.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -657,6 +658,9 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
     double v[1] = {0.0};
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         const int row = sl * kC + lane;
+        // the FP64 p and A p streams are requested before the row's gathers
+        const double po = fresh ? 0.0 : p[row];
+        const double apo = fresh ? 0.0 : ap[row];
         float zd;
         const double az = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd);
         double pn, an;
@@ -664,8 +668,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
             pn = static_cast<double>(zd);
             an = az;
         } else {
-            pn = static_cast<double>(zd) + beta * p[row];
-            an = az + beta * ap[row];
+            pn = static_cast<double>(zd) + beta * po;
+            an = az + beta * apo;
         }
         p[row] = pn;
         ap[row] = an;
@@ -1051,6 +1055,28 @@ __global__ void k_mask_padding_blocks(int n_pad, const int* __restrict__ perm, f
         if (pad_row || perm[blk0 + c] < 0) b[c] = c == j ? 1.0f : 0.0f;
 }
 
+// One workgroup-grid launch on L.stream, timed by L's events when set.
+#define XP_LAUNCH(L, K, G, ...)                                                                   \
+    do {                                                                                       \
+        if ((L).ev_start != nullptr)                                                           \
+            hipExtLaunchKernelGGL(K, G, dim3(kBlock), 0, (L).stream, (L).ev_start, (L).ev_stop, 0, \
+                                  __VA_ARGS__);                                                \
+        else                                                                                   \
+            hipLaunchKernelGGL(K, G, dim3(kBlock), 0, (L).stream, __VA_ARGS__);                 \
+    } while (0)
+
+// SELL kernels templated on <IDX16, V>.
+#define XP_LAUNCH_SELL(L, A, K, ...)                                                              \
+    do {                                                                                       \
+        if ((A).idx16) {                                                                       \
+            if ((A).variant == 1) XP_LAUNCH(L, (K<true, 1>), dim3((L).grid), __VA_ARGS__);       \
+            else XP_LAUNCH(L, (K<true, 0>), dim3((L).grid), __VA_ARGS__);                        \
+        } else {                                                                               \
+            if ((A).variant == 1) XP_LAUNCH(L, (K<false, 1>), dim3((L).grid), __VA_ARGS__);      \
+            else XP_LAUNCH(L, (K<false, 0>), dim3((L).grid), __VA_ARGS__);                       \
+        }                                                                                      \
+    } while (0)
+
 #define XP_DISPATCH_SELL(A, KERNEL, ...)                                                            \
     do {                                                                                        \
         if ((A).idx16) {                                                                        \
@@ -1073,28 +1099,26 @@ void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const flo
 }
 
 void launch_pcg_spmv(const Launch& L, const SellView& A, const SysState* st, const float* p, float* ap) {
-    XP_DISPATCH_SELL(A, k_pcg_spmv, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st, p, ap);
+    XP_LAUNCH_SELL(L, A, k_pcg_spmv, L, A, st, p, ap);
 }
 
 void launch_pcg_spmv_fused(const Launch& L, const SellView& A, const SysState* st_cur, SysState* st_next,
                            TraceRow* trace, int trace_stride, const float* z, float* p, float* ap) {
-    XP_DISPATCH_SELL(A, k_pcg_spmv_fused, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st_cur, st_next, trace,
-                     trace_stride, z, p, ap);
+    XP_LAUNCH_SELL(L, A, k_pcg_spmv_fused, L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
 }
 
 void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
                            const float* ap, const float* minv, const float* blockM, float* z) {
-    hipLaunchKernelGGL(k_pcg_update_xrz, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, x, p, r, ap, minv, blockM, z);
+    XP_LAUNCH(L, k_pcg_update_xrz, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
 }
 
 void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
                          int trace_stride, const float* z, float* p) {
-    hipLaunchKernelGGL(k_pcg_update_p, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st_cur, st_next, trace,
-                       trace_stride, z, p);
+    XP_LAUNCH(L, k_pcg_update_p, dim3(L.grid), L, st_cur, st_next, trace, trace_stride, z, p);
 }
 
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y) {
-    XP_DISPATCH_SELL(A, k_spmv_f32, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, y);
+    XP_LAUNCH_SELL(L, A, k_spmv_f32, L, A, x, y);
 }
 
 void launch_spmv_f64(const Launch& L, const SellView& A, bool use_lo, const double* x, double* y) {
@@ -1235,14 +1259,18 @@ void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysSta
 void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, bool use_lo, const SysState* st_cur,
                                SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
                                double* ap) {
-    XP_DISPATCH_ACC(A, use_lo, k_pcg_spmv_fused_acc, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, st_cur, st_next,
-                    trace, trace_stride, z, p, ap);
+    if (A.idx16) {
+        if (use_lo) XP_LAUNCH(L, (k_pcg_spmv_fused_acc<true, 1>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
+        else XP_LAUNCH(L, (k_pcg_spmv_fused_acc<true, 0>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
+    } else {
+        if (use_lo) XP_LAUNCH(L, (k_pcg_spmv_fused_acc<false, 1>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
+        else XP_LAUNCH(L, (k_pcg_spmv_fused_acc<false, 0>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
+    }
 }
 
 void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,
                                const double* ap, const float* minv, const float* blockM, float* z) {
-    hipLaunchKernelGGL(k_pcg_update_xrz_acc, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, x, p, r, ap, minv, blockM,
-                       z);
+    XP_LAUNCH(L, k_pcg_update_xrz_acc, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
 }
 
 void launch_fold_f32_into_f64(int n_pad, float* d, double* x, hipStream_t s) {

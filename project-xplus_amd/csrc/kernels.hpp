@@ -21,6 +21,10 @@ struct Launch {
     int u_blocks = 0;                   // > 0: one system, uniform split of u_slices over u_blocks
     int u_slices = 0;
     hipStream_t stream = nullptr;
+    // Per-kernel timing only (never under graph capture): when set, the
+    // iteration and SpMV launchers dispatch their kernel with these events
+    // recorded at its start and end (hipExtLaunchKernelGGL).
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 
 // ---- FP32 solve-order PCG step kernels (state machine in SysState) ----

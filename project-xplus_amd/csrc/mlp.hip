@@ -280,8 +280,13 @@ struct Buf {
 // ---- host side ---------------------------------------------------------------------
 
 BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const double* values) {
+    return from_csr(n, row_ptr, col_idx, values, natural_blocking(n));
+}
+
+BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, const Blocking& blk) {
+    if (blk.n != n) throw std::runtime_error("blocking does not match the matrix");
     BlockSet s;
-    const int nb = (n + 7) / 8;
+    const int nb = blk.units();
     s.count = nb;
     s.feat.assign(static_cast<size_t>(nb) * kMlpIn, 0.0f);
     s.ahat.assign(static_cast<size_t>(nb) * 64, 0.0f);
@@ -295,30 +300,44 @@ BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const
                 if (values[o] > 0.0) dis[static_cast<size_t>(r)] = 1.0 / std::sqrt(values[o]);
                 break;
             }
+    // slot of every row inside its block
+    std::vector<int> slot_of(static_cast<size_t>(n), -1), block_of(static_cast<size_t>(n), -1);
+    for (int b = 0; b < nb; ++b)
+        for (int j = 0; j < 8; ++j) {
+            const int r = blk.slots[static_cast<size_t>(b) * 8 + static_cast<size_t>(j)];
+            if (r >= 0) {
+                slot_of[static_cast<size_t>(r)] = j;
+                block_of[static_cast<size_t>(r)] = b;
+            }
+        }
     for (int b = 0; b < nb; ++b) {
-        const int r0 = 8 * b, m = std::min(8, n - r0);
+        const int* rows = blk.slots.data() + static_cast<size_t>(b) * 8;
         double a[8][8] = {};
         double off_abs[8] = {};
-        std::vector<std::pair<int, double>> off[8];  // scaled off-block entries per row
-        for (int i = 0; i < m; ++i) {
-            const double di = dis[static_cast<size_t>(r0 + i)];
-            for (int o = row_ptr[r0 + i]; o < row_ptr[r0 + i + 1]; ++o) {
-                const int c = col_idx[o] - r0;
-                const double v = values[o] * di * dis[static_cast<size_t>(col_idx[o])];
-                if (c >= 0 && c < m)
-                    a[i][c] += v;
-                else {
+        std::vector<std::pair<int, double>> off[8];  // scaled off-block entries per slot
+        for (int i = 0; i < 8; ++i) {
+            const int r = rows[i];
+            if (r < 0) {
+                a[i][i] = 1.0;  // padding slot: identity
+                continue;
+            }
+            const double di = dis[static_cast<size_t>(r)];
+            for (int o = row_ptr[r]; o < row_ptr[r + 1]; ++o) {
+                const int c = col_idx[o];
+                const double v = values[o] * di * dis[static_cast<size_t>(c)];
+                if (block_of[static_cast<size_t>(c)] == b) {
+                    a[i][slot_of[static_cast<size_t>(c)]] += v;
+                } else {
                     off_abs[i] += std::fabs(v);
-                    off[i].push_back({col_idx[o], v});
+                    off[i].push_back({c, v});
                 }
             }
         }
-        for (int i = m; i < 8; ++i) a[i][i] = 1.0;  // partial block: identity rows
         float* f = s.feat.data() + static_cast<size_t>(b) * kMlpIn;
         float* ah = s.ahat.data() + static_cast<size_t>(b) * 64;
         float* hm = s.hmat.data() + static_cast<size_t>(b) * 64;
         float* sc = s.scale.data() + static_cast<size_t>(b) * 8;
-        for (int i = 0; i < 8; ++i) sc[i] = i < m ? static_cast<float>(dis[static_cast<size_t>(r0 + i)]) : 1.0f;
+        for (int i = 0; i < 8; ++i) sc[i] = rows[i] >= 0 ? static_cast<float>(dis[static_cast<size_t>(rows[i])]) : 1.0f;
         int t = 0;
         for (int i = 0; i < 8; ++i)
             for (int j = i; j < 8; ++j) f[t++] = static_cast<float>(a[i][j]);

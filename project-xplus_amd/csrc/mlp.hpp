@@ -5,6 +5,8 @@
 #include <memory>
 #include <vector>
 
+#include "blocking.hpp"
+
 namespace xp {
 
 constexpr int kMlpIn = 48;      // 36 block entries + 8 off-block sums + 4 zero pad
@@ -12,14 +14,16 @@ constexpr int kMlpHidden = 64;
 constexpr int kMlpOut = 48;     // 36 entries of L + 12 unused
 constexpr double kMlpLambda = 0.5;  // weight of the off-block term of the training loss
 
-// Features and training targets of the aligned 8-row blocks of a matrix.
+// Features and training targets of the 8-row blocks of a matrix (rows in
+// a block's slot order; padding slots are identity rows with scale 1).
 struct BlockSet {
     int count = 0;
     std::vector<float> feat;   // count x kMlpIn
     std::vector<float> ahat;   // count x 64: A_bb / s_b
     std::vector<float> hmat;   // count x 64: Ahat^2 + lambda G
     std::vector<float> scale;  // count x 8: D^-1/2 of the block's rows (1 on padding rows)
-    static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values);
+    static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values);  // natural blocks
+    static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, const Blocking& b);
     void append(const BlockSet& o);
 };
 

@@ -138,7 +138,11 @@ struct xp_matrix {
     int n = 0, n_pad = 0, nslices = 0, nsys = 0, sigma = 0, idx16 = 0, variant = 0;
     int solvers = 0;                  // live xp_pcg objects built on this matrix
     int layout_version = 0;           // bumped when the work map / partials are reallocated
-    int block_rows = 1;               // 8: aligned 8-row groups kept together
+    int block_rows = 1;               // 8: blocks of up to 8 rows kept together as 8 solve rows
+    int blocking = 0;                 // blocks: 0 consecutive natural rows, 1 strength aggregates
+    std::vector<xp::Blocking> blockings;  // per system (block_rows = 8)
+    std::vector<int> block_unit;      // solve block -> block id (-1: all padding)
+    std::vector<int> unit_first;      // [nsys + 1] first block id per system
     bool keep_lo = false, keep64 = false;
     std::vector<int> h_row_ptr, h_col_idx, h_ns;  // host copy of the CSR batch (for re-sorting)
     std::vector<double> h_values;
@@ -255,7 +259,10 @@ void load_sell(xp_matrix& m, int sigma) {
         rp_off += static_cast<size_t>(nk) + 1;
         nz_off += static_cast<size_t>(rp[nk]);
     }
-    xp::SellHost h = xp::build_sell(systems, sigma, m.keep_lo, m.keep64, m.block_rows);
+    xp::SellHost h = xp::build_sell(systems, sigma, m.keep_lo, m.keep64, m.block_rows,
+                                    m.block_rows == 8 ? &m.blockings : nullptr);
+    m.block_unit = h.block_unit;
+    m.unit_first = h.unit_first;
     m.n = h.n;
     m.n_pad = h.n_pad;
     m.nslices = h.nslices;
@@ -320,6 +327,18 @@ std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, c
         m->h_row_ptr.insert(m->h_row_ptr.end(), a.row_ptr, a.row_ptr + a.n + 1);
         m->h_col_idx.insert(m->h_col_idx.end(), a.col_idx, a.col_idx + nnz);
         m->h_values.insert(m->h_values.end(), a.values, a.values + nnz);
+    }
+    if (m->block_rows == 8) {
+        m->blocking = (o != nullptr && o->blocking == 1) ? 1 : 0;
+        size_t rp_off = 0, nz_off = 0;
+        for (int nk : m->h_ns) {
+            const int* rp = m->h_row_ptr.data() + rp_off;
+            m->blockings.push_back(m->blocking == 1 ? xp::strength_blocking(nk, rp, m->h_col_idx.data() + nz_off,
+                                                                             m->h_values.data() + nz_off)
+                                                    : xp::natural_blocking(nk));
+            rp_off += static_cast<size_t>(nk) + 1;
+            nz_off += static_cast<size_t>(rp[nk]);
+        }
     }
     device();
     load_sell(*m, (o != nullptr && o->sigma > 0) ? o->sigma : 256);
@@ -922,6 +941,14 @@ int xp_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes) {
     });
 }
 
+int xp_memcpy_dtod(void* dst, const void* src, unsigned long long bytes) {
+    return guarded([&] {
+        const hipStream_t s = device().stream;
+        HIPX(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice, s));
+        HIPX(hipStreamSynchronize(s));
+    });
+}
+
 int xp_device_synchronize(void) {
     return guarded([&] { HIPX(hipStreamSynchronize(device().stream)); HIPX(hipDeviceSynchronize()); });
 }
@@ -936,18 +963,25 @@ int xp_spmv_f32_time(xp_matrix* m, int warmup, int reps, double* avg_us) {
         DevBuf<float> x(np), y(np);
         x.upload(xh.data(), np, s);
         for (int k = 0; k < warmup; ++k) xp::launch_spmv_f32(m->launch(s), m->view(), x.p, y.p);
-        hipEvent_t e0, e1;
-        HIPX(hipEventCreate(&e0));
-        HIPX(hipEventCreate(&e1));
-        HIPX(hipEventRecord(e0, s));
-        for (int k = 0; k < reps; ++k) xp::launch_spmv_f32(m->launch(s), m->view(), x.p, y.p);
-        HIPX(hipEventRecord(e1, s));
-        HIPX(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        HIPX(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        *avg_us = 1000.0 * ms / reps;
+        // every launch dispatched with its own start / end events: the
+        // kernel's execution time, as a kernel trace reports it
+        std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps));
+        for (auto& e : ev) HIPX(hipEventCreate(&e));
+        for (int k = 0; k < reps; ++k) {
+            xp::Launch L = m->launch(s);
+            L.ev_start = ev[2 * static_cast<size_t>(k)];
+            L.ev_stop = ev[2 * static_cast<size_t>(k) + 1];
+            xp::launch_spmv_f32(L, m->view(), x.p, y.p);
+        }
+        HIPX(hipStreamSynchronize(s));
+        double total = 0.0;
+        for (int k = 0; k < reps; ++k) {
+            float ms = 0.0f;
+            HIPX(hipEventElapsedTime(&ms, ev[2 * static_cast<size_t>(k)], ev[2 * static_cast<size_t>(k) + 1]));
+            total += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *avg_us = 1000.0 * total / reps;
     });
 }
 
@@ -1078,6 +1112,68 @@ int xp_pcg_get_trace(xp_pcg* s, int sys, double* rows, int cap) {
         }
     });
     return rc != 0 ? rc : copied;
+}
+
+// Average device time of each kernel of the iteration over `iters`
+// iterations launched directly, every kernel dispatched with start / end
+// events (hipExtLaunchKernelGGL), so the figure is the kernels' own
+// execution time, as a kernel trace reports it.  The iterations start from
+// x0 with the convergence test off (tau = 0), so every kernel does its full
+// work; the solver's vectors and state are left as after them.
+int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels) {
+    return guarded([&] {
+        if (s == nullptr || us == nullptr || !s->have_rhs || iters <= 0) throw std::runtime_error("bad argument");
+        xp_pcg& S = *s;
+        xp_matrix& m = *S.m;
+        const int nk = (S.opts.fused || S.opts.accurate) ? 2 : 3;
+        std::vector<hipEvent_t> ev(static_cast<size_t>(iters) * static_cast<size_t>(2 * nk));
+        for (auto& e : ev) HIPX(hipEventCreate(&e));
+        S.reset_x();
+        std::vector<xp::SysState> init = S.st_init;
+        for (auto& t : init) {
+            t.tau = 0.0;
+            t.max_iters = iters + 2;
+            t.stall_limit = 0;
+        }
+        S.st.upload(init.data(), init.size(), S.stream);
+        S.enqueue_init();
+        const xp::SellView A = m.view();
+        const size_t ns = static_cast<size_t>(m.nsys);
+        for (int k = 0; k < iters; ++k) {
+            hipEvent_t* e = ev.data() + static_cast<size_t>(k) * static_cast<size_t>(2 * nk);
+            xp::Launch L1 = m.launch(S.stream), L2 = L1, L3 = L1;
+            L1.ev_start = e[0]; L1.ev_stop = e[1];
+            L2.ev_start = e[2]; L2.ev_stop = e[3];
+            if (nk == 3) { L3.ev_start = e[4]; L3.ev_stop = e[5]; }
+            xp::SysState* cur = S.st.p + (k % 2) * ns;
+            xp::SysState* nxt = S.st.p + ((k + 1) % 2) * ns;
+            if (S.opts.accurate) {
+                xp::launch_pcg_spmv_fused_acc(L1, A, !m.has64, cur, nxt, S.trace.p, S.trace_stride, S.z.p, S.p64.p,
+                                              S.ap64.p);
+                xp::launch_pcg_update_xrz_acc(L2, nxt, S.x64.p, S.p64.p, S.r.p, S.ap64.p, S.minv.p, S.bm(), S.z.p);
+            } else if (S.opts.fused) {
+                xp::launch_pcg_spmv_fused(L1, A, cur, nxt, S.trace.p, S.trace_stride, S.z.p, S.p.p, S.ap.p);
+                xp::launch_pcg_update_xrz(L2, nxt, S.x.p, S.p.p, S.r.p, S.ap.p, S.minv.p, S.bm(), S.z.p);
+            } else {
+                xp::launch_pcg_spmv(L1, A, cur, S.p.p, S.ap.p);
+                xp::launch_pcg_update_xrz(L2, cur, S.x.p, S.p.p, S.r.p, S.ap.p, S.minv.p, S.bm(), S.z.p);
+                xp::launch_pcg_update_p(L3, cur, nxt, S.trace.p, S.trace_stride, S.z.p, S.p.p);
+            }
+        }
+        HIPX(hipStreamSynchronize(S.stream));
+        for (int j = 0; j < 3; ++j) us[j] = 0.0;
+        for (int k = 0; k < iters; ++k) {
+            const hipEvent_t* e = ev.data() + static_cast<size_t>(k) * static_cast<size_t>(2 * nk);
+            for (int j = 0; j < nk; ++j) {
+                float ms = 0.0f;
+                HIPX(hipEventElapsedTime(&ms, e[2 * j], e[2 * j + 1]));
+                us[j] += static_cast<double>(ms) * 1000.0 / iters;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        S.x64_valid = false;
+        if (nkernels != nullptr) *nkernels = nk;
+    });
 }
 
 int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches) {
@@ -1253,7 +1349,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
     return guarded([&] {
         if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run_reliable: call xp_pcg_set_rhs first");
         if (!s->m->has_lo && !s->m->has64) throw std::runtime_error("reliable updates need a matrix built with keep_lo");
-        if (!(rel_tol > 0.0) || !(delta > 0.0) || !(delta < 1.0)) throw std::runtime_error("bad tolerances");
+        if (!(rel_tol > 0.0) || !(delta >= 0.0) || !(delta < 1.0)) throw std::runtime_error("bad tolerances");
         xp_pcg& S = *s;
         xp_matrix& m = *S.m;
         const size_t ns = static_cast<size_t>(m.nsys);
@@ -1311,7 +1407,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
             for (size_t k = 0; k < ns; ++k) {
                 const xp::SysState& t = S.st_host[k];
                 any_active = any_active || t.active;
-                if (t.active && t.rr <= delta * delta * rr_last[k]) need = true;
+                if (delta > 0.0 && t.active && t.rr <= delta * delta * rr_last[k]) need = true;
                 if (!t.active && t.status == xp::kStatusConverged && !verified[k]) need = true;
             }
             if (!need) {
@@ -1608,28 +1704,37 @@ extern "C" int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int i
 namespace {
 
 int num_blocks(const xp_matrix& m) {
+    if (m.block_rows == 8) return m.unit_first.back();
     int nb = 0;
     for (int nk : m.h_ns) nb += (nk + 7) / 8;
     return nb;
 }
 
-// Exact inverse of every natural 8-row diagonal block (FP64 Cholesky; a block
-// that is not positive definite falls back to its diagonal inverse).
+// Exact inverse of every 8-row diagonal block (FP64 Cholesky; a block that
+// is not positive definite falls back to its diagonal inverse).  Block b of
+// a system holds the rows of its Blocking's unit b, in slot order.
 std::vector<float> exact_block_inverses(const xp_matrix& m) {
     std::vector<float> out(static_cast<size_t>(num_blocks(m)) * 64, 0.0f);
     size_t rp_off = 0, nz_off = 0, bid = 0;
-    for (int nk : m.h_ns) {
+    for (size_t k = 0; k < m.h_ns.size(); ++k) {
+        const int nk = m.h_ns[k];
         const int* rp = m.h_row_ptr.data() + rp_off;
         const int* ci = m.h_col_idx.data() + nz_off;
         const double* va = m.h_values.data() + nz_off;
-        for (int b0 = 0; b0 < nk; b0 += 8, ++bid) {
-            const int nb = std::min(8, nk - b0);
+        const xp::Blocking& B = m.blockings[k];
+        for (int u = 0; u < B.units(); ++u, ++bid) {
+            // local rows of the block's slots; the real rows come first in a
+            // block's slots, the padding after them
+            int rows[8];
+            int nb = 0;
+            for (int j = 0; j < 8; ++j)
+                if (B.slots[static_cast<size_t>(u) * 8 + static_cast<size_t>(j)] >= 0)
+                    rows[nb++] = B.slots[static_cast<size_t>(u) * 8 + static_cast<size_t>(j)];
             double a[8][8] = {};
             for (int i = 0; i < nb; ++i)
-                for (int o = rp[b0 + i]; o < rp[b0 + i + 1]; ++o) {
-                    const int c = ci[o] - b0;
-                    if (c >= 0 && c < nb) a[i][c] += va[o];
-                }
+                for (int o = rp[rows[i]]; o < rp[rows[i] + 1]; ++o)
+                    for (int c = 0; c < nb; ++c)
+                        if (ci[o] == rows[c]) a[i][c] += va[o];
             double l[8][8] = {};
             bool ok = true;
             for (int j = 0; j < nb && ok; ++j) {
@@ -1670,33 +1775,16 @@ std::vector<float> exact_block_inverses(const xp_matrix& m) {
     return out;
 }
 
-// Natural block id (systems' blocks numbered consecutively) of solve-order
-// block q, or -1 for an all-padding block.
-int natural_block_of(const xp_matrix& m, const std::vector<int>& block_first, int q) {
-    int g = -1;
-    for (int j = 0; j < 8 && g < 0; ++j) {
-        const int pj = m.perm[static_cast<size_t>(q) * 8 + j];
-        g = pj >= 0 ? pj - j : -1;
-    }
-    if (g < 0) return -1;
-    int k = 0;
-    while (k + 1 < m.nsys && g >= m.sys_nat_begin[static_cast<size_t>(k) + 1]) ++k;
-    return block_first[static_cast<size_t>(k)] + (g - m.sys_nat_begin[static_cast<size_t>(k)]) / 8;
-}
-
-std::vector<int> system_block_first(const xp_matrix& m) {
-    std::vector<int> block_first(m.h_ns.size() + 1, 0);
-    for (size_t k = 0; k < m.h_ns.size(); ++k) block_first[k + 1] = block_first[k] + (m.h_ns[k] + 7) / 8;
-    return block_first;
-}
+// Block id (the blocks of all systems numbered consecutively, each system's
+// in its Blocking's order) of solve-order block q, or -1 for all padding.
+int block_of(const xp_matrix& m, int q) { return m.block_unit[static_cast<size_t>(q)]; }
 
 // Natural-order blocks -> solve-order block array (padding rows: identity).
 std::vector<float> arrange_blocks(const xp_matrix& m, const float* nat) {
     std::vector<float> out(static_cast<size_t>(m.n_pad) * 8, 0.0f);
-    const std::vector<int> block_first = system_block_first(m);
     for (int q = 0; q < m.n_pad / 8; ++q) {
         float* dst = out.data() + static_cast<size_t>(q) * 64;
-        const int bid = natural_block_of(m, block_first, q);
+        const int bid = block_of(m, q);
         if (bid < 0) {  // an all-padding block
             for (int j = 0; j < 8; ++j) dst[j * 8 + j] = 1.0f;
             continue;
@@ -1760,6 +1848,16 @@ void set_preconditioner(xp_pcg& s, int mode, const float* blocks) {
 
 }  // namespace
 
+extern "C" int xp_matrix_block_rows(const xp_matrix* m, int* rows) {
+    return guarded([&] {
+        if (m == nullptr || rows == nullptr) throw std::runtime_error("null argument");
+        if (m->block_rows != 8) throw std::runtime_error("the matrix has no blocks (block_rows = 1)");
+        size_t o = 0;
+        for (size_t k = 0; k < m->blockings.size(); ++k)
+            for (int r : m->blockings[k].slots) rows[o++] = r >= 0 ? m->sys_nat_begin[k] + r : -1;
+    });
+}
+
 extern "C" int xp_matrix_num_blocks(const xp_matrix* m) {
     if (m == nullptr) return -1;
     return num_blocks(*m);
@@ -1816,6 +1914,29 @@ int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const do
         auto b = std::make_unique<xp_blockset>();
         b->set = xp::BlockSet::from_csr(n, row_ptr, col_idx, values);
         *out = b.release();
+    });
+}
+
+int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, const double* values, int blocking,
+                                 xp_blockset** out) {
+    return guarded([&] {
+        if (out == nullptr || row_ptr == nullptr || n <= 0 || blocking < 0 || blocking > 1)
+            throw std::runtime_error("bad argument");
+        auto b = std::make_unique<xp_blockset>();
+        b->set = xp::BlockSet::from_csr(n, row_ptr, col_idx, values,
+                                        blocking == 1 ? xp::strength_blocking(n, row_ptr, col_idx, values)
+                                                      : xp::natural_blocking(n));
+        *out = b.release();
+    });
+}
+
+int xp_strength_blocking(int n, const int* row_ptr, const int* col_idx, const double* values, int* slots,
+                         int* nblocks) {
+    return guarded([&] {
+        if (slots == nullptr || nblocks == nullptr || n < 0) throw std::runtime_error("bad argument");
+        const xp::Blocking b = xp::strength_blocking(n, row_ptr, col_idx, values);
+        std::memcpy(slots, b.slots.data(), b.slots.size() * sizeof(int));
+        *nblocks = b.units();
     });
 }
 
@@ -1899,9 +2020,8 @@ int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale) {
         HIPX(hipMemcpyAsync(hf.data(), f.feat.p, hf.size() * sizeof(float), hipMemcpyDeviceToHost, st));
         HIPX(hipMemcpyAsync(hs.data(), f.scale.p, hs.size() * sizeof(float), hipMemcpyDeviceToHost, st));
         HIPX(hipStreamSynchronize(st));
-        const std::vector<int> block_first = system_block_first(*m);
         for (int q = 0; q < f.blocks; ++q) {
-            const int bid = natural_block_of(*m, block_first, q);
+            const int bid = block_of(*m, q);
             if (bid < 0) continue;
             std::memcpy(feat + static_cast<size_t>(bid) * 48, hf.data() + static_cast<size_t>(q) * 48, 48 * sizeof(float));
             std::memcpy(scale + static_cast<size_t>(bid) * 8, hs.data() + static_cast<size_t>(q) * 8, 8 * sizeof(float));

@@ -13,12 +13,16 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 }  // namespace
 
-SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64, int block_rows) {
+SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64, int block_rows,
+                    const std::vector<Blocking>* blocks) {
     if (systems.empty()) throw std::runtime_error("build_sell: no systems");
     if (sigma <= 0) sigma = kC;
     if (sigma % kC != 0) throw std::runtime_error("build_sell: sigma must be a multiple of 64");
     if (block_rows != 1 && block_rows != 8) throw std::runtime_error("build_sell: block_rows must be 1 or 8");
     const int br = block_rows;
+
+    if (blocks != nullptr && (br != 8 || blocks->size() != systems.size()))
+        throw std::runtime_error("build_sell: blocks need block_rows = 8 and one Blocking per system");
 
     SellHost s;
     s.nsys = static_cast<int>(systems.size());
@@ -26,14 +30,32 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
     s.block_rows = br;
     s.sys_row_begin.assign(systems.size() + 1, 0);
     s.sys_nat_begin.assign(systems.size() + 1, 0);
+    s.unit_first.assign(systems.size() + 1, 0);
+    // the units (blocks of br solve rows) of every system
+    std::vector<Blocking> units(systems.size());
     for (size_t k = 0; k < systems.size(); ++k) {
         const CsrRef& a = systems[k];
         if (a.n < 0) throw std::runtime_error("build_sell: negative n");
         if (a.n > 0 && (a.row_ptr == nullptr || a.col_idx == nullptr || a.values == nullptr))
             throw std::runtime_error("build_sell: null CSR array");
+        if (br == 8) {
+            units[k] = blocks != nullptr ? (*blocks)[k] : natural_blocking(a.n);
+            const Blocking& b = units[k];
+            if (b.n != a.n || b.slots.size() % 8 != 0) throw std::runtime_error("build_sell: blocking does not match the system");
+            std::vector<char> seen(static_cast<size_t>(a.n), 0);
+            for (int r : b.slots) {
+                if (r < -1 || r >= a.n || (r >= 0 && seen[static_cast<size_t>(r)]))
+                    throw std::runtime_error("build_sell: blocking is not a partition of the rows");
+                if (r >= 0) seen[static_cast<size_t>(r)] = 1;
+            }
+            for (char c : seen)
+                if (!c) throw std::runtime_error("build_sell: blocking is not a partition of the rows");
+        }
+        const int nu = br == 8 ? units[k].units() : a.n;
+        s.unit_first[k + 1] = s.unit_first[k] + (br == 8 ? nu : 0);
         // Every system gets at least one slice so that it owns a workgroup;
-        // with blocks, a partial last block keeps its own 8 solve rows.
-        const int padded = std::max(round_up(round_up(a.n, br), kC), kC);
+        // every block keeps its own 8 solve rows.
+        const int padded = std::max(round_up(nu * br, kC), kC);
         s.sys_row_begin[k + 1] = s.sys_row_begin[k] + padded;
         s.sys_nat_begin[k + 1] = s.sys_nat_begin[k] + a.n;
     }
@@ -74,15 +96,20 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
     // Permutation: stable sort by descending off-diagonal count inside each
     // window of sigma rows of each system (units of br rows, keyed by the
     // longest row of the unit, when block_rows = 8).
+    if (br == 8) s.block_unit.assign(static_cast<size_t>(s.n_pad / 8), -1);
     for (int k = 0; k < s.nsys; ++k) {
         const int nat0 = s.sys_nat_begin[static_cast<size_t>(k)];
         const int n_k = s.sys_nat_begin[static_cast<size_t>(k) + 1] - nat0;
         const int solve0 = s.sys_row_begin[static_cast<size_t>(k)];
-        const int nu = (n_k + br - 1) / br;  // units
+        const int nu = br == 8 ? units[static_cast<size_t>(k)].units() : n_k;
+        // local row of slot j of unit u (-1: padding)
+        auto slot = [&](int u, int j) { return br == 8 ? units[static_cast<size_t>(k)].slots[static_cast<size_t>(u) * 8 + static_cast<size_t>(j)] : u; };
         std::vector<int> key(static_cast<size_t>(nu), 0);
         for (int u = 0; u < nu; ++u)
-            for (int j = 0; j < br && u * br + j < n_k; ++j)
-                key[static_cast<size_t>(u)] = std::max(key[static_cast<size_t>(u)], offcnt[static_cast<size_t>(nat0 + u * br + j)]);
+            for (int j = 0; j < br; ++j) {
+                const int r = slot(u, j);
+                if (r >= 0) key[static_cast<size_t>(u)] = std::max(key[static_cast<size_t>(u)], offcnt[static_cast<size_t>(nat0 + r)]);
+            }
         std::vector<int> order(static_cast<size_t>(nu));
         std::iota(order.begin(), order.end(), 0);
         const int wu = sigma / br;  // units per window
@@ -93,8 +120,11 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
         }
         for (int pos = 0; pos < nu; ++pos) {
             const int u = order[static_cast<size_t>(pos)];
-            for (int j = 0; j < br && u * br + j < n_k; ++j) {
-                const int g = nat0 + u * br + j;
+            if (br == 8) s.block_unit[static_cast<size_t>(solve0 / 8 + pos)] = s.unit_first[static_cast<size_t>(k)] + u;
+            for (int j = 0; j < br; ++j) {
+                const int r = slot(u, j);
+                if (r < 0) continue;
+                const int g = nat0 + r;
                 const int i = solve0 + pos * br + j;
                 s.perm[static_cast<size_t>(i)] = g;
                 s.iperm[static_cast<size_t>(g)] = i;

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "blocking.hpp"
 #include "xp_types.hpp"
 
 namespace xp {
@@ -45,15 +46,21 @@ struct SellHost {
     std::vector<uint16_t> col16;
     std::vector<int> col32;
     std::vector<uint8_t> has_diag;  // per solve row: 1 if the CSR row had a diagonal entry
+    // block_rows = 8: block (unit) id of every 8-row solve block (units of
+    // all systems numbered consecutively), -1 for an all-padding block
+    std::vector<int> block_unit;
+    std::vector<int> unit_first;    // [nsys + 1] first unit id of each system
 };
 
 // Builds the SELL form of a block-diagonal batch of CSR systems.  sigma is
 // the sort window in rows (a multiple of 64, or 0/64 for no sorting).
 // keep_lo / keep64 request the hi+lo and exact-FP64 value streams.
 // Throws std::runtime_error on malformed input.
-// block_rows = 8 keeps aligned groups of 8 natural rows together as 8
-// consecutive solve rows (sorted as units), for block preconditioners.
+// block_rows = 8 keeps each block of rows together as 8 consecutive solve
+// rows (sorted as units; smaller blocks padded), for block preconditioners:
+// the blocks of `blocks` (one Blocking per system), or 8 consecutive natural
+// rows when it is null.
 SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64,
-                    int block_rows = 1);
+                    int block_rows = 1, const std::vector<Blocking>* blocks = nullptr);
 
 }  // namespace xp

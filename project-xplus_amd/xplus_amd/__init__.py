@@ -42,7 +42,7 @@ STATUS_NAMES = {0: "converged", 1: "max_iter", 2: "breakdown", 3: "stagnation", 
 
 class MatrixOpts(C.Structure):
     _fields_ = [("sigma", C.c_int), ("keep_lo", C.c_int), ("keep64", C.c_int), ("grid", C.c_int),
-                ("spmv_variant", C.c_int), ("block_rows", C.c_int)]
+                ("spmv_variant", C.c_int), ("block_rows", C.c_int), ("blocking", C.c_int)]
 
 
 class MatrixInfo(C.Structure):
@@ -109,6 +109,7 @@ _dev_alloc = _sig("xp_device_alloc", C.c_int, [C.c_ulonglong, C.POINTER(_vp)])
 _dev_free = _sig("xp_device_free", C.c_int, [_vp])
 _htod = _sig("xp_memcpy_htod", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dtoh = _sig("xp_memcpy_dtoh", C.c_int, [_vp, _vp, C.c_ulonglong])
+_dtod = _sig("xp_memcpy_dtod", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dev_sync = _sig("xp_device_synchronize", C.c_int, [])
 _spmv_time = _sig("xp_spmv_f32_time", C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)])
 
@@ -121,6 +122,7 @@ _p_run_refined = _sig("xp_pcg_run_refined", C.c_int,
                       [_vp, C.c_double, C.c_double, C.c_int, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
 _p_run_reliable = _sig("xp_pcg_run_reliable", C.c_int,
                        [_vp, C.c_double, C.c_double, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
+_p_kernel_times = _sig("xp_pcg_kernel_times", C.c_int, [_vp, C.c_int, _f64p, C.POINTER(C.c_int)])
 _p_status = _sig("xp_pcg_get_status", C.c_int, [_vp, C.c_int, C.POINTER(Status)])
 _p_get_x = _sig("xp_pcg_get_x", C.c_int, [_vp, _f64p])
 _p_trace = _sig("xp_pcg_get_trace", C.c_int, [_vp, C.c_int, _f64p, C.c_int])
@@ -137,6 +139,11 @@ _mlp_nparams = _sig("xp_mlp_num_params", C.c_int, [])
 _mlp_get = _sig("xp_mlp_get_params", C.c_int, [_vp, _f32p])
 _mlp_set = _sig("xp_mlp_set_params", C.c_int, [_vp, _f32p])
 _bs_from = _sig("xp_blockset_from_csr", C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)])
+_bs_from_blocked = _sig("xp_blockset_from_csr_blocked", C.c_int,
+                        [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.POINTER(_vp)])
+_strength_blocking = _sig("xp_strength_blocking", C.c_int,
+                          [C.c_int, _i32p, _i32p, _f64p, _i32p, C.POINTER(C.c_int)])
+_m_block_rows = _sig("xp_matrix_block_rows", C.c_int, [_vp, _i32p])
 _bs_append = _sig("xp_blockset_append", C.c_int, [_vp, _vp])
 _bs_size = _sig("xp_blockset_size", C.c_int, [_vp])
 _bs_copy = _sig("xp_blockset_copy", C.c_int, [_vp, _f32p, _f32p, _f32p, _f32p])
@@ -347,6 +354,11 @@ class DeviceArray:
         _check(_dtoh(out.ctypes.data_as(_vp), self.h, out.nbytes))
         return out
 
+    def copy_from(self, other: "DeviceArray") -> None:
+        """Device-to-device copy of other's bytes (the runtime's copy kernel)."""
+        n = min(self.count * self.dtype.itemsize, other.count * other.dtype.itemsize)
+        _check(_dtod(self.h, other.h, n))
+
     def close(self) -> None:
         if getattr(self, "h", None) and self.h.value:
             _dev_free(self.h)
@@ -365,11 +377,13 @@ class Matrix:
     """Device-resident SELL-C-sigma matrix (one system or a batch)."""
 
     def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0,
-                 spmv_variant: int = 0, block_rows: int = 1):
+                 spmv_variant: int = 0, block_rows: int = 1, blocking: int = 0):
+        """blocking (with block_rows = 8): 0 blocks of 8 consecutive rows,
+        1 greedy strength aggregates."""
         if isinstance(systems, Csr):
             systems = [systems]
         self.systems = list(systems)
-        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant, block_rows)
+        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant, block_rows, blocking)
         h = _vp()
         if len(self.systems) == 1:
             a = self.systems[0]
@@ -392,6 +406,12 @@ class Matrix:
 
     def num_blocks(self) -> int:
         return _m_num_blocks(self.h)
+
+    def block_rows(self) -> np.ndarray:
+        """Rows of every block (global natural rows, -1 padding): nblocks x 8."""
+        out = np.empty(self.num_blocks() * 8, np.int32)
+        _check(_m_block_rows(self.h, out))
+        return out.reshape(-1, 8)
 
     def block_features(self):
         """Device-computed block features (natural block order): feat nb x 48, scale nb x 8."""
@@ -541,6 +561,13 @@ class Pcg:
     def set_mlp_preconditioner(self, net: "Mlp") -> None:
         _check(_p_set_mlp(self.h, net.h))
 
+    def kernel_times(self, iters: int = 100):
+        """Average device time (us) of each kernel of the iteration (HIP events)."""
+        us = np.zeros(3)
+        nk = C.c_int()
+        _check(_p_kernel_times(self.h, iters, us, C.byref(nk)))
+        return us[:nk.value]
+
     def last_timing(self):
         ms, launches = C.c_double(), C.c_int()
         _check(_p_timing(self.h, C.byref(ms), C.byref(launches)))
@@ -551,11 +578,11 @@ class Pcg:
 class BlockSet:
     """Features / training targets of the aligned 8-row blocks (host)."""
 
-    def __init__(self, a: Csr = None):
+    def __init__(self, a: Csr = None, blocking: int = 0):
         self.h = None
         if a is not None:
             h = _vp()
-            _check(_bs_from(a.n, a.row_ptr, a.col_idx, a.values, C.byref(h)))
+            _check(_bs_from_blocked(a.n, a.row_ptr, a.col_idx, a.values, blocking, C.byref(h)))
             self.h = h
 
     def append(self, other: "BlockSet") -> None:
@@ -582,13 +609,21 @@ class BlockSet:
         self.close()
 
 
-def blockset(systems) -> BlockSet:
+def blockset(systems, blocking: int = 0) -> BlockSet:
     if isinstance(systems, Csr):
         systems = [systems]
-    out = BlockSet(systems[0])
+    out = BlockSet(systems[0], blocking)
     for a in systems[1:]:
-        out.append(BlockSet(a))
+        out.append(BlockSet(a, blocking))
     return out
+
+
+def strength_blocking(a: Csr) -> np.ndarray:
+    """Greedy strength aggregation of a (no GPU): nblocks x 8 local rows, -1 padding."""
+    slots = np.empty(8 * max(a.n, 1), np.int32)
+    nb = C.c_int()
+    _check(_strength_blocking(a.n, a.row_ptr, a.col_idx, a.values, slots, C.byref(nb)))
+    return slots[:8 * nb.value].reshape(-1, 8)
 
 
 class Mlp:

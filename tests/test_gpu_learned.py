@@ -27,6 +27,7 @@ import xplus_amd as X
 
 sys.path.insert(0, str(O.ROOT / "oracle"))
 import block_mlp as B  # noqa: E402
+import blocking as BK  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -123,24 +124,27 @@ def test_training_is_deterministic_and_descends(gpu):
     assert np.mean(h1[-10:]) < 0.7 * np.mean(h1[:5])
 
 
-@pytest.mark.parametrize("which", ["c1", "fem"])
-def test_block_pcg_matches_oracle(gpu, which):
+@pytest.mark.parametrize("which,blocking", [("c1", 0), ("fem", 0), ("c1", 1), ("fem", 1)])
+def test_block_pcg_matches_oracle(gpu, which, blocking):
     """Full solves on the config-1 matrix; on the 1e3-contrast FEM matrix FP32
     storage moves the iterates (as for plain Jacobi), so there the bars are
-    checked at a fixed K = 20."""
+    checked at a fixed K = 20.  blocking 1: strength aggregates (the oracle
+    gets the oracle's own aggregation)."""
     a = X.gen_diffusion2d(32, 32, 1.0, 20265001) if which == "c1" else X.gen_fem2d(34, 1.0, 4, 1e3, 0.3, 7)
     tau = 1e-12 * float(a.b @ a.b)
     K = None if which == "c1" else 20
-    m = X.Matrix(a, block_rows=8, keep_lo=True)
+    m = X.Matrix(a, block_rows=8, keep_lo=True, blocking=blocking)
+    groups = BK.strength_blocking(a.row_ptr, a.col_idx, a.values) if blocking else B.natural_groups(a.n)
+    rows = m.block_rows()
+    assert [[int(r) for r in row if r >= 0] for row in rows] == groups
     # exact block inverses (mode 1) and arbitrary SPD blocks from a network (mode 2)
     net = X.Mlp(seed=21)
-    net_blocks = net.blocks(X.blockset(a)).astype(np.float64)
+    net_blocks = net.blocks(X.blockset(a, blocking)).astype(np.float64)
     A = a.to_scipy()
-    nb = (a.n + 7) // 8
-    exact = np.zeros((nb, 8, 8))
-    for b in range(nb):
-        r0, k = 8 * b, min(8, a.n - 8 * b)
-        exact[b, :k, :k] = np.linalg.inv(A[r0:r0 + k, r0:r0 + k].toarray())
+    exact = np.zeros((len(groups), 8, 8))
+    for b, g in enumerate(groups):
+        k = len(g)
+        exact[b, :k, :k] = np.linalg.inv(A[g][:, g].toarray())
         exact[b, k:, k:] = np.eye(8 - k)
     for mode, blocks in ((1, exact), (2, net_blocks)):
         s = X.Pcg(m, tau=tau if K is None else 1e-300, trace_cap=5000)
@@ -152,8 +156,8 @@ def test_block_pcg_matches_oracle(gpu, which):
             s.run_fixed(K)
         st = s.status()
         x_ref, it_ref, tr_ref = B.block_pcg(A, a.b, blocks.astype(np.float32).astype(np.float64),
-                                            tau if K is None else 1e-300, max_iters=K or 100000)
-        print(f"{which} mode {mode}: gpu iters {st['iterations']} oracle {it_ref}", flush=True)
+                                            tau if K is None else 1e-300, max_iters=K or 100000, groups=groups)
+        print(f"{which} blocking {blocking} mode {mode}: gpu iters {st['iterations']} oracle {it_ref}", flush=True)
         assert st["status"] == (X.CONVERGED if K is None else X.MAX_ITER)
         assert abs(st["iterations"] - it_ref) <= 1
         x = s.x()
@@ -171,7 +175,7 @@ def test_block_pcg_matches_oracle(gpu, which):
         np.testing.assert_allclose(tr[:k, 3][early], tr_ref[:k][early], rtol=1e-4)
 
 
-@pytest.mark.parametrize("case", ["c1", "fem", "ragged", "batch"])
+@pytest.mark.parametrize("case", ["c1", "fem", "ragged", "batch", "fem_agg", "batch_agg"])
 def test_device_features_match_oracle(gpu, case):
     """Features computed on the GPU from the SELL matrix (solve order, mapped
     back to natural block order) against the oracle's features_from_csr."""
@@ -181,13 +185,17 @@ def test_device_features_match_oracle(gpu, case):
         systems = [X.gen_fem2d(40, 1.0, 4, 1e3, 0.3, 31)]
     elif case == "ragged":                                  # n = 143: a partial last block
         systems = [X.gen_diffusion2d(13, 11, 1.0, 7)]
+    elif case == "fem_agg":
+        systems = [X.gen_fem2d(40, 1.0, 4, 1e3, 0.3, 31)]
     else:                                                    # two systems of uneven size
         systems = [X.gen_diffusion2d(13, 11, 1.0, 8), X.gen_fem2d(30, 1.0, 3, 1e3, 0.3, 9)]
-    m = X.Matrix(systems, sigma=256, block_rows=8, keep_lo=True)
+    blocking = 1 if case.endswith("_agg") else 0
+    m = X.Matrix(systems, sigma=256, block_rows=8, keep_lo=True, blocking=blocking)
     feat, scale = m.block_features()
     ref_f, ref_s = [], []
     for a in systems:
-        f, _, _, sc = B.features_from_csr(a.row_ptr, a.col_idx, a.values)
+        groups = BK.strength_blocking(a.row_ptr, a.col_idx, a.values) if blocking else None
+        f, _, _, sc = B.features_from_csr(a.row_ptr, a.col_idx, a.values, groups)
         ref_f.append(f)
         ref_s.append(sc)
     ref_f, ref_s = np.concatenate(ref_f), np.concatenate(ref_s)
@@ -197,21 +205,22 @@ def test_device_features_match_oracle(gpu, case):
     np.testing.assert_allclose(scale, ref_s, rtol=1e-6)
 
 
-def test_device_mlp_preconditioner_matches_host_blocks(gpu):
+@pytest.mark.parametrize("blocking", [0, 1])
+def test_device_mlp_preconditioner_matches_host_blocks(gpu, blocking):
     """set_mlp_preconditioner (device features + inference + solve-order
     blocks) against mode 2 with host-built blocks of the same network."""
     train = X.gen_fem2d(66, 1.0, 6, 1e3, 0.3, 20265300)
     a = X.gen_fem2d(61, 1.0, 6, 1e3, 0.3, 20265301)          # n = 3481: a partial last block
     net = X.Mlp(seed=4)
-    net.train(X.blockset(train), 100, batch=512, lr=2e-3, seed=2)
-    m = X.Matrix(a, block_rows=8, keep_lo=True)
+    net.train(X.blockset(train, blocking), 100, batch=512, lr=2e-3, seed=2)
+    m = X.Matrix(a, block_rows=8, keep_lo=True, blocking=blocking)
     res = []
     for how in ("device", "host"):
         s = X.Pcg(m, rel_tol=1e-6, stall_limit=0)
         if how == "device":
             s.set_mlp_preconditioner(net)
         else:
-            s.set_preconditioner(2, net.blocks(X.blockset(a)))
+            s.set_preconditioner(2, net.blocks(X.blockset(a, blocking)))
         s.set_rhs(a.b)
         s.run()
         res.append((s.status(), s.x()))
@@ -223,29 +232,33 @@ def test_device_mlp_preconditioner_matches_host_blocks(gpu):
 
 
 def test_trained_network_beats_jacobi_on_held_out(gpu):
-    """Train on FEM matrices of one seed split, compare on another."""
+    """Train on FEM matrices of one seed split, compare on another, with
+    blocks of consecutive rows and with strength aggregates."""
     train = [X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265100 + i) for i in range(3)]
     test = X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265200)
-    net = X.Mlp(seed=5)
-    hist = net.train(X.blockset(train), 400, batch=4096, lr=2e-3, seed=1)
-    print("train loss", hist[0], "->", hist[-1], flush=True)
     tau = (1e-6 * np.linalg.norm(test.b)) ** 2
     results = {}
-    m = X.Matrix(test, block_rows=8, keep_lo=True)
-    for name in ("jacobi", "exact_block", "learned"):
-        s = X.Pcg(m, tau=tau, stall_limit=0)
-        if name == "exact_block":
-            s.set_preconditioner(1)
-        elif name == "learned":
-            s.set_mlp_preconditioner(net)
-        s.set_rhs(test.b)
-        s.run()
-        st = s.status()
-        assert st["status"] == X.CONVERGED, (name, st)
-        results[name] = st["iterations"]
-    print("held-out iterations:", results, "gain learned vs jacobi: %.3f" %
-          (results["jacobi"] / results["learned"]), flush=True)
-    assert results["learned"] < results["jacobi"]
+    for blocking in (0, 1):
+        net = X.Mlp(seed=5)
+        hist = net.train(X.blockset(train, blocking), 400, batch=4096, lr=2e-3, seed=1)
+        print(f"blocking {blocking}: train loss", hist[0], "->", hist[-1], flush=True)
+        m = X.Matrix(test, block_rows=8, keep_lo=True, blocking=blocking)
+        for name in ("jacobi", "exact_block", "learned"):
+            s = X.Pcg(m, tau=tau, stall_limit=0)
+            if name == "exact_block":
+                s.set_preconditioner(1)
+            elif name == "learned":
+                s.set_mlp_preconditioner(net)
+            s.set_rhs(test.b)
+            s.run()
+            st = s.status()
+            assert st["status"] == X.CONVERGED, (name, st)
+            results[(name, blocking)] = st["iterations"]
+    print("held-out iterations:", results, "gain learned vs jacobi: strips %.3f, aggregates %.3f" %
+          (results[("jacobi", 0)] / results[("learned", 0)], results[("jacobi", 1)] / results[("learned", 1)]),
+          flush=True)
+    assert results[("learned", 0)] < results[("jacobi", 0)]
+    assert results[("learned", 1)] < results[("learned", 0)]
 
 
 def test_useless_blocks_fall_back_to_jacobi(gpu):

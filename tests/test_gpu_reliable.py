@@ -168,8 +168,13 @@ def test_accurate_fem_true_residual(gpu, block):
     rec_rel = true_rel(a, s.x())
     rel, repl, iters = s.run_reliable(tol, 0.1)
     x = s.x()
+    rel0, repl0, iters0 = s.run_reliable(tol, 0.0)     # replacement only once converged
     print(f"fem accurate block={block}: run {st['iterations']} iterations (true rel {rec_rel:.3e}), "
-          f"reliable {iters} ({repl} replacements, rel {rel[0]:.3e}); golden {g['iterations']}", flush=True)
+          f"reliable {iters} ({repl} replacements, rel {rel[0]:.3e}), verify-only {iters0} ({repl0}, "
+          f"rel {rel0[0]:.3e}); golden {g['iterations']}", flush=True)
+    assert rel0[0] <= tol
+    assert true_rel(a, s.x()) <= tol * (1 + 1e-6)
+    assert iters0 <= st["iterations"] + 64
     assert st["status"] == X.CONVERGED
     assert rec_rel <= 2 * tol           # the recursive residual tracks the true one
     assert rel[0] <= tol
