@@ -81,6 +81,18 @@ def spmv_roofline(m, a, reps: int = 200) -> dict:
     return r
 
 
+def profiled_traffic() -> dict:
+    """HBM bytes per launch measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes committed under profiles/ (lines 'traffic_bytes_per_launch K B')."""
+    out = {}
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
+                out[parts[1]] = (int(parts[2]), f"profiles/{f.name}")
+    return out
+
+
 def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
     """The accurate iteration's two kernels timed with HIP events between
     launches on the solver stream; the dominant one is k_pcg_spmv_fused_acc."""
@@ -92,6 +104,10 @@ def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
     r["update_kernel"] = roof("k_pcg_update_xrz_acc", b_upd, us[1],
                               ("76" if block else "48") + " n (x r/w FP64, p, A p FP64, r r/w, "
                               + ("8x8 block row 32 B" if block else "1/diag") + ", z)")
+    prof = profiled_traffic()
+    for rr in (r, r["update_kernel"]):
+        if rr["kernel"] in prof:
+            rr["traffic"], rr["traffic_source"] = prof[rr["kernel"]]
     r["iteration_us"] = round(float(us[0] + us[1]), 3)
     r["iteration_gbs"] = round((b_fused + b_upd) / ((us[0] + us[1]) * 1e-6) / 1e9, 1)
     return r
@@ -188,10 +204,16 @@ def profile_run(X, a, args) -> int:
         dst.copy_from(src)
     src.close()
     dst.close()
+    # the standalone SpMV on the Jacobi matrix (the layout the autotuner picks
+    # for this workload: sigma 64, 512 workgroups)
+    m = X.Matrix(a, sigma=64, grid=512)
+    spmv = spmv_roofline(m, a)
+    m.close()
     net, _ = train_network(X, args)
     ml, _ = learned_matrix(X, a, args, tune=False)
     out = accurate_solve(X, a, ml, args, net=net, kernels=True)
     out["matrix"] = ml.info()
+    out["spmv_roofline"] = spmv
     print(json.dumps({"profile": "learned accurate solve", **out}), flush=True)
     return 0
 
