@@ -251,9 +251,13 @@ int xp_pcg_true_residual(xp_pcg* s, double* out_norms);
  * 8 natural rows of each system (the last one may be partial), each an 8x8
  * row-major FP32 matrix M_b applied as z_b = M_b r_b.
  *   mode 0: Jacobi;  mode 1: exact block Jacobi (M_b = A_bb^-1, FP64 Cholesky
- *   on the host);  mode 2: user blocks from `blocks` (nblocks x 64 floats,
- *   natural block order).  With a block preconditioner and stall_limit > 0,
- *   a stagnating or breaking-down solve is redone with Jacobi (fallback). */
+ *   on the host);  mode 2: user SPD blocks from `blocks` (nblocks x 64
+ *   floats, block order).  The device stores each block factored,
+ *   M_b = S C C^T S with s_i = sqrt(M_b,ii) and C the Cholesky factor of
+ *   S^-1 M_b S^-1 in FP16 (14 bytes per row instead of 32); a block that is
+ *   not positive definite keeps only its diagonal.  With a block
+ *   preconditioner and stall_limit > 0, a stagnating or breaking-down solve
+ *   is redone with Jacobi (fallback). */
 int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks);
 /* Number of 8-row blocks of the matrix (all systems): sum ceil(n_k / 8)
  * for consecutive-row blocks, the aggregate count for strength blocking.
@@ -304,9 +308,10 @@ int xp_mlp_get_params(xp_mlp* net, float* out);
 int xp_mlp_set_params(xp_mlp* net, const float* params);
 /* Blocks of a host CSR system (no GPU needed); append concatenates sets. */
 int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, xp_blockset** out);
-/* The same over the blocks of a blocking (0: consecutive rows, 1: strength). */
+/* The same over the blocks of a blocking (0: consecutive rows, 1: strength),
+ * with the off-block weight of the training loss (< 0: 0.5). */
 int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, const double* values, int blocking,
-                                 xp_blockset** out);
+                                 double lambda, xp_blockset** out);
 int xp_blockset_append(xp_blockset* dst, const xp_blockset* src);
 int xp_blockset_size(const xp_blockset* set);
 /* feat: size x 48, ahat: size x 64, hmat (Ahat^2 + 0.5 G): size x 64, scale (D^-1/2): size x 8. */

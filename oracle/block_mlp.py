@@ -8,8 +8,13 @@ the independent CPU restatement its GPU kernels are checked against:
                         H = Ahat^2 + LAMBDA G, D^-1/2 per row, for natural
                         8-row groups or any blocks of at most 8 rows
   forward            -- the MLP 48 -> 64 -> 64 -> 48 (ReLU, ReLU, linear)
+  l0_from_feat       -- L0 = chol(Ahat^-1), the exact block inverse's lower
+                        factor (numpy Cholesky of numpy's inverse); the
+                        network's output is a correction: L = L0 + tril(out)
   loss_grad          -- mean block-SPAI loss and its analytic gradient
   blocks_from_out    -- M_b = D^-1/2 (L L^T + 1e-3 I) D^-1/2
+  stored_blocks      -- the operator the GPU applies for blocks M_b: s = sqrt(diag M_b),
+                        C = chol(S^-1 M_b S^-1) rounded to FP16, S C C^T S
   block_pcg          -- FP64 PCG with a block-diagonal preconditioner
 
 Pinning: loss_grad against central finite differences of loss (FP64,
@@ -120,11 +125,37 @@ def l_from_out(out):
     return L
 
 
+def ahat_from_feat(feat):
+    A = np.zeros((feat.shape[0], 8, 8))
+    iu = np.triu_indices(8)
+    A[:, iu[0], iu[1]] = feat[:, :36]
+    A[:, iu[1], iu[0]] = feat[:, :36]
+    return A
+
+
+def l0_from_feat(feat):
+    """Lower Cholesky factor of Ahat^-1 per block (identity where Ahat is not
+    positive definite), Ahat rebuilt from the feature row."""
+    A = ahat_from_feat(np.asarray(feat, dtype=np.float64))
+    L0 = np.zeros_like(A)
+    for k in range(A.shape[0]):
+        try:
+            np.linalg.cholesky(A[k])
+            L0[k] = np.linalg.cholesky(np.linalg.inv(A[k]))
+        except np.linalg.LinAlgError:
+            L0[k] = np.eye(8)
+    return L0
+
+
+def factor(out, feat):
+    return l0_from_feat(feat) + l_from_out(out)
+
+
 def loss_grad(params, feat, ahat, hmat):
     """Mean over samples of tr(M H M^T) - 2 tr(M A) + 8, M = L L^T + eps I."""
     out, (x, z1, h1, z2, h2) = forward(params, feat, keep=True)
     n = out.shape[0]
-    L = l_from_out(out)
+    L = factor(out, feat)
     M = L @ np.transpose(L, (0, 2, 1)) + EPS_M * np.eye(8)
     MH = M @ hmat
     loss = np.einsum("nij,nij->n", MH, M) - 2.0 * np.einsum("nij,nji->n", M, ahat) + 8.0
@@ -146,10 +177,31 @@ def loss_grad(params, feat, ahat, hmat):
     return float(loss.mean()), pack(g)
 
 
-def blocks_from_out(out, scale):
-    L = l_from_out(out)
+def blocks_from_out(out, scale, feat):
+    L = factor(out, feat)
     M = L @ np.transpose(L, (0, 2, 1)) + EPS_M * np.eye(8)
     return M * scale[:, :, None] * scale[:, None, :]
+
+
+def stored_blocks(blocks):
+    """What the device stores and applies for SPD blocks (nb x 8 x 8):
+    S C C^T S with s_i = sqrt(M_ii) and C the Cholesky factor of the
+    unit-diagonal S^-1 M S^-1 rounded to FP16 (a block that is not positive
+    definite keeps its diagonal)."""
+    blocks = np.asarray(blocks, dtype=np.float64)
+    out = np.empty_like(blocks)
+    for k, M in enumerate(blocks):
+        d = np.diag(M)
+        s = np.where(d > 0, np.sqrt(np.where(d > 0, d, 1.0)), 1.0)
+        N = M / np.outer(s, s)
+        try:
+            if not np.all(d > 0):
+                raise np.linalg.LinAlgError
+            C = np.linalg.cholesky(N).astype(np.float16).astype(np.float64)
+        except np.linalg.LinAlgError:
+            C = np.eye(8)
+        out[k] = np.outer(s, s) * (C @ C.T)
+    return out
 
 
 def block_pcg(A, b, blocks, tau, max_iters=100000, groups=None):

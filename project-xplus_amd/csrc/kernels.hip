@@ -147,25 +147,23 @@ __device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int 
     const size_t e0 = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
     const int base = A.slice_base[slice];
     if (V == 1 && w <= kFastW) {
+        // Branch-free: slots past the row's width re-read its last slot (a
+        // cached line) and contribute 0, so every load is issued before the
+        // first use instead of one guarded load at a time.
         float v[kFastW], xv[kFastW];
         int c[kFastW];
 #pragma unroll
         for (int k = 0; k < kFastW; ++k) {
-            if (k < w) {
-                v[k] = A.val[e0 + static_cast<size_t>(k) * kC];
-                c[k] = k == 0 ? row
-                              : (IDX16 ? base + static_cast<int>(A.col16[e0 + static_cast<size_t>(k) * kC])
-                                       : A.col32[e0 + static_cast<size_t>(k) * kC]);
-            }
+            const size_t e = e0 + static_cast<size_t>(k < w ? k : w - 1) * kC;
+            v[k] = k < w ? A.val[e] : 0.0f;
+            c[k] = k == 0 ? row : (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]);
         }
 #pragma unroll
-        for (int k = 0; k < kFastW; ++k)
-            if (k < w) xv[k] = x[c[k]];
+        for (int k = 0; k < kFastW; ++k) xv[k] = x[c[k]];
         xdiag = xv[0];
         float acc = v[0] * xv[0];
 #pragma unroll
-        for (int k = 1; k < kFastW; ++k)
-            if (k < w) acc = fmaf(v[k], xv[k], acc);
+        for (int k = 1; k < kFastW; ++k) acc = fmaf(v[k], xv[k], acc);
         return acc;
     }
     size_t e = e0;
@@ -219,24 +217,26 @@ __device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int
         if (VMODE == 0) return A.val64[e];
         return static_cast<double>(A.val[e]) + static_cast<double>(A.val_lo[e]);
     };
-    if (w <= kFastW) {  // indices and gathers issued first; values stream into the sum
+    if (w <= kFastW) {
+        // Branch-free, as in sell_row_f32: slots past the row's width re-read
+        // its last slot and contribute 0.  Indices and gathers first, the
+        // value pairs stream into the FP64 sum.
         float xv[kFastW];
         int c[kFastW];
 #pragma unroll
         for (int k = 0; k < kFastW; ++k) {
-            if (k < w)
-                c[k] = k == 0 ? row
-                              : (IDX16 ? base + static_cast<int>(A.col16[e0 + static_cast<size_t>(k) * kC])
-                                       : A.col32[e0 + static_cast<size_t>(k) * kC]);
+            const size_t e = e0 + static_cast<size_t>(k < w ? k : w - 1) * kC;
+            c[k] = k == 0 ? row : (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]);
         }
 #pragma unroll
-        for (int k = 0; k < kFastW; ++k)
-            if (k < w) xv[k] = x[c[k]];
+        for (int k = 0; k < kFastW; ++k) xv[k] = x[c[k]];
         xdiag = xv[0];
         double acc = value(e0) * static_cast<double>(xv[0]);
 #pragma unroll
-        for (int k = 1; k < kFastW; ++k)
-            if (k < w) acc = fma(value(e0 + static_cast<size_t>(k) * kC), static_cast<double>(xv[k]), acc);
+        for (int k = 1; k < kFastW; ++k) {
+            const double v = value(e0 + static_cast<size_t>(k < w ? k : w - 1) * kC);
+            acc = fma(k < w ? v : 0.0, static_cast<double>(xv[k]), acc);
+        }
         return acc;
     }
     size_t e = e0;
@@ -253,25 +253,99 @@ __device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int
 
 // z = M r for the 8x8 block of this lane's row: the 8 rows of a block are 8
 // consecutive lanes (blocks start at multiples of 8 in solve order), so the
-// block's r values are exchanged with cross-lane shuffles.  M is stored per
-// block as 8 rows of 8 floats (row-major), i.e. row `row` at M + 8 * row.
+// block's values are exchanged with cross-lane shuffles.  With
+// M = S C C^T S (BlockPre): u = S r, w = C^T u, z = S C w.  Every lane loads
+// the whole 80-byte factor of its block (the 8 lanes of a block read the
+// same lines) and computes all of w and of C w in FP32 from its unpacked
+// values, keeping its own row: register indices stay compile-time.
 // Every lane of the wavefront must call it.
-__device__ __forceinline__ float block_apply(const float* __restrict__ M, int row, float r) {
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    const float4* mr = reinterpret_cast<const float4*>(M + static_cast<size_t>(row) * 8);
-    const float4 m0 = mr[0], m1 = mr[1];
-    const float mv[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const float rj = __shfl(r, (lane & ~7) | j, 64);
-        acc += static_cast<double>(mv[j]) * static_cast<double>(rj);
-    }
-    return static_cast<float>(acc);
+__device__ __forceinline__ float half_at(const uint4 (&q)[5], int t) {  // t compile-time after unrolling
+    const unsigned word = (t >> 1) % 4 == 0 ? q[t >> 3].x : (t >> 1) % 4 == 1 ? q[t >> 3].y
+                        : (t >> 1) % 4 == 2 ? q[t >> 3].z : q[t >> 3].w;
+    const unsigned short h = static_cast<unsigned short>((t & 1) ? (word >> 16) : (word & 0xFFFFu));
+    return static_cast<float>(__builtin_bit_cast(_Float16, h));
 }
 
-#define XP_WAVE_LANE                                                  \
-    const int wave = static_cast<int>(threadIdx.x) >> 6;              \
+__device__ __forceinline__ float block_apply(const BlockPre* __restrict__ M, int row, float r) {
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int j = row & 7;
+    const BlockPre* b = M + (row >> 3);
+    const uint4* cp = reinterpret_cast<const uint4*>(b->c);
+    uint4 q[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[k] = cp[k];
+    const float sj = b->s[j];
+    const float u = sj * r;
+    float uu[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) uu[i] = __shfl(u, (lane & ~7) | i, 64);
+    float w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int i = k; i < 8; ++i) acc = fmaf(half_at(q, i * (i + 1) / 2 + k), uu[i], acc);
+        w[k] = acc;
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k <= i; ++k) acc = fmaf(half_at(q, i * (i + 1) / 2 + k), w[k], acc);
+        v = i == j ? acc : v;
+    }
+    return sj * v;
+}
+
+// Factors full FP32 blocks (solve order, block q = 64 floats row-major) into
+// BlockPre: s_i = sqrt(M_ii), C = chol(S^-1 M S^-1) in FP64, rounded to FP16.
+// A block that is not positive definite keeps its diagonal (C = I).  One
+// thread per block.
+__global__ void k_block_factor(int nblocks, const float* __restrict__ full, BlockPre* __restrict__ out) {
+    const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (q >= nblocks) return;
+    const float* m = full + static_cast<size_t>(q) * 64;
+    double s[8], a[8][8], l[8][8] = {};
+    bool ok = true;
+    for (int i = 0; i < 8; ++i) {
+        const double d = static_cast<double>(m[i * 8 + i]);
+        ok = ok && d > 0.0;
+        s[i] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    for (int i = 0; i < 8; ++i)
+        for (int k = 0; k < 8; ++k) a[i][k] = static_cast<double>(m[i * 8 + k]) / (s[i] * s[k]);
+    for (int k = 0; k < 8 && ok; ++k) {
+        double d = a[k][k];
+        for (int t = 0; t < k; ++t) d -= l[k][t] * l[k][t];
+        if (!(d > 0.0)) {
+            ok = false;
+            break;
+        }
+        l[k][k] = sqrt(d);
+        for (int i = k + 1; i < 8; ++i) {
+            double v = a[i][k];
+            for (int t = 0; t < k; ++t) v -= l[i][t] * l[k][t];
+            l[i][k] = v / l[k][k];
+        }
+    }
+    BlockPre o;
+    int t = 0;
+    for (int i = 0; i < 8; ++i)
+        for (int k = 0; k <= i; ++k, ++t) {
+            const float v = ok ? static_cast<float>(l[i][k]) : (i == k ? 1.0f : 0.0f);
+            o.c[t] = __builtin_bit_cast(unsigned short, static_cast<_Float16>(v));
+        }
+    for (; t < 40; ++t) o.c[t] = 0;
+    for (int i = 0; i < 8; ++i) o.s[i] = static_cast<float>(s[i]);
+    out[q] = o;
+}
+
+// The wavefront index is made visibly uniform (a scalar): loop bounds and
+// slice metadata then live in scalar registers and branches on them are
+// uniform.
+#define XP_WAVE_LANE                                                                    \
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6); \
     const int lane = static_cast<int>(threadIdx.x) & 63;
 
 // ---------------------------------------------------------------------------
@@ -282,7 +356,7 @@ template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_init(Launch L, SellView A, const float* __restrict__ x,
                                                      const float* __restrict__ b,
                                                      const float* __restrict__ minv,
-                                                     const float* __restrict__ blockM,
+                                                     const BlockPre* __restrict__ blockM,
                                                      float* __restrict__ r, float* __restrict__ z,
                                                      float* __restrict__ p) {
     const int lb = logical_block();
@@ -372,13 +446,16 @@ constexpr int kRowsPerThread = 4;
 // rz_new, rr_new.  update_xrz_kernel (archived/kernels/cg_kernels.cpp:118-147).
 // A breakdown (bad pAp or alpha) leaves the vectors untouched; the next
 // kernel records it.
+template <bool BLOCK>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const SysState* S, float* __restrict__ x,
                                                            const float* __restrict__ p,
                                                            float* __restrict__ r,
                                                            const float* __restrict__ ap,
                                                            const float* __restrict__ minv,
-                                                           const float* __restrict__ blockM,
+                                                           const BlockPre* __restrict__ blockM,
                                                            float* __restrict__ z) {
+    // the block apply holds a block factor per row in flight: one row per pass
+    constexpr int ROWS = BLOCK ? 1 : kRowsPerThread;
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
@@ -390,17 +467,17 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
     double v[2] = {0.0, 0.0};
     bool first = true;
     double alpha = 0.0;
-    for (int base = r0; base < r1; base += kRowsPerThread * kBlock) {
-        float xv[kRowsPerThread], pv[kRowsPerThread], rv[kRowsPerThread], av[kRowsPerThread], mv[kRowsPerThread];
+    for (int base = r0; base < r1; base += ROWS * kBlock) {
+        float xv[ROWS], pv[ROWS], rv[ROWS], av[ROWS], mv[ROWS];
 #pragma unroll
-        for (int k = 0; k < kRowsPerThread; ++k) {
+        for (int k = 0; k < ROWS; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
             if (row < r1) {
                 xv[k] = x[row];
                 pv[k] = p[row];
                 rv[k] = r[row];
                 av[k] = ap[row];
-                mv[k] = blockM == nullptr ? minv[row] : 0.0f;
+                mv[k] = BLOCK ? 0.0f : minv[row];
             }
         }
         if (first) {
@@ -412,12 +489,12 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
             if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;
         }
 #pragma unroll
-        for (int k = 0; k < kRowsPerThread; ++k) {
+        for (int k = 0; k < ROWS; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
             if (row < r1) {  // uniform per wavefront: r1 - base is a multiple of 64
                 const float xn = static_cast<float>(static_cast<double>(xv[k]) + alpha * static_cast<double>(pv[k]));
                 const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * static_cast<double>(av[k]));
-                const float zn = blockM != nullptr
+                const float zn = BLOCK
                                      ? block_apply(blockM, row, rn)
                                      : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
                 x[row] = xn;
@@ -620,6 +697,10 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
 
 // First kernel: scalar tail of the previous iteration, p = z + beta p,
 // ap = A z + beta ap, pAp partials.  Right after init (fresh): p = z, ap = A z.
+// A z does not depend on beta, so every wavefront computes its first slice's
+// A z (and loads its p, ap) before waiting on the cross-workgroup partial
+// sums and the scalar step: with ~2-3 slices per wavefront that wait would
+// otherwise be exposed.
 template <bool IDX16, int VMODE>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, SellView A, const SysState* Sc,
                                                                   SysState* Sn, TraceRow* trace, int trace_stride,
@@ -631,6 +712,19 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
     if (w.sys < 0) return;
     const bool leader_block = lb == sys_range(L, w.sys).x;
     if (threadIdx.x == 0) s_sh = Sc[w.sys];
+    XP_WAVE_LANE
+    // the first slice, ahead of the scalar step (right after init p and ap
+    // hold stale values; they are loaded but not used then)
+    int sl = w.slice_begin + wave;
+    const bool have0 = sl < w.slice_end;
+    double po0 = 0.0, apo0 = 0.0, az0 = 0.0;
+    float zd0 = 0.0f;
+    if (have0) {
+        const int row = sl * kC + lane;
+        po0 = p[row];
+        apo0 = ap[row];
+        az0 = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd0);
+    }
     double t[3];
     sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);  // ends with a barrier
     if (!s_sh.active) {
@@ -654,9 +748,16 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
     __syncthreads();
     if (!s_sh.active) return;
     const double beta = s_sh.beta;
-    XP_WAVE_LANE
     double v[1] = {0.0};
-    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+    if (have0) {
+        const int row = sl * kC + lane;
+        const double pn = fresh ? static_cast<double>(zd0) : static_cast<double>(zd0) + beta * po0;
+        const double an = fresh ? az0 : az0 + beta * apo0;
+        p[row] = pn;
+        ap[row] = an;
+        v[0] += pn * an;
+    }
+    for (sl += kWaves; sl < w.slice_end; sl += kWaves) {
         const int row = sl * kC + lane;
         // the FP64 p and A p streams are requested before the row's gathers
         const double po = fresh ? 0.0 : p[row];
@@ -683,14 +784,16 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
 // partials of rz_new, rr_new.  Rows in flight per thread: the FP64 streams
 // double the registers per row.
 constexpr int kRowsAcc = 2;
+template <bool BLOCK>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, const SysState* S,
                                                                double* __restrict__ x,
                                                                const double* __restrict__ p,
                                                                float* __restrict__ r,
                                                                const double* __restrict__ ap,
                                                                const float* __restrict__ minv,
-                                                               const float* __restrict__ blockM,
+                                                               const BlockPre* __restrict__ blockM,
                                                                float* __restrict__ z) {
+    constexpr int ROWS = BLOCK ? 1 : kRowsAcc;  // the block apply holds a block factor in flight
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
@@ -700,18 +803,18 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
     double v[2] = {0.0, 0.0};
     bool first = true;
     double alpha = 0.0;
-    for (int base = r0; base < r1; base += kRowsAcc * kBlock) {
-        double xv[kRowsAcc], pv[kRowsAcc], av[kRowsAcc];
-        float rv[kRowsAcc], mv[kRowsAcc];
+    for (int base = r0; base < r1; base += ROWS * kBlock) {
+        double xv[ROWS], pv[ROWS], av[ROWS];
+        float rv[ROWS], mv[ROWS];
 #pragma unroll
-        for (int k = 0; k < kRowsAcc; ++k) {
+        for (int k = 0; k < ROWS; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
             if (row < r1) {
                 xv[k] = x[row];
                 pv[k] = p[row];
                 rv[k] = r[row];
                 av[k] = ap[row];
-                mv[k] = blockM == nullptr ? minv[row] : 0.0f;
+                mv[k] = BLOCK ? 0.0f : minv[row];
             }
         }
         if (first) {
@@ -723,11 +826,11 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
             if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;
         }
 #pragma unroll
-        for (int k = 0; k < kRowsAcc; ++k) {
+        for (int k = 0; k < ROWS; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
             if (row < r1) {  // uniform per wavefront: r1 - base is a multiple of 64
                 const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * av[k]);
-                const float zn = blockM != nullptr
+                const float zn = BLOCK
                                      ? block_apply(blockM, row, rn)
                                      : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
                 x[row] = xv[k] + alpha * pv[k];
@@ -803,7 +906,7 @@ template <bool IDX16, int VMODE>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_replace(Launch L, SellView A, const double* __restrict__ x,
                                                         const double* __restrict__ b,
                                                         const float* __restrict__ minv,
-                                                        const float* __restrict__ blockM, float* __restrict__ r,
+                                                        const BlockPre* __restrict__ blockM, float* __restrict__ r,
                                                         float* __restrict__ z) {
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
@@ -1093,7 +1196,7 @@ __global__ void k_mask_padding_blocks(int n_pad, const int* __restrict__ perm, f
     } while (0)
 
 void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
-                     const float* b, const float* minv, const float* blockM, float* r, float* z, float* p) {
+                     const float* b, const float* minv, const BlockPre* blockM, float* r, float* z, float* p) {
     XP_DISPATCH_SELL(A, k_pcg_init, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, blockM, r, z, p);
     hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
 }
@@ -1108,8 +1211,11 @@ void launch_pcg_spmv_fused(const Launch& L, const SellView& A, const SysState* s
 }
 
 void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
-                           const float* ap, const float* minv, const float* blockM, float* z) {
-    XP_LAUNCH(L, k_pcg_update_xrz, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
+                           const float* ap, const float* minv, const BlockPre* blockM, float* z) {
+    if (blockM != nullptr)
+        XP_LAUNCH(L, k_pcg_update_xrz<true>, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
+    else
+        XP_LAUNCH(L, k_pcg_update_xrz<false>, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
 }
 
 void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
@@ -1219,7 +1325,7 @@ void launch_block_features(const SellView& A, double* dis, float* feat, float* s
 }
 
 void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const double* x, const double* b,
-                        const float* minv, const float* blockM, float* r, float* z, double* out) {
+                        const float* minv, const BlockPre* blockM, float* r, float* z, double* out) {
     const dim3 g(L.grid), blk(kBlock);
     if (A.idx16) {
         if (use_lo)
@@ -1251,7 +1357,7 @@ void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const d
     } while (0)
 
 void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
-                         const double* b, const float* minv, const float* blockM, float* r, float* z) {
+                         const double* b, const float* minv, const BlockPre* blockM, float* r, float* z) {
     XP_DISPATCH_ACC(A, use_lo, k_pcg_replace, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, blockM, r, z);
     hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
 }
@@ -1269,13 +1375,22 @@ void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, bool use_lo, 
 }
 
 void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,
-                               const double* ap, const float* minv, const float* blockM, float* z) {
-    XP_LAUNCH(L, k_pcg_update_xrz_acc, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
+                               const double* ap, const float* minv, const BlockPre* blockM, float* z) {
+    if (blockM != nullptr)
+        XP_LAUNCH(L, k_pcg_update_xrz_acc<true>, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
+    else
+        XP_LAUNCH(L, k_pcg_update_xrz_acc<false>, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
 }
 
 void launch_fold_f32_into_f64(int n_pad, float* d, double* x, hipStream_t s) {
     hipLaunchKernelGGL(k_fold_f32_into_f64, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad,
                        d, x);
+}
+
+void launch_block_factor(int nblocks, const float* full, BlockPre* out, hipStream_t s) {
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_block_factor, dim3(blocks_for(static_cast<size_t>(nblocks), 256)), dim3(256), 0, s, nblocks,
+                       full, out);
 }
 
 void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s) {

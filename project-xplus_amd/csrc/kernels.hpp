@@ -30,9 +30,9 @@ struct Launch {
 // ---- FP32 solve-order PCG step kernels (state machine in SysState) ----
 // init: r = b - A x; z = minv r; p = z; then the state after init in st[sys]
 // (which must hold the configuration: tau, max_iters, trace_cap, stall_limit).
-// blockM: null for Jacobi (z = minv r), else 8x8 blocks (z = M_b r_b).
+// blockM: null for Jacobi (z = minv r), else factored 8x8 blocks (z = M_b r_b).
 void launch_pcg_init(const Launch& L, const SellView& A, SysState* st, const float* x,
-                     const float* b, const float* minv, const float* blockM, float* r, float* z, float* p);
+                     const float* b, const float* minv, const BlockPre* blockM, float* r, float* z, float* p);
 // ap = A p and the pAp partials.
 void launch_pcg_spmv(const Launch& L, const SellView& A, const SysState* st, const float* p, float* ap);
 // Fused iteration, first kernel: scalar tail of the previous iteration,
@@ -41,7 +41,7 @@ void launch_pcg_spmv_fused(const Launch& L, const SellView& A, const SysState* s
                            TraceRow* trace, int trace_stride, const float* z, float* p, float* ap);
 // alpha; x += alpha p; r -= alpha ap; z = minv r; rz, rr partials.
 void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const float* p, float* r,
-                           const float* ap, const float* minv, const float* blockM, float* z);
+                           const float* ap, const float* minv, const BlockPre* blockM, float* z);
 // scalar tail (beta, tests, trace) and p = z + beta p; reads st_cur, writes st_next.
 void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
                          int trace_stride, const float* z, float* p);
@@ -51,12 +51,12 @@ void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_n
 // init: r = fp32(b - A x) (FP64), z = M r, rz / rr partials, then the state
 // after init; the first fused iteration sets p = z, ap = A z (state.fresh).
 void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
-                         const double* b, const float* minv, const float* blockM, float* r, float* z);
+                         const double* b, const float* minv, const BlockPre* blockM, float* r, float* z);
 void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, bool use_lo, const SysState* st_cur,
                                SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
                                double* ap);
 void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,
-                               const double* ap, const float* minv, const float* blockM, float* z);
+                               const double* ap, const float* minv, const BlockPre* blockM, float* z);
 
 // ---- plain SpMV (no reduction) ----
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y);
@@ -70,7 +70,7 @@ void launch_residual_f64(const Launch& L, const SellView& A, bool use_lo, const 
 // rr partials of the next scalar step rewritten from them;
 // out[3 sys + 0..2] = (rz, rr, FP64 true r^T r) of each system.
 void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const double* x, const double* b,
-                        const float* minv, const float* blockM, float* r, float* z, double* out);
+                        const float* minv, const BlockPre* blockM, float* r, float* z, double* out);
 // x64 += (double) d; d = 0.
 void launch_fold_f32_into_f64(int n_pad, float* d, double* x, hipStream_t s);
 // x64 += (double) d32 over the solve-order rows.
@@ -108,6 +108,8 @@ void launch_fill_u32(size_t n, unsigned v, unsigned* dst, hipStream_t s);
 // feat [n_pad/8 x 48] (upper S_bb, off-block |s| sums, 4 zeros), scale
 // [n_pad] = D^-1/2 per solve row; dis [n_pad] is FP64 scratch.
 void launch_block_features(const SellView& A, double* dis, float* feat, float* scale, hipStream_t s);
+// Full FP32 solve-order blocks (64 floats each) -> factored BlockPre.
+void launch_block_factor(int nblocks, const float* full, BlockPre* out, hipStream_t s);
 // Identity rows/columns for the padding rows (perm < 0) of solve-order 8x8 blocks.
 void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s);
 

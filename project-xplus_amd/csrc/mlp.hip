@@ -92,10 +92,59 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(const uint16_t* __restrict__ X
     }
 }
 
+// The exact block inverse as a lower-triangular factor: Ahat (from the 36
+// upper entries of a feature row) = J R R^T J with R = chol(J Ahat J), J the
+// exchange matrix, so Ahat^-1 = L0 L0^T with L0 = J R^-T J lower triangular
+// (L0[i][j] = (R^-1)[7-j][7-i]).  FP64; an Ahat that is not positive
+// definite gives L0 = I.  l0: the 36 entries (r, c <= r), r-major.
+__host__ __device__ inline void block_l0(const float* __restrict__ f, float* __restrict__ l0) {
+    double a[8][8];
+    int t = 0;
+    for (int i = 0; i < 8; ++i)
+        for (int j = i; j < 8; ++j) {
+            a[i][j] = static_cast<double>(f[t]);
+            a[j][i] = static_cast<double>(f[t]);
+            ++t;
+        }
+    double r[8][8] = {};
+    bool ok = true;
+    for (int j = 0; j < 8 && ok; ++j) {  // Cholesky of B = J A J: B[p][q] = A[7-p][7-q]
+        double d = a[7 - j][7 - j];
+        for (int k = 0; k < j; ++k) d -= r[j][k] * r[j][k];
+        if (!(d > 0.0)) {
+            ok = false;
+            break;
+        }
+        r[j][j] = sqrt(d);
+        for (int i = j + 1; i < 8; ++i) {
+            double v = a[7 - i][7 - j];
+            for (int k = 0; k < j; ++k) v -= r[i][k] * r[j][k];
+            r[i][j] = v / r[j][j];
+        }
+    }
+    double ri[8][8] = {};  // R^-1 (lower), column by column
+    if (ok)
+        for (int c = 0; c < 8; ++c)
+            for (int i = c; i < 8; ++i) {
+                double v = i == c ? 1.0 : 0.0;
+                for (int k = c; k < i; ++k) v -= r[i][k] * ri[k][c];
+                ri[i][c] = v / r[i][i];
+            }
+    t = 0;
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j <= i; ++j) l0[t++] = ok ? static_cast<float>(ri[7 - j][7 - i]) : (i == j ? 1.0f : 0.0f);
+}
+
+__global__ void k_block_l0(const float* __restrict__ feat, int count, float* __restrict__ l0) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i < count) block_l0(feat + static_cast<size_t>(i) * 48, l0 + static_cast<size_t>(i) * 36);
+}
+
 // Loss and dloss/dout per sample (one thread per sample).  out: [rows x 48]
 // network outputs; ahat, hmat: [rows x 64]; dout: [rows x 48] (BF16 copy in
 // dout_b for the MFMA backward).  loss_part[block] = sum of this block's losses.
-__global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ out, const float* __restrict__ ahat,
+__global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ out, const float* __restrict__ l0,
+                                                  const float* __restrict__ ahat,
                                                   const float* __restrict__ hmat, int count, float inv_count,
                                                   float* __restrict__ dout, uint16_t* __restrict__ dout_b,
                                                   double* __restrict__ loss_part) {
@@ -104,12 +153,16 @@ __global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ out,
     double loss = 0.0;
     if (i < count) {
         const float* o = out + static_cast<size_t>(i) * 48;
+        const float* o0 = l0 + static_cast<size_t>(i) * 36;
         const float* A = ahat + static_cast<size_t>(i) * 64;
         const float* H = hmat + static_cast<size_t>(i) * 64;
         float L[8][8];
         int t = 0;
         for (int r = 0; r < 8; ++r)
-            for (int c = 0; c < 8; ++c) L[r][c] = c <= r ? o[t++] : 0.0f;
+            for (int c = 0; c < 8; ++c) {
+                L[r][c] = c <= r ? o0[t] + o[t] : 0.0f;
+                if (c <= r) ++t;
+            }
         float M[8][8];
         for (int r = 0; r < 8; ++r)
             for (int c = 0; c < 8; ++c) {
@@ -226,15 +279,19 @@ __global__ void k_cast_rows(const float* __restrict__ src, size_t n, uint16_t* _
 
 // M_b = D^-1/2 (L L^T + eps I) D^-1/2 for every sample: blocks [count x 64];
 // dis: [count x 8] diagonal of D^-1/2.
-__global__ void k_mlp_blocks(const float* __restrict__ out, const float* __restrict__ dis, int count,
-                             float* __restrict__ blocks) {
+__global__ void k_mlp_blocks(const float* __restrict__ out, const float* __restrict__ l0,
+                             const float* __restrict__ dis, int count, float* __restrict__ blocks) {
     const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (i >= count) return;
     const float* o = out + static_cast<size_t>(i) * 48;
+    const float* o0 = l0 + static_cast<size_t>(i) * 36;
     float L[8][8];
     int t = 0;
     for (int r = 0; r < 8; ++r)
-        for (int c = 0; c < 8; ++c) L[r][c] = c <= r ? o[t++] : 0.0f;
+        for (int c = 0; c < 8; ++c) {
+            L[r][c] = c <= r ? o0[t] + o[t] : 0.0f;
+            if (c <= r) ++t;
+        }
     const float* di = dis + static_cast<size_t>(i) * 8;
     float* mb = blocks + static_cast<size_t>(i) * 64;
     for (int r = 0; r < 8; ++r)
@@ -283,7 +340,9 @@ BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const
     return from_csr(n, row_ptr, col_idx, values, natural_blocking(n));
 }
 
-BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, const Blocking& blk) {
+BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, const Blocking& blk,
+                            double lambda) {
+    if (!(lambda >= 0.0)) throw std::runtime_error("the off-block weight must be >= 0");
     if (blk.n != n) throw std::runtime_error("blocking does not match the matrix");
     BlockSet s;
     const int nb = blk.units();
@@ -344,7 +403,7 @@ BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const
         for (int i = 0; i < 8; ++i) f[t++] = static_cast<float>(off_abs[i]);
         for (int i = 0; i < 8; ++i)
             for (int j = 0; j < 8; ++j) ah[i * 8 + j] = static_cast<float>(a[i][j]);
-        // H = Ahat^2 + kLambda G, G_ij = sum over shared off-block columns
+        // H = Ahat^2 + lambda G, G_ij = sum over shared off-block columns
         for (int i = 0; i < 8; ++i)
             for (int j = 0; j < 8; ++j) {
                 double v = 0.0;
@@ -353,7 +412,7 @@ BlockSet BlockSet::from_csr(int n, const int* row_ptr, const int* col_idx, const
                 for (const auto& e1 : off[i])
                     for (const auto& e2 : off[j])
                         if (e1.first == e2.first) g += e1.second * e2.second;
-                hm[i * 8 + j] = static_cast<float>(v + kMlpLambda * g);
+                hm[i * 8 + j] = static_cast<float>(v + lambda * g);
             }
     }
     return s;
@@ -373,7 +432,8 @@ struct Mlp::Dev {
     Buf<float> xf, h1, h2, out, dout, dh2, dh1, ahat, hmat, scale, part, blocks, dummy;
     Buf<uint16_t> xb, h1b, h2b, doutb, dh2b, dh1b;
     Buf<double> loss_part, hist;
-    Buf<float> set_feat, set_ahat, set_hmat;   // device-resident training set (+64 zero rows)
+    Buf<float> set_feat, set_ahat, set_hmat, set_l0;   // device-resident training set (+64 zero rows)
+    Buf<float> l0;                    // exact-inverse factors of the staged rows (rows x 36)
     hipStream_t s = nullptr;
 };
 
@@ -405,17 +465,14 @@ Mlp::Mlp(uint64_t seed) : d_(new Dev) {
     };
     for (int i = 0; i < kW1; ++i) params_[static_cast<size_t>(kOffW1 + i)] = static_cast<float>(normal() * std::sqrt(2.0 / 44.0));
     for (int i = 0; i < kW2; ++i) params_[static_cast<size_t>(kOffW2 + i)] = static_cast<float>(normal() * std::sqrt(2.0 / kMlpHidden));
-    for (int i = 0; i < kW3; ++i) params_[static_cast<size_t>(kOffW3 + i)] = static_cast<float>(normal() * std::sqrt(1.0 / kMlpHidden) * 0.1);
+    for (int i = 0; i < kW3; ++i) params_[static_cast<size_t>(kOffW3 + i)] = static_cast<float>(normal() * std::sqrt(1.0 / kMlpHidden) * 0.01);
     // zero the unused input rows (features 44..47) and output columns 36..47
     for (int k = 44; k < kMlpIn; ++k)
         for (int n = 0; n < kMlpHidden; ++n) params_[static_cast<size_t>(kOffW1 + k * kMlpHidden + n)] = 0.0f;
     for (int k = 0; k < kMlpHidden; ++k)
         for (int n = 36; n < kMlpOut; ++n) params_[static_cast<size_t>(kOffW3 + k * kMlpOut + n)] = 0.0f;
-    // bias of the diagonal of L: start at the identity (Mh = I: plain Jacobi)
-    int t = 0;
-    for (int r = 0; r < 8; ++r)
-        for (int c = 0; c <= r; ++c, ++t)
-            if (r == c) params_[static_cast<size_t>(kOffB3 + t)] = 1.0f;
+    // b3 = 0 and a small W3: the network starts as a small correction to the
+    // exact block inverse (L = L0 + out)
     upload_params();
     const size_t np = static_cast<size_t>(num_params());
     d_->m.ensure(np);
@@ -471,6 +528,7 @@ void Mlp::ensure_rows(int rows) {
     d.dh2.ensure(static_cast<size_t>(rows) * kMlpHidden); d.dh2b.ensure(static_cast<size_t>(rows) * kMlpHidden);
     d.dh1.ensure(static_cast<size_t>(rows) * kMlpHidden); d.dh1b.ensure(static_cast<size_t>(rows) * kMlpHidden);
     d.loss_part.ensure(static_cast<size_t>(nblk(static_cast<size_t>(rows), 256)));
+    d.l0.ensure(static_cast<size_t>(rows) * 36);
 }
 
 int Mlp::stage(const BlockSet& data, int first, int count) {
@@ -495,6 +553,7 @@ int Mlp::stage(const BlockSet& data, int first, int count) {
                          static_cast<size_t>(count) * 8 * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
     hipLaunchKernelGGL(k_cast_rows, dim3(nblk(static_cast<size_t>(rows) * kMlpIn, 256)), dim3(256), 0, d.s, d.xf.p,
                        static_cast<size_t>(rows) * kMlpIn, d.xb.p);
+    hipLaunchKernelGGL(k_block_l0, dim3(nblk(static_cast<size_t>(rows), 256)), dim3(256), 0, d.s, d.xf.p, rows, d.l0.p);
     return rows;
 }
 
@@ -509,10 +568,11 @@ void Mlp::forward_staged(int rows) {
                        d.out.p, nullptr, rows);
 }
 
-void Mlp::backward_staged(int rows, int count, const float* xf, const float* ahat, const float* hmat) {
+void Mlp::backward_staged(int rows, int count, const float* xf, const float* l0, const float* ahat,
+                          const float* hmat) {
     Dev& d = *d_;
     const int lb = nblk(static_cast<size_t>(count), 256);
-    hipLaunchKernelGGL(k_mlp_loss, dim3(lb), dim3(256), 0, d.s, d.out.p, ahat, hmat, count,
+    hipLaunchKernelGGL(k_mlp_loss, dim3(lb), dim3(256), 0, d.s, d.out.p, l0, ahat, hmat, count,
                        1.0f / static_cast<float>(count), d.dout.p, d.doutb.p, d.loss_part.p);
     // zero the gradient of the padding rows (loss kernel writes rows < count only)
     if (rows > count) {
@@ -541,7 +601,7 @@ void Mlp::backward_staged(int rows, int count, const float* xf, const float* aha
 
 double Mlp::loss_grad_staged(int rows, int count) {
     Dev& d = *d_;
-    backward_staged(rows, count, d.xf.p, d.ahat.p, d.hmat.p);
+    backward_staged(rows, count, d.xf.p, d.l0.p, d.ahat.p, d.hmat.p);
     const int lb = nblk(static_cast<size_t>(count), 256);
     std::vector<double> lp(static_cast<size_t>(lb));
     check(hipMemcpyAsync(lp.data(), d.loss_part.p, lp.size() * sizeof(double), hipMemcpyDeviceToHost, d.s), "download");
@@ -580,7 +640,8 @@ std::vector<float> Mlp::blocks(const BlockSet& data) {
         const int rows = stage(data, first, count);
         forward_staged(rows);
         d.blocks.ensure(static_cast<size_t>(rows) * 64);
-        hipLaunchKernelGGL(k_mlp_blocks, dim3(nblk(static_cast<size_t>(count), 256)), dim3(256), 0, d.s, d.out.p, d.scale.p,
+        hipLaunchKernelGGL(k_mlp_blocks, dim3(nblk(static_cast<size_t>(count), 256)), dim3(256), 0, d.s, d.out.p, d.l0.p,
+                           d.scale.p,
                            count, d.blocks.p);
         check(hipMemcpyAsync(out.data() + static_cast<size_t>(first) * 64, d.blocks.p,
                              static_cast<size_t>(count) * 64 * sizeof(float), hipMemcpyDeviceToHost, d.s), "download");
@@ -596,9 +657,10 @@ void Mlp::blocks_device(const float* feat, const float* scale, int count, float*
     ensure_rows(rows);
     hipLaunchKernelGGL(k_cast_rows, dim3(nblk(static_cast<size_t>(rows) * kMlpIn, 256)), dim3(256), 0, d.s, feat,
                        static_cast<size_t>(rows) * kMlpIn, d.xb.p);
+    hipLaunchKernelGGL(k_block_l0, dim3(nblk(static_cast<size_t>(count), 256)), dim3(256), 0, d.s, feat, count, d.l0.p);
     forward_staged(rows);
-    hipLaunchKernelGGL(k_mlp_blocks, dim3(nblk(static_cast<size_t>(count), 256)), dim3(256), 0, d.s, d.out.p, scale,
-                       count, blocks);
+    hipLaunchKernelGGL(k_mlp_blocks, dim3(nblk(static_cast<size_t>(count), 256)), dim3(256), 0, d.s, d.out.p, d.l0.p,
+                       scale, count, blocks);
     check(hipStreamSynchronize(d.s), "sync");
 }
 
@@ -614,6 +676,7 @@ std::vector<double> Mlp::train(const BlockSet& data, int steps, int batch, float
     d.set_feat.ensure(pad * kMlpIn);
     d.set_ahat.ensure(pad * 64);
     d.set_hmat.ensure(pad * 64);
+    d.set_l0.ensure(pad * 36);
     check(hipMemsetAsync(d.set_feat.p + cnt * kMlpIn, 0, 64 * kMlpIn * sizeof(float), d.s), "memset");
     check(hipMemsetAsync(d.set_ahat.p + cnt * 64, 0, 64 * 64 * sizeof(float), d.s), "memset");
     check(hipMemsetAsync(d.set_hmat.p + cnt * 64, 0, 64 * 64 * sizeof(float), d.s), "memset");
@@ -621,6 +684,7 @@ std::vector<double> Mlp::train(const BlockSet& data, int steps, int batch, float
           "upload");
     check(hipMemcpyAsync(d.set_ahat.p, data.ahat.data(), cnt * 64 * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
     check(hipMemcpyAsync(d.set_hmat.p, data.hmat.data(), cnt * 64 * sizeof(float), hipMemcpyHostToDevice, d.s), "upload");
+    hipLaunchKernelGGL(k_block_l0, dim3(nblk(pad, 256)), dim3(256), 0, d.s, d.set_feat.p, static_cast<int>(pad), d.set_l0.p);
     d.hist.ensure(static_cast<size_t>(steps));
     const int rows = (batch + 63) / 64 * 64;
     ensure_rows(rows);
@@ -642,7 +706,7 @@ std::vector<double> Mlp::train(const BlockSet& data, int steps, int batch, float
         hipLaunchKernelGGL(k_cast_rows, dim3(nblk(static_cast<size_t>(rows) * kMlpIn, 256)), dim3(256), 0, d.s, xf,
                            static_cast<size_t>(rows) * kMlpIn, d.xb.p);
         forward_staged(rows);
-        backward_staged(rows, batch, xf, d.set_ahat.p + first * 64, d.set_hmat.p + first * 64);
+        backward_staged(rows, batch, xf, d.set_l0.p + first * 36, d.set_ahat.p + first * 64, d.set_hmat.p + first * 64);
         hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, d.s, d.loss_part.p, lb, 1.0 / batch, d.hist.p + (it - 1));
         const float c1 = 1.0f - std::pow(0.9f, static_cast<float>(it));
         const float c2 = 1.0f - std::pow(0.999f, static_cast<float>(it));

@@ -11,7 +11,7 @@ namespace xp {
 
 constexpr int kMlpIn = 48;      // 36 block entries + 8 off-block sums + 4 zero pad
 constexpr int kMlpHidden = 64;
-constexpr int kMlpOut = 48;     // 36 entries of L + 12 unused
+constexpr int kMlpOut = 48;     // 36 entries of the correction to L0 + 12 unused
 constexpr double kMlpLambda = 0.5;  // weight of the off-block term of the training loss
 
 // Features and training targets of the 8-row blocks of a matrix (rows in
@@ -23,7 +23,8 @@ struct BlockSet {
     std::vector<float> hmat;   // count x 64: Ahat^2 + lambda G
     std::vector<float> scale;  // count x 8: D^-1/2 of the block's rows (1 on padding rows)
     static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values);  // natural blocks
-    static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, const Blocking& b);
+    static BlockSet from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, const Blocking& b,
+                             double lambda = kMlpLambda);
     void append(const BlockSet& o);
 };
 
@@ -54,7 +55,7 @@ class Mlp {
     void ensure_rows(int rows);
     int stage(const BlockSet& data, int first, int count);
     void forward_staged(int rows);
-    void backward_staged(int rows, int count, const float* xf, const float* ahat, const float* hmat);
+    void backward_staged(int rows, int count, const float* xf, const float* l0, const float* ahat, const float* hmat);
     double loss_grad_staged(int rows, int count);
 };
 

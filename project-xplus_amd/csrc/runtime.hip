@@ -373,7 +373,7 @@ struct xp_pcg {
     // refinement buffers (allocated on first use)
     DevBuf<double> x64, b64, r64, rr_out, tmp_nat, repl_out;
     DevBuf<float> r32;
-    DevBuf<float> blockM;             // 8x8 blocks in solve order (block preconditioner)
+    DevBuf<xp::BlockPre> blockM;      // factored 8x8 blocks in solve order (block preconditioner)
     DevBuf<double> p64, ap64, x0s64;  // accurate iteration: FP64 p, A p, and x0 in solve order
     bool use_block = false;
     int fallback = 0;
@@ -390,7 +390,7 @@ struct xp_pcg {
         if (stream != nullptr) (void)hipStreamDestroy(stream);
     }
 
-    const float* bm() const { return use_block ? blockM.p : nullptr; }
+    const xp::BlockPre* bm() const { return use_block ? blockM.p : nullptr; }
 
     // Iteration k reads state slot k % 2 and writes slot (k + 1) % 2; the
     // graph holds an even number of iterations, so slot 0 is current
@@ -1838,8 +1838,11 @@ void set_preconditioner(xp_pcg& s, int mode, const float* blocks) {
             throw std::runtime_error("bad preconditioner mode");
         }
         const std::vector<float> sol = arrange_blocks(m, src);
-        if (s.blockM.n != sol.size()) s.blockM.alloc(sol.size());
-        s.blockM.upload(sol.data(), sol.size(), s.stream);
+        DevBuf<float> full(sol.size());
+        full.upload(sol.data(), sol.size(), s.stream);
+        const int nq = m.n_pad / 8;
+        if (s.blockM.n != static_cast<size_t>(nq)) s.blockM.alloc(static_cast<size_t>(nq));
+        xp::launch_block_factor(nq, full.p, s.blockM.p, s.stream);
         HIPX(hipStreamSynchronize(s.stream));
         s.use_block = true;
     }
@@ -1918,14 +1921,15 @@ int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const do
 }
 
 int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, const double* values, int blocking,
-                                 xp_blockset** out) {
+                                 double lambda, xp_blockset** out) {
     return guarded([&] {
         if (out == nullptr || row_ptr == nullptr || n <= 0 || blocking < 0 || blocking > 1)
             throw std::runtime_error("bad argument");
         auto b = std::make_unique<xp_blockset>();
         b->set = xp::BlockSet::from_csr(n, row_ptr, col_idx, values,
                                         blocking == 1 ? xp::strength_blocking(n, row_ptr, col_idx, values)
-                                                      : xp::natural_blocking(n));
+                                                      : xp::natural_blocking(n),
+                                        lambda < 0.0 ? xp::kMlpLambda : lambda);
         *out = b.release();
     });
 }
@@ -2001,10 +2005,11 @@ int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net) {
         // features from the resident SELL matrix, BF16 MFMA inference, blocks
         // written in solve order: nothing crosses PCIe
         DeviceFeatures f = device_block_features(m, s->stream);
-        const size_t nb = static_cast<size_t>(m.n_pad) * 8;
-        if (s->blockM.n != nb) s->blockM.alloc(nb);
-        net->net.blocks_device(f.feat.p, f.scale.p, f.blocks, s->blockM.p);
-        xp::launch_mask_padding_blocks(m.n_pad, m.d_perm.p, s->blockM.p, s->stream);
+        DevBuf<float> full(static_cast<size_t>(m.n_pad) * 8);
+        net->net.blocks_device(f.feat.p, f.scale.p, f.blocks, full.p);
+        xp::launch_mask_padding_blocks(m.n_pad, m.d_perm.p, full.p, s->stream);
+        if (s->blockM.n != static_cast<size_t>(f.blocks)) s->blockM.alloc(static_cast<size_t>(f.blocks));
+        xp::launch_block_factor(f.blocks, full.p, s->blockM.p, s->stream);
         HIPX(hipStreamSynchronize(s->stream));
         s->use_block = true;
         if (s->opts.use_graph) s->build_graph(s->opts.graph_iters);

@@ -79,6 +79,17 @@ struct TraceRow {
     double rr;
 };
 
+// A block preconditioner's block q (solve rows 8q .. 8q+7), factored:
+// M_q = diag(s) C C^T diag(s) with C lower triangular with unit-scale
+// entries (the Cholesky factor of M_q with its diagonal scaled to 1), stored
+// as 36 FP16 values r-major (r, c <= r), padded to 40; 112 bytes per block,
+// 14 per row.
+struct alignas(16) BlockPre {
+    uint16_t c[40];
+    float s[8];
+};
+static_assert(sizeof(BlockPre) == 112, "BlockPre layout");
+
 // Device view of a SELL matrix (all pointers are device pointers).
 struct SellView {
     int n_pad;

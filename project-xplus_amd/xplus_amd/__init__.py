@@ -140,7 +140,7 @@ _mlp_get = _sig("xp_mlp_get_params", C.c_int, [_vp, _f32p])
 _mlp_set = _sig("xp_mlp_set_params", C.c_int, [_vp, _f32p])
 _bs_from = _sig("xp_blockset_from_csr", C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)])
 _bs_from_blocked = _sig("xp_blockset_from_csr_blocked", C.c_int,
-                        [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.POINTER(_vp)])
+                        [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_double, C.POINTER(_vp)])
 _strength_blocking = _sig("xp_strength_blocking", C.c_int,
                           [C.c_int, _i32p, _i32p, _f64p, _i32p, C.POINTER(C.c_int)])
 _m_block_rows = _sig("xp_matrix_block_rows", C.c_int, [_vp, _i32p])
@@ -578,11 +578,12 @@ class Pcg:
 class BlockSet:
     """Features / training targets of the aligned 8-row blocks (host)."""
 
-    def __init__(self, a: Csr = None, blocking: int = 0):
+    def __init__(self, a: Csr = None, blocking: int = 0, lam: float = -1.0):
+        """lam: off-block weight of the training loss (default 0.5)."""
         self.h = None
         if a is not None:
             h = _vp()
-            _check(_bs_from_blocked(a.n, a.row_ptr, a.col_idx, a.values, blocking, C.byref(h)))
+            _check(_bs_from_blocked(a.n, a.row_ptr, a.col_idx, a.values, blocking, lam, C.byref(h)))
             self.h = h
 
     def append(self, other: "BlockSet") -> None:

@@ -53,7 +53,7 @@ def test_oracle_loss_closed_form():
     a, (feat, ahat, hmat, scale) = small_set(4)
     params = random_params(5)
     out = B.forward(params, feat)
-    L = B.l_from_out(out)
+    L = B.factor(out, feat)
     M = L @ np.transpose(L, (0, 2, 1)) + 1e-3 * np.eye(8)
     g = hmat - ahat @ ahat                     # = LAMBDA S_off S_off^T
     direct = [np.linalg.norm(M[k] @ ahat[k] - np.eye(8)) ** 2 + np.trace(M[k] @ g[k] @ M[k].T)
@@ -122,3 +122,33 @@ def test_block_pcg_exact_inverse_on_block_diagonal_matrix():
     x, it, _ = B.block_pcg(dense, np.ones(n), inv, 1e-24)
     assert it == 1
     np.testing.assert_allclose(dense @ x, np.ones(n), rtol=1e-10)
+
+
+def test_oracle_l0_is_the_exact_inverse_factor():
+    """L0 L0^T = Ahat^-1 (closed form), L0 lower with a positive diagonal."""
+    a, (feat, ahat, hmat, scale) = small_set(6)
+    L0 = B.l0_from_feat(feat)
+    for k in range(len(L0)):
+        assert np.allclose(np.triu(L0[k], 1), 0.0)
+        assert np.all(np.diag(L0[k]) > 0)
+        np.testing.assert_allclose(L0[k] @ L0[k].T @ ahat[k], np.eye(8), atol=1e-9)
+    # a block that is not positive definite falls back to the identity
+    bad = feat[:1].copy()
+    bad[0, 0] = -1.0
+    np.testing.assert_array_equal(B.l0_from_feat(bad)[0], np.eye(8))
+
+
+def test_oracle_stored_blocks():
+    """Diagonal blocks are stored exactly; SPD blocks within FP16 rounding of
+    the unit-diagonal factor; the result is symmetric positive definite."""
+    rng = np.random.default_rng(8)
+    D = np.array([np.diag(rng.uniform(0.01, 100.0, 8)) for _ in range(4)])
+    np.testing.assert_allclose(B.stored_blocks(D), D, rtol=1e-15)
+    Q = rng.standard_normal((6, 8, 8))
+    M = Q @ np.transpose(Q, (0, 2, 1)) + 0.5 * np.eye(8)
+    S = B.stored_blocks(M)
+    for k in range(len(M)):
+        np.testing.assert_allclose(S[k], S[k].T, atol=0)
+        assert np.linalg.eigvalsh(S[k]).min() > 0
+        rel = np.abs(S[k] - M[k]) / np.sqrt(np.outer(np.diag(M[k]), np.diag(M[k])))
+        assert rel.max() < 4e-3

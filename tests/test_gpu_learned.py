@@ -70,7 +70,7 @@ def oracle_grad_bf16(params, feat, ahat, hmat):
     h2 = np.maximum(z2, 0)
     out = bf16(h2) @ bf16(p["W3"]) + p["b3"]
     n = out.shape[0]
-    L = B.l_from_out(out)
+    L = B.factor(out, feat)
     M = L @ np.transpose(L, (0, 2, 1)) + B.EPS_M * np.eye(8)
     D = 2.0 * (M @ hmat - ahat) / n
     dL = (D + np.transpose(D, (0, 2, 1))) @ L
@@ -113,6 +113,23 @@ def test_loss_and_gradient_match_oracle(gpu):
         o += k
 
 
+def test_untrained_network_gives_exact_block_inverses(gpu):
+    """With all parameters zero the network's correction vanishes and the
+    blocks are D^-1/2 (Ahat^-1 + 1e-3 I) D^-1/2: the device's reverse
+    Cholesky factor L0 against numpy's inverse."""
+    a = X.gen_fem2d(30, 1.0, 3, 1e3, 0.3, 41)
+    for blocking in (0, 1):
+        bs = X.blockset(a, blocking)
+        net = X.Mlp(seed=1)
+        net.set_params(np.zeros(X.Mlp.num_params(), np.float32))
+        got = net.blocks(bs).astype(np.float64)
+        feat, ahat, _, scale = (v.astype(np.float64) for v in bs.arrays())
+        ref = (np.linalg.inv(B.ahat_from_feat(feat)) + 1e-3 * np.eye(8)) * scale[:, :, None] * scale[:, None, :]
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"blocking {blocking}: max rel err {err:.2e}", flush=True)
+        assert err <= 1e-5
+
+
 def test_training_is_deterministic_and_descends(gpu):
     sets = X.blockset([X.gen_fem2d(66, 1.0, 6, 1e3, 0.3, 20265003 + i) for i in range(2)])
     n1, n2 = X.Mlp(seed=3), X.Mlp(seed=3)
@@ -120,8 +137,10 @@ def test_training_is_deterministic_and_descends(gpu):
     h2 = n2.train(sets, 60, batch=1024, lr=2e-3, seed=9)
     np.testing.assert_array_equal(h1, h2)
     np.testing.assert_array_equal(n1.params(), n2.params())
-    print("loss", h1[0], "->", h1[-1], flush=True)
-    assert np.mean(h1[-10:]) < 0.7 * np.mean(h1[:5])
+    print("loss", h1[0], "->", h1[-1], "ratio", np.mean(h1[-10:]) / np.mean(h1[:5]), flush=True)
+    # the network starts as the exact block inverse, so training refines a
+    # good start: the loss must still fall
+    assert np.mean(h1[-10:]) < 0.95 * np.mean(h1[:5])
 
 
 @pytest.mark.parametrize("which,blocking", [("c1", 0), ("fem", 0), ("c1", 1), ("fem", 1)])
@@ -155,7 +174,8 @@ def test_block_pcg_matches_oracle(gpu, which, blocking):
         else:
             s.run_fixed(K)
         st = s.status()
-        x_ref, it_ref, tr_ref = B.block_pcg(A, a.b, blocks.astype(np.float32).astype(np.float64),
+        # the oracle applies what the device stores: FP16 factors of the FP32 blocks
+        x_ref, it_ref, tr_ref = B.block_pcg(A, a.b, B.stored_blocks(blocks.astype(np.float32)),
                                             tau if K is None else 1e-300, max_iters=K or 100000, groups=groups)
         print(f"{which} blocking {blocking} mode {mode}: gpu iters {st['iterations']} oracle {it_ref}", flush=True)
         assert st["status"] == (X.CONVERGED if K is None else X.MAX_ITER)
