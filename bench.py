@@ -219,6 +219,77 @@ def profile_run(X, a, args) -> int:
     return 0
 
 
+def workload_c3(X, args) -> int:
+    """FP64-refined solves of the FEM family at 1100 x 1100 and 1600 x 1600
+    nodes to true relative residual 1e-10: the restarted FP32 refinement the
+    configuration describes (inner FP32 Jacobi-PCG to 1e-5, FP64 outer
+    residual and solution update) and the accurate iteration with the final
+    FP64 check, each timed with everything resident."""
+    out = []
+    for mesh in (1100, 1600):
+        a = X.gen_fem2d(mesh, 1.0, 20, 1e3, 0.3, 20261203 + 1000 * 3 + mesh)
+        m = X.Matrix(a, sigma=256, keep_lo=True)
+        m.autotune(fused=True, tune_sigma=False, iters=64)
+        r = X.Pcg(m, rel_tol=1e-10, fused=True)
+        r.set_rhs(a.b, a.x0)
+        ref_ms, (rel, outer, inner) = timed(X, lambda: r.run_refined(1e-10, 1e-5, 60), 0, 1)
+        r.close()
+        m.autotune(fused=2, tune_sigma=False, iters=64)
+        s = X.Pcg(m, rel_tol=1e-10, accurate=True)
+        s.set_rhs(a.b, a.x0)
+        acc_ms, (rel2, repl, iters) = timed(X, lambda: s.run_reliable(1e-10, 0.1), 0, 1)
+        s.close()
+        row = {"mesh": mesh, "n": a.n, "nnz": a.nnz,
+               "refined": {"ms": round(ref_ms, 1), "outer": outer, "inner_iterations": inner,
+                           "true_rel_residual": float(rel[0])},
+               "accurate": {"ms": round(acc_ms, 1), "iterations": iters, "replacements": repl,
+                            "true_rel_residual": float(rel2[0])}}
+        log(f"c3: {row}")  # one line per grid: the run prints as it goes
+        out.append(row)
+    print(json.dumps({"workload": "FP64-refined thermal-FEM to true relative residual 1e-10", "unit": "ms",
+                      "results": out}), flush=True)
+    return 0
+
+
+def workload_c4(X, args) -> int:
+    """256 independent 3D 7-point diffusion systems, edges k ~ U{16..64},
+    log-normal coefficients sigma 1.5, solved together in one batched solve
+    (one SELL matrix, per-system state, one graph) to relative residual 1e-6
+    each: the FP32 iteration to the recursive residual and the accurate
+    iteration to the true one."""
+    rng = np.random.default_rng(20264203)
+    ks = rng.integers(16, 65, 256)
+    t0 = time.perf_counter()
+    systems = [X.gen_diffusion3d(int(k), 1.5, 20261203 + 1000 * 4 + i) for i, k in enumerate(ks)]
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m = X.Matrix(systems, sigma=256, keep_lo=True)
+    build_s = time.perf_counter() - t0
+    n, nnz = sum(a.n for a in systems), sum(a.nnz for a in systems)
+    log(f"c4: {len(systems)} systems, n={n}, nnz={nnz}, generated in {gen_s:.1f} s, SELL built in {build_s:.1f} s")
+    b = np.concatenate([a.b for a in systems])
+    res = {"systems": len(systems), "n": n, "nnz": nnz, "k_min": int(ks.min()), "k_max": int(ks.max())}
+    for name, kw in (("fp32", {"fused": True}), ("accurate", {"accurate": True})):
+        m.autotune(fused=2 if name == "accurate" else True, tune_sigma=False, iters=32)
+        s = X.Pcg(m, rel_tol=1e-6, **kw)
+        s.set_rhs(b)
+        if name == "fp32":
+            ms, _ = timed(X, s.run, args.warmup, args.steps)
+            rel = s.true_residual() / np.array([np.linalg.norm(a.b) for a in systems])
+        else:
+            ms, (rel, _, _) = timed(X, lambda: s.run_reliable(1e-6, 0.0), args.warmup, args.steps)
+        its = [s.status(k)["iterations"] for k in range(len(systems))]
+        sts = [s.status(k)["status"] for k in range(len(systems))]
+        res[name] = {"ms": round(ms, 2), "iterations_min": min(its), "iterations_max": max(its),
+                     "converged": int(sum(1 for v in sts if v == X.CONVERGED)),
+                     "max_true_rel_residual": float(np.max(rel))}
+        log(f"c4 {name}: {res[name]}")
+        s.close()
+    print(json.dumps({"workload": "256 batched 3D 7-point systems to relative residual 1e-6", "unit": "ms",
+                      **res}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +304,9 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-grids", default="512,700,900,1100")
     ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2: the benchmark (default); c3: FP64-refined 1100/1600 FEM to 1e-10; "
+                         "c4: 256 batched 3D systems")
     ap.add_argument("--blocking", type=int, default=1, help="8-row blocks: 0 consecutive rows, 1 strength aggregates")
     # off-block weight of the training loss: on strength aggregates any
     # sizeable weight moves the blocks away from the exact inverses the
@@ -249,6 +323,10 @@ def main() -> int:
     if X.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.workload == "c3":
+        return workload_c3(X, args)
+    if args.workload == "c4":
+        return workload_c4(X, args)
     t0 = time.time()
     a = X.gen_fem2d(args.mesh, 1.0, 20, 1e3, 0.3, SEED)
     log(f"generated FEM mesh {args.mesh}: n={a.n} nnz={a.nnz} in {time.time() - t0:.1f}s")

