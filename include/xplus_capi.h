@@ -191,11 +191,15 @@ typedef struct {
                           iterations without halving the best r^T r */
     int fused;         /* 1: two kernels per iteration (p update folded into the
                           SpMV via A p = A z + beta A p_old); 0: three kernels */
-    int accurate;      /* 1: FP64-accumulated SpMV with hi+lo values (needs
-                          keep_lo or keep64), FP64 p, A p and x, FP32 r and z;
-                          two kernels per iteration.  The recursive residual
-                          then tracks b - A x, so ill-conditioned systems reach
-                          a true 1e-6 without refinement.  0: FP32 iteration. */
+    int accurate;      /* 1: FP64-accumulated SpMV (values: FP32 hi + a 16-bit
+                          remainder, ~39 bits; exact FP64 with keep64), FP64 p,
+                          A p and x, FP32 r and z; two kernels per iteration.
+                          The recursive residual then tracks b - A x, so
+                          ill-conditioned systems reach a true 1e-6 without
+                          refinement.  2: the same with the FP32 hi + lo pair
+                          (~48 bits), which xp_pcg_run_reliable also uses for
+                          targets below 1e-8.  Needs keep_lo or keep64.  0:
+                          FP32 iteration.  True residuals always use hi + lo. */
 } xp_solve_opts;
 
 typedef struct {

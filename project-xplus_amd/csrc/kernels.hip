@@ -201,7 +201,8 @@ __device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int
 }
 
 // A x for an FP32 vector x with FP64 values (VMODE 0: val64; 1: val +
-// val_lo) and FP64 accumulation.  An FP64 value times an FP32 operand rounds
+// val_lo; 2: val + val_lo16, ~39 bits, 2 bytes less per entry) and FP64
+// accumulation.  An FP64 value times an FP32 operand rounds
 // once in FP64 and the row sum carries only FP64 rounding, so A x keeps its
 // relative accuracy when the row's terms cancel (smooth x on an
 // ill-conditioned matrix), which an FP32 sum does not.  xdiag = x[row].
@@ -215,6 +216,12 @@ __device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int
     const int base = A.slice_base[slice];
     auto value = [&](size_t e) -> double {
         if (VMODE == 0) return A.val64[e];
+        if (VMODE == 2) {  // hi + lo16 * 2^(E-38), the double's exponent field built from hi's
+            const float hi = A.val[e];
+            const int eb = static_cast<int>((__float_as_uint(hi) >> 23) & 0xFFu);  // E + 127
+            const double scale = __longlong_as_double(static_cast<long long>(eb - 127 - 38 + 1023) << 52);
+            return static_cast<double>(hi) + static_cast<double>(A.val_lo16[e]) * scale;
+        }
         return static_cast<double>(A.val[e]) + static_cast<double>(A.val_lo[e]);
     };
     if (w <= kFastW) {
@@ -1362,16 +1369,21 @@ void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysSta
     hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
 }
 
-void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, bool use_lo, const SysState* st_cur,
+void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, int vmode, const SysState* st_cur,
                                SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
                                double* ap) {
+#define XP_FUSED_ACC(I, V) \
+    XP_LAUNCH(L, (k_pcg_spmv_fused_acc<I, V>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap)
     if (A.idx16) {
-        if (use_lo) XP_LAUNCH(L, (k_pcg_spmv_fused_acc<true, 1>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
-        else XP_LAUNCH(L, (k_pcg_spmv_fused_acc<true, 0>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
+        if (vmode == 2) XP_FUSED_ACC(true, 2);
+        else if (vmode == 1) XP_FUSED_ACC(true, 1);
+        else XP_FUSED_ACC(true, 0);
     } else {
-        if (use_lo) XP_LAUNCH(L, (k_pcg_spmv_fused_acc<false, 1>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
-        else XP_LAUNCH(L, (k_pcg_spmv_fused_acc<false, 0>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap);
+        if (vmode == 2) XP_FUSED_ACC(false, 2);
+        else if (vmode == 1) XP_FUSED_ACC(false, 1);
+        else XP_FUSED_ACC(false, 0);
     }
+#undef XP_FUSED_ACC
 }
 
 void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,

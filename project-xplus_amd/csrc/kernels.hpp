@@ -52,7 +52,8 @@ void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_n
 // after init; the first fused iteration sets p = z, ap = A z (state.fresh).
 void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
                          const double* b, const float* minv, const BlockPre* blockM, float* r, float* z);
-void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, bool use_lo, const SysState* st_cur,
+// vmode: 0 FP64 values, 1 hi + lo (FP32 pair), 2 hi + lo16.
+void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, int vmode, const SysState* st_cur,
                                SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
                                double* ap);
 void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,

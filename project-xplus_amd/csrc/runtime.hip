@@ -155,6 +155,7 @@ struct xp_matrix {
     std::vector<int> diag_state;      // 0 ok, 1 missing, 2 zero (|d| <= DBL_MIN)
     DevBuf<int> d_slice_ptr, d_slice_base, d_perm;
     DevBuf<float> d_val, d_val_lo;
+    DevBuf<int16_t> d_val_lo16;
     DevBuf<double> d_val64;
     DevBuf<uint16_t> d_col16;
     DevBuf<int> d_col32;
@@ -176,6 +177,7 @@ struct xp_matrix {
         v.slice_base = d_slice_base.p;
         v.val = d_val.p;
         v.val_lo = d_val_lo.p;
+        v.val_lo16 = d_val_lo16.p;
         v.val64 = d_val64.p;
         v.col16 = d_col16.p;
         v.col32 = d_col32.p;
@@ -299,10 +301,12 @@ void load_sell(xp_matrix& m, int sigma) {
     m.d_perm = to_device(h.perm, s);
     m.d_val = to_device(h.val, s);
     m.d_val_lo.release();
+    m.d_val_lo16.release();
     m.d_val64.release();
     m.d_col16.release();
     m.d_col32.release();
     if (m.keep_lo) m.d_val_lo = to_device(h.val_lo, s);
+    if (m.keep_lo) m.d_val_lo16 = to_device(h.val_lo16, s);
     if (m.keep64) m.d_val64 = to_device(h.val64, s);
     if (h.idx16)
         m.d_col16 = to_device(h.col16, s);
@@ -364,6 +368,7 @@ struct xp_pcg {
     hipGraphExec_t exec = nullptr;
     int graph_iters = 0;
     int graph_version = -1;
+    int graph_vmode = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;
@@ -392,6 +397,12 @@ struct xp_pcg {
     }
 
     const xp::BlockPre* bm() const { return use_block ? blockM.p : nullptr; }
+    // values of the accurate iteration's SpMV: exact FP64 when kept, else
+    // hi + lo16 (accurate = 1) or hi + lo (accurate = 2)
+    // hi + lo16 carries ~2^-39: enough for targets of 1e-8 and above; tighter
+    // solves (tight, set per xp_pcg_run_reliable) use hi + lo
+    int acc_vmode() const { return m->has64 ? 0 : (opts.accurate == 2 || tight ? 1 : 2); }
+    bool tight = false;
 
     // Iteration k reads state slot k % 2 and writes slot (k + 1) % 2; the
     // graph holds an even number of iterations, so slot 0 is current
@@ -403,7 +414,7 @@ struct xp_pcg {
         xp::SysState* cur = st.p + (k % 2) * ns;
         xp::SysState* nxt = st.p + ((k + 1) % 2) * ns;
         if (opts.accurate) {
-            xp::launch_pcg_spmv_fused_acc(L, A, !m->has64, cur, nxt, trace.p, trace_stride, z.p, p64.p, ap64.p);
+            xp::launch_pcg_spmv_fused_acc(L, A, acc_vmode(), cur, nxt, trace.p, trace_stride, z.p, p64.p, ap64.p);
             xp::launch_pcg_update_xrz_acc(L, nxt, x64.p, p64.p, r.p, ap64.p, minv.p, bm(), z.p);
         } else if (opts.fused) {
             xp::launch_pcg_spmv_fused(L, A, cur, nxt, trace.p, trace_stride, z.p, p.p, ap.p);
@@ -424,12 +435,14 @@ struct xp_pcg {
         HIPX(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         graph_iters = iters;
         graph_version = m->layout_version;
+        graph_vmode = acc_vmode();
     }
 
     // A graph captures the matrix's work map and partials by address: after
     // a workgroup-count change (autotuning) it is captured again.
     void ensure_graph() {
-        if (opts.use_graph && (exec == nullptr || graph_version != m->layout_version)) build_graph(opts.graph_iters);
+        if (opts.use_graph && (exec == nullptr || graph_version != m->layout_version || graph_vmode != acc_vmode()))
+            build_graph(opts.graph_iters);
     }
 
     void reset_state() {
@@ -532,7 +545,7 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
     d.graph_iters += d.graph_iters % 2;  // even: the state slots alternate
     if (d.accurate) {
         if (!m->has_lo && !m->has64) throw std::runtime_error("the accurate iteration needs a matrix built with keep_lo");
-        d.accurate = 1;
+        d.accurate = d.accurate == 2 ? 2 : 1;
         d.fused = 1;
     }
     s->opts = d;
@@ -1018,6 +1031,7 @@ int xp_pcg_run(xp_pcg* s) {
         long long cap = 0;
         for (const auto& t : s->st_init) cap = std::max<long long>(cap, t.max_iters);
         s->fallback = 0;
+        s->tight = false;
         // Every run starts from x0 and the configured state.
         s->reset_x();
         s->reset_state();
@@ -1054,6 +1068,7 @@ int xp_pcg_run_fixed(xp_pcg* s, int iters) {
     return guarded([&] {
         if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_pcg_run_fixed: call xp_pcg_set_rhs first");
         if (iters < 0) throw std::runtime_error("iters must be >= 0");
+        s->tight = false;
         s->reset_x();
         std::vector<xp::SysState> init = s->st_init;
         for (auto& t : init) t.max_iters = std::min(t.max_iters, std::max(iters, 0));
@@ -1149,8 +1164,8 @@ int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels) {
             xp::SysState* cur = S.st.p + (k % 2) * ns;
             xp::SysState* nxt = S.st.p + ((k + 1) % 2) * ns;
             if (S.opts.accurate) {
-                xp::launch_pcg_spmv_fused_acc(L1, A, !m.has64, cur, nxt, S.trace.p, S.trace_stride, S.z.p, S.p64.p,
-                                              S.ap64.p);
+                xp::launch_pcg_spmv_fused_acc(L1, A, S.acc_vmode(), cur, nxt, S.trace.p, S.trace_stride, S.z.p,
+                                              S.p64.p, S.ap64.p);
                 xp::launch_pcg_update_xrz_acc(L2, nxt, S.x64.p, S.p64.p, S.r.p, S.ap64.p, S.minv.p, S.bm(), S.z.p);
             } else if (S.opts.fused) {
                 xp::launch_pcg_spmv_fused(L1, A, cur, nxt, S.trace.p, S.trace_stride, S.z.p, S.p.p, S.ap.p);
@@ -1360,6 +1375,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
             if (S.opts.use_graph) S.build_graph(S.opts.graph_iters);
         }
         S.fallback = 0;
+        S.tight = rel_tol < 1e-8;
         ensure_f64(S);
         S.ensure_graph();
         if (S.repl_out.n < 3 * ns) S.repl_out.alloc(3 * ns);

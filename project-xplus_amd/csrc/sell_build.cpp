@@ -2,6 +2,7 @@
 #include "sell_build.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <stdexcept>
 
@@ -12,6 +13,15 @@ namespace {
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 }  // namespace
+
+int16_t lo16_of(double v, float hi) {
+    if (hi == 0.0f || !std::isfinite(hi)) return 0;
+    int e = 0;
+    (void)std::frexp(static_cast<double>(hi), &e);  // hi = f 2^e, 0.5 <= |f| < 1: E = e - 1
+    const double q = std::ldexp(v - static_cast<double>(hi), 38 - (e - 1));
+    const double r = std::nearbyint(q);
+    return static_cast<int16_t>(std::max(-32767.0, std::min(32767.0, r)));
+}
 
 SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64, int block_rows,
                     const std::vector<Blocking>* blocks) {
@@ -146,6 +156,7 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
     const size_t total = static_cast<size_t>(s.stored);
     s.val.assign(total, 0.0f);
     if (keep_lo) s.val_lo.assign(total, 0.0f);
+    if (keep_lo) s.val_lo16.assign(total, 0);
     if (keep64) s.val64.assign(total, 0.0);
     std::vector<int> cols(total, 0);
     s.slice_base.assign(static_cast<size_t>(s.nslices), 0);
@@ -181,6 +192,7 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
             s.has_diag[static_cast<size_t>(i)] = dp >= 0 ? 1 : 0;
             s.val[e0] = static_cast<float>(dv);
             if (keep_lo) s.val_lo[e0] = static_cast<float>(dv - static_cast<double>(s.val[e0]));
+            if (keep_lo) s.val_lo16[e0] = lo16_of(dv, s.val[e0]);
             if (keep64) s.val64[e0] = dv;
             int j = 1;
             for (int o = a.row_ptr[r]; o < a.row_ptr[r + 1]; ++o) {
@@ -190,6 +202,7 @@ SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo,
                 const double v = a.values[o];
                 s.val[e] = static_cast<float>(v);
                 if (keep_lo) s.val_lo[e] = static_cast<float>(v - static_cast<double>(s.val[e]));
+                if (keep_lo) s.val_lo16[e] = lo16_of(v, s.val[e]);
                 if (keep64) s.val64[e] = v;
                 cols[e] = c;
                 cmin = std::min(cmin, c);

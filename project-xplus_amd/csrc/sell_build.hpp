@@ -42,6 +42,7 @@ struct SellHost {
     std::vector<int> slice_base;
     std::vector<float> val;
     std::vector<float> val_lo;
+    std::vector<int16_t> val_lo16;  // with keep_lo: the same remainder in 16 bits (see lo16_of)
     std::vector<double> val64;
     std::vector<uint16_t> col16;
     std::vector<int> col32;
@@ -60,6 +61,12 @@ struct SellHost {
 // rows (sorted as units; smaller blocks padded), for block preconditioners:
 // the blocks of `blocks` (one Blocking per system), or 8 consecutive natural
 // rows when it is null.
+// The remainder v - (double)hi of an FP64 value v with FP32 part hi, as an
+// integer count of 2^(E-38) where hi = m 2^E, 1 <= |m| < 2: |remainder| <=
+// ulp(hi) / 2 = 2^(E-24), so the count fits in 15 bits, and hi + count *
+// 2^(E-38) reproduces v to ~2^-39 relative.  0 for hi = 0.
+int16_t lo16_of(double v, float hi);
+
 SellHost build_sell(const std::vector<CsrRef>& systems, int sigma, bool keep_lo, bool keep64,
                     int block_rows = 1, const std::vector<Blocking>* blocks = nullptr);
 

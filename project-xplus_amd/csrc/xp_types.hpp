@@ -101,6 +101,8 @@ struct SellView {
     const int* slice_base;   // [nslices]
     const float* val;        // [slice_ptr[nslices] * 64], diagonal first
     const float* val_lo;     // optional: v - (double)val, rounded to float
+    const int16_t* val_lo16; // optional: v - (double)val in units of 2^(E-38), E = exponent of val
+                             // (val + lo16 carries ~39 bits: the accurate iteration's SpMV)
     const double* val64;     // optional: exact FP64 values
     const uint16_t* col16;
     const int* col32;

@@ -483,7 +483,7 @@ class Pcg:
         the matrix needs keep_lo) with FP64 p, A p and x."""
         self.matrix = matrix
         self.opts = SolveOpts(tau, rel_tol, max_iters, graph_iters, int(use_graph), trace_cap, stall_limit,
-                              int(fused), int(accurate))
+                              int(fused), int(accurate))  # accurate: True/1 (hi + lo16) or 2 (hi + lo)
         h = _vp()
         _check(_p_create(matrix.h, C.byref(self.opts), C.byref(h)))
         self.h = h

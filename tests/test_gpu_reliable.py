@@ -247,3 +247,30 @@ def test_accurate_to_1e10_against_fp64_oracle(gpu, delta):
     # a restart (delta 0) discards the Krylov space once: up to 2x
     assert iters <= (1.5 if delta > 0 else 2.0) * g["iterations"]
     assert err <= 1e-6
+
+
+def test_accurate_value_modes_agree(gpu):
+    """The accurate iteration's SpMV with FP32 hi + 16-bit remainder (mode 1,
+    ~39-bit values) against hi + FP32 remainder (mode 2, ~48 bits): the rr
+    traces agree to 1e-6 relative over 20 iterations, the solutions to 1e-5,
+    and both meet the true tolerance."""
+    a = X.gen_fem2d(66, 1.0, 6, 1e3, 0.3, 20264501)
+    m = X.Matrix(a, keep_lo=True)
+    out = {}
+    for mode in (1, 2):
+        s = X.Pcg(m, rel_tol=1e-6, accurate=mode, trace_cap=5000)
+        s.set_rhs(a.b)
+        rel, repl, iters = s.run_reliable(1e-6, 0.0)
+        out[mode] = (s.trace()[:, 3], iters, s.x(), rel[0])
+    t1, t2 = out[1][0], out[2][0]
+    xdiff = np.linalg.norm(out[1][2] - out[2][2]) / np.linalg.norm(out[2][2])
+    print("iterations", out[1][1], out[2][1], "x diff", xdiff,
+          "trace rel diff at 10/30/100:", [abs(t1[i] / t2[i] - 1) for i in (10, 30, 100)], flush=True)
+    # CG on this 1e3-contrast matrix amplifies a 1e-12 change of the values
+    # along the run (measured: 30x apart in rr by iteration ~400, while the
+    # solutions agree to 3e-7; 4e-10 apart at iteration 10, 5e-7 at 30, 1e-2 at
+    # 100): the traces are compared over 20 iterations
+    np.testing.assert_allclose(t1[:20], t2[:20], rtol=1e-6)
+    assert xdiff <= 1e-5
+    assert abs(out[1][1] - out[2][1]) <= max(2, 0.02 * out[2][1])   # 577 vs 581 measured
+    assert out[1][3] <= 1e-6 and out[2][3] <= 1e-6
