@@ -273,26 +273,17 @@ __device__ __forceinline__ float half_at(const uint4 (&q)[5], int t) {  // t com
     return static_cast<float>(__builtin_bit_cast(_Float16, h));
 }
 
-__device__ __forceinline__ float block_apply(const BlockPre* __restrict__ M, int row, float r) {
+// The apply itself, from a block factor already in registers.
+__device__ __forceinline__ float block_apply_factor(const uint4 (&q)[5], float sj, int j, float r) {
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int j = row & 7;
-    const BlockPre* b = M + (row >> 3);
-    const uint4* cp = reinterpret_cast<const uint4*>(b->c);
-    uint4 q[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) q[k] = cp[k];
-    const float sj = b->s[j];
     const float u = sj * r;
-    float uu[8];
+    // w = C^T u, accumulated row of C by row of C: one shuffled u_i live at a time
+    float w[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int i = 0; i < 8; ++i) uu[i] = __shfl(u, (lane & ~7) | i, 64);
-    float w[8];
+    for (int i = 0; i < 8; ++i) {
+        const float ui = __shfl(u, (lane & ~7) | i, 64);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int i = k; i < 8; ++i) acc = fmaf(half_at(q, i * (i + 1) / 2 + k), uu[i], acc);
-        w[k] = acc;
+        for (int k = 0; k <= i; ++k) w[k] = fmaf(half_at(q, i * (i + 1) / 2 + k), ui, w[k]);
     }
     float v = 0.0f;
 #pragma unroll
@@ -303,6 +294,15 @@ __device__ __forceinline__ float block_apply(const BlockPre* __restrict__ M, int
         v = i == j ? acc : v;
     }
     return sj * v;
+}
+
+__device__ __forceinline__ float block_apply(const BlockPre* __restrict__ M, int row, float r) {
+    const BlockPre* b = M + (row >> 3);
+    const uint4* cp = reinterpret_cast<const uint4*>(b->c);
+    uint4 q[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[k] = cp[k];
+    return block_apply_factor(q, b->s[row & 7], row & 7, r);
 }
 
 // Factors full FP32 blocks (solve order, block q = 64 floats row-major) into
@@ -800,7 +800,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
                                                                const float* __restrict__ minv,
                                                                const BlockPre* __restrict__ blockM,
                                                                float* __restrict__ z) {
-    constexpr int ROWS = BLOCK ? 1 : kRowsAcc;  // the block apply holds a block factor in flight
+    constexpr int ROWS = BLOCK ? 1 : kRowsAcc;  // measured: two rows per pass is slower with blocks
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
