@@ -73,14 +73,6 @@ def roof(kernel: str, alg_bytes: float, us: float, note: str) -> dict:
             "alg_bytes": int(alg_bytes), "alg_bytes_formula": note}
 
 
-def spmv_roofline(m, a, reps: int = 200) -> dict:
-    """Standalone FP32 SELL SpMV (k_spmv_f32), HIP events on its stream."""
-    us = m.spmv_f32_time(20, reps)
-    r = roof("k_spmv_f32", 8 * a.nnz + 12 * a.n, us, "8 nnz + 12 n (FP32 value + int32 column; row ptr, x, y)")
-    r["stream_bytes"] = m.info()["bytes_spmv"]
-    return r
-
-
 def profiled_traffic() -> dict:
     """HBM bytes per launch measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE
     passes committed under profiles/ (lines 'traffic_bytes_per_launch K B')."""
@@ -91,6 +83,17 @@ def profiled_traffic() -> dict:
             if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
                 out[parts[1]] = (int(parts[2]), f"profiles/{f.name}")
     return out
+
+
+def spmv_roofline(m, a, reps: int = 200) -> dict:
+    """Standalone FP32 SELL SpMV (k_spmv_f32), HIP events on its stream."""
+    us = m.spmv_f32_time(20, reps)
+    r = roof("k_spmv_f32", 8 * a.nnz + 12 * a.n, us, "8 nnz + 12 n (FP32 value + int32 column; row ptr, x, y)")
+    r["stream_bytes"] = m.info()["bytes_spmv"]
+    prof = profiled_traffic()
+    if r["kernel"] in prof:
+        r["traffic"], r["traffic_source"] = prof[r["kernel"]]
+    return r
 
 
 def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
