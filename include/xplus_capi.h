@@ -348,6 +348,42 @@ typedef struct xp_csr xp_csr;
  * same messages (via xp_last_error). */
 int xp_dataset_load(const char* dir, xp_csr** out);
 int xp_dataset_save(const xp_csr* a, const char* dir);
+/* MatrixMarket ingestion: parse_coordinate_matrix, parse_vector,
+ * reference_solution and convert_dataset of
+ * scripts/download_suitesparse_data.py:224-349,418-444.  Coordinate
+ * real|integer|pattern matrices, general|symmetric; explicit zeros skipped,
+ * duplicates summed in file order, symmetric entries mirrored, the leading
+ * subset_n x subset_n principal submatrix kept (subset_n <= 0: all), columns
+ * ascending, sums equal to 0 dropped.  b is read from b_path (array or
+ * coordinate; NULL or "" for none), else b = A x_ref with the converter's
+ * x_ref and a sequential FP64 row sum; x0 = 0.  The converter's error
+ * messages; a row or column index below 1 is an error. */
+typedef struct xp_mtx_info {
+    long long nrows, ncols, nnz_reported;  /* the size line */
+    long long entries;                     /* entry lines read */
+    int field;                             /* 0 real, 1 integer, 2 pattern */
+    int symmetry;                          /* 0 general, 1 symmetric */
+    int n, m;                              /* shape kept */
+    long long nnz;                         /* stored entries */
+    int has_diag;                          /* a nonzero diagonal entry is kept */
+    double diag_min_abs;                   /* smallest |a_ii| kept (0 when none) */
+} xp_mtx_info;
+int xp_mtx_load(const char* mtx_path, const char* b_path, long long subset_n, xp_mtx_info* info, xp_csr** out);
+/* parse_vector (download_suitesparse_data.py:287-336): writes
+ * min(expected_n, subset_n) entries (expected_n when subset_n <= 0). */
+int xp_mtx_read_vector(const char* path, int expected_n, long long subset_n, double* out);
+/* reference_solution (download_suitesparse_data.py:339-349), n entries. */
+int xp_reference_solution(int n, double* x);
+/* Binary CSR cache (no counterpart in the reference, which reparses the
+ * text files on every run): one file holding row_ptr, col_idx, values, b, x0
+ * with a payload checksum.  load_cached reads dir's five text files through
+ * the cache at cache_path (NULL or "": dir/csr_cache.xpbin), which is used
+ * when the recorded sizes and modification times of the text files match and
+ * rewritten otherwise (a read-only location is not an error); *hit says
+ * which.  Same validation and messages as xp_dataset_load. */
+int xp_dataset_save_binary(const xp_csr* a, const char* path);
+int xp_dataset_load_binary(const char* path, xp_csr** out);
+int xp_dataset_load_cached(const char* dir, const char* cache_path, int* hit, xp_csr** out);
 /* Generators (seeded, deterministic).  b ~ U[-1,1], x0 = 0. */
 int xp_gen_diffusion2d(int nx, int ny, double sigma, unsigned long long seed, xp_csr** out);
 int xp_gen_diffusion3d(int k, double sigma, unsigned long long seed, xp_csr** out);

@@ -24,6 +24,7 @@
 #include "generators.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
+#include "mtx_ingest.hpp"
 #include "sell_build.hpp"
 
 namespace {
@@ -1539,6 +1540,75 @@ int xp_dataset_save(const xp_csr* a, const char* dir) {
         if (a == nullptr || dir == nullptr) throw std::runtime_error("null argument");
         xp::CsrDataset d = xp::CsrDataset::from_arrays(a->a.row_ptr, a->a.col_idx, a->a.values, a->a.b, a->a.x0);
         d.save(dir);
+    });
+}
+
+int xp_mtx_load(const char* mtx_path, const char* b_path, long long subset_n, xp_mtx_info* info, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr || mtx_path == nullptr) throw std::runtime_error("null argument");
+        *out = nullptr;
+        xp::MtxMatrix M;
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::mtx_dataset(mtx_path, b_path != nullptr ? b_path : "", subset_n, &M);
+        if (info != nullptr) {
+            info->nrows = M.nrows;
+            info->ncols = M.ncols;
+            info->nnz_reported = M.nnz_reported;
+            info->entries = M.entries;
+            info->field = M.field;
+            info->symmetry = M.symmetry;
+            info->n = M.n;
+            info->m = M.m;
+            info->nnz = static_cast<long long>(h->a.col_idx.size());
+            info->has_diag = M.has_diag ? 1 : 0;
+            info->diag_min_abs = M.diag_min_abs;
+        }
+        *out = h.release();
+    });
+}
+
+int xp_mtx_read_vector(const char* path, int expected_n, long long subset_n, double* out) {
+    return guarded([&] {
+        if (path == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        const std::vector<double> v = xp::read_mtx_vector(path, expected_n, subset_n);
+        std::copy(v.begin(), v.end(), out);
+    });
+}
+
+int xp_reference_solution(int n, double* x) {
+    return guarded([&] {
+        if (x == nullptr && n > 0) throw std::runtime_error("null argument");
+        const std::vector<double> v = xp::reference_solution(n);
+        std::copy(v.begin(), v.end(), x);
+    });
+}
+
+int xp_dataset_save_binary(const xp_csr* a, const char* path) {
+    return guarded([&] {
+        if (a == nullptr || path == nullptr) throw std::runtime_error("null argument");
+        xp::save_binary(a->a, path, xp::SourceStamp{});
+    });
+}
+
+int xp_dataset_load_binary(const char* path, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr || path == nullptr) throw std::runtime_error("null argument");
+        *out = nullptr;
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::load_binary(path, nullptr);
+        *out = h.release();
+    });
+}
+
+int xp_dataset_load_cached(const char* dir, const char* cache_path, int* hit, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr || dir == nullptr) throw std::runtime_error("null argument");
+        *out = nullptr;
+        bool was_hit = false;
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::load_cached(dir, cache_path != nullptr ? cache_path : "", &was_hit);
+        if (hit != nullptr) *hit = was_hit ? 1 : 0;
+        *out = h.release();
     });
 }
 

@@ -163,6 +163,21 @@ _csr_from = _sig("xp_csr_from_arrays", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _
 _csr_info = _sig("xp_csr_info", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)])
 _csr_copy = _sig("xp_csr_copy", C.c_int, [_vp, _i32p, _i32p, _f64p, _f64p, _f64p])
 _csr_free = _sig("xp_csr_free", None, [_vp])
+
+
+class MtxInfo(C.Structure):
+    _fields_ = [("nrows", C.c_longlong), ("ncols", C.c_longlong), ("nnz_reported", C.c_longlong),
+                ("entries", C.c_longlong), ("field", C.c_int), ("symmetry", C.c_int), ("n", C.c_int),
+                ("m", C.c_int), ("nnz", C.c_longlong), ("has_diag", C.c_int), ("diag_min_abs", C.c_double)]
+
+
+_mtx_load = _sig("xp_mtx_load", C.c_int, [C.c_char_p, C.c_char_p, C.c_longlong, C.POINTER(MtxInfo), C.POINTER(_vp)])
+_mtx_vector = _sig("xp_mtx_read_vector", C.c_int, [C.c_char_p, C.c_int, C.c_longlong, _f64p])
+_ref_solution = _sig("xp_reference_solution", C.c_int, [C.c_int, _f64p])
+_ds_save_bin = _sig("xp_dataset_save_binary", C.c_int, [_vp, C.c_char_p])
+_ds_load_bin = _sig("xp_dataset_load_binary", C.c_int, [C.c_char_p, C.POINTER(_vp)])
+_ds_load_cached = _sig("xp_dataset_load_cached", C.c_int,
+                       [C.c_char_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(_vp)])
 _sell_build = _sig("xp_sell_build_host", C.c_int,
                    [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                     C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
@@ -264,6 +279,14 @@ class Csr:
         finally:
             _csr_free(h)
 
+    def save_binary(self, path) -> None:
+        """One-file binary copy (row_ptr, col_idx, values, b, x0 + checksum)."""
+        h = self._handle()
+        try:
+            _check(_ds_save_bin(h, str(path).encode()))
+        finally:
+            _csr_free(h)
+
     def to_dense(self) -> np.ndarray:
         a = np.zeros((self.n, self.n))
         for r in range(self.n):
@@ -282,6 +305,92 @@ def load_dataset(directory) -> Csr:
     h = _vp()
     _check(_ds_load(str(directory).encode(), C.byref(h)))
     return Csr._from_handle(h)
+
+
+def load_binary(path) -> Csr:
+    h = _vp()
+    _check(_ds_load_bin(str(path).encode(), C.byref(h)))
+    return Csr._from_handle(h)
+
+
+def load_dataset_cached(directory, cache_path=None) -> tuple:
+    """load_dataset through the binary cache: (Csr, True when the cache was
+    used).  The cache (default directory/csr_cache.xpbin) is rewritten when
+    the text files' sizes or modification times changed."""
+    h = _vp()
+    hit = C.c_int()
+    cache = str(cache_path).encode() if cache_path is not None else None
+    _check(_ds_load_cached(str(directory).encode(), cache, C.byref(hit), C.byref(h)))
+    return Csr._from_handle(h), bool(hit.value)
+
+
+MTX_FIELDS = ("real", "integer", "pattern")
+MTX_SYMMETRIES = ("general", "symmetric")
+X_REF_SOURCE = "A * deterministic smooth x_ref"
+
+
+def load_mtx(path, b_path=None, subset_n=None) -> tuple:
+    """MatrixMarket matrix -> (Csr, info), as the reference's converter
+    (scripts/download_suitesparse_data.py:224-349,418-444) builds a dataset:
+    b read from b_path, else b = A x_ref; x0 = 0; subset_n keeps the leading
+    principal submatrix."""
+    h = _vp()
+    info = MtxInfo()
+    bp = str(b_path).encode() if b_path else None
+    _check(_mtx_load(str(path).encode(), bp, int(subset_n or 0), C.byref(info), C.byref(h)))
+    d = {k: getattr(info, k) for k, _ in MtxInfo._fields_}
+    d["field"] = MTX_FIELDS[info.field]
+    d["symmetry"] = MTX_SYMMETRIES[info.symmetry]
+    d["has_diag"] = bool(info.has_diag)
+    d["diag_min_abs"] = info.diag_min_abs if info.has_diag else None
+    return Csr._from_handle(h), d
+
+
+def read_mtx_vector(path, expected_n: int, subset_n=None) -> np.ndarray:
+    limit = min(expected_n, subset_n) if subset_n else expected_n
+    out = np.zeros(max(limit, 1), np.float64)
+    _check(_mtx_vector(str(path).encode(), expected_n, int(subset_n or 0), out))
+    return out[:limit]
+
+
+def reference_solution(n: int) -> np.ndarray:
+    out = np.zeros(max(n, 1), np.float64)
+    _check(_ref_solution(n, out))
+    return out[:n]
+
+
+def convert_mtx(mtx_path, out_dir, b_path=None, subset_n=None, name=None, source=None,
+                description: str = "") -> dict:
+    """convert_dataset (scripts/download_suitesparse_data.py:418-444) minus the
+    download: writes row_ptr/col_idx/values/b/x0 .txt (%.17g) and
+    meta.json/meta.txt (write_metadata, :379-415) into out_dir."""
+    import json
+    mtx_path, out_dir = Path(mtx_path), Path(out_dir)
+    a, info = load_mtx(mtx_path, b_path, subset_n)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    a.save(out_dir)
+    name = name or out_dir.name
+    meta = {"name": name, "source": source or str(mtx_path), "shape": [a.n, a.n], "nnz_csr": a.nnz,
+            "matrix_market_path": str(mtx_path), "b_source": Path(b_path).name if b_path else X_REF_SOURCE,
+            "x0": "zeros", "matrix_market_storage": "coordinate", "matrix_market_field": info["field"],
+            "matrix_market_symmetry": info["symmetry"], "original_shape": [info["nrows"], info["ncols"]],
+            "has_nonzero_diag_entries": info["has_diag"]}
+    if description:
+        meta["description"] = description
+    if info["diag_min_abs"] is not None:
+        meta["diag_min_abs"] = info["diag_min_abs"]
+    if subset_n is not None:
+        meta["subset_type"] = "leading_principal_submatrix"
+        meta["subset_n"] = subset_n
+    (out_dir / "meta.json").write_text(json.dumps(meta, indent=2, sort_keys=True) + "\n", encoding="utf-8")
+    lines = [f"name={name}", f"source={meta['source']}", f"n={a.n}", f"m={a.n}", f"nnz={a.nnz}",
+             f"b_source={meta['b_source']}", "x0=zeros"]
+    if description:
+        lines.append(f"description={description}")
+    if subset_n is not None:
+        lines += ["subset_type=leading_principal_submatrix", f"subset_n={subset_n}"]
+    (out_dir / "meta.txt").write_text("\n".join(lines) + "\n", encoding="utf-8")
+    return meta
 
 
 def gen_diffusion2d(nx: int, ny: int, sigma: float = 1.0, seed: int = 0) -> Csr:
