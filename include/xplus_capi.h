@@ -341,6 +341,46 @@ int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net);
 int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale);
 
 /* ===================================================================== *
+ * Plain Jacobi iteration x_new = D^-1 (b - R x), A = D + R
+ * (DLC/Cuper-jacobi-iteration/host/main.cpp:957-1053, the reference's
+ * PrepareJacobiVectorsAndR + RunJacobiCpu and its CuperJacobiIteration
+ * kernel, kernels/detail/jacobi_controller.hpp:155-283).  FP32 like the
+ * reference's VALUE_TYPE: D is the FP32 sum of each row's diagonal entries
+ * in CSR order, Diag_inv = 1 / D, R x summed per row in CSR order with one
+ * rounding per product and per sum, so every update is the reference's bit
+ * for bit.  max_iters updates (the reference's fixed-count contract); with
+ * use_tau, the iteration also stops once max_i |x_new - x_old| <= tau (the
+ * stop rule of docs/jacobi_iteration.md:99-124).  A non-finite difference
+ * stops it as breakdown.  Error "missing or zero diagonal at row R".
+ * ===================================================================== */
+typedef struct xp_jacobi xp_jacobi;
+typedef struct xp_jacobi_opts {
+    double tau;       /* default 1e-5 (the reference's TAU) */
+    int max_iters;    /* default 1 (the reference's MAX_ITERS); 0 allowed */
+    int use_tau;      /* 0: fixed count */
+    int graph_iters;  /* updates per graph launch (default 32, rounded up to even) */
+    int use_graph;
+} xp_jacobi_opts;
+typedef struct xp_jacobi_status {
+    int status;       /* 0 converged, 1 max_iter (kJacobiStatusMaxIter), 2 breakdown */
+    int iterations;   /* updates done (Status[2] of the reference kernel) */
+    double diff;      /* max |x_new - x_old| of the last update */
+} xp_jacobi_status;
+int xp_jacobi_create(xp_matrix* m, const xp_jacobi_opts* opts, xp_jacobi** out);
+void xp_jacobi_destroy(xp_jacobi* s);
+/* b and x0 in natural order (NULL x0: zeros, as the reference starts). */
+int xp_jacobi_set_rhs(xp_jacobi* s, const double* b, const double* x0);
+int xp_jacobi_run(xp_jacobi* s);
+int xp_jacobi_get_status(xp_jacobi* s, int sys, xp_jacobi_status* out);
+int xp_jacobi_get_x(xp_jacobi* s, double* x);
+int xp_jacobi_last_timing(xp_jacobi* s, double* ms, int* graph_launches);
+/* Average device time of one update kernel over `iters` direct launches. */
+int xp_jacobi_kernel_time(xp_jacobi* s, int iters, double* us);
+/* The reference's right-hand side when a dataset has no b: A * ones in FP32
+ * (host/main.cpp:969-973), on the host. */
+int xp_rhs_ones_f32(int n, const int* row_ptr, const int* col_idx, const double* values, double* b);
+
+/* ===================================================================== *
  * Host helpers: loader and generators (no GPU needed).
  * ===================================================================== */
 typedef struct xp_csr xp_csr;

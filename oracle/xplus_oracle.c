@@ -29,6 +29,14 @@
  *   orc_multi_kernel_solver -- bitwise equality with the reference
  *                              run_local_multi_kernel_solver (n <= 1024).
  *   orc_cuper_style_spmv    -- bitwise equality with reference cuper_style_spmv.
+ *   orc_jacobi_prepare_f32, orc_jacobi_run_f32 -- the reference's Jacobi host
+ *                              (DLC/Cuper-jacobi-iteration/host/main.cpp)
+ *                              needs TAPA/XRT and cannot be built here; pinned
+ *                              by the closed forms in tests/test_jacobi.py:
+ *                              one update of a diagonal system, the error
+ *                              recursion e_k = (-D^-1 R)^k e_0 in FP64, the
+ *                              fixed point A^-1 b, brute-force diff_max,
+ *                              duplicate-diagonal sums, the A * ones rows.
  */
 #include <math.h>
 #include <float.h>
@@ -489,6 +497,80 @@ void orc_cuper_style_spmv(int n, const int* row_ptr, const int* col_idx, const d
         y[rows[e]] += (double)prod;
     }
     free(start); free(order); free(rows);
+}
+
+/* ---- Plain Jacobi iteration, FP32 (VALUE_TYPE float), ----
+ * DLC/Cuper-jacobi-iteration/host/main.cpp:957-1009 PrepareJacobiVectorsAndR:
+ * values rounded to float; without a b, b = A * ones (SpMV_CPU_CSR,
+ * include/Cuper_common.h:299-320: float sum from 0 in CSR order); D = float
+ * sum of the row's diagonal entries in CSR order; R = the other entries in
+ * CSR order; Diag_inv = 1.0f / D.  Returns -1, or the first row with a
+ * missing or zero diagonal. */
+int orc_jacobi_prepare_f32(int n, const int* row_ptr, const int* col_idx, const double* values, const double* b_in,
+                           float* b, int* r_rp, int* r_ci, float* r_val, float* dinv) {
+    for (int row = 0; row < n; ++row) {
+        if (b_in != NULL) {
+            b[row] = (float)b_in[row];
+        } else {
+            float sum = 0.0f;
+            for (int o = row_ptr[row]; o < row_ptr[row + 1]; ++o) sum += 1.0f * (float)values[o];
+            b[row] = sum;
+        }
+    }
+    int k = 0;
+    for (int row = 0; row < n; ++row) {
+        int found = 0;
+        float d = 0.0f;
+        r_rp[row] = k;
+        for (int o = row_ptr[row]; o < row_ptr[row + 1]; ++o) {
+            if (col_idx[o] == row) {
+                d += (float)values[o];
+                found = 1;
+            } else {
+                r_ci[k] = col_idx[o];
+                r_val[k] = (float)values[o];
+                ++k;
+            }
+        }
+        if (!found || d == 0.0f) return row;
+        dinv[row] = 1.0f / d;
+    }
+    r_rp[n] = k;
+    return -1;
+}
+
+/* RunJacobiCpu (host/main.cpp:1011-1053): every update forms x_next from the
+ * whole old x (R x in float, CSR order, from 0), next = (b - rx) * Diag_inv,
+ * diff_max = max |next - x|, then swaps.  max_iters updates; with use_tau
+ * the loop also ends once diff_max <= tau (docs/jacobi_iteration.md:99-124),
+ * and a NaN or infinite difference ends it as breakdown (:148-154).
+ * x holds x0 on entry and the result on return; returns the update count. */
+int orc_jacobi_run_f32(int n, const int* r_rp, const int* r_ci, const float* r_val, const float* b, const float* dinv,
+                       int max_iters, int use_tau, double tau, float* x, float* final_diff, int* status) {
+    float* nx = (float*)malloc((size_t)(n > 0 ? n : 1) * sizeof(float));
+    int it = 0;
+    *status = ORC_MAX_ITER;
+    *final_diff = 0.0f;
+    while (it < max_iters) {
+        float dmax = 0.0f;
+        int nan_seen = 0;
+        for (int row = 0; row < n; ++row) {
+            float rx = 0.0f;
+            for (int o = r_rp[row]; o < r_rp[row + 1]; ++o) rx += x[r_ci[o]] * r_val[o];
+            const float next = (b[row] - rx) * dinv[row];
+            const float diff = fabsf(next - x[row]);
+            nx[row] = next;
+            if (diff > dmax) dmax = diff;
+            if (isnan(diff)) nan_seen = 1;
+        }
+        memcpy(x, nx, (size_t)n * sizeof(float));
+        ++it;
+        *final_diff = nan_seen ? NAN : dmax;   /* a NaN difference is reported as NaN */
+        if (nan_seen || isinf(dmax)) { *status = ORC_BREAKDOWN; break; }
+        if (use_tau && (double)dmax <= tau) { *status = ORC_CONVERGED; break; }
+    }
+    free(nx);
+    return it;
 }
 
 void orc_free(void* ptr) { free(ptr); }

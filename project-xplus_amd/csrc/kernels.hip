@@ -1090,6 +1090,148 @@ __global__ void k_fill(size_t n, T v, T* __restrict__ dst) {
 
 int blocks_for(size_t n, int bs) { return static_cast<int>((n + static_cast<size_t>(bs) - 1) / static_cast<size_t>(bs)); }
 
+
+// ---------------------------------------------------------------------------
+// Jacobi iteration x_new = D^-1 (b - R x), A = D + R
+// (DLC/Cuper-jacobi-iteration/host/main.cpp:957-1053, docs/jacobi_iteration.md:1-60).
+// FP32 throughout, as the reference's VALUE_TYPE float.  R x is summed per
+// row in CSR order from 0, one rounding per product and one per sum (no
+// fused multiply-add), skipping every entry on the diagonal: the SELL slots
+// after the first diagonal entry keep the CSR order, so each row's R x, and
+// with it x_new, is the reference's bit for bit.
+
+// NaN-propagating maximum: a NaN in b wins, a NaN in a is kept.
+__device__ __forceinline__ double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// Maximum of partial slot `slot` over a system's logical blocks; every
+// thread receives it; ends with a barrier.
+__device__ __forceinline__ double max_partials(const Launch& L, int sys, int slot) {
+    __shared__ double red[kWaves];
+    const int2 range = sys_range(L, sys);
+    double m = 0.0;
+    for (int b = range.x + static_cast<int>(threadIdx.x); b < range.y; b += kBlock)
+        m = nan_max(m, L.partials[static_cast<size_t>(slot) * L.grid + b]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0) red[wave] = m;
+    __syncthreads();
+    m = red[0];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) m = nan_max(m, red[k]);
+    __syncthreads();
+    return m;
+}
+
+// R x of one SELL row: the off-diagonal slots in CSR order, entries whose
+// column is the row itself (padding, duplicate diagonal entries) skipped.
+template <bool IDX16>
+__device__ __forceinline__ float sell_row_offdiag_seq(const SellView& A, int slice, int lane,
+                                                      const float* __restrict__ x) {
+    const int j0 = A.slice_ptr[slice];
+    const int w = A.slice_ptr[slice + 1] - j0;
+    const int row = slice * kC + lane;
+    const size_t e0 = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
+    const int base = A.slice_base[slice];
+    float acc = 0.0f;
+    if (w <= kFastW) {
+        // All loads first (slots past the width re-read slot 0, masked below).
+        float v[kFastW], xv[kFastW];
+        int c[kFastW];
+#pragma unroll
+        for (int k = 1; k < kFastW; ++k) {
+            const size_t e = e0 + static_cast<size_t>(k < w ? k : 0) * kC;
+            v[k] = A.val[e];
+            c[k] = k < w ? (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]) : row;
+        }
+#pragma unroll
+        for (int k = 1; k < kFastW; ++k) xv[k] = x[c[k]];
+#pragma unroll
+        for (int k = 1; k < kFastW; ++k)
+            acc = (k < w && c[k] != row) ? __fadd_rn(acc, __fmul_rn(v[k], xv[k])) : acc;
+        return acc;
+    }
+    size_t e = e0;
+    for (int j = 1; j < w; ++j) {
+        e += kC;
+        const int c = IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e];
+        if (c != row) acc = __fadd_rn(acc, __fmul_rn(A.val[e], x[c]));
+    }
+    return acc;
+}
+
+// One Jacobi update.  Reads state slot Sc, writes Sn.  First the stop test
+// of the previous update (its max |x_new - x_old| from the partials):
+// non-finite -> breakdown; use_tau and diff <= tau -> converged; then
+// iters == max_iters -> max_iter.  Update k reads x from buffer k % 2 and
+// writes buffer (k + 1) % 2; a stopped system's kernels are no-ops.
+template <bool IDX16>
+__global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A, const JacState* Sc, JacState* Sn,
+                                                         const float* __restrict__ b,
+                                                         const float* __restrict__ dinv, float* __restrict__ xa,
+                                                         float* __restrict__ xb) {
+    __shared__ JacState s_sh;
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    const bool leader_block = lb == sys_range(L, w.sys).x;
+    if (threadIdx.x == 0) s_sh = Sc[w.sys];
+    const double prev = max_partials(L, w.sys, 0);  // ends with a barrier
+    if (!s_sh.active) {
+        if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
+        return;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        JacState& s = s_sh;
+        if (s.pending) {
+            s.pending = 0;
+            s.diff = prev;
+            if (!isfinite(prev)) {
+                s.status = kStatusBreakdown;
+                s.active = 0;
+            } else if (s.use_tau && prev <= s.tau) {
+                s.status = kStatusConverged;
+                s.active = 0;
+            }
+        }
+        if (s.active && s.iters >= s.max_iters) {
+            s.status = kStatusMaxIter;
+            s.active = 0;
+        }
+        if (s.active) {
+            s.iters += 1;
+            s.pending = 1;
+        }
+        if (leader_block) Sn[w.sys] = s;
+    }
+    __syncthreads();
+    if (!s_sh.active) return;
+    const bool odd = ((s_sh.iters - 1) & 1) != 0;
+    const float* __restrict__ xo = odd ? xb : xa;
+    float* __restrict__ xn = odd ? xa : xb;
+    XP_WAVE_LANE
+    double m = 0.0;
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
+        const float rx = sell_row_offdiag_seq<IDX16>(A, sl, lane, xo);
+        const float next = __fmul_rn(__fsub_rn(b[row], rx), dinv[row]);
+        const float diff = fabsf(__fsub_rn(next, xo[row]));
+        xn[row] = next;
+        m = nan_max(m, static_cast<double>(diff));
+    }
+    __shared__ double red[kWaves];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+    if (lane == 0) red[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int k = 1; k < kWaves; ++k) t = nan_max(t, red[k]);
+        put_partial(L, 0, lb, t);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1408,6 +1550,15 @@ void launch_block_factor(int nblocks, const float* full, BlockPre* out, hipStrea
 void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_mask_padding_blocks, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad,
                        perm, blocks);
+}
+
+
+void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
+                        const float* b, const float* dinv, float* xa, float* xb) {
+    if (A.idx16)
+        XP_LAUNCH(L, k_jacobi_iter<true>, dim3(L.grid), L, A, st_cur, st_next, b, dinv, xa, xb);
+    else
+        XP_LAUNCH(L, k_jacobi_iter<false>, dim3(L.grid), L, A, st_cur, st_next, b, dinv, xa, xb);
 }
 
 }  // namespace xp

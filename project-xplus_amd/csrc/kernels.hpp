@@ -114,4 +114,11 @@ void launch_block_factor(int nblocks, const float* full, BlockPre* out, hipStrea
 // Identity rows/columns for the padding rows (perm < 0) of solve-order 8x8 blocks.
 void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s);
 
+// ---- Jacobi iteration (state machine in JacState, two slots) ----
+// x_new = D^-1 (b - R x) on the solve-order vectors; xa / xb alternate per
+// update (update k reads buffer k % 2); partial slot 0 of L holds each
+// workgroup's max |x_new - x_old|.
+void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
+                        const float* b, const float* dinv, float* xa, float* xb);
+
 }  // namespace xp

@@ -1203,6 +1203,309 @@ int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches) {
 
 }  // extern "C"
 
+// ---- Jacobi iteration ------------------------------------------------------------
+// x_new = D^-1 (b - R x), A = D + R, in FP32 on the device matrix
+// (DLC/Cuper-jacobi-iteration/host/main.cpp:957-1053).  One kernel per update
+// (k_jacobi_iter), captured in a HIP graph; the stop test of each update runs
+// in the next kernel, so a run has no host round trip until every system
+// has stopped.
+struct xp_jacobi {
+    xp_matrix* m = nullptr;
+    xp_jacobi_opts opts{};
+    hipStream_t stream = nullptr;
+    DevBuf<float> xa, xb, b, dinv, x0s;
+    DevBuf<xp::JacState> st;
+    DevBuf<double> partials;               // partial maxima, [grid]
+    std::vector<xp::JacState> st_init;
+    xp::JacState* st_host = nullptr;       // pinned readback
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_iters = 0;
+    int graph_version = -1;
+    int partials_version = -1;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int last_launches = 0;
+    bool have_rhs = false;
+
+    ~xp_jacobi() {
+        if (m != nullptr) --m->solvers;
+        if (exec != nullptr) (void)hipGraphExecDestroy(exec);
+        if (graph != nullptr) (void)hipGraphDestroy(graph);
+        if (ev0 != nullptr) (void)hipEventDestroy(ev0);
+        if (ev1 != nullptr) (void)hipEventDestroy(ev1);
+        if (st_host != nullptr) (void)hipHostFree(st_host);
+        if (stream != nullptr) (void)hipStreamDestroy(stream);
+    }
+
+    // The matrix's work map with this solver's own partials buffer.
+    xp::Launch launch() {
+        if (partials_version != m->layout_version) {
+            partials.alloc(static_cast<size_t>(m->grid));
+            HIPX(hipMemsetAsync(partials.p, 0, partials.n * sizeof(double), stream));
+            partials_version = m->layout_version;
+        }
+        xp::Launch L = m->launch(stream);
+        L.partials = partials.p;
+        return L;
+    }
+
+    // Update k reads state slot k % 2 and writes slot (k + 1) % 2; graphs
+    // hold an even number of updates, so slot 0 is current between launches.
+    void enqueue(int k, const xp::Launch& L) {
+        const size_t ns = static_cast<size_t>(m->nsys);
+        xp::launch_jacobi_iter(L, m->view(), st.p + (k % 2) * ns, st.p + ((k + 1) % 2) * ns, b.p, dinv.p, xa.p,
+                               xb.p);
+    }
+
+    void build_graph() {
+        if (exec != nullptr) { HIPX(hipGraphExecDestroy(exec)); exec = nullptr; }
+        if (graph != nullptr) { HIPX(hipGraphDestroy(graph)); graph = nullptr; }
+        const xp::Launch L = launch();   // any reallocation happens before the capture
+        HIPX(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < graph_iters; ++k) enqueue(k, L);
+        HIPX(hipStreamEndCapture(stream, &graph));
+        HIPX(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        graph_version = m->layout_version;
+    }
+
+    bool any_active() {
+        const size_t ns = static_cast<size_t>(m->nsys);
+        HIPX(hipMemcpyAsync(st_host, st.p, ns * sizeof(xp::JacState), hipMemcpyDeviceToHost, stream));
+        HIPX(hipStreamSynchronize(stream));
+        for (size_t k = 0; k < ns; ++k)
+            if (st_host[k].active) return true;
+        return false;
+    }
+
+    void run(const std::vector<xp::JacState>& init) {
+        if (opts.use_graph && (exec == nullptr || graph_version != m->layout_version)) build_graph();
+        HIPX(hipMemcpyAsync(xa.p, x0s.p, x0s.n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+        st.upload(init.data(), init.size(), stream);
+        HIPX(hipEventRecord(ev0, stream));
+        last_launches = 0;
+        while (any_active()) {
+            if (opts.use_graph) {
+                HIPX(hipGraphLaunch(exec, stream));
+            } else {
+                const xp::Launch L = launch();
+                for (int k = 0; k < graph_iters; ++k) enqueue(k, L);
+            }
+            ++last_launches;
+        }
+        HIPX(hipEventRecord(ev1, stream));
+        HIPX(hipEventSynchronize(ev1));
+        float ms = 0.0f;
+        HIPX(hipEventElapsedTime(&ms, ev0, ev1));
+        last_ms = ms;
+    }
+};
+
+namespace {
+
+std::unique_ptr<xp_jacobi> make_jacobi(xp_matrix* m, const xp_jacobi_opts* o) {
+    if (m == nullptr) throw std::runtime_error("null matrix");
+    // D = the sum of each row's diagonal entries in FP32, in CSR order, and
+    // Diag_inv = 1 / D (PrepareJacobiVectorsAndR, host/main.cpp:980-1005).
+    std::vector<float> dinv_nat(static_cast<size_t>(m->n), 1.0f);
+    size_t rp_off = 0, nz_off = 0;
+    for (int k = 0; k < m->nsys; ++k) {
+        const int nk = m->h_ns[static_cast<size_t>(k)];
+        const int* rp = m->h_row_ptr.data() + rp_off;
+        const int g0 = m->sys_nat_begin[static_cast<size_t>(k)];
+        for (int r = 0; r < nk; ++r) {
+            float d = 0.0f;
+            bool found = false;
+            for (int o = rp[r]; o < rp[r + 1]; ++o) {
+                if (m->h_col_idx[nz_off + static_cast<size_t>(o)] != r) continue;
+                d = d + static_cast<float>(m->h_values[nz_off + static_cast<size_t>(o)]);
+                found = true;
+            }
+            if (!found || d == 0.0f) throw std::runtime_error("missing or zero diagonal at row " + std::to_string(r));
+            dinv_nat[static_cast<size_t>(g0 + r)] = 1.0f / d;
+        }
+        rp_off += static_cast<size_t>(nk) + 1;
+        nz_off += static_cast<size_t>(rp[nk]);
+    }
+    auto s = std::make_unique<xp_jacobi>();
+    s->m = m;
+    ++m->solvers;
+    xp_jacobi_opts d{};
+    d.max_iters = 1;       // the reference's MAX_ITERS default (host/main.cpp:1626)
+    d.tau = 1.0e-5;        // the reference's TAU default (host/main.cpp:1627)
+    d.use_graph = 1;
+    if (o != nullptr) d = *o;
+    if (d.max_iters < 0) throw std::runtime_error("max_iters must be >= 0");
+    if (const char* e = std::getenv("XPLUS_NO_GRAPH")) {
+        if (e[0] == '1') d.use_graph = 0;
+    }
+    if (d.graph_iters <= 0) d.graph_iters = 32;
+    d.graph_iters += d.graph_iters % 2;
+    s->opts = d;
+    s->graph_iters = d.graph_iters;
+    HIPX(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIPX(hipEventCreate(&s->ev0));
+    HIPX(hipEventCreate(&s->ev1));
+    const size_t np = static_cast<size_t>(m->n_pad);
+    s->xa.alloc(np);
+    s->xb.alloc(np);
+    s->b.alloc(np);
+    s->x0s.alloc(np);
+    std::vector<float> dinv(np, 1.0f);       // padding rows: identity
+    for (size_t i = 0; i < np; ++i) {
+        const int g = m->perm[i];
+        if (g >= 0) dinv[i] = dinv_nat[static_cast<size_t>(g)];
+    }
+    s->dinv = to_device(dinv, s->stream);
+    HIPX(hipMemsetAsync(s->xb.p, 0, np * sizeof(float), s->stream));
+    s->st.alloc(2 * static_cast<size_t>(m->nsys));
+    s->st_init.assign(static_cast<size_t>(m->nsys), xp::JacState{});
+    for (auto& t : s->st_init) {
+        t.tau = d.tau;
+        t.use_tau = d.use_tau ? 1 : 0;
+        t.max_iters = d.max_iters;
+        t.status = xp::kStatusRunning;
+        t.active = 1;
+    }
+    HIPX(hipHostMalloc(reinterpret_cast<void**>(&s->st_host), static_cast<size_t>(m->nsys) * sizeof(xp::JacState)));
+    if (d.use_graph) s->build_graph();
+    HIPX(hipStreamSynchronize(s->stream));
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xp_jacobi_create(xp_matrix* m, const xp_jacobi_opts* opts, xp_jacobi** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null argument");
+        *out = nullptr;
+        *out = make_jacobi(m, opts).release();
+    });
+}
+
+void xp_jacobi_destroy(xp_jacobi* s) { delete s; }
+
+int xp_jacobi_set_rhs(xp_jacobi* s, const double* b, const double* x0) {
+    return guarded([&] {
+        if (s == nullptr || b == nullptr) throw std::runtime_error("null argument");
+        xp_matrix& m = *s->m;
+        const size_t n = static_cast<size_t>(m.n);
+        DevBuf<double> tmp(std::max<size_t>(n, 1));
+        tmp.upload(b, n, s->stream);
+        xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s->b.p, s->stream);
+        if (x0 != nullptr) {
+            tmp.upload(x0, n, s->stream);
+            xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s->x0s.p, s->stream);
+        } else {
+            HIPX(hipMemsetAsync(s->x0s.p, 0, s->x0s.n * sizeof(float), s->stream));
+        }
+        HIPX(hipStreamSynchronize(s->stream));
+        s->have_rhs = true;
+    });
+}
+
+int xp_jacobi_run(xp_jacobi* s) {
+    return guarded([&] {
+        if (s == nullptr || !s->have_rhs) throw std::runtime_error("xp_jacobi_run: call xp_jacobi_set_rhs first");
+        s->run(s->st_init);
+    });
+}
+
+int xp_jacobi_get_status(xp_jacobi* s, int sys, xp_jacobi_status* out) {
+    return guarded([&] {
+        if (s == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        if (sys < 0 || sys >= s->m->nsys) throw std::runtime_error("system index out of range");
+        xp::JacState t;
+        HIPX(hipMemcpyAsync(&t, s->st.p + sys, sizeof t, hipMemcpyDeviceToHost, s->stream));
+        HIPX(hipStreamSynchronize(s->stream));
+        out->status = t.status;
+        out->iterations = t.iters;
+        out->diff = t.diff;
+    });
+}
+
+int xp_jacobi_get_x(xp_jacobi* s, double* x) {
+    return guarded([&] {
+        if (s == nullptr || x == nullptr) throw std::runtime_error("null argument");
+        xp_matrix& m = *s->m;
+        const size_t ns = static_cast<size_t>(m.nsys), np = static_cast<size_t>(m.n_pad);
+        // each system's solution is in the buffer its own update count selects
+        std::vector<xp::JacState> st(ns);
+        HIPX(hipMemcpyAsync(st.data(), s->st.p, ns * sizeof(xp::JacState), hipMemcpyDeviceToHost, s->stream));
+        std::vector<float> xa(np), xb(np);
+        s->xa.download(xa.data(), np, s->stream);
+        s->xb.download(xb.data(), np, s->stream);
+        HIPX(hipStreamSynchronize(s->stream));
+        for (size_t k = 0; k < ns; ++k) {
+            const std::vector<float>& src = (st[k].iters & 1) ? xb : xa;
+            for (int i = m.sys_row_begin[k]; i < m.sys_row_begin[k + 1]; ++i) {
+                const int g = m.perm[static_cast<size_t>(i)];
+                if (g >= 0) x[g] = static_cast<double>(src[static_cast<size_t>(i)]);
+            }
+        }
+    });
+}
+
+int xp_jacobi_last_timing(xp_jacobi* s, double* ms, int* graph_launches) {
+    return guarded([&] {
+        if (s == nullptr) throw std::runtime_error("null argument");
+        if (ms != nullptr) *ms = s->last_ms;
+        if (graph_launches != nullptr) *graph_launches = s->last_launches;
+    });
+}
+
+// Average device time of k_jacobi_iter over `iters` updates launched
+// directly with start / end events, from x0, without the tau test.
+int xp_jacobi_kernel_time(xp_jacobi* s, int iters, double* us) {
+    return guarded([&] {
+        if (s == nullptr || us == nullptr || !s->have_rhs || iters <= 0) throw std::runtime_error("bad argument");
+        xp_jacobi& S = *s;
+        std::vector<hipEvent_t> ev(2 * static_cast<size_t>(iters));
+        for (auto& e : ev) HIPX(hipEventCreate(&e));
+        HIPX(hipMemcpyAsync(S.xa.p, S.x0s.p, S.x0s.n * sizeof(float), hipMemcpyDeviceToDevice, S.stream));
+        std::vector<xp::JacState> init = S.st_init;
+        for (auto& t : init) {
+            t.use_tau = 0;
+            t.max_iters = iters + 2;
+        }
+        S.st.upload(init.data(), init.size(), S.stream);
+        for (int k = 0; k < iters; ++k) {
+            xp::Launch L = S.launch();
+            L.ev_start = ev[2 * static_cast<size_t>(k)];
+            L.ev_stop = ev[2 * static_cast<size_t>(k) + 1];
+            S.enqueue(k, L);
+        }
+        HIPX(hipStreamSynchronize(S.stream));
+        double t = 0.0;
+        for (int k = 0; k < iters; ++k) {
+            float ms = 0.0f;
+            HIPX(hipEventElapsedTime(&ms, ev[2 * static_cast<size_t>(k)], ev[2 * static_cast<size_t>(k) + 1]));
+            t += static_cast<double>(ms) * 1000.0;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *us = t / iters;
+    });
+}
+
+// A * ones in FP32, one sequential sum per row in CSR order: the right-hand
+// side the reference builds when a dataset has no b (host/main.cpp:969-973).
+// Data preparation on the host; the iteration itself runs on the device.
+int xp_rhs_ones_f32(int n, const int* row_ptr, const int* col_idx, const double* values, double* b) {
+    return guarded([&] {
+        if (n < 0 || row_ptr == nullptr || b == nullptr) throw std::runtime_error("bad argument");
+        (void)col_idx;
+        for (int r = 0; r < n; ++r) {
+            float sum = 0.0f;
+            for (int o = row_ptr[r]; o < row_ptr[r + 1]; ++o) sum = sum + 1.0f * static_cast<float>(values[o]);
+            b[r] = static_cast<double>(sum);
+        }
+    });
+}
+
+}  // extern "C"
+
 // ---- FP64 residuals and refinement ----------------------------------------------
 namespace {
 

@@ -62,6 +62,20 @@ struct SysState {
     int fresh;        // 1 right after init (fused iteration: no previous step to finish)
 };
 
+// Per-system Jacobi iteration state (two slots alternating per update,
+// like SysState).  The stop test of update k runs at the start of the next
+// kernel, from the partial maxima update k wrote.
+struct JacState {
+    double diff;      // max_i |x_new_i - x_old_i| of the last tested update
+    double tau;       // with use_tau: converged when diff <= tau
+    int iters;        // updates done
+    int max_iters;
+    int status;       // kStatus* below
+    int active;       // 1 while iterating
+    int pending;      // the last update's diff has not been tested yet
+    int use_tau;      // 0: fixed count of max_iters updates
+};
+
 enum : int {
     kStatusConverged = 0,
     kStatusMaxIter = 1,

@@ -130,6 +130,26 @@ _p_true_res = _sig("xp_pcg_true_residual", C.c_int, [_vp, _f64p])
 _p_timing = _sig("xp_pcg_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int)])
 
 _p_set_precond = _sig("xp_pcg_set_preconditioner", C.c_int, [_vp, C.c_int, _vp])
+
+
+class JacobiOpts(C.Structure):
+    _fields_ = [("tau", C.c_double), ("max_iters", C.c_int), ("use_tau", C.c_int), ("graph_iters", C.c_int),
+                ("use_graph", C.c_int)]
+
+
+class JacobiStatus(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("diff", C.c_double)]
+
+
+_j_create = _sig("xp_jacobi_create", C.c_int, [_vp, C.POINTER(JacobiOpts), C.POINTER(_vp)])
+_j_destroy = _sig("xp_jacobi_destroy", None, [_vp])
+_j_set_rhs = _sig("xp_jacobi_set_rhs", C.c_int, [_vp, _f64p, _vp])
+_j_run = _sig("xp_jacobi_run", C.c_int, [_vp])
+_j_status = _sig("xp_jacobi_get_status", C.c_int, [_vp, C.c_int, C.POINTER(JacobiStatus)])
+_j_get_x = _sig("xp_jacobi_get_x", C.c_int, [_vp, _f64p])
+_j_timing = _sig("xp_jacobi_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int)])
+_j_kernel_time = _sig("xp_jacobi_kernel_time", C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)])
+_rhs_ones = _sig("xp_rhs_ones_f32", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _f64p])
 _p_set_mlp = _sig("xp_pcg_set_mlp_preconditioner", C.c_int, [_vp, _vp])
 _m_num_blocks = _sig("xp_matrix_num_blocks", C.c_int, [_vp])
 _m_block_features = _sig("xp_matrix_block_features", C.c_int, [_vp, _f32p, _f32p])
@@ -581,6 +601,69 @@ def csr_spmv_f64(a: Csr, x) -> np.ndarray:
 
 
 # ---------------------------------------------------------------- solver
+class Jacobi:
+    """Plain Jacobi iteration x_new = D^-1 (b - R x) in FP32 over a Matrix
+    (the reference's Cuper-jacobi-iteration host and kernel contract):
+    max_iters updates, or fewer with use_tau once max |x_new - x_old| <= tau."""
+
+    def __init__(self, matrix: "Matrix", max_iters: int = 1, tau: float = 1e-5, use_tau: bool = False,
+                 graph_iters: int = 0, use_graph: bool = True):
+        self.matrix = matrix
+        self.opts = JacobiOpts(tau, max_iters, int(use_tau), graph_iters, int(use_graph))
+        h = _vp()
+        _check(_j_create(matrix.h, C.byref(self.opts), C.byref(h)))
+        self.h = h
+        self.nsys = len(matrix.systems)
+        self.n = sum(a.n for a in matrix.systems)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _j_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_rhs(self, b, x0=None) -> None:
+        b = _f64(b)
+        if x0 is None:
+            _check(_j_set_rhs(self.h, b, None))
+        else:
+            x0 = _f64(x0)
+            _check(_j_set_rhs(self.h, b, x0.ctypes.data_as(_vp)))
+
+    def run(self) -> None:
+        _check(_j_run(self.h))
+
+    def status(self, sys: int = 0) -> dict:
+        s = JacobiStatus()
+        _check(_j_status(self.h, sys, C.byref(s)))
+        return {"status": s.status, "iterations": s.iterations, "diff": s.diff}
+
+    def x(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1))
+        _check(_j_get_x(self.h, out))
+        return out[:self.n]
+
+    def timing(self) -> tuple:
+        ms, launches = C.c_double(), C.c_int()
+        _check(_j_timing(self.h, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
+
+    def kernel_time(self, iters: int = 20) -> float:
+        us = C.c_double()
+        _check(_j_kernel_time(self.h, iters, C.byref(us)))
+        return us.value
+
+
+def rhs_ones_f32(a: Csr) -> np.ndarray:
+    """A * ones in FP32, row by row in CSR order: the reference's right-hand
+    side for a dataset without b (Cuper-jacobi-iteration host/main.cpp:969-973)."""
+    out = np.zeros(max(a.n, 1))
+    _check(_rhs_ones(a.n, a.row_ptr, a.col_idx, a.values, out))
+    return out[:a.n]
+
+
 class Pcg:
     """Graph-captured Jacobi-PCG over a Matrix (one or many systems)."""
 

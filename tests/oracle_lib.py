@@ -56,6 +56,10 @@ def _load_oracle():
                                             _f64p, C.POINTER(Trace), C.c_int, C.POINTER(C.c_int),
                                             C.POINTER(C.c_int), C.POINTER(C.c_int), _f64p, C.c_char_p, C.c_int]
     lib.orc_cuper_style_spmv.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+    _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    lib.orc_jacobi_prepare_f32.argtypes = [C.c_int, _i32p, _i32p, _f64p, _vp, _f32p, _i32p, _i32p, _f32p, _f32p]
+    lib.orc_jacobi_run_f32.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, C.c_double,
+                                       _f32p, C.POINTER(C.c_float), C.POINTER(C.c_int)]
     return lib
 
 
@@ -151,3 +155,35 @@ def multi_kernel(a, tau=1e-10, max_iters=0, b=None, x0=None):
         raise OracleError(err.value.decode())
     return {"x": x, "iterations": it.value, "status": stt.value, "effective_max_iters": emi.value,
             "final_rr": sc[0], "residual_l2": sc[1], "residual_rel": sc[2]}
+
+
+def jacobi_prepare(a, b=None):
+    """orc_jacobi_prepare_f32: dict(b, dinv, r_rp, r_ci, r_val) in FP32; b=None
+    builds A * ones.  Raises OracleError('missing or zero diagonal at row R')."""
+    n = a.n
+    bb = np.zeros(max(n, 1), np.float32)
+    dinv = np.zeros(max(n, 1), np.float32)
+    rrp = np.zeros(n + 1, np.int32)
+    rci = np.zeros(max(a.nnz, 1), np.int32)
+    rva = np.zeros(max(a.nnz, 1), np.float32)
+    bp = None if b is None else np.ascontiguousarray(b, np.float64)
+    bad = oracle.orc_jacobi_prepare_f32(n, a.row_ptr, a.col_idx, a.values,
+                                        None if bp is None else bp.ctypes.data_as(_vp), bb, rrp, rci, rva, dinv)
+    if bad >= 0:
+        raise OracleError(f"missing or zero diagonal at row {bad}")
+    return {"b": bb[:n], "dinv": dinv[:n], "r_rp": rrp, "r_ci": rci, "r_val": rva}
+
+
+def jacobi(a, b=None, x0=None, max_iters=1, use_tau=False, tau=1e-5):
+    """orc_jacobi_run_f32 after jacobi_prepare: dict(x (float32), iterations,
+    diff, status, b, dinv)."""
+    pr = jacobi_prepare(a, b)
+    x = np.zeros(max(a.n, 1), np.float32)
+    if x0 is not None:
+        x[:a.n] = np.asarray(x0, np.float64).astype(np.float32)
+    fd, st = C.c_float(), C.c_int()
+    bb = np.ascontiguousarray(np.r_[pr["b"], np.zeros(1, np.float32)])
+    dv = np.ascontiguousarray(np.r_[pr["dinv"], np.zeros(1, np.float32)])
+    it = oracle.orc_jacobi_run_f32(a.n, pr["r_rp"], pr["r_ci"], pr["r_val"], bb, dv, max_iters, int(use_tau), tau,
+                                   x, C.byref(fd), C.byref(st))
+    return {"x": x[:a.n], "iterations": it, "diff": fd.value, "status": st.value, "b": pr["b"], "dinv": pr["dinv"]}
