@@ -201,14 +201,18 @@ def learned_matrix(X, a, args, tune: bool):
     return ml, t
 
 
-def profile_run(X, a, args) -> int:
-    # a copy of known size first: 256 MiB read + 256 MiB written by the
-    # runtime's copy kernel, the scale for memory-traffic counters
+def calibration_copies(X) -> None:
+    """A copy of known size: 256 MiB read + 256 MiB written by the runtime's
+    copy kernel, three times -- the scale for memory-traffic counters."""
     src, dst = X.DeviceArray(1 << 26, np.float32), X.DeviceArray(1 << 26, np.float32)
     for _ in range(3):
         dst.copy_from(src)
     src.close()
     dst.close()
+
+
+def profile_run(X, a, args) -> int:
+    calibration_copies(X)
     # the standalone SpMV on the Jacobi matrix (the layout the autotuner picks
     # for this workload: sigma 64, 512 workgroups)
     m = X.Matrix(a, sigma=64, grid=512)
@@ -294,6 +298,56 @@ def workload_c4(X, args) -> int:
     return 0
 
 
+def workload_jacobi(X, args) -> int:
+    """Plain Jacobi iteration (the reference's Cuper-jacobi-iteration
+    contract: a fixed count of FP32 updates x = D^-1 (b - R x) from x0 = 0)
+    on a thermal2-sized 2D diffusion matrix (1108 x 1108 grid,
+    n = 1,227,664, 5-point, log-normal conductivity sigma 1) with
+    b = A x_ref (the dataset converter's smooth x_ref).  1000 updates per
+    step, graph-captured, matrix and vectors resident."""
+    if args.profile:
+        calibration_copies(X)
+    a = X.gen_diffusion2d(1108, 1108, 1.0, 20261203 + 1000 * 5)
+    b = a.to_scipy() @ X.reference_solution(a.n)      # CSR-order FP64 row sums, as the converter
+    m = X.Matrix(a)
+    info = m.info()
+    iters = 1000
+    s = X.Jacobi(m, max_iters=iters)
+    s.set_rhs(b)
+    ms, _ = timed(X, s.run, args.warmup, args.steps)
+    st = s.status()
+    us = s.kernel_time(200)
+    off = a.nnz - a.n
+    alg = (6 if info["idx16"] else 8) * off + 16 * a.n
+    r = roof("k_jacobi_iter", alg, us, ("6" if info["idx16"] else "8") + " (nnz - n) + 16 n (FP32 off-diagonal value + "
+             "column; x, b, 1/D read, x_new written)")
+    prof = profiled_traffic()
+    if r["kernel"] in prof:
+        r["traffic"], r["traffic_source"] = prof[r["kernel"]]
+    cpu = None
+    if not args.no_cpu and not args.profile:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib as O
+        k = 4
+        t0 = time.perf_counter()
+        O.jacobi(a, b=b, max_iters=k)
+        dt = time.perf_counter() - t0
+        k2 = max(k, min(400, int(k * 15.0 / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        O.jacobi(a, b=b, max_iters=k2)
+        dt2 = time.perf_counter() - t0
+        cpu = {"value": round(dt2 * 1000.0 / k2 * iters, 1), "unit": f"ms per {iters} updates (projected)",
+               "cores": 1, "kind": "port",
+               "sample": f"{k2} updates of the same system in {dt2:.2f} s (oracle FP32 Jacobi, one thread)"}
+    print(json.dumps({"workload": f"Jacobi iteration, {iters} FP32 updates, 2D diffusion 1108 x 1108 "
+                                  f"(n={a.n}, nnz={a.nnz})", "metric": "ms per 1000 Jacobi updates",
+                      "value": round(ms, 2), "unit": "ms", "higher_is_better": False,
+                      "updates_per_s": round(iters / (ms * 1e-3), 1), "iterations": st["iterations"],
+                      "final_diff": st["diff"], "graph_launches": s.timing()[1], "roofline": r,
+                      "cpu_baseline": cpu, "dtype": "f32", "data": "synthetic"}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,9 +362,9 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-grids", default="512,700,900,1100")
     ap.add_argument("--profile", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "jacobi"],
                     help="c2: the benchmark (default); c3: FP64-refined 1100/1600 FEM to 1e-10; "
-                         "c4: 256 batched 3D systems")
+                         "c4: 256 batched 3D systems; jacobi: 1000 plain Jacobi updates")
     ap.add_argument("--blocking", type=int, default=1, help="8-row blocks: 0 consecutive rows, 1 strength aggregates")
     # off-block weight of the training loss: on strength aggregates any
     # sizeable weight moves the blocks away from the exact inverses the
@@ -331,6 +385,8 @@ def main() -> int:
         return workload_c3(X, args)
     if args.workload == "c4":
         return workload_c4(X, args)
+    if args.workload == "jacobi":
+        return workload_jacobi(X, args)
     t0 = time.time()
     a = X.gen_fem2d(args.mesh, 1.0, 20, 1e3, 0.3, SEED)
     log(f"generated FEM mesh {args.mesh}: n={a.n} nnz={a.nnz} in {time.time() - t0:.1f}s")

@@ -1096,7 +1096,8 @@ int blocks_for(size_t n, int bs) { return static_cast<int>((n + static_cast<size
 // (DLC/Cuper-jacobi-iteration/host/main.cpp:957-1053, docs/jacobi_iteration.md:1-60).
 // FP32 throughout, as the reference's VALUE_TYPE float.  R x is summed per
 // row in CSR order from 0, one rounding per product and one per sum (no
-// fused multiply-add), skipping every entry on the diagonal: the SELL slots
+// fused multiply-add: contraction is switched off in the helpers below),
+// skipping every entry on the diagonal: the SELL slots
 // after the first diagonal entry keep the CSR order, so each row's R x, and
 // with it x_new, is the reference's bit for bit.
 
@@ -1123,6 +1124,24 @@ __device__ __forceinline__ double max_partials(const Launch& L, int sys, int slo
     return m;
 }
 
+// acc + v * x with two roundings.  The HIP default contracts a product and
+// a sum into one fused multiply-add (a single rounding); the pragma keeps
+// them apart, as the reference's host loop computes them.
+__device__ __forceinline__ float add_product_rn(float acc, float v, float x) {
+#pragma clang fp contract(off)
+    const float p = v * x;
+    return acc + p;
+}
+
+// next = (b - rx) * dinv and |next - x_old|, each operation rounded (no
+// product fused into the following difference).
+__device__ __forceinline__ float jacobi_next(float b, float rx, float dinv, float xold, float& diff) {
+#pragma clang fp contract(off)
+    const float next = (b - rx) * dinv;
+    diff = fabsf(next - xold);
+    return next;
+}
+
 // R x of one SELL row: the off-diagonal slots in CSR order, entries whose
 // column is the row itself (padding, duplicate diagonal entries) skipped.
 template <bool IDX16>
@@ -1135,12 +1154,15 @@ __device__ __forceinline__ float sell_row_offdiag_seq(const SellView& A, int sli
     const int base = A.slice_base[slice];
     float acc = 0.0f;
     if (w <= kFastW) {
-        // All loads first (slots past the width re-read slot 0, masked below).
+        // All loads first.  Slots past the width re-read the last real slot
+        // (slot 0, whose values are not otherwise needed, only when the
+        // slice has no off-diagonal column) and are masked below.
         float v[kFastW], xv[kFastW];
         int c[kFastW];
+        const int last = w > 1 ? w - 1 : 0;
 #pragma unroll
         for (int k = 1; k < kFastW; ++k) {
-            const size_t e = e0 + static_cast<size_t>(k < w ? k : 0) * kC;
+            const size_t e = e0 + static_cast<size_t>(k < w ? k : last) * kC;
             v[k] = A.val[e];
             c[k] = k < w ? (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]) : row;
         }
@@ -1148,14 +1170,14 @@ __device__ __forceinline__ float sell_row_offdiag_seq(const SellView& A, int sli
         for (int k = 1; k < kFastW; ++k) xv[k] = x[c[k]];
 #pragma unroll
         for (int k = 1; k < kFastW; ++k)
-            acc = (k < w && c[k] != row) ? __fadd_rn(acc, __fmul_rn(v[k], xv[k])) : acc;
+            acc = (k < w && c[k] != row) ? add_product_rn(acc, v[k], xv[k]) : acc;
         return acc;
     }
     size_t e = e0;
     for (int j = 1; j < w; ++j) {
         e += kC;
         const int c = IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e];
-        if (c != row) acc = __fadd_rn(acc, __fmul_rn(A.val[e], x[c]));
+        if (c != row) acc = add_product_rn(acc, A.val[e], x[c]);
     }
     return acc;
 }
@@ -1215,8 +1237,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A,
     for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         const int row = sl * kC + lane;
         const float rx = sell_row_offdiag_seq<IDX16>(A, sl, lane, xo);
-        const float next = __fmul_rn(__fsub_rn(b[row], rx), dinv[row]);
-        const float diff = fabsf(__fsub_rn(next, xo[row]));
+        float diff;
+        const float next = jacobi_next(b[row], rx, dinv[row], xo[row], diff);
         xn[row] = next;
         m = nan_max(m, static_cast<double>(diff));
     }

@@ -80,9 +80,9 @@ def spd_as_mtx(a, path, rng):
     lines = []
     for r, c, v in zip(A.row[lower] + 1, A.col[lower] + 1, A.data[lower]):
         if rng.random() < 0.33:
-            lines += [f"{r} {c} {v / 2!r}", f"{r} {c} {v / 2!r}"]
+            lines += [f"{r} {c} {float(v) / 2!r}", f"{r} {c} {float(v) / 2!r}"]
         else:
-            lines.append(f"{r} {c} {v!r}")
+            lines.append(f"{r} {c} {float(v)!r}")
         if rng.random() < 0.05:
             lines.append(f"{r} {c} 0.0")
     order = rng.permutation(len(lines))
