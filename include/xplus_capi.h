@@ -434,6 +434,25 @@ int xp_csr_from_arrays(int n, const int* row_ptr, const int* col_idx, const doub
 int xp_csr_info(const xp_csr* a, int* n, int* nnz);
 int xp_csr_copy(const xp_csr* a, int* row_ptr, int* col_idx, double* values, double* b, double* x0);
 void xp_csr_free(xp_csr* a);
+/* The reference's Cuper packed matrix format (host/cuper_control_matrix.hpp:
+ * 14-207 build_cuper_control_matrix; the same 64-bit word layout as
+ * DLC/Cuper/include/Cuper_common.h:535-774): 8192-column batches, rows dealt
+ * to 128 PEs, 16 channels of 8 lanes, word = local column << 50 | row code
+ * << 32 | FP32 value bits, empty slot 0x3FFFF << 32.  xp_cuper_pack
+ * reproduces the reference's packer word for word; xp_cuper_unpack turns
+ * any such matrix (16 channel buffers of channel_size words, the batch table
+ * sp_element_list_ptr) back into CSR (columns ascending, FP32 values) so it
+ * can be built into a device matrix.  At most 16,777,216 rows. */
+typedef struct xp_cuper_packed xp_cuper_packed;
+int xp_cuper_pack(int n, const int* row_ptr, const int* col_idx, const double* values, xp_cuper_packed** out);
+int xp_cuper_packed_info(const xp_cuper_packed* p, int* batch_count, int* matrix_len, int* channel_size);
+/* sp_element_list_ptr: batch_count + 1 entries; channels: 16 buffers of
+ * channel_size words each (the reference's matrix_fpga_data_[0..15]). */
+int xp_cuper_packed_copy(const xp_cuper_packed* p, int* sp_element_list_ptr, unsigned long long* const* channels);
+int xp_cuper_packed_create(int batch_count, int channel_size, const int* sp_element_list_ptr,
+                           const unsigned long long* const* channels, xp_cuper_packed** out);
+int xp_cuper_unpack(const xp_cuper_packed* p, int n, xp_csr** out);
+void xp_cuper_packed_free(xp_cuper_packed* p);
 /* Host-side SELL build for inspection and CPU tests (no GPU): returns the
  * arrays of the format described in the library's xp_types.hpp. */
 int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const double* values,

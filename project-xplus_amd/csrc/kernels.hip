@@ -1185,19 +1185,42 @@ __device__ __forceinline__ float sell_row_offdiag_seq(const SellView& A, int sli
 // One Jacobi update.  Reads state slot Sc, writes Sn.  First the stop test
 // of the previous update (its max |x_new - x_old| from the partials):
 // non-finite -> breakdown; use_tau and diff <= tau -> converged; then
-// iters == max_iters -> max_iter.  Update k reads x from buffer k % 2 and
-// writes buffer (k + 1) % 2; a stopped system's kernels are no-ops.
-template <bool IDX16>
+// iters == max_iters -> max_iter.  Update j runs in launch j and reads x
+// from buffer j % 2 (xo), writing buffer (j + 1) % 2 (xn): the caller passes
+// them by launch parity, so each wavefront reads and sums its first slice
+// before the stop test is known (only the writes wait for it).  A stopped
+// system's kernels are no-ops.
+template <bool IDX16, int AHEAD>
 __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A, const JacState* Sc, JacState* Sn,
                                                          const float* __restrict__ b,
-                                                         const float* __restrict__ dinv, float* __restrict__ xa,
-                                                         float* __restrict__ xb) {
+                                                         const float* __restrict__ dinv,
+                                                         const float* __restrict__ xo, float* __restrict__ xn) {
     __shared__ JacState s_sh;
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
     if (w.sys < 0) return;
     const bool leader_block = lb == sys_range(L, w.sys).x;
     if (threadIdx.x == 0) s_sh = Sc[w.sys];
+    XP_WAVE_LANE
+    // The wavefront's first AHEAD slices (all of them when the launcher can
+    // arrange it), ahead of the stop test: loads and R x only.  A missing
+    // slice reads the workgroup's last one instead (branch-free, so the
+    // loads of all of them are in flight together) and is not written.
+    // (Work items are never empty; the guard keeps an empty one from
+    // reading outside the matrix.)
+    constexpr int kAhead = AHEAD;
+    const int sl0 = w.slice_begin + wave;
+    float rxa[kAhead] = {}, ba[kAhead] = {}, da[kAhead] = {}, xa[kAhead] = {};
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+        if (w.slice_end <= w.slice_begin) break;
+        const int su = min(sl0 + u * kWaves, w.slice_end - 1);
+        const int row = su * kC + lane;
+        rxa[u] = sell_row_offdiag_seq<IDX16>(A, su, lane, xo);
+        ba[u] = b[row];
+        da[u] = dinv[row];
+        xa[u] = xo[row];
+    }
     const double prev = max_partials(L, w.sys, 0);  // ends with a barrier
     if (!s_sh.active) {
         if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
@@ -1229,17 +1252,20 @@ __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A,
     }
     __syncthreads();
     if (!s_sh.active) return;
-    const bool odd = ((s_sh.iters - 1) & 1) != 0;
-    const float* __restrict__ xo = odd ? xb : xa;
-    float* __restrict__ xn = odd ? xa : xb;
-    XP_WAVE_LANE
     double m = 0.0;
-    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+#pragma unroll
+    for (int u = 0; u < kAhead; ++u) {
+        if (sl0 + u * kWaves < w.slice_end) {
+            float diff;
+            xn[(sl0 + u * kWaves) * kC + lane] = jacobi_next(ba[u], rxa[u], da[u], xa[u], diff);
+            m = nan_max(m, static_cast<double>(diff));
+        }
+    }
+    for (int sl = sl0 + kAhead * kWaves; sl < w.slice_end; sl += kWaves) {
         const int row = sl * kC + lane;
         const float rx = sell_row_offdiag_seq<IDX16>(A, sl, lane, xo);
         float diff;
-        const float next = jacobi_next(b[row], rx, dinv[row], xo[row], diff);
-        xn[row] = next;
+        xn[row] = jacobi_next(b[row], rx, dinv[row], xo[row], diff);
         m = nan_max(m, static_cast<double>(diff));
     }
     __shared__ double red[kWaves];
@@ -1576,11 +1602,23 @@ void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipSt
 
 
 void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
-                        const float* b, const float* dinv, float* xa, float* xb) {
-    if (A.idx16)
-        XP_LAUNCH(L, k_jacobi_iter<true>, dim3(L.grid), L, A, st_cur, st_next, b, dinv, xa, xb);
-    else
-        XP_LAUNCH(L, k_jacobi_iter<false>, dim3(L.grid), L, A, st_cur, st_next, b, dinv, xa, xb);
+                        const float* b, const float* dinv, const float* x_old, float* x_new) {
+    // slices per wavefront of the largest work item, 2..4: every load of a
+    // wavefront's rows in flight at once when it fits
+    const int per_block = (A.nslices + L.grid - 1) / L.grid;
+    const int ahead = std::min(4, std::max(2, (per_block + kWaves - 1) / kWaves));
+#define XP_JACOBI_LAUNCH(I16, AH) \
+    XP_LAUNCH(L, (k_jacobi_iter<I16, AH>), dim3(L.grid), L, A, st_cur, st_next, b, dinv, x_old, x_new)
+    if (A.idx16) {
+        if (ahead == 2) XP_JACOBI_LAUNCH(true, 2);
+        else if (ahead == 3) XP_JACOBI_LAUNCH(true, 3);
+        else XP_JACOBI_LAUNCH(true, 4);
+    } else {
+        if (ahead == 2) XP_JACOBI_LAUNCH(false, 2);
+        else if (ahead == 3) XP_JACOBI_LAUNCH(false, 3);
+        else XP_JACOBI_LAUNCH(false, 4);
+    }
+#undef XP_JACOBI_LAUNCH
 }
 
 }  // namespace xp

@@ -115,10 +115,11 @@ void launch_block_factor(int nblocks, const float* full, BlockPre* out, hipStrea
 void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipStream_t s);
 
 // ---- Jacobi iteration (state machine in JacState, two slots) ----
-// x_new = D^-1 (b - R x) on the solve-order vectors; xa / xb alternate per
-// update (update k reads buffer k % 2); partial slot 0 of L holds each
-// workgroup's max |x_new - x_old|.
+// x_new = D^-1 (b - R x) on the solve-order vectors.  Launch j of a run
+// passes x_old = buffer j % 2 and x_new = buffer (j + 1) % 2; update j runs
+// in launch j, so a system stopped after k updates holds its x in buffer
+// k % 2.  Partial slot 0 of L holds each workgroup's max |x_new - x_old|.
 void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
-                        const float* b, const float* dinv, float* xa, float* xb);
+                        const float* b, const float* dinv, const float* x_old, float* x_new);
 
 }  // namespace xp

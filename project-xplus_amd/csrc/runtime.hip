@@ -21,6 +21,7 @@
 
 #include "../../include/xplus_capi.h"
 #include "csr_dataset.hpp"
+#include "cuper_format.hpp"
 #include "generators.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
@@ -1250,12 +1251,16 @@ struct xp_jacobi {
         return L;
     }
 
-    // Update k reads state slot k % 2 and writes slot (k + 1) % 2; graphs
-    // hold an even number of updates, so slot 0 is current between launches.
+    // Launch k reads state slot k % 2 and x buffer k % 2, writes slot and
+    // buffer (k + 1) % 2; graphs hold an even number of launches, so slot 0
+    // and buffer xa are current between graph launches.  Updates run in
+    // consecutive launches from the first, so a system that stopped after
+    // j updates holds its x in buffer j % 2.
     void enqueue(int k, const xp::Launch& L) {
         const size_t ns = static_cast<size_t>(m->nsys);
-        xp::launch_jacobi_iter(L, m->view(), st.p + (k % 2) * ns, st.p + ((k + 1) % 2) * ns, b.p, dinv.p, xa.p,
-                               xb.p);
+        const bool odd = (k % 2) != 0;
+        xp::launch_jacobi_iter(L, m->view(), st.p + (k % 2) * ns, st.p + ((k + 1) % 2) * ns, b.p, dinv.p,
+                               odd ? xb.p : xa.p, odd ? xa.p : xb.p);
     }
 
     void build_graph() {
@@ -1914,6 +1919,78 @@ int xp_dataset_load_cached(const char* dir, const char* cache_path, int* hit, xp
         *out = h.release();
     });
 }
+
+struct xp_cuper_packed {
+    xp::CuperPacked p;
+};
+
+int xp_cuper_pack(int n, const int* row_ptr, const int* col_idx, const double* values, xp_cuper_packed** out) {
+    return guarded([&] {
+        if (out == nullptr || row_ptr == nullptr || (n > 0 && (col_idx == nullptr || values == nullptr)))
+            throw std::runtime_error("null argument");
+        *out = nullptr;
+        auto h = std::make_unique<xp_cuper_packed>();
+        h->p = xp::cuper_pack(n, row_ptr, col_idx, values);
+        *out = h.release();
+    });
+}
+
+int xp_cuper_packed_info(const xp_cuper_packed* p, int* batch_count, int* matrix_len, int* channel_size) {
+    return guarded([&] {
+        if (p == nullptr) throw std::runtime_error("null argument");
+        if (batch_count != nullptr) *batch_count = p->p.batch_count;
+        if (matrix_len != nullptr) *matrix_len = p->p.matrix_len;
+        if (channel_size != nullptr) *channel_size = p->p.channel_size;
+    });
+}
+
+int xp_cuper_packed_copy(const xp_cuper_packed* p, int* sp_element_list_ptr, unsigned long long* const* channels) {
+    return guarded([&] {
+        if (p == nullptr || sp_element_list_ptr == nullptr || channels == nullptr)
+            throw std::runtime_error("null argument");
+        std::copy(p->p.ptr.begin(), p->p.ptr.end(), sp_element_list_ptr);
+        const size_t cs = static_cast<size_t>(p->p.channel_size);
+        for (int c = 0; c < xp::kCuperChannels; ++c) {
+            if (cs == 0) continue;
+            if (channels[c] == nullptr) throw std::runtime_error("null channel buffer");
+            std::memcpy(channels[c], p->p.data.data() + static_cast<size_t>(c) * cs, cs * sizeof(uint64_t));
+        }
+    });
+}
+
+int xp_cuper_packed_create(int batch_count, int channel_size, const int* sp_element_list_ptr,
+                           const unsigned long long* const* channels, xp_cuper_packed** out) {
+    return guarded([&] {
+        if (out == nullptr || sp_element_list_ptr == nullptr || (channel_size > 0 && channels == nullptr))
+            throw std::runtime_error("null argument");
+        if (batch_count < 0 || channel_size < 0) throw std::runtime_error("sizes must be non-negative");
+        *out = nullptr;
+        auto h = std::make_unique<xp_cuper_packed>();
+        h->p.batch_count = batch_count;
+        h->p.channel_size = channel_size;
+        h->p.ptr.assign(sp_element_list_ptr, sp_element_list_ptr + batch_count + 1);
+        h->p.matrix_len = h->p.ptr.back();
+        const size_t cs = static_cast<size_t>(channel_size);
+        h->p.data.resize(static_cast<size_t>(xp::kCuperChannels) * cs);
+        for (int c = 0; c < xp::kCuperChannels && cs > 0; ++c) {
+            if (channels[c] == nullptr) throw std::runtime_error("null channel buffer");
+            std::memcpy(h->p.data.data() + static_cast<size_t>(c) * cs, channels[c], cs * sizeof(uint64_t));
+        }
+        *out = h.release();
+    });
+}
+
+int xp_cuper_unpack(const xp_cuper_packed* p, int n, xp_csr** out) {
+    return guarded([&] {
+        if (p == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        *out = nullptr;
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::cuper_unpack(p->p, n);
+        *out = h.release();
+    });
+}
+
+void xp_cuper_packed_free(xp_cuper_packed* p) { delete p; }
 
 int xp_gen_diffusion2d(int nx, int ny, double sigma, unsigned long long seed, xp_csr** out) {
     return guarded([&] {

@@ -24,6 +24,7 @@ import numpy as np
 PKG_DIR = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_DIR / "lib" / "libxplus_amd.so"
 CLI_PATH = PKG_DIR / "lib" / "xplus_local"
+CUPER_CLI_PATH = PKG_DIR / "lib" / "xplus_cuper"
 
 if not LIB_PATH.exists():
     raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C project-xplus_amd)")
@@ -184,6 +185,13 @@ _csr_info = _sig("xp_csr_info", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c
 _csr_copy = _sig("xp_csr_copy", C.c_int, [_vp, _i32p, _i32p, _f64p, _f64p, _f64p])
 _csr_free = _sig("xp_csr_free", None, [_vp])
 
+_u64pp = C.POINTER(C.POINTER(C.c_ulonglong))
+_cu_pack = _sig("xp_cuper_pack", C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)])
+_cu_info = _sig("xp_cuper_packed_info", C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)])
+_cu_copy = _sig("xp_cuper_packed_copy", C.c_int, [_vp, _i32p, _u64pp])
+_cu_create = _sig("xp_cuper_packed_create", C.c_int, [C.c_int, C.c_int, _i32p, _u64pp, C.POINTER(_vp)])
+_cu_unpack = _sig("xp_cuper_unpack", C.c_int, [_vp, C.c_int, C.POINTER(_vp)])
+_cu_free = _sig("xp_cuper_packed_free", None, [_vp])
 
 class MtxInfo(C.Structure):
     _fields_ = [("nrows", C.c_longlong), ("ncols", C.c_longlong), ("nnz_reported", C.c_longlong),
@@ -411,6 +419,53 @@ def convert_mtx(mtx_path, out_dir, b_path=None, subset_n=None, name=None, source
         lines += ["subset_type=leading_principal_submatrix", f"subset_n={subset_n}"]
     (out_dir / "meta.txt").write_text("\n".join(lines) + "\n", encoding="utf-8")
     return meta
+
+
+class CuperPacked:
+    """The reference's Cuper packed matrix (host/cuper_control_matrix.hpp):
+    ptr = sp_element_list_ptr [batch_count + 1], data = uint64 [16, channel_size]
+    (the 16 matrix_fpga_data channel buffers)."""
+
+    def __init__(self, ptr, data):
+        self.ptr = np.ascontiguousarray(ptr, np.int32)
+        self.data = np.ascontiguousarray(data, np.uint64)
+        self.batch_count = len(self.ptr) - 1
+        self.channel_size = self.data.shape[1]
+
+    @property
+    def matrix_len(self) -> int:
+        return int(self.ptr[-1])
+
+    def _rows(self):
+        rows = (C.POINTER(C.c_ulonglong) * 16)()
+        for c in range(16):
+            rows[c] = self.data[c].ctypes.data_as(C.POINTER(C.c_ulonglong))
+        return rows
+
+    def unpack(self, n: int) -> "Csr":
+        h = _vp()
+        _check(_cu_create(self.batch_count, self.channel_size, self.ptr, self._rows(), C.byref(h)))
+        try:
+            out = _vp()
+            _check(_cu_unpack(h, n, C.byref(out)))
+            return Csr._from_handle(out)
+        finally:
+            _cu_free(h)
+
+
+def cuper_pack(a: Csr) -> CuperPacked:
+    h = _vp()
+    _check(_cu_pack(a.n, a.row_ptr, a.col_idx, a.values, C.byref(h)))
+    try:
+        bc, ml, cs = C.c_int(), C.c_int(), C.c_int()
+        _check(_cu_info(h, C.byref(bc), C.byref(ml), C.byref(cs)))
+        ptr = np.zeros(bc.value + 1, np.int32)
+        data = np.zeros((16, max(cs.value, 0)), np.uint64)
+        p = CuperPacked(ptr, data)
+        _check(_cu_copy(h, p.ptr, p._rows()))
+        return p
+    finally:
+        _cu_free(h)
 
 
 def gen_diffusion2d(nx: int, ny: int, sigma: float = 1.0, seed: int = 0) -> Csr:

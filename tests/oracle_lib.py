@@ -76,6 +76,12 @@ if REF_SO.exists():
     ref.ref_multi_kernel.argtypes = [C.c_char_p, C.c_double, C.c_int, _f64p, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), _f64p, C.c_char_p, C.c_int]
     ref.ref_cuper_style_spmv.argtypes = [C.c_char_p, _f64p, _f64p, C.c_char_p, C.c_int]
+    if hasattr(ref, "ref_cuper_pack_info"):
+        ref.ref_cuper_pack_info.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.c_char_p, C.c_int]
+        ref.ref_cuper_pack_copy.argtypes = [C.c_char_p, _i32p,
+                                            np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"),
+                                            C.c_char_p, C.c_int]
     for name in ("spmv_csr_kernel", "spmv_blocked_kernel"):
         getattr(ref, name).argtypes = [_i32p, _i32p, _f64p, _f64p, _f64p, C.c_int]
     ref.init_pcg_kernel.argtypes = [_f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, C.c_int]
@@ -187,3 +193,17 @@ def jacobi(a, b=None, x0=None, max_iters=1, use_tau=False, tau=1e-5):
     it = oracle.orc_jacobi_run_f32(a.n, pr["r_rp"], pr["r_ci"], pr["r_val"], bb, dv, max_iters, int(use_tau), tau,
                                    x, C.byref(fd), C.byref(st))
     return {"x": x[:a.n], "iterations": it, "diff": fd.value, "status": st.value, "b": pr["b"], "dinv": pr["dinv"]}
+
+
+def ref_cuper_pack(directory):
+    """The reference's own build_cuper_control_matrix over a dataset
+    directory: (ptr int32 [batches + 1], data uint64 [16, channel_size])."""
+    bc, ml, cs = C.c_int(), C.c_int(), C.c_int()
+    err = C.create_string_buffer(512)
+    if ref.ref_cuper_pack_info(str(directory).encode(), C.byref(bc), C.byref(ml), C.byref(cs), err, 512) != 0:
+        raise OracleError(err.value.decode())
+    ptr = np.zeros(bc.value + 1, np.int32)
+    data = np.zeros((16, max(cs.value, 1)), np.uint64)
+    if ref.ref_cuper_pack_copy(str(directory).encode(), ptr, data, err, 512) != 0:
+        raise OracleError(err.value.decode())
+    return ptr, data[:, :cs.value]
