@@ -185,6 +185,7 @@ int device_pcg(const Cli& cli, const xp::CsrDataset& d) {
     const auto t0 = std::chrono::steady_clock::now();
     xp_matrix* m = nullptr;
     xp_matrix_opts mo{};
+    mo.keep_lo = 1;  // hi + lo values: the true residual in FP64
     check(xp_matrix_create(d.n(), d.row_ptr().data(), d.col_idx().data(), d.values().data(), &mo, &m));
     xp_solve_opts so{};
     so.tau = cli.tau;
