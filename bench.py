@@ -314,9 +314,19 @@ def workload_jacobi(X, args) -> int:
     iters = 1000
     s = X.Jacobi(m, max_iters=iters)
     s.set_rhs(b)
+    sweep = {}
+    if args.grid_sweep:
+        for g in (256, 512, 1024, 2048, 4096):
+            m.set_grid(g)
+            sweep[g] = round(s.kernel_time(100), 3)
+            log(f"jacobi grid {g}: {sweep[g]} us")
+        m.set_grid(min(sweep, key=sweep.get) if args.grid == 0 else args.grid)
+    elif args.grid:
+        m.set_grid(args.grid)
     ms, _ = timed(X, s.run, args.warmup, args.steps)
     st = s.status()
     us = s.kernel_time(200)
+    spmv_us = m.spmv_f32_time(20, 200)     # the plain FP32 SpMV of the same matrix, for scale
     off = a.nnz - a.n
     alg = (6 if info["idx16"] else 8) * off + 16 * a.n
     r = roof("k_jacobi_iter", alg, us, ("6" if info["idx16"] else "8") + " (nnz - n) + 16 n (FP32 off-diagonal value + "
@@ -343,7 +353,8 @@ def workload_jacobi(X, args) -> int:
                                   f"(n={a.n}, nnz={a.nnz})", "metric": "ms per 1000 Jacobi updates",
                       "value": round(ms, 2), "unit": "ms", "higher_is_better": False,
                       "updates_per_s": round(iters / (ms * 1e-3), 1), "iterations": st["iterations"],
-                      "final_diff": st["diff"], "graph_launches": s.timing()[1], "roofline": r,
+                      "final_diff": st["diff"], "graph_launches": s.timing()[1], "grid": m.info()["grid"],
+                      "grid_sweep_us": sweep, "spmv_f32_us_same_matrix": round(spmv_us, 3), "roofline": r,
                       "cpu_baseline": cpu, "dtype": "f32", "data": "synthetic"}), flush=True)
     return 0
 
@@ -362,6 +373,7 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-grids", default="512,700,900,1100")
     ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--grid-sweep", action="store_true", help="jacobi: time the update kernel at several grids")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "jacobi"],
                     help="c2: the benchmark (default); c3: FP64-refined 1100/1600 FEM to 1e-10; "
                          "c4: 256 batched 3D systems; jacobi: 1000 plain Jacobi updates")

@@ -1182,15 +1182,69 @@ __device__ __forceinline__ float sell_row_offdiag_seq(const SellView& A, int sli
     return acc;
 }
 
+// R x for U slices of one wavefront at once (its slices sl_first,
+// sl_first + kWaves, ...), every slice at most FW columns wide: all value,
+// column and x loads of the U slices are issued before the first sum.
+// Slices past sl_last re-read sl_last (not written by the caller); slots
+// past a slice's width re-read its last slot (slot 0 when it has no
+// off-diagonal column) and are masked, as are entries on the diagonal.
+template <bool IDX16, int FW, int U>
+__device__ __forceinline__ void jacobi_group(const SellView& A, int sl_first, int sl_last, int lane,
+                                             const float* __restrict__ x, const float* __restrict__ b,
+                                             const float* __restrict__ dinv, float (&rx)[U], float (&bv)[U],
+                                             float (&dv)[U], float (&xr)[U]) {
+    int row[U], w[U], base[U], last[U];
+    size_t e0[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int s = min(sl_first + u * kWaves, sl_last);
+        const int j0 = A.slice_ptr[s];
+        w[u] = A.slice_ptr[s + 1] - j0;
+        row[u] = s * kC + lane;
+        e0[u] = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
+        base[u] = A.slice_base[s];
+        last[u] = w[u] > 1 ? w[u] - 1 : 0;
+    }
+    float v[U][FW];
+    int c[U][FW];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int k = 1; k < FW; ++k) {
+            const size_t e = e0[u] + static_cast<size_t>(k < w[u] ? k : last[u]) * kC;
+            v[u][k] = A.val[e];
+            c[u][k] = k < w[u] ? (IDX16 ? base[u] + static_cast<int>(A.col16[e]) : A.col32[e]) : row[u];
+        }
+        bv[u] = b[row[u]];
+        dv[u] = dinv[row[u]];
+        xr[u] = x[row[u]];
+    }
+    float xg[U][FW];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int k = 1; k < FW; ++k) xg[u][k] = x[c[u][k]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 1; k < FW; ++k)
+            acc = (k < w[u] && c[u][k] != row[u]) ? add_product_rn(acc, v[u][k], xg[u][k]) : acc;
+        rx[u] = acc;
+    }
+}
+
 // One Jacobi update.  Reads state slot Sc, writes Sn.  First the stop test
 // of the previous update (its max |x_new - x_old| from the partials):
 // non-finite -> breakdown; use_tau and diff <= tau -> converged; then
 // iters == max_iters -> max_iter.  Update j runs in launch j and reads x
 // from buffer j % 2 (xo), writing buffer (j + 1) % 2 (xn): the caller passes
-// them by launch parity, so each wavefront reads and sums its first slice
-// before the stop test is known (only the writes wait for it).  A stopped
-// system's kernels are no-ops.
-template <bool IDX16, int AHEAD>
+// them by launch parity, so each wavefront loads and sums its first U
+// slices before the stop test is known (only the writes wait for it).  A
+// stopped system's kernels are no-ops.  FW > 0: every slice is at most FW
+// columns wide and U slices go through jacobi_group together; FW = 0: any
+// width, one slice at a time.
+template <bool IDX16, int FW, int U>
 __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A, const JacState* Sc, JacState* Sn,
                                                          const float* __restrict__ b,
                                                          const float* __restrict__ dinv,
@@ -1202,24 +1256,23 @@ __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A,
     const bool leader_block = lb == sys_range(L, w.sys).x;
     if (threadIdx.x == 0) s_sh = Sc[w.sys];
     XP_WAVE_LANE
-    // The wavefront's first AHEAD slices (all of them when the launcher can
-    // arrange it), ahead of the stop test: loads and R x only.  A missing
-    // slice reads the workgroup's last one instead (branch-free, so the
-    // loads of all of them are in flight together) and is not written.
-    // (Work items are never empty; the guard keeps an empty one from
-    // reading outside the matrix.)
-    constexpr int kAhead = AHEAD;
     const int sl0 = w.slice_begin + wave;
-    float rxa[kAhead] = {}, ba[kAhead] = {}, da[kAhead] = {}, xa[kAhead] = {};
-#pragma unroll
-    for (int u = 0; u < kAhead; ++u) {
-        if (w.slice_end <= w.slice_begin) break;
-        const int su = min(sl0 + u * kWaves, w.slice_end - 1);
-        const int row = su * kC + lane;
-        rxa[u] = sell_row_offdiag_seq<IDX16>(A, su, lane, xo);
-        ba[u] = b[row];
-        da[u] = dinv[row];
-        xa[u] = xo[row];
+    // Work items are never empty; the guard keeps an empty one from
+    // reading outside the matrix.
+    const bool any = w.slice_end > w.slice_begin;
+    const int sl_last = w.slice_end - 1;
+    float rx[U] = {}, bv[U] = {}, dv[U] = {}, xr[U] = {};
+    if (any) {
+        if (FW > 0) {
+            jacobi_group<IDX16, (FW > 0 ? FW : 1), U>(A, sl0, sl_last, lane, xo, b, dinv, rx, bv, dv, xr);
+        } else {
+            const int s = min(sl0, sl_last);
+            const int row = s * kC + lane;
+            rx[0] = sell_row_offdiag_seq<IDX16>(A, s, lane, xo);
+            bv[0] = b[row];
+            dv[0] = dinv[row];
+            xr[0] = xo[row];
+        }
     }
     const double prev = max_partials(L, w.sys, 0);  // ends with a barrier
     if (!s_sh.active) {
@@ -1252,21 +1305,29 @@ __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A,
     }
     __syncthreads();
     if (!s_sh.active) return;
+    constexpr int kStep = (FW > 0 ? U : 1) * kWaves;
     double m = 0.0;
-#pragma unroll
-    for (int u = 0; u < kAhead; ++u) {
-        if (sl0 + u * kWaves < w.slice_end) {
-            float diff;
-            xn[(sl0 + u * kWaves) * kC + lane] = jacobi_next(ba[u], rxa[u], da[u], xa[u], diff);
-            m = nan_max(m, static_cast<double>(diff));
+    for (int sl = sl0; sl < w.slice_end; sl += kStep) {
+        if (sl != sl0) {   // the first group was loaded ahead of the stop test
+            if (FW > 0) {
+                jacobi_group<IDX16, (FW > 0 ? FW : 1), U>(A, sl, sl_last, lane, xo, b, dinv, rx, bv, dv, xr);
+            } else {
+                const int row = sl * kC + lane;
+                rx[0] = sell_row_offdiag_seq<IDX16>(A, sl, lane, xo);
+                bv[0] = b[row];
+                dv[0] = dinv[row];
+                xr[0] = xo[row];
+            }
         }
-    }
-    for (int sl = sl0 + kAhead * kWaves; sl < w.slice_end; sl += kWaves) {
-        const int row = sl * kC + lane;
-        const float rx = sell_row_offdiag_seq<IDX16>(A, sl, lane, xo);
-        float diff;
-        xn[row] = jacobi_next(b[row], rx, dinv[row], xo[row], diff);
-        m = nan_max(m, static_cast<double>(diff));
+#pragma unroll
+        for (int u = 0; u < (FW > 0 ? U : 1); ++u) {
+            const int su = sl + u * kWaves;
+            if (su < w.slice_end) {
+                float diff;
+                xn[su * kC + lane] = jacobi_next(bv[u], rx[u], dv[u], xr[u], diff);
+                m = nan_max(m, static_cast<double>(diff));
+            }
+        }
     }
     __shared__ double red[kWaves];
 #pragma unroll
@@ -1603,21 +1664,22 @@ void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipSt
 
 void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
                         const float* b, const float* dinv, const float* x_old, float* x_new) {
-    // slices per wavefront of the largest work item, 2..4: every load of a
-    // wavefront's rows in flight at once when it fits
-    const int per_block = (A.nslices + L.grid - 1) / L.grid;
-    const int ahead = std::min(4, std::max(2, (per_block + kWaves - 1) / kWaves));
-#define XP_JACOBI_LAUNCH(I16, AH) \
-    XP_LAUNCH(L, (k_jacobi_iter<I16, AH>), dim3(L.grid), L, A, st_cur, st_next, b, dinv, x_old, x_new)
-    if (A.idx16) {
-        if (ahead == 2) XP_JACOBI_LAUNCH(true, 2);
-        else if (ahead == 3) XP_JACOBI_LAUNCH(true, 3);
-        else XP_JACOBI_LAUNCH(true, 4);
-    } else {
-        if (ahead == 2) XP_JACOBI_LAUNCH(false, 2);
-        else if (ahead == 3) XP_JACOBI_LAUNCH(false, 3);
-        else XP_JACOBI_LAUNCH(false, 4);
-    }
+    // Slices up to 5 columns (2D 5-point stencils) go two at a time (16-bit
+    // columns; one with 32-bit columns); up to
+    // 8 or 10 columns one at a time with all loads ahead (more would not
+    // fit the 64 registers of full occupancy); wider ones through the loop.
+#define XP_JACOBI_LAUNCH(I16, FW, U) \
+    XP_LAUNCH(L, (k_jacobi_iter<I16, FW, U>), dim3(L.grid), L, A, st_cur, st_next, b, dinv, x_old, x_new)
+#define XP_JACOBI_WIDTHS(I16)                                  \
+    do {                                                       \
+        if (A.max_width <= 5) XP_JACOBI_LAUNCH(I16, 5, (I16 ? 2 : 1)); \
+        else if (A.max_width <= 8) XP_JACOBI_LAUNCH(I16, 8, 1); \
+        else if (A.max_width <= 10) XP_JACOBI_LAUNCH(I16, 10, 1); \
+        else XP_JACOBI_LAUNCH(I16, 0, 1);                      \
+    } while (0)
+    if (A.idx16) XP_JACOBI_WIDTHS(true);
+    else XP_JACOBI_WIDTHS(false);
+#undef XP_JACOBI_WIDTHS
 #undef XP_JACOBI_LAUNCH
 }
 

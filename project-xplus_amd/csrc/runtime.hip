@@ -138,6 +138,7 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 // ---- matrices ----------------------------------------------------------------
 struct xp_matrix {
     int n = 0, n_pad = 0, nslices = 0, nsys = 0, sigma = 0, idx16 = 0, variant = 0;
+    int max_width = 0;                // widest slice in columns
     int solvers = 0;                  // live xp_pcg objects built on this matrix
     int layout_version = 0;           // bumped when the work map / partials are reallocated
     int block_rows = 1;               // 8: blocks of up to 8 rows kept together as 8 solve rows
@@ -175,6 +176,7 @@ struct xp_matrix {
         v.nsys = nsys;
         v.idx16 = idx16;
         v.variant = variant;
+        v.max_width = max_width;
         v.slice_ptr = d_slice_ptr.p;
         v.slice_base = d_slice_base.p;
         v.val = d_val.p;
@@ -270,6 +272,9 @@ void load_sell(xp_matrix& m, int sigma) {
     m.n = h.n;
     m.n_pad = h.n_pad;
     m.nslices = h.nslices;
+    m.max_width = 0;
+    for (int sl = 0; sl < h.nslices; ++sl)
+        m.max_width = std::max(m.max_width, h.slice_ptr[static_cast<size_t>(sl) + 1] - h.slice_ptr[static_cast<size_t>(sl)]);
     m.nsys = h.nsys;
     m.sigma = h.sigma;
     m.idx16 = h.idx16;

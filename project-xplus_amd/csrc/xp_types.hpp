@@ -111,6 +111,7 @@ struct SellView {
     int nsys;
     int idx16;               // 1: col16 deltas from slice_base; 0: col32 absolute
     int variant;             // SpMV inner loop: 0 rolled loop, 1 fixed-width unrolled
+    int max_width;           // widest slice, in columns (diagonal included)
     const int* slice_ptr;    // [nslices + 1], in units of 64-entry columns
     const int* slice_base;   // [nslices]
     const float* val;        // [slice_ptr[nslices] * 64], diagonal first
