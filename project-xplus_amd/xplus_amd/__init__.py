@@ -25,6 +25,7 @@ PKG_DIR = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_DIR / "lib" / "libxplus_amd.so"
 CLI_PATH = PKG_DIR / "lib" / "xplus_local"
 CUPER_CLI_PATH = PKG_DIR / "lib" / "xplus_cuper"
+DEVICE_CLI_PATH = PKG_DIR / "lib" / "xplus_device"
 
 if not LIB_PATH.exists():
     raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C project-xplus_amd)")
