@@ -300,7 +300,8 @@ def workload_c4(X, args) -> int:
 
 def workload_jacobi(X, args) -> int:
     """Plain Jacobi iteration (the reference's Cuper-jacobi-iteration
-    contract: a fixed count of FP32 updates x = D^-1 (b - R x) from x0 = 0)
+    contract: a fixed count of FP32 updates x = D^-1 (b - R x) from x0 = 0,
+    one k_jacobi_sweep launch per update, no stop test)
     on a thermal2-sized 2D diffusion matrix (1108 x 1108 grid,
     n = 1,227,664, 5-point, log-normal conductivity sigma 1) with
     b = A x_ref (the dataset converter's smooth x_ref).  1000 updates per
@@ -329,7 +330,7 @@ def workload_jacobi(X, args) -> int:
     spmv_us = m.spmv_f32_time(20, 200)     # the plain FP32 SpMV of the same matrix, for scale
     off = a.nnz - a.n
     alg = (6 if info["idx16"] else 8) * off + 16 * a.n
-    r = roof("k_jacobi_iter", alg, us, ("6" if info["idx16"] else "8") + " (nnz - n) + 16 n (FP32 off-diagonal value + "
+    r = roof("k_jacobi_sweep", alg, us, ("6" if info["idx16"] else "8") + " (nnz - n) + 16 n (FP32 off-diagonal value + "
              "column; x, b, 1/D read, x_new written)")
     prof = profiled_traffic()
     if r["kernel"] in prof:

@@ -348,10 +348,12 @@ int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale);
  * reference's VALUE_TYPE: D is the FP32 sum of each row's diagonal entries
  * in CSR order, Diag_inv = 1 / D, R x summed per row in CSR order with one
  * rounding per product and per sum, so every update is the reference's bit
- * for bit.  max_iters updates (the reference's fixed-count contract); with
- * use_tau, the iteration also stops once max_i |x_new - x_old| <= tau (the
- * stop rule of docs/jacobi_iteration.md:99-124).  A non-finite difference
- * stops it as breakdown.  Error "missing or zero diagonal at row R".
+ * for bit.  Without use_tau: exactly max_iters updates with no stop test
+ * (the reference's fixed-count contract), then status breakdown if the last
+ * update's difference is not finite, else max_iter.  With use_tau the
+ * iteration also stops once max_i |x_new - x_old| <= tau (the stop rule of
+ * docs/jacobi_iteration.md:99-124), and at the first non-finite difference
+ * as breakdown.  Error "missing or zero diagonal at row R".
  * ===================================================================== */
 typedef struct xp_jacobi xp_jacobi;
 typedef struct xp_jacobi_opts {

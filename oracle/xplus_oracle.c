@@ -541,9 +541,11 @@ int orc_jacobi_prepare_f32(int n, const int* row_ptr, const int* col_idx, const 
 
 /* RunJacobiCpu (host/main.cpp:1011-1053): every update forms x_next from the
  * whole old x (R x in float, CSR order, from 0), next = (b - rx) * Diag_inv,
- * diff_max = max |next - x|, then swaps.  max_iters updates; with use_tau
- * the loop also ends once diff_max <= tau (docs/jacobi_iteration.md:99-124),
- * and a NaN or infinite difference ends it as breakdown (:148-154).
+ * diff_max = max |next - x|, then swaps.  Without use_tau exactly max_iters
+ * updates (the reference's fixed count), status breakdown when the last
+ * difference is NaN or infinite, else max_iter.  With use_tau the loop also
+ * ends once diff_max <= tau (docs/jacobi_iteration.md:99-124), and a NaN or
+ * infinite difference ends it as breakdown (:148-154).
  * x holds x0 on entry and the result on return; returns the update count. */
 int orc_jacobi_run_f32(int n, const int* r_rp, const int* r_ci, const float* r_val, const float* b, const float* dinv,
                        int max_iters, int use_tau, double tau, float* x, float* final_diff, int* status) {
@@ -566,9 +568,11 @@ int orc_jacobi_run_f32(int n, const int* r_rp, const int* r_ci, const float* r_v
         memcpy(x, nx, (size_t)n * sizeof(float));
         ++it;
         *final_diff = nan_seen ? NAN : dmax;   /* a NaN difference is reported as NaN */
+        if (!use_tau) continue;                /* fixed count: every update runs */
         if (nan_seen || isinf(dmax)) { *status = ORC_BREAKDOWN; break; }
-        if (use_tau && (double)dmax <= tau) { *status = ORC_CONVERGED; break; }
+        if ((double)dmax <= tau) { *status = ORC_CONVERGED; break; }
     }
+    if (!use_tau && it > 0 && !isfinite(*final_diff)) *status = ORC_BREAKDOWN;
     free(nx);
     return it;
 }

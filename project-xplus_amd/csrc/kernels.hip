@@ -1341,6 +1341,73 @@ __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A,
     }
 }
 
+
+// One Jacobi update of the fixed-count contract (host/main.cpp:1011-1053:
+// max_iters updates, no stop test): no state, no wait on other
+// workgroups; each workgroup leaves its max |x_new - x_old| in partial
+// slot 0, which k_jacobi_finish reads after the last update.  Buffers by
+// launch parity, as k_jacobi_iter.
+template <bool IDX16, int FW, int U>
+__global__ __launch_bounds__(kBlock, 8) void k_jacobi_sweep(Launch L, SellView A, const float* __restrict__ b,
+                                                          const float* __restrict__ dinv,
+                                                          const float* __restrict__ xo, float* __restrict__ xn) {
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    const int sl_last = w.slice_end - 1;
+    constexpr int kStep = (FW > 0 ? U : 1) * kWaves;
+    double m = 0.0;
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kStep) {
+        float rx[U] = {}, bv[U] = {}, dv[U] = {}, xr[U] = {};
+        if (FW > 0) {
+            jacobi_group<IDX16, (FW > 0 ? FW : 1), U>(A, sl, sl_last, lane, xo, b, dinv, rx, bv, dv, xr);
+        } else {
+            const int row = sl * kC + lane;
+            rx[0] = sell_row_offdiag_seq<IDX16>(A, sl, lane, xo);
+            bv[0] = b[row];
+            dv[0] = dinv[row];
+            xr[0] = xo[row];
+        }
+#pragma unroll
+        for (int u = 0; u < (FW > 0 ? U : 1); ++u) {
+            const int su = sl + u * kWaves;
+            if (su < w.slice_end) {
+                float diff;
+                xn[su * kC + lane] = jacobi_next(bv[u], rx[u], dv[u], xr[u], diff);
+                m = nan_max(m, static_cast<double>(diff));
+            }
+        }
+    }
+    __shared__ double red[kWaves];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+    if (lane == 0) red[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int k = 1; k < kWaves; ++k) t = nan_max(t, red[k]);
+        put_partial(L, 0, lb, t);
+    }
+}
+
+// After `iters` fixed-count updates: one workgroup per system turns the
+// last update's partial maxima into the system's state (diff; max_iter, or
+// breakdown when that difference is not finite).  iters = 0: no update ran
+// and diff is 0.
+__global__ __launch_bounds__(kBlock, 1) void k_jacobi_finish(Launch L, JacState* S, int iters) {
+    const int sys = static_cast<int>(blockIdx.x);
+    const double m = iters > 0 ? max_partials(L, sys, 0) : 0.0;
+    if (threadIdx.x == 0) {
+        JacState s = S[sys];
+        s.diff = m;
+        s.iters = iters;
+        s.pending = 0;
+        s.active = 0;
+        s.status = isfinite(m) ? kStatusMaxIter : kStatusBreakdown;
+        S[sys] = s;
+    }
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1681,6 +1748,27 @@ void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_c
     else XP_JACOBI_WIDTHS(false);
 #undef XP_JACOBI_WIDTHS
 #undef XP_JACOBI_LAUNCH
+}
+
+void launch_jacobi_sweep(const Launch& L, const SellView& A, const float* b, const float* dinv,
+                         const float* x_old, float* x_new) {
+#define XP_JACOBI_SWEEP(I16, FW, U) \
+    XP_LAUNCH(L, (k_jacobi_sweep<I16, FW, U>), dim3(L.grid), L, A, b, dinv, x_old, x_new)
+#define XP_JACOBI_SWEEP_WIDTHS(I16)                                      \
+    do {                                                                 \
+        if (A.max_width <= 5) XP_JACOBI_SWEEP(I16, 5, (I16 ? 2 : 1));    \
+        else if (A.max_width <= 8) XP_JACOBI_SWEEP(I16, 8, 1);           \
+        else if (A.max_width <= 10) XP_JACOBI_SWEEP(I16, 10, 1);         \
+        else XP_JACOBI_SWEEP(I16, 0, 1);                                 \
+    } while (0)
+    if (A.idx16) XP_JACOBI_SWEEP_WIDTHS(true);
+    else XP_JACOBI_SWEEP_WIDTHS(false);
+#undef XP_JACOBI_SWEEP_WIDTHS
+#undef XP_JACOBI_SWEEP
+}
+
+void launch_jacobi_finish(const Launch& L, int nsys, JacState* st, int iters) {
+    hipLaunchKernelGGL(k_jacobi_finish, dim3(nsys), dim3(kBlock), 0, L.stream, L, st, iters);
 }
 
 }  // namespace xp

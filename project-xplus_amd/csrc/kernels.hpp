@@ -122,4 +122,10 @@ void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipSt
 void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
                         const float* b, const float* dinv, const float* x_old, float* x_new);
 
+// The fixed-count form: one update with no stop test (partial maxima of
+// |x_new - x_old| in slot 0), and the state after `iters` such updates.
+void launch_jacobi_sweep(const Launch& L, const SellView& A, const float* b, const float* dinv,
+                         const float* x_old, float* x_new);
+void launch_jacobi_finish(const Launch& L, int nsys, JacState* st, int iters);
+
 }  // namespace xp

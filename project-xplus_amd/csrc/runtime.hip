@@ -1268,12 +1268,23 @@ struct xp_jacobi {
                                odd ? xb.p : xa.p, odd ? xa.p : xb.p);
     }
 
+    // The fixed-count form: launch k of a run, no stop test.
+    void enqueue_sweep(int k, const xp::Launch& L) {
+        const bool odd = (k % 2) != 0;
+        xp::launch_jacobi_sweep(L, m->view(), b.p, dinv.p, odd ? xb.p : xa.p, odd ? xa.p : xb.p);
+    }
+
+    // Graphs of graph_iters launches: with the stop test (use_tau) or the
+    // fixed-count sweeps.
     void build_graph() {
         if (exec != nullptr) { HIPX(hipGraphExecDestroy(exec)); exec = nullptr; }
         if (graph != nullptr) { HIPX(hipGraphDestroy(graph)); graph = nullptr; }
         const xp::Launch L = launch();   // any reallocation happens before the capture
         HIPX(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < graph_iters; ++k) enqueue(k, L);
+        for (int k = 0; k < graph_iters; ++k) {
+            if (opts.use_tau) enqueue(k, L);
+            else enqueue_sweep(k, L);
+        }
         HIPX(hipStreamEndCapture(stream, &graph));
         HIPX(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         graph_version = m->layout_version;
@@ -1294,7 +1305,23 @@ struct xp_jacobi {
         st.upload(init.data(), init.size(), stream);
         HIPX(hipEventRecord(ev0, stream));
         last_launches = 0;
-        while (any_active()) {
+        if (!opts.use_tau) {
+            // Fixed count: max_iters updates with no stop test and no host
+            // round trip (whole graphs, then the remainder launched
+            // directly; graph_iters is even, so launch parity carries
+            // over), then the state from the last update's partials.
+            const int total = init.empty() ? 0 : init[0].max_iters;
+            const int full = opts.use_graph ? total / graph_iters : 0;
+            for (int g = 0; g < full; ++g) {
+                HIPX(hipGraphLaunch(exec, stream));
+                ++last_launches;
+            }
+            const xp::Launch L = launch();
+            for (int k = full * graph_iters; k < total; ++k) enqueue_sweep(k, L);
+            xp::launch_jacobi_finish(L, m->nsys, st.p, total);
+            HIPX(hipGetLastError());
+        }
+        while (opts.use_tau && any_active()) {
             if (opts.use_graph) {
                 HIPX(hipGraphLaunch(exec, stream));
             } else {
@@ -1466,8 +1493,9 @@ int xp_jacobi_last_timing(xp_jacobi* s, double* ms, int* graph_launches) {
     });
 }
 
-// Average device time of k_jacobi_iter over `iters` updates launched
-// directly with start / end events, from x0, without the tau test.
+// Average device time of the update kernel a run uses (k_jacobi_iter with
+// use_tau, k_jacobi_sweep without) over `iters` updates launched directly
+// with start / end events, from x0, without the tau test.
 int xp_jacobi_kernel_time(xp_jacobi* s, int iters, double* us) {
     return guarded([&] {
         if (s == nullptr || us == nullptr || !s->have_rhs || iters <= 0) throw std::runtime_error("bad argument");
@@ -1485,7 +1513,8 @@ int xp_jacobi_kernel_time(xp_jacobi* s, int iters, double* us) {
             xp::Launch L = S.launch();
             L.ev_start = ev[2 * static_cast<size_t>(k)];
             L.ev_stop = ev[2 * static_cast<size_t>(k) + 1];
-            S.enqueue(k, L);
+            if (S.opts.use_tau) S.enqueue(k, L);
+            else S.enqueue_sweep(k, L);
         }
         HIPX(hipStreamSynchronize(S.stream));
         double t = 0.0;

@@ -6,7 +6,9 @@ Bar: bit for bit.  The kernel sums each row's R x in CSR order with one
 rounding per product and per sum, as the reference's CPU loop does, so x
 (every entry), the update count, the status and the final diff_max must be
 identical -- on single systems, batches, duplicate and unsorted entries,
-early stops (use_tau), breakdown and the graph / direct launch paths.
+early stops (use_tau), breakdown and the graph / direct launch paths, in
+both forms: the fixed count (k_jacobi_sweep, no stop test) and the
+checked iteration (k_jacobi_iter, use_tau).
 """
 from __future__ import annotations
 
@@ -82,6 +84,13 @@ def check(a_list, b_list=None, x0_list=None, max_iters=25, use_tau=False, tau=1e
     return s, out
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_fixed_count_graph_remainder(gpu, use_graph):
+    """A count that is not a multiple of the graph length: whole graphs,
+    then the remainder launched directly, with launch parity carried over."""
+    check([dominant(2500, 14)], max_iters=23, graph_iters=6, use_graph=use_graph)
+
+
 @pytest.mark.parametrize("which", ["c1", "diff3d", "dominant", "dup", "big"])
 def test_fixed_count_bit_exact(gpu, which):
     a = {"c1": lambda: X.gen_diffusion2d(32, 32, 1.0, 20261204),
@@ -128,16 +137,20 @@ def test_zero_iterations_and_repeat_runs(gpu):
     np.testing.assert_array_equal(s2.x(), O.jacobi(a, b=a.b, max_iters=9)["x"].astype(np.float64))
 
 
-def test_breakdown_on_divergence(gpu):
-    """rho(D^-1 R) = 2: the iterates double until they overflow; the update
-    whose difference is infinite stops the run as breakdown."""
+@pytest.mark.parametrize("use_tau", [False, True])
+def test_breakdown_on_divergence(gpu, use_tau):
+    """rho(D^-1 R) = 2: the iterates double until they overflow.  Fixed
+    count: all updates run (as the reference's), the status is breakdown as
+    the last difference is not finite.  With the stop test the first
+    infinite difference stops the run."""
     import scipy.sparse as sp
     n = 256
     blk = sp.csr_matrix(np.array([[1.0, 2.0], [2.0, 1.0]]))
     A = sp.csr_matrix(sp.block_diag([blk] * (n // 2)))
     a = X.Csr(A.indptr, A.indices, A.data, np.ones(n), np.zeros(n))
-    _, out = check([a], max_iters=1000)
+    _, out = check([a], max_iters=1000, use_tau=use_tau, tau=1e-30)
     assert out[0][0]["status"] == 2
+    assert (out[0][0]["iterations"] == 1000) != use_tau
 
 
 def test_missing_diagonal_rejected(gpu):

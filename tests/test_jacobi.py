@@ -179,3 +179,18 @@ def test_tiny_case_float32_scalars():
             nx.append(np.float32(np.float32(np.float32(b[r]) - rx) * np.float32(np.float32(1) / d)))
         x = nx
     np.testing.assert_array_equal(O.jacobi(a, b=b, max_iters=6)["x"], np.array(x, np.float32))
+
+
+def test_fixed_count_runs_through_breakdown():
+    """Without use_tau every update runs, as the reference's RunJacobiCpu;
+    the status says breakdown when the last difference is not finite.  With
+    use_tau the first infinite difference stops the loop."""
+    A = np.kron(np.eye(8), np.array([[1.0, 2.0], [2.0, 1.0]]))
+    a = csr_of(A)
+    r = O.jacobi(a, b=np.ones(16), max_iters=400)
+    assert r["iterations"] == 400 and r["status"] == 2 and not np.isfinite(r["diff"])
+    t = O.jacobi(a, b=np.ones(16), max_iters=400, use_tau=True, tau=1e-30)
+    assert t["status"] == 2 and t["iterations"] < 400 and np.isinf(t["diff"])
+    # the updates before the overflow are the same in both forms
+    u = O.jacobi(a, b=np.ones(16), max_iters=t["iterations"])
+    np.testing.assert_array_equal(u["x"], t["x"])
