@@ -296,13 +296,27 @@ __device__ __forceinline__ float block_apply_factor(const uint4 (&q)[5], float s
     return sj * v;
 }
 
-__device__ __forceinline__ float block_apply(const BlockPre* __restrict__ M, int row, float r) {
+// A row's block factor and scale, loaded ahead of the residual it is
+// applied to (the update kernels request it with the row's vectors, so the
+// block is not a dependent load after r is known).
+struct BlockRow {
+    uint4 q[5];
+    float s;
+};
+
+__device__ __forceinline__ BlockRow block_load(const BlockPre* __restrict__ M, int row) {
     const BlockPre* b = M + (row >> 3);
     const uint4* cp = reinterpret_cast<const uint4*>(b->c);
-    uint4 q[5];
+    BlockRow o;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) q[k] = cp[k];
-    return block_apply_factor(q, b->s[row & 7], row & 7, r);
+    for (int k = 0; k < 5; ++k) o.q[k] = cp[k];
+    o.s = b->s[row & 7];
+    return o;
+}
+
+__device__ __forceinline__ float block_apply(const BlockPre* __restrict__ M, int row, float r) {
+    const BlockRow b = block_load(M, row);
+    return block_apply_factor(b.q, b.s, row & 7, r);
 }
 
 // Factors full FP32 blocks (solve order, block q = 64 floats row-major) into
@@ -476,6 +490,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
     double alpha = 0.0;
     for (int base = r0; base < r1; base += ROWS * kBlock) {
         float xv[ROWS], pv[ROWS], rv[ROWS], av[ROWS], mv[ROWS];
+        BlockRow bl[ROWS];
 #pragma unroll
         for (int k = 0; k < ROWS; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
@@ -485,6 +500,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
                 rv[k] = r[row];
                 av[k] = ap[row];
                 mv[k] = BLOCK ? 0.0f : minv[row];
+                if (BLOCK) bl[k] = block_load(blockM, row);
             }
         }
         if (first) {
@@ -502,7 +518,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
                 const float xn = static_cast<float>(static_cast<double>(xv[k]) + alpha * static_cast<double>(pv[k]));
                 const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * static_cast<double>(av[k]));
                 const float zn = BLOCK
-                                     ? block_apply(blockM, row, rn)
+                                     ? block_apply_factor(bl[k].q, bl[k].s, row & 7, rn)
                                      : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
                 x[row] = xn;
                 r[row] = rn;
@@ -813,6 +829,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
     for (int base = r0; base < r1; base += ROWS * kBlock) {
         double xv[ROWS], pv[ROWS], av[ROWS];
         float rv[ROWS], mv[ROWS];
+        BlockRow bl[ROWS];
 #pragma unroll
         for (int k = 0; k < ROWS; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
@@ -822,6 +839,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
                 rv[k] = r[row];
                 av[k] = ap[row];
                 mv[k] = BLOCK ? 0.0f : minv[row];
+                if (BLOCK) bl[k] = block_load(blockM, row);
             }
         }
         if (first) {
@@ -838,7 +856,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
             if (row < r1) {  // uniform per wavefront: r1 - base is a multiple of 64
                 const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * av[k]);
                 const float zn = BLOCK
-                                     ? block_apply(blockM, row, rn)
+                                     ? block_apply_factor(bl[k].q, bl[k].s, row & 7, rn)
                                      : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
                 x[row] = xv[k] + alpha * pv[k];
                 r[row] = rn;
