@@ -90,6 +90,12 @@ def spmv_roofline(m, a, reps: int = 200) -> dict:
     us = m.spmv_f32_time(20, reps)
     r = roof("k_spmv_f32", 8 * a.nnz + 12 * a.n, us, "8 nnz + 12 n (FP32 value + int32 column; row ptr, x, y)")
     r["stream_bytes"] = m.info()["bytes_spmv"]
+    # warm (above: the working set can stay in the 256 MB MALL between
+    # launches) and cold (a 1 GiB copy before every launch: HBM)
+    cold = m.spmv_f32_time_cold(20, 1 << 30)
+    r["cold"] = {"avg_us": round(cold, 3), "achieved": round((8 * a.nnz + 12 * a.n) / (cold * 1e-6) / 1e9, 1),
+                 "frac": round((8 * a.nnz + 12 * a.n) / (cold * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                 "protocol": "1 GiB device copy before each of 20 launches"}
     prof = profiled_traffic()
     if r["kernel"] in prof:
         r["traffic"], r["traffic_source"] = prof[r["kernel"]]

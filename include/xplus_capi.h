@@ -157,6 +157,10 @@ int xp_device_synchronize(void);
  * library stream), after `warmup` untimed launches; x is filled with a
  * fixed pattern.  avg_us receives the mean time per launch. */
 int xp_spmv_f32_time(xp_matrix* m, int warmup, int reps, double* avg_us);
+/* The same with cold caches: a device-to-device copy of flush_bytes before
+ * every timed launch evicts the L2s and the MALL, so the matrix and x are
+ * read from HBM (the warm figure above can be served partly from the MALL). */
+int xp_spmv_f32_time_cold(xp_matrix* m, int reps, unsigned long long flush_bytes, double* avg_us);
 
 /* ===================================================================== *
  * 3. Solver.

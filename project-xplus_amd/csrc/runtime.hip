@@ -1006,6 +1006,42 @@ int xp_spmv_f32_time(xp_matrix* m, int warmup, int reps, double* avg_us) {
     });
 }
 
+// The same SpMV timing with the caches cold: before every timed launch a
+// device-to-device copy of flush_bytes (more than the 256 MB MALL and the
+// L2s together) replaces the cached lines, so the matrix and x come from HBM.
+int xp_spmv_f32_time_cold(xp_matrix* m, int reps, unsigned long long flush_bytes, double* avg_us) {
+    return guarded([&] {
+        if (m == nullptr || avg_us == nullptr || reps <= 0 || flush_bytes == 0) throw std::runtime_error("bad argument");
+        const hipStream_t s = device().stream;
+        const size_t np = static_cast<size_t>(m->n_pad);
+        std::vector<float> xh(np);
+        for (size_t i = 0; i < np; ++i) xh[i] = 1.0f + static_cast<float>(i % 7) * 0.125f;
+        DevBuf<float> x(np), y(np);
+        x.upload(xh.data(), np, s);
+        const size_t words = static_cast<size_t>(flush_bytes / 4);
+        DevBuf<float> fa(words), fb(words);
+        HIPX(hipMemsetAsync(fa.p, 0, words * 4, s));
+        std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps));
+        for (auto& e : ev) HIPX(hipEventCreate(&e));
+        for (int k = 0; k < reps; ++k) {
+            HIPX(hipMemcpyAsync(fb.p, fa.p, words * 4, hipMemcpyDeviceToDevice, s));
+            xp::Launch L = m->launch(s);
+            L.ev_start = ev[2 * static_cast<size_t>(k)];
+            L.ev_stop = ev[2 * static_cast<size_t>(k) + 1];
+            xp::launch_spmv_f32(L, m->view(), x.p, y.p);
+        }
+        HIPX(hipStreamSynchronize(s));
+        double total = 0.0;
+        for (int k = 0; k < reps; ++k) {
+            float ms = 0.0f;
+            HIPX(hipEventElapsedTime(&ms, ev[2 * static_cast<size_t>(k)], ev[2 * static_cast<size_t>(k) + 1]));
+            total += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *avg_us = 1000.0 * total / reps;
+    });
+}
+
 int xp_csr_spmv_f64(int n, const int* row_ptr, const int* col_idx, const double* values,
                     const double* x, double* y) {
     if (n <= 0) return 0;

@@ -114,6 +114,7 @@ _dtoh = _sig("xp_memcpy_dtoh", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dtod = _sig("xp_memcpy_dtod", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dev_sync = _sig("xp_device_synchronize", C.c_int, [])
 _spmv_time = _sig("xp_spmv_f32_time", C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)])
+_spmv_time_cold = _sig("xp_spmv_f32_time_cold", C.c_int, [_vp, C.c_int, C.c_ulonglong, C.POINTER(C.c_double)])
 
 _p_create = _sig("xp_pcg_create", C.c_int, [_vp, C.POINTER(SolveOpts), C.POINTER(_vp)])
 _p_destroy = _sig("xp_pcg_destroy", None, [_vp])
@@ -641,6 +642,12 @@ class Matrix:
         """Mean microseconds per FP32 SpMV launch (HIP events on the launch stream)."""
         us = C.c_double()
         _check(_spmv_time(self.h, warmup, reps, C.byref(us)))
+        return us.value
+
+    def spmv_f32_time_cold(self, reps: int = 20, flush_bytes: int = 1 << 30) -> float:
+        """The same with the caches flushed (a flush_bytes copy) before every launch."""
+        us = C.c_double()
+        _check(_spmv_time_cold(self.h, reps, flush_bytes, C.byref(us)))
         return us.value
 
     def to_solve_order(self, nat_ptr: int, solve_ptr: int, stream: int = 0) -> None:
