@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <exception>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -52,12 +53,30 @@ void schedule(std::vector<Elem>& in, int base_col, std::vector<int>& window, std
     touched.clear();
 }
 
+// f(0) .. f(workers - 1), one thread each; exceptions rethrown after the
+// join, a thread that cannot be started runs its share inline.
 template <typename F>
 void for_workers(int workers, F&& f) {
+    std::vector<std::exception_ptr> err(static_cast<size_t>(std::max(workers, 1)));
+    auto run = [&](int w) {
+        try {
+            f(w);
+        } catch (...) {
+            err[static_cast<size_t>(w)] = std::current_exception();
+        }
+    };
     std::vector<std::thread> th;
-    for (int w = 1; w < workers; ++w) th.emplace_back(f, w);
-    f(0);
+    for (int w = 1; w < workers; ++w) {
+        try {
+            th.emplace_back(run, w);
+        } catch (...) {
+            run(w);
+        }
+    }
+    run(0);
     for (auto& t : th) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 }  // namespace

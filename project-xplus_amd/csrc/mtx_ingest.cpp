@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <filesystem>
 #include <stdexcept>
 #include <thread>
@@ -191,16 +192,31 @@ int worker_count(size_t bytes) {
     return static_cast<int>(std::min<size_t>({static_cast<size_t>(hw), by_size, 16}));
 }
 
+// f(0) .. f(workers - 1), one thread each.  A worker's exception is
+// rethrown after every thread has joined; a thread that cannot be started
+// runs its share inline.
 template <typename F>
 void parallel_for(int workers, F&& f) {
-    if (workers <= 1) {
-        f(0);
-        return;
-    }
+    std::vector<std::exception_ptr> err(static_cast<size_t>(std::max(workers, 1)));
+    auto run = [&](int w) {
+        try {
+            f(w);
+        } catch (...) {
+            err[static_cast<size_t>(w)] = std::current_exception();
+        }
+    };
     std::vector<std::thread> th;
-    for (int w = 1; w < workers; ++w) th.emplace_back(f, w);
-    f(0);
+    for (int w = 1; w < workers; ++w) {
+        try {
+            th.emplace_back(run, w);
+        } catch (...) {
+            run(w);
+        }
+    }
+    run(0);
     for (auto& t : th) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 }  // namespace

@@ -81,7 +81,11 @@ inline std::string json_escape(const std::string& in) {
         else if (ch == '\n') out += "\\n";
         else if (ch == '\r') out += "\\r";
         else if (ch == '\t') out += "\\t";
-        else out += ch;
+        else if (static_cast<unsigned char>(ch) < 0x20) {
+            char u[8];
+            std::snprintf(u, sizeof u, "\\u%04x", static_cast<unsigned>(static_cast<unsigned char>(ch)));
+            out += u;
+        } else out += ch;
     }
     return out;
 }
