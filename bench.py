@@ -391,8 +391,9 @@ def main() -> int:
     ap.add_argument("--lam", type=float, default=0.05)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the solver shards as described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(a system would shard as a row partition with one halo exchange and two scalar "
+              "all-reduces per iteration; batches by system)", file=sys.stderr)
         return 2
 
     import xplus_amd as X
