@@ -189,3 +189,23 @@ def test_cli_rejects_bad_arguments(tmp_path):
     out = subprocess.run([str(X.CUPER_CLI_PATH), str(tmp_path / "missing")], capture_output=True, text=True,
                          timeout=60)
     assert out.returncode == 1 and "failed to open" in out.stderr
+
+
+def test_import_is_independent_of_slot_order():
+    """Packers of this format differ in where equal columns land and in the
+    slot schedule (the DLC packer, DLC/Cuper/include/Cuper_common.h:535-774,
+    orders entries by its slice traversal): import must not depend on it.
+    Every (channel, lane, batch) segment shuffled, empty slots included."""
+    a = X.gen_fem2d(120, 1.0, 6, 1e3, 0.3, 4)
+    p = X.cuper_pack(a)
+    rng = np.random.default_rng(3)
+    data = p.data.copy()
+    for ch in range(16):
+        for lane in range(8):
+            for b in range(p.batch_count):
+                idx = np.arange(p.ptr[b], p.ptr[b + 1]) * 8 + lane
+                data[ch, idx] = data[ch, rng.permutation(idx)]
+    back = X.CuperPacked(p.ptr, data).unpack(a.n)
+    np.testing.assert_array_equal(back.row_ptr, a.row_ptr)
+    np.testing.assert_array_equal(back.col_idx, a.col_idx)
+    np.testing.assert_array_equal(back.values, fp32(a))
