@@ -134,6 +134,26 @@ int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int iters, xp_tu
 int xp_spmv_f32(xp_matrix* m, const float* x, float* y, void* stream);
 /* y = A x in FP64: exact values if keep64, else hi+lo pairs (needs keep_lo). */
 int xp_spmv_f64(xp_matrix* m, const double* x, double* y, void* stream);
+/* Stream-ordered FP32 device operations on solve-order vectors of n_pad
+ * entries: the five fused operations (archived/kernels/cg_kernels.cpp:73-158)
+ * on device pointers, for callers that drive their own iteration with
+ * xp_spmv_f32.  Each stored value is evaluated in FP64 from the FP32
+ * operands (no fused multiply-add) and rounded to FP32 when stored;
+ * reductions are FP64 sums of the exact FP32 products in a fixed order, one
+ * per system, written to device memory: metrics[2 k] = r^T z,
+ * metrics[2 k + 1] = r^T r, out[k] = a^T b.  Padding rows of the inputs
+ * must be finite (xp_to_solve_order writes zeros there).  They use the
+ * matrix's reduction scratch, as its solvers do: do not overlap them with
+ * a solve on the same matrix.
+ *   init:       r = b - ax, z = minv r, p = z
+ *   update_xrz: x += alpha p, r -= alpha ap, z = minv r
+ *   update_p:   p = z + beta p */
+int xp_init_pcg_f32(xp_matrix* m, const float* b, const float* ax, const float* minv, float* r, float* z, float* p,
+                    double* metrics, void* stream);
+int xp_dot_f32(xp_matrix* m, const float* a, const float* b, double* out, void* stream);
+int xp_update_xrz_f32(xp_matrix* m, float* x, const float* p, float* r, const float* ap, const float* minv, float* z,
+                      double* metrics, double alpha, void* stream);
+int xp_update_p_f32(xp_matrix* m, const float* z, float* p, double beta, void* stream);
 /* natural (n, FP64) -> solve order (n_pad, FP32), and back. */
 int xp_to_solve_order(xp_matrix* m, const double* nat, float* solve, void* stream);
 int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* stream);

@@ -1426,6 +1426,109 @@ __global__ __launch_bounds__(kBlock, 1) void k_jacobi_finish(Launch L, JacState*
         S[sys] = s;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Stream-ordered FP32 device operations: the five fused operations of
+// archived/kernels/cg_kernels.cpp:73-158 on device vectors in solve order,
+// for callers that drive their own iteration.  Each value is computed in
+// FP64 from the FP32 operands (no fused multiply-add) and rounded once to
+// FP32; the dot products are FP64 sums of the exact FP32 products, one per
+// workgroup into the partial slots, finished by k_op_finish in a fixed
+// order.
+
+__global__ __launch_bounds__(kBlock, 8) void k_op_init(Launch L, const float* __restrict__ b,
+                                                     const float* __restrict__ ax,
+                                                     const float* __restrict__ minv, float* __restrict__ r,
+                                                     float* __restrict__ z, float* __restrict__ p) {
+#pragma clang fp contract(off)
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    double v[2] = {0.0, 0.0};
+    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock) {
+        const float rv = static_cast<float>(static_cast<double>(b[row]) - static_cast<double>(ax[row]));
+        const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
+        r[row] = rv;
+        z[row] = zv;
+        p[row] = zv;
+        v[0] += static_cast<double>(rv) * static_cast<double>(zv);
+        v[1] += static_cast<double>(rv) * static_cast<double>(rv);
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        put_partial(L, kSlotRz, lb, v[0]);
+        put_partial(L, kSlotRr, lb, v[1]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock, 8) void k_op_dot(Launch L, const float* __restrict__ a,
+                                                    const float* __restrict__ b) {
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    double v[1] = {0.0};
+    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock)
+        v[0] += static_cast<double>(a[row]) * static_cast<double>(b[row]);
+    block_sum<1>(v);
+    if (threadIdx.x == 0) put_partial(L, kSlotAux, lb, v[0]);
+}
+
+__global__ __launch_bounds__(kBlock, 8) void k_op_update_xrz(Launch L, float* __restrict__ x,
+                                                           const float* __restrict__ p, float* __restrict__ r,
+                                                           const float* __restrict__ ap,
+                                                           const float* __restrict__ minv, float* __restrict__ z,
+                                                           double alpha) {
+#pragma clang fp contract(off)
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    double v[2] = {0.0, 0.0};
+    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock) {
+        const double ap_a = alpha * static_cast<double>(ap[row]);
+        const double p_a = alpha * static_cast<double>(p[row]);
+        const float rv = static_cast<float>(static_cast<double>(r[row]) - ap_a);
+        const float zv = static_cast<float>(static_cast<double>(minv[row]) * static_cast<double>(rv));
+        x[row] = static_cast<float>(static_cast<double>(x[row]) + p_a);
+        r[row] = rv;
+        z[row] = zv;
+        v[0] += static_cast<double>(rv) * static_cast<double>(zv);
+        v[1] += static_cast<double>(rv) * static_cast<double>(rv);
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        put_partial(L, kSlotRz, lb, v[0]);
+        put_partial(L, kSlotRr, lb, v[1]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock, 8) void k_op_update_p(Launch L, const float* __restrict__ z,
+                                                         float* __restrict__ p, double beta) {
+#pragma clang fp contract(off)
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock) {
+        const double bp = beta * static_cast<double>(p[row]);
+        p[row] = static_cast<float>(static_cast<double>(z[row]) + bp);
+    }
+}
+
+// One workgroup per system: out[sys * nslot + k] = the sum of partial slot
+// slot0 + k over the system's workgroups.
+__global__ __launch_bounds__(kBlock, 1) void k_op_finish(Launch L, int slot0, int nslot, double* __restrict__ out) {
+    const int sys = static_cast<int>(blockIdx.x);
+    if (nslot == 1) {
+        double t[1];
+        sum_partials<1>(L, sys, {slot0}, t);
+        if (threadIdx.x == 0) out[sys] = t[0];
+    } else {
+        double t[2];
+        sum_partials<2>(L, sys, {slot0, slot0 + 1}, t);
+        if (threadIdx.x == 0) {
+            out[2 * sys] = t[0];
+            out[2 * sys + 1] = t[1];
+        }
+    }
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1787,6 +1890,27 @@ void launch_jacobi_sweep(const Launch& L, const SellView& A, const float* b, con
 
 void launch_jacobi_finish(const Launch& L, int nsys, JacState* st, int iters) {
     hipLaunchKernelGGL(k_jacobi_finish, dim3(nsys), dim3(kBlock), 0, L.stream, L, st, iters);
+}
+
+void launch_op_init(const Launch& L, int nsys, const float* b, const float* ax, const float* minv, float* r,
+                    float* z, float* p, double* metrics) {
+    XP_LAUNCH(L, k_op_init, dim3(L.grid), L, b, ax, minv, r, z, p);
+    hipLaunchKernelGGL(k_op_finish, dim3(nsys), dim3(kBlock), 0, L.stream, L, static_cast<int>(kSlotRz), 2, metrics);
+}
+
+void launch_op_dot(const Launch& L, int nsys, const float* a, const float* b, double* out) {
+    XP_LAUNCH(L, k_op_dot, dim3(L.grid), L, a, b);
+    hipLaunchKernelGGL(k_op_finish, dim3(nsys), dim3(kBlock), 0, L.stream, L, static_cast<int>(kSlotAux), 1, out);
+}
+
+void launch_op_update_xrz(const Launch& L, int nsys, float* x, const float* p, float* r, const float* ap,
+                          const float* minv, float* z, double* metrics, double alpha) {
+    XP_LAUNCH(L, k_op_update_xrz, dim3(L.grid), L, x, p, r, ap, minv, z, alpha);
+    hipLaunchKernelGGL(k_op_finish, dim3(nsys), dim3(kBlock), 0, L.stream, L, static_cast<int>(kSlotRz), 2, metrics);
+}
+
+void launch_op_update_p(const Launch& L, const float* z, float* p, double beta) {
+    XP_LAUNCH(L, k_op_update_p, dim3(L.grid), L, z, p, beta);
 }
 
 }  // namespace xp

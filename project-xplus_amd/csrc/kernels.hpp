@@ -128,4 +128,13 @@ void launch_jacobi_sweep(const Launch& L, const SellView& A, const float* b, con
                          const float* x_old, float* x_new);
 void launch_jacobi_finish(const Launch& L, int nsys, JacState* st, int iters);
 
+// ---- stream-ordered FP32 device operations (solve-order vectors) ----
+// metrics: [2 sys] = r^T z, [2 sys + 1] = r^T r; out: [sys] = a^T b.
+void launch_op_init(const Launch& L, int nsys, const float* b, const float* ax, const float* minv, float* r,
+                    float* z, float* p, double* metrics);
+void launch_op_dot(const Launch& L, int nsys, const float* a, const float* b, double* out);
+void launch_op_update_xrz(const Launch& L, int nsys, float* x, const float* p, float* r, const float* ap,
+                          const float* minv, float* z, double* metrics, double alpha);
+void launch_op_update_p(const Launch& L, const float* z, float* p, double beta);
+
 }  // namespace xp

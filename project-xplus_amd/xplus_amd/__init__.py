@@ -113,6 +113,10 @@ _htod = _sig("xp_memcpy_htod", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dtoh = _sig("xp_memcpy_dtoh", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dtod = _sig("xp_memcpy_dtod", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dev_sync = _sig("xp_device_synchronize", C.c_int, [])
+_op_init = _sig("xp_init_pcg_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
+_op_dot = _sig("xp_dot_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp])
+_op_xrz = _sig("xp_update_xrz_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp])
+_op_p = _sig("xp_update_p_f32", C.c_int, [_vp, _vp, _vp, C.c_double, _vp])
 _spmv_time = _sig("xp_spmv_f32_time", C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)])
 _spmv_time_cold = _sig("xp_spmv_f32_time_cold", C.c_int, [_vp, C.c_int, C.c_ulonglong, C.POINTER(C.c_double)])
 
@@ -631,12 +635,30 @@ class Matrix:
         _check(_m_autotune(self.h, int(fused), int(tune_sigma), iters, C.byref(r)))
         return {k: getattr(r, k) for k, _ in r._fields_}
 
-    # device-pointer operations (ints are device addresses, e.g. tensor.data_ptr())
+    # device-pointer operations: ints are device addresses in the library's HIP
+    # runtime (DeviceArray.ptr, or hipMalloc through the same libamdhip64)
     def spmv_f32(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
         _check(_spmv_f32(self.h, x_ptr, y_ptr, stream or None))
 
     def spmv_f64(self, x_ptr: int, y_ptr: int, stream: int = 0) -> None:
         _check(_spmv_f64(self.h, x_ptr, y_ptr, stream or None))
+
+    # the fused vector operations of a PCG iteration on solve-order FP32
+    # vectors (n_pad entries); metrics / out are FP64 device arrays:
+    # [2 k] = r^T z, [2 k + 1] = r^T r per system k, out[k] = a^T b
+    def init_pcg_f32(self, b_ptr, ax_ptr, minv_ptr, r_ptr, z_ptr, p_ptr, metrics_ptr, stream: int = 0) -> None:
+        _check(_op_init(self.h, b_ptr, ax_ptr, minv_ptr, r_ptr, z_ptr, p_ptr, metrics_ptr, stream or None))
+
+    def dot_f32(self, a_ptr, b_ptr, out_ptr, stream: int = 0) -> None:
+        _check(_op_dot(self.h, a_ptr, b_ptr, out_ptr, stream or None))
+
+    def update_xrz_f32(self, x_ptr, p_ptr, r_ptr, ap_ptr, minv_ptr, z_ptr, metrics_ptr, alpha: float,
+                       stream: int = 0) -> None:
+        _check(_op_xrz(self.h, x_ptr, p_ptr, r_ptr, ap_ptr, minv_ptr, z_ptr, metrics_ptr, float(alpha),
+                       stream or None))
+
+    def update_p_f32(self, z_ptr, p_ptr, beta: float, stream: int = 0) -> None:
+        _check(_op_p(self.h, z_ptr, p_ptr, float(beta), stream or None))
 
     def spmv_f32_time(self, warmup: int = 20, reps: int = 200) -> float:
         """Mean microseconds per FP32 SpMV launch (HIP events on the launch stream)."""
