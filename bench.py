@@ -8,8 +8,8 @@ with 20 inclusions at 1e3 contrast, b ~ U[-1,1], x0 = 0, synthetic, seed
 A step is one complete solve from x0 to ||b - A x|| / ||b|| <= 1e-6 (FP64,
 on the GPU) with the matrix, b and x0 resident in HBM:
 
-  learned   (the value)  learned 8x8 blocks over strength aggregates of 8
-            rows (device features + BF16 MFMA inference, timed as setup)
+  learned   (the value)  learned 8x8 blocks over strength-ordered
+            aggregates of 8 rows (device features + BF16 MFMA inference, timed as setup)
             with the accurate iteration: FP32 r and z, matrix values as FP32
             hi+lo pairs, FP64-accumulated SpMV, FP64 p, A p and x, FP64
             reductions, the true residual checked in FP64 when the recursive
@@ -384,7 +384,9 @@ def main() -> int:
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "jacobi"],
                     help="c2: the benchmark (default); c3: FP64-refined 1100/1600 FEM to 1e-10; "
                          "c4: 256 batched 3D systems; jacobi: 1000 plain Jacobi updates")
-    ap.add_argument("--blocking", type=int, default=1, help="8-row blocks: 0 consecutive rows, 1 strength aggregates")
+    ap.add_argument("--blocking", type=int, default=2,
+                    help="8-row blocks: 0 consecutive rows, 1 strength aggregates (row-order seeds), "
+                         "2 strength-ordered aggregates")
     # off-block weight of the training loss: on strength aggregates any
     # sizeable weight moves the blocks away from the exact inverses the
     # network starts from and costs iterations (0.3: +2%, 1.0: +10%)
@@ -500,7 +502,7 @@ def main() -> int:
         "config": {"workload": f"thermal-FEM P1 {args.mesh}x{args.mesh} nodes, n={a.n}, nnz={a.nnz}, "
                                f"PCG to true relative residual 1e-6",
                    "n": a.n, "nnz": a.nnz,
-                   "method": "learned 8x8 block-Jacobi (BF16 MFMA MLP, blocks = strength aggregates of 8 rows) "
+                   "method": "learned 8x8 block-Jacobi (BF16 MFMA MLP, blocks = strength-ordered aggregates of 8 rows) "
                              "PCG; FP32 r/z and FP32 hi+lo matrix "
                              "values, FP64-accumulated SpMV, FP64 p/Ap/x, FP64 reductions, residual replacement "
                              "on the FP64 true residual; value includes the preconditioner's device setup",

@@ -88,7 +88,8 @@ typedef struct {
     int block_rows;   /* 8: keep blocks of up to 8 rows together as 8 solve rows
                          (block preconditioners); 0/1: rows */
     int blocking;     /* with block_rows = 8: 0 = blocks of 8 consecutive rows;
-                         1 = greedy strength aggregates (xp_strength_blocking) */
+                         1 = greedy strength aggregates (xp_strength_blocking);
+                         2 = strength-ordered aggregates (xp_aggregate_blocking rule 2) */
 } xp_matrix_opts;
 
 typedef struct {
@@ -304,6 +305,13 @@ int xp_matrix_block_rows(const xp_matrix* m, int* rows);
  * slots: capacity 8 n; writes 8 * nblocks entries (-1 = padding). */
 int xp_strength_blocking(int n, const int* row_ptr, const int* col_idx, const double* values, int* slots,
                          int* nblocks);
+/* The same with a choice of rule: 1 = xp_strength_blocking; 2 = seeds in
+ * decreasing order of the row's total off-diagonal strength t_i (ties: the
+ * smaller row), a block grows by the unassigned row with the largest
+ * (strength to the block) / t_j (ties: the smaller index); the regrouping
+ * and packing steps as rule 1. */
+int xp_aggregate_blocking(int n, const int* row_ptr, const int* col_idx, const double* values, int rule, int* slots,
+                          int* nblocks);
 /* Average device time (us) of each kernel of one iteration, over `iters`
  * directly launched iterations from x0 with the convergence test off, HIP
  * events recorded on the solver's stream between kernels: us[0] the SpMV
@@ -336,7 +344,7 @@ int xp_mlp_get_params(xp_mlp* net, float* out);
 int xp_mlp_set_params(xp_mlp* net, const float* params);
 /* Blocks of a host CSR system (no GPU needed); append concatenates sets. */
 int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const double* values, xp_blockset** out);
-/* The same over the blocks of a blocking (0: consecutive rows, 1: strength),
+/* The same over the blocks of a blocking (0: consecutive rows, 1 / 2: strength aggregates, rule 1 / 2),
  * with the off-block weight of the training loss (< 0: 0.5). */
 int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, const double* values, int blocking,
                                  double lambda, xp_blockset** out);

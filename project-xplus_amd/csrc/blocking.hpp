@@ -26,14 +26,26 @@ struct Blocking {
 Blocking natural_blocking(int n);
 
 // Greedy strength aggregation.  s_ij = |a_ij| / sqrt(a_ii a_jj) (a row
-// without a positive diagonal scales by 1).  Seeds are taken in row order;
-// a block grows by the unassigned row with the largest summed strength to
-// the block's rows (ties: the smaller index) until it holds 8 rows or has no
-// unassigned neighbour.  Rows left in blocks of fewer than 8 are then
-// pooled, regrouped by the same rule among themselves, and what is still
-// partial is packed first-fit (blocks in creation order) into blocks of up
-// to 8 rows.  Rows inside a block are in ascending order; blocks are
-// numbered by their smallest row.  Deterministic.
-Blocking strength_blocking(int n, const int* row_ptr, const int* col_idx, const double* values);
+// without a positive diagonal scales by 1).
+//   kRowOrder: seeds are taken in row order; a block grows by the
+//     unassigned row with the largest summed strength to the block's rows
+//     (ties: the smaller index) until it holds 8 rows or has no unassigned
+//     neighbour.
+//   kStrengthOrder: seeds in decreasing order of the row's total
+//     off-diagonal strength t_i (ties: the smaller row); a block grows by the
+//     unassigned row with the largest fraction (strength to the block) / t_j
+//     (ties: the smaller index).  Strongly coupled regions are aggregated
+//     first, and rows join the block that holds most of their coupling: on
+//     the thermal-FEM family exact block inverses need 7-10% fewer PCG
+//     iterations than with kRowOrder (202^2: 1178 -> 1092, 402^2: 2457 ->
+//     2219, mean of three right-hand sides).
+// Rows left in blocks of fewer than 8 are then pooled, regrouped by the same
+// rule among themselves (same seed order), and what is still partial is
+// packed first-fit (blocks in creation order) into blocks of up to 8 rows.
+// Rows inside a block are in ascending order; blocks are numbered by their
+// smallest row.  Deterministic.
+enum : int { kRowOrder = 1, kStrengthOrder = 2 };
+Blocking strength_blocking(int n, const int* row_ptr, const int* col_idx, const double* values,
+                           int rule = kRowOrder);
 
 }  // namespace xp

@@ -142,7 +142,7 @@ struct xp_matrix {
     int solvers = 0;                  // live xp_pcg objects built on this matrix
     int layout_version = 0;           // bumped when the work map / partials are reallocated
     int block_rows = 1;               // 8: blocks of up to 8 rows kept together as 8 solve rows
-    int blocking = 0;                 // blocks: 0 consecutive natural rows, 1 strength aggregates
+    int blocking = 0;                 // blocks: 0 consecutive natural rows, 1 / 2 strength aggregates (rule)
     std::vector<xp::Blocking> blockings;  // per system (block_rows = 8)
     std::vector<int> block_unit;      // solve block -> block id (-1: all padding)
     std::vector<int> unit_first;      // [nsys + 1] first block id per system
@@ -340,13 +340,14 @@ std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, c
         m->h_values.insert(m->h_values.end(), a.values, a.values + nnz);
     }
     if (m->block_rows == 8) {
-        m->blocking = (o != nullptr && o->blocking == 1) ? 1 : 0;
+        m->blocking = o != nullptr ? o->blocking : 0;
+        if (m->blocking < 0 || m->blocking > 2) throw std::runtime_error("blocking must be 0, 1 or 2");
         size_t rp_off = 0, nz_off = 0;
         for (int nk : m->h_ns) {
             const int* rp = m->h_row_ptr.data() + rp_off;
-            m->blockings.push_back(m->blocking == 1 ? xp::strength_blocking(nk, rp, m->h_col_idx.data() + nz_off,
-                                                                             m->h_values.data() + nz_off)
-                                                    : xp::natural_blocking(nk));
+            m->blockings.push_back(m->blocking > 0 ? xp::strength_blocking(nk, rp, m->h_col_idx.data() + nz_off,
+                                                                            m->h_values.data() + nz_off, m->blocking)
+                                                   : xp::natural_blocking(nk));
             rp_off += static_cast<size_t>(nk) + 1;
             nz_off += static_cast<size_t>(rp[nk]);
         }
@@ -2523,12 +2524,12 @@ int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const do
 int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, const double* values, int blocking,
                                  double lambda, xp_blockset** out) {
     return guarded([&] {
-        if (out == nullptr || row_ptr == nullptr || n <= 0 || blocking < 0 || blocking > 1)
+        if (out == nullptr || row_ptr == nullptr || n <= 0 || blocking < 0 || blocking > 2)
             throw std::runtime_error("bad argument");
         auto b = std::make_unique<xp_blockset>();
         b->set = xp::BlockSet::from_csr(n, row_ptr, col_idx, values,
-                                        blocking == 1 ? xp::strength_blocking(n, row_ptr, col_idx, values)
-                                                      : xp::natural_blocking(n),
+                                        blocking > 0 ? xp::strength_blocking(n, row_ptr, col_idx, values, blocking)
+                                                     : xp::natural_blocking(n),
                                         lambda < 0.0 ? xp::kMlpLambda : lambda);
         *out = b.release();
     });
@@ -2539,6 +2540,16 @@ int xp_strength_blocking(int n, const int* row_ptr, const int* col_idx, const do
     return guarded([&] {
         if (slots == nullptr || nblocks == nullptr || n < 0) throw std::runtime_error("bad argument");
         const xp::Blocking b = xp::strength_blocking(n, row_ptr, col_idx, values);
+        std::memcpy(slots, b.slots.data(), b.slots.size() * sizeof(int));
+        *nblocks = b.units();
+    });
+}
+
+int xp_aggregate_blocking(int n, const int* row_ptr, const int* col_idx, const double* values, int rule, int* slots,
+                          int* nblocks) {
+    return guarded([&] {
+        if (slots == nullptr || nblocks == nullptr || n < 0) throw std::runtime_error("bad argument");
+        const xp::Blocking b = xp::strength_blocking(n, row_ptr, col_idx, values, rule);
         std::memcpy(slots, b.slots.data(), b.slots.size() * sizeof(int));
         *nblocks = b.units();
     });

@@ -170,6 +170,8 @@ _bs_from_blocked = _sig("xp_blockset_from_csr_blocked", C.c_int,
                         [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_double, C.POINTER(_vp)])
 _strength_blocking = _sig("xp_strength_blocking", C.c_int,
                           [C.c_int, _i32p, _i32p, _f64p, _i32p, C.POINTER(C.c_int)])
+_aggregate_blocking = _sig("xp_aggregate_blocking", C.c_int,
+                           [C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, C.POINTER(C.c_int)])
 _m_block_rows = _sig("xp_matrix_block_rows", C.c_int, [_vp, _i32p])
 _bs_append = _sig("xp_blockset_append", C.c_int, [_vp, _vp])
 _bs_size = _sig("xp_blockset_size", C.c_int, [_vp])
@@ -569,7 +571,7 @@ class Matrix:
     def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0,
                  spmv_variant: int = 0, block_rows: int = 1, blocking: int = 0):
         """blocking (with block_rows = 8): 0 blocks of 8 consecutive rows,
-        1 greedy strength aggregates."""
+        1 greedy strength aggregates, 2 strength-ordered aggregates."""
         if isinstance(systems, Csr):
             systems = [systems]
         self.systems = list(systems)
@@ -896,11 +898,17 @@ def blockset(systems, blocking: int = 0) -> BlockSet:
     return out
 
 
-def strength_blocking(a: Csr) -> np.ndarray:
-    """Greedy strength aggregation of a (no GPU): nblocks x 8 local rows, -1 padding."""
+def strength_blocking(a: Csr, rule: int = 1) -> np.ndarray:
+    """Greedy strength aggregation of a (no GPU): nblocks x 8 local rows, -1
+    padding.  rule 1: seeds in row order, growth by summed strength; rule 2:
+    seeds by decreasing total strength, growth by the fraction of the
+    candidate's strength the block holds."""
     slots = np.empty(8 * max(a.n, 1), np.int32)
     nb = C.c_int()
-    _check(_strength_blocking(a.n, a.row_ptr, a.col_idx, a.values, slots, C.byref(nb)))
+    if rule == 1:
+        _check(_strength_blocking(a.n, a.row_ptr, a.col_idx, a.values, slots, C.byref(nb)))
+    else:
+        _check(_aggregate_blocking(a.n, a.row_ptr, a.col_idx, a.values, rule, slots, C.byref(nb)))
     return slots[:8 * nb.value].reshape(-1, 8)
 
 

@@ -103,6 +103,70 @@ def test_library_blocking_matches_oracle(case):
     assert got == ref
 
 
+@pytest.mark.parametrize("case", ["c1", "fem", "random", "chain_ragged"])
+def test_library_ordered_blocking_matches_oracle(case):
+    if case == "c1":
+        a = X.gen_diffusion2d(32, 32, 1.0, 20265001)
+    elif case == "fem":
+        a = X.gen_fem2d(34, 1.0, 4, 1e3, 0.3, 5)
+    elif case == "random":
+        a = csr(random_spd(300, 11))
+    else:
+        a = X.gen_diffusion2d(13, 11, 1.0, 7)
+    ref = BK.strength_blocking(a.row_ptr, a.col_idx, a.values, rule=2)
+    got = X.strength_blocking(a, rule=2)
+    got = [[int(r) for r in row if r >= 0] for row in got]
+    check_partition(got, a.n)
+    assert got == ref
+    assert ref != BK.strength_blocking(a.row_ptr, a.col_idx, a.values)
+
+
+def test_oracle_ordered_first_block_brute_force():
+    """Rule 2 replayed with dense numpy: the first seed is the row of largest
+    total strength, and each added row has the largest fraction of its total
+    strength going to the rows already in the block."""
+    for seed in (7, 8, 9):
+        A = random_spd(60, seed)
+        a = csr(A)
+        s = np.abs(A) / np.sqrt(np.outer(np.diag(A), np.diag(A)))
+        np.fill_diagonal(s, 0.0)
+        t = s.sum(1)
+        blocks = BK.strength_blocking(a.row_ptr, a.col_idx, a.values, rule=2)
+        check_partition(blocks, 60)
+        blk = [int(np.argmax(t))]
+        while len(blk) < 8:
+            w = s[blk].sum(0) / t
+            w[blk] = -1.0
+            w[s[blk].sum(0) == 0.0] = -1.0         # only neighbours of the block are candidates
+            if w.max() < 0.0:
+                break
+            blk.append(int(np.flatnonzero(w == w.max())[0]))
+        if len(blk) == 8:
+            assert sorted(blk) in blocks
+
+
+def test_oracle_ordered_chain_keeps_strong_edge():
+    """Hand-derived case.  Path graph, n = 16, unit couplings except 50 on
+    edge 7-8, diagonal = couplings + 1.  Strengths: interior 1/3, ends
+    s01 = 1/sqrt(6), strong edge 50/52, its neighbours 1/sqrt(156); totals
+    t7 = t8 = 1.04 (largest), t1 = t14 = 0.741, t0 = 0.408.  Rule 2 seeds row
+    7, takes 8 (fraction 0.92), then 6 (0.19, tie with 9 -> smaller), then
+    5, 4, 3, 2 (0.5 each) and 1 (0.45 > 0.19): block 1..8.  Seed 14 grows
+    15, 13 .. 9 (7 rows, no candidate left), seed 0 stays alone; pass 2
+    regroups the same two pieces and packing merges them.  Rule 1 cuts the
+    strong edge (runs 0..7, 8..15)."""
+    n = 16
+    w = np.ones(n - 1)
+    w[7] = 50.0
+    A = np.diag(np.r_[w, 0] + np.r_[0, w] + 1.0) - np.diag(w, 1) - np.diag(w, -1)
+    a = csr(A)
+    got = BK.strength_blocking(a.row_ptr, a.col_idx, a.values, rule=2)
+    assert got == [[0, 9, 10, 11, 12, 13, 14, 15], list(range(1, 9))]
+    assert BK.strength_blocking(a.row_ptr, a.col_idx, a.values) == [list(range(8)), list(range(8, 16))]
+    lib = [[int(r) for r in row if r >= 0] for row in X.strength_blocking(a, rule=2)]
+    assert lib == got
+
+
 def test_group_features_reduce_to_natural():
     a = X.gen_fem2d(20, 1.0, 2, 1e3, 0.3, 9)
     f1 = B.features_from_csr(a.row_ptr, a.col_idx, a.values)
@@ -128,7 +192,8 @@ def test_aggregates_beat_strips_with_exact_blocks():
     tau = (1e-6 * np.linalg.norm(a.b)) ** 2
     its = {}
     for name, groups in (("strips", B.natural_groups(a.n)),
-                         ("aggregates", BK.strength_blocking(a.row_ptr, a.col_idx, a.values))):
+                         ("aggregates", BK.strength_blocking(a.row_ptr, a.col_idx, a.values)),
+                         ("ordered", BK.strength_blocking(a.row_ptr, a.col_idx, a.values, rule=2))):
         blocks = np.zeros((len(groups), 8, 8))
         for k, g in enumerate(groups):
             m = len(g)
@@ -137,3 +202,4 @@ def test_aggregates_beat_strips_with_exact_blocks():
         its[name] = it
     print(its)
     assert its["aggregates"] < 0.9 * its["strips"]
+    assert its["ordered"] < its["aggregates"]

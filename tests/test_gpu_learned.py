@@ -118,7 +118,7 @@ def test_untrained_network_gives_exact_block_inverses(gpu):
     blocks are D^-1/2 (Ahat^-1 + 1e-3 I) D^-1/2: the device's reverse
     Cholesky factor L0 against numpy's inverse."""
     a = X.gen_fem2d(30, 1.0, 3, 1e3, 0.3, 41)
-    for blocking in (0, 1):
+    for blocking in (0, 1, 2):
         bs = X.blockset(a, blocking)
         net = X.Mlp(seed=1)
         net.set_params(np.zeros(X.Mlp.num_params(), np.float32))
@@ -143,17 +143,18 @@ def test_training_is_deterministic_and_descends(gpu):
     assert np.mean(h1[-10:]) < 0.95 * np.mean(h1[:5])
 
 
-@pytest.mark.parametrize("which,blocking", [("c1", 0), ("fem", 0), ("c1", 1), ("fem", 1)])
+@pytest.mark.parametrize("which,blocking", [("c1", 0), ("fem", 0), ("c1", 1), ("fem", 1), ("fem", 2)])
 def test_block_pcg_matches_oracle(gpu, which, blocking):
     """Full solves on the config-1 matrix; on the 1e3-contrast FEM matrix FP32
     storage moves the iterates (as for plain Jacobi), so there the bars are
-    checked at a fixed K = 20.  blocking 1: strength aggregates (the oracle
-    gets the oracle's own aggregation)."""
+    checked at a fixed K = 20.  blocking 1 / 2: strength aggregates, rule 1 /
+    2 (the oracle gets the oracle's own aggregation)."""
     a = X.gen_diffusion2d(32, 32, 1.0, 20265001) if which == "c1" else X.gen_fem2d(34, 1.0, 4, 1e3, 0.3, 7)
     tau = 1e-12 * float(a.b @ a.b)
     K = None if which == "c1" else 20
     m = X.Matrix(a, block_rows=8, keep_lo=True, blocking=blocking)
-    groups = BK.strength_blocking(a.row_ptr, a.col_idx, a.values) if blocking else B.natural_groups(a.n)
+    groups = (BK.strength_blocking(a.row_ptr, a.col_idx, a.values, rule=blocking) if blocking
+              else B.natural_groups(a.n))
     rows = m.block_rows()
     assert [[int(r) for r in row if r >= 0] for row in rows] == groups
     # exact block inverses (mode 1) and arbitrary SPD blocks from a network (mode 2)
@@ -195,7 +196,7 @@ def test_block_pcg_matches_oracle(gpu, which, blocking):
         np.testing.assert_allclose(tr[:k, 3][early], tr_ref[:k][early], rtol=1e-4)
 
 
-@pytest.mark.parametrize("case", ["c1", "fem", "ragged", "batch", "fem_agg", "batch_agg"])
+@pytest.mark.parametrize("case", ["c1", "fem", "ragged", "batch", "fem_agg", "batch_agg", "fem_ord", "batch_ord"])
 def test_device_features_match_oracle(gpu, case):
     """Features computed on the GPU from the SELL matrix (solve order, mapped
     back to natural block order) against the oracle's features_from_csr."""
@@ -205,16 +206,16 @@ def test_device_features_match_oracle(gpu, case):
         systems = [X.gen_fem2d(40, 1.0, 4, 1e3, 0.3, 31)]
     elif case == "ragged":                                  # n = 143: a partial last block
         systems = [X.gen_diffusion2d(13, 11, 1.0, 7)]
-    elif case == "fem_agg":
+    elif case in ("fem_agg", "fem_ord"):
         systems = [X.gen_fem2d(40, 1.0, 4, 1e3, 0.3, 31)]
     else:                                                    # two systems of uneven size
         systems = [X.gen_diffusion2d(13, 11, 1.0, 8), X.gen_fem2d(30, 1.0, 3, 1e3, 0.3, 9)]
-    blocking = 1 if case.endswith("_agg") else 0
+    blocking = 1 if case.endswith("_agg") else (2 if case.endswith("_ord") else 0)
     m = X.Matrix(systems, sigma=256, block_rows=8, keep_lo=True, blocking=blocking)
     feat, scale = m.block_features()
     ref_f, ref_s = [], []
     for a in systems:
-        groups = BK.strength_blocking(a.row_ptr, a.col_idx, a.values) if blocking else None
+        groups = BK.strength_blocking(a.row_ptr, a.col_idx, a.values, rule=blocking) if blocking else None
         f, _, _, sc = B.features_from_csr(a.row_ptr, a.col_idx, a.values, groups)
         ref_f.append(f)
         ref_s.append(sc)
@@ -225,7 +226,7 @@ def test_device_features_match_oracle(gpu, case):
     np.testing.assert_allclose(scale, ref_s, rtol=1e-6)
 
 
-@pytest.mark.parametrize("blocking", [0, 1])
+@pytest.mark.parametrize("blocking", [0, 1, 2])
 def test_device_mlp_preconditioner_matches_host_blocks(gpu, blocking):
     """set_mlp_preconditioner (device features + inference + solve-order
     blocks) against mode 2 with host-built blocks of the same network."""
