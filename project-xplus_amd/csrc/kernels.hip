@@ -455,7 +455,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv(Launch L, SellView A, co
         v[0] += static_cast<double>(pd) * static_cast<double>(y);
     }
     block_sum<1>(v);
-    if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
+    if (threadIdx.x == 0) put_partial(L, L.pap_slot, lb, v[0]);
 }
 
 // Rows handled per thread in one pass of the vector kernels: every load of a
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
         if (first) {
             first = false;
             double t[1];
-            sum_partials<1>(L, w.sys, {kSlotPap}, t);
+            sum_partials<1>(L, w.sys, {L.pap_slot}, t);
             if (!active) return;
             alpha = rz_old / t[0];
             if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;
@@ -621,7 +621,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_p(Launch L, const SysS
         if (first) {
             first = false;
             double t[3];
-            sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);  // ends with a barrier
+            sum_partials<3>(L, w.sys, {L.pap_slot, kSlotRz, kSlotRr}, t);  // ends with a barrier
             if (!s_sh.active) {
                 if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;  // carry the stopped state forward
                 return;
@@ -667,7 +667,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
     // barrier-free streaming loop.
     if (threadIdx.x == 0) s_sh = Sc[w.sys];
     double t[3];
-    sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);  // ends with a barrier
+    sum_partials<3>(L, w.sys, {L.pap_prev, kSlotRz, kSlotRr}, t);  // ends with a barrier
     if (!s_sh.active) {
         if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
         return;
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
         v[0] += static_cast<double>(pn) * static_cast<double>(an);
     }
     block_sum<1>(v);
-    if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
+    if (threadIdx.x == 0) put_partial(L, L.pap_slot, lb, v[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -749,7 +749,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
         az0 = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd0);
     }
     double t[3];
-    sum_partials<3>(L, w.sys, {kSlotPap, kSlotRz, kSlotRr}, t);  // ends with a barrier
+    sum_partials<3>(L, w.sys, {L.pap_prev, kSlotRz, kSlotRr}, t);  // ends with a barrier
     if (!s_sh.active) {
         if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
         return;
@@ -800,7 +800,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
         v[0] += pn * an;
     }
     block_sum<1>(v);
-    if (threadIdx.x == 0) put_partial(L, kSlotPap, lb, v[0]);
+    if (threadIdx.x == 0) put_partial(L, L.pap_slot, lb, v[0]);
 }
 
 // alpha = rz / pAp; x += alpha p (FP64); r = fp32(r - alpha ap); z = M r;
@@ -845,7 +845,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
         if (first) {
             first = false;
             double t[1];
-            sum_partials<1>(L, w.sys, {kSlotPap}, t);
+            sum_partials<1>(L, w.sys, {L.pap_slot}, t);
             if (!active) return;
             alpha = rz_old / t[0];
             if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;

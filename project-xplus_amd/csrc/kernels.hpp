@@ -21,6 +21,13 @@ struct Launch {
     int u_blocks = 0;                   // > 0: one system, uniform split of u_slices over u_blocks
     int u_slices = 0;
     hipStream_t stream = nullptr;
+    // pAp partial slots of the PCG iteration: this iteration's (pap_slot)
+    // and the previous one's (pap_prev); set_parity alternates them.
+    int pap_slot = kSlotPap, pap_prev = kSlotPap1;
+    void set_parity(int k) {
+        pap_slot = (k & 1) ? kSlotPap1 : kSlotPap;
+        pap_prev = (k & 1) ? kSlotPap : kSlotPap1;
+    }
     // Per-kernel timing only (never under graph capture): when set, the
     // iteration and SpMV launchers dispatch their kernel with these events
     // recorded at its start and end (hipExtLaunchKernelGGL).

@@ -412,11 +412,13 @@ struct xp_pcg {
     int acc_vmode() const { return m->has64 ? 0 : (opts.accurate == 2 || tight ? 1 : 2); }
     bool tight = false;
 
-    // Iteration k reads state slot k % 2 and writes slot (k + 1) % 2; the
-    // graph holds an even number of iterations, so slot 0 is current
+    // Iteration k reads state slot k % 2 and writes slot (k + 1) % 2, and
+    // puts its pAp partials in the slot of parity k (Launch::set_parity);
+    // the graph holds an even number of iterations, so slot 0 is current
     // between launches.
     void enqueue_iteration(int k) {
-        const xp::Launch L = m->launch(stream);
+        xp::Launch L = m->launch(stream);
+        L.set_parity(k);
         const xp::SellView A = m->view();
         const size_t ns = static_cast<size_t>(m->nsys);
         xp::SysState* cur = st.p + (k % 2) * ns;
@@ -1235,7 +1237,9 @@ int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels) {
         const size_t ns = static_cast<size_t>(m.nsys);
         for (int k = 0; k < iters; ++k) {
             hipEvent_t* e = ev.data() + static_cast<size_t>(k) * static_cast<size_t>(2 * nk);
-            xp::Launch L1 = m.launch(S.stream), L2 = L1, L3 = L1;
+            xp::Launch L1 = m.launch(S.stream);
+            L1.set_parity(k);
+            xp::Launch L2 = L1, L3 = L1;
             L1.ev_start = e[0]; L1.ev_stop = e[1];
             L2.ev_start = e[2]; L2.ev_stop = e[3];
             if (nk == 3) { L3.ev_start = e[4]; L3.ev_stop = e[5]; }

@@ -37,7 +37,11 @@ struct WorkItem {
 };
 
 // Reduction slots of the per-workgroup partial sums (partials[slot * grid + block]).
-enum : int { kSlotPap = 0, kSlotRz = 1, kSlotRr = 2, kSlotAux = 3, kNumSlots = 4 };
+// Partial slots.  pAp alternates between kSlotPap and kSlotPap1 by
+// iteration parity (Launch::pap_slot): the fused kernel reads the previous
+// iteration's pAp partials in its prologue while its own workgroups write
+// the new ones, so the two must not share a slot.
+enum : int { kSlotPap = 0, kSlotRz = 1, kSlotRr = 2, kSlotAux = 3, kSlotPap1 = 4, kNumSlots = 5 };
 
 // Per-system PCG state machine, resident on the device.  Written only by the
 // last workgroup of a reduction (see kernels.hip), read by every kernel of

@@ -222,6 +222,36 @@ def test_solver_survives_workgroup_change(gpu):
         assert np.linalg.norm(s.x() - x1) <= 1e-4 * np.linalg.norm(x1)
 
 
+@pytest.mark.parametrize("form", ["three", "fused", "accurate"])
+def test_more_workgroups_than_resident(gpu, form):
+    """2048 workgroups of 1024 threads are four times what 256 CUs hold at
+    once, so late workgroups start after early ones have finished.  The fused
+    kernel reads the previous iteration's pAp partials while its own
+    workgroups write the new ones (alternating slots).  Against 256
+    workgroups on a 400 x 400 diffusion matrix: the first 50 trace rows to
+    1e-4 relative, both converged, iteration counts within 3 % (the
+    reductions are grouped by workgroup, so the FP32 iterates of the two runs
+    drift apart slowly; on the 1e3-contrast FEM matrix to 1e-5 the FP32 fused
+    form measured 6,126 vs 6,676 iterations)."""
+    a = X.gen_diffusion2d(400, 400, 1.0, 20264402)
+    res = {}
+    for g in (256, 2048):
+        m = X.Matrix(a, keep_lo=True)
+        assert m.info()["nslices"] >= 2048
+        m.set_grid(g)
+        assert m.info()["grid"] == g
+        s = X.Pcg(m, rel_tol=1e-4, trace_cap=50, fused=form == "fused", accurate=form == "accurate")
+        s.set_rhs(a.b)
+        s.run()
+        res[g] = (s.status(), s.trace())
+        print(f"{form} grid {g}: {res[g][0]['iterations']} iterations", flush=True)
+    (s1, t1), (s2, t2) = res[256], res[2048]
+    assert s1["status"] == s2["status"] == X.CONVERGED
+    assert abs(s1["iterations"] - s2["iterations"]) <= max(2, 0.03 * s1["iterations"])
+    assert len(t1) == len(t2) == 50
+    np.testing.assert_allclose(t2, t1, rtol=1e-4)
+
+
 @pytest.mark.parametrize("delta", [0.1, 0.0])
 def test_accurate_to_1e10_against_fp64_oracle(gpu, delta):
     """The refinement target of the FP64-refined configuration: true relative
