@@ -129,6 +129,15 @@ __device__ __forceinline__ void put_partial(const Launch& L, int slot, int lb, d
     L.partials[static_cast<size_t>(slot) * L.grid + lb] = v;
 }
 
+// A fused kernel whose system does not iterate carries its workgroup's pAp
+// partial into this launch's slot: a residual replacement may reactivate a
+// converged system and rerun its last scalar step, which then reads the
+// slot of whatever parity comes next.
+__device__ __forceinline__ void carry_pap(const Launch& L, int lb) {
+    if (threadIdx.x == 0)
+        put_partial(L, L.pap_slot, lb, L.partials[static_cast<size_t>(L.pap_prev) * L.grid + lb]);
+}
+
 __device__ __forceinline__ bool bad(double v) { return !isfinite(v); }
 
 // One SELL row: y = sum_j val * x[col] in FP32, diagonal first.  xdiag
@@ -670,6 +679,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
     sum_partials<3>(L, w.sys, {L.pap_prev, kSlotRz, kSlotRr}, t);  // ends with a barrier
     if (!s_sh.active) {
         if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
+        carry_pap(L, lb);
         return;
     }
     const bool fresh = s_sh.fresh != 0;
@@ -687,7 +697,10 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
         if (leader_block) Sn[w.sys] = s_sh;
     }
     __syncthreads();
-    if (!s_sh.active) return;
+    if (!s_sh.active) {
+        carry_pap(L, lb);
+        return;
+    }
     const double beta = s_sh.beta;
     XP_WAVE_LANE
     double v[1] = {0.0};
@@ -752,6 +765,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
     sum_partials<3>(L, w.sys, {L.pap_prev, kSlotRz, kSlotRr}, t);  // ends with a barrier
     if (!s_sh.active) {
         if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
+        carry_pap(L, lb);
         return;
     }
     const bool fresh = s_sh.fresh != 0;
@@ -769,7 +783,10 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
         if (leader_block) Sn[w.sys] = s_sh;
     }
     __syncthreads();
-    if (!s_sh.active) return;
+    if (!s_sh.active) {
+        carry_pap(L, lb);
+        return;
+    }
     const double beta = s_sh.beta;
     double v[1] = {0.0};
     if (have0) {

@@ -1153,7 +1153,10 @@ int xp_pcg_run_fixed(xp_pcg* s, int iters) {
         std::vector<xp::SysState> init = s->st_init;
         for (auto& t : init) t.max_iters = std::min(t.max_iters, std::max(iters, 0));
         s->st.upload(init.data(), init.size(), s->stream);
-        s->run_loop(iters);
+        // + 1: in the fused forms step k is completed (stop tests, trace) by
+        // the first kernel of iteration k + 1, which may sit in the next
+        // graph launch
+        s->run_loop(static_cast<long long>(iters) + 1);
     });
 }
 

@@ -222,6 +222,33 @@ def test_solver_survives_workgroup_change(gpu):
         assert np.linalg.norm(s.x() - x1) <= 1e-4 * np.linalg.norm(x1)
 
 
+@pytest.mark.parametrize("K", [1, 2, 5, 63, 64, 65, 128])
+def test_accurate_fixed_count_x(gpu, K):
+    """K steps of the accurate iteration, with K around the 64-iteration graph:
+    step k is completed (stop tests, trace) by the first kernel of iteration
+    k + 1, so K = 64 needs the first kernel of a second graph launch.  Status
+    MAX_ITER after exactly K steps, and x_K against the FP64 golden's x_K to
+    1e-6 relative on a 96 x 96 diffusion matrix, where the FP32 r and z of
+    this iteration track the golden to ~6e-8 (CPU simulation) while a lost or
+    repeated last step moves x by |x_K - x_(K-1)| >= 1.4e-3 relative
+    (checked here from the golden)."""
+    a = X.gen_diffusion2d(96, 96, 0.5, 20264403)
+    g = O.golden(a, tau=1e-300, max_iters=K)
+    prev = O.golden(a, tau=1e-300, max_iters=K - 1)["x"] if K > 1 else a.x0
+    last = np.linalg.norm(g["x"] - prev) / np.linalg.norm(g["x"])
+    assert last >= 1e-3
+    m = X.Matrix(a, keep_lo=True)
+    s = X.Pcg(m, tau=1e-300, trace_cap=K, accurate=True, graph_iters=64)
+    s.set_rhs(a.b, a.x0)
+    s.run_fixed(K)
+    st = s.status()
+    rel = np.linalg.norm(s.x() - g["x"]) / np.linalg.norm(g["x"])
+    print(f"K={K}: iterations {st['iterations']} status {st['status']} rel {rel:.2e} (last step {last:.1e})",
+          flush=True)
+    assert st["iterations"] == K and st["status"] == X.MAX_ITER
+    assert rel <= 1e-6
+
+
 @pytest.mark.parametrize("form", ["three", "fused", "accurate"])
 def test_more_workgroups_than_resident(gpu, form):
     """2048 workgroups of 1024 threads are four times what 256 CUs hold at
