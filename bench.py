@@ -19,7 +19,8 @@ on the GPU) with the matrix, b and x0 resident in HBM:
   refined   FP32 Jacobi-PCG inner solves restarted from FP64 outer residuals;
   plain     FP32 Jacobi-PCG stopped on its recursive residual (its true
             residual is reported: FP32 arithmetic does not reach 1e-6 on this
-            matrix).
+            matrix), timed in the three-kernel form (the reference's) and the
+            fused form; the faster is the baseline of the speedup.
 
 Prints one JSON line.
 
@@ -424,21 +425,34 @@ def main() -> int:
     tune_f = m.autotune(fused=True, tune_sigma=not args.no_tune_sigma, iters=64)
     tune_u = m.autotune(fused=False, tune_sigma=False, iters=64)
     fused = tune_f["us_per_iter"] < tune_u["us_per_iter"]
-    if fused:
-        m.autotune(fused=True, tune_sigma=not args.no_tune_sigma, iters=64)  # cached: re-applies the fused choice
     tune_ms = (time.perf_counter() - t0) * 1000.0
     info = m.info()
     log(f"autotune ({tune_ms:.0f} ms): fused={tune_f} unfused={tune_u} -> fused={fused}; matrix {info}")
 
-    # plain FP32 Jacobi-PCG to recursive relative residual 1e-6
-    s = X.Pcg(m, rel_tol=1e-6, fused=fused)
-    s.set_rhs(a.b, a.x0)
-    plain_ms, _ = timed(X, s.run, args.warmup, args.steps)
-    st = s.status()
-    plain_true = float(s.true_residual()[0] / bnorm)
-    log(f"plain: {st['iterations']} iterations status={X.STATUS_NAMES[st['status']]} {plain_ms:.2f} ms/solve, "
-        f"true rel residual {plain_true:.3e}")
-    s.close()
+    # plain FP32 Jacobi-PCG to recursive relative residual 1e-6, in both
+    # iteration forms, each on its own tuned layout (the tuner's choices are
+    # cached: calling it again re-applies them).  The three-kernel form is
+    # the reference's (A p by an SpMV of p); the fused form carries A p by
+    # recurrence, which in FP32 drifts and costs iterations on this matrix.
+    # The baseline is the faster of the two.
+    plain = {}
+    for form in ("unfused", "fused"):
+        if form == "fused":
+            m.autotune(fused=True, tune_sigma=not args.no_tune_sigma, iters=64)
+        s = X.Pcg(m, rel_tol=1e-6, fused=form == "fused")
+        s.set_rhs(a.b, a.x0)
+        ms, _ = timed(X, s.run, args.warmup, args.steps)
+        st = s.status()
+        plain[form] = {"ms": round(ms, 3), "iterations": st["iterations"], "status": X.STATUS_NAMES[st["status"]],
+                       "true_rel_residual": float(s.true_residual()[0] / bnorm)}
+        s.close()
+        log(f"plain ({form}): {plain[form]}")
+    plain_form = min(plain, key=lambda k: plain[k]["ms"])
+    plain_ms = plain[plain_form]["ms"]
+    st = {"iterations": plain[plain_form]["iterations"]}
+    plain_true = plain[plain_form]["true_rel_residual"]
+    if not fused:
+        m.autotune(fused=False, tune_sigma=False, iters=64)  # the refined solve's layout
 
     # refined: FP32 inner Jacobi-PCG restarted from FP64 outer residuals
     r = X.Pcg(m, rel_tol=1e-6, fused=fused)
@@ -518,6 +532,8 @@ def main() -> int:
         "plain_ms": round(plain_ms, 3),
         "plain_iterations": st["iterations"],
         "plain_true_rel_residual": plain_true,
+        "plain_form": plain_form,
+        "plain_forms": plain,
         "speedup_learned_vs_plain": round(plain_ms / wall_ms, 3),
         "speedup_learned_vs_refined_jacobi": round(ref_ms / wall_ms, 3),
         "speedup_learned_vs_accurate_jacobi": round(acc["total_ms"] / wall_ms, 3),
