@@ -922,7 +922,9 @@ int xp_spmv_f64(xp_matrix* m, const double* x, double* y, void* stream) {
 int xp_init_pcg_f32(xp_matrix* m, const float* b, const float* ax, const float* minv, float* r, float* z, float* p,
                     double* metrics, void* stream) {
     return guarded([&] {
-        if (m == nullptr || metrics == nullptr) throw std::runtime_error("null argument");
+        if (m == nullptr || b == nullptr || ax == nullptr || minv == nullptr || r == nullptr || z == nullptr ||
+            p == nullptr || metrics == nullptr)
+            throw std::runtime_error("null argument");
         xp::launch_op_init(m->launch(pick_stream(stream)), m->nsys, b, ax, minv, r, z, p, metrics);
         HIPX(hipGetLastError());
     });
@@ -930,7 +932,7 @@ int xp_init_pcg_f32(xp_matrix* m, const float* b, const float* ax, const float* 
 
 int xp_dot_f32(xp_matrix* m, const float* a, const float* b, double* out, void* stream) {
     return guarded([&] {
-        if (m == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        if (m == nullptr || a == nullptr || b == nullptr || out == nullptr) throw std::runtime_error("null argument");
         xp::launch_op_dot(m->launch(pick_stream(stream)), m->nsys, a, b, out);
         HIPX(hipGetLastError());
     });
@@ -939,7 +941,9 @@ int xp_dot_f32(xp_matrix* m, const float* a, const float* b, double* out, void* 
 int xp_update_xrz_f32(xp_matrix* m, float* x, const float* p, float* r, const float* ap, const float* minv, float* z,
                       double* metrics, double alpha, void* stream) {
     return guarded([&] {
-        if (m == nullptr || metrics == nullptr) throw std::runtime_error("null argument");
+        if (m == nullptr || x == nullptr || p == nullptr || r == nullptr || ap == nullptr || minv == nullptr ||
+            z == nullptr || metrics == nullptr)
+            throw std::runtime_error("null argument");
         xp::launch_op_update_xrz(m->launch(pick_stream(stream)), m->nsys, x, p, r, ap, minv, z, metrics, alpha);
         HIPX(hipGetLastError());
     });
@@ -947,7 +951,7 @@ int xp_update_xrz_f32(xp_matrix* m, float* x, const float* p, float* r, const fl
 
 int xp_update_p_f32(xp_matrix* m, const float* z, float* p, double beta, void* stream) {
     return guarded([&] {
-        if (m == nullptr) throw std::runtime_error("null argument");
+        if (m == nullptr || z == nullptr || p == nullptr) throw std::runtime_error("null argument");
         xp::launch_op_update_p(m->launch(pick_stream(stream)), z, p, beta);
         HIPX(hipGetLastError());
     });

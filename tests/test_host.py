@@ -51,6 +51,18 @@ def test_no_gpu_fails_loudly():
     assert "no GPU visible" in X.last_error()
 
 
+def test_device_ops_reject_null_arguments():
+    """The stream-ordered FP32 operations check their pointers before any
+    device work (no GPU needed): a null matrix or vector is an error."""
+    import xplus_amd as XA
+    assert XA._op_dot(None, None, None, None, None) == -1
+    assert "null argument" in X.last_error()
+    assert XA._op_init(None, None, None, None, None, None, None, None, None) == -1
+    assert XA._op_xrz(None, None, None, None, None, None, None, None, 0.5, None) == -1
+    assert XA._op_p(None, None, None, 0.5, None) == -1
+    assert "null argument" in X.last_error()
+
+
 # ------------------------------------------------------------------ generators
 def test_diffusion2d_structure():
     a = X.gen_diffusion2d(32, 32, 1.0, 20261204)
