@@ -10,7 +10,7 @@
  *   2. Device-resident matrices and stream-ordered operations (xp_*).
  *   3. The solver: graph-captured Jacobi-PCG, batched systems, FP64
  *      refinement, status and trace readback.
- * Plus host helpers (dataset loader, generators) used by the drivers.
+ * Plus host helpers (dataset loader, generators) used by the command-line programs.
  *
  * Unless stated otherwise an xp_* function returns 0 on success and a
  * negative code on failure; xp_last_error() then returns the message.
