@@ -2262,9 +2262,15 @@ extern "C" int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int i
         uint64_t h = mix_hash(0x7475ull, m->h_row_ptr.data(), m->h_row_ptr.size() * sizeof(int));
         h = mix_hash(h, m->h_col_idx.data(), m->h_col_idx.size() * sizeof(int));
         h = mix_hash(h, m->h_values.data(), m->h_values.size() * sizeof(double));
+        // The layout a choice belongs to: block rows and blocking rule, value
+        // streams, and the sort window when it is not itself tuned (a cached
+        // choice re-applies its sigma).
         const std::vector<uint64_t> key = {static_cast<uint64_t>(m->n), static_cast<uint64_t>(m->nnz), h,
                                            static_cast<uint64_t>(fused), static_cast<uint64_t>(tune_sigma),
-                                           static_cast<uint64_t>(m->nsys)};
+                                           static_cast<uint64_t>(m->nsys), static_cast<uint64_t>(m->block_rows),
+                                           static_cast<uint64_t>(m->blocking),
+                                           static_cast<uint64_t>(m->keep_lo) | (static_cast<uint64_t>(m->keep64) << 1),
+                                           static_cast<uint64_t>(tune_sigma ? 0 : m->sigma)};
         TuneChoice best{m->sigma, m->variant, m->grid, 1e300, 0.0, 0};
         bool cached = false;
         for (const auto& kv : g_tune_cache)

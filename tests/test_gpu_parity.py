@@ -268,6 +268,23 @@ def test_pcg_fem_fixed_iterations(gpu):
         np.testing.assert_allclose(s.trace()[:, 3], g["trace"][:, 4], rtol=1e-3)
 
 
+def test_autotune_cache_is_per_layout(gpu):
+    """The tuner's cache is keyed by the layout as well as the matrix: a
+    matrix built with another sort window, or with 8-row blocks, gets its own
+    choice, and without sigma tuning a matrix keeps its window."""
+    a = X.gen_fem2d(80, 1.0, 6, 1e3, 0.3, 20264405)
+    seen = []
+    for sigma, block_rows in ((64, 1), (1024, 1), (1024, 8)):
+        m = X.Matrix(a, sigma=sigma, keep_lo=True, block_rows=block_rows, blocking=2 if block_rows == 8 else 0)
+        m.autotune(fused=True, tune_sigma=False, iters=16)
+        info = m.info()
+        seen.append((info["sigma"], info["stored"]))
+        assert info["sigma"] == sigma
+        m.close()
+    print("layouts:", seen, flush=True)
+    assert seen[0][1] >= seen[1][1]          # a wider window stores no more padding
+
+
 def test_autotune_keeps_results(gpu):
     """The tuner only changes the layout (sigma, inner loop, workgroups): the
     solve still meets the golden bars afterwards, and a second call hits the cache."""
