@@ -121,6 +121,23 @@ def test_library_ordered_blocking_matches_oracle(case):
     assert ref != BK.strength_blocking(a.row_ptr, a.col_idx, a.values)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_library_ordered_blocking_matches_oracle_random(seed):
+    """Rule 2 on random matrices of assorted sizes and densities, including
+    exact ties in the strengths (integer values): library == oracle."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(9, 400))
+    A = random_spd(n, 200 + seed, density=float(rng.uniform(0.01, 0.1)))
+    if seed % 2:
+        A = np.round(A)                     # many equal strengths: the tie rules decide
+        np.fill_diagonal(A, np.abs(A).sum(1) + 1.0)
+    a = csr(A)
+    ref = BK.strength_blocking(a.row_ptr, a.col_idx, a.values, rule=2)
+    got = [[int(r) for r in row if r >= 0] for row in X.strength_blocking(a, rule=2)]
+    check_partition(got, n)
+    assert got == ref
+
+
 def test_oracle_ordered_first_block_brute_force():
     """Rule 2 replayed with dense numpy: the first seed is the row of largest
     total strength, and each added row has the largest fraction of its total
