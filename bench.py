@@ -108,13 +108,14 @@ def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
     launches on the solver stream; the dominant one is k_pcg_spmv_fused_acc."""
     us = s.kernel_times(iters)
     b_fused = 12 * a.nnz + 40 * a.n
-    b_upd = (62 if block else 48) * a.n
+    b_upd = (54 if block else 40) * a.n
     r = roof("k_pcg_spmv_fused_acc", b_fused, us[0],
              "12 nnz + 40 n (FP64 value + int32 column; row ptr, z, p and A p read and written in FP64)")
     r["update_kernel"] = roof("k_pcg_update_xrz_acc", b_upd, us[1],
-                              ("62" if block else "48") + " n (x r/w FP64, p, A p FP64, r r/w, "
+                              ("54" if block else "40") + " n on average over an even and an odd iteration ("
+                              + ("38 / 70" if block else "24 / 56") + " n: r r/w, A p FP64, "
                               + ("the 36 FP32 entries of an SPD 8x8 block, 18 B per row" if block else "1/diag")
-                              + ", z)")
+                              + ", z; the odd one adds x r/w FP64 and two FP64 directions)")
     prof = profiled_traffic()
     for rr in (r, r["update_kernel"]):
         if rr["kernel"] in prof:

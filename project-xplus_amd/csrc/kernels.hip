@@ -433,6 +433,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_init_state(Launch L, SysState
     s.alpha = 0.0;
     s.beta = 0.0;
     s.fresh = 1;
+    s.x_pend = 0;
     if (s.rr <= s.tau) {
         s.status = kStatusConverged;
         s.active = 0;
@@ -740,7 +741,9 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
 template <bool IDX16, int VMODE>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, SellView A, const SysState* Sc,
                                                                   SysState* Sn, TraceRow* trace, int trace_stride,
-                                                                  const float* __restrict__ z, double* __restrict__ p,
+                                                                  const float* __restrict__ z,
+                                                                  const double* __restrict__ p_old,
+                                                                  double* __restrict__ p_new,
                                                                   double* __restrict__ ap) {
     __shared__ SysState s_sh;
     const int lb = logical_block();
@@ -757,7 +760,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
     float zd0 = 0.0f;
     if (have0) {
         const int row = sl * kC + lane;
-        po0 = p[row];
+        po0 = p_old[row];
         apo0 = ap[row];
         az0 = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd0);
     }
@@ -776,7 +779,14 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
             if (leader_block && trace != nullptr && s_sh.iters < s_sh.trace_cap)
                 tr = trace + static_cast<size_t>(w.sys) * static_cast<size_t>(trace_stride) +
                      static_cast<size_t>(s_sh.iters);
+            const double pap = t[0];
             pcg_scalar_step(s_sh, t, tr);
+            // Stopping here after an even update (this launch is odd) leaves
+            // that update's alpha p out of x, unless its alpha was unusable
+            // (then the update made no step either).
+            if (!s_sh.active && L.pap_slot == kSlotPap1 &&
+                !(bad(pap) || fabs(pap) <= kBreakdownEps || bad(s_sh.alpha)))
+                s_sh.x_pend = 1;
         } else {
             s_sh.fresh = 0;
         }
@@ -784,7 +794,13 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
     }
     __syncthreads();
     if (!s_sh.active) {
+        // both p buffers keep the last direction (a replacement may
+        // reactivate the system at either parity)
         carry_pap(L, lb);
+        for (int s2 = w.slice_begin + wave; s2 < w.slice_end; s2 += kWaves) {
+            const int row = s2 * kC + lane;
+            p_new[row] = p_old[row];
+        }
         return;
     }
     const double beta = s_sh.beta;
@@ -793,14 +809,14 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
         const int row = sl * kC + lane;
         const double pn = fresh ? static_cast<double>(zd0) : static_cast<double>(zd0) + beta * po0;
         const double an = fresh ? az0 : az0 + beta * apo0;
-        p[row] = pn;
+        p_new[row] = pn;
         ap[row] = an;
         v[0] += pn * an;
     }
     for (sl += kWaves; sl < w.slice_end; sl += kWaves) {
         const int row = sl * kC + lane;
         // the FP64 p and A p streams are requested before the row's gathers
-        const double po = fresh ? 0.0 : p[row];
+        const double po = fresh ? 0.0 : p_old[row];
         const double apo = fresh ? 0.0 : ap[row];
         float zd;
         const double az = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd);
@@ -812,7 +828,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
             pn = static_cast<double>(zd) + beta * po;
             an = az + beta * apo;
         }
-        p[row] = pn;
+        p_new[row] = pn;
         ap[row] = an;
         v[0] += pn * an;
     }
@@ -820,14 +836,20 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
     if (threadIdx.x == 0) put_partial(L, L.pap_slot, lb, v[0]);
 }
 
-// alpha = rz / pAp; x += alpha p (FP64); r = fp32(r - alpha ap); z = M r;
-// partials of rz_new, rr_new.  Rows in flight per thread: the FP64 streams
-// double the registers per row.
+// alpha = rz / pAp; r = fp32(r - alpha ap); z = M r; partials of rz_new,
+// rr_new; x in FP64 every second iteration.  XM = 0 (even iteration): x is
+// left alone; XM = 1 (odd): x += alpha_prev p_prev + alpha p, with alpha_prev
+// the previous update's (the state's, from the scalar step that closed it)
+// and p_prev its direction, still in the other p buffer: two steps' x
+// traffic (x read and written, two directions) instead of four.  A system
+// that stops after an even update keeps x_pend (k_pcg_flush_x_acc).  Rows in
+// flight per thread: the FP64 streams double the registers per row.
 constexpr int kRowsAcc = 2;
-template <bool BLOCK>
+template <bool BLOCK, int XM>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, const SysState* S,
                                                                double* __restrict__ x,
                                                                const double* __restrict__ p,
+                                                               const double* __restrict__ p_prev,
                                                                float* __restrict__ r,
                                                                const double* __restrict__ ap,
                                                                const float* __restrict__ minv,
@@ -839,20 +861,24 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
     if (w.sys < 0) return;
     const int active = S[w.sys].active;
     const double rz_old = S[w.sys].rz;
+    const double a_prev = XM ? S[w.sys].alpha : 0.0;
     const int r0 = w.slice_begin * kC, r1 = w.slice_end * kC;
     double v[2] = {0.0, 0.0};
     bool first = true;
     double alpha = 0.0;
     for (int base = r0; base < r1; base += ROWS * kBlock) {
-        double xv[ROWS], pv[ROWS], av[ROWS];
+        double xv[ROWS], pv[ROWS], qv[ROWS], av[ROWS];
         float rv[ROWS], mv[ROWS];
         BlockRow bl[ROWS];
 #pragma unroll
         for (int k = 0; k < ROWS; ++k) {
             const int row = base + k * kBlock + static_cast<int>(threadIdx.x);
             if (row < r1) {
-                xv[k] = x[row];
-                pv[k] = p[row];
+                if (XM) {
+                    xv[k] = x[row];
+                    pv[k] = p[row];
+                    qv[k] = p_prev[row];
+                }
                 rv[k] = r[row];
                 av[k] = ap[row];
                 mv[k] = BLOCK ? 0.0f : minv[row];
@@ -865,7 +891,13 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
             sum_partials<1>(L, w.sys, {L.pap_slot}, t);
             if (!active) return;
             alpha = rz_old / t[0];
-            if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) return;
+            if (bad(t[0]) || fabs(t[0]) <= kBreakdownEps || bad(alpha)) {
+                // no step here; the previous one still goes into x
+                if (XM)
+                    for (int row = r0 + static_cast<int>(threadIdx.x); row < r1; row += kBlock)
+                        x[row] += a_prev * p_prev[row];
+                return;
+            }
         }
 #pragma unroll
         for (int k = 0; k < ROWS; ++k) {
@@ -875,7 +907,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
                 const float zn = BLOCK
                                      ? block_apply_factor(bl[k].q, bl[k].s, row & 7, rn)
                                      : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
-                x[row] = xv[k] + alpha * pv[k];
+                if (XM) x[row] = (xv[k] + a_prev * qv[k]) + alpha * pv[k];
                 r[row] = rn;
                 z[row] = zn;
                 v[0] += static_cast<double>(rn) * static_cast<double>(zn);
@@ -888,6 +920,25 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
         put_partial(L, kSlotRz, lb, v[0]);
         put_partial(L, kSlotRr, lb, v[1]);
     }
+}
+
+// The x update a system still lacks after stopping behind an even update
+// (state.x_pend): x += alpha p with its last step's alpha and direction
+// (both p buffers hold it).  Then k_pcg_clear_x_pend, a kernel of its own so
+// that every workgroup here sees the flag set.
+__global__ __launch_bounds__(kBlock, 8) void k_pcg_flush_x_acc(Launch L, const SysState* S,
+                                                            const double* __restrict__ p, double* __restrict__ x) {
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    if (!S[w.sys].x_pend) return;
+    const double alpha = S[w.sys].alpha;
+    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock)
+        x[row] += alpha * p[row];
+}
+
+__global__ void k_pcg_clear_x_pend(int nsys, SysState* S) {
+    const int sys = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (sys < nsys) S[sys].x_pend = 0;
 }
 
 template <bool IDX16, int V>
@@ -1826,10 +1877,11 @@ void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysSta
 }
 
 void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, int vmode, const SysState* st_cur,
-                               SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
-                               double* ap) {
-#define XP_FUSED_ACC(I, V) \
-    XP_LAUNCH(L, (k_pcg_spmv_fused_acc<I, V>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p, ap)
+                               SysState* st_next, TraceRow* trace, int trace_stride, const float* z,
+                               const double* p_old, double* p_new, double* ap) {
+#define XP_FUSED_ACC(I, V)                                                                               \
+    XP_LAUNCH(L, (k_pcg_spmv_fused_acc<I, V>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p_old, \
+              p_new, ap)
     if (A.idx16) {
         if (vmode == 2) XP_FUSED_ACC(true, 2);
         else if (vmode == 1) XP_FUSED_ACC(true, 1);
@@ -1842,12 +1894,24 @@ void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, int vmode, co
 #undef XP_FUSED_ACC
 }
 
-void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,
-                               const double* ap, const float* minv, const BlockPre* blockM, float* z) {
-    if (blockM != nullptr)
-        XP_LAUNCH(L, k_pcg_update_xrz_acc<true>, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
-    else
-        XP_LAUNCH(L, k_pcg_update_xrz_acc<false>, dim3(L.grid), L, st, x, p, r, ap, minv, blockM, z);
+void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, bool odd, double* x, const double* p,
+                               const double* p_prev, float* r, const double* ap, const float* minv,
+                               const BlockPre* blockM, float* z) {
+#define XP_UPD_ACC(B, XM) \
+    XP_LAUNCH(L, (k_pcg_update_xrz_acc<B, XM>), dim3(L.grid), L, st, x, p, p_prev, r, ap, minv, blockM, z)
+    if (blockM != nullptr) {
+        if (odd) XP_UPD_ACC(true, 1);
+        else XP_UPD_ACC(true, 0);
+    } else {
+        if (odd) XP_UPD_ACC(false, 1);
+        else XP_UPD_ACC(false, 0);
+    }
+#undef XP_UPD_ACC
+}
+
+void launch_pcg_flush_x_acc(const Launch& L, int nsys, SysState* st, const double* p, double* x) {
+    hipLaunchKernelGGL(k_pcg_flush_x_acc, dim3(L.grid), dim3(kBlock), 0, L.stream, L, st, p, x);
+    hipLaunchKernelGGL(k_pcg_clear_x_pend, dim3((nsys + 255) / 256), dim3(256), 0, L.stream, nsys, st);
 }
 
 void launch_fold_f32_into_f64(int n_pad, float* d, double* x, hipStream_t s) {

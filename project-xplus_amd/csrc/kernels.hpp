@@ -59,12 +59,19 @@ void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_n
 // after init; the first fused iteration sets p = z, ap = A z (state.fresh).
 void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
                          const double* b, const float* minv, const BlockPre* blockM, float* r, float* z);
-// vmode: 0 FP64 values, 1 hi + lo (FP32 pair), 2 hi + lo16.
+// vmode: 0 FP64 values, 1 hi + lo (FP32 pair), 2 hi + lo16.  p alternates
+// between two buffers by iteration parity (p_old the previous iteration's).
 void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, int vmode, const SysState* st_cur,
-                               SysState* st_next, TraceRow* trace, int trace_stride, const float* z, double* p,
-                               double* ap);
-void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, double* x, const double* p, float* r,
-                               const double* ap, const float* minv, const BlockPre* blockM, float* z);
+                               SysState* st_next, TraceRow* trace, int trace_stride, const float* z,
+                               const double* p_old, double* p_new, double* ap);
+// odd iterations also add both steps' alpha p to x (p_prev: the previous
+// iteration's direction); even ones leave x alone.
+void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, bool odd, double* x, const double* p,
+                               const double* p_prev, float* r, const double* ap, const float* minv,
+                               const BlockPre* blockM, float* z);
+// x += alpha p for the systems with state.x_pend (stopped behind an even
+// update), then clears the flag; st: the current state slot.
+void launch_pcg_flush_x_acc(const Launch& L, int nsys, SysState* st, const double* p, double* x);
 
 // ---- plain SpMV (no reduction) ----
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y);

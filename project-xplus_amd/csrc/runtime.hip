@@ -388,6 +388,8 @@ struct xp_pcg {
     DevBuf<float> r32;
     DevBuf<xp::BlockPre> blockM;      // factored 8x8 blocks in solve order (block preconditioner)
     DevBuf<double> p64, ap64, x0s64;  // accurate iteration: FP64 p, A p, and x0 in solve order
+    DevBuf<double> p64b;              // the second p buffer (p alternates by iteration parity)
+    double* pb(int k) { return (k & 1) ? p64b.p : p64.p; }
     bool use_block = false;
     int fallback = 0;
     int last_restarts = 0;            // recurrence restarts of the last xp_pcg_run_reliable
@@ -424,8 +426,10 @@ struct xp_pcg {
         xp::SysState* cur = st.p + (k % 2) * ns;
         xp::SysState* nxt = st.p + ((k + 1) % 2) * ns;
         if (opts.accurate) {
-            xp::launch_pcg_spmv_fused_acc(L, A, acc_vmode(), cur, nxt, trace.p, trace_stride, z.p, p64.p, ap64.p);
-            xp::launch_pcg_update_xrz_acc(L, nxt, x64.p, p64.p, r.p, ap64.p, minv.p, bm(), z.p);
+            xp::launch_pcg_spmv_fused_acc(L, A, acc_vmode(), cur, nxt, trace.p, trace_stride, z.p, pb(k + 1), pb(k),
+                                          ap64.p);
+            xp::launch_pcg_update_xrz_acc(L, nxt, (k & 1) != 0, x64.p, pb(k), pb(k + 1), r.p, ap64.p, minv.p, bm(),
+                                          z.p);
         } else if (opts.fused) {
             xp::launch_pcg_spmv_fused(L, A, cur, nxt, trace.p, trace_stride, z.p, p.p, ap.p);
             xp::launch_pcg_update_xrz(L, nxt, x.p, p.p, r.p, ap.p, minv.p, bm(), z.p);
@@ -477,6 +481,14 @@ struct xp_pcg {
                             z.p, p.p);
     }
 
+    // Accurate iteration: the x update of the systems that stopped behind an
+    // even update (state.x_pend), after an even number of iterations (state
+    // slot 0 current).
+    void flush_x() {
+        if (!opts.accurate) return;
+        xp::launch_pcg_flush_x_acc(m->launch(stream), m->nsys, st.p, p64.p, x64.p);
+    }
+
     bool any_active() {
         const size_t ns = static_cast<size_t>(m->nsys);
         HIPX(hipMemcpyAsync(st_host, st.p, ns * sizeof(xp::SysState), hipMemcpyDeviceToHost, stream));
@@ -517,6 +529,7 @@ struct xp_pcg {
                 if (!any_active()) break;
             }
         }
+        flush_x();
     }
 };
 
@@ -576,6 +589,7 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
     if (d.accurate) {
         ensure_f64(*s);
         s->p64.alloc(np);
+        s->p64b.alloc(np);
         s->ap64.alloc(np);
         s->x0s64.alloc(np);
     }
@@ -1256,8 +1270,9 @@ int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels) {
             xp::SysState* nxt = S.st.p + ((k + 1) % 2) * ns;
             if (S.opts.accurate) {
                 xp::launch_pcg_spmv_fused_acc(L1, A, S.acc_vmode(), cur, nxt, S.trace.p, S.trace_stride, S.z.p,
-                                              S.p64.p, S.ap64.p);
-                xp::launch_pcg_update_xrz_acc(L2, nxt, S.x64.p, S.p64.p, S.r.p, S.ap64.p, S.minv.p, S.bm(), S.z.p);
+                                              S.pb(k + 1), S.pb(k), S.ap64.p);
+                xp::launch_pcg_update_xrz_acc(L2, nxt, (k & 1) != 0, S.x64.p, S.pb(k), S.pb(k + 1), S.r.p, S.ap64.p,
+                                              S.minv.p, S.bm(), S.z.p);
             } else if (S.opts.fused) {
                 xp::launch_pcg_spmv_fused(L1, A, cur, nxt, S.trace.p, S.trace_stride, S.z.p, S.p.p, S.ap.p);
                 xp::launch_pcg_update_xrz(L2, nxt, S.x.p, S.p.p, S.r.p, S.ap.p, S.minv.p, S.bm(), S.z.p);
@@ -1860,6 +1875,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
             }
             // fold and replace (all systems: a replacement is valid at any point)
             if (!acc) xp::launch_fold_f32_into_f64(m.n_pad, S.x.p, S.x64.p, S.stream);
+            S.flush_x();
             xp::launch_pcg_replace(L, m.view(), !m.has64, S.x64.p, S.b64.p, S.minv.p, S.bm(), S.r.p, S.z.p,
                                    S.repl_out.p);
             S.repl_out.download(sums.data(), sums.size(), S.stream);
@@ -1869,6 +1885,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
             running = false;
             for (size_t k = 0; k < ns; ++k) {
                 xp::SysState& t = S.st_host[k];
+                t.x_pend = 0;  // flush_x added the pending step and cleared the flag
                 rr_true[k] = sums[3 * k + 2];
                 rr_last[k] = rr_true[k];
                 const bool converged = !t.active && t.status == xp::kStatusConverged && !verified[k];
@@ -1906,6 +1923,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
         }
         // final fold and true residuals
         if (!acc) xp::launch_fold_f32_into_f64(m.n_pad, S.x.p, S.x64.p, S.stream);
+        S.flush_x();
         const std::vector<double> rn = residual_norms(S, S.x64.p, nullptr);
         HIPX(hipEventRecord(S.ev1, S.stream));
         HIPX(hipEventSynchronize(S.ev1));

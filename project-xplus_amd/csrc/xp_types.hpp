@@ -64,6 +64,8 @@ struct SysState {
     int stall_limit;  // 0 = stagnation watch off
     int trace_cap;
     int fresh;        // 1 right after init (fused iteration: no previous step to finish)
+    int x_pend;       // accurate iteration: x still lacks alpha p of the system's last
+                      // step (it stopped after an even update: k_pcg_flush_x_acc adds it)
 };
 
 // Per-system Jacobi iteration state (two slots alternating per update,
