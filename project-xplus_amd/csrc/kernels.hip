@@ -661,8 +661,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_p(Launch L, const SysS
 // tail from the rz/rr partials, as k_pcg_update_p does) and forms the new
 // direction without a separate pass, using A p_new = A z + beta A p_old:
 //   p = z + beta p;  ap = A z + beta ap;  partials of pAp.
-// Right after init (state.fresh) p = z already and ap = A z.  Reads slot Sc,
-// writes slot Sn; the k_pcg_update_xrz that follows reads Sn.
+// Right after init or a restart (state.fresh) p = z and ap = A z.  Reads slot
+// Sc, writes slot Sn; the k_pcg_update_xrz that follows reads Sn.
 template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView A, const SysState* Sc,
                                                               SysState* Sn, TraceRow* trace, int trace_stride,
@@ -710,14 +710,14 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
         float zd;
         const float az = sell_row_f32<IDX16, V>(A, sl, lane, z, zd);
         float pn, an;
-        if (fresh) {  // right after init p = z already; ap = A z
+        if (fresh) {  // after init or a restart from a replaced residual: p = z, ap = A z
             pn = zd;
             an = az;
         } else {
             pn = static_cast<float>(static_cast<double>(zd) + beta * static_cast<double>(p[row]));
             an = static_cast<float>(static_cast<double>(az) + beta * static_cast<double>(ap[row]));
-            p[row] = pn;
         }
+        p[row] = pn;
         ap[row] = an;
         v[0] += static_cast<double>(pn) * static_cast<double>(an);
     }
