@@ -266,10 +266,7 @@ def workload_c3(X, args) -> int:
         log(f"c3 {mesh}: refined {ref_ms:.1f} ms, accurate {acc_ms:.1f} ms ({iters} iterations), "
             f"delta 0.01: {acc01_ms:.1f} ms ({iters4})")
         # block solves capped at 3x the Jacobi count, so one that stalls ends;
-        # replacement when r^T r dropped 1e4-fold (delta 0.01): at 1600 x 1600
-        # the block-preconditioned recurrence stalls between 1e-8 and 1e-10
-        # with delta 0.1, 0.3 or 0 (replacement at convergence only) and
-        # reaches 1e-10 with 0.01
+        # replacement when r^T r dropped 1e4-fold (delta 0.01)
         ml, _ = learned_matrix(X, a, args, tune=True, keep64=True)
         blocks = {}
         for name in ("exact", "learned"):
