@@ -240,8 +240,8 @@ def calibration_copies(X) -> None:
 
 def profile_run(X, a, args) -> int:
     calibration_copies(X)
-    # the standalone SpMV on the Jacobi matrix (the layout the autotuner picks
-    # for this workload: sigma 64, 512 workgroups)
+    # the standalone SpMV on the Jacobi matrix, in the layout spmv_tuned picks
+    # for this workload (sigma 64, inner loop 0, 512 workgroups)
     m = X.Matrix(a, sigma=64, grid=512)
     spmv = spmv_roofline(m, a)
     m.close()
