@@ -1,5 +1,7 @@
-"""bench.py's command-line contract (CPU): one GPU only -- --gpus above 1,
-or a multi-rank launch, exits with a message before touching a device."""
+"""bench.py's contract on the CPU: one GPU only -- --gpus above 1, or a
+multi-rank launch, exits with a message before touching a device -- and the
+roofline figures of its JSON line (algorithmic bytes over the measured
+duration, and the HBM traffic read from the committed PMC summaries)."""
 from __future__ import annotations
 
 import os
@@ -26,3 +28,32 @@ def test_more_than_one_gpu_is_refused():
 def test_unknown_workload_is_an_argument_error():
     out = run(["--workload", "nope"])
     assert out.returncode == 2 and "invalid choice" in out.stderr
+
+
+def _bench_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", O.ROOT / "bench.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_roofline_arithmetic():
+    # 12 nnz + 40 n bytes of the 1100 x 1100 FEM system (nnz 8,430,446,
+    # n 1,205,604) in 22.199 us: 149,389,512 B / 22.199e-6 s = 6,729.6 GB/s,
+    # 84.12 % of 8,000 GB/s.
+    B = _bench_module()
+    r = B.roof("k", 12 * 8_430_446 + 40 * 1_205_604, 22.199, "")
+    assert r["alg_bytes"] == 149_389_512
+    assert r["achieved"] == 6729.6 and r["frac"] == 0.8412
+    assert r["bound"] == "hbm" and r["peak"] == 8000.0 and r["unit"] == "GB/s"
+    # 1 GB in 250 us is 4,000 GB/s
+    assert B.roof("k", 1e9, 250.0, "")["achieved"] == 4000.0
+
+
+def test_traffic_comes_from_the_committed_pmc_passes():
+    B = _bench_module()
+    prof = B.profiled_traffic()
+    assert prof["k_pcg_spmv_fused_acc"][0] == 133_288_200
+    assert prof["k_pcg_update_xrz_acc"][0] == 60_541_950
+    assert all(src.startswith("profiles/") for _, src in prof.values())
