@@ -264,6 +264,13 @@ int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_ou
  * recursive residual, which suits the accurate iteration (its recursive
  * residual tracks the true one, and each replacement perturbs the Krylov
  * recurrence).
+ * With a block preconditioner and stall_limit > 0 the true residual judges
+ * stagnation: every stall_limit iterations (at the next graph boundary) the
+ * FP64 ||b - A x|| of every active system is computed, and when one has not
+ * halved its best r^T r since the previous check, or a system broke down,
+ * the solver switches to Jacobi and restarts every unfinished system from
+ * its current x (fallback = 1 in xp_status; the blocks are used again by
+ * the next run).  The recursive stagnation watch is off in this mode.
  * Writes true relative residuals, the number of replacements and the
  * largest iteration count. */
 int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res, int* replacements,
@@ -287,7 +294,16 @@ int xp_pcg_true_residual(xp_pcg* s, double* out_norms);
  *   not positive definite keeps only its diagonal.  With a block
  *   preconditioner and stall_limit > 0, a stagnating or breaking-down solve
  *   is redone with Jacobi (fallback). */
-int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks);
+int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks, long long count);
+/* count: the number of floats in `blocks` (mode 2: must be
+ * xp_matrix_num_blocks(m) * 64, else the call fails; ignored otherwise). */
+/* The operator the device applies for each block, evaluated in FP64 from
+ * the stored FP16 factor and FP32 scales (S C C^T S), in block order:
+ * out[64 b + 8 i + j], xp_matrix_num_blocks(m) * 64 doubles; padding slots
+ * hold identity rows and columns.  Fails when no block preconditioner is
+ * installed.  (No reference counterpart: a check of the learned
+ * preconditioner against the FP64 block-PCG oracle.) */
+int xp_pcg_block_operator(xp_pcg* s, double* out);
 /* Number of 8-row blocks of the matrix (all systems): sum ceil(n_k / 8)
  * for consecutive-row blocks, the aggregate count for strength blocking.
  * Block-ordered arrays (mode 2 blocks, xp_matrix_block_features) list the

@@ -16,12 +16,21 @@ the independent CPU restatement its GPU kernels are checked against:
   stored_blocks      -- the operator the GPU applies for blocks M_b: s = sqrt(diag M_b),
                         C = chol(S^-1 M_b S^-1) rounded to FP16, S C C^T S
   block_pcg          -- FP64 PCG with a block-diagonal preconditioner
+  block_pcg_accurate -- the same PCG in the library's accurate arithmetic
+                        (r, z rounded to FP32; p, A p, x FP64; A p by the
+                        recurrence A z + beta A p) stopped on the TRUE
+                        residual: when the recursive r^T r meets tau but
+                        ||b - A x||^2 does not, r and z are replaced by the
+                        true residual and the recurrence restarts (p = z)
 
 Pinning: loss_grad against central finite differences of loss (FP64,
 tests/test_block_oracle.py); block_pcg against the golden Jacobi-PCG when
 the blocks are diagonal, and against finite termination with exact block
-inverses on block-diagonal matrices; features_from_csr against dense
-closed forms on hand-built blocks.
+inverses on block-diagonal matrices; block_pcg_accurate without its FP32
+rounding against block_pcg (same iterates up to FP64 rounding), and with it
+against the properties the rounding must keep (r and z FP32-representable,
+the stop on the true residual); features_from_csr against dense closed
+forms on hand-built blocks.
 """
 from __future__ import annotations
 
@@ -243,3 +252,56 @@ def block_pcg(A, b, blocks, tau, max_iters=100000, groups=None):
         p = z + (rzn / rz) * p
         rz = rzn
     return x, it, np.array(trace)
+
+
+def block_pcg_accurate(A, b, blocks, tau, max_iters=200000, groups=None, round32=True):
+    """PCG in the accurate iteration's arithmetic with the operator blocks:
+    r and z stored in FP32 (each rounded once from FP64 arithmetic; z from
+    the FP64 block product), p, A p and x in FP64, A p carried by the
+    recurrence A p = A z + beta A p_old, dots in FP64.  The recursion stops
+    when r^T r <= tau; the true residual b - A x is then checked, and when
+    its square is above tau, r = fp32(b - A x), z = M r and the recurrence
+    restarts (p = z).  round32=False keeps every vector in FP64.
+    Returns (x, iterations, restarts)."""
+    n = A.shape[0]
+    if groups is None:
+        groups = natural_groups(n)
+    nb = len(groups)
+    idx = np.full((nb, 8), n)
+    for k, g in enumerate(groups):
+        idx[k, :len(g)] = g
+    blocks = np.asarray(blocks, dtype=np.float64)
+
+    def apply(r):
+        rp = np.append(r, 0.0)[idx]
+        z = np.zeros(n + 1)
+        z[idx] = np.einsum("bij,bj->bi", blocks, rp)
+        return z[:n]
+
+    f = (lambda v: v.astype(np.float32).astype(np.float64)) if round32 else (lambda v: v)
+    x = np.zeros(n)
+    it = restarts = 0
+    r = f(b - A @ x)
+    while True:
+        z = f(apply(r))
+        p = z.copy()
+        ap = A @ z
+        rz, rr = r @ z, r @ r
+        while rr > tau and it < max_iters:
+            alpha = rz / (p @ ap)
+            x += alpha * p
+            r = f(r - alpha * ap)
+            z = f(apply(r))
+            rzn, rr = r @ z, r @ r
+            it += 1
+            if rr <= tau:
+                break
+            beta = rzn / rz
+            rz = rzn
+            p = z + beta * p
+            ap = A @ z + beta * ap
+        rt = b - A @ x
+        if rt @ rt <= tau or it >= max_iters:
+            return x, it, restarts
+        r = f(rt)
+        restarts += 1

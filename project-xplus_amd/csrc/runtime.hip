@@ -1833,8 +1833,20 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
         for (size_t k = 0; k < ns; ++k) {
             tau[k] = (rel_tol * bn[k]) * (rel_tol * bn[k]);
             init[k].tau = tau[k];
+            init[k].stall_limit = 0;  // stagnation is judged here, on the true residual
             rr_last[k] = rn0[k] * rn0[k];
         }
+        // Stagnation fallback on the true residual: with a block
+        // preconditioner and stall_limit > 0, every stall_limit iterations
+        // (at the next graph boundary) the FP64 true r^T r of the current x
+        // is computed; a system whose true r^T r has not halved against its
+        // best since the previous check, or that broke down, makes the solver
+        // switch to Jacobi and restart every unfinished system from its
+        // current x (r, z replaced by the true residual, p = z).
+        const bool watch = S.use_block && S.opts.stall_limit > 0;
+        std::vector<double> best_true(ns);
+        for (size_t k = 0; k < ns; ++k) best_true[k] = rr_last[k];
+        long long next_check = S.opts.stall_limit;
         S.st.upload(init.data(), init.size(), S.stream);
         if (acc) {
             S.enqueue_init();  // r = fp32(b - A x64), z = M r
@@ -1862,6 +1874,57 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
             ++launches;
             HIPX(hipMemcpyAsync(S.st_host, S.st.p, ns * sizeof(xp::SysState), hipMemcpyDeviceToHost, S.stream));
             HIPX(hipStreamSynchronize(S.stream));
+            if (watch && S.fallback == 0) {
+                bool failed = false;
+                int itmax = 0;
+                for (size_t k = 0; k < ns; ++k) {
+                    failed = failed || S.st_host[k].status == xp::kStatusBreakdown;
+                    if (S.st_host[k].active) itmax = std::max(itmax, S.st_host[k].iters);
+                }
+                if (!failed && itmax >= next_check) {
+                    next_check = itmax + S.opts.stall_limit;
+                    if (!acc) xp::launch_fold_f32_into_f64(m.n_pad, S.x.p, S.x64.p, S.stream);
+                    const std::vector<double> rt = residual_norms(S, S.x64.p, nullptr);
+                    for (size_t k = 0; k < ns; ++k) {
+                        if (!S.st_host[k].active) continue;
+                        const double rr = rt[k] * rt[k];
+                        if (!(rr <= 0.5 * best_true[k])) failed = true;
+                        best_true[k] = std::min(best_true[k], rr);
+                    }
+                }
+                if (failed) {
+                    S.use_block = false;
+                    S.fallback = 1;
+                    if (S.opts.use_graph) S.build_graph(S.opts.graph_iters);
+                    if (!acc) xp::launch_fold_f32_into_f64(m.n_pad, S.x.p, S.x64.p, S.stream);
+                    S.flush_x();
+                    xp::launch_pcg_replace(L, m.view(), !m.has64, S.x64.p, S.b64.p, S.minv.p, nullptr, S.r.p, S.z.p,
+                                           S.repl_out.p);
+                    S.repl_out.download(sums.data(), sums.size(), S.stream);
+                    HIPX(hipStreamSynchronize(S.stream));
+                    ++repl;
+                    running = false;
+                    for (size_t k = 0; k < ns; ++k) {
+                        xp::SysState& t = S.st_host[k];
+                        t.x_pend = 0;
+                        rr_true[k] = rr_last[k] = best_true[k] = sums[3 * k + 2];
+                        const bool done = !t.active && t.status == xp::kStatusConverged && verified[k];
+                        if (!done && (t.active || t.status != xp::kStatusMaxIter)) {
+                            t.active = 1;
+                            t.status = xp::kStatusRunning;
+                            t.fresh = 1;
+                            t.rz = sums[3 * k + 0];
+                            t.rr = sums[3 * k + 1];
+                            t.best_rr = t.rr;
+                            t.stall = 0;
+                            ++restarts;
+                        }
+                        running = running || t.active;
+                    }
+                    HIPX(hipMemcpyAsync(S.st.p, S.st_host, ns * sizeof(xp::SysState), hipMemcpyHostToDevice, S.stream));
+                    continue;
+                }
+            }
             bool need = false, any_active = false;
             for (size_t k = 0; k < ns; ++k) {
                 const xp::SysState& t = S.st_host[k];
@@ -1930,10 +1993,10 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
         S.last_ms = ms;
         S.last_launches = launches;
         S.x64_valid = true;
-        if (!was_fused) {
-            S.opts.fused = 0;
-            if (S.opts.use_graph) S.build_graph(S.opts.graph_iters);
-        }
+        const bool rebuild = !was_fused || (S.fallback != 0 && !S.use_block);
+        if (!was_fused) S.opts.fused = 0;
+        if (S.fallback != 0) S.use_block = true;  // the next run tries the blocks again
+        if (rebuild && S.opts.use_graph) S.build_graph(S.opts.graph_iters);
         if (rel_res != nullptr)
             for (size_t k = 0; k < ns; ++k) rel_res[k] = rn[k] / std::max(bn[k], 1.0e-300);
         if (replacements != nullptr) *replacements = repl;
@@ -2502,10 +2565,56 @@ extern "C" int xp_matrix_num_blocks(const xp_matrix* m) {
     return num_blocks(*m);
 }
 
-extern "C" int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks) {
+extern "C" int xp_pcg_set_preconditioner(xp_pcg* s, int mode, const float* blocks, long long count) {
     return guarded([&] {
         if (s == nullptr) throw std::runtime_error("null solver");
+        if (mode == 2) {
+            // the block array is read as num_blocks x 64 floats: refuse a
+            // shorter (or longer) one instead of reading past its end
+            const long long want = 64LL * num_blocks(*s->m);
+            if (blocks == nullptr || count != want)
+                throw std::runtime_error("mode 2 needs num_blocks x 64 = " + std::to_string(want) +
+                                         " block values, got " + std::to_string(count));
+        }
         set_preconditioner(*s, mode, blocks);
+    });
+}
+
+// The operator the device applies for each block (S C C^T S from the stored
+// FP16 factor and FP32 scales, evaluated in FP64), in block order: out[64 b +
+// 8 i + j]; padding slots hold identity rows and columns.
+extern "C" int xp_pcg_block_operator(xp_pcg* s, double* out) {
+    return guarded([&] {
+        if (s == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        if (!s->use_block) throw std::runtime_error("the solver has no block preconditioner");
+        const xp_matrix& m = *s->m;
+        const int nq = m.n_pad / 8;
+        std::vector<xp::BlockPre> h(static_cast<size_t>(nq));
+        s->blockM.download(h.data(), h.size(), s->stream);
+        HIPX(hipStreamSynchronize(s->stream));
+        const size_t nb = static_cast<size_t>(num_blocks(m));
+        for (size_t b = 0; b < nb; ++b)
+            for (int i = 0; i < 64; ++i) out[b * 64 + static_cast<size_t>(i)] = (i % 9 == 0) ? 1.0 : 0.0;
+        for (int q = 0; q < nq; ++q) {
+            const int bid = block_of(m, q);
+            if (bid < 0) continue;
+            const xp::BlockPre& f = h[static_cast<size_t>(q)];
+            double c[8][8] = {};
+            int t = 0;
+            for (int i = 0; i < 8; ++i)
+                for (int k = 0; k <= i; ++k, ++t)
+                    c[i][k] = static_cast<double>(static_cast<float>(__builtin_bit_cast(_Float16, f.c[t])));
+            for (int i = 0; i < 8; ++i) {
+                if (m.perm[static_cast<size_t>(q) * 8 + static_cast<size_t>(i)] < 0) continue;
+                for (int j = 0; j < 8; ++j) {
+                    if (m.perm[static_cast<size_t>(q) * 8 + static_cast<size_t>(j)] < 0) continue;
+                    double acc = 0.0;
+                    for (int k = 0; k < 8; ++k) acc += c[i][k] * c[j][k];
+                    out[static_cast<size_t>(bid) * 64 + static_cast<size_t>(i) * 8 + static_cast<size_t>(j)] =
+                        static_cast<double>(f.s[i]) * acc * static_cast<double>(f.s[j]);
+                }
+            }
+        }
     });
 }
 

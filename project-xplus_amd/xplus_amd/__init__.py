@@ -136,7 +136,8 @@ _p_trace = _sig("xp_pcg_get_trace", C.c_int, [_vp, C.c_int, _f64p, C.c_int])
 _p_true_res = _sig("xp_pcg_true_residual", C.c_int, [_vp, _f64p])
 _p_timing = _sig("xp_pcg_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int)])
 
-_p_set_precond = _sig("xp_pcg_set_preconditioner", C.c_int, [_vp, C.c_int, _vp])
+_p_set_precond = _sig("xp_pcg_set_preconditioner", C.c_int, [_vp, C.c_int, _vp, C.c_longlong])
+_p_block_op = _sig("xp_pcg_block_operator", C.c_int, [_vp, _f64p])
 
 
 class JacobiOpts(C.Structure):
@@ -832,10 +833,18 @@ class Pcg:
     def set_preconditioner(self, mode: int, blocks=None) -> None:
         """0 Jacobi, 1 exact 8x8 block Jacobi, 2 given blocks (nblocks x 8 x 8, natural order)."""
         if blocks is None:
-            _check(_p_set_precond(self.h, mode, None))
+            _check(_p_set_precond(self.h, mode, None, 0))
         else:
             b = np.ascontiguousarray(blocks, dtype=np.float32)
-            _check(_p_set_precond(self.h, mode, b.ctypes.data_as(_vp)))
+            _check(_p_set_precond(self.h, mode, b.ctypes.data_as(_vp), b.size))
+
+    def block_operator(self) -> np.ndarray:
+        """The blocks the device applies (S C C^T S of the stored FP16
+        factors, FP64), block order: nblocks x 8 x 8."""
+        nb = self.matrix.num_blocks()
+        out = np.zeros(nb * 64)
+        _check(_p_block_op(self.h, out))
+        return out.reshape(nb, 8, 8)
 
     def set_mlp_preconditioner(self, net: "Mlp") -> None:
         _check(_p_set_mlp(self.h, net.h))

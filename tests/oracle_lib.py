@@ -16,6 +16,7 @@ ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 REF_SO = ROOT / "oracle" / "_ref" / "libref.so"
 REF_BIN = ROOT / "oracle" / "_ref" / "xplus_local"
 REF_ON_AMD_BIN = ROOT / "oracle" / "_ref" / "xplus_local_on_amd"
+REF_CUPER_ON_AMD_BIN = ROOT / "oracle" / "_ref" / "cuper_pcg_on_amd"
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

@@ -152,3 +152,54 @@ def test_oracle_stored_blocks():
         assert np.linalg.eigvalsh(S[k]).min() > 0
         rel = np.abs(S[k] - M[k]) / np.sqrt(np.outer(np.diag(M[k]), np.diag(M[k])))
         assert rel.max() < 4e-3
+
+
+def _exact_blocks(A, groups):
+    out = np.zeros((len(groups), 8, 8))
+    for b, g in enumerate(groups):
+        k = len(g)
+        out[b, :k, :k] = np.linalg.inv(A[g][:, g].toarray())
+        out[b, k:, k:] = np.eye(8 - k)
+    return out
+
+
+def test_accurate_block_pcg_without_rounding_is_block_pcg():
+    """With FP64 vectors the accurate arithmetic is block PCG with A p carried
+    by its recurrence: the same iterates up to FP64 rounding on a
+    well-conditioned system."""
+    a = X.gen_diffusion2d(20, 18, 0.5, 31)
+    A = a.to_scipy()
+    groups = B.natural_groups(a.n)
+    blocks = _exact_blocks(A, groups)
+    tau = (1e-8 * np.linalg.norm(a.b)) ** 2
+    x1, it1, _ = B.block_pcg(A, a.b, blocks, tau, groups=groups)
+    x2, it2, restarts = B.block_pcg_accurate(A, a.b, blocks, tau, groups=groups, round32=False)
+    assert abs(it1 - it2) <= 1 and restarts == 0
+    assert np.linalg.norm(x1 - x2) <= 1e-9 * np.linalg.norm(x1)
+
+
+def test_accurate_block_pcg_stops_on_the_true_residual():
+    """With FP32 r and z: the returned x meets tau in the TRUE residual; the
+    rounding changes the iterates against FP64 vectors; on a block-diagonal
+    matrix with exact blocks one step meets 1e-6, and a 1e-12 target needs
+    restarts from the true residual (each FP32 step gains ~7 digits)."""
+    a = X.gen_fem2d(24, 1.0, 3, 1e3, 0.3, 32)
+    A = a.to_scipy()
+    groups = B.natural_groups(a.n)
+    blocks = B.stored_blocks(_exact_blocks(A, groups).astype(np.float32))
+    tau = (1e-6 * np.linalg.norm(a.b)) ** 2
+    x, it, _ = B.block_pcg_accurate(A, a.b, blocks, tau, groups=groups)
+    r = a.b - A @ x
+    assert r @ r <= tau
+    x64, _, _ = B.block_pcg_accurate(A, a.b, blocks, tau, groups=groups, round32=False)
+    assert np.linalg.norm(x - x64) > 0.0
+    import scipy.sparse as sp
+    D = sp.block_diag([np.eye(8) * 2 + np.diag(np.full(7, -0.5), 1) + np.diag(np.full(7, -0.5), -1)] * 3).tocsr()
+    g = B.natural_groups(24)
+    bd = _exact_blocks(D, g)
+    b = np.linspace(-1, 1, 24)
+    xd, itd, rs = B.block_pcg_accurate(D, b, bd, (1e-6 * np.linalg.norm(b)) ** 2, groups=g)
+    assert itd == 1 and rs == 0
+    xd, itd, rs = B.block_pcg_accurate(D, b, bd, (1e-12 * np.linalg.norm(b)) ** 2, groups=g)
+    assert rs >= 1 and itd <= 2 * (rs + 1)
+    assert np.linalg.norm(D @ xd - b) <= 1e-12 * np.linalg.norm(b)

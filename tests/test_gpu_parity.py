@@ -144,6 +144,40 @@ def test_reference_main_linked_against_library(gpu):
     assert "[done] iter=41" in out.stdout
 
 
+@pytest.mark.parametrize("backend", ["csr64", "sell32"])
+@pytest.mark.parametrize("case", ["n512", "c1"])
+def test_reference_cuper_pcg_loop_with_library_spmv(gpu, backend, case, tmp_path):
+    """The reference's pluggable-SpMV PCG loop (run_cuper_pcg_solver_with_backend,
+    host/cuper_pcg_solver.hpp:169-309), compiled unchanged, with the library as
+    its SpmvRunner: it must pass the reference host's own check against
+    run_cpu_reference (max_abs_diff or max_rel_diff <= diff_tol, 1e-3 by
+    default, host/cuper_tapa_pcg_main.cpp:47,382) -- and the FP64 backend
+    also the local host's 1e-9 with the golden's iteration count."""
+    import re
+    import subprocess
+    if not O.REF_CUPER_ON_AMD_BIN.exists():
+        pytest.skip("oracle/_ref/cuper_pcg_on_amd not built")
+    if case == "n512":
+        d, args = N512, ["--tau", "1e-8", "--max-iters", "100"]
+    else:
+        a = X.gen_diffusion2d(32, 32, 1.0, 20261204)
+        d = tmp_path / "c1"
+        a.save(d)
+        args = ["--tau", repr(1e-12 * float(a.b @ a.b)), "--max-iters", "2000"]
+    out = subprocess.run([str(O.REF_CUPER_ON_AMD_BIN), str(d), "--backend", backend, *args],
+                         capture_output=True, text=True, timeout=120)
+    print(out.stdout[-700:], flush=True)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr
+    done = re.search(r"\[done\] iter=(\d+) .* status=(\w+)", out.stdout)
+    chk = re.search(r"golden_iter=(\d+) .* max_abs_diff=(\S+) max_rel_diff=(\S+)", out.stdout)
+    assert done and chk and done.group(2) == "converged"
+    it, git, dabs = int(done.group(1)), int(chk.group(1)), float(chk.group(2))
+    if backend == "csr64":
+        assert abs(it - git) <= 1 and dabs <= 1e-9
+    else:
+        assert abs(it - git) <= 1 and dabs <= 1e-3
+
+
 def test_cli_drop_in(gpu):
     import subprocess
     out = subprocess.run([str(X.CLI_PATH), str(N512), "--tau", "1e-8", "--max-iters", "100", "--device-solver"],
