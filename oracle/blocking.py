@@ -22,9 +22,16 @@ smaller row) and candidates scored by (summed strength to the block) / t_j;
 the remaining partial blocks are packed in order of their smallest row, each
 into the most recent of the last 16 packed blocks with room (else a new one).
 
+rule 3 (parallel strength-ordered, the device aggregation): the rule-2
+ingredients (t_i, the fraction score) in rounds that every row can take at
+once -- see parallel_blocking.
+
 Pinning (tests/test_blocking.py): partition and size invariants; on a 1-D
 chain (tridiagonal) the blocks are the consecutive runs of 8; the greedy
-choice is checked step by step against brute force on small matrices.
+choice is checked step by step against brute force on small matrices; for
+rule 3, the seeds of each round are checked against a brute-force
+distance-2 maximum search and every growth step against the choice rule
+replayed from scratch.
 """
 from __future__ import annotations
 
@@ -141,3 +148,150 @@ def strength_blocking(row_ptr, col_idx, values, rule=1):
     full = [sorted(b) for b in full]
     full.sort(key=lambda b: b[0])
     return full
+
+
+# ---------------------------------------------------------------- rule 3
+GROW_STEPS = 8      # growth steps per round
+PACK_WINDOW = 16    # packing looks at the last 16 packed blocks
+
+
+def _f32_bits(v):
+    """Bits of v rounded to the nearest FP32 (v >= 0): order-preserving."""
+    import struct
+    return struct.unpack("<I", struct.pack("<f", v))[0]
+
+
+def totals(row_ptr, col_idx, s):
+    n = len(row_ptr) - 1
+    t = [0.0] * n
+    for r in range(n):
+        for o in range(row_ptr[r], row_ptr[r + 1]):
+            if col_idx[o] != r:
+                t[r] += s[o]
+    return t
+
+
+def _better(a, b, t):
+    """Row a precedes row b in the seed order (larger t, then smaller row)."""
+    return b < 0 or t[a] > t[b] or (t[a] == t[b] and a < b)
+
+
+def seeds_of_round(row_ptr, col_idx, t, elig, owner):
+    """Unassigned eligible rows that precede every unassigned eligible row
+    within distance 2 (neighbours: the columns of a row's entries)."""
+    n = len(row_ptr) - 1
+    free = [elig[i] and owner[i] < 0 for i in range(n)]
+    m1 = [-1] * n
+    for i in range(n):
+        best = i if free[i] else -1
+        for o in range(row_ptr[i], row_ptr[i + 1]):
+            c = col_idx[o]
+            if c != i and free[c] and _better(c, best, t):
+                best = c
+        m1[i] = best
+    seeds = []
+    for i in range(n):
+        if not free[i]:
+            continue
+        best = m1[i]
+        for o in range(row_ptr[i], row_ptr[i + 1]):
+            c = col_idx[o]
+            if c != i and m1[c] >= 0 and _better(m1[c], best, t):
+                best = m1[c]
+        if best == i:
+            seeds.append(i)
+    return seeds
+
+
+def growth_bids(row_ptr, col_idx, s, t, elig, owner, size):
+    """One growth step's choices: {block: row}.  Every unassigned eligible
+    row j scores each block with room (< 8 rows) that owns one of its
+    neighbours by (sum of s over j's entries into the block, in CSR order) /
+    t_j (0 when t_j = 0), and bids for the best (ties: the smaller block
+    id); each block takes the bid with the largest FP32-rounded score
+    (ties: the smaller row)."""
+    n = len(row_ptr) - 1
+    best_of_block = {}
+    for j in range(n):
+        if not elig[j] or owner[j] >= 0:
+            continue
+        sums = {}
+        for o in range(row_ptr[j], row_ptr[j + 1]):
+            c = col_idx[o]
+            if c == j:
+                continue
+            B = owner[c]
+            if B < 0 or size[B] >= 8:
+                continue
+            sums[B] = sums[B] + s[o] if B in sums else s[o]
+        if not sums:
+            continue
+        bB, bf = -1, -1.0
+        for B, w in sums.items():
+            f = w / t[j] if t[j] > 0 else 0.0
+            if f > bf or (f == bf and B < bB):
+                bB, bf = B, f
+        key = (_f32_bits(bf), -j)
+        if bB not in best_of_block or key > best_of_block[bB][0]:
+            best_of_block[bB] = (key, j)
+    return {B: kj[1] for B, kj in best_of_block.items()}
+
+
+def _rounds(row_ptr, col_idx, s, t, elig, owner, size, members, log=None):
+    n = len(row_ptr) - 1
+    while any(elig[i] and owner[i] < 0 for i in range(n)):
+        for i in seeds_of_round(row_ptr, col_idx, t, elig, owner):
+            owner[i] = i
+            size[i] = 1
+            members[i] = [i]
+        for _ in range(GROW_STEPS):
+            take = growth_bids(row_ptr, col_idx, s, t, elig, owner, size)
+            if log is not None:
+                log.append((list(owner), dict(take)))    # the owner map the step chose from
+            for B, j in take.items():
+                owner[j] = B
+                size[B] += 1
+                members[B].append(j)
+
+
+def parallel_blocking(row_ptr, col_idx, values, log=None):
+    """Rule 3: blocks as a list of row lists (ascending rows, ordered by
+    first row).
+      rounds: seeds = unassigned rows that precede every unassigned row
+              within distance 2 in the rule-2 order (larger t_i, then
+              smaller row), each a block named by its seed row; then
+              GROW_STEPS growth steps (growth_bids), all blocks at once;
+              rounds repeat until every row has a block;
+      regroup: the rows of blocks with fewer than 8 rows are pooled and
+              the rounds rerun on the pool;
+      pack:   the blocks still smaller than 8 (pieces), rows ascending,
+              in order of their smallest row, each into the most recent of
+              the last PACK_WINDOW packed blocks with room, else a new one."""
+    n = len(row_ptr) - 1
+    if n == 0:
+        return []
+    s = strengths(row_ptr, col_idx, values)
+    t = totals(row_ptr, col_idx, s)
+    owner, size, members = [-1] * n, [0] * n, [None] * n
+    _rounds(row_ptr, col_idx, s, t, [True] * n, owner, size, members, log)
+    pool = [size[owner[r]] < 8 for r in range(n)]
+    for r in range(n):
+        if pool[r]:
+            owner[r] = -1
+    for B in range(n):
+        if size[B] < 8:
+            size[B] = 0
+    _rounds(row_ptr, col_idx, s, t, pool, owner, size, members, log)
+    full = [sorted(members[B]) for B in range(n) if size[B] == 8]
+    pieces = sorted((sorted(members[B]) for B in range(n) if 0 < size[B] < 8), key=lambda b: b[0])
+    packed = []
+    for p in pieces:
+        for q in reversed(packed[-PACK_WINDOW:]):
+            if len(q) + len(p) <= 8:
+                q.extend(p)
+                break
+        else:
+            packed.append(list(p))
+    out = full + [sorted(q) for q in packed]
+    out.sort(key=lambda b: b[0])
+    return out
