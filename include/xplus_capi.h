@@ -89,7 +89,11 @@ typedef struct {
                          (block preconditioners); 0/1: rows */
     int blocking;     /* with block_rows = 8: 0 = blocks of 8 consecutive rows;
                          1 = greedy strength aggregates (xp_strength_blocking);
-                         2 = strength-ordered aggregates (xp_aggregate_blocking rule 2) */
+                         2 = strength-ordered aggregates (xp_aggregate_blocking rule 2);
+                         3 = parallel strength-ordered aggregates, computed on the
+                             device (implies device_build) */
+    int device_build; /* 1: set the matrix up on the device (xp_matrix_create
+                         uploads the CSR; one system only) */
 } xp_matrix_opts;
 
 typedef struct {
@@ -102,6 +106,24 @@ typedef struct {
 /* One system (batch of 1).  Host CSR arrays; values are FP64. */
 int xp_matrix_create(int n, const int* row_ptr, const int* col_idx, const double* values,
                      const xp_matrix_opts* opts, xp_matrix** out);
+/* One system from a CSR already in device memory (device pointers), set up
+ * entirely on the device: CSR validation, row facts, blocks (block_rows = 8
+ * with blocking 0 or 3) and the SELL arrays, equal array for array to the
+ * host builder's for the same blocks.  The matrix keeps its own device copy
+ * of the CSR.  sigma <= 4096 (block_rows = 1) or 32768 (block_rows = 8).
+ * No reference counterpart: the reference packs its matrices on the host
+ * (host/cuper_control_matrix.hpp:130-207); this is the set-up the timed
+ * learned solve includes. */
+int xp_matrix_create_device(int n, const int* row_ptr, const int* col_idx, const double* values,
+                            const xp_matrix_opts* opts, xp_matrix** out);
+/* stats[5]: device-built (0/1), aggregation rounds, regroup rounds, pieces
+ * packed, blocks. */
+int xp_matrix_setup_stats(const xp_matrix* m, int* stats);
+/* Downloads the SELL arrays (sizes from xp_matrix_get_info: n_pad, nslices
+ * + 1, nslices, then `stored` entries each; any pointer may be NULL, value
+ * streams the matrix does not keep are skipped). */
+int xp_matrix_sell_copy(const xp_matrix* m, int* perm, int* slice_ptr, int* slice_base, float* val, float* val_lo,
+                        short* val_lo16, double* val64, unsigned short* col16, int* col32);
 /* A batch of independent systems solved together (block-diagonal).  The
  * arrays are the systems' CSR arrays concatenated: row_ptrs holds
  * sum(ns[k] + 1) entries (each system's own row_ptr, starting at 0),
@@ -242,6 +264,8 @@ void xp_pcg_destroy(xp_pcg* s);
 /* Uploads b and x0 (host, natural order, FP64, sum of n_k entries; x0 may
  * be NULL for zero) and derives tau per system.  Not part of a timed solve. */
 int xp_pcg_set_rhs(xp_pcg* s, const double* b, const double* x0);
+/* The same from device memory (natural order, FP64; x0 may be NULL). */
+int xp_pcg_set_rhs_device(xp_pcg* s, const double* b, const double* x0);
 /* Runs init + iterations until every system stops.  Device-resident. */
 int xp_pcg_run(xp_pcg* s);
 /* Runs init + exactly `iters` iterations (graph-captured as one launch),
@@ -278,6 +302,8 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
 int xp_pcg_get_status(xp_pcg* s, int sys, xp_status* out);
 /* Solution in natural order, FP64 (sum of n_k entries). */
 int xp_pcg_get_x(xp_pcg* s, double* x);
+/* The solution into device memory (natural order, FP64, n entries). */
+int xp_pcg_get_x_device(xp_pcg* s, double* x);
 /* Trace rows {alpha, beta, rz, rr} of system `sys`; returns rows copied. */
 int xp_pcg_get_trace(xp_pcg* s, int sys, double* rows, int cap);
 /* FP64 true residual ||b - A x||_2 per system on the GPU. */
@@ -504,11 +530,15 @@ int xp_cuper_packed_create(int batch_count, int channel_size, const int* sp_elem
 int xp_cuper_unpack(const xp_cuper_packed* p, int n, xp_csr** out);
 void xp_cuper_packed_free(xp_cuper_packed* p);
 /* Host-side SELL build for inspection and CPU tests (no GPU): returns the
- * arrays of the format described in the library's xp_types.hpp. */
-int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const double* values,
-                       int sigma, int* n_pad, int* nslices, long long* stored, int* idx16);
-int xp_sell_copy_host(int* perm, int* slice_ptr, int* slice_base, float* val,
-                      unsigned short* col16, int* col32);
+ * arrays of the format described in the library's xp_types.hpp.  With
+ * block_rows = 8 and `slots` (units x 8 rows, -1 padding) the blocks are
+ * kept together as in a block matrix; slots = NULL: 8 consecutive rows. */
+int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const double* values, int sigma,
+                       int keep_lo, int keep64, int block_rows, const int* slots, int units, int* n_pad,
+                       int* nslices, long long* stored, int* idx16);
+/* The arrays of the last xp_sell_build_host of this thread (any may be NULL). */
+int xp_sell_copy_host(int* perm, int* slice_ptr, int* slice_base, float* val, float* val_lo, short* val_lo16,
+                      double* val64, unsigned short* col16, int* col32);
 
 #ifdef __cplusplus
 }

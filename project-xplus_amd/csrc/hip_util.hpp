@@ -1,0 +1,67 @@
+// hip_util.hpp -- HIP error checks and an owning device buffer, shared by
+// the host runtime (runtime.hip) and the device-side matrix builder
+// (device_build.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace xp {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPX(expr) ::xp::hip_check((expr), #expr)
+
+// Owning device allocation of `n` elements of T (hipMalloc / hipFree).
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        HIPX(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) { if (count > n) alloc(std::max(count, n + n / 2)); }
+    void release() {
+        if (p != nullptr) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        if (count) HIPX(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* dst, size_t count, hipStream_t s) const {
+        if (count) HIPX(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+template <typename T>
+DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
+    DevBuf<T> d(v.size());
+    d.upload(v.data(), v.size(), s);
+    return d;
+}
+
+}  // namespace xp

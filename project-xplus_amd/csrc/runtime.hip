@@ -22,7 +22,9 @@
 #include "../../include/xplus_capi.h"
 #include "csr_dataset.hpp"
 #include "cuper_format.hpp"
+#include "device_build.hpp"
 #include "generators.hpp"
+#include "hip_util.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "mtx_ingest.hpp"
@@ -34,14 +36,9 @@ thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPX(expr) hip_check((expr), #expr)
+using xp::DevBuf;
+using xp::HipError;
+using xp::to_device;
 
 template <typename F>
 int guarded(F&& f) {
@@ -90,47 +87,6 @@ Device& device() {
 
 hipStream_t pick_stream(void* s) { return s != nullptr ? static_cast<hipStream_t>(s) : device().stream; }
 
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    explicit DevBuf(size_t count) { alloc(count); }
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void alloc(size_t count) {
-        release();
-        if (count == 0) return;
-        HIPX(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
-        n = count;
-    }
-    void ensure(size_t count) { if (count > n) alloc(std::max(count, n + n / 2)); }
-    void release() {
-        if (p != nullptr) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void upload(const T* src, size_t count, hipStream_t s) {
-        if (count) HIPX(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-    void download(T* dst, size_t count, hipStream_t s) const {
-        if (count) HIPX(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
-    }
-};
-
-template <typename T>
-DevBuf<T> to_device(const std::vector<T>& v, hipStream_t s) {
-    DevBuf<T> d(v.size());
-    d.upload(v.data(), v.size(), s);
-    return d;
-}
-
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 }  // namespace
@@ -162,6 +118,21 @@ struct xp_matrix {
     DevBuf<double> d_val64;
     DevBuf<uint16_t> d_col16;
     DevBuf<int> d_col32;
+    // device-built matrices (xp_matrix_create_device): the arrays above are
+    // produced on the device; the host copies (CSR, perm, blocks, diagonal)
+    // are downloaded only when a host-side function needs them
+    bool device_built = false;
+    bool host_ready = true;
+    DevBuf<int> d_rp, d_ci;           // the matrix's own device copy of its CSR
+    DevBuf<double> d_va;
+    DevBuf<int> d_slots;              // blocks (units x 8) of a device-built block matrix
+    xp::AggStats agg{};
+    // both kinds
+    DevBuf<int> d_block_unit;         // solve block -> block id (block_rows = 8)
+    DevBuf<float> d_minv;             // solve order 1 / diag (padding rows 1)
+    DevBuf<int> d_sys_nat;            // [nsys + 1] natural row offsets
+    int bad_row = -1;                 // first natural row failing the Jacobi checks (-1: none)
+    int bad_kind = 0;                 // 1 missing diagonal, 2 |d| <= DBL_MIN, 3 |d| <= 1e-30
     // launch machinery
     int grid = 0;
     int u_blocks = 0;                 // single system: workgroups with work (uniform split)
@@ -303,6 +274,26 @@ void load_sell(xp_matrix& m, int sigma) {
             m.diag_state[g] = state;
         }
     }
+    m.bad_row = -1;
+    m.bad_kind = 0;
+    for (size_t g = 0; g < m.diag.size() && m.bad_row < 0; ++g) {
+        const int kind = m.diag_state[g] != 0 ? m.diag_state[g] : (std::fabs(m.diag[g]) <= 1.0e-30 ? 3 : 0);
+        if (kind != 0) {
+            m.bad_row = static_cast<int>(g);
+            m.bad_kind = kind;
+        }
+    }
+    std::vector<float> minv(static_cast<size_t>(h.n_pad), 1.0f);
+    for (size_t i = 0; i < minv.size(); ++i) {
+        const int g = h.perm[i];
+        if (g >= 0) minv[i] = static_cast<float>(1.0 / m.diag[static_cast<size_t>(g)]);
+    }
+    m.d_minv = to_device(minv, s);
+    m.d_sys_nat = to_device(h.sys_nat_begin, s);
+    if (m.block_rows == 8)
+        m.d_block_unit = to_device(h.block_unit, s);
+    else
+        m.d_block_unit.release();
     m.d_slice_ptr = to_device(h.slice_ptr, s);
     m.d_slice_base = to_device(h.slice_base, s);
     m.d_perm = to_device(h.perm, s);
@@ -359,6 +350,123 @@ std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, c
     return m;
 }
 
+// One system from a CSR in device memory, set up on the device: row facts,
+// blocks (block_rows = 8: consecutive rows, blocking 0, or the parallel
+// strength aggregation, blocking 3) and the SELL arrays.  Host copies are
+// made only on demand (ensure_host_mirror).
+std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const int* col_idx, const double* values,
+                                              const xp_matrix_opts* o) {
+    const hipStream_t s = device().stream;
+    auto m = std::make_unique<xp_matrix>();
+    m->keep_lo = o != nullptr && o->keep_lo;
+    m->keep64 = o != nullptr && o->keep64;
+    m->variant = (o != nullptr && o->spmv_variant == 1) ? 1 : 0;
+    m->block_rows = (o != nullptr && o->block_rows == 8) ? 8 : 1;
+    m->blocking = m->block_rows == 8 && o != nullptr ? o->blocking : 0;
+    if (m->block_rows == 8 && m->blocking != 0 && m->blocking != 3)
+        throw std::runtime_error("the device builder aggregates with rule 3 (blocking 3) or uses consecutive rows "
+                                 "(blocking 0); rules 1 and 2 run on the host (xp_matrix_create)");
+    xp::DevCsr a{n, 0, row_ptr, col_idx, values};
+    xp::DevRows rows;
+    xp::device_rows(a, rows, s);
+    // the matrix keeps its own copy of the CSR (the caller's may go away)
+    const size_t nnz = static_cast<size_t>(a.nnz);
+    m->d_rp.alloc(static_cast<size_t>(n) + 1);
+    m->d_ci.alloc(std::max<size_t>(nnz, 1));
+    m->d_va.alloc(std::max<size_t>(nnz, 1));
+    HIPX(hipMemcpyAsync(m->d_rp.p, row_ptr, (static_cast<size_t>(n) + 1) * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (nnz) HIPX(hipMemcpyAsync(m->d_ci.p, col_idx, nnz * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (nnz) HIPX(hipMemcpyAsync(m->d_va.p, values, nnz * sizeof(double), hipMemcpyDeviceToDevice, s));
+    a.row_ptr = m->d_rp.p;
+    a.col_idx = m->d_ci.p;
+    a.values = m->d_va.p;
+    int units = 0;
+    if (m->block_rows == 8)
+        units = m->blocking == 3 ? xp::device_aggregate(a, rows, m->d_slots, &m->agg, s)
+                                 : xp::device_natural_slots(n, m->d_slots, s);
+    xp::DevSell out;
+    xp::device_sell(a, rows, (o != nullptr && o->sigma > 0) ? o->sigma : 256, m->keep_lo, m->keep64, m->block_rows,
+                    m->d_slots.p, units, out, s);
+    m->device_built = true;
+    m->host_ready = false;
+    m->n = n;
+    m->n_pad = out.n_pad;
+    m->nslices = out.nslices;
+    m->nsys = 1;
+    m->sigma = out.sigma;
+    m->idx16 = out.idx16;
+    m->max_width = out.max_width;
+    m->nnz = out.nnz;
+    m->stored = out.stored;
+    m->has_lo = m->keep_lo;
+    m->has64 = m->keep64;
+    m->h_ns = {n};
+    m->sys_row_begin = {0, out.n_pad};
+    m->sys_nat_begin = {0, n};
+    m->unit_first = {0, m->block_rows == 8 ? units : 0};
+    m->bad_row = out.bad_row;
+    m->bad_kind = out.bad_kind;
+    m->d_perm = std::move(out.perm);
+    m->d_slice_ptr = std::move(out.slice_ptr);
+    m->d_slice_base = std::move(out.slice_base);
+    m->d_val = std::move(out.val);
+    m->d_val_lo = std::move(out.val_lo);
+    m->d_val_lo16 = std::move(out.val_lo16);
+    m->d_val64 = std::move(out.val64);
+    m->d_col16 = std::move(out.col16);
+    m->d_col32 = std::move(out.col32);
+    m->d_block_unit = std::move(out.block_unit);
+    m->d_minv = std::move(out.minv);
+    m->d_sys_nat = to_device(m->sys_nat_begin, s);
+    m->diag.clear();
+    const int grid = (o != nullptr && o->grid > 0) ? o->grid : default_grid(*m);
+    build_work_map(*m, grid);
+    return m;
+}
+
+// The host copies of a device-built matrix (CSR, permutation, blocks,
+// diagonal), downloaded on first need: host-side preconditioner set-up,
+// autotuning's re-sorts, the Jacobi solver, inspection.
+void ensure_host_mirror(const xp_matrix& cm) {
+    xp_matrix& m = const_cast<xp_matrix&>(cm);
+    if (m.host_ready) return;
+    const hipStream_t s = device().stream;
+    const int n = m.n;
+    m.h_row_ptr.resize(static_cast<size_t>(n) + 1);
+    m.d_rp.download(m.h_row_ptr.data(), m.h_row_ptr.size(), s);
+    HIPX(hipStreamSynchronize(s));
+    const size_t nnz = static_cast<size_t>(m.h_row_ptr[static_cast<size_t>(n)]);
+    m.h_col_idx.resize(nnz);
+    m.h_values.resize(nnz);
+    m.d_ci.download(m.h_col_idx.data(), nnz, s);
+    m.d_va.download(m.h_values.data(), nnz, s);
+    m.perm.resize(static_cast<size_t>(m.n_pad));
+    m.d_perm.download(m.perm.data(), m.perm.size(), s);
+    if (m.block_rows == 8) {
+        m.block_unit.resize(static_cast<size_t>(m.n_pad / 8));
+        m.d_block_unit.download(m.block_unit.data(), m.block_unit.size(), s);
+        xp::Blocking b;
+        b.n = n;
+        b.slots.resize(static_cast<size_t>(m.unit_first.back()) * 8);
+        m.d_slots.download(b.slots.data(), b.slots.size(), s);
+        m.blockings = {b};
+    }
+    HIPX(hipStreamSynchronize(s));
+    m.diag.assign(static_cast<size_t>(n), 0.0);
+    m.diag_state.assign(static_cast<size_t>(n), 0);
+    for (int r = 0; r < n; ++r) {
+        int state = 1;
+        for (int o2 = m.h_row_ptr[static_cast<size_t>(r)]; o2 < m.h_row_ptr[static_cast<size_t>(r) + 1]; ++o2)
+            if (m.h_col_idx[static_cast<size_t>(o2)] == r) {
+                m.diag[static_cast<size_t>(r)] = m.h_values[static_cast<size_t>(o2)];
+                state = std::fabs(m.h_values[static_cast<size_t>(o2)]) <= std::numeric_limits<double>::min() ? 2 : 0;
+                break;
+            }
+        m.diag_state[static_cast<size_t>(r)] = state;
+    }
+    m.host_ready = true;
+}
+
 }  // namespace
 
 // ---- solver ------------------------------------------------------------------
@@ -381,10 +489,12 @@ struct xp_pcg {
     double last_ms = 0.0;
     int last_launches = 0;
     bool have_rhs = false;
-    // natural-order FP64 copies of b and x0 (host) for refinement / residuals
-    std::vector<double> b_nat, x0_nat;
+    // natural-order FP64 b and x0 on the device (refinement, residual
+    // replacement and restarts read them there), and ||b_k|| per system
+    DevBuf<double> b_nat, x0_nat;
+    std::vector<double> bnorm;
     // refinement buffers (allocated on first use)
-    DevBuf<double> x64, b64, r64, rr_out, tmp_nat, repl_out;
+    DevBuf<double> x64, b64, r64, rr_out, repl_out;
     DevBuf<float> r32;
     DevBuf<xp::BlockPre> blockM;      // factored 8x8 blocks in solve order (block preconditioner)
     DevBuf<double> p64, ap64, x0s64;  // accurate iteration: FP64 p, A p, and x0 in solve order
@@ -451,6 +561,10 @@ struct xp_pcg {
         graph_version = m->layout_version;
         graph_vmode = acc_vmode();
     }
+
+    // The next run captures the graph again (a preconditioner change: the
+    // captured kernels hold the block pointer or its absence).
+    void invalidate_graph() { graph_version = -1; }
 
     // A graph captures the matrix's work map and partials by address: after
     // a workgroup-count change (autotuning) it is captured again.
@@ -541,15 +655,15 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
     if (m == nullptr) throw std::runtime_error("null matrix");
     // The reference refuses a bad Jacobi diagonal before iterating
     // (src/CsrDataset.hpp:107-112, archived/host/multi_kernel_solver.hpp:63-75).
-    for (size_t g = 0; g < m->diag_state.size(); ++g) {
+    if (m->bad_row >= 0) {
         // report in system-local row numbers, as the reference would for that system
+        const int g = m->bad_row;
         int k = 0;
-        while (k + 1 < m->nsys && static_cast<int>(g) >= m->sys_nat_begin[static_cast<size_t>(k) + 1]) ++k;
-        const int local = static_cast<int>(g) - m->sys_nat_begin[static_cast<size_t>(k)];
-        if (m->diag_state[g] == 1) throw std::runtime_error("missing diagonal entry at row " + std::to_string(local));
-        if (m->diag_state[g] == 2) throw std::runtime_error("zero diagonal entry at row " + std::to_string(local));
-        if (std::fabs(m->diag[g]) <= 1.0e-30)
-            throw std::runtime_error("zero diagonal entry while building Jacobi inverse");
+        while (k + 1 < m->nsys && g >= m->sys_nat_begin[static_cast<size_t>(k) + 1]) ++k;
+        const int local = g - m->sys_nat_begin[static_cast<size_t>(k)];
+        if (m->bad_kind == 1) throw std::runtime_error("missing diagonal entry at row " + std::to_string(local));
+        if (m->bad_kind == 2) throw std::runtime_error("zero diagonal entry at row " + std::to_string(local));
+        throw std::runtime_error("zero diagonal entry while building Jacobi inverse");
     }
     auto s = std::make_unique<xp_pcg>();
     s->m = m;
@@ -579,13 +693,9 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
     HIPX(hipEventCreate(&s->ev1));
     const size_t np = static_cast<size_t>(m->n_pad);
     s->x.alloc(np); s->r.alloc(np); s->z.alloc(np); s->p.alloc(np); s->ap.alloc(np); s->b.alloc(np);
-    // minv in solve order (padding rows: 1)
-    std::vector<float> minv(np, 1.0f);
-    for (size_t i = 0; i < np; ++i) {
-        const int g = m->perm[i];
-        if (g >= 0) minv[i] = static_cast<float>(1.0 / m->diag[static_cast<size_t>(g)]);
-    }
-    s->minv = to_device(minv, s->stream);
+    // minv in solve order (padding rows: 1), built with the matrix
+    s->minv.alloc(np);
+    HIPX(hipMemcpyAsync(s->minv.p, m->d_minv.p, np * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
     if (d.accurate) {
         ensure_f64(*s);
         s->p64.alloc(np);
@@ -607,45 +717,64 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
         t.trace_cap = s->trace_stride;
     }
     HIPX(hipHostMalloc(reinterpret_cast<void**>(&s->st_host), static_cast<size_t>(m->nsys) * sizeof(xp::SysState)));
-    if (d.use_graph) s->build_graph(d.graph_iters);
-    s->graph_iters = d.graph_iters;
+    s->graph_iters = d.graph_iters;  // the graph is captured by the first run (ensure_graph)
     HIPX(hipStreamSynchronize(s->stream));
     return s;
 }
 
-void set_rhs(xp_pcg& s, const double* b, const double* x0) {
+// Takes b and x0 from s.b_nat / s.x0_nat (natural order, device): the
+// per-system ||b_k|| and tau, and the solve-order FP32 (and FP64) copies.
+void set_rhs_common(xp_pcg& s) {
     xp_matrix& m = *s.m;
-    const size_t n = static_cast<size_t>(m.n);
-    s.b_nat.assign(b, b + n);
-    if (x0 != nullptr)
-        s.x0_nat.assign(x0, x0 + n);
-    else
-        s.x0_nat.assign(n, 0.0);
-    // tau per system (absolute, or relative to ||b_k||)
-    for (int k = 0; k < m.nsys; ++k) {
-        xp::SysState& t = s.st_init[static_cast<size_t>(k)];
-        if (s.opts.rel_tol > 0.0) {
-            double bb = 0.0;
-            for (int g = m.sys_nat_begin[static_cast<size_t>(k)]; g < m.sys_nat_begin[static_cast<size_t>(k) + 1]; ++g)
-                bb += b[g] * b[g];
-            t.tau = s.opts.rel_tol * s.opts.rel_tol * bb;
-        } else {
-            t.tau = s.opts.tau;
-        }
-    }
-    DevBuf<double> tmp(std::max<size_t>(n, 1));
-    tmp.upload(b, n, s.stream);
-    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s.b.p, s.stream);
-    tmp.upload(s.x0_nat.data(), n, s.stream);
+    const size_t ns = static_cast<size_t>(m.nsys);
+    DevBuf<double> bb(ns);
+    xp::launch_sys_sumsq(m.nsys, m.d_sys_nat.p, s.b_nat.p, bb.p, s.stream);
+    std::vector<double> h(ns);
+    bb.download(h.data(), ns, s.stream);
+    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, s.b_nat.p, s.b.p, s.stream);
     if (s.x0s.n != static_cast<size_t>(m.n_pad)) s.x0s.alloc(static_cast<size_t>(m.n_pad));
-    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, tmp.p, s.x0s.p, s.stream);
+    xp::launch_gather_d2f(m.n_pad, m.d_perm.p, s.x0_nat.p, s.x0s.p, s.stream);
     if (s.opts.accurate) {
-        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, tmp.p, s.x0s64.p, s.stream);
-        tmp.upload(b, n, s.stream);
-        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, tmp.p, s.b64.p, s.stream);
+        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, s.x0_nat.p, s.x0s64.p, s.stream);
+        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, s.b_nat.p, s.b64.p, s.stream);
     }
     HIPX(hipStreamSynchronize(s.stream));
+    s.bnorm.resize(ns);
+    // tau per system (absolute, or relative to ||b_k||)
+    for (size_t k = 0; k < ns; ++k) {
+        s.bnorm[k] = std::sqrt(h[k]);
+        s.st_init[k].tau = s.opts.rel_tol > 0.0 ? s.opts.rel_tol * s.opts.rel_tol * h[k] : s.opts.tau;
+    }
     s.have_rhs = true;
+}
+
+void alloc_rhs(xp_pcg& s) {
+    const size_t n = std::max<size_t>(static_cast<size_t>(s.m->n), 1);
+    if (s.b_nat.n != n) s.b_nat.alloc(n);
+    if (s.x0_nat.n != n) s.x0_nat.alloc(n);
+}
+
+void set_rhs(xp_pcg& s, const double* b, const double* x0) {
+    const size_t n = static_cast<size_t>(s.m->n);
+    alloc_rhs(s);
+    s.b_nat.upload(b, n, s.stream);
+    if (x0 != nullptr)
+        s.x0_nat.upload(x0, n, s.stream);
+    else
+        HIPX(hipMemsetAsync(s.x0_nat.p, 0, n * sizeof(double), s.stream));
+    set_rhs_common(s);
+}
+
+// b and x0 already in device memory (natural order; x0 may be null: zero).
+void set_rhs_device(xp_pcg& s, const double* b, const double* x0) {
+    const size_t n = static_cast<size_t>(s.m->n);
+    alloc_rhs(s);
+    HIPX(hipMemcpyAsync(s.b_nat.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    if (x0 != nullptr)
+        HIPX(hipMemcpyAsync(s.x0_nat.p, x0, n * sizeof(double), hipMemcpyDeviceToDevice, s.stream));
+    else
+        HIPX(hipMemsetAsync(s.x0_nat.p, 0, n * sizeof(double), s.stream));
+    set_rhs_common(s);
 }
 
 }  // namespace
@@ -853,8 +982,63 @@ int xp_matrix_create(int n, const int* row_ptr, const int* col_idx, const double
         if (out == nullptr) throw std::runtime_error("null output pointer");
         *out = nullptr;
         if (n <= 0) throw std::runtime_error("n must be positive");
+        if (opts != nullptr && (opts->device_build || (opts->block_rows == 8 && opts->blocking == 3))) {
+            // upload the CSR and set the matrix up on the device
+            if (row_ptr == nullptr || col_idx == nullptr || values == nullptr)
+                throw std::runtime_error("build_sell: null CSR array");
+            const hipStream_t s = device().stream;
+            const size_t nnz = static_cast<size_t>(std::max(row_ptr[n], 0));
+            DevBuf<int> rp(static_cast<size_t>(n) + 1), ci(std::max<size_t>(nnz, 1));
+            DevBuf<double> va(std::max<size_t>(nnz, 1));
+            rp.upload(row_ptr, static_cast<size_t>(n) + 1, s);
+            ci.upload(col_idx, nnz, s);
+            va.upload(values, nnz, s);
+            *out = make_matrix_device(n, rp.p, ci.p, va.p, opts).release();
+            return;
+        }
         xp::CsrRef a{n, row_ptr, col_idx, values};
         *out = make_matrix({a}, opts).release();
+    });
+}
+
+int xp_matrix_create_device(int n, const int* row_ptr, const int* col_idx, const double* values,
+                            const xp_matrix_opts* opts, xp_matrix** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null output pointer");
+        *out = nullptr;
+        if (n <= 0) throw std::runtime_error("n must be positive");
+        *out = make_matrix_device(n, row_ptr, col_idx, values, opts).release();
+    });
+}
+
+int xp_matrix_setup_stats(const xp_matrix* m, int* stats) {
+    return guarded([&] {
+        if (m == nullptr || stats == nullptr) throw std::runtime_error("null argument");
+        stats[0] = m->device_built ? 1 : 0;
+        stats[1] = m->agg.rounds;
+        stats[2] = m->agg.regroup_rounds;
+        stats[3] = m->agg.pieces;
+        stats[4] = m->block_rows == 8 ? m->unit_first.back() : 0;
+    });
+}
+
+int xp_matrix_sell_copy(const xp_matrix* m, int* perm, int* slice_ptr, int* slice_base, float* val, float* val_lo,
+                        short* val_lo16, double* val64, unsigned short* col16, int* col32) {
+    return guarded([&] {
+        if (m == nullptr) throw std::runtime_error("null matrix");
+        const hipStream_t s = device().stream;
+        const size_t np = static_cast<size_t>(m->n_pad), nsl = static_cast<size_t>(m->nslices);
+        const size_t st = static_cast<size_t>(m->stored);
+        if (perm) m->d_perm.download(perm, np, s);
+        if (slice_ptr) m->d_slice_ptr.download(slice_ptr, nsl + 1, s);
+        if (slice_base) m->d_slice_base.download(slice_base, nsl, s);
+        if (val) m->d_val.download(val, st, s);
+        if (val_lo && m->d_val_lo.p) m->d_val_lo.download(val_lo, st, s);
+        if (val_lo16 && m->d_val_lo16.p) m->d_val_lo16.download(reinterpret_cast<int16_t*>(val_lo16), st, s);
+        if (val64 && m->d_val64.p) m->d_val64.download(val64, st, s);
+        if (col16 && m->d_col16.p) m->d_col16.download(col16, st, s);
+        if (col32 && m->d_col32.p) m->d_col32.download(col32, st, s);
+        HIPX(hipStreamSynchronize(s));
     });
 }
 
@@ -899,6 +1083,7 @@ int xp_matrix_get_info(const xp_matrix* m, xp_matrix_info* out) {
 int xp_matrix_permutation(const xp_matrix* m, int* perm) {
     return guarded([&] {
         if (m == nullptr || perm == nullptr) throw std::runtime_error("null argument");
+        ensure_host_mirror(*m);
         std::memcpy(perm, m->perm.data(), m->perm.size() * sizeof(int));
     });
 }
@@ -1122,6 +1307,24 @@ int xp_pcg_set_rhs(xp_pcg* s, const double* b, const double* x0) {
     return guarded([&] {
         if (s == nullptr || b == nullptr) throw std::runtime_error("null argument");
         set_rhs(*s, b, x0);
+    });
+}
+
+int xp_pcg_set_rhs_device(xp_pcg* s, const double* b, const double* x0) {
+    return guarded([&] {
+        if (s == nullptr || b == nullptr) throw std::runtime_error("null argument");
+        set_rhs_device(*s, b, x0);
+    });
+}
+
+int xp_pcg_get_x_device(xp_pcg* s, double* x) {
+    return guarded([&] {
+        if (s == nullptr || x == nullptr) throw std::runtime_error("null argument");
+        if (s->x64_valid)
+            xp::launch_scatter_d2d(s->m->n_pad, s->m->d_perm.p, s->x64.p, x, s->stream);
+        else
+            xp::launch_scatter_f2d(s->m->n_pad, s->m->d_perm.p, s->x.p, x, s->stream);
+        HIPX(hipStreamSynchronize(s->stream));
     });
 }
 
@@ -1441,6 +1644,7 @@ namespace {
 
 std::unique_ptr<xp_jacobi> make_jacobi(xp_matrix* m, const xp_jacobi_opts* o) {
     if (m == nullptr) throw std::runtime_error("null matrix");
+    ensure_host_mirror(*m);
     // D = the sum of each row's diagonal entries in FP32, in CSR order, and
     // Diag_inv = 1 / D (PrepareJacobiVectorsAndR, host/main.cpp:980-1005).
     std::vector<float> dinv_nat(static_cast<size_t>(m->n), 1.0f);
@@ -1654,8 +1858,6 @@ void ensure_f64(xp_pcg& s) {
     if (s.r64.n != np) s.r64.alloc(np);
     if (s.r32.n != np) s.r32.alloc(np);
     if (s.rr_out.n != static_cast<size_t>(s.m->nsys)) s.rr_out.alloc(static_cast<size_t>(s.m->nsys));
-    const size_t nn = std::max<size_t>(static_cast<size_t>(s.m->n), 1);
-    if (s.tmp_nat.n < nn) s.tmp_nat.alloc(nn);
 }
 
 // ||b - A x|| per system, x in solve order FP64; r32 (optional) receives
@@ -1672,21 +1874,15 @@ std::vector<double> residual_norms(xp_pcg& s, const double* x64, float* r32) {
 }
 
 void upload_b64(xp_pcg& s) {
-    s.tmp_nat.upload(s.b_nat.data(), s.b_nat.size(), s.stream);
-    xp::launch_gather_d2d(s.m->n_pad, s.m->d_perm.p, s.tmp_nat.p, s.b64.p, s.stream);
+    xp::launch_gather_d2d(s.m->n_pad, s.m->d_perm.p, s.b_nat.p, s.b64.p, s.stream);
 }
 
-std::vector<double> system_bnorms(const xp_pcg& s) {
-    const xp_matrix& m = *s.m;
-    std::vector<double> out(static_cast<size_t>(m.nsys), 0.0);
-    for (int k = 0; k < m.nsys; ++k) {
-        double acc = 0.0;
-        for (int g = m.sys_nat_begin[static_cast<size_t>(k)]; g < m.sys_nat_begin[static_cast<size_t>(k) + 1]; ++g)
-            acc += s.b_nat[static_cast<size_t>(g)] * s.b_nat[static_cast<size_t>(g)];
-        out[static_cast<size_t>(k)] = std::sqrt(acc);
-    }
-    return out;
+// x64 = x0 in solve order
+void load_x0_64(xp_pcg& s) {
+    xp::launch_gather_d2d(s.m->n_pad, s.m->d_perm.p, s.x0_nat.p, s.x64.p, s.stream);
 }
+
+std::vector<double> system_bnorms(const xp_pcg& s) { return s.bnorm; }
 
 }  // namespace
 
@@ -1725,8 +1921,7 @@ int xp_pcg_run_refined(xp_pcg* s, double outer_rel, double inner_rel, int max_ou
         const std::vector<double> bn = system_bnorms(S);
         HIPX(hipEventRecord(S.ev0, S.stream));
         upload_b64(S);
-        S.tmp_nat.upload(S.x0_nat.data(), S.x0_nat.size(), S.stream);
-        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, S.tmp_nat.p, S.x64.p, S.stream);
+        load_x0_64(S);
         std::vector<double> rn = residual_norms(S, S.x64.p, S.r32.p);
         std::vector<long long> iters(static_cast<size_t>(m.nsys), 0);
         int outer = 0;
@@ -1824,8 +2019,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
         const std::vector<double> bn = system_bnorms(S);
         HIPX(hipEventRecord(S.ev0, S.stream));
         upload_b64(S);
-        S.tmp_nat.upload(S.x0_nat.data(), S.x0_nat.size(), S.stream);
-        xp::launch_gather_d2d(m.n_pad, m.d_perm.p, S.tmp_nat.p, S.x64.p, S.stream);
+        load_x0_64(S);
         const bool acc = S.opts.accurate != 0;  // the iteration updates x64 itself: nothing to fold
         const std::vector<double> rn0 = residual_norms(S, S.x64.p, S.r32.p);
         std::vector<xp::SysState> init = S.st_init;
@@ -2256,11 +2450,21 @@ int xp_csr_copy(const xp_csr* a, int* row_ptr, int* col_idx, double* values, dou
 
 void xp_csr_free(xp_csr* a) { delete a; }
 
-int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const double* values, int sigma,
-                       int* n_pad, int* nslices, long long* stored, int* idx16) {
+int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const double* values, int sigma, int keep_lo,
+                       int keep64, int block_rows, const int* slots, int units, int* n_pad, int* nslices,
+                       long long* stored, int* idx16) {
     return guarded([&] {
         xp::CsrRef a{n, row_ptr, col_idx, values};
-        g_sell_host = xp::build_sell({a}, sigma, false, false);
+        std::vector<xp::Blocking> blocks;
+        if (block_rows == 8 && slots != nullptr) {
+            if (units < 0) throw std::runtime_error("negative block count");
+            xp::Blocking b;
+            b.n = n;
+            b.slots.assign(slots, slots + static_cast<size_t>(units) * 8);
+            blocks.push_back(std::move(b));
+        }
+        g_sell_host = xp::build_sell({a}, sigma, keep_lo != 0, keep64 != 0, block_rows == 8 ? 8 : 1,
+                                     blocks.empty() ? nullptr : &blocks);
         if (n_pad) *n_pad = g_sell_host.n_pad;
         if (nslices) *nslices = g_sell_host.nslices;
         if (stored) *stored = g_sell_host.stored;
@@ -2268,13 +2472,17 @@ int xp_sell_build_host(int n, const int* row_ptr, const int* col_idx, const doub
     });
 }
 
-int xp_sell_copy_host(int* perm, int* slice_ptr, int* slice_base, float* val, unsigned short* col16, int* col32) {
+int xp_sell_copy_host(int* perm, int* slice_ptr, int* slice_base, float* val, float* val_lo, short* val_lo16,
+                      double* val64, unsigned short* col16, int* col32) {
     return guarded([&] {
         const auto& h = g_sell_host;
         if (perm) std::memcpy(perm, h.perm.data(), h.perm.size() * sizeof(int));
         if (slice_ptr) std::memcpy(slice_ptr, h.slice_ptr.data(), h.slice_ptr.size() * sizeof(int));
         if (slice_base) std::memcpy(slice_base, h.slice_base.data(), h.slice_base.size() * sizeof(int));
         if (val) std::memcpy(val, h.val.data(), h.val.size() * sizeof(float));
+        if (val_lo && !h.val_lo.empty()) std::memcpy(val_lo, h.val_lo.data(), h.val_lo.size() * sizeof(float));
+        if (val_lo16 && !h.val_lo16.empty()) std::memcpy(val_lo16, h.val_lo16.data(), h.val_lo16.size() * sizeof(int16_t));
+        if (val64 && !h.val64.empty()) std::memcpy(val64, h.val64.data(), h.val64.size() * sizeof(double));
         if (col16 && h.idx16) std::memcpy(col16, h.col16.data(), h.col16.size() * sizeof(uint16_t));
         if (col32 && !h.idx16) std::memcpy(col32, h.col32.data(), h.col32.size() * sizeof(int));
     });
@@ -2337,6 +2545,7 @@ extern "C" int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int i
         if (m == nullptr) throw std::runtime_error("null matrix");
         if (tune_sigma && m->solvers > 0)
             throw std::runtime_error("autotune with tune_sigma re-sorts the matrix: call it before creating solvers");
+        ensure_host_mirror(*m);
         if (iters <= 0) iters = 64;
         iters += iters % 2;
         std::lock_guard<std::mutex> lock(g_tune_mu);
@@ -2524,6 +2733,7 @@ DeviceFeatures device_block_features(const xp_matrix& m, hipStream_t stream) {
 
 void set_preconditioner(xp_pcg& s, int mode, const float* blocks) {
     xp_matrix& m = *s.m;
+    if (mode != 0) ensure_host_mirror(m);
     if (mode == 0) {
         s.use_block = false;
     } else {
@@ -2545,7 +2755,7 @@ void set_preconditioner(xp_pcg& s, int mode, const float* blocks) {
         HIPX(hipStreamSynchronize(s.stream));
         s.use_block = true;
     }
-    if (s.opts.use_graph) s.build_graph(s.opts.graph_iters);
+    s.invalidate_graph();
 }
 
 }  // namespace
@@ -2554,6 +2764,7 @@ extern "C" int xp_matrix_block_rows(const xp_matrix* m, int* rows) {
     return guarded([&] {
         if (m == nullptr || rows == nullptr) throw std::runtime_error("null argument");
         if (m->block_rows != 8) throw std::runtime_error("the matrix has no blocks (block_rows = 1)");
+        ensure_host_mirror(*m);
         size_t o = 0;
         for (size_t k = 0; k < m->blockings.size(); ++k)
             for (int r : m->blockings[k].slots) rows[o++] = r >= 0 ? m->sys_nat_begin[k] + r : -1;
@@ -2588,6 +2799,7 @@ extern "C" int xp_pcg_block_operator(xp_pcg* s, double* out) {
         if (s == nullptr || out == nullptr) throw std::runtime_error("null argument");
         if (!s->use_block) throw std::runtime_error("the solver has no block preconditioner");
         const xp_matrix& m = *s->m;
+        ensure_host_mirror(m);
         const int nq = m.n_pad / 8;
         std::vector<xp::BlockPre> h(static_cast<size_t>(nq));
         s->blockM.download(h.data(), h.size(), s->stream);
@@ -2767,13 +2979,14 @@ int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net) {
         xp::launch_block_factor(f.blocks, full.p, s->blockM.p, s->stream);
         HIPX(hipStreamSynchronize(s->stream));
         s->use_block = true;
-        if (s->opts.use_graph) s->build_graph(s->opts.graph_iters);
+        s->invalidate_graph();
     });
 }
 
 int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale) {
     return guarded([&] {
         if (m == nullptr || feat == nullptr || scale == nullptr) throw std::runtime_error("null argument");
+        ensure_host_mirror(*m);
         hipStream_t st = device().stream;
         DeviceFeatures f = device_block_features(*m, st);
         std::vector<float> hf(static_cast<size_t>(f.blocks) * 48), hs(static_cast<size_t>(m->n_pad));

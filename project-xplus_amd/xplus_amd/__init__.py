@@ -44,7 +44,8 @@ STATUS_NAMES = {0: "converged", 1: "max_iter", 2: "breakdown", 3: "stagnation", 
 
 class MatrixOpts(C.Structure):
     _fields_ = [("sigma", C.c_int), ("keep_lo", C.c_int), ("keep64", C.c_int), ("grid", C.c_int),
-                ("spmv_variant", C.c_int), ("block_rows", C.c_int), ("blocking", C.c_int)]
+                ("spmv_variant", C.c_int), ("block_rows", C.c_int), ("blocking", C.c_int),
+                ("device_build", C.c_int)]
 
 
 class MatrixInfo(C.Structure):
@@ -216,9 +217,15 @@ _ds_load_bin = _sig("xp_dataset_load_binary", C.c_int, [C.c_char_p, C.POINTER(_v
 _ds_load_cached = _sig("xp_dataset_load_cached", C.c_int,
                        [C.c_char_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(_vp)])
 _sell_build = _sig("xp_sell_build_host", C.c_int,
-                   [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                    C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
-_sell_copy = _sig("xp_sell_copy_host", C.c_int, [_i32p, _i32p, _i32p, _f32p, _u16p, _i32p])
+                   [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
+                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)])
+_sell_copy = _sig("xp_sell_copy_host", C.c_int, [_vp] * 9)
+_m_create_dev = _sig("xp_matrix_create_device", C.c_int,
+                     [C.c_int, _vp, _vp, _vp, C.POINTER(MatrixOpts), C.POINTER(_vp)])
+_m_setup_stats = _sig("xp_matrix_setup_stats", C.c_int, [_vp, _i32p])
+_m_sell_copy = _sig("xp_matrix_sell_copy", C.c_int, [_vp] * 10)
+_p_set_rhs_dev = _sig("xp_pcg_set_rhs_device", C.c_int, [_vp, _vp, _vp])
+_p_get_x_dev = _sig("xp_pcg_get_x_device", C.c_int, [_vp, _vp])
 
 
 class XplusError(RuntimeError):
@@ -496,22 +503,43 @@ def gen_fem2d(m: int, sigma: float = 1.0, n_incl: int = 20, contrast: float = 1e
     return Csr._from_handle(h)
 
 
-def sell_build_host(a: Csr, sigma: int = 256) -> dict:
-    """Host-side SELL build (no GPU): returns the arrays of the format."""
+def _sell_arrays(n_pad, nsl, stored, idx16):
+    return {"perm": np.empty(n_pad, np.int32), "slice_ptr": np.empty(nsl + 1, np.int32),
+            "slice_base": np.empty(nsl, np.int32), "val": np.empty(stored, np.float32),
+            "val_lo": np.empty(stored, np.float32), "val_lo16": np.empty(stored, np.int16),
+            "val64": np.empty(stored, np.float64),
+            "col16": np.empty(stored if idx16 else 1, np.uint16), "col32": np.empty(1 if idx16 else stored, np.int32)}
+
+
+def _ptrs(arrs, names):
+    return [arrs[k].ctypes.data_as(_vp) if k in arrs else None for k in names]
+
+
+_SELL_NAMES = ["perm", "slice_ptr", "slice_base", "val", "val_lo", "val_lo16", "val64", "col16", "col32"]
+
+
+def sell_build_host(a: Csr, sigma: int = 256, keep_lo: bool = False, keep64: bool = False, block_rows: int = 1,
+                    slots=None) -> dict:
+    """Host-side SELL build (no GPU): returns the arrays of the format.
+    block_rows = 8 with slots (units x 8 rows, -1 padding) keeps those
+    blocks together; without slots, 8 consecutive rows."""
     n_pad, nsl, idx16 = C.c_int(), C.c_int(), C.c_int()
     stored = C.c_longlong()
-    _check(_sell_build(a.n, a.row_ptr, a.col_idx, a.values, sigma, C.byref(n_pad), C.byref(nsl),
+    sl = None if slots is None else _i32(np.asarray(slots).ravel())
+    units = 0 if sl is None else len(sl) // 8
+    _check(_sell_build(a.n, a.row_ptr, a.col_idx, a.values, sigma, int(keep_lo), int(keep64), block_rows,
+                       None if sl is None else sl.ctypes.data_as(_vp), units, C.byref(n_pad), C.byref(nsl),
                        C.byref(stored), C.byref(idx16)))
-    perm = np.empty(n_pad.value, np.int32)
-    sp = np.empty(nsl.value + 1, np.int32)
-    sb = np.empty(nsl.value, np.int32)
-    val = np.empty(stored.value, np.float32)
-    c16 = np.empty(stored.value if idx16.value else 1, np.uint16)
-    c32 = np.empty(1 if idx16.value else stored.value, np.int32)
-    _check(_sell_copy(perm, sp, sb, val, c16, c32))
-    return {"n_pad": n_pad.value, "nslices": nsl.value, "stored": stored.value, "idx16": idx16.value,
-            "perm": perm, "slice_ptr": sp, "slice_base": sb, "val": val,
-            "col16": c16 if idx16.value else None, "col32": None if idx16.value else c32}
+    arrs = _sell_arrays(n_pad.value, nsl.value, stored.value, idx16.value)
+    if not keep_lo:
+        del arrs["val_lo"], arrs["val_lo16"]
+    if not keep64:
+        del arrs["val64"]
+    _check(_sell_copy(*_ptrs(arrs, _SELL_NAMES)))
+    out = {"n_pad": n_pad.value, "nslices": nsl.value, "stored": stored.value, "idx16": idx16.value, **arrs}
+    out["col16"] = arrs["col16"] if idx16.value else None
+    out["col32"] = None if idx16.value else arrs["col32"]
+    return out
 
 
 # ---------------------------------------------------------------- device memory
@@ -561,6 +589,22 @@ class DeviceArray:
         self.close()
 
 
+class DeviceCsr:
+    """A CSR system (and its b) uploaded once to device memory."""
+
+    def __init__(self, a: Csr):
+        self.host = a
+        self.n = a.n
+        self.row_ptr = DeviceArray.from_numpy(_i32(a.row_ptr))
+        self.col_idx = DeviceArray.from_numpy(_i32(a.col_idx))
+        self.values = DeviceArray.from_numpy(_f64(a.values))
+        self.b = DeviceArray.from_numpy(_f64(a.b))
+
+    def close(self) -> None:
+        for k in ("row_ptr", "col_idx", "values", "b"):
+            getattr(self, k).close()
+
+
 def synchronize() -> None:
     _check(_dev_sync())
 
@@ -570,15 +614,22 @@ class Matrix:
     """Device-resident SELL-C-sigma matrix (one system or a batch)."""
 
     def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0,
-                 spmv_variant: int = 0, block_rows: int = 1, blocking: int = 0):
+                 spmv_variant: int = 0, block_rows: int = 1, blocking: int = 0, device_build: bool = False,
+                 _device_csr: "DeviceCsr" = None):
         """blocking (with block_rows = 8): 0 blocks of 8 consecutive rows,
-        1 greedy strength aggregates, 2 strength-ordered aggregates."""
+        1 greedy strength aggregates, 2 strength-ordered aggregates, 3 the
+        parallel strength aggregation on the device.  device_build: set the
+        matrix up on the device (one system)."""
         if isinstance(systems, Csr):
             systems = [systems]
         self.systems = list(systems)
-        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant, block_rows, blocking)
+        opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant, block_rows, blocking,
+                          int(device_build))
         h = _vp()
-        if len(self.systems) == 1:
+        if _device_csr is not None:
+            d = _device_csr
+            _check(_m_create_dev(d.n, d.row_ptr.ptr, d.col_idx.ptr, d.values.ptr, C.byref(opts), C.byref(h)))
+        elif len(self.systems) == 1:
             a = self.systems[0]
             _check(_m_create(a.n, a.row_ptr, a.col_idx, a.values, C.byref(opts), C.byref(h)))
         else:
@@ -596,6 +647,25 @@ class Matrix:
 
     def __del__(self):
         self.close()
+
+    @classmethod
+    def from_device(cls, d: "DeviceCsr", **kw) -> "Matrix":
+        """A matrix set up on the device from a CSR already in device memory."""
+        return cls(d.host, _device_csr=d, **kw)
+
+    def setup_stats(self) -> dict:
+        st = np.zeros(5, np.int32)
+        _check(_m_setup_stats(self.h, st))
+        return {"device_built": bool(st[0]), "rounds": int(st[1]), "regroup_rounds": int(st[2]),
+                "pieces": int(st[3]), "blocks": int(st[4])}
+
+    def sell_arrays(self) -> dict:
+        """The device SELL arrays (downloaded)."""
+        i = self.info()
+        arrs = _sell_arrays(i["n_pad"], i["nslices"], i["stored"], i["idx16"])
+        _check(_m_sell_copy(self.h, *_ptrs(arrs, _SELL_NAMES)))
+        return {**i, **arrs, "col16": arrs["col16"] if i["idx16"] else None,
+                "col32": None if i["idx16"] else arrs["col32"]}
 
     def num_blocks(self) -> int:
         return _m_num_blocks(self.h)
@@ -785,6 +855,13 @@ class Pcg:
         else:
             x0 = _f64(x0)
             _check(_p_set_rhs(self.h, b, x0.ctypes.data_as(_vp)))
+
+    def set_rhs_device(self, b_ptr: int, x0_ptr: int = 0) -> None:
+        """b (and x0) from device memory: natural order, FP64."""
+        _check(_p_set_rhs_dev(self.h, b_ptr, x0_ptr or None))
+
+    def x_device(self, out_ptr: int) -> None:
+        _check(_p_get_x_dev(self.h, out_ptr))
 
     def run(self) -> None:
         _check(_p_run(self.h))

@@ -1,7 +1,8 @@
 """The benchmark's solve path end to end against the oracle, at a size the
 FP64 oracle finishes in seconds.
 
-The path: strength-ordered 8-row aggregates (blocking 2), a block network
+The path: strength-ordered 8-row aggregates (blocking 2 on the host, 3 on
+the device), a block network
 trained on other matrices of the family, its blocks computed on the device
 (features, BF16 MFMA inference, FP16 factors), the accurate iteration (FP32
 r and z, FP64 p / A p / x, FP64-accumulated SpMV over hi + lo16 values) and
@@ -63,10 +64,13 @@ def solve_headline(a, net, blocking, stall_limit=0):
     return out
 
 
-def test_headline_path_matches_oracle(gpu, trained_net):
+@pytest.mark.parametrize("blocking", [2, 3])
+def test_headline_path_matches_oracle(gpu, trained_net, blocking):
+    """blocking 2: the host aggregation; 3: the parallel aggregation the
+    device set-up computes (the benchmark's)."""
     a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, HELD_OUT_SEED)
     A = a.to_scipy()
-    g = solve_headline(a, trained_net, blocking=2)
+    g = solve_headline(a, trained_net, blocking=blocking)
     tau = (1e-6 * np.linalg.norm(a.b)) ** 2
     x_ref, it_ref, _ = B.block_pcg(A, a.b, g["blocks"], tau, groups=g["groups"])
     x_sim, it_sim, rs_sim = B.block_pcg_accurate(A, a.b, g["blocks"], tau, groups=g["groups"])
