@@ -390,6 +390,10 @@ int xp_blockset_from_csr(int n, const int* row_ptr, const int* col_idx, const do
  * with the off-block weight of the training loss (< 0: 0.5). */
 int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, const double* values, int blocking,
                                  double lambda, xp_blockset** out);
+/* The same over given blocks (units x 8 rows of slots, -1 padding; e.g.
+ * xp_matrix_block_rows of a device-aggregated matrix, blocking 3). */
+int xp_blockset_from_csr_slots(int n, const int* row_ptr, const int* col_idx, const double* values, const int* slots,
+                               int units, double lambda, xp_blockset** out);
 int xp_blockset_append(xp_blockset* dst, const xp_blockset* src);
 int xp_blockset_size(const xp_blockset* set);
 /* feat: size x 48, ahat: size x 64, hmat (Ahat^2 + 0.5 G): size x 64, scale (D^-1/2): size x 8. */

@@ -2891,6 +2891,28 @@ int xp_blockset_from_csr_blocked(int n, const int* row_ptr, const int* col_idx, 
     });
 }
 
+int xp_blockset_from_csr_slots(int n, const int* row_ptr, const int* col_idx, const double* values, const int* slots,
+                               int units, double lambda, xp_blockset** out) {
+    return guarded([&] {
+        if (out == nullptr || row_ptr == nullptr || slots == nullptr || n <= 0 || units <= 0)
+            throw std::runtime_error("bad argument");
+        xp::Blocking b;
+        b.n = n;
+        b.slots.assign(slots, slots + static_cast<size_t>(units) * 8);
+        std::vector<char> seen(static_cast<size_t>(n), 0);
+        for (int r : b.slots) {
+            if (r < -1 || r >= n || (r >= 0 && seen[static_cast<size_t>(r)]))
+                throw std::runtime_error("blocks are not a partition of the rows");
+            if (r >= 0) seen[static_cast<size_t>(r)] = 1;
+        }
+        for (char c : seen)
+            if (!c) throw std::runtime_error("blocks are not a partition of the rows");
+        auto bs = std::make_unique<xp_blockset>();
+        bs->set = xp::BlockSet::from_csr(n, row_ptr, col_idx, values, b, lambda < 0.0 ? xp::kMlpLambda : lambda);
+        *out = bs.release();
+    });
+}
+
 int xp_strength_blocking(int n, const int* row_ptr, const int* col_idx, const double* values, int* slots,
                          int* nblocks) {
     return guarded([&] {

@@ -168,6 +168,8 @@ _mlp_nparams = _sig("xp_mlp_num_params", C.c_int, [])
 _mlp_get = _sig("xp_mlp_get_params", C.c_int, [_vp, _f32p])
 _mlp_set = _sig("xp_mlp_set_params", C.c_int, [_vp, _f32p])
 _bs_from = _sig("xp_blockset_from_csr", C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.POINTER(_vp)])
+_bs_from_slots = _sig("xp_blockset_from_csr_slots", C.c_int,
+                      [C.c_int, _i32p, _i32p, _f64p, _i32p, C.c_int, C.c_double, C.POINTER(_vp)])
 _bs_from_blocked = _sig("xp_blockset_from_csr_blocked", C.c_int,
                         [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_double, C.POINTER(_vp)])
 _strength_blocking = _sig("xp_strength_blocking", C.c_int,
@@ -943,12 +945,22 @@ class Pcg:
 class BlockSet:
     """Features / training targets of the aligned 8-row blocks (host)."""
 
-    def __init__(self, a: Csr = None, blocking: int = 0, lam: float = -1.0):
-        """lam: off-block weight of the training loss (default 0.5)."""
+    def __init__(self, a: Csr = None, blocking: int = 0, lam: float = -1.0, slots=None):
+        """lam: off-block weight of the training loss (default 0.5).
+        blocking 3 (the device aggregation) computes the blocks on the GPU;
+        slots (nblocks x 8, -1 padding) gives them explicitly."""
         self.h = None
         if a is not None:
             h = _vp()
-            _check(_bs_from_blocked(a.n, a.row_ptr, a.col_idx, a.values, blocking, lam, C.byref(h)))
+            if blocking == 3 and slots is None:
+                m = Matrix(a, block_rows=8, blocking=3)
+                slots = m.block_rows()
+                m.close()
+            if slots is not None:
+                sl = _i32(np.asarray(slots).ravel())
+                _check(_bs_from_slots(a.n, a.row_ptr, a.col_idx, a.values, sl, len(sl) // 8, lam, C.byref(h)))
+            else:
+                _check(_bs_from_blocked(a.n, a.row_ptr, a.col_idx, a.values, blocking, lam, C.byref(h)))
             self.h = h
 
     def append(self, other: "BlockSet") -> None:

@@ -57,3 +57,12 @@ def test_traffic_comes_from_the_committed_pmc_passes():
     assert prof["k_pcg_spmv_fused_acc"][0] == 133_288_200
     assert prof["k_pcg_update_xrz_acc"][0] == 60_541_950
     assert all(src.startswith("profiles/") for _, src in prof.values())
+
+
+def test_break_even_right_hand_sides():
+    """The learned method's set-up costs more and its solve less: it is ahead
+    from the first k with S_l + k T_l <= S_b + k T_b."""
+    B = _bench_module()
+    assert B.break_even({"setup_ms": 14.0, "solve_ms": 240.0}, {"setup_ms": 5.0, "solve_ms": 520.0}) == 1
+    assert B.break_even({"setup_ms": 900.0, "solve_ms": 240.0}, {"setup_ms": 5.0, "solve_ms": 520.0}) == 4
+    assert B.break_even({"setup_ms": 14.0, "solve_ms": 600.0}, {"setup_ms": 5.0, "solve_ms": 520.0}) is None
