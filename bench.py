@@ -706,12 +706,14 @@ def main() -> int:
     ap.add_argument("--blocking", type=int, default=3,
                     help="8-row blocks: 0 consecutive rows, 1 strength aggregates (row-order seeds), "
                          "2 strength-ordered aggregates (host), 3 parallel strength aggregates (device)")
-    ap.add_argument("--stall-limit", type=int, default=1024,
+    ap.add_argument("--stall-limit", type=int, default=4096,
                     help="learned solve: iterations between true-residual stagnation checks (0: off)")
-    # off-block weight of the training loss: on strength aggregates any
-    # sizeable weight moves the blocks away from the exact inverses the
-    # network starts from and costs iterations (0.3: +2%, 1.0: +10%)
-    ap.add_argument("--lam", type=float, default=0.05)
+    # off-block weight of the training loss: 0 makes the loss's minimiser the
+    # exact block inverse the network starts from; any off-block weight
+    # moves the blocks away from it and costs iterations (strength-ordered
+    # aggregates, FP32 iteration: 0.3 +2%, 1.0 +10%; parallel aggregates,
+    # accurate iteration: 0.05 +7%, 0.5 +14%)
+    ap.add_argument("--lam", type=float, default=0.0)
     ap.add_argument("--cpu-golden-worker", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_golden_worker:
