@@ -268,64 +268,73 @@ __device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int
 }
 
 // z = M r for the 8x8 block of this lane's row: the 8 rows of a block are 8
-// consecutive lanes (blocks start at multiples of 8 in solve order), so the
-// block's values are exchanged with cross-lane shuffles.  With
-// M = S C C^T S (BlockPre): u = S r, w = C^T u, z = S C w.  Every lane loads
-// the whole 80-byte factor of its block (the 8 lanes of a block read the
-// same lines) and computes all of w and of C w in FP32 from its unpacked
-// values, keeping its own row: register indices stay compile-time.
-// Every lane of the wavefront must call it.
-__device__ __forceinline__ float half_at(const uint4 (&q)[5], int t) {  // t compile-time after unrolling
-    const unsigned word = (t >> 1) % 4 == 0 ? q[t >> 3].x : (t >> 1) % 4 == 1 ? q[t >> 3].y
-                        : (t >> 1) % 4 == 2 ? q[t >> 3].z : q[t >> 3].w;
-    const unsigned short h = static_cast<unsigned short>((t & 1) ? (word >> 16) : (word & 0xFFFFu));
-    return static_cast<float>(__builtin_bit_cast(_Float16, h));
-}
-
-// The apply itself, from a block factor already in registers.
-__device__ __forceinline__ float block_apply_factor(const uint4 (&q)[5], float sj, int j, float r) {
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    const float u = sj * r;
-    // w = C^T u, accumulated row of C by row of C: one shuffled u_i live at a time
-    float w[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const float ui = __shfl(u, (lane & ~7) | i, 64);
-#pragma unroll
-        for (int k = 0; k <= i; ++k) w[k] = fmaf(half_at(q, i * (i + 1) / 2 + k), ui, w[k]);
-    }
-    float v = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        float acc = 0.0f;
-#pragma unroll
-        for (int k = 0; k <= i; ++k) acc = fmaf(half_at(q, i * (i + 1) / 2 + k), w[k], acc);
-        v = i == j ? acc : v;
-    }
-    return sj * v;
-}
-
-// A row's block factor and scale, loaded ahead of the residual it is
-// applied to (the update kernels request it with the row's vectors, so the
-// block is not a dependent load after r is known).
+// consecutive lanes (blocks start at multiples of 8 in solve order).  With
+// M = S C C^T S (BlockPre): u = S r, w = C^T u, z = S C w.  The factor is
+// spread over the block's lanes: lane i loads only the 20 bytes of its row
+// pair (row i of C) and its scale; it forms its terms C_ik u_i of every
+// w_k, the 8 lanes sum them with a butterfly of DPP moves inside the 8-lane
+// group (half-row mirror, then quad swaps), and each lane finishes its own
+// z_i = s_i sum_k C_ik w_k.  Every lane of the wavefront must call it.
 struct BlockRow {
-    uint4 q[5];
-    float s;
+    uint32_t w[5];  // the row pair's 10 FP16 entries
+    float s;        // s_i
 };
 
 __device__ __forceinline__ BlockRow block_load(const BlockPre* __restrict__ M, int row) {
-    const BlockPre* b = M + (row >> 3);
-    const uint4* cp = reinterpret_cast<const uint4*>(b->c);
+    const int i = row & 7;
+    const uint32_t* c = reinterpret_cast<const uint32_t*>(M[row >> 3].c) + 5 * (i < 4 ? i : 7 - i);
     BlockRow o;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) o.q[k] = cp[k];
-    o.s = b->s[row & 7];
+    for (int q = 0; q < 5; ++q) o.w[q] = c[q];
+    o.s = M[row >> 3].s[i];
     return o;
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+    const int o = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false);
+    return v + __builtin_bit_cast(float, o);
+}
+
+__device__ __forceinline__ float block_apply_factor(const BlockRow& b, int i, float r) {
+    // row i of C: the pair's halves off .. off + 7 (off = 0 for rows 0-3, 8 - i
+    // for rows 4-7), entries past the diagonal masked to 0
+    const int off = i < 4 ? 0 : 8 - i;
+    const int dw = off >> 1;
+    uint32_t e[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t w0 = b.w[q], w1 = b.w[q + 1], w2 = q + 2 < 5 ? b.w[q + 2] : 0u, w3 = q + 3 < 5 ? b.w[q + 3] : 0u;
+        const uint32_t lo = dw == 0 ? w0 : (dw == 1 ? w1 : w2);
+        const uint32_t hi = dw == 0 ? w1 : (dw == 1 ? w2 : w3);
+        e[q] = (off & 1) ? __builtin_amdgcn_alignbit(hi, lo, 16) : lo;
+    }
+    float c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const unsigned short h = static_cast<unsigned short>((k & 1) ? (e[k >> 1] >> 16) : (e[k >> 1] & 0xFFFFu));
+        c[k] = k <= i ? static_cast<float>(__builtin_bit_cast(_Float16, h)) : 0.0f;
+    }
+    const float u = b.s * r;
+    float w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = c[k] * u;
+    // sums over the 8 lanes of the block: pairs (i, 7 - i), then (i, i ^ 2), (i, i ^ 1)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = dpp_add<0x141>(w[k]);  // row_half_mirror
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = dpp_add<0x4E>(w[k]);   // quad_perm(2, 3, 0, 1)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[k] = dpp_add<0xB1>(w[k]);   // quad_perm(1, 0, 3, 2)
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc = fmaf(c[k], w[k], acc);
+    return b.s * acc;
 }
 
 __device__ __forceinline__ float block_apply(const BlockPre* __restrict__ M, int row, float r) {
     const BlockRow b = block_load(M, row);
-    return block_apply_factor(b.q, b.s, row & 7, r);
+    return block_apply_factor(b, row & 7, r);
 }
 
 // Factors full FP32 blocks (solve order, block q = 64 floats row-major) into
@@ -360,13 +369,12 @@ __global__ void k_block_factor(int nblocks, const float* __restrict__ full, Bloc
         }
     }
     BlockPre o;
-    int t = 0;
+    for (int t = 0; t < 40; ++t) o.c[t] = 0;
     for (int i = 0; i < 8; ++i)
-        for (int k = 0; k <= i; ++k, ++t) {
+        for (int k = 0; k <= i; ++k) {
             const float v = ok ? static_cast<float>(l[i][k]) : (i == k ? 1.0f : 0.0f);
-            o.c[t] = __builtin_bit_cast(unsigned short, static_cast<_Float16>(v));
+            o.c[block_half_index(i, k)] = __builtin_bit_cast(unsigned short, static_cast<_Float16>(v));
         }
-    for (; t < 40; ++t) o.c[t] = 0;
     for (int i = 0; i < 8; ++i) o.s[i] = static_cast<float>(s[i]);
     out[q] = o;
 }
@@ -485,7 +493,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
                                                            const float* __restrict__ minv,
                                                            const BlockPre* __restrict__ blockM,
                                                            float* __restrict__ z) {
-    // the block apply holds a block factor per row in flight: one row per pass
+    // with blocks one row per pass (two or more measured slower in the
+    // accurate form's update, below)
     constexpr int ROWS = BLOCK ? 1 : kRowsPerThread;
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
@@ -528,7 +537,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz(Launch L, const Sy
                 const float xn = static_cast<float>(static_cast<double>(xv[k]) + alpha * static_cast<double>(pv[k]));
                 const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * static_cast<double>(av[k]));
                 const float zn = BLOCK
-                                     ? block_apply_factor(bl[k].q, bl[k].s, row & 7, rn)
+                                     ? block_apply_factor(bl[k], row & 7, rn)
                                      : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
                 x[row] = xn;
                 r[row] = rn;
@@ -855,8 +864,11 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
                                                                const float* __restrict__ minv,
                                                                const BlockPre* __restrict__ blockM,
                                                                float* __restrict__ z) {
-    // measured: two rows per pass is slower with blocks (also in the even
-    // launches, which leave x alone: 12.6 -> 16.9 us per update)
+    // One row per pass with blocks.  Measured on the 1.2M-row FEM matrix
+    // (average of an even and an odd update, 256 / 512 workgroups): the
+    // distributed block factor 12.50 / 12.33 us against 13.14 / 12.68 with
+    // every lane loading the whole factor; two rows per pass 12.84 / 13.13,
+    // three in the even launches 13.41 / 13.72.
     constexpr int ROWS = BLOCK ? 1 : kRowsAcc;
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
@@ -907,7 +919,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
             if (row < r1) {  // uniform per wavefront: r1 - base is a multiple of 64
                 const float rn = static_cast<float>(static_cast<double>(rv[k]) - alpha * av[k]);
                 const float zn = BLOCK
-                                     ? block_apply_factor(bl[k].q, bl[k].s, row & 7, rn)
+                                     ? block_apply_factor(bl[k], row & 7, rn)
                                      : static_cast<float>(static_cast<double>(mv[k]) * static_cast<double>(rn));
                 if (XM) x[row] = (xv[k] + a_prev * qv[k]) + alpha * pv[k];
                 r[row] = rn;

@@ -2812,10 +2812,10 @@ extern "C" int xp_pcg_block_operator(xp_pcg* s, double* out) {
             if (bid < 0) continue;
             const xp::BlockPre& f = h[static_cast<size_t>(q)];
             double c[8][8] = {};
-            int t = 0;
             for (int i = 0; i < 8; ++i)
-                for (int k = 0; k <= i; ++k, ++t)
-                    c[i][k] = static_cast<double>(static_cast<float>(__builtin_bit_cast(_Float16, f.c[t])));
+                for (int k = 0; k <= i; ++k)
+                    c[i][k] = static_cast<double>(
+                        static_cast<float>(__builtin_bit_cast(_Float16, f.c[xp::block_half_index(i, k)])));
             for (int i = 0; i < 8; ++i) {
                 if (m.perm[static_cast<size_t>(q) * 8 + static_cast<size_t>(i)] < 0) continue;
                 for (int j = 0; j < 8; ++j) {

@@ -101,14 +101,21 @@ struct TraceRow {
 
 // A block preconditioner's block q (solve rows 8q .. 8q+7), factored:
 // M_q = diag(s) C C^T diag(s) with C lower triangular with unit-scale
-// entries (the Cholesky factor of M_q with its diagonal scaled to 1), stored
-// as 36 FP16 values r-major (r, c <= r), padded to 40; 112 bytes per block,
-// 14 per row.
+// entries (the Cholesky factor of M_q with its diagonal scaled to 1), the
+// 36 entries stored as FP16 in four 10-entry pairs of rows: pair p (0..3)
+// holds row p (p + 1 entries) then row 7 - p (8 - p entries) and one pad,
+// so the lane of row i finds its whole row in the 20 bytes of pair
+// min(i, 7 - i) (block_half_index).  112 bytes per block, 14 per row.
 struct alignas(16) BlockPre {
     uint16_t c[40];
     float s[8];
 };
 static_assert(sizeof(BlockPre) == 112, "BlockPre layout");
+
+// Position in BlockPre::c of C[i][k], k <= i.
+constexpr int block_pair(int i) { return i < 4 ? i : 7 - i; }
+constexpr int block_row_offset(int i) { return i < 4 ? 0 : 8 - i; }  // within the pair
+constexpr int block_half_index(int i, int k) { return 10 * block_pair(i) + block_row_offset(i) + k; }
 
 // Device view of a SELL matrix (all pointers are device pointers).
 struct SellView {
