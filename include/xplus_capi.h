@@ -259,6 +259,17 @@ typedef struct {
                           the solve was redone with plain Jacobi */
 } xp_status;
 
+/* dot_kernel on device vectors, allocating nothing: out[0] = a^T b (FP64,
+ * the same kernels, split and summation order as dot_kernel, so the same
+ * bits), stream-ordered on `stream` (NULL: the library stream).  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its per-workgroup partials live in one fixed
+ * 8 KiB device array per GPU (at most 1024 partials; larger splits are
+ * refused with an error), held until the process ends; calls that overlap
+ * on one GPU share that array and race.  n < 0: no-op (as dot_kernel). */
+int xp_dot_f64_device(const double* a, const double* b, double* out, int n, void* stream);
+
 int xp_pcg_create(xp_matrix* m, const xp_solve_opts* opts, xp_pcg** out);
 void xp_pcg_destroy(xp_pcg* s);
 /* Uploads b and x0 (host, natural order, FP64, sum of n_k entries; x0 may

@@ -1284,6 +1284,54 @@ int xp_spmv_f32_time_cold(xp_matrix* m, int reps, unsigned long long flush_bytes
     });
 }
 
+}  // extern "C"
+
+// ---- allocation-free FP64 dot -------------------------------------------------------
+// dot_kernel's kernels (k_dot_f64 + k_flat_finish, one FP64 partial per
+// workgroup) on device vectors, with the partials in a fixed device array
+// instead of library-allocated memory: no hipMalloc / hipFree on any call.
+namespace {
+
+constexpr int kScratchDoubles = 1024;  // one partial per workgroup, at most 1024 workgroups: 8 KiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[256] = {};
+
+// The device address of `scratch` on the current device (each GPU holds its
+// own copy), looked up once per device.
+double* scratch_on_current_device() {
+    int dev = 0;
+    HIPX(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int xp_dot_f64_device(const double* a, const double* b, double* out, int n, void* stream) {
+    if (n < 0) return 0;  // the reference's silent no-op for a negative size
+    return guarded([&] {
+        if (a == nullptr || b == nullptr || out == nullptr) throw std::runtime_error("null argument");
+        const hipStream_t s = pick_stream(stream);
+        double* partials = scratch_on_current_device();
+        const int grid = std::max(1, std::min(1024, (n + 1023) / 1024));  // dot_kernel's split
+        if (grid > kScratchDoubles)
+            throw std::runtime_error("xp_dot_f64_device: " + std::to_string(grid) + " partials exceed the " +
+                                     std::to_string(kScratchDoubles) + "-entry scratch array");
+        xp::launch_dot_f64(n, a, b, out, partials, grid, s);
+        HIPX(hipGetLastError());
+    });
+}
+
 int xp_csr_spmv_f64(int n, const int* row_ptr, const int* col_idx, const double* values,
                     const double* x, double* y) {
     if (n <= 0) return 0;
