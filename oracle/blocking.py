@@ -152,7 +152,8 @@ def strength_blocking(row_ptr, col_idx, values, rule=1):
 
 # ---------------------------------------------------------------- rule 3
 GROW_STEPS = 8      # growth steps per round
-PACK_WINDOW = 16    # packing looks at the last 16 packed blocks
+PACK_WINDOW = 16    # packing looks at the last 16 packed blocks of its segment
+PACK_SEGMENT = 32   # pieces are packed in independent runs of 32 (one device thread each)
 
 
 def _f32_bits(v):
@@ -265,8 +266,10 @@ def parallel_blocking(row_ptr, col_idx, values, log=None):
       regroup: the rows of blocks with fewer than 8 rows are pooled and
               the rounds rerun on the pool;
       pack:   the blocks still smaller than 8 (pieces), rows ascending,
-              in order of their smallest row, each into the most recent of
-              the last PACK_WINDOW packed blocks with room, else a new one."""
+              in order of their smallest row, cut into consecutive runs of
+              PACK_SEGMENT pieces; within a run each piece goes into the most
+              recent of the run's last PACK_WINDOW packed blocks with room,
+              else a new one (packed blocks never mix runs)."""
     n = len(row_ptr) - 1
     if n == 0:
         return []
@@ -285,13 +288,16 @@ def parallel_blocking(row_ptr, col_idx, values, log=None):
     full = [sorted(members[B]) for B in range(n) if size[B] == 8]
     pieces = sorted((sorted(members[B]) for B in range(n) if 0 < size[B] < 8), key=lambda b: b[0])
     packed = []
-    for p in pieces:
-        for q in reversed(packed[-PACK_WINDOW:]):
-            if len(q) + len(p) <= 8:
-                q.extend(p)
-                break
-        else:
-            packed.append(list(p))
+    for s0 in range(0, len(pieces), PACK_SEGMENT):
+        run = []
+        for p in pieces[s0:s0 + PACK_SEGMENT]:
+            for q in reversed(run[-PACK_WINDOW:]):
+                if len(q) + len(p) <= 8:
+                    q.extend(p)
+                    break
+            else:
+                run.append(list(p))
+        packed.extend(run)
     out = full + [sorted(q) for q in packed]
     out.sort(key=lambda b: b[0])
     return out
