@@ -31,7 +31,7 @@
 #include <string>
 #include <vector>
 
-#include "multi_kernel_solver.hpp"
+#include "host_loop.hpp"
 
 namespace {
 
@@ -205,7 +205,7 @@ int device_pcg(const Cli& cli, const xp::CsrDataset& d) {
     check(xp_pcg_get_x(s, x.data()));
     double res_abs = 0.0;
     check(xp_pcg_true_residual(s, &res_abs));
-    const double bnorm = xp::l2(d.b());
+    const double bnorm = xp::norm2(d.b());
     // the SpMV-carrying kernel of the iteration, timed per launch
     double us[3] = {0.0, 0.0, 0.0};
     int nk = 0;
@@ -213,11 +213,11 @@ int device_pcg(const Cli& cli, const xp::CsrDataset& d) {
     xp_pcg_destroy(s);
     xp_matrix_destroy(m);
 
-    xp::SolverConfig cfg;
+    xp::LoopSettings cfg;
     cfg.tau = cli.tau;
     cfg.max_iters = cli.max_iters;
-    const xp::MultiKernelResult golden = xp::run_local_multi_kernel_solver(d, cfg, nullptr, spmv_csr_kernel);
-    const Diff df = max_diff(x, golden.solution);
+    const xp::LoopResult golden = xp::run_host_pcg(d, cfg, nullptr, spmv_csr_kernel);
+    const Diff df = max_diff(x, golden.x);
     const bool converged = st.status == XP_CONVERGED;
     const char* status = st.status == XP_CONVERGED ? "converged"
                          : st.status == XP_BREAKDOWN ? "breakdown" : "max_iter";

@@ -24,7 +24,7 @@
 #include <string>
 #include <vector>
 
-#include "multi_kernel_solver.hpp"
+#include "host_loop.hpp"
 #include "report_io.hpp"
 
 namespace {
@@ -162,12 +162,12 @@ int main(int argc, char** argv) {
         r.converged = st.status == XP_CONVERGED;
         r.breakdown = st.status == XP_BREAKDOWN;
         check(xp_pcg_true_residual(s, &r.residual_l2));
-        r.residual_rel = r.residual_l2 / std::max(xp::l2(d.b()), xp::kBreakdownEps);
-        xp::SolverConfig cfg;
+        r.residual_rel = r.residual_l2 / std::max(xp::norm2(d.b()), xp::kBreakdownEps);
+        xp::LoopSettings cfg;
         cfg.tau = opt.tau;
         cfg.max_iters = opt.max_iters;
-        const xp::MultiKernelResult golden = xp::run_local_multi_kernel_solver(d, cfg, nullptr, spmv_csr_kernel);
-        for (size_t k = 0; k < x.size(); ++k) r.max_abs_diff = std::max(r.max_abs_diff, std::fabs(x[k] - golden.solution[k]));
+        const xp::LoopResult golden = xp::run_host_pcg(d, cfg, nullptr, spmv_csr_kernel);
+        for (size_t k = 0; k < x.size(); ++k) r.max_abs_diff = std::max(r.max_abs_diff, std::fabs(x[k] - golden.x[k]));
         r.golden_residual_l2 = golden.residual_l2;
         r.pass = r.converged && r.max_abs_diff <= 1.0e-8;
         for (int k = 0; k < got; ++k) {

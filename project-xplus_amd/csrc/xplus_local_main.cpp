@@ -13,7 +13,7 @@
 #include <stdexcept>
 #include <string>
 
-#include "multi_kernel_solver.hpp"
+#include "host_loop.hpp"
 
 namespace {
 
@@ -56,16 +56,16 @@ int main(int argc, char** argv) {
     try {
         const Cli cli = parse(argc, argv);
         const xp::CsrDataset d = xp::CsrDataset::load(cli.dir);
-        xp::SolverConfig cfg;
+        xp::LoopSettings cfg;
         cfg.tau = cli.tau;
         cfg.max_iters = cli.max_iters;
         std::cout << "[xplus] dataset=" << cli.dir << " mode=local-multi-kernel spmv="
                   << (cli.blocked ? "blocked" : "csr") << " device=mi355x\n";
-        const xp::MultiKernelResult res = xp::run_local_multi_kernel_solver(
+        const xp::LoopResult res = xp::run_host_pcg(
             d, cfg, &std::cout, cli.blocked ? spmv_blocked_kernel : spmv_csr_kernel);
         std::cout << std::scientific << std::setprecision(12);
-        std::cout << "[done] iter=" << res.iterations << " residual_abs=" << res.residual_l2
-                  << " residual_rel=" << res.residual_rel << " status=" << xp::to_string(res.status) << "\n";
+        std::cout << "[done] iter=" << res.steps << " residual_abs=" << res.residual_l2
+                  << " residual_rel=" << res.residual_rel << " status=" << xp::outcome_name(res.outcome) << "\n";
         if (cli.device_solver) {
             xp_matrix* m = nullptr;
             xp_matrix_opts mo{};
@@ -92,7 +92,7 @@ int main(int argc, char** argv) {
             double ms = 0.0;
             xp_pcg_last_timing(s, &ms, nullptr);
             double md = 0.0;
-            for (size_t k = 0; k < x.size(); ++k) md = std::max(md, std::fabs(x[k] - res.solution[k]));
+            for (size_t k = 0; k < x.size(); ++k) md = std::max(md, std::fabs(x[k] - res.x[k]));
             std::cout << "[device] iter=" << st.iterations << " status=" << st.status << " rr=" << st.rr
                       << " residual_abs=" << res_norm << " solve_ms=" << ms << " max_abs_diff_vs_multi_kernel=" << md
                       << "\n";
