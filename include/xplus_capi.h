@@ -84,7 +84,9 @@ typedef struct {
     int keep_lo; /* keep the FP32 low-part stream (FP64-accurate SpMV, refinement) */
     int keep64;  /* keep exact FP64 values (drop-in FP64 SpMV) */
     int grid;    /* workgroups of the streaming kernels; 0 = library default */
-    int spmv_variant; /* SpMV inner loop: 0 rolled loop, 1 fixed-width unrolled */
+    int spmv_variant; /* SpMV inner loop: 0 rolled loop, 1 fixed-width unrolled,
+                         2 unrolled with non-temporal loads of the matrix streams
+                         (for matrices that do not stay in the 256 MB MALL) */
     int block_rows;   /* 8: keep blocks of up to 8 rows together as 8 solve rows
                          (block preconditioners); 0/1: rows */
     int blocking;     /* with block_rows = 8: 0 = blocks of 8 consecutive rows;
@@ -181,6 +183,11 @@ int xp_update_p_f32(xp_matrix* m, const float* z, float* p, double beta, void* s
 int xp_to_solve_order(xp_matrix* m, const double* nat, float* solve, void* stream);
 int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* stream);
 /* Host-pointer convenience: y = A x in FP64 on the GPU for a CSR given once. */
+/* The achievable streaming rate of the GPU, for roofline figures: a
+ * project-written kernel reads (mode 0) or copies (mode 1) `bytes` with
+ * 16-byte loads in `grid` workgroups; average microseconds per launch over
+ * `reps` launches, each between its own events.  No reference counterpart. */
+int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, double* avg_us);
 int xp_csr_spmv_f64(int n, const int* row_ptr, const int* col_idx, const double* values,
                     const double* x, double* y);
 

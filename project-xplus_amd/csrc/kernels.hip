@@ -147,15 +147,25 @@ __device__ __forceinline__ bool bad(double v) { return !isfinite(v); }
 // ~3 w loads in flight; the width test is wave-uniform (no divergence).
 constexpr int kFastW = 10;
 
+// Matrix streams read once per SpMV: V = 2 loads them non-temporally (they
+// do not displace the vectors from the caches; faster when the matrix does
+// not stay in the 256 MB MALL between SpMVs, slower when it does).
+template <bool NT, typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
 template <bool IDX16, int V>
 __device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int lane,
                                               const float* __restrict__ x, float& xdiag) {
+    constexpr bool NT = V == 2;
     const int j0 = A.slice_ptr[slice];
     const int w = A.slice_ptr[slice + 1] - j0;
     const int row = slice * kC + lane;
     const size_t e0 = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
     const int base = A.slice_base[slice];
-    if (V == 1 && w <= kFastW) {
+    if (V >= 1 && w <= kFastW) {
         // Branch-free: slots past the row's width re-read its last slot (a
         // cached line) and contribute 0, so every load is issued before the
         // first use instead of one guarded load at a time.
@@ -164,8 +174,9 @@ __device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int 
 #pragma unroll
         for (int k = 0; k < kFastW; ++k) {
             const size_t e = e0 + static_cast<size_t>(k < w ? k : w - 1) * kC;
-            v[k] = k < w ? A.val[e] : 0.0f;
-            c[k] = k == 0 ? row : (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]);
+            v[k] = k < w ? stream_load<NT>(A.val + e) : 0.0f;
+            c[k] = k == 0 ? row
+                          : (IDX16 ? base + static_cast<int>(stream_load<NT>(A.col16 + e)) : stream_load<NT>(A.col32 + e));
         }
 #pragma unroll
         for (int k = 0; k < kFastW; ++k) xv[k] = x[c[k]];
@@ -215,7 +226,7 @@ __device__ __forceinline__ double sell_row_f64(const SellView& A, int slice, int
 // once in FP64 and the row sum carries only FP64 rounding, so A x keeps its
 // relative accuracy when the row's terms cancel (smooth x on an
 // ill-conditioned matrix), which an FP32 sum does not.  xdiag = x[row].
-template <bool IDX16, int VMODE>
+template <bool IDX16, int VMODE, bool NT = false>
 __device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int lane, const float* __restrict__ x,
                                                float& xdiag) {
     const int j0 = A.slice_ptr[slice];
@@ -224,14 +235,14 @@ __device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int
     const size_t e0 = static_cast<size_t>(j0) * kC + static_cast<size_t>(lane);
     const int base = A.slice_base[slice];
     auto value = [&](size_t e) -> double {
-        if (VMODE == 0) return A.val64[e];
+        if (VMODE == 0) return stream_load<NT>(A.val64 + e);
         if (VMODE == 2) {  // hi + lo16 * 2^(E-38), the double's exponent field built from hi's
-            const float hi = A.val[e];
+            const float hi = stream_load<NT>(A.val + e);
             const int eb = static_cast<int>((__float_as_uint(hi) >> 23) & 0xFFu);  // E + 127
             const double scale = __longlong_as_double(static_cast<long long>(eb - 127 - 38 + 1023) << 52);
-            return static_cast<double>(hi) + static_cast<double>(A.val_lo16[e]) * scale;
+            return static_cast<double>(hi) + static_cast<double>(stream_load<NT>(A.val_lo16 + e)) * scale;
         }
-        return static_cast<double>(A.val[e]) + static_cast<double>(A.val_lo[e]);
+        return static_cast<double>(stream_load<NT>(A.val + e)) + static_cast<double>(stream_load<NT>(A.val_lo + e));
     };
     if (w <= kFastW) {
         // Branch-free, as in sell_row_f32: slots past the row's width re-read
@@ -242,7 +253,8 @@ __device__ __forceinline__ double sell_row_acc(const SellView& A, int slice, int
 #pragma unroll
         for (int k = 0; k < kFastW; ++k) {
             const size_t e = e0 + static_cast<size_t>(k < w ? k : w - 1) * kC;
-            c[k] = k == 0 ? row : (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]);
+            c[k] = k == 0 ? row
+                          : (IDX16 ? base + static_cast<int>(stream_load<NT>(A.col16 + e)) : stream_load<NT>(A.col32 + e));
         }
 #pragma unroll
         for (int k = 0; k < kFastW; ++k) xv[k] = x[c[k]];
@@ -747,7 +759,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused(Launch L, SellView
 // A z (and loads its p, ap) before waiting on the cross-workgroup partial
 // sums and the scalar step: with ~2-3 slices per wavefront that wait would
 // otherwise be exposed.
-template <bool IDX16, int VMODE>
+template <bool IDX16, int VMODE, bool NT>
 __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, SellView A, const SysState* Sc,
                                                                   SysState* Sn, TraceRow* trace, int trace_stride,
                                                                   const float* __restrict__ z,
@@ -771,7 +783,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
         const int row = sl * kC + lane;
         po0 = p_old[row];
         apo0 = ap[row];
-        az0 = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd0);
+        az0 = sell_row_acc<IDX16, VMODE, NT>(A, sl, lane, z, zd0);
     }
     double t[3];
     sum_partials<3>(L, w.sys, {L.pap_prev, kSlotRz, kSlotRr}, t);  // ends with a barrier
@@ -828,7 +840,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_spmv_fused_acc(Launch L, Sell
         const double po = fresh ? 0.0 : p_old[row];
         const double apo = fresh ? 0.0 : ap[row];
         float zd;
-        const double az = sell_row_acc<IDX16, VMODE>(A, sl, lane, z, zd);
+        const double az = sell_row_acc<IDX16, VMODE, NT>(A, sl, lane, z, zd);
         double pn, an;
         if (fresh) {
             pn = static_cast<double>(zd);
@@ -964,6 +976,31 @@ __global__ __launch_bounds__(kBlock, 8) void k_spmv_f32(Launch L, SellView A, co
         float xd;
         y[sl * kC + lane] = sell_row_f32<IDX16, V>(A, sl, lane, x, xd);
     }
+}
+
+// Streaming reference kernels for the achievable HBM rate: every byte of a
+// buffer read once with 16-byte loads (XCD-aware contiguous ranges, the
+// SpMV's mapping), folded into one word per workgroup so the loads are not
+// dropped; and a 16-byte copy (read + write).
+__global__ __launch_bounds__(kBlock, 8) void k_stream_read(const uint4* __restrict__ src, size_t n16,
+                                                        unsigned* __restrict__ sink) {
+    const int lb = logical_block();
+    const size_t per = (n16 + gridDim.x - 1) / gridDim.x;
+    const size_t i0 = static_cast<size_t>(lb) * per, i1 = min(n16, i0 + per);
+    unsigned acc = 0;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += kBlock) {
+        const uint4 v = src[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9E3779B9u) sink[lb] = acc;  // data-dependent, almost never taken
+}
+
+__global__ __launch_bounds__(kBlock, 8) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                        size_t n16) {
+    const int lb = logical_block();
+    const size_t per = (n16 + gridDim.x - 1) / gridDim.x;
+    const size_t i0 = static_cast<size_t>(lb) * per, i1 = min(n16, i0 + per);
+    for (size_t i = i0 + threadIdx.x; i < i1; i += kBlock) dst[i] = src[i];
 }
 
 template <bool IDX16, int VMODE>
@@ -1700,10 +1737,12 @@ __global__ void k_mask_padding_blocks(int n_pad, const int* __restrict__ perm, f
 #define XP_LAUNCH_SELL(L, A, K, ...)                                                              \
     do {                                                                                       \
         if ((A).idx16) {                                                                       \
-            if ((A).variant == 1) XP_LAUNCH(L, (K<true, 1>), dim3((L).grid), __VA_ARGS__);       \
+            if ((A).variant == 2) XP_LAUNCH(L, (K<true, 2>), dim3((L).grid), __VA_ARGS__);       \
+            else if ((A).variant == 1) XP_LAUNCH(L, (K<true, 1>), dim3((L).grid), __VA_ARGS__);  \
             else XP_LAUNCH(L, (K<true, 0>), dim3((L).grid), __VA_ARGS__);                        \
         } else {                                                                               \
-            if ((A).variant == 1) XP_LAUNCH(L, (K<false, 1>), dim3((L).grid), __VA_ARGS__);      \
+            if ((A).variant == 2) XP_LAUNCH(L, (K<false, 2>), dim3((L).grid), __VA_ARGS__);      \
+            else if ((A).variant == 1) XP_LAUNCH(L, (K<false, 1>), dim3((L).grid), __VA_ARGS__); \
             else XP_LAUNCH(L, (K<false, 0>), dim3((L).grid), __VA_ARGS__);                       \
         }                                                                                      \
     } while (0)
@@ -1749,6 +1788,17 @@ void launch_pcg_update_xrz(const Launch& L, const SysState* st, float* x, const 
 void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_next, TraceRow* trace,
                          int trace_stride, const float* z, float* p) {
     XP_LAUNCH(L, k_pcg_update_p, dim3(L.grid), L, st_cur, st_next, trace, trace_stride, z, p);
+}
+
+void launch_stream(int grid, const void* src, void* dst, size_t bytes, unsigned* sink, hipStream_t s,
+                   hipEvent_t e0, hipEvent_t e1) {
+    const size_t n16 = bytes / 16;
+    if (dst == nullptr)
+        hipExtLaunchKernelGGL(k_stream_read, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0,
+                              static_cast<const uint4*>(src), n16, sink);
+    else
+        hipExtLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0,
+                              static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
 }
 
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y) {
@@ -1893,9 +1943,15 @@ void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysSta
 void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, int vmode, const SysState* st_cur,
                                SysState* st_next, TraceRow* trace, int trace_stride, const float* z,
                                const double* p_old, double* p_new, double* ap) {
-#define XP_FUSED_ACC(I, V)                                                                               \
-    XP_LAUNCH(L, (k_pcg_spmv_fused_acc<I, V>), dim3(L.grid), L, A, st_cur, st_next, trace, trace_stride, z, p_old, \
-              p_new, ap)
+#define XP_FUSED_ACC(I, V)                                                                                   \
+    do {                                                                                                     \
+        if (A.variant == 2)                                                                                  \
+            XP_LAUNCH(L, (k_pcg_spmv_fused_acc<I, V, true>), dim3(L.grid), L, A, st_cur, st_next, trace,     \
+                      trace_stride, z, p_old, p_new, ap);                                                    \
+        else                                                                                                 \
+            XP_LAUNCH(L, (k_pcg_spmv_fused_acc<I, V, false>), dim3(L.grid), L, A, st_cur, st_next, trace,    \
+                      trace_stride, z, p_old, p_new, ap);                                                    \
+    } while (0)
     if (A.idx16) {
         if (vmode == 2) XP_FUSED_ACC(true, 2);
         else if (vmode == 1) XP_FUSED_ACC(true, 1);

@@ -317,7 +317,7 @@ std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, c
     auto m = std::make_unique<xp_matrix>();
     m->keep_lo = o != nullptr && o->keep_lo;
     m->keep64 = o != nullptr && o->keep64;
-    m->variant = (o != nullptr && o->spmv_variant == 1) ? 1 : 0;
+    m->variant = (o != nullptr && o->spmv_variant >= 1 && o->spmv_variant <= 2) ? o->spmv_variant : 0;
     m->block_rows = (o != nullptr && o->block_rows == 8) ? 8 : 1;
     for (const auto& a : systems) {
         if (a.n < 0) throw std::runtime_error("build_sell: negative n");
@@ -360,7 +360,7 @@ std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const i
     auto m = std::make_unique<xp_matrix>();
     m->keep_lo = o != nullptr && o->keep_lo;
     m->keep64 = o != nullptr && o->keep64;
-    m->variant = (o != nullptr && o->spmv_variant == 1) ? 1 : 0;
+    m->variant = (o != nullptr && o->spmv_variant >= 1 && o->spmv_variant <= 2) ? o->spmv_variant : 0;
     m->block_rows = (o != nullptr && o->block_rows == 8) ? 8 : 1;
     m->blocking = m->block_rows == 8 && o != nullptr ? o->blocking : 0;
     if (m->block_rows == 8 && m->blocking != 0 && m->blocking != 3)
@@ -1097,7 +1097,7 @@ int xp_matrix_set_grid(xp_matrix* m, int grid) {
 
 int xp_matrix_set_variant(xp_matrix* m, int spmv_variant) {
     return guarded([&] {
-        if (m == nullptr || spmv_variant < 0 || spmv_variant > 1) throw std::runtime_error("bad argument");
+        if (m == nullptr || spmv_variant < 0 || spmv_variant > 2) throw std::runtime_error("bad argument");
         m->variant = spmv_variant;
         ++m->layout_version;  // captured graphs hold the previous kernel
     });
@@ -1329,6 +1329,36 @@ int xp_dot_f64_device(const double* a, const double* b, double* out, int n, void
                                      std::to_string(kScratchDoubles) + "-entry scratch array");
         xp::launch_dot_f64(n, a, b, out, partials, grid, s);
         HIPX(hipGetLastError());
+    });
+}
+
+// The achievable streaming rate: a project-written read (mode 0) or copy
+// (mode 1) of `bytes` through 16-byte loads, `reps` launches after 3
+// untimed ones, each timed with its own events; buffers of `bytes` each
+// (above the 256 MB MALL, every launch streams from HBM).
+int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, double* avg_us) {
+    return guarded([&] {
+        if (avg_us == nullptr || reps <= 0 || bytes < 16 || (mode != 0 && mode != 1) || grid <= 0)
+            throw std::runtime_error("bad argument");
+        const hipStream_t s = device().stream;
+        const size_t b = static_cast<size_t>(bytes) / 16 * 16;
+        DevBuf<uint4> src(b / 16), dst(mode == 1 ? b / 16 : 0);
+        DevBuf<unsigned> sink(static_cast<size_t>(grid));
+        HIPX(hipMemsetAsync(src.p, 1, b, s));
+        std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps + 3));
+        for (auto& e : ev) HIPX(hipEventCreate(&e));
+        for (int k = 0; k < reps + 3; ++k)
+            xp::launch_stream(grid, src.p, mode == 1 ? dst.p : nullptr, b, sink.p, s, ev[2 * static_cast<size_t>(k)],
+                              ev[2 * static_cast<size_t>(k) + 1]);
+        HIPX(hipStreamSynchronize(s));
+        double total = 0.0;
+        for (int k = 3; k < reps + 3; ++k) {
+            float ms = 0.0f;
+            HIPX(hipEventElapsedTime(&ms, ev[2 * static_cast<size_t>(k)], ev[2 * static_cast<size_t>(k) + 1]));
+            total += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *avg_us = 1000.0 * total / reps;
     });
 }
 
@@ -2622,8 +2652,11 @@ extern "C" int xp_matrix_autotune(xp_matrix* m, int fused, int tune_sigma, int i
                 std::vector<int> grids = {256, 512, 768, 1024, 2048, default_grid(*m)};
                 std::sort(grids.begin(), grids.end());
                 grids.erase(std::unique(grids.begin(), grids.end()), grids.end());
-                // the accurate iteration has one SpMV inner loop: no variant axis
-                for (int v = 0; v <= (fused == 2 ? 0 : 1); ++v) {
+                // variants: 0 rolled inner loop, 1 unrolled, 2 unrolled with
+                // non-temporal matrix loads; the accurate iteration's SpMV has
+                // one inner loop, with (2) or without (0) non-temporal loads
+                for (int v = 0; v <= 2; ++v) {
+                    if (fused == 2 && v == 1) continue;
                     m->variant = v;
                     for (int g : grids) {
                         if (g > m->nslices) continue;

@@ -119,6 +119,7 @@ _op_dot = _sig("xp_dot_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp])
 _op_xrz = _sig("xp_update_xrz_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp])
 _op_p = _sig("xp_update_p_f32", C.c_int, [_vp, _vp, _vp, C.c_double, _vp])
 _spmv_time = _sig("xp_spmv_f32_time", C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_double)])
+_stream_time = _sig("xp_stream_time", C.c_int, [C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)])
 _spmv_time_cold = _sig("xp_spmv_f32_time_cold", C.c_int, [_vp, C.c_int, C.c_ulonglong, C.POINTER(C.c_double)])
 
 _p_create = _sig("xp_pcg_create", C.c_int, [_vp, C.POINTER(SolveOpts), C.POINTER(_vp)])
@@ -589,6 +590,14 @@ class DeviceArray:
 
     def __del__(self):
         self.close()
+
+
+def stream_time(nbytes: int, mode: int = 0, grid: int = 2048, reps: int = 20) -> float:
+    """Average us per launch of the streaming read (mode 0) / copy (mode 1)
+    reference kernel over `nbytes`."""
+    us = C.c_double()
+    _check(_stream_time(nbytes, mode, grid, reps, C.byref(us)))
+    return us.value
 
 
 class DeviceCsr:

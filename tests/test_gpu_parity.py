@@ -424,3 +424,31 @@ def test_batched_matches_individual_and_golden(gpu):
         xi = xb[off:off + a.n]
         assert np.linalg.norm(xi - g["x"]) / np.linalg.norm(g["x"]) <= 1e-5
         off += a.n
+
+
+def test_nontemporal_variant_is_the_same_arithmetic(gpu):
+    """SpMV variant 2 (non-temporal matrix loads) only changes how the
+    streams are fetched: the FP32 SpMV equals variant 1 bit for bit, and an
+    accurate solve equals variant 0's (same iteration count, same x)."""
+    a = X.gen_fem2d(120, 1.0, 8, 1e3, 0.3, 12)
+    m = X.Matrix(a, sigma=64, keep_lo=True)
+    n_pad = m.info()["n_pad"]
+    x = X.DeviceArray.from_numpy(np.random.default_rng(1).standard_normal(n_pad).astype(np.float32))
+    y = X.DeviceArray(n_pad, np.float32)
+    out = {}
+    for v in (1, 2):
+        m.set_variant(v)
+        m.spmv_f32(x.ptr, y.ptr)
+        X.synchronize()
+        out[v] = y.numpy()
+    np.testing.assert_array_equal(out[1], out[2])
+    res = {}
+    for v in (0, 2):
+        m.set_variant(v)
+        s = X.Pcg(m, rel_tol=1e-6, accurate=True)
+        s.set_rhs(a.b)
+        rel, _, it = s.run_reliable(1e-6, 0.0)
+        res[v] = (it, s.x())
+        s.close()
+    assert res[0][0] == res[2][0]
+    np.testing.assert_array_equal(res[0][1], res[2][1])
