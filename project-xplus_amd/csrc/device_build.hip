@@ -30,6 +30,28 @@ constexpr int kSortThreads = 1024;
 
 int blocks_of(long long n, int per) { return static_cast<int>((n + per - 1) / per); }
 
+// Work buffers of the set-up, kept between builds and grown on demand: a
+// build then makes no hipMalloc / hipFree calls for its temporaries (about
+// 100 bytes per row), only for the arrays the matrix keeps.  Builds are
+// serialised by set_up_mutex().
+enum Work : int {
+    kWErr, kWOwner, kWSize, kWMembers, kWM1, kWCounter, kWTaken, kWBid, kWElig, kWFlagUnit, kWUnitSrc, kWFlagPiece,
+    kWPieceSrc, kWPos, kWScan, kWTotals, kWPieces, kWPsz, kWPdst, kWProws, kWPsize, kWKey, kWOrder, kWWidth,
+    kWFlags, kWSmin, kWSmax, kWOffcnt, kWDiagPos, kWDis, kWTotal, kWCount
+};
+
+DevBuf<unsigned char>& work_slot(int id) {
+    static DevBuf<unsigned char> slots[kWCount];
+    return slots[id];
+}
+
+template <typename T>
+T* take(int id, size_t count) {
+    DevBuf<unsigned char>& b = work_slot(id);
+    b.ensure(std::max<size_t>(count, 1) * sizeof(T));
+    return reinterpret_cast<T*>(b.p);
+}
+
 // ---- exclusive scan of int counts ----------------------------------------------
 __device__ __forceinline__ int wave_incl_scan(int v) {
     const int lane = static_cast<int>(threadIdx.x) & 63;
@@ -104,12 +126,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_add(int n, int* __restric
 }
 
 // out[i] = sum in[0..i), *total = sum in[0..n) (device pointer, may be null).
-void scan_exclusive(const int* in, int* out, int n, int* total, DevBuf<int>& tmp, hipStream_t s) {
+void scan_exclusive(const int* in, int* out, int n, int* total, hipStream_t s) {
     const int ntiles = std::max(1, blocks_of(n, kScanTile));
-    tmp.ensure(static_cast<size_t>(ntiles));
-    hipLaunchKernelGGL(k_scan_tile, dim3(ntiles), dim3(kScanThreads), 0, s, n, in, out, tmp.p);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, ntiles, tmp.p, total);
-    hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(kScanThreads), 0, s, n, out, tmp.p);
+    int* tmp = take<int>(kWScan, static_cast<size_t>(ntiles));
+    hipLaunchKernelGGL(k_scan_tile, dim3(ntiles), dim3(kScanThreads), 0, s, n, in, out, tmp);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, s, ntiles, tmp, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(ntiles), dim3(kScanThreads), 0, s, n, out, tmp);
 }
 
 template <typename T>
@@ -119,8 +141,8 @@ __global__ void k_fill_n(long long n, T v, T* __restrict__ dst) {
 }
 
 template <typename T>
-void fill(DevBuf<T>& b, long long n, T v, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_fill_n<T>, dim3(blocks_of(n, kRowThreads)), dim3(kRowThreads), 0, s, n, v, b.p);
+void fill(T* p, long long n, T v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_n<T>, dim3(blocks_of(n, kRowThreads)), dim3(kRowThreads), 0, s, n, v, p);
 }
 
 // Sum of one value over a 256-thread workgroup, then one atomic per group.
@@ -239,8 +261,10 @@ __global__ void k_grow_bid(int n, const int* __restrict__ rp, const int* __restr
                            const double* __restrict__ va, const double* __restrict__ dis,
                            const double* __restrict__ t, const uint8_t* __restrict__ elig,
                            const int* __restrict__ owner, const int* __restrict__ size,
-                           unsigned long long* __restrict__ bid) {
+                           unsigned long long* __restrict__ bid, const int* __restrict__ prev_taken) {
 #pragma clang fp contract(off)
+    // a step after one that took no row would find the same bids: nothing to do
+    if (prev_taken != nullptr && *prev_taken == 0) return;
     const int j = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
     if (j >= n || !elig[j] || owner[j] >= 0) return;
     const int b = rp[j], e = rp[j + 1];
@@ -279,11 +303,14 @@ __global__ void k_grow_bid(int n, const int* __restrict__ rp, const int* __restr
 
 // Taking half: every block with a bid adds the winning row.
 __global__ void k_grow_take(int n, unsigned long long* __restrict__ bid, int* __restrict__ owner,
-                            int* __restrict__ size, int* __restrict__ members) {
+                            int* __restrict__ size, int* __restrict__ members, const int* __restrict__ prev_taken,
+                            int* __restrict__ taken) {
+    if (prev_taken != nullptr && *prev_taken == 0) return;
     const int B = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
     if (B >= n) return;
     const unsigned long long k = bid[B];
     if (k == 0) return;
+    *taken = 1;  // every writer stores the same value
     const int j = static_cast<int>(0xFFFFFFFFu - static_cast<unsigned>(k & 0xFFFFFFFFull));
     const int sz = size[B];
     members[static_cast<size_t>(B) * 8 + static_cast<size_t>(sz)] = j;
@@ -292,10 +319,13 @@ __global__ void k_grow_take(int n, unsigned long long* __restrict__ bid, int* __
     bid[B] = 0;
 }
 
+constexpr int kCountGroups = 256;
 __global__ void k_count_free(int n, const uint8_t* __restrict__ elig, const int* __restrict__ owner,
                              int* __restrict__ counter) {
-    const int i = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
-    const int v = (i < n && elig[i] && owner[i] < 0) ? 1 : 0;
+    int v = 0;
+    for (int i = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x); i < n;
+         i += kCountGroups * kRowThreads)
+        v += (elig[i] && owner[i] < 0) ? 1 : 0;
     group_count(v, counter);
 }
 
@@ -355,33 +385,49 @@ __global__ void k_compact(int n, const int* __restrict__ flag, const int* __rest
     if (r < n && flag[r]) out[pos[r]] = src[r];
 }
 
+__global__ void k_piece_sizes(int npieces, const int* __restrict__ pieces, const int* __restrict__ size,
+                              int* __restrict__ psz) {
+    const int p = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    if (p < npieces) psz[p] = size[pieces[p]];
+}
+
 // One thread per run of kPackSegment pieces (in smallest-row order): each
 // piece into the most recent of the run's last kPackWindow packed blocks
-// with room, else a new one.  Packed block k of run g is pid g * kPackSegment + k.
-__global__ void k_pack(int npieces, const int* __restrict__ pieces, const int* __restrict__ size,
-                       const int* __restrict__ members, int* __restrict__ prows, int* __restrict__ psize) {
+// with room, else a new one.  Packed block k of run g is pid g * kPackSegment
+// + k; pdst[p] = 8 pid + the piece's first slot.  The run's sizes are read
+// up front, so the choice loop waits on no memory.
+__global__ void k_pack(int npieces, const int* __restrict__ psz, int* __restrict__ pdst, int* __restrict__ psize) {
     const int g = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
     const int p0 = g * kPackSegment;
     if (p0 >= npieces) return;
-    const int p1 = min(npieces, p0 + kPackSegment);
-    int ps[kPackSegment];
-    for (int k = 0; k < kPackSegment; ++k) ps[k] = 0;
+    const int cnt = min(npieces - p0, kPackSegment);
+    int sz[kPackSegment], ps[kPackSegment];
+#pragma unroll
+    for (int k = 0; k < kPackSegment; ++k) {
+        sz[k] = k < cnt ? psz[p0 + k] : 0;
+        ps[k] = 0;
+    }
     int np = 0;
-    for (int p = p0; p < p1; ++p) {
-        const int B = pieces[p];
-        const int sz = size[B];
+    for (int k = 0; k < cnt; ++k) {
         int dst = -1;
-        for (int k = np - 1; k >= max(0, np - kPackWindow); --k)
-            if (ps[k] + sz <= 8) {
-                dst = k;
+        for (int q = np - 1; q >= max(0, np - kPackWindow); --q)
+            if (ps[q] + sz[k] <= 8) {
+                dst = q;
                 break;
             }
         if (dst < 0) dst = np++;
-        const size_t pid = static_cast<size_t>(g) * kPackSegment + static_cast<size_t>(dst);
-        for (int i = 0; i < sz; ++i) prows[pid * 8 + static_cast<size_t>(ps[dst] + i)] = members[static_cast<size_t>(B) * 8 + i];
-        ps[dst] += sz;
+        pdst[p0 + k] = (g * kPackSegment + dst) * 8 + ps[dst];
+        ps[dst] += sz[k];
     }
     for (int k = 0; k < kPackSegment; ++k) psize[static_cast<size_t>(g) * kPackSegment + k] = ps[k];
+}
+
+__global__ void k_pack_copy(int npieces, const int* __restrict__ pieces, const int* __restrict__ psz,
+                            const int* __restrict__ pdst, const int* __restrict__ members, int* __restrict__ prows) {
+    const int p = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    if (p >= npieces) return;
+    const int B = pieces[p];
+    for (int i = 0; i < psz[p]; ++i) prows[pdst[p] + i] = members[static_cast<size_t>(B) * 8 + i];
 }
 
 __global__ void k_packed_mark(int npacked, int* __restrict__ prows, const int* __restrict__ psize,
@@ -487,43 +533,56 @@ __device__ __forceinline__ int wave_max(int v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
     return v;
 }
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
+
+// Slice shapes from the rows in natural order (coalesced CSR reads): every
+// row raises its slice's width to 1 + its off-diagonal count and widens its
+// slice's column span (in solve order) with integer atomics; spans start at
+// the slice's own rows.  Then per slice: slice_base, the 16-bit test and
+// the widest slice.
+__global__ void k_slice_init(int nslices, int* __restrict__ width, int* __restrict__ smin, int* __restrict__ smax) {
+    const int sl = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    if (sl >= nslices) return;
+    width[sl] = 1;
+    smin[sl] = sl * kC;
+    smax[sl] = sl * kC + kC - 1;
 }
 
-// One wavefront per slice: width (1 + longest row), the column span of its
-// entries in solve order (always covering its own rows) -> slice_base and
-// the 16-bit test.
-__global__ void k_slice_shape(int nslices, const int* __restrict__ perm, const int* __restrict__ iperm,
-                              const int* __restrict__ rp, const int* __restrict__ ci,
-                              const int* __restrict__ diag_pos, const int* __restrict__ offcnt,
-                              int* __restrict__ width, int* __restrict__ slice_base, int* __restrict__ flags) {
-    const int sl = static_cast<int>(blockIdx.x) * (kRowThreads / 64) + (static_cast<int>(threadIdx.x) >> 6);
-    if (sl >= nslices) return;
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int i = sl * kC + lane;
-    const int r = perm[i];
-    int w = 0, cmin = sl * kC, cmax = sl * kC + kC - 1;
-    if (r >= 0) {
-        w = offcnt[r];
-        const int dp = diag_pos[r];
-        for (int o = rp[r]; o < rp[r + 1]; ++o) {
-            if (o == dp) continue;
-            const int c = iperm[ci[o]];
-            cmin = min(cmin, c);
-            cmax = max(cmax, c);
-        }
+__global__ void k_slice_rows(int n, const int* __restrict__ iperm, const int* __restrict__ rp,
+                             const int* __restrict__ ci, const int* __restrict__ diag_pos,
+                             const int* __restrict__ offcnt, int* __restrict__ width, int* __restrict__ smin,
+                             int* __restrict__ smax) {
+    const int r = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    if (r >= n) return;
+    const int sl = iperm[r] / kC;
+    const int dp = diag_pos[r];
+    int cmin = 0x7FFFFFFF, cmax = -1;
+    for (int o = rp[r]; o < rp[r + 1]; ++o) {
+        if (o == dp) continue;
+        const int c = iperm[ci[o]];
+        cmin = min(cmin, c);
+        cmax = max(cmax, c);
+    }
+    if (offcnt[r] > 0) atomicMax(width + sl, 1 + offcnt[r]);
+    if (cmax >= 0) {
+        atomicMin(smin + sl, cmin);
+        atomicMax(smax + sl, cmax);
+    }
+}
+
+__global__ void k_slice_finish(int nslices, const int* __restrict__ width, const int* __restrict__ smin,
+                               const int* __restrict__ smax, int* __restrict__ slice_base, int* __restrict__ flags) {
+    const int sl = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    int w = 0, wide = 0;
+    if (sl < nslices) {
+        slice_base[sl] = smin[sl];
+        w = width[sl];
+        wide = smax[sl] - smin[sl] > 65535 ? 1 : 0;
     }
     w = wave_max(w);
-    cmin = wave_min(cmin);
-    cmax = wave_max(cmax);
-    if (lane == 0) {
-        width[sl] = 1 + w;
-        slice_base[sl] = cmin;
-        if (cmax - cmin > 65535) atomicAnd(flags, ~1);
-        atomicMax(flags + 1, 1 + w);
+    wide = wave_max(wide);
+    if ((threadIdx.x & 63) == 0) {
+        if (wide) atomicAnd(flags, ~1);
+        atomicMax(flags + 1, w);
     }
 }
 
@@ -557,34 +616,42 @@ __device__ __forceinline__ void put_entry(const FillOut& f, size_t e, double v, 
     if (f.col32 != nullptr) f.col32[e] = col;
 }
 
-// One wavefront per slice, one lane per row: diagonal first, then the
-// off-diagonal entries in CSR order (columns in solve order), then padding
-// slots pointing at the row itself with value 0; padding rows are identity.
-__global__ void k_sell_fill(int nslices, const int* __restrict__ perm, const int* __restrict__ iperm,
-                            const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ va,
-                            const int* __restrict__ diag_pos, const int* __restrict__ slice_ptr,
-                            const int* __restrict__ slice_base, FillOut f) {
-    const int sl = static_cast<int>(blockIdx.x) * (kRowThreads / 64) + (static_cast<int>(threadIdx.x) >> 6);
-    if (sl >= nslices) return;
-    const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int i = sl * kC + lane;
-    const int r = perm[i];
+// One thread per natural row: its slots in its slice (diagonal first, the
+// off-diagonal entries in CSR order with columns in solve order, then
+// padding slots pointing at the row itself with value 0).
+__global__ void k_sell_fill_rows(int n, const int* __restrict__ iperm, const int* __restrict__ rp,
+                                 const int* __restrict__ ci, const double* __restrict__ va,
+                                 const int* __restrict__ diag_pos, const int* __restrict__ slice_ptr,
+                                 const int* __restrict__ slice_base, FillOut f) {
+    const int r = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    if (r >= n) return;
+    const int i = iperm[r];
+    const int sl = i / kC;
     const int cb = slice_base[sl];
     const int w = slice_ptr[sl + 1] - slice_ptr[sl];
-    const size_t e0 = static_cast<size_t>(slice_ptr[sl]) * kC + static_cast<size_t>(lane);
+    const size_t e0 = static_cast<size_t>(slice_ptr[sl]) * kC + static_cast<size_t>(i % kC);
+    const int dp = diag_pos[r];
+    put_entry(f, e0, dp >= 0 ? va[dp] : 0.0, i, cb, true);
     int j = 1;
-    if (r < 0) {
-        put_entry(f, e0, 1.0, i, cb, true);
-    } else {
-        const int dp = diag_pos[r];
-        put_entry(f, e0, dp >= 0 ? va[dp] : 0.0, i, cb, true);
-        for (int o = rp[r]; o < rp[r + 1]; ++o) {
-            if (o == dp) continue;
-            put_entry(f, e0 + static_cast<size_t>(j) * kC, va[o], iperm[ci[o]], cb, false);
-            ++j;
-        }
+    for (int o = rp[r]; o < rp[r + 1]; ++o) {
+        if (o == dp) continue;
+        put_entry(f, e0 + static_cast<size_t>(j) * kC, va[o], iperm[ci[o]], cb, false);
+        ++j;
     }
     for (; j < w; ++j) put_entry(f, e0 + static_cast<size_t>(j) * kC, 0.0, i, cb, false);
+}
+
+// Padding rows (no natural row): identity.
+__global__ void k_sell_fill_pad(int n_pad, const int* __restrict__ perm, const int* __restrict__ slice_ptr,
+                                const int* __restrict__ slice_base, FillOut f) {
+    const int i = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    if (i >= n_pad || perm[i] >= 0) return;
+    const int sl = i / kC;
+    const int cb = slice_base[sl];
+    const int w = slice_ptr[sl + 1] - slice_ptr[sl];
+    const size_t e0 = static_cast<size_t>(slice_ptr[sl]) * kC + static_cast<size_t>(i % kC);
+    put_entry(f, e0, 1.0, i, cb, true);
+    for (int j = 1; j < w; ++j) put_entry(f, e0 + static_cast<size_t>(j) * kC, 0.0, i, cb, false);
 }
 
 // Natural diagonal, solve-order 1 / diag, and the first row that fails the
@@ -612,22 +679,35 @@ __global__ void k_natural_slots(int n, int units, int* __restrict__ slots) {
     if (i < units * 8) slots[i] = i < n ? i : -1;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_sys_sumsq(const int* __restrict__ sys_begin,
-                                                            const double* __restrict__ x, double* __restrict__ out) {
-    __shared__ double part[kScanThreads / 64];
-    const int k = static_cast<int>(blockIdx.x);
+constexpr int kSumGroups = kSysSumParts;  // workgroups per system
+
+// Partial sums of x^2 over a system's rows: workgroup g of system k takes
+// rows sys_begin[k] + g*256 + j*64*256 ..; then one fixed-order sum per system.
+__global__ void k_sys_sumsq_part(const int* __restrict__ sys_begin, const double* __restrict__ x,
+                                 double* __restrict__ part) {
+    __shared__ double red[kRowThreads / 64];
+    const int k = static_cast<int>(blockIdx.y), g = static_cast<int>(blockIdx.x);
     double acc = 0.0;
-    for (int i = sys_begin[k] + static_cast<int>(threadIdx.x); i < sys_begin[k + 1]; i += kScanThreads)
+    for (int i = sys_begin[k] + g * kRowThreads + static_cast<int>(threadIdx.x); i < sys_begin[k + 1];
+         i += kSumGroups * kRowThreads)
         acc += x[i] * x[i];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
-        for (int w = 0; w < kScanThreads / 64; ++w) t += part[w];
-        out[k] = t;
+        for (int w = 0; w < kRowThreads / 64; ++w) t += red[w];
+        part[k * kSumGroups + g] = t;
     }
+}
+
+__global__ void k_sys_sumsq_finish(int nsys, const double* __restrict__ part, double* __restrict__ out) {
+    const int k = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    if (k >= nsys) return;
+    double t = 0.0;
+    for (int g = 0; g < kSumGroups; ++g) t += part[k * kSumGroups + g];
+    out[k] = t;
 }
 
 }  // namespace
@@ -641,11 +721,18 @@ int device_natural_slots(int n, DevBuf<int>& slots, hipStream_t s) {
     return units;
 }
 
-void launch_sys_sumsq(int nsys, const int* sys_begin, const double* x, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_sys_sumsq, dim3(nsys), dim3(kScanThreads), 0, s, sys_begin, x, out);
+void launch_sys_sumsq(int nsys, const int* sys_begin, const double* x, double* out, double* part, hipStream_t s) {
+    hipLaunchKernelGGL(k_sys_sumsq_part, dim3(kSumGroups, nsys), dim3(kRowThreads), 0, s, sys_begin, x, part);
+    hipLaunchKernelGGL(k_sys_sumsq_finish, dim3(blocks_of(nsys, kRowThreads)), dim3(kRowThreads), 0, s, nsys, part,
+                       out);
 }
 
 // ---- host side ------------------------------------------------------------------------------
+std::mutex& set_up_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+
 void device_rows(DevCsr& a, DevRows& rows, hipStream_t s) {
     const int n = a.n;
     if (n <= 0) throw std::runtime_error("n must be positive");
@@ -658,30 +745,35 @@ void device_rows(DevCsr& a, DevRows& rows, hipStream_t s) {
     if (ends[0] != 0) throw std::runtime_error("row_ptr[0] must be 0");
     if (ends[1] < 0) throw std::runtime_error("row_ptr must be nondecreasing");
     a.nnz = ends[1];
-    rows.offcnt.alloc(static_cast<size_t>(n));
-    rows.diag_pos.alloc(static_cast<size_t>(n));
-    rows.dis.alloc(static_cast<size_t>(n));
-    rows.total.alloc(static_cast<size_t>(n));
-    DevBuf<int> err(1);
-    HIPX(hipMemsetAsync(err.p, 0, sizeof(int), s));
+    rows.offcnt = take<int>(kWOffcnt, static_cast<size_t>(n));
+    rows.diag_pos = take<int>(kWDiagPos, static_cast<size_t>(n));
+    rows.dis = take<double>(kWDis, static_cast<size_t>(n));
+    rows.total = take<double>(kWTotal, static_cast<size_t>(n));
+    int* err = take<int>(kWErr, 1);
+    HIPX(hipMemsetAsync(err, 0, sizeof(int), s));
     const dim3 g(blocks_of(n, kRowThreads)), b(kRowThreads);
-    hipLaunchKernelGGL(k_row_info, g, b, 0, s, n, a.nnz, a.row_ptr, a.col_idx, a.values, rows.offcnt.p,
-                       rows.diag_pos.p, rows.dis.p, err.p);
+    hipLaunchKernelGGL(k_row_info, g, b, 0, s, n, a.nnz, a.row_ptr, a.col_idx, a.values, rows.offcnt,
+                       rows.diag_pos, rows.dis, err);
     int e = 0;
-    HIPX(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     if (e & kErrRowPtr) throw std::runtime_error("row_ptr must be nondecreasing");
     if (e & kErrCol) throw std::runtime_error("col_idx out of range");
-    hipLaunchKernelGGL(k_totals, g, b, 0, s, n, a.row_ptr, a.col_idx, a.values, rows.dis.p, rows.total.p);
+    hipLaunchKernelGGL(k_totals, g, b, 0, s, n, a.row_ptr, a.col_idx, a.values, rows.dis, rows.total);
     HIPX(hipGetLastError());
 }
 
 int device_aggregate(const DevCsr& a, const DevRows& rows, DevBuf<int>& slots, AggStats* st, hipStream_t s) {
     const int n = a.n;
     const size_t nn = static_cast<size_t>(n);
-    DevBuf<int> owner(nn), size(nn), members(8 * nn), m1(nn), counter(1);
-    DevBuf<unsigned long long> bid(nn);
-    DevBuf<uint8_t> elig(nn);
+    int* owner = take<int>(kWOwner, nn);
+    int* size = take<int>(kWSize, nn);
+    int* members = take<int>(kWMembers, 8 * nn);
+    int* m1 = take<int>(kWM1, nn);
+    int* counter = take<int>(kWCounter, 1);
+    int* taken = take<int>(kWTaken, kGrowSteps);
+    unsigned long long* bid = take<unsigned long long>(kWBid, nn);
+    uint8_t* elig = take<uint8_t>(kWElig, nn);
     fill(owner, n, -1, s);
     fill(size, n, 0, s);
     fill(bid, n, 0ull, s);
@@ -690,63 +782,75 @@ int device_aggregate(const DevCsr& a, const DevRows& rows, DevBuf<int>& slots, A
     AggStats local;
     auto rounds = [&](int& count) {
         for (;;) {
-            hipLaunchKernelGGL(k_seed_best1, g, b, 0, s, n, a.row_ptr, a.col_idx, rows.total.p, elig.p, owner.p, m1.p);
-            hipLaunchKernelGGL(k_seed_pick, g, b, 0, s, n, a.row_ptr, a.col_idx, rows.total.p, elig.p, m1.p, owner.p,
-                               size.p, members.p);
+            hipLaunchKernelGGL(k_seed_best1, g, b, 0, s, n, a.row_ptr, a.col_idx, rows.total, elig, owner, m1);
+            hipLaunchKernelGGL(k_seed_pick, g, b, 0, s, n, a.row_ptr, a.col_idx, rows.total, elig, m1, owner,
+                               size, members);
+            fill(taken, kGrowSteps, 0, s);
             for (int step = 0; step < kGrowSteps; ++step) {
-                hipLaunchKernelGGL(k_grow_bid, g, b, 0, s, n, a.row_ptr, a.col_idx, a.values, rows.dis.p,
-                                   rows.total.p, elig.p, owner.p, size.p, bid.p);
-                hipLaunchKernelGGL(k_grow_take, g, b, 0, s, n, bid.p, owner.p, size.p, members.p);
+                const int* prev = step > 0 ? taken + step - 1 : nullptr;
+                hipLaunchKernelGGL(k_grow_bid, g, b, 0, s, n, a.row_ptr, a.col_idx, a.values, rows.dis,
+                                   rows.total, elig, owner, size, bid, prev);
+                hipLaunchKernelGGL(k_grow_take, g, b, 0, s, n, bid, owner, size, members, prev,
+                                   taken + step);
             }
             ++count;
-            HIPX(hipMemsetAsync(counter.p, 0, sizeof(int), s));
-            hipLaunchKernelGGL(k_count_free, g, b, 0, s, n, elig.p, owner.p, counter.p);
+            HIPX(hipMemsetAsync(counter, 0, sizeof(int), s));
+            hipLaunchKernelGGL(k_count_free, dim3(kCountGroups), b, 0, s, n, elig, owner, counter);
             int left = 0;
-            HIPX(hipMemcpyAsync(&left, counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPX(hipMemcpyAsync(&left, counter, sizeof(int), hipMemcpyDeviceToHost, s));
             HIPX(hipStreamSynchronize(s));
             if (left == 0) break;
         }
     };
     rounds(local.rounds);
-    hipLaunchKernelGGL(k_pool, g, b, 0, s, n, owner.p, size.p, elig.p);
-    hipLaunchKernelGGL(k_pool_sizes, g, b, 0, s, n, size.p);
-    HIPX(hipMemsetAsync(counter.p, 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_count_free, g, b, 0, s, n, elig.p, owner.p, counter.p);
+    hipLaunchKernelGGL(k_pool, g, b, 0, s, n, owner, size, elig);
+    hipLaunchKernelGGL(k_pool_sizes, g, b, 0, s, n, size);
+    HIPX(hipMemsetAsync(counter, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_count_free, dim3(kCountGroups), b, 0, s, n, elig, owner, counter);
     int pooled = 0;
-    HIPX(hipMemcpyAsync(&pooled, counter.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&pooled, counter, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     if (pooled > 0) rounds(local.regroup_rounds);
 
     // units and pieces, keyed by their smallest row
-    DevBuf<int> flag_unit(nn), unit_src(nn), flag_piece(nn), piece_src(nn), pos(nn), tmp, totals(2);
+    int* flag_unit = take<int>(kWFlagUnit, nn);
+    int* unit_src = take<int>(kWUnitSrc, nn);
+    int* flag_piece = take<int>(kWFlagPiece, nn);
+    int* piece_src = take<int>(kWPieceSrc, nn);
+    int* pos = take<int>(kWPos, nn);
+    int* totals = take<int>(kWTotals, 2);
     fill(flag_unit, n, 0, s);
     fill(flag_piece, n, 0, s);
-    hipLaunchKernelGGL(k_unit_mark, g, b, 0, s, n, size.p, members.p, flag_unit.p, unit_src.p, flag_piece.p,
-                       piece_src.p);
-    scan_exclusive(flag_piece.p, pos.p, n, totals.p, tmp, s);
+    hipLaunchKernelGGL(k_unit_mark, g, b, 0, s, n, size, members, flag_unit, unit_src, flag_piece,
+                       piece_src);
+    scan_exclusive(flag_piece, pos, n, totals, s);
     int npieces = 0;
-    HIPX(hipMemcpyAsync(&npieces, totals.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&npieces, totals, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     const int nseg = (npieces + kPackSegment - 1) / kPackSegment;
     const int npacked = nseg * kPackSegment;
-    DevBuf<int> pieces(static_cast<size_t>(std::max(npieces, 1)));
-    DevBuf<int> prows(static_cast<size_t>(std::max(npacked, 1)) * 8), psize(static_cast<size_t>(std::max(npacked, 1)));
+    int* pieces = take<int>(kWPieces, static_cast<size_t>(npieces));
+    int* psz = take<int>(kWPsz, static_cast<size_t>(npieces));
+    int* pdst = take<int>(kWPdst, static_cast<size_t>(npieces));
+    int* prows = take<int>(kWProws, static_cast<size_t>(npacked) * 8);
+    int* psize = take<int>(kWPsize, static_cast<size_t>(npacked));
     if (npieces > 0) {
-        hipLaunchKernelGGL(k_compact, g, b, 0, s, n, flag_piece.p, pos.p, piece_src.p, pieces.p);
-        hipLaunchKernelGGL(k_pack, dim3(blocks_of(nseg, kRowThreads)), b, 0, s, npieces, pieces.p, size.p, members.p,
-                           prows.p, psize.p);
-        hipLaunchKernelGGL(k_packed_mark, dim3(blocks_of(npacked, kRowThreads)), b, 0, s, npacked, prows.p, psize.p,
-                           flag_unit.p, unit_src.p);
+        const dim3 gp(blocks_of(npieces, kRowThreads));
+        hipLaunchKernelGGL(k_compact, g, b, 0, s, n, flag_piece, pos, piece_src, pieces);
+        hipLaunchKernelGGL(k_piece_sizes, gp, b, 0, s, npieces, pieces, size, psz);
+        hipLaunchKernelGGL(k_pack, dim3(blocks_of(nseg, kRowThreads)), b, 0, s, npieces, psz, pdst, psize);
+        hipLaunchKernelGGL(k_pack_copy, gp, b, 0, s, npieces, pieces, psz, pdst, members, prows);
+        hipLaunchKernelGGL(k_packed_mark, dim3(blocks_of(npacked, kRowThreads)), b, 0, s, npacked, prows, psize,
+                           flag_unit, unit_src);
     }
-    scan_exclusive(flag_unit.p, pos.p, n, totals.p + 1, tmp, s);
+    scan_exclusive(flag_unit, pos, n, totals + 1, s);
     int units = 0;
-    HIPX(hipMemcpyAsync(&units, totals.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&units, totals + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     slots.alloc(static_cast<size_t>(units) * 8);
-    hipLaunchKernelGGL(k_units_write, g, b, 0, s, n, flag_unit.p, pos.p, unit_src.p, members.p, prows.p, psize.p,
+    hipLaunchKernelGGL(k_units_write, g, b, 0, s, n, flag_unit, pos, unit_src, members, prows, psize,
                        slots.p);
     HIPX(hipGetLastError());
-    HIPX(hipStreamSynchronize(s));  // the work buffers are freed on return
     local.pieces = npieces;
     local.units = units;
     if (st != nullptr) *st = local;
@@ -773,36 +877,42 @@ void device_sell(const DevCsr& a, const DevRows& rows, int sigma, bool keep_lo, 
     out.n_pad = std::max((nu * br + kC - 1) / kC * kC, kC);
     out.nslices = out.n_pad / kC;
     const dim3 b(kRowThreads);
-    DevBuf<int> key(static_cast<size_t>(nu)), order(static_cast<size_t>(nu)), width(static_cast<size_t>(out.nslices)),
-        flags(4), tmp;
-    hipLaunchKernelGGL(k_unit_keys, dim3(blocks_of(nu, kRowThreads)), b, 0, s, nu, br, d_slots, rows.offcnt.p, key.p);
+    int* key = take<int>(kWKey, static_cast<size_t>(nu));
+    int* order = take<int>(kWOrder, static_cast<size_t>(nu));
+    int* width = take<int>(kWWidth, static_cast<size_t>(out.nslices));
+    int* flags = take<int>(kWFlags, 4);
+    hipLaunchKernelGGL(k_unit_keys, dim3(blocks_of(nu, kRowThreads)), b, 0, s, nu, br, d_slots, rows.offcnt, key);
     hipLaunchKernelGGL(k_window_sort, dim3(blocks_of(nu, W)), dim3(kSortThreads),
-                       static_cast<size_t>(P) * sizeof(unsigned long long), s, nu, W, P, key.p, order.p);
+                       static_cast<size_t>(P) * sizeof(unsigned long long), s, nu, W, P, key, order);
     out.perm.alloc(static_cast<size_t>(out.n_pad));
     out.iperm.alloc(static_cast<size_t>(n));
-    fill(out.perm, out.n_pad, -1, s);
+    fill(out.perm.p, out.n_pad, -1, s);
     if (br == 8) {
         out.block_unit.alloc(static_cast<size_t>(out.n_pad / 8));
-        fill(out.block_unit, out.n_pad / 8, -1, s);
+        fill(out.block_unit.p, out.n_pad / 8, -1, s);
     } else {
         out.block_unit.release();
     }
-    hipLaunchKernelGGL(k_perm, dim3(blocks_of(nu, kRowThreads)), b, 0, s, nu, br, order.p, d_slots, out.perm.p,
+    hipLaunchKernelGGL(k_perm, dim3(blocks_of(nu, kRowThreads)), b, 0, s, nu, br, order, d_slots, out.perm.p,
                        out.iperm.p, out.block_unit.p);
     const int flags0[4] = {1, 0, 0x7FFFFFFF, 0};  // fits16, max width, bad key
-    HIPX(hipMemcpyAsync(flags.p, flags0, sizeof flags0, hipMemcpyHostToDevice, s));
+    HIPX(hipMemcpyAsync(flags, flags0, sizeof flags0, hipMemcpyHostToDevice, s));
     out.slice_base.alloc(static_cast<size_t>(out.nslices));
     out.slice_ptr.alloc(static_cast<size_t>(out.nslices) + 1);
-    const dim3 gs(blocks_of(out.nslices, kRowThreads / 64));
-    hipLaunchKernelGGL(k_slice_shape, gs, b, 0, s, out.nslices, out.perm.p, out.iperm.p, a.row_ptr, a.col_idx,
-                       rows.diag_pos.p, rows.offcnt.p, width.p, out.slice_base.p, flags.p);
-    scan_exclusive(width.p, out.slice_ptr.p, out.nslices, out.slice_ptr.p + out.nslices, tmp, s);
+    int* smin = take<int>(kWSmin, static_cast<size_t>(out.nslices));
+    int* smax = take<int>(kWSmax, static_cast<size_t>(out.nslices));
+    const dim3 gsl(blocks_of(out.nslices, kRowThreads)), gr(blocks_of(n, kRowThreads));
+    hipLaunchKernelGGL(k_slice_init, gsl, b, 0, s, out.nslices, width, smin, smax);
+    hipLaunchKernelGGL(k_slice_rows, gr, b, 0, s, n, out.iperm.p, a.row_ptr, a.col_idx, rows.diag_pos,
+                       rows.offcnt, width, smin, smax);
+    hipLaunchKernelGGL(k_slice_finish, gsl, b, 0, s, out.nslices, width, smin, smax, out.slice_base.p, flags);
+    scan_exclusive(width, out.slice_ptr.p, out.nslices, out.slice_ptr.p + out.nslices, s);
     out.diag.alloc(static_cast<size_t>(n));
     out.minv.alloc(static_cast<size_t>(out.n_pad));
     hipLaunchKernelGGL(k_diag_minv, dim3(blocks_of(out.n_pad, kRowThreads)), b, 0, s, out.n_pad, n, out.perm.p,
-                       rows.diag_pos.p, a.values, out.diag.p, out.minv.p, flags.p + 2);
+                       rows.diag_pos, a.values, out.diag.p, out.minv.p, flags + 2);
     int got[3] = {0, 0, 0}, cols = 0;
-    HIPX(hipMemcpyAsync(got, flags.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(got, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipMemcpyAsync(&cols, out.slice_ptr.p + out.nslices, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     out.idx16 = got[0] & 1;
@@ -827,10 +937,11 @@ void device_sell(const DevCsr& a, const DevRows& rows, int sigma, bool keep_lo, 
     else
         out.col32.alloc(total);
     const FillOut f{out.val.p, out.val_lo.p, out.val_lo16.p, out.val64.p, out.col16.p, out.col32.p};
-    hipLaunchKernelGGL(k_sell_fill, gs, b, 0, s, out.nslices, out.perm.p, out.iperm.p, a.row_ptr, a.col_idx, a.values,
-                       rows.diag_pos.p, out.slice_ptr.p, out.slice_base.p, f);
+    hipLaunchKernelGGL(k_sell_fill_rows, gr, b, 0, s, n, out.iperm.p, a.row_ptr, a.col_idx, a.values,
+                       rows.diag_pos, out.slice_ptr.p, out.slice_base.p, f);
+    hipLaunchKernelGGL(k_sell_fill_pad, dim3(blocks_of(out.n_pad, kRowThreads)), b, 0, s, out.n_pad, out.perm.p,
+                       out.slice_ptr.p, out.slice_base.p, f);
     HIPX(hipGetLastError());
-    HIPX(hipStreamSynchronize(s));  // the work buffers are freed on return
 }
 
 }  // namespace xp

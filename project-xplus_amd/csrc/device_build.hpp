@@ -27,6 +27,7 @@
 #pragma once
 
 #include <cstdint>
+#include <mutex>
 
 #include "hip_util.hpp"
 #include "xp_types.hpp"
@@ -50,10 +51,16 @@ struct DevCsr {
 // off-diagonal count (every entry but that one), D^-1/2 from the first
 // diagonal entry (1 where it is missing or not positive) and the total
 // off-diagonal strength t_i = sum |a_ij| d_i d_j (CSR order).
+// (The arrays are the set-up's reusable work buffers: valid until the next
+// build; builds are serialised by set_up_mutex.)
 struct DevRows {
-    DevBuf<int> offcnt, diag_pos;
-    DevBuf<double> dis, total;
+    int* offcnt = nullptr;
+    int* diag_pos = nullptr;
+    double* dis = nullptr;
+    double* total = nullptr;
 };
+
+std::mutex& set_up_mutex();
 
 // Validates the CSR on the device (throws the host builder's messages:
 // "row_ptr[0] must be 0", "row_ptr must be nondecreasing", "col_idx out of
@@ -99,7 +106,8 @@ void device_sell(const DevCsr& a, const DevRows& rows, int sigma, bool keep_lo, 
 int device_natural_slots(int n, DevBuf<int>& slots, hipStream_t s);
 
 // out[k] = sum of x[i]^2 over the natural rows [sys_begin[k], sys_begin[k+1])
-// of system k, in a fixed order (one workgroup per system).
-void launch_sys_sumsq(int nsys, const int* sys_begin, const double* x, double* out, hipStream_t s);
+// of system k, in a fixed order; part: kSysSumParts * nsys doubles of scratch.
+constexpr int kSysSumParts = 64;
+void launch_sys_sumsq(int nsys, const int* sys_begin, const double* x, double* out, double* part, hipStream_t s);
 
 }  // namespace xp

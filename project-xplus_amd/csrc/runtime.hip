@@ -366,6 +366,7 @@ std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const i
     if (m->block_rows == 8 && m->blocking != 0 && m->blocking != 3)
         throw std::runtime_error("the device builder aggregates with rule 3 (blocking 3) or uses consecutive rows "
                                  "(blocking 0); rules 1 and 2 run on the host (xp_matrix_create)");
+    std::lock_guard<std::mutex> lock(xp::set_up_mutex());  // the set-up's work buffers are shared
     xp::DevCsr a{n, 0, row_ptr, col_idx, values};
     xp::DevRows rows;
     xp::device_rows(a, rows, s);
@@ -727,8 +728,8 @@ std::unique_ptr<xp_pcg> make_pcg(xp_matrix* m, const xp_solve_opts* o) {
 void set_rhs_common(xp_pcg& s) {
     xp_matrix& m = *s.m;
     const size_t ns = static_cast<size_t>(m.nsys);
-    DevBuf<double> bb(ns);
-    xp::launch_sys_sumsq(m.nsys, m.d_sys_nat.p, s.b_nat.p, bb.p, s.stream);
+    DevBuf<double> bb(ns), part(ns * xp::kSysSumParts);
+    xp::launch_sys_sumsq(m.nsys, m.d_sys_nat.p, s.b_nat.p, bb.p, part.p, s.stream);
     std::vector<double> h(ns);
     bb.download(h.data(), ns, s.stream);
     xp::launch_gather_d2f(m.n_pad, m.d_perm.p, s.b_nat.p, s.b.p, s.stream);
