@@ -98,7 +98,7 @@ def spmv_tuned(X, a, reps: int = 200):
     layout that is slower for a lone SpMV)."""
     m = X.Matrix(a, sigma=64)
     best = None
-    for v in (0, 1):
+    for v in (0, 1, 2):
         m.set_variant(v)
         for g in (256, 512, 1024):
             m.set_grid(g)
@@ -125,6 +125,49 @@ def spmv_roofline(m, a, reps: int = 200) -> dict:
     if r["kernel"] in prof:
         r["traffic"], r["traffic_source"] = prof[r["kernel"]]
     return r
+
+
+def stream_reference(X) -> dict:
+    """The project-written streaming kernels over 1 GiB (beyond the MALL):
+    the HBM rate a plain read / copy reaches on this GPU, for scale."""
+    nb = 1 << 30
+    r = X.stream_time(nb, 0, 2048)
+    c = X.stream_time(nb, 1, 2048)
+    return {"read_gbs": round(nb / (r * 1e-6) / 1e9, 1), "copy_gbs": round(2 * nb / (c * 1e-6) / 1e9, 1),
+            "read_frac": round(nb / (r * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "bytes": nb,
+            "kernels": "k_stream_read / k_stream_copy, 16-byte loads, 2048 workgroups"}
+
+
+def spmv_beyond_mall(X, mesh: int = 2400) -> dict:
+    """The FP32 SpMV on a matrix of the same family too large for the 256 MB
+    MALL (algorithmic bytes 391 MB at 2400 x 2400), so that warm launches
+    stream the matrix from HBM: tuned over inner loop x workgroups, warm and
+    cold (1 GiB copy before each launch)."""
+    a = X.gen_fem2d(mesh, 1.0, 20, 1e3, 0.3, SEED + mesh)
+    d = X.DeviceCsr(a)
+    m = X.Matrix.from_device(d, sigma=64)
+    alg = 8 * a.nnz + 12 * a.n
+    best = None
+    for v in (0, 1, 2):
+        m.set_variant(v)
+        for g in (512, 1024):
+            m.set_grid(g)
+            us = m.spmv_f32_time(10, 50)
+            if best is None or us < best[0]:
+                best = (us, v, g)
+    m.set_variant(best[1])
+    m.set_grid(best[2])
+    cold = m.spmv_f32_time_cold(10, 1 << 30)
+    m.close()
+    d.close()
+    out = {"mesh": mesh, "n": a.n, "nnz": a.nnz, "alg_bytes": alg, "layout": {"spmv_variant": best[1],
+                                                                                 "grid": best[2]},
+           "warm_us": round(best[0], 3), "warm_gbs": round(alg / (best[0] * 1e-6) / 1e9, 1),
+           "warm_frac": round(alg / (best[0] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+           "cold_us": round(cold, 3), "cold_gbs": round(alg / (cold * 1e-6) / 1e9, 1),
+           "cold_frac": round(alg / (cold * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"spmv beyond the MALL: {out}")
+    return out
 
 
 def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
@@ -367,7 +410,9 @@ def workload_c4(X, args) -> int:
     log-normal coefficients sigma 1.5, solved together in one batched solve
     (one SELL matrix, per-system state, one graph) to relative residual 1e-6
     each: the FP32 iteration to the recursive residual and the accurate
-    iteration to the true one."""
+    iteration to the true one; each with the workgroups of stopped systems
+    marked idle between graph launches (the default) and with every
+    workgroup launched ("_every_workgroup")."""
     rng = np.random.default_rng(20264203)
     ks = rng.integers(16, 65, 256)
     t0 = time.perf_counter()
@@ -382,20 +427,22 @@ def workload_c4(X, args) -> int:
     res = {"systems": len(systems), "n": n, "nnz": nnz, "k_min": int(ks.min()), "k_max": int(ks.max())}
     for name, kw in (("fp32", {"fused": True}), ("accurate", {"accurate": True})):
         m.autotune(fused=2 if name == "accurate" else True, tune_sigma=False, iters=32)
-        s = X.Pcg(m, rel_tol=1e-6, **kw)
-        s.set_rhs(b)
-        if name == "fp32":
-            ms, _ = timed(X, s.run, args.warmup, args.steps)
-            rel = s.true_residual() / np.array([np.linalg.norm(a.b) for a in systems])
-        else:
-            ms, (rel, _, _) = timed(X, lambda: s.run_reliable(1e-6, 0.0), args.warmup, args.steps)
-        its = [s.status(k)["iterations"] for k in range(len(systems))]
-        sts = [s.status(k)["status"] for k in range(len(systems))]
-        res[name] = {"ms": round(ms, 2), "iterations_min": min(its), "iterations_max": max(its),
-                     "converged": int(sum(1 for v in sts if v == X.CONVERGED)),
-                     "max_true_rel_residual": float(np.max(rel))}
-        log(f"c4 {name}: {res[name]}")
-        s.close()
+        for full in (True, False):
+            s = X.Pcg(m, rel_tol=1e-6, full_work_map=full, **kw)
+            s.set_rhs(b)
+            if name == "fp32":
+                ms, _ = timed(X, s.run, args.warmup, args.steps)
+                rel = s.true_residual() / np.array([np.linalg.norm(a.b) for a in systems])
+            else:
+                ms, (rel, _, _) = timed(X, lambda: s.run_reliable(1e-6, 0.0), args.warmup, args.steps)
+            its = [s.status(k)["iterations"] for k in range(len(systems))]
+            sts = [s.status(k)["status"] for k in range(len(systems))]
+            key = name if not full else name + "_every_workgroup"
+            res[key] = {"ms": round(ms, 2), "iterations_min": min(its), "iterations_max": max(its),
+                        "converged": int(sum(1 for v in sts if v == X.CONVERGED)),
+                        "max_true_rel_residual": float(np.max(rel))}
+            log(f"c4 {key}: {res[key]}")
+            s.close()
     print(json.dumps({"workload": "256 batched 3D 7-point systems to relative residual 1e-6", "unit": "ms",
                       **res}), flush=True)
     return 0
@@ -623,8 +670,12 @@ def workload_c2(X, args, cpu_proc) -> int:
     spmv = spmv_roofline(ms_, a)
     spmv["layout"] = layout
     ms_.close()
-    log(f"spmv: {spmv}")
     d.close()
+    spmv["stream_reference"] = stream_reference(X)
+    spmv["beyond_mall"] = spmv_beyond_mall(X)
+    spmv["beyond_mall"]["frac_of_stream_read"] = round(spmv["beyond_mall"]["warm_gbs"] /
+                                                       spmv["stream_reference"]["read_gbs"], 4)
+    log(f"spmv: {spmv}")
     cpu = None
     if cpu_proc is not None:
         cpu = finish_cpu_golden(cpu_proc, 300.0)

@@ -254,6 +254,10 @@ typedef struct {
                           (~48 bits), which xp_pcg_run_reliable also uses for
                           targets below 1e-8.  Needs keep_lo or keep64.  0:
                           FP32 iteration.  True residuals always use hi + lo. */
+    int full_work_map; /* batches: 0 (default) marks the workgroups of systems
+                          that have stopped idle between graph launches (they
+                          return at once instead of streaming a first slice and
+                          re-summing partials); 1 keeps every workgroup */
 } xp_solve_opts;
 
 typedef struct {

@@ -137,6 +137,7 @@ struct xp_matrix {
     int grid = 0;
     int u_blocks = 0;                 // single system: workgroups with work (uniform split)
     DevBuf<xp::WorkItem> d_work;
+    std::vector<xp::WorkItem> h_work;     // host copy of the work map
     DevBuf<int2> d_sys_blocks;
     DevBuf<double> d_partials;
 
@@ -212,6 +213,7 @@ void build_work_map(xp_matrix& m, int target) {
     while (static_cast<int>(work.size()) < grid) work.push_back(xp::WorkItem{-1, 0, 0, 0});
     m.grid = grid;
     ++m.layout_version;  // solvers re-capture graphs that hold the old buffers
+    m.h_work = work;
     m.d_work = to_device(work, s);
     m.d_sys_blocks = to_device(sb, s);
     m.d_partials.alloc(static_cast<size_t>(grid) * xp::kNumSlots);
@@ -486,6 +488,14 @@ struct xp_pcg {
     int graph_iters = 0;
     int graph_version = -1;
     int graph_vmode = -1;
+    // Batches: the iteration kernels read the solver's own copy of the work
+    // map, in which the workgroups of systems that have stopped are marked
+    // idle (sys = -1, they return at once) between graph launches; the
+    // other kernels (init, replacement, x flush, residuals) use the matrix's
+    // full map.  work_mask: the active flags the copy was built for.
+    DevBuf<xp::WorkItem> work_own;
+    std::vector<uint8_t> work_mask;
+    int work_version = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;
@@ -529,8 +539,42 @@ struct xp_pcg {
     // puts its pAp partials in the slot of parity k (Launch::set_parity);
     // the graph holds an even number of iterations, so slot 0 is current
     // between launches.
-    void enqueue_iteration(int k) {
+    bool compacting() const { return m->nsys > 1 && opts.full_work_map == 0; }
+
+    // The iteration kernels' launch: the solver's work map for batches.
+    xp::Launch iter_launch() {
         xp::Launch L = m->launch(stream);
+        if (compacting()) {
+            if (work_version != m->layout_version) {
+                work_own = to_device(m->h_work, stream);
+                work_mask.assign(static_cast<size_t>(m->nsys), 1);
+                work_version = m->layout_version;
+            }
+            L.work = work_own.p;
+        }
+        return L;
+    }
+
+    // Marks the workgroups of inactive systems idle in the solver's work map
+    // (active: per system; null: all active), uploading only on a change.
+    void set_work_mask(const xp::SysState* states) {
+        if (!compacting()) return;
+        iter_launch();  // the copy exists for the current layout
+        const size_t ns = static_cast<size_t>(m->nsys);
+        std::vector<uint8_t> mask(ns, 1);
+        if (states != nullptr)
+            for (size_t k = 0; k < ns; ++k) mask[k] = states[k].active ? 1 : 0;
+        if (mask == work_mask) return;
+        std::vector<xp::WorkItem> w = m->h_work;
+        for (auto& it : w)
+            if (it.sys >= 0 && !mask[static_cast<size_t>(it.sys)]) it = xp::WorkItem{-1, 0, 0, 0};
+        work_own.upload(w.data(), w.size(), stream);
+        HIPX(hipStreamSynchronize(stream));  // w is a temporary
+        work_mask = mask;
+    }
+
+    void enqueue_iteration(int k) {
+        xp::Launch L = iter_launch();
         L.set_parity(k);
         const xp::SellView A = m->view();
         const size_t ns = static_cast<size_t>(m->nsys);
@@ -554,6 +598,7 @@ struct xp_pcg {
     void build_graph(int iters) {
         if (exec != nullptr) { HIPX(hipGraphExecDestroy(exec)); exec = nullptr; }
         if (graph != nullptr) { HIPX(hipGraphDestroy(graph)); graph = nullptr; }
+        if (compacting()) iter_launch();  // the work-map copy is allocated outside the capture
         HIPX(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < iters; ++k) enqueue_iteration(k);
         HIPX(hipStreamEndCapture(stream, &graph));
@@ -633,6 +678,7 @@ struct xp_pcg {
         long long issued = 0;
         if (any_active()) {
             while (issued < cap) {
+                set_work_mask(st_host);
                 if (opts.use_graph) {
                     HIPX(hipGraphLaunch(exec, stream));
                     issued += graph_iters;
@@ -644,6 +690,7 @@ struct xp_pcg {
                 if (!any_active()) break;
             }
         }
+        set_work_mask(nullptr);
         flush_x();
     }
 };
@@ -2138,6 +2185,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
         long long issued = 0;
         bool running = S.any_active();
         while (running && issued < cap + S.graph_iters) {
+            S.set_work_mask(S.st_host);
             if (S.opts.use_graph) {
                 HIPX(hipGraphLaunch(S.exec, S.stream));
             } else {
@@ -2255,6 +2303,7 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
             }
             if (changed) HIPX(hipMemcpyAsync(S.st.p, S.st_host, ns * sizeof(xp::SysState), hipMemcpyHostToDevice, S.stream));
         }
+        S.set_work_mask(nullptr);
         // final fold and true residuals
         if (!acc) xp::launch_fold_f32_into_f64(m.n_pad, S.x.p, S.x64.p, S.stream);
         S.flush_x();

@@ -57,7 +57,7 @@ class MatrixInfo(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("tau", C.c_double), ("rel_tol", C.c_double), ("max_iters", C.c_int),
                 ("graph_iters", C.c_int), ("use_graph", C.c_int), ("trace_cap", C.c_int),
-                ("stall_limit", C.c_int), ("fused", C.c_int), ("accurate", C.c_int)]
+                ("stall_limit", C.c_int), ("fused", C.c_int), ("accurate", C.c_int), ("full_work_map", C.c_int)]
 
 
 class TuneReport(C.Structure):
@@ -838,13 +838,13 @@ class Pcg:
 
     def __init__(self, matrix: Matrix, tau: float = 1e-10, rel_tol: float = 0.0, max_iters: int = 0,
                  graph_iters: int = 0, use_graph: bool = True, trace_cap: int = 0, stall_limit: int = 0,
-                 fused: bool = False, accurate: bool = False):
+                 fused: bool = False, accurate: bool = False, full_work_map: bool = False):
         """tau: absolute bound on r^T r; rel_tol > 0 overrides it per system as
         (rel_tol ||b_k||)^2.  accurate: FP64-accumulated SpMV (hi+lo values,
         the matrix needs keep_lo) with FP64 p, A p and x."""
         self.matrix = matrix
         self.opts = SolveOpts(tau, rel_tol, max_iters, graph_iters, int(use_graph), trace_cap, stall_limit,
-                              int(fused), int(accurate))  # accurate: True/1 (hi + lo16) or 2 (hi + lo)
+                              int(fused), int(accurate), int(full_work_map))  # accurate: 1 (hi + lo16) or 2 (hi + lo)
         h = _vp()
         _check(_p_create(matrix.h, C.byref(self.opts), C.byref(h)))
         self.h = h
