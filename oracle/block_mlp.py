@@ -114,13 +114,29 @@ def features_from_csr(row_ptr, col_idx, values, groups=None):
     return feat, ahat, hmat, scale
 
 
-def forward(params, feat, keep=False):
+def bf16(x):
+    """Round to BF16 (nearest, ties to even), as the GPU kernels do before
+    their matrix products."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _ident(x):
+    return x
+
+
+def forward(params, feat, keep=False, round_bf16=False):
+    """round_bf16: round the inputs of every matrix product (activations and
+    weights) to BF16, where the device's MFMA GEMMs round them."""
     p = unpack(params)
-    z1 = feat @ p["W1"] + p["b1"]
+    r = bf16 if round_bf16 else _ident
+    z1 = r(feat) @ r(p["W1"]) + p["b1"]
     h1 = np.maximum(z1, 0.0)
-    z2 = h1 @ p["W2"] + p["b2"]
+    z2 = r(h1) @ r(p["W2"]) + p["b2"]
     h2 = np.maximum(z2, 0.0)
-    out = h2 @ p["W3"] + p["b3"]
+    out = r(h2) @ r(p["W3"]) + p["b3"]
     if keep:
         return out, (feat, z1, h1, z2, h2)
     return out
@@ -160,9 +176,14 @@ def factor(out, feat):
     return l0_from_feat(feat) + l_from_out(out)
 
 
-def loss_grad(params, feat, ahat, hmat):
-    """Mean over samples of tr(M H M^T) - 2 tr(M A) + 8, M = L L^T + eps I."""
-    out, (x, z1, h1, z2, h2) = forward(params, feat, keep=True)
+def loss_grad(params, feat, ahat, hmat, round_bf16=False):
+    """Mean over samples of tr(M H M^T) - 2 tr(M A) + 8, M = L L^T + eps I.
+    round_bf16: the device's rounding points -- BF16 inputs of the forward
+    products and of the two input-gradient products (output gradient,
+    hidden gradient, weights); weight gradients stay FP64 here (FP32 on the
+    device)."""
+    rnd = bf16 if round_bf16 else _ident
+    out, (x, z1, h1, z2, h2) = forward(params, feat, keep=True, round_bf16=round_bf16)
     n = out.shape[0]
     L = factor(out, feat)
     M = L @ np.transpose(L, (0, 2, 1)) + EPS_M * np.eye(8)
@@ -177,10 +198,10 @@ def loss_grad(params, feat, ahat, hmat):
     g = {}
     g["W3"] = h2.T @ dout
     g["b3"] = dout.sum(0)
-    dh2 = (dout @ p["W3"].T) * (z2 > 0)
+    dh2 = (rnd(dout) @ rnd(p["W3"]).T) * (z2 > 0)
     g["W2"] = h1.T @ dh2
     g["b2"] = dh2.sum(0)
-    dh1 = (dh2 @ p["W2"].T) * (z1 > 0)
+    dh1 = (rnd(dh2) @ rnd(p["W2"]).T) * (z1 > 0)
     g["W1"] = x.T @ dh1
     g["b1"] = dh1.sum(0)
     return float(loss.mean()), pack(g)

@@ -17,6 +17,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/xplus_capi.h"
@@ -852,7 +853,6 @@ struct Legacy {
     const double* va = nullptr;
     int n = -1;
     int nnz = -1;
-    uint64_t hash = 0;
     std::unique_ptr<xp_matrix> mat;
     // natural-order work buffers
     DevBuf<double> buf[8];
@@ -867,20 +867,45 @@ struct Legacy {
     }
     int flat_grid(int n) const { return std::max(1, std::min(1024, (n + 1023) / 1024)); }
 
+    // The cached device matrix is reused while the caller's CSR arrays are
+    // the same arrays with the same contents: compared byte for byte with
+    // the matrix's host copy (an in-place edit is noticed), in parallel
+    // chunks -- a plain memory comparison, far cheaper than the rebuild.
+    static bool same_bytes(const void* a, const void* b, size_t bytes) {
+        constexpr size_t kChunk = size_t{4} << 20;
+        if (bytes <= kChunk) return std::memcmp(a, b, bytes) == 0;
+        const size_t parts = std::min<size_t>(8, (bytes + kChunk - 1) / kChunk);
+        std::vector<char> ok(parts, 1);
+        std::vector<std::thread> th;
+        const size_t per = (bytes + parts - 1) / parts;
+        for (size_t k = 0; k < parts; ++k) {
+            const size_t o = k * per, len = std::min(per, bytes - std::min(bytes, o));
+            th.emplace_back([&, k, o, len] {
+                ok[k] = std::memcmp(static_cast<const char*>(a) + o, static_cast<const char*>(b) + o, len) == 0;
+            });
+        }
+        for (auto& t : th) t.join();
+        for (char c : ok)
+            if (!c) return false;
+        return true;
+    }
+
     xp_matrix& matrix(const int* row_ptr, const int* col_idx, const double* values, int n_) {
         const int nnz_ = row_ptr[n_];
         if (nnz_ < 0) throw std::runtime_error("row_ptr[n] must be non-negative");
-        uint64_t h = mix_hash(0x12345678ull, row_ptr, sizeof(int) * (static_cast<size_t>(n_) + 1));
-        h = mix_hash(h, col_idx, sizeof(int) * static_cast<size_t>(nnz_));
-        h = mix_hash(h, values, sizeof(double) * static_cast<size_t>(nnz_));
-        if (!mat || rp != row_ptr || ci != col_idx || va != values || n != n_ || nnz != nnz_ || hash != h) {
+        const size_t nz = static_cast<size_t>(nnz_);
+        const bool same = mat && rp == row_ptr && ci == col_idx && va == values && n == n_ && nnz == nnz_ &&
+                          same_bytes(row_ptr, mat->h_row_ptr.data(), sizeof(int) * (static_cast<size_t>(n_) + 1)) &&
+                          same_bytes(col_idx, mat->h_col_idx.data(), sizeof(int) * nz) &&
+                          same_bytes(values, mat->h_values.data(), sizeof(double) * nz);
+        if (!same) {
             mat.reset();
             xp::CsrRef a{n_, row_ptr, col_idx, values};
             xp_matrix_opts o{};
             o.sigma = 256;
             o.keep64 = 1;
             mat = make_matrix({a}, &o);
-            rp = row_ptr; ci = col_idx; va = values; n = n_; nnz = nnz_; hash = h;
+            rp = row_ptr; ci = col_idx; va = values; n = n_; nnz = nnz_;
         }
         return *mat;
     }

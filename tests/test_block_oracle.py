@@ -203,3 +203,29 @@ def test_accurate_block_pcg_stops_on_the_true_residual():
     xd, itd, rs = B.block_pcg_accurate(D, b, bd, (1e-12 * np.linalg.norm(b)) ** 2, groups=g)
     assert rs >= 1 and itd <= 2 * (rs + 1)
     assert np.linalg.norm(D @ xd - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_bf16_rounding_known_answers():
+    """Round to nearest, ties to even, at the 8-bit significand of BF16."""
+    one = 1.0
+    ulp = 2.0 ** -7                      # BF16 spacing at 1
+    vals = np.array([one, one + ulp / 2, one + 1.5 * ulp, one + ulp / 2 + 2.0 ** -20, -3.0, 0.0])
+    want = np.array([one, one, one + 2 * ulp, one + ulp, -3.0, 0.0])
+    np.testing.assert_array_equal(B.bf16(vals), want)
+
+
+def test_bf16_rounding_points_move_the_gradient_at_bf16_level():
+    """loss_grad with the device's rounding points against the exact one:
+    the same function up to BF16 rounding (relative 2^-8 per product input),
+    and identical when nothing needs rounding (all-zero parameters)."""
+    a, (feat, ahat, hmat, scale) = small_set(6)
+    params = random_params(7)
+    l0, g0 = B.loss_grad(params, feat, ahat, hmat)
+    l1, g1 = B.loss_grad(params, feat, ahat, hmat, round_bf16=True)
+    assert abs(l1 - l0) <= 2e-2 * abs(l0)
+    assert 0 < np.linalg.norm(g1 - g0) <= 0.1 * np.linalg.norm(g0)
+    z = np.zeros(B.N_PARAMS)
+    lz0, gz0 = B.loss_grad(z, feat, ahat, hmat)
+    lz1, gz1 = B.loss_grad(z, feat, ahat, hmat, round_bf16=True)
+    assert lz0 == lz1
+    np.testing.assert_array_equal(gz0, gz1)

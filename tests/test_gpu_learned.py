@@ -4,10 +4,11 @@ Bars:
   * MFMA forward (BF16 inputs/weights, FP32 accumulation) against the FP64
     oracle: normwise relative error <= 2e-2 per output;
   * loss and gradient (BF16 MFMA input-gradients, FP32 weight gradients):
-    against the oracle evaluated with the kernels' BF16 rounding points,
-    normwise relative error <= 1e-3 per parameter tensor; against the plain
-    FP64 oracle (BF16 rounding of the backward GEMM inputs included),
-    <= 0.15 for the hidden layers and <= 5e-2 for W3/b3, loss within 1e-2;
+    against oracle/block_mlp.py's loss_grad with its BF16 rounding points
+    (round_bf16), normwise relative error <= 1e-3 per parameter tensor;
+    against the plain FP64 loss_grad, <= 0.15 for the hidden layers and
+    <= 5e-2 for W3/b3 (what BF16 rounding of the backward products' inputs
+    alone moves), loss within 1e-2;
   * block-preconditioned FP32 PCG against the FP64 oracle block PCG with the
     same blocks: iterations within +-1, x within relative 1e-5, rr trace
     within 1e-4 while rr > 1e-3 rr0 (config-1 size);
@@ -32,14 +33,6 @@ import blocking as BK  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-def bf16(x):
-    """Round to BF16 (nearest even), as the GPU kernels do."""
-    a = np.ascontiguousarray(x, dtype=np.float32)
-    u = a.view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
-    return u.astype(np.uint32).view(np.float32).astype(np.float64)
-
-
 def test_mfma_forward_matches_oracle(gpu):
     a = X.gen_fem2d(40, 1.0, 4, 1e3, 0.3, 3)
     bs = X.blockset(a)
@@ -48,41 +41,12 @@ def test_mfma_forward_matches_oracle(gpu):
     out = net.forward(bs)
     feat = bs.arrays()[0].astype(np.float64)
     ref = B.forward(params, feat)
-    # the oracle on BF16-rounded inputs and weights (hidden layers rounded too)
-    p = B.unpack(params)
-    h1 = np.maximum(bf16(feat) @ bf16(p["W1"]) + p["b1"], 0)
-    h2 = np.maximum(bf16(h1) @ bf16(p["W2"]) + p["b2"], 0)
-    ref_bf = bf16(h2) @ bf16(p["W3"]) + p["b3"]
+    # the oracle with the device's BF16 rounding points
+    ref_bf = B.forward(params, feat, round_bf16=True)
     print("forward rel err vs fp64:", np.linalg.norm(out - ref) / np.linalg.norm(ref),
           "vs bf16 model:", np.linalg.norm(out - ref_bf) / np.linalg.norm(ref_bf), flush=True)
     assert np.linalg.norm(out - ref) <= 2e-2 * np.linalg.norm(ref)
     assert np.linalg.norm(out - ref_bf) <= 1e-5 * np.linalg.norm(ref_bf) + 1e-6
-
-
-def oracle_grad_bf16(params, feat, ahat, hmat):
-    """B.loss_grad with BF16 rounding where the GPU kernels round: GEMM inputs
-    (activations, output gradient, hidden gradient) and weights."""
-    p = B.unpack(params)
-    x = bf16(feat)
-    z1 = x @ bf16(p["W1"]) + p["b1"]
-    h1 = np.maximum(z1, 0)
-    z2 = bf16(h1) @ bf16(p["W2"]) + p["b2"]
-    h2 = np.maximum(z2, 0)
-    out = bf16(h2) @ bf16(p["W3"]) + p["b3"]
-    n = out.shape[0]
-    L = B.factor(out, feat)
-    M = L @ np.transpose(L, (0, 2, 1)) + B.EPS_M * np.eye(8)
-    D = 2.0 * (M @ hmat - ahat) / n
-    dL = (D + np.transpose(D, (0, 2, 1))) @ L
-    dout = np.zeros_like(out)
-    for t, (r, c) in enumerate(B.TRIL):
-        dout[:, t] = dL[:, r, c]
-    g = {"W3": h2.T @ dout, "b3": dout.sum(0)}
-    dh2 = (bf16(dout) @ bf16(p["W3"]).T) * (z2 > 0)
-    g["W2"], g["b2"] = h1.T @ dh2, dh2.sum(0)
-    dh1 = (bf16(dh2) @ bf16(p["W2"]).T) * (z1 > 0)
-    g["W1"], g["b1"] = feat.T @ dh1, dh1.sum(0)
-    return B.pack(g)
 
 
 def test_loss_and_gradient_match_oracle(gpu):
@@ -95,7 +59,7 @@ def test_loss_and_gradient_match_oracle(gpu):
     loss, grad = net.loss_grad(bs)
     feat, ahat, hmat, _ = (v.astype(np.float64) for v in bs.arrays())
     lref, gref = B.loss_grad(params, feat, ahat, hmat)
-    gbf = oracle_grad_bf16(params, feat, ahat, hmat)
+    _, gbf = B.loss_grad(params, feat, ahat, hmat, round_bf16=True)
     print(f"loss gpu={loss:.6e} oracle={lref:.6e}", flush=True)
     assert loss == pytest.approx(lref, rel=1e-2)
     o = 0
@@ -297,3 +261,36 @@ def test_useless_blocks_fall_back_to_jacobi(gpu):
     print("fallback status", st, flush=True)
     assert st["status"] == X.CONVERGED
     assert st["fallback"] == 1
+
+
+def test_trained_network_reproduces_exact_block_inverses(gpu):
+    """What the network is for on these aggregates, measured: trained with
+    the benchmark's loss (no off-block weight: its minimiser is the exact
+    block inverse) on other matrices of the family, its blocks on a held-out
+    matrix need within 3% of the iterations of exact block inverses (the
+    benchmark matrix: 6,129-6,238 against 6,179), and far fewer than Jacobi.
+    It does not beat exact 8x8 block inverses: the gain over Jacobi is the
+    block structure."""
+    train = [X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265700 + i) for i in range(3)]
+    test = X.gen_fem2d(140, 1.0, 20, 1e3, 0.3, 20265710)
+    sets = X.BlockSet(train[0], blocking=3, lam=0.0)
+    for t in train[1:]:
+        sets.append(X.BlockSet(t, blocking=3, lam=0.0))
+    net = X.Mlp(seed=20265700)
+    net.train(sets, 1000, batch=4096, lr=2e-3, seed=1)
+    m = X.Matrix(test, block_rows=8, keep_lo=True, blocking=3)
+    its = {}
+    for name in ("jacobi", "exact", "learned"):
+        s = X.Pcg(m, rel_tol=1e-6, accurate=True)
+        if name == "exact":
+            s.set_preconditioner(1)
+        elif name == "learned":
+            s.set_mlp_preconditioner(net)
+        s.set_rhs(test.b)
+        rel, _, it = s.run_reliable(1e-6, 0.0)
+        assert rel[0] <= 1e-6
+        its[name] = it
+        s.close()
+    print("held-out iterations:", its, flush=True)
+    assert abs(its["learned"] - its["exact"]) <= 0.03 * its["exact"]
+    assert its["learned"] < 0.75 * its["jacobi"]

@@ -22,11 +22,16 @@ Bars (written in each test):
 """
 from __future__ import annotations
 
+import sys
+
 import numpy as np
 import pytest
 
 import oracle_lib as O
 import xplus_amd as X
+
+sys.path.insert(0, str(O.ROOT / "oracle"))
+import block_mlp as B  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -283,11 +288,14 @@ def test_more_workgroups_than_resident(gpu, form):
 def test_accurate_to_1e10_against_fp64_oracle(gpu, delta):
     """The refinement target of the FP64-refined configuration: true relative
     residual 1e-10 on the FEM matrix, the error against the FP64 golden's
-    solution (solved to the same tolerance) reported and bounded, and no more
-    than 1.5x the golden's 2,806 iterations.  delta 0 checks only at
-    convergence: the true residual is then far above the recursive one and
-    the recurrence restarts from the FP64 residual (measured 5,176
-    iterations: bar 2x)."""
+    solution (solved to the same tolerance) reported and bounded.  delta 0.1
+    (replacements as the residual falls): no more than 1.5x the golden's
+    2,806 iterations.  delta 0 (a check only at convergence, where the true
+    residual is far above the recursive one and the recurrence restarts from
+    the FP64 residual): the count of oracle/block_pcg_accurate, which
+    simulates exactly that arithmetic and restart rule on the CPU (5,383
+    iterations, 2 restarts; the GPU 5,176, 1 restart), within 10% -- the
+    restarts make the count sensitive to last-bit differences."""
     a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, 20263203)
     tol = 1e-10
     g = O.golden(a, tau=(tol * np.linalg.norm(a.b)) ** 2, trace_cap=1)
@@ -301,8 +309,19 @@ def test_accurate_to_1e10_against_fp64_oracle(gpu, delta):
           f"rel {rel[0]:.3e}, error vs FP64 golden {err:.3e}", flush=True)
     assert rel[0] <= tol
     assert true_rel(a, x) <= tol * (1 + 1e-3)
-    # a restart (delta 0) discards the Krylov space once: up to 2x
-    assert iters <= (1.5 if delta > 0 else 2.0) * g["iterations"]
+    if delta > 0:
+        assert iters <= 1.5 * g["iterations"]
+    else:
+        A = a.to_scipy()
+        groups = B.natural_groups(a.n)
+        d = A.diagonal()
+        jac = np.zeros((len(groups), 8, 8))
+        for k, grp in enumerate(groups):
+            for j in range(8):
+                jac[k, j, j] = 1.0 / d[grp[j]] if j < len(grp) else 1.0
+        _, it_sim, rs_sim = B.block_pcg_accurate(A, a.b, jac, (tol * np.linalg.norm(a.b)) ** 2, groups=groups)
+        print(f"accurate-arithmetic simulation: {it_sim} iterations, {rs_sim} restarts", flush=True)
+        assert abs(iters - it_sim) <= 0.1 * it_sim
     assert err <= 1e-6
 
 
@@ -357,3 +376,34 @@ def test_accurate_value_modes_agree(gpu):
     assert xdiff <= 1e-5
     assert abs(out[1][1] - out[2][1]) <= max(2, 0.02 * out[2][1])   # 577 vs 581 measured
     assert out[1][3] <= 1e-6 and out[2][3] <= 1e-6
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_fp32_fem_full_solve_bar(gpu, fused):
+    """Plain FP32 Jacobi-PCG on the thermal-FEM family, a full solve to the
+    recursive 1e-6.  FP32 cannot meet a true 1e-6 here: every FP32 SpMV row
+    sum cancels (||A| |x|| is ~1e4 ||b||), so its rounding, u ||A| |x||
+    with u = 2^-24, is the floor the recursion drifts above.  Bars: converged
+    on the recursive residual; at most 1.5x the FP64 golden's iterations in
+    the three-kernel form (A p by SpMV; the benchmark matrix: 16,577 against
+    13,331, 1.24x) and 2x in the fused form, whose FP32 recurrence for A p
+    drifts further (benchmark matrix: 23,203, 1.74x); and a true relative
+    residual within 100 u ||A| |x|| / ||b|| (the benchmark matrix, 1.2M rows:
+    0.21 against 100 u ||A| |x|| / ||b|| = 0.24)."""
+    a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, 20263203)
+    tau = (1e-6 * np.linalg.norm(a.b)) ** 2
+    g = O.golden(a, tau=tau, trace_cap=1)
+    m = X.Matrix(a, sigma=256)
+    s = X.Pcg(m, rel_tol=1e-6, fused=fused)
+    s.set_rhs(a.b)
+    s.run()
+    st = s.status()
+    x = s.x()
+    A = a.to_scipy()
+    floor = 2.0 ** -24 * np.linalg.norm(abs(A) @ np.abs(g["x"])) / np.linalg.norm(a.b)
+    res = true_rel(a, x)
+    print(f"fp32 fused={fused}: {st['iterations']} iterations (golden {g['iterations']}), true rel {res:.3e}, "
+          f"u |A||x| / |b| = {floor:.3e}", flush=True)
+    assert st["status"] == X.CONVERGED
+    assert st["iterations"] <= (2.0 if fused else 1.5) * g["iterations"]
+    assert res <= 100 * floor
