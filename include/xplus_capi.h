@@ -118,6 +118,36 @@ int xp_matrix_create(int n, const int* row_ptr, const int* col_idx, const double
  * learned solve includes. */
 int xp_matrix_create_device(int n, const int* row_ptr, const int* col_idx, const double* values,
                             const xp_matrix_opts* opts, xp_matrix** out);
+/* The reference's 4x4 bitmap-block format (include/cg_common.hpp:26-50;
+ * packer archived/host/xrt_host.cpp:229-349; SpMV
+ * archived/kernels/spmv_blocked_kernel.cpp:19-89 and
+ * archived/kernels/pcg_control_kernel.cpp:125-245).  xp_spmv_block has the
+ * layout of SpmvBlock: bit 4 * local_row + local_col of the mask
+ * indices[2] | indices[3] << 8 marks a stored entry, values compact in
+ * ascending bit order, the rest zero.  A layout is (n, window_size,
+ * row_ptr, col_idx, blocks): window_size 0 = plain (block-row CSR, row_ptr
+ * of ceil(n / 4) + 1 entries: the standalone spmv_blocked_kernel's input),
+ * else window-major (the reference uses 512, kSpmvWindowSize) with
+ * ceil(n / window_size) * (ceil(n / 4) + 1) pointers, window w's at
+ * [w (nbr + 1), (w + 1)(nbr + 1)), each window starting where the last one
+ * ends.  Block column bc covers columns 4 bc .. 4 bc + 3; an entry outside
+ * the matrix or its window multiplies 0.0. */
+typedef struct xp_spmv_block {
+    double values[16];
+    unsigned char indices[16];
+} xp_spmv_block;
+/* One system from a bitmap-block layout in device memory (see
+ * xp_spmv_block): validated and unpacked to CSR on the device (the CSR of
+ * xp_bitmap_unpack, entry for entry), then set up as xp_matrix_create_device
+ * sets up that CSR.  nblocks <= 134,217,727.  No reference counterpart: the
+ * reference's blocked SpMV reads the layout directly; this puts matrices
+ * packed for it under the solver. */
+int xp_matrix_create_bitmap(int n, int window_size, const int* row_ptr, const int* col_idx,
+                            const xp_spmv_block* blocks, long long nblocks, const xp_matrix_opts* opts,
+                            xp_matrix** out);
+/* The CSR a one-system matrix was built from (host arrays of n + 1 and
+ * nnz entries, nnz from xp_matrix_get_info). */
+int xp_matrix_csr_copy(const xp_matrix* m, int* row_ptr, int* col_idx, double* values);
 /* stats[5]: device-built (0/1), aggregation rounds, regroup rounds, pieces
  * packed, blocks. */
 int xp_matrix_setup_stats(const xp_matrix* m, int* stats);
@@ -190,6 +220,26 @@ int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* str
 int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, double* avg_us);
 int xp_csr_spmv_f64(int n, const int* row_ptr, const int* col_idx, const double* values,
                     const double* x, double* y);
+
+/* y = A x straight from a bitmap-block layout in device memory (all
+ * pointers device, y n doubles; stream-ordered).  FP64, each row summed in
+ * the format's order from +0.0 with product and sum rounded separately:
+ * bit for bit the reference's sequential loops.  Replaces the HLS
+ * spmv_blocked_kernel(b_row_ptr, b_col_idx, blocks, x, y, n)
+ * (archived/kernels/spmv_blocked_kernel.cpp:131-161; window_size 0) and the
+ * windowed SpMV of pcg_control_kernel (spmv_blocked_windowed,
+ * archived/kernels/pcg_control_kernel.cpp:317-506; window_size 512).  The
+ * layout is not validated here (xp_matrix_create_bitmap and
+ * xp_bitmap_unpack do): pointers outside [0, nblocks] are clamped and
+ * negative block columns read 0.0, so malformed input reads nothing out of
+ * bounds. */
+int xp_bitmap_spmv_f64(int n, int window_size, const int* row_ptr, const int* col_idx, const xp_spmv_block* blocks,
+                       long long nblocks, const double* x, double* y, void* stream);
+/* Average microseconds per launch of xp_bitmap_spmv_f64 over `reps`
+ * launches after `warmup`, each between its own HIP events on the library
+ * stream (device pointers, x and y of n doubles). */
+int xp_bitmap_spmv_time(int n, int window_size, const int* row_ptr, const int* col_idx, const xp_spmv_block* blocks,
+                        long long nblocks, const double* x, double* y, int warmup, int reps, double* avg_us);
 
 /* Device memory and timing helpers for drivers that hold no other GPU
  * runtime (tests, benchmarks): plain hipMalloc/hipMemcpy on the library's
@@ -555,6 +605,25 @@ int xp_cuper_packed_create(int batch_count, int channel_size, const int* sp_elem
                            const unsigned long long* const* channels, xp_cuper_packed** out);
 int xp_cuper_unpack(const xp_cuper_packed* p, int n, xp_csr** out);
 void xp_cuper_packed_free(xp_cuper_packed* p);
+/* The bitmap-block format (xp_spmv_block) on the host: the reference's
+ * packer and the unpack to CSR. */
+typedef struct xp_bitmap xp_bitmap;
+/* build_windowed_blocked_spmv_matrix: duplicate CSR entries summed in CSR
+ * order, positions whose sum is 0.0 and empty blocks dropped, block
+ * columns ascending within a block row.  The host builder's CSR checks. */
+int xp_bitmap_pack(int n, const int* row_ptr, const int* col_idx, const double* values, int window_size,
+                   xp_bitmap** out);
+int xp_bitmap_info(const xp_bitmap* b, int* num_windows, int* num_block_rows, long long* nblocks);
+/* row_ptr: num_windows * (num_block_rows + 1) entries; col_idx, blocks: nblocks. */
+int xp_bitmap_copy(const xp_bitmap* b, int* row_ptr, int* col_idx, xp_spmv_block* blocks);
+void xp_bitmap_free(xp_bitmap* b);
+/* CSR of a layout held in host memory: per row the entries y = A x reads,
+ * in its order (windows, blocks, bits), entries outside the matrix or their
+ * window dropped, stored zeros kept; b = x0 = 0.  Errors "bitmap row_ptr
+ * out of range", "bitmap row_ptr must be nondecreasing", "bitmap row_ptr
+ * windows must be contiguous", "bitmap col_idx must be non-negative". */
+int xp_bitmap_unpack(int n, int window_size, const int* row_ptr, const int* col_idx, const xp_spmv_block* blocks,
+                     long long nblocks, xp_csr** out);
 /* Host-side SELL build for inspection and CPU tests (no GPU): returns the
  * arrays of the format described in the library's xp_types.hpp.  With
  * block_rows = 8 and `slots` (units x 8 rows, -1 padding) the blocks are

@@ -727,6 +727,11 @@ void launch_sys_sumsq(int nsys, const int* sys_begin, const double* x, double* o
                        out);
 }
 
+void device_scan_exclusive(const int* in, int* out, int n, int* total, hipStream_t s) {
+    scan_exclusive(in, out, n, total, s);
+    HIPX(hipGetLastError());
+}
+
 // ---- host side ------------------------------------------------------------------------------
 std::mutex& set_up_mutex() {
     static std::mutex mu;

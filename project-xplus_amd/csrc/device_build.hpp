@@ -105,6 +105,10 @@ void device_sell(const DevCsr& a, const DevRows& rows, int sigma, bool keep_lo, 
 // Blocks of 8 consecutive rows (the last one padded): ceil(n / 8) units.
 int device_natural_slots(int n, DevBuf<int>& slots, hipStream_t s);
 
+// out[i] = sum in[0..i), *total (device pointer, may be null) = sum in[0..n),
+// n >= 0.  Uses the set-up's work buffers: call with set_up_mutex held.
+void device_scan_exclusive(const int* in, int* out, int n, int* total, hipStream_t s);
+
 // out[k] = sum of x[i]^2 over the natural rows [sys_begin[k], sys_begin[k+1])
 // of system k, in a fixed order; part: kSysSumParts * nsys doubles of scratch.
 constexpr int kSysSumParts = 64;

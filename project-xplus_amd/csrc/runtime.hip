@@ -21,6 +21,8 @@
 #include <vector>
 
 #include "../../include/xplus_capi.h"
+#include "bitmap_blocks.hpp"
+#include "bitmap_device.hpp"
 #include "csr_dataset.hpp"
 #include "cuper_format.hpp"
 #include "device_build.hpp"
@@ -1084,6 +1086,38 @@ int xp_matrix_create_device(int n, const int* row_ptr, const int* col_idx, const
     });
 }
 
+static_assert(sizeof(xp_spmv_block) == sizeof(xp::BitmapBlock), "xp_spmv_block layout");
+
+int xp_matrix_create_bitmap(int n, int window_size, const int* row_ptr, const int* col_idx,
+                            const xp_spmv_block* blocks, long long nblocks, const xp_matrix_opts* opts,
+                            xp_matrix** out) {
+    return guarded([&] {
+        if (out == nullptr) throw std::runtime_error("null output pointer");
+        *out = nullptr;
+        if (n <= 0) throw std::runtime_error("n must be positive");
+        const hipStream_t s = device().stream;
+        DevBuf<int> rp, ci;
+        DevBuf<double> va;
+        xp::device_bitmap_csr(n, window_size, row_ptr, col_idx, reinterpret_cast<const xp::BitmapBlock*>(blocks),
+                              nblocks, rp, ci, va, s);
+        *out = make_matrix_device(n, rp.p, ci.p, va.p, opts).release();
+        HIPX(hipStreamSynchronize(s));  // before the temporaries go
+    });
+}
+
+int xp_matrix_csr_copy(const xp_matrix* m, int* row_ptr, int* col_idx, double* values) {
+    return guarded([&] {
+        if (m == nullptr || row_ptr == nullptr) throw std::runtime_error("null argument");
+        if (m->nsys != 1) throw std::runtime_error("xp_matrix_csr_copy: one-system matrices only");
+        ensure_host_mirror(*m);
+        std::copy(m->h_row_ptr.begin(), m->h_row_ptr.end(), row_ptr);
+        if (!m->h_col_idx.empty() && (col_idx == nullptr || values == nullptr))
+            throw std::runtime_error("null argument");
+        std::copy(m->h_col_idx.begin(), m->h_col_idx.end(), col_idx);
+        std::copy(m->h_values.begin(), m->h_values.end(), values);
+    });
+}
+
 int xp_matrix_setup_stats(const xp_matrix* m, int* stats) {
     return guarded([&] {
         if (m == nullptr || stats == nullptr) throw std::runtime_error("null argument");
@@ -1324,6 +1358,40 @@ int xp_spmv_f32_time(xp_matrix* m, int warmup, int reps, double* avg_us) {
 // The same SpMV timing with the caches cold: before every timed launch a
 // device-to-device copy of flush_bytes (more than the 256 MB MALL and the
 // L2s together) replaces the cached lines, so the matrix and x come from HBM.
+int xp_bitmap_spmv_f64(int n, int window_size, const int* row_ptr, const int* col_idx, const xp_spmv_block* blocks,
+                       long long nblocks, const double* x, double* y, void* stream) {
+    return guarded([&] {
+        xp::launch_bitmap_spmv(n, window_size, row_ptr, col_idx, reinterpret_cast<const xp::BitmapBlock*>(blocks),
+                               nblocks, x, y, pick_stream(stream));
+    });
+}
+
+int xp_bitmap_spmv_time(int n, int window_size, const int* row_ptr, const int* col_idx, const xp_spmv_block* blocks,
+                        long long nblocks, const double* x, double* y, int warmup, int reps, double* avg_us) {
+    return guarded([&] {
+        if (avg_us == nullptr || reps <= 0 || n <= 0) throw std::runtime_error("bad argument");
+        const hipStream_t s = device().stream;
+        const auto* b = reinterpret_cast<const xp::BitmapBlock*>(blocks);
+        for (int k = 0; k < warmup; ++k) xp::launch_bitmap_spmv(n, window_size, row_ptr, col_idx, b, nblocks, x, y, s);
+        std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps));
+        for (auto& e : ev) HIPX(hipEventCreate(&e));
+        for (int k = 0; k < reps; ++k) {
+            HIPX(hipEventRecord(ev[2 * static_cast<size_t>(k)], s));
+            xp::launch_bitmap_spmv(n, window_size, row_ptr, col_idx, b, nblocks, x, y, s);
+            HIPX(hipEventRecord(ev[2 * static_cast<size_t>(k) + 1], s));
+        }
+        HIPX(hipStreamSynchronize(s));
+        double total = 0.0;
+        for (int k = 0; k < reps; ++k) {
+            float ms = 0.0f;
+            HIPX(hipEventElapsedTime(&ms, ev[2 * static_cast<size_t>(k)], ev[2 * static_cast<size_t>(k) + 1]));
+            total += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *avg_us = 1000.0 * total / reps;
+    });
+}
+
 int xp_spmv_f32_time_cold(xp_matrix* m, int reps, unsigned long long flush_bytes, double* avg_us) {
     return guarded([&] {
         if (m == nullptr || avg_us == nullptr || reps <= 0 || flush_bytes == 0) throw std::runtime_error("bad argument");
@@ -2535,6 +2603,56 @@ int xp_cuper_unpack(const xp_cuper_packed* p, int n, xp_csr** out) {
 }
 
 void xp_cuper_packed_free(xp_cuper_packed* p) { delete p; }
+
+struct xp_bitmap {
+    xp::BitmapMatrix b;
+};
+
+int xp_bitmap_pack(int n, const int* row_ptr, const int* col_idx, const double* values, int window_size,
+                   xp_bitmap** out) {
+    return guarded([&] {
+        if (out == nullptr || row_ptr == nullptr || (n > 0 && (col_idx == nullptr || values == nullptr)))
+            throw std::runtime_error("null argument");
+        *out = nullptr;
+        auto h = std::make_unique<xp_bitmap>();
+        h->b = xp::bitmap_pack(n, row_ptr, col_idx, values, window_size);
+        *out = h.release();
+    });
+}
+
+int xp_bitmap_info(const xp_bitmap* b, int* num_windows, int* num_block_rows, long long* nblocks) {
+    return guarded([&] {
+        if (b == nullptr) throw std::runtime_error("null argument");
+        if (num_windows != nullptr) *num_windows = b->b.num_windows;
+        if (num_block_rows != nullptr) *num_block_rows = b->b.num_block_rows;
+        if (nblocks != nullptr) *nblocks = static_cast<long long>(b->b.blocks.size());
+    });
+}
+
+int xp_bitmap_copy(const xp_bitmap* b, int* row_ptr, int* col_idx, xp_spmv_block* blocks) {
+    return guarded([&] {
+        if (b == nullptr) throw std::runtime_error("null argument");
+        if (row_ptr != nullptr) std::copy(b->b.row_ptr.begin(), b->b.row_ptr.end(), row_ptr);
+        if (col_idx != nullptr) std::copy(b->b.col_idx.begin(), b->b.col_idx.end(), col_idx);
+        if (blocks != nullptr && !b->b.blocks.empty())
+            std::memcpy(blocks, b->b.blocks.data(), b->b.blocks.size() * sizeof(xp::BitmapBlock));
+    });
+}
+
+void xp_bitmap_free(xp_bitmap* b) { delete b; }
+
+int xp_bitmap_unpack(int n, int window_size, const int* row_ptr, const int* col_idx, const xp_spmv_block* blocks,
+                     long long nblocks, xp_csr** out) {
+    return guarded([&] {
+        if (out == nullptr || (n > 0 && row_ptr == nullptr) || (nblocks > 0 && (col_idx == nullptr || blocks == nullptr)))
+            throw std::runtime_error("null argument");
+        *out = nullptr;
+        auto h = std::make_unique<xp_csr>();
+        h->a = xp::bitmap_unpack(n, window_size, row_ptr, col_idx, reinterpret_cast<const xp::BitmapBlock*>(blocks),
+                                 nblocks);
+        *out = h.release();
+    });
+}
 
 int xp_gen_diffusion2d(int nx, int ny, double sigma, unsigned long long seed, xp_csr** out) {
     return guarded([&] {

@@ -205,6 +205,16 @@ _cu_copy = _sig("xp_cuper_packed_copy", C.c_int, [_vp, _i32p, _u64pp])
 _cu_create = _sig("xp_cuper_packed_create", C.c_int, [C.c_int, C.c_int, _i32p, _u64pp, C.POINTER(_vp)])
 _cu_unpack = _sig("xp_cuper_unpack", C.c_int, [_vp, C.c_int, C.POINTER(_vp)])
 _cu_free = _sig("xp_cuper_packed_free", None, [_vp])
+_bm_pack = _sig("xp_bitmap_pack", C.c_int, [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.POINTER(_vp)])
+_bm_info = _sig("xp_bitmap_info", C.c_int,
+                [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_longlong)])
+_bm_copy = _sig("xp_bitmap_copy", C.c_int, [_vp, _i32p, _i32p, _vp])
+_bm_free = _sig("xp_bitmap_free", None, [_vp])
+_bm_unpack = _sig("xp_bitmap_unpack", C.c_int,
+                  [C.c_int, C.c_int, _i32p, _i32p, _vp, C.c_longlong, C.POINTER(_vp)])
+_bm_spmv = _sig("xp_bitmap_spmv_f64", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, C.c_longlong, _vp, _vp, _vp])
+_bm_time = _sig("xp_bitmap_spmv_time", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, C.c_longlong, _vp, _vp,
+                                                   C.c_int, C.c_int, C.POINTER(C.c_double)])
 
 class MtxInfo(C.Structure):
     _fields_ = [("nrows", C.c_longlong), ("ncols", C.c_longlong), ("nnz_reported", C.c_longlong),
@@ -226,6 +236,9 @@ _sell_copy = _sig("xp_sell_copy_host", C.c_int, [_vp] * 9)
 _m_create_dev = _sig("xp_matrix_create_device", C.c_int,
                      [C.c_int, _vp, _vp, _vp, C.POINTER(MatrixOpts), C.POINTER(_vp)])
 _m_setup_stats = _sig("xp_matrix_setup_stats", C.c_int, [_vp, _i32p])
+_m_create_bitmap = _sig("xp_matrix_create_bitmap", C.c_int, [C.c_int, C.c_int, _vp, _vp, _vp, C.c_longlong,
+                                                             C.POINTER(MatrixOpts), C.POINTER(_vp)])
+_m_csr_copy = _sig("xp_matrix_csr_copy", C.c_int, [_vp, _i32p, _i32p, _f64p])
 _m_sell_copy = _sig("xp_matrix_sell_copy", C.c_int, [_vp] * 10)
 _p_set_rhs_dev = _sig("xp_pcg_set_rhs_device", C.c_int, [_vp, _vp, _vp])
 _p_get_x_dev = _sig("xp_pcg_get_x_device", C.c_int, [_vp, _vp])
@@ -472,6 +485,116 @@ class CuperPacked:
             _cu_free(h)
 
 
+# The reference's SpmvBlock (include/cg_common.hpp:47-50): 16 FP64 values,
+# 16 index bytes whose bytes 2..3 hold the 16-bit occupancy mask.
+SPMV_BLOCK = np.dtype([("values", "<f8", (16,)), ("indices", "u1", (16,))])
+assert SPMV_BLOCK.itemsize == 144
+
+
+class BitmapBlocks:
+    """A matrix in the reference's 4x4 bitmap-block layout (host arrays):
+    window_size 0 = plain block-row CSR (row_ptr of nbr + 1 entries), else
+    window-major with num_windows * (nbr + 1) pointers."""
+
+    def __init__(self, n: int, window_size: int, row_ptr, col_idx, blocks):
+        self.n = int(n)
+        self.window_size = int(window_size)
+        self.row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+        self.col_idx = np.ascontiguousarray(col_idx, np.int32)
+        self.blocks = np.ascontiguousarray(blocks, SPMV_BLOCK)
+        # the C-ABI takes bare pointers: the array lengths are checked here
+        want = self.num_windows * (self.num_block_rows + 1) if self.n > 0 else len(self.row_ptr)
+        if len(self.row_ptr) != want:
+            raise ValueError(f"row_ptr holds {len(self.row_ptr)} entries, the layout needs {want}")
+        if len(self.col_idx) != len(self.blocks):
+            raise ValueError("col_idx and blocks differ in length")
+
+    @property
+    def num_block_rows(self) -> int:
+        return (self.n + 3) // 4
+
+    @property
+    def num_windows(self) -> int:
+        return -(-self.n // self.window_size) if self.window_size > 0 else 1
+
+    @property
+    def nblocks(self) -> int:
+        return len(self.blocks)
+
+    def masks(self) -> np.ndarray:
+        return self.blocks["indices"][:, 2].astype(np.uint32) | self.blocks["indices"][:, 3].astype(np.uint32) << 8
+
+    def unpack(self) -> "Csr":
+        """The CSR the layout describes (xp_bitmap_unpack)."""
+        h = _vp()
+        _check(_bm_unpack(self.n, self.window_size, self.row_ptr, self.col_idx, self.blocks.ctypes.data_as(_vp),
+                          self.nblocks, C.byref(h)))
+        return Csr._from_handle(h)
+
+    def to_device(self) -> "DeviceBitmapBlocks":
+        return DeviceBitmapBlocks(self)
+
+
+def bitmap_pack(a: Csr, window_size: int = 512) -> BitmapBlocks:
+    """build_windowed_blocked_spmv_matrix (archived/host/xrt_host.cpp:229-349);
+    window_size 0: the plain layout."""
+    h = _vp()
+    _check(_bm_pack(a.n, a.row_ptr, a.col_idx, a.values, window_size, C.byref(h)))
+    try:
+        nw, nbr, nb = C.c_int(), C.c_int(), C.c_longlong()
+        _check(_bm_info(h, C.byref(nw), C.byref(nbr), C.byref(nb)))
+        rp = np.zeros(nw.value * (nbr.value + 1), np.int32)
+        ci = np.zeros(nb.value, np.int32)
+        blocks = np.zeros(nb.value, SPMV_BLOCK)
+        _check(_bm_copy(h, rp, ci, blocks.ctypes.data_as(_vp)))
+        return BitmapBlocks(a.n, window_size, rp, ci, blocks)
+    finally:
+        _bm_free(h)
+
+
+class DeviceBitmapBlocks:
+    """A bitmap-block layout uploaded to device memory."""
+
+    def __init__(self, bm: BitmapBlocks):
+        self.host = bm
+        self.n = bm.n
+        self.window_size = bm.window_size
+        self.nblocks = bm.nblocks
+        self.row_ptr = DeviceArray.from_numpy(bm.row_ptr if len(bm.row_ptr) else np.zeros(1, np.int32))
+        self.col_idx = DeviceArray.from_numpy(bm.col_idx if bm.nblocks else np.zeros(1, np.int32))
+        self.blocks = DeviceArray.from_numpy(bm.blocks.view(np.uint8).reshape(-1) if bm.nblocks
+                                             else np.zeros(144, np.uint8))
+
+    def _args(self):
+        return (self.n, self.window_size, self.row_ptr.ptr, self.col_idx.ptr, self.blocks.ptr, self.nblocks)
+
+    def spmv(self, x: "DeviceArray", y: "DeviceArray", stream: int = 0) -> None:
+        """y = A x on device arrays of n doubles (xp_bitmap_spmv_f64)."""
+        _check(_bm_spmv(*self._args(), x.ptr, y.ptr, stream or None))
+
+    def spmv_numpy(self, x) -> np.ndarray:
+        """y = A x for a host vector (uploads x, downloads y)."""
+        xd = DeviceArray.from_numpy(_f64(x))
+        yd = DeviceArray(max(self.n, 1), np.float64)
+        try:
+            self.spmv(xd, yd)
+            synchronize()
+            return yd.numpy()[:self.n]
+        finally:
+            xd.close()
+            yd.close()
+
+    def time(self, x: "DeviceArray", y: "DeviceArray", warmup: int = 10, reps: int = 100) -> float:
+        """Average microseconds per SpMV launch (HIP events on its stream)."""
+        us = C.c_double()
+        _check(_bm_time(*self._args(), x.ptr, y.ptr, warmup, reps, C.byref(us)))
+        return us.value
+
+    def close(self) -> None:
+        for k in ("row_ptr", "col_idx", "blocks"):
+            getattr(self, k).close()
+
+
 def cuper_pack(a: Csr) -> CuperPacked:
     h = _vp()
     _check(_cu_pack(a.n, a.row_ptr, a.col_idx, a.values, C.byref(h)))
@@ -621,23 +744,33 @@ def synchronize() -> None:
 
 
 # ---------------------------------------------------------------- device matrix
+class _Shape:
+    """The size of a system whose CSR lives only on the device."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+
+
 class Matrix:
     """Device-resident SELL-C-sigma matrix (one system or a batch)."""
 
     def __init__(self, systems, sigma: int = 0, keep_lo: bool = False, keep64: bool = False, grid: int = 0,
                  spmv_variant: int = 0, block_rows: int = 1, blocking: int = 0, device_build: bool = False,
-                 _device_csr: "DeviceCsr" = None):
+                 _device_csr: "DeviceCsr" = None, _device_bitmap: "DeviceBitmapBlocks" = None):
         """blocking (with block_rows = 8): 0 blocks of 8 consecutive rows,
         1 greedy strength aggregates, 2 strength-ordered aggregates, 3 the
         parallel strength aggregation on the device.  device_build: set the
         matrix up on the device (one system)."""
-        if isinstance(systems, Csr):
+        if isinstance(systems, (Csr, _Shape)):
             systems = [systems]
         self.systems = list(systems)
         opts = MatrixOpts(sigma, int(keep_lo), int(keep64), grid, spmv_variant, block_rows, blocking,
                           int(device_build))
         h = _vp()
-        if _device_csr is not None:
+        if _device_bitmap is not None:
+            d = _device_bitmap
+            _check(_m_create_bitmap(*d._args(), C.byref(opts), C.byref(h)))
+        elif _device_csr is not None:
             d = _device_csr
             _check(_m_create_dev(d.n, d.row_ptr.ptr, d.col_idx.ptr, d.values.ptr, C.byref(opts), C.byref(h)))
         elif len(self.systems) == 1:
@@ -663,6 +796,21 @@ class Matrix:
     def from_device(cls, d: "DeviceCsr", **kw) -> "Matrix":
         """A matrix set up on the device from a CSR already in device memory."""
         return cls(d.host, _device_csr=d, **kw)
+
+    @classmethod
+    def from_bitmap(cls, d: "DeviceBitmapBlocks", **kw) -> "Matrix":
+        """A matrix set up on the device from a bitmap-block layout in device
+        memory (xp_matrix_create_bitmap)."""
+        return cls(_Shape(d.n), _device_bitmap=d, **kw)
+
+    def csr(self):
+        """The CSR the matrix was built from: (row_ptr, col_idx, values)."""
+        i = self.info()
+        rp = np.empty(i["n"] + 1, np.int32)
+        ci = np.empty(max(i["nnz"], 1), np.int32)
+        va = np.empty(max(i["nnz"], 1), np.float64)
+        _check(_m_csr_copy(self.h, rp, ci, va))
+        return rp, ci[:i["nnz"]], va[:i["nnz"]]
 
     def setup_stats(self) -> dict:
         st = np.zeros(5, np.int32)

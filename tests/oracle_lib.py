@@ -17,6 +17,7 @@ REF_SO = ROOT / "oracle" / "_ref" / "libref.so"
 REF_BIN = ROOT / "oracle" / "_ref" / "xplus_local"
 REF_ON_AMD_BIN = ROOT / "oracle" / "_ref" / "xplus_local_on_amd"
 REF_CUPER_ON_AMD_BIN = ROOT / "oracle" / "_ref" / "cuper_pcg_on_amd"
+REF_BLOCKED_SO = ROOT / "oracle" / "_ref" / "libref_blocked.so"
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
