@@ -170,6 +170,39 @@ def spmv_beyond_mall(X, mesh: int = 2400) -> dict:
     return out
 
 
+def bitmap_block_spmv(X, a) -> dict:
+    """The same matrix in the reference's 4x4 bitmap-block layout (plain
+    layout, 144-byte blocks + a 4-byte block column each): the direct FP64
+    SpMV (k_bitmap_spmv, HIP events around each launch) against the bytes
+    the format streams, the host packer, and the import into a device matrix
+    (device unpack + rule-3 set-up)."""
+    t0 = time.perf_counter()
+    bm = X.bitmap_pack(a, 0)
+    pack_s = time.perf_counter() - t0
+    d = bm.to_device()
+    x = X.DeviceArray.from_numpy(np.random.default_rng(1).standard_normal(a.n))
+    y = X.DeviceArray(a.n, np.float64)
+    us = d.time(x, y, 20, 200)
+    nbr = (a.n + 3) // 4
+    alg = bm.nblocks * (144 + 4) + 4 * (nbr + 1) + 16 * a.n
+    X.synchronize()
+    t0 = time.perf_counter()
+    m = X.Matrix.from_bitmap(d, block_rows=8, blocking=3, keep_lo=True)
+    X.synchronize()
+    imp_ms = 1e3 * (time.perf_counter() - t0)
+    m.close()
+    for buf in (x, y, d):
+        buf.close()
+    out = {"kernel": "k_bitmap_spmv", "layout": "plain (window 0)", "nblocks": bm.nblocks,
+           "entries_per_block": round(a.nnz / max(bm.nblocks, 1), 3), "avg_us": round(us, 3), "alg_bytes": alg,
+           "alg_bytes_note": "148 B per block (SpmvBlock + block column) + 4 B per block row + x and y (16 B per row)",
+           "achieved_gbs": round(alg / (us * 1e-6) / 1e9, 1),
+           "frac": round(alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+           "pack_host_s": round(pack_s, 3), "import_setup_ms": round(imp_ms, 2)}
+    log(f"bitmap-block SpMV: {out}")
+    return out
+
+
 def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
     """The accurate iteration's two kernels timed with HIP events between
     launches on the solver stream; the dominant one is k_pcg_spmv_fused_acc."""
@@ -329,11 +362,13 @@ def profile_run(X, a, args) -> int:
     m = X.Matrix(a, sigma=64, grid=512)
     spmv = spmv_roofline(m, a)
     m.close()
+    bitmap = bitmap_block_spmv(X, a)
     net, _ = train_network(X, args)
     ml, _ = learned_matrix(X, a, args, tune=False)
     out = accurate_solve(X, a, ml, args, net=net, kernels=True)
     out["matrix"] = ml.info()
     out["spmv_roofline"] = spmv
+    out["bitmap_block_spmv"] = bitmap
     print(json.dumps({"profile": "learned accurate solve", **out}), flush=True)
     return 0
 
@@ -676,6 +711,7 @@ def workload_c2(X, args, cpu_proc) -> int:
     spmv["beyond_mall"]["frac_of_stream_read"] = round(spmv["beyond_mall"]["warm_gbs"] /
                                                        spmv["stream_reference"]["read_gbs"], 4)
     log(f"spmv: {spmv}")
+    bitmap = bitmap_block_spmv(X, a)
     cpu = None
     if cpu_proc is not None:
         cpu = finish_cpu_golden(cpu_proc, 300.0)
@@ -727,6 +763,7 @@ def workload_c2(X, args, cpu_proc) -> int:
         "fp32_iterations_to_recursive_1e-6": it,
         "spmv_hbm_gbs": spmv["achieved"],
         "spmv_roofline": spmv,
+        "bitmap_block_spmv": bitmap,
         "roofline": iroof,
         "cpu_baseline": cpu,
     }

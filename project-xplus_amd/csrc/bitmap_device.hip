@@ -95,8 +95,18 @@ __device__ __forceinline__ double sum(double acc, double p) {
 // loads of kSpmvBatch blocks are issued together: first their masks and
 // columns, then all their values and x entries, then the products are
 // added in order.  A row's blocks then cost about two memory round trips
-// per batch instead of two per block.
-constexpr int kSpmvBatch = 8;
+// per batch instead of two per block.  Measured on the thermal-FEM 1100^2
+// matrix (2.1 M blocks, 4.0 entries each; the format streams 332 MB):
+//   one block at a time                                   103 us
+//   batches of 8 (182 VGPRs)                               87 us
+//   batches of 4                                         80.6 us, 541 MB read
+//   the 4 lanes of a block row copy each batch into LDS
+//   with 16-byte loads, then read masks and values there  93.8 us, 392 MB read
+//   a workgroup's blocks staged in LDS behind barriers     122 us
+// The re-reads of this form (541 MB against 332 MB: the 16 block-row
+// streams of a wavefront outlive the L2 between a block's mask load and its
+// value loads) cost less than the extra dependent LDS step that removes them.
+constexpr int kSpmvBatch = 4;
 
 __global__ __launch_bounds__(kBitmapThreads) void k_bitmap_spmv(BitmapView v, const double* __restrict__ x,
                                                                double* __restrict__ y) {
