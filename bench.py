@@ -661,8 +661,13 @@ def break_even(learned: dict, base: dict):
 
 def workload_c2(X, args, cpu_proc) -> int:
     a = X.gen_fem2d(args.mesh, 1.0, 20, 1e3, 0.3, SEED)
-    d = X.DeviceCsr(a)
-    log(f"FEM mesh {args.mesh}: n={a.n} nnz={a.nnz}, CSR and b resident in HBM")
+    X.DeviceCsr(a).close()          # first-touch of the runtime's transfer path, untimed
+    X.synchronize()
+    t0 = time.perf_counter()
+    d = X.DeviceCsr(a)              # CSR + b from host memory over PCIe
+    X.synchronize()
+    upload_ms = (time.perf_counter() - t0) * 1000.0
+    log(f"FEM mesh {args.mesh}: n={a.n} nnz={a.nnz}, CSR and b resident in HBM (upload {upload_ms:.2f} ms)")
     net, tinfo = train_network(X, args)
     layouts = tune_layouts(X, args)
     res = {}
@@ -766,6 +771,10 @@ def workload_c2(X, args, cpu_proc) -> int:
         "bitmap_block_spmv": bitmap,
         "roofline": iroof,
         "cpu_baseline": cpu,
+        "host_upload_ms": round(upload_ms, 3),
+        "value_with_host_upload_ms": round(value + upload_ms, 3),
+        "host_upload_note": "CSR (row_ptr, col_idx, values) and b copied from pageable host memory; value starts "
+                            "with them resident in HBM",
     }
     if cpu is not None:
         out["speedup_vs_cpu_golden"] = round(cpu["value"] / value, 1)
