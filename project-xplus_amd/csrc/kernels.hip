@@ -880,7 +880,9 @@ __global__ __launch_bounds__(kBlock, 8) void k_pcg_update_xrz_acc(Launch L, cons
     // (average of an even and an odd update, 256 / 512 workgroups): the
     // distributed block factor 12.50 / 12.33 us against 13.14 / 12.68 with
     // every lane loading the whole factor; two rows per pass 12.84 / 13.13,
-    // three in the even launches 13.41 / 13.72.
+    // three in the even launches 13.41 / 13.72; a software pipeline keeping
+    // the loads of the next two passes in flight (three pass buffers in
+    // fixed roles) 24.0 us at 512 workgroups.
     constexpr int ROWS = BLOCK ? 1 : kRowsAcc;
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
