@@ -34,8 +34,9 @@ extern "C" {
  *     reductions in a fixed parallel order instead of a sequential loop;
  *   - a malformed CSR (which the reference would read out of bounds) makes
  *     the call a no-op and records the reason for xp_last_error().
- * The SpMV caches the device copy of the last matrix seen, keyed by the
- * three array pointers, n, nnz and a 64-bit hash of the arrays' contents.
+ * The SpMV caches the device copy of the last matrix seen and reuses it
+ * while a call's three arrays hold the same bytes (compared with a host
+ * copy, so in-place edits are seen).
  * ===================================================================== */
 
 /* y = A x, CSR.  archived/include/cg_kernels.hpp:11-16 */
@@ -57,7 +58,7 @@ void update_xrz_kernel(double* x, const double* p, double* r, const double* ap,
 void update_p_kernel(const double* z, double* p, double beta, int n);
 
 /* Drops the cached device matrix of the drop-in SpMV (also done
- * automatically when the content hash changes). */
+ * automatically when the arrays' bytes change). */
 void xp_legacy_reset(void);
 
 /* ===================================================================== *
