@@ -129,13 +129,21 @@ def spmv_roofline(m, a, reps: int = 200) -> dict:
 
 def stream_reference(X) -> dict:
     """The project-written streaming kernels over 1 GiB (beyond the MALL):
-    the HBM rate a plain read / copy reaches on this GPU, for scale."""
+    the best HBM rate a read / copy reaches on this GPU, for scale -- plain
+    16-byte loads and non-temporal 16-byte loads four in flight per thread,
+    over 1024 / 4096 / 16384 workgroups."""
     nb = 1 << 30
-    r = X.stream_time(nb, 0, 2048)
-    c = X.stream_time(nb, 1, 2048)
+    grids = (1024, 4096, 16384)
+    reads = {(mode, g): X.stream_time(nb, mode, g) for mode in (0, 2) for g in grids}
+    copies = {g: X.stream_time(nb, 1, g) for g in grids}
+    (mode, g), r = min(reads.items(), key=lambda kv: kv[1])
+    gc, c = min(copies.items(), key=lambda kv: kv[1])
+    plain = min(v for (m, _), v in reads.items() if m == 0)
     return {"read_gbs": round(nb / (r * 1e-6) / 1e9, 1), "copy_gbs": round(2 * nb / (c * 1e-6) / 1e9, 1),
             "read_frac": round(nb / (r * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "bytes": nb,
-            "kernels": "k_stream_read / k_stream_copy, 16-byte loads, 2048 workgroups"}
+            "plain_read_gbs": round(nb / (plain * 1e-6) / 1e9, 1),
+            "kernels": f"best read: {'k_stream_read_nt (non-temporal, 4 loads in flight)' if mode == 2 else 'k_stream_read'} "
+                       f"at {g} workgroups; copy: k_stream_copy at {gc} workgroups; 16-byte loads"}
 
 
 def spmv_beyond_mall(X, mesh: int = 2400) -> dict:

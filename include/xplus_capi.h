@@ -215,7 +215,8 @@ int xp_to_solve_order(xp_matrix* m, const double* nat, float* solve, void* strea
 int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* stream);
 /* Host-pointer convenience: y = A x in FP64 on the GPU for a CSR given once. */
 /* The achievable streaming rate of the GPU, for roofline figures: a
- * project-written kernel reads (mode 0) or copies (mode 1) `bytes` with
+ * project-written kernel reads (mode 0), copies (mode 1) or reads with
+ * non-temporal loads, four in flight per thread (mode 2), `bytes` with
  * 16-byte loads in `grid` workgroups; average microseconds per launch over
  * `reps` launches, each between its own events.  No reference counterpart. */
 int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, double* avg_us);

@@ -997,6 +997,33 @@ __global__ __launch_bounds__(kBlock, 8) void k_stream_read(const uint4* __restri
     if (acc == 0x9E3779B9u) sink[lb] = acc;  // data-dependent, almost never taken
 }
 
+// The same read with non-temporal 16-byte loads, four in flight per thread
+// (on this GPU the fastest way to stream a buffer once: 6.7-6.8 TB/s
+// against 5.8-6.2 for the plain loads).
+typedef unsigned stream4_t __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock, 8) void k_stream_read_nt(const uint4* __restrict__ src, size_t n16,
+                                                           unsigned* __restrict__ sink) {
+    const int lb = logical_block();
+    const size_t per = (n16 + gridDim.x - 1) / gridDim.x;
+    const size_t i0 = static_cast<size_t>(lb) * per, i1 = min(n16, i0 + per);
+    const stream4_t* p = reinterpret_cast<const stream4_t*>(src);
+    unsigned acc = 0;
+    size_t i = i0 + threadIdx.x;
+    for (; i + 3 * kBlock < i1; i += 4 * kBlock) {
+        const stream4_t a = __builtin_nontemporal_load(p + i);
+        const stream4_t b = __builtin_nontemporal_load(p + i + kBlock);
+        const stream4_t c = __builtin_nontemporal_load(p + i + 2 * kBlock);
+        const stream4_t d = __builtin_nontemporal_load(p + i + 3 * kBlock);
+        const stream4_t t = a ^ b ^ c ^ d;
+        acc ^= t.x ^ t.y ^ t.z ^ t.w;
+    }
+    for (; i < i1; i += kBlock) {
+        const stream4_t t = __builtin_nontemporal_load(p + i);
+        acc ^= t.x ^ t.y ^ t.z ^ t.w;
+    }
+    if (acc == 0x9E3779B9u) sink[lb] = acc;  // data-dependent, almost never taken
+}
+
 __global__ __launch_bounds__(kBlock, 8) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                         size_t n16) {
     const int lb = logical_block();
@@ -1793,9 +1820,12 @@ void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_n
 }
 
 void launch_stream(int grid, const void* src, void* dst, size_t bytes, unsigned* sink, hipStream_t s,
-                   hipEvent_t e0, hipEvent_t e1) {
+                   hipEvent_t e0, hipEvent_t e1, bool nontemporal) {
     const size_t n16 = bytes / 16;
-    if (dst == nullptr)
+    if (dst == nullptr && nontemporal)
+        hipExtLaunchKernelGGL(k_stream_read_nt, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0,
+                              static_cast<const uint4*>(src), n16, sink);
+    else if (dst == nullptr)
         hipExtLaunchKernelGGL(k_stream_read, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0,
                               static_cast<const uint4*>(src), n16, sink);
     else

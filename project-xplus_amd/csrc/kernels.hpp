@@ -77,7 +77,8 @@ void launch_pcg_flush_x_acc(const Launch& L, int nsys, SysState* st, const doubl
 // Streaming reference: read `bytes` of src once (dst null) or copy them to
 // dst, grid workgroups, between events e0 / e1 on stream s.
 void launch_stream(int grid, const void* src, void* dst, size_t bytes, unsigned* sink, hipStream_t s,
-                   hipEvent_t e0, hipEvent_t e1);
+                   hipEvent_t e0, hipEvent_t e1,
+                   bool nontemporal = false);
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y);
 // FP64 SpMV with exact FP64 values (val64) or hi+lo pairs (val + val_lo).
 void launch_spmv_f64(const Launch& L, const SellView& A, bool use_lo, const double* x, double* y);

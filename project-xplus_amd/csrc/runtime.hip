@@ -1479,7 +1479,7 @@ int xp_dot_f64_device(const double* a, const double* b, double* out, int n, void
 // (above the 256 MB MALL, every launch streams from HBM).
 int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, double* avg_us) {
     return guarded([&] {
-        if (avg_us == nullptr || reps <= 0 || bytes < 16 || (mode != 0 && mode != 1) || grid <= 0)
+        if (avg_us == nullptr || reps <= 0 || bytes < 16 || mode < 0 || mode > 2 || grid <= 0)
             throw std::runtime_error("bad argument");
         const hipStream_t s = device().stream;
         const size_t b = static_cast<size_t>(bytes) / 16 * 16;
@@ -1490,7 +1490,7 @@ int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, doubl
         for (auto& e : ev) HIPX(hipEventCreate(&e));
         for (int k = 0; k < reps + 3; ++k)
             xp::launch_stream(grid, src.p, mode == 1 ? dst.p : nullptr, b, sink.p, s, ev[2 * static_cast<size_t>(k)],
-                              ev[2 * static_cast<size_t>(k) + 1]);
+                              ev[2 * static_cast<size_t>(k) + 1], mode == 2);
         HIPX(hipStreamSynchronize(s));
         double total = 0.0;
         for (int k = 3; k < reps + 3; ++k) {
