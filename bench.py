@@ -485,9 +485,18 @@ def workload_c4(X, args) -> int:
             its = [s.status(k)["iterations"] for k in range(len(systems))]
             sts = [s.status(k)["status"] for k in range(len(systems))]
             key = name if not full else name + "_every_workgroup"
+            # algorithmic bytes of every system's iterations (the kernels'
+            # per-iteration formulas, FP32: 8 nnz + 56 n; accurate: 12 nnz +
+            # 94 n), over the whole batch's time
+            per_nnz, per_n = (8, 56) if name == "fp32" else (12, 94)
+            moved = sum(it * (per_nnz * a.nnz + per_n * a.n) for it, a in zip(its, systems))
             res[key] = {"ms": round(ms, 2), "iterations_min": min(its), "iterations_max": max(its),
+                        "iterations_sum": int(sum(its)),
                         "converged": int(sum(1 for v in sts if v == X.CONVERGED)),
-                        "max_true_rel_residual": float(np.max(rel))}
+                        "max_true_rel_residual": float(np.max(rel)),
+                        "alg_bytes": int(moved), "achieved_gbs": round(moved / (ms * 1e-3) / 1e9, 1),
+                        "frac": round(moved / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "alg_bytes_formula": f"sum over systems of iterations x ({per_nnz} nnz + {per_n} n)"}
             log(f"c4 {key}: {res[key]}")
             s.close()
     print(json.dumps({"workload": "256 batched 3D 7-point systems to relative residual 1e-6", "unit": "ms",
