@@ -523,7 +523,7 @@ def workload_jacobi(X, args) -> int:
     s.set_rhs(b)
     sweep = {}
     if args.grid_sweep:
-        for g in (256, 512, 1024, 2048, 4096):
+        for g in SPMV_GRIDS:
             m.set_grid(g)
             sweep[g] = round(s.kernel_time(100), 3)
             log(f"jacobi grid {g}: {sweep[g]} us")

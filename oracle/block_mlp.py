@@ -34,6 +34,8 @@ forms on hand-built blocks.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 N_IN, N_H, N_OUT = 48, 64, 48
@@ -43,6 +45,28 @@ N_PARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 EPS_M = 1e-3
 LAMBDA = 0.5
 TRIL = [(r, c) for r in range(8) for c in range(r + 1)]
+
+
+def _dot(a, b):
+    """a . b in an order that does not depend on the CPU's vector width or
+    BLAS: the products, then column sums of a 1024-wide reshape added row
+    after row (elementwise adds), then the 1024 partial sums added exactly
+    (math.fsum).  The PCG iteration counts below are sensitive to last-bit
+    differences in these sums; this keeps them the same on every machine."""
+    v = np.asarray(a, dtype=np.float64) * np.asarray(b, dtype=np.float64)
+    pad = (-v.shape[0]) % 1024
+    if pad:
+        v = np.concatenate([v, np.zeros(pad)])
+    return math.fsum(v.reshape(-1, 1024).sum(axis=0).tolist())
+
+
+def _block_apply(blocks, rp):
+    """z_bi = sum_j M_bij rp_bj, the eight terms added in order j = 0..7
+    (elementwise operations only, so no CPU-dependent summation order)."""
+    z = blocks[:, :, 0] * rp[:, None, 0]
+    for j in range(1, 8):
+        z = z + blocks[:, :, j] * rp[:, None, j]
+    return z
 
 
 def unpack(flat):
@@ -249,23 +273,23 @@ def block_pcg(A, b, blocks, tau, max_iters=100000, groups=None):
     def apply(r):
         rp = np.append(r, 0.0)[idx]
         z = np.zeros(n + 1)
-        z[idx] = np.einsum("bij,bj->bi", blocks, rp)
+        z[idx] = _block_apply(blocks, rp)
         return z[:n]
 
     x = np.zeros(n)
     r = b.copy()
     z = apply(r)
     p = z.copy()
-    rz, rr = r @ z, r @ r
+    rz, rr = _dot(r, z), _dot(r, r)
     trace = []
     it = 0
     while it < max_iters and rr > tau:
         ap = A @ p
-        alpha = rz / (p @ ap)
+        alpha = rz / _dot(p, ap)
         x += alpha * p
         r -= alpha * ap
         z = apply(r)
-        rzn, rr = r @ z, r @ r
+        rzn, rr = _dot(r, z), _dot(r, r)
         trace.append(rr)
         it += 1
         if rr <= tau:
@@ -296,7 +320,7 @@ def block_pcg_accurate(A, b, blocks, tau, max_iters=200000, groups=None, round32
     def apply(r):
         rp = np.append(r, 0.0)[idx]
         z = np.zeros(n + 1)
-        z[idx] = np.einsum("bij,bj->bi", blocks, rp)
+        z[idx] = _block_apply(blocks, rp)
         return z[:n]
 
     f = (lambda v: v.astype(np.float32).astype(np.float64)) if round32 else (lambda v: v)
@@ -307,13 +331,13 @@ def block_pcg_accurate(A, b, blocks, tau, max_iters=200000, groups=None, round32
         z = f(apply(r))
         p = z.copy()
         ap = A @ z
-        rz, rr = r @ z, r @ r
+        rz, rr = _dot(r, z), _dot(r, r)
         while rr > tau and it < max_iters:
-            alpha = rz / (p @ ap)
+            alpha = rz / _dot(p, ap)
             x += alpha * p
             r = f(r - alpha * ap)
             z = f(apply(r))
-            rzn, rr = r @ z, r @ r
+            rzn, rr = _dot(r, z), _dot(r, r)
             it += 1
             if rr <= tau:
                 break
@@ -322,7 +346,7 @@ def block_pcg_accurate(A, b, blocks, tau, max_iters=200000, groups=None, round32
             p = z + beta * p
             ap = A @ z + beta * ap
         rt = b - A @ x
-        if rt @ rt <= tau or it >= max_iters:
+        if _dot(rt, rt) <= tau or it >= max_iters:
             return x, it, restarts
         r = f(rt)
         restarts += 1

@@ -78,10 +78,37 @@ __device__ __forceinline__ int2 sys_range(const Launch& L, int sys) {
     return L.sys_blocks[sys];
 }
 
+// v of the lane that DPP control CTRL names (both 32-bit halves moved).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = static_cast<unsigned>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<unsigned>(b)), CTRL, 0xF, 0xF, false));
+    const unsigned hi = static_cast<unsigned>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(static_cast<unsigned>(b >> 32)), CTRL, 0xF, 0xF, false));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b)), lane));
+    const unsigned hi =
+        static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned>(b >> 32)), lane));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// Sum of v over the 64 lanes, in every lane; all lanes must be active.
+// Inside each 16-lane row by DPP moves (pairs, quads, the half-row mirror,
+// the row mirror: each step adds the same two partial sums in every lane,
+// so all lanes of a row hold the same bits), then the four row sums read
+// with readlane and added as (row0 + row1) + (row2 + row3).  No LDS
+// traffic and no waits on it, unlike a ds_bpermute butterfly.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64<0xB1>(v);   // quad_perm(1, 0, 3, 2)
+    v += dpp_f64<0x4E>(v);   // quad_perm(2, 3, 0, 1)
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // Sums NV values over the workgroup (kBlock threads).  Every thread gets the
@@ -1414,7 +1441,8 @@ template <bool IDX16, int FW, int U>
 __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A, const JacState* Sc, JacState* Sn,
                                                          const float* __restrict__ b,
                                                          const float* __restrict__ dinv,
-                                                         const float* __restrict__ xo, float* __restrict__ xn) {
+                                                         const float* __restrict__ xo, float* __restrict__ xn,
+                                                         int slot_read, int slot_write) {
     __shared__ JacState s_sh;
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
@@ -1440,7 +1468,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A,
             xr[0] = xo[row];
         }
     }
-    const double prev = max_partials(L, w.sys, 0);  // ends with a barrier
+    const double prev = max_partials(L, w.sys, slot_read);  // ends with a barrier
     if (!s_sh.active) {
         if (leader_block && threadIdx.x == 0) Sn[w.sys] = s_sh;
         return;
@@ -1503,7 +1531,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_jacobi_iter(Launch L, SellView A,
     if (threadIdx.x == 0) {
         double t = red[0];
         for (int k = 1; k < kWaves; ++k) t = nan_max(t, red[k]);
-        put_partial(L, 0, lb, t);
+        put_partial(L, slot_write, lb, t);
     }
 }
 
@@ -2034,13 +2062,15 @@ void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipSt
 
 
 void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
-                        const float* b, const float* dinv, const float* x_old, float* x_new) {
+                        const float* b, const float* dinv, const float* x_old, float* x_new, int slot_read,
+                        int slot_write) {
     // Slices up to 5 columns (2D 5-point stencils) go two at a time (16-bit
     // columns; one with 32-bit columns); up to
     // 8 or 10 columns one at a time with all loads ahead (more would not
     // fit the 64 registers of full occupancy); wider ones through the loop.
 #define XP_JACOBI_LAUNCH(I16, FW, U) \
-    XP_LAUNCH(L, (k_jacobi_iter<I16, FW, U>), dim3(L.grid), L, A, st_cur, st_next, b, dinv, x_old, x_new)
+    XP_LAUNCH(L, (k_jacobi_iter<I16, FW, U>), dim3(L.grid), L, A, st_cur, st_next, b, dinv, x_old, x_new, \
+              slot_read, slot_write)
 #define XP_JACOBI_WIDTHS(I16)                                  \
     do {                                                       \
         if (A.max_width <= 5) XP_JACOBI_LAUNCH(I16, 5, (I16 ? 2 : 1)); \

@@ -137,9 +137,14 @@ void launch_mask_padding_blocks(int n_pad, const int* perm, float* blocks, hipSt
 // x_new = D^-1 (b - R x) on the solve-order vectors.  Launch j of a run
 // passes x_old = buffer j % 2 and x_new = buffer (j + 1) % 2; update j runs
 // in launch j, so a system stopped after k updates holds its x in buffer
-// k % 2.  Partial slot 0 of L holds each workgroup's max |x_new - x_old|.
+// k % 2.  Each workgroup leaves its max |x_new - x_old| in partial slot
+// slot_write and every workgroup reads the previous update's from slot_read
+// (launch j: write j % 2, read (j + 1) % 2): with a single slot, a
+// workgroup that finished early could overwrite a maximum that a workgroup
+// starting late had still to read for the stop test.
 void launch_jacobi_iter(const Launch& L, const SellView& A, const JacState* st_cur, JacState* st_next,
-                        const float* b, const float* dinv, const float* x_old, float* x_new);
+                        const float* b, const float* dinv, const float* x_old, float* x_new, int slot_read,
+                        int slot_write);
 
 // The fixed-count form: one update with no stop test (partial maxima of
 // |x_new - x_old| in slot 0), and the state after `iters` such updates.

@@ -1742,7 +1742,7 @@ struct xp_jacobi {
     hipStream_t stream = nullptr;
     DevBuf<float> xa, xb, b, dinv, x0s;
     DevBuf<xp::JacState> st;
-    DevBuf<double> partials;               // partial maxima, [grid]
+    DevBuf<double> partials;               // partial maxima, two slots of [grid]
     std::vector<xp::JacState> st_init;
     xp::JacState* st_host = nullptr;       // pinned readback
     hipGraph_t graph = nullptr;
@@ -1768,7 +1768,7 @@ struct xp_jacobi {
     // The matrix's work map with this solver's own partials buffer.
     xp::Launch launch() {
         if (partials_version != m->layout_version) {
-            partials.alloc(static_cast<size_t>(m->grid));
+            partials.alloc(2 * static_cast<size_t>(m->grid));
             HIPX(hipMemsetAsync(partials.p, 0, partials.n * sizeof(double), stream));
             partials_version = m->layout_version;
         }
@@ -1778,15 +1778,17 @@ struct xp_jacobi {
     }
 
     // Launch k reads state slot k % 2 and x buffer k % 2, writes slot and
-    // buffer (k + 1) % 2; graphs hold an even number of launches, so slot 0
-    // and buffer xa are current between graph launches.  Updates run in
+    // buffer (k + 1) % 2; the partial maxima go the other way (written to
+    // slot k % 2, the previous launch's read from (k + 1) % 2).  Graphs hold
+    // an even number of launches, so slot 0 and buffer xa are current
+    // between graph launches.  Updates run in
     // consecutive launches from the first, so a system that stopped after
     // j updates holds its x in buffer j % 2.
     void enqueue(int k, const xp::Launch& L) {
         const size_t ns = static_cast<size_t>(m->nsys);
         const bool odd = (k % 2) != 0;
         xp::launch_jacobi_iter(L, m->view(), st.p + (k % 2) * ns, st.p + ((k + 1) % 2) * ns, b.p, dinv.p,
-                               odd ? xb.p : xa.p, odd ? xa.p : xb.p);
+                               odd ? xb.p : xa.p, odd ? xa.p : xb.p, (k + 1) % 2, k % 2);
     }
 
     // The fixed-count form: launch k of a run, no stop test.

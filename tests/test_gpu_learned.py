@@ -266,11 +266,16 @@ def test_useless_blocks_fall_back_to_jacobi(gpu):
 def test_trained_network_reproduces_exact_block_inverses(gpu):
     """What the network is for on these aggregates, measured: trained with
     the benchmark's loss (no off-block weight: its minimiser is the exact
-    block inverse) on other matrices of the family, its blocks on a held-out
-    matrix need within 3% of the iterations of exact block inverses (the
-    benchmark matrix: 6,129-6,238 against 6,179), and far fewer than Jacobi.
-    It does not beat exact 8x8 block inverses: the gain over Jacobi is the
-    block structure."""
+    block inverse) on other matrices of the family, the operator it makes on
+    a held-out matrix (read back from the device: S C C^T S of the stored
+    FP16 factors) needs within 3% of the iterations of the exact block
+    inverses' stored operator in the FP64 oracle block PCG, on the device's
+    aggregates -- a count that does not move with the order of the GPU's
+    sums (the accurate GPU solves of the same 140^2 system took 900 / 899
+    iterations with one reduction order and 967 / 902 with another) -- and
+    the GPU solve with the network's blocks takes far fewer iterations than
+    Jacobi.  It does not beat exact 8x8 block inverses: the gain over Jacobi
+    is the block structure."""
     train = [X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265700 + i) for i in range(3)]
     test = X.gen_fem2d(140, 1.0, 20, 1e3, 0.3, 20265710)
     sets = X.BlockSet(train[0], blocking=3, lam=0.0)
@@ -279,7 +284,10 @@ def test_trained_network_reproduces_exact_block_inverses(gpu):
     net = X.Mlp(seed=20265700)
     net.train(sets, 1000, batch=4096, lr=2e-3, seed=1)
     m = X.Matrix(test, block_rows=8, keep_lo=True, blocking=3)
-    its = {}
+    groups = [[int(r) for r in row if r >= 0] for row in m.block_rows()]
+    A = test.to_scipy()
+    tau = (1e-6 * np.linalg.norm(test.b)) ** 2
+    its, oracle_its = {}, {}
     for name in ("jacobi", "exact", "learned"):
         s = X.Pcg(m, rel_tol=1e-6, accurate=True)
         if name == "exact":
@@ -290,7 +298,9 @@ def test_trained_network_reproduces_exact_block_inverses(gpu):
         rel, _, it = s.run_reliable(1e-6, 0.0)
         assert rel[0] <= 1e-6
         its[name] = it
+        if name != "jacobi":
+            _, oracle_its[name], _ = B.block_pcg(A, test.b, s.block_operator(), tau, groups=groups)
         s.close()
-    print("held-out iterations:", its, flush=True)
-    assert abs(its["learned"] - its["exact"]) <= 0.03 * its["exact"]
+    print("held-out iterations, GPU:", its, "FP64 oracle with the stored operators:", oracle_its, flush=True)
+    assert abs(oracle_its["learned"] - oracle_its["exact"]) <= 0.03 * oracle_its["exact"]
     assert its["learned"] < 0.75 * its["jacobi"]
