@@ -139,15 +139,20 @@ def stream_reference(X) -> dict:
     nb = 1 << 30
     grids = (1024, 4096, 16384)
     reads = {(mode, g): X.stream_time(nb, mode, g) for mode in (0, 2) for g in grids}
-    copies = {g: X.stream_time(nb, 1, g) for g in grids}
+    copies = {(mode, g): X.stream_time(nb, mode, g) for mode in (1, 3) for g in grids}
     (mode, g), r = min(reads.items(), key=lambda kv: kv[1])
-    gc, c = min(copies.items(), key=lambda kv: kv[1])
+    (mc, gc), c = min(copies.items(), key=lambda kv: kv[1])
     plain = min(v for (m, _), v in reads.items() if m == 0)
+    # a launch that moves (almost) nothing: 16 KiB read by 256 / 512 workgroups
+    # of 1024 threads, the fixed cost every iteration kernel pays
+    floor = {g: round(X.stream_time(1 << 14, 0, g, 200), 3) for g in (256, 512)}
     return {"read_gbs": round(nb / (r * 1e-6) / 1e9, 1), "copy_gbs": round(2 * nb / (c * 1e-6) / 1e9, 1),
             "read_frac": round(nb / (r * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "bytes": nb,
-            "plain_read_gbs": round(nb / (plain * 1e-6) / 1e9, 1),
+            "plain_read_gbs": round(nb / (plain * 1e-6) / 1e9, 1), "launch_floor_us": floor,
             "kernels": f"best read: {'k_stream_read_nt (non-temporal, 4 loads in flight)' if mode == 2 else 'k_stream_read'} "
-                       f"at {g} workgroups; copy: k_stream_copy at {gc} workgroups; 16-byte loads"}
+                       f"at {g} workgroups; copy: "
+                       f"{'k_stream_copy_nt (non-temporal stores)' if mc == 3 else 'k_stream_copy'} at {gc} workgroups; "
+                       f"16-byte loads"}
 
 
 def spmv_beyond_mall(X, mesh: int = 2400) -> dict:
@@ -218,7 +223,8 @@ def bitmap_block_spmv(X, a) -> dict:
 def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
     """The accurate iteration's two kernels timed with HIP events between
     launches on the solver stream; the dominant one is k_pcg_spmv_fused_acc."""
-    us = s.kernel_times(iters)
+    even, odd = s.kernel_times_split(iters)
+    us = (even + odd) / 2.0
     b_fused = 12 * a.nnz + 40 * a.n
     b_upd = (54 if block else 40) * a.n
     r = roof("k_pcg_spmv_fused_acc", b_fused, us[0],
@@ -228,6 +234,9 @@ def iteration_roofline(s, a, block: bool, iters: int = 200) -> dict:
                               + ("38 / 70" if block else "24 / 56") + " n: r r/w, A p FP64, "
                               + ("the 36 FP32 entries of an SPD 8x8 block, 18 B per row" if block else "1/diag")
                               + ", z; the odd one adds x r/w FP64 and two FP64 directions)")
+    for name, t, per_n in (("even", even[1], 38 if block else 24), ("odd", odd[1], 70 if block else 56)):
+        q = roof("k_pcg_update_xrz_acc", per_n * a.n, t, f"{per_n} n")
+        r["update_kernel"][name] = {k: q[k] for k in ("avg_us", "alg_bytes", "achieved", "frac")}
     prof = profiled_traffic()
     for rr in (r, r["update_kernel"]):
         if rr["kernel"] in prof:

@@ -94,7 +94,9 @@ typedef struct {
                          1 = greedy strength aggregates (xp_strength_blocking);
                          2 = strength-ordered aggregates (xp_aggregate_blocking rule 2);
                          3 = parallel strength-ordered aggregates, computed on the
-                             device (implies device_build) */
+                             device (implies device_build);
+                         4 = the same with seeds that precede only their
+                             distance-1 neighbours (fewer rounds on dense graphs) */
     int device_build; /* 1: set the matrix up on the device (xp_matrix_create
                          uploads the CSR; one system only) */
 } xp_matrix_opts;
@@ -216,7 +218,9 @@ int xp_from_solve_order(xp_matrix* m, const float* solve, double* nat, void* str
 /* Host-pointer convenience: y = A x in FP64 on the GPU for a CSR given once. */
 /* The achievable streaming rate of the GPU, for roofline figures: a
  * project-written kernel reads (mode 0), copies (mode 1) or reads with
- * non-temporal loads, four in flight per thread (mode 2), `bytes` with
+ * non-temporal loads, four in flight per thread (mode 2), copies with
+ * non-temporal stores (mode 3) or non-temporal loads and stores (mode 4),
+ * four in flight per thread, `bytes` with
  * 16-byte loads in `grid` workgroups; average microseconds per launch over
  * `reps` launches, each between its own events.  No reference counterpart. */
 int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, double* avg_us);
@@ -434,6 +438,10 @@ int xp_aggregate_blocking(int n, const int* row_ptr, const int* col_idx, const d
  * kernel (fused or not), us[1] update_xrz, us[2] update_p (three-kernel
  * form only).  Leaves the solver's vectors as after those iterations. */
 int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels);
+/* The same measurement split by iteration parity: us6[0..2] the kernels'
+ * averages over the even iterations, us6[3..5] over the odd ones (the
+ * accurate iteration's update kernel adds x only in odd iterations). */
+int xp_pcg_kernel_times_split(xp_pcg* s, int iters, double* us6, int* nkernels);
 /* Device time of the last xp_pcg_run* (ms, HIP events) and launches made. */
 int xp_pcg_last_timing(xp_pcg* s, double* ms, int* graph_launches);
 
@@ -487,6 +495,40 @@ int xp_mlp_train(xp_mlp* net, const xp_blockset* set, int steps, int batch, floa
  * inference, blocks written in solve order; padding rows stay identity.
  * The matrix must be built with block_rows = 8. */
 int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net);
+
+/* Multilevel preconditioner over the block aggregates (no reference
+ * counterpart: it extends the block-Jacobi preconditioner the reference's
+ * Jacobi-PCG loop would take in place of its inverse diagonal,
+ * src/CgSolverGolden.hpp:33-40).  The installed 8x8 blocks (exact or
+ * learned) become the smoother of level 0; level l + 1 is the Galerkin
+ * product P^T A P over the 8-row aggregates of level l (P piecewise
+ * constant), set up on the device with its own parallel strength
+ * aggregates and exact block factors, down to a dense coarsest level of at
+ * most coarse_max rows (inverted once); coarse_blocking 3 aggregates the
+ * coarse levels like level 0, 4 with distance-1 seeds (far fewer rounds on
+ * their denser graphs).  Each PCG step then applies one
+ * symmetric V-cycle: z = omega M r, coarse correction scaled by gamma, one
+ * post-smoothing step; FP32 vectors inside the cycle.  Needs the accurate
+ * iteration on one system set up on the device with block_rows = 8 and
+ * blocking 3, and a block preconditioner already installed; coarse_max <= 0
+ * removes the hierarchy.  A fallback to Jacobi turns it off with the
+ * blocks. */
+int xp_pcg_set_multilevel(xp_pcg* s, int coarse_max, double omega, double gamma, int coarse_blocking);
+/* Levels of the hierarchy (*nlev, 0 when none): rows and stored entries of
+ * the first `cap` levels (level 0 is the solver's matrix, the last the dense
+ * coarsest one) and the set-up time in ms. */
+int xp_pcg_multilevel_info(const xp_pcg* s, int cap, int* nlev, int* n, long long* nnz, double* setup_ms);
+/* The set-up's time per level in ms, three entries per level below the
+ * coarsest: Galerkin product, the coarse matrix's device set-up, its block
+ * factors (the dense level: product, inverse, 0); *count entries. */
+int xp_pcg_multilevel_phases(const xp_pcg* s, int cap, double* ms, int* count);
+/* Level `level` >= 1 as a natural-order CSR (rows = the blocks of level
+ * level - 1) and those blocks (rows x 8 natural rows of level level - 1, -1
+ * padding); null outputs are skipped (sizes from xp_pcg_multilevel_info). */
+int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_idx, double* values, int* slots);
+/* z = M r followed by one V-cycle, for device vectors of level 0 (solve
+ * order, n_pad entries). */
+int xp_pcg_multilevel_apply(xp_pcg* s, const float* r, float* z);
 /* The device-computed features of a block_rows = 8 matrix, in natural block
  * order (xp_matrix_num_blocks of them): feat nblocks x 48, scale nblocks x 8
  * (the same quantities as xp_blockset_copy's feat and scale). */

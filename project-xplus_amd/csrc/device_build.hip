@@ -233,14 +233,17 @@ __global__ void k_seed_best1(int n, const int* __restrict__ rp, const int* __res
     m1[i] = best;
 }
 
+// Seeds: the free rows that precede every free row within distance 2
+// (dist2), or within distance 1 (the variant for dense coarse graphs, where
+// distance-2 neighbourhoods admit few seeds per round).
 __global__ void k_seed_pick(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                             const double* __restrict__ t, const uint8_t* __restrict__ elig,
                             const int* __restrict__ m1, int* __restrict__ owner, int* __restrict__ size,
-                            int* __restrict__ members) {
+                            int* __restrict__ members, int dist2) {
     const int i = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
     if (i >= n || !elig[i] || owner[i] >= 0) return;
     int best = m1[i];
-    for (int o = rp[i]; o < rp[i + 1]; ++o) {
+    for (int o = rp[i]; dist2 && o < rp[i + 1]; ++o) {
         const int c = ci[o];
         if (c == i) continue;
         const int mc = m1[c];
@@ -768,7 +771,8 @@ void device_rows(DevCsr& a, DevRows& rows, hipStream_t s) {
     HIPX(hipGetLastError());
 }
 
-int device_aggregate(const DevCsr& a, const DevRows& rows, DevBuf<int>& slots, AggStats* st, hipStream_t s) {
+int device_aggregate(const DevCsr& a, const DevRows& rows, DevBuf<int>& slots, AggStats* st, hipStream_t s,
+                     int seed_dist) {
     const int n = a.n;
     const size_t nn = static_cast<size_t>(n);
     int* owner = take<int>(kWOwner, nn);
@@ -789,7 +793,7 @@ int device_aggregate(const DevCsr& a, const DevRows& rows, DevBuf<int>& slots, A
         for (;;) {
             hipLaunchKernelGGL(k_seed_best1, g, b, 0, s, n, a.row_ptr, a.col_idx, rows.total, elig, owner, m1);
             hipLaunchKernelGGL(k_seed_pick, g, b, 0, s, n, a.row_ptr, a.col_idx, rows.total, elig, m1, owner,
-                               size, members);
+                               size, members, seed_dist == 1 ? 0 : 1);
             fill(taken, kGrowSteps, 0, s);
             for (int step = 0; step < kGrowSteps; ++step) {
                 const int* prev = step > 0 ? taken + step - 1 : nullptr;

@@ -76,7 +76,10 @@ struct AggStats {
 
 // Rule-3 aggregation: slots[8u + j] = row of slot j of block u (-1 for
 // padding, real rows first, ascending).  Returns the block count.
-int device_aggregate(const DevCsr& a, const DevRows& rows, DevBuf<int>& slots, AggStats* st, hipStream_t s);
+// seed_dist 1: seeds need only precede their distance-1 neighbours (rule 4,
+// for the dense coarse levels of the multilevel hierarchy).
+int device_aggregate(const DevCsr& a, const DevRows& rows, DevBuf<int>& slots, AggStats* st, hipStream_t s,
+                     int seed_dist = 2);
 
 struct DevSell {
     int n = 0, n_pad = 0, nslices = 0, sigma = 0, idx16 = 0, max_width = 0, units = 0;

@@ -1059,6 +1059,31 @@ __global__ __launch_bounds__(kBlock, 8) void k_stream_copy(const uint4* __restri
     for (size_t i = i0 + threadIdx.x; i < i1; i += kBlock) dst[i] = src[i];
 }
 
+// The copy with non-temporal stores (NTL: and non-temporal loads), four
+// 16-byte moves in flight per thread: the written lines stream out instead
+// of waiting as dirty lines in the L2 for the kernel's end.
+template <bool NTL>
+__global__ __launch_bounds__(kBlock, 8) void k_stream_copy_nt(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                           size_t n16) {
+    const int lb = logical_block();
+    const size_t per = (n16 + gridDim.x - 1) / gridDim.x;
+    const size_t i0 = static_cast<size_t>(lb) * per, i1 = min(n16, i0 + per);
+    const stream4_t* p = reinterpret_cast<const stream4_t*>(src);
+    stream4_t* q = reinterpret_cast<stream4_t*>(dst);
+    size_t i = i0 + threadIdx.x;
+    for (; i + 3 * kBlock < i1; i += 4 * kBlock) {
+        stream4_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = NTL ? __builtin_nontemporal_load(p + i + k * kBlock) : p[i + k * kBlock];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(v[k], q + i + k * kBlock);
+    }
+    for (; i < i1; i += kBlock) {
+        const stream4_t v = NTL ? __builtin_nontemporal_load(p + i) : p[i];
+        __builtin_nontemporal_store(v, q + i);
+    }
+}
+
 template <bool IDX16, int VMODE>
 __global__ __launch_bounds__(kBlock, 8) void k_spmv_f64(Launch L, SellView A, const double* __restrict__ x,
                                                      double* __restrict__ y) {
@@ -1705,6 +1730,375 @@ __global__ __launch_bounds__(kBlock, 1) void k_op_finish(Launch L, int slot0, in
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Multilevel (aggregation) preconditioner.  Every level is a SELL matrix
+// whose 8-row blocks are the strength aggregates of that level; block q of
+// level l is row down[q] of level l + 1 (the next level's solve order, or the
+// natural order of the dense coarsest level).  P is piecewise constant on
+// the blocks (real rows only), the coarse operator is P^T A P, and one
+// V-cycle is
+//   z = omega M r;  r' = P^T (r - A z);  z += gamma P V(r');  z += omega M (r - A z)
+// with M the level's block factors and the coarsest level solved by a dense
+// inverse.  The cycle is symmetric (the same smoother before and after).
+// Vectors, values and sums are FP32 (a preconditioner needs no more); every
+// kernel returns at once when the system has stopped.
+
+// dst[down[q]] = sum over the real rows of block q of (r - c (A z))_row.
+template <bool IDX16, int V>
+__global__ __launch_bounds__(kBlock, 8) void k_mg_restrict(Launch L, SellView A, const SysState* st,
+                                                        const float* __restrict__ r, const float* __restrict__ z,
+                                                        float c, const int* __restrict__ perm,
+                                                        const int* __restrict__ down, float* __restrict__ dst) {
+    if (st != nullptr && !st[0].active) return;
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {  // wave-uniform: DPP sums
+        const int row = sl * kC + lane;
+        float zd;
+        const float az = sell_row_f32<IDX16, V>(A, sl, lane, z, zd);
+        float t = perm[row] >= 0 ? fmaf(-c, az, r[row]) : 0.0f;
+        t = dpp_add<0x141>(t);  // the block's 8 lanes: pairs (i, 7 - i),
+        t = dpp_add<0x4E>(t);   // then (i, i ^ 2),
+        t = dpp_add<0xB1>(t);   // then (i, i ^ 1)
+        if ((lane & 7) == 0) {
+            const int d = down[row >> 3];
+            if (d >= 0) dst[d] = t;
+        }
+    }
+}
+
+// z = omega M r (the smoother from a zero start).
+__global__ __launch_bounds__(kBlock, 8) void k_mg_smooth(Launch L, const SysState* st,
+                                                      const BlockPre* __restrict__ M, const float* __restrict__ r,
+                                                      float omega, float* __restrict__ z) {
+    if (st != nullptr && !st[0].active) return;
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
+        z[row] = omega * block_apply(M, row, r[row]);
+    }
+}
+
+// t = cz z + gamma e[down[q]] on the real rows of block q (padding rows 0).
+__global__ __launch_bounds__(kBlock, 8) void k_mg_prolong(Launch L, const SysState* st,
+                                                       const float* __restrict__ z, float cz,
+                                                       const float* __restrict__ e, float gamma,
+                                                       const int* __restrict__ perm, const int* __restrict__ down,
+                                                       float* __restrict__ t) {
+    if (st != nullptr && !st[0].active) return;
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock) {
+        const int d = down[row >> 3];
+        t[row] = (perm[row] >= 0 && d >= 0) ? fmaf(gamma, e[d], cz * z[row]) : 0.0f;
+    }
+}
+
+// zout = t + omega M (r - A t); RZ: the r^T zout partials of the PCG step
+// (the solver's slot kSlotRz: the cycle's z replaces the update kernel's).
+template <bool IDX16, int V, bool RZ>
+__device__ __forceinline__ void mg_post(const Launch& L, const SellView& A, const SysState* st,
+                                                    const BlockPre* __restrict__ M, const float* __restrict__ r,
+                                                    const float* __restrict__ t, float omega,
+                                                    float* __restrict__ zout) {
+    if (st != nullptr && !st[0].active) return;
+    const int lb = logical_block();
+    const WorkItem w = work_item(L, lb);
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    double v[1] = {0.0};
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
+        float td;
+        const float at = sell_row_f32<IDX16, V>(A, sl, lane, t, td);
+        const float rv = r[row];
+        const float zz = fmaf(omega, block_apply(M, row, rv - at), td);
+        zout[row] = zz;
+        if (RZ) v[0] += static_cast<double>(rv) * static_cast<double>(zz);
+    }
+    if (RZ) {
+        block_sum<1>(v);
+        if (threadIdx.x == 0) put_partial(L, kSlotRz, lb, v[0]);
+    }
+}
+
+// e = Cinv rc on the dense coarsest level: one wavefront per row.
+__global__ __launch_bounds__(kBlock, 8) void k_mg_coarse(const SysState* st, int nc, const float* __restrict__ cinv,
+                                                      const float* __restrict__ rc, float* __restrict__ ec) {
+    if (st != nullptr && !st[0].active) return;
+    const int row = static_cast<int>(blockIdx.x) * kWaves + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    if (row >= nc) return;  // wave-uniform
+    double s = 0.0;
+    for (int j = lane; j < nc; j += kC) s += static_cast<double>(cinv[static_cast<size_t>(row) * nc + j]) * rc[j];
+    s = wave_sum(s);
+    if (lane == 0) ec[row] = static_cast<float>(s);
+}
+
+template <bool IDX16, int V>
+__global__ __launch_bounds__(kBlock, 8) void k_mg_post_norz(Launch L, SellView A, const SysState* st,
+                                                         const BlockPre* __restrict__ M, const float* __restrict__ r,
+                                                         const float* __restrict__ t, float omega,
+                                                         float* __restrict__ zout) {
+    mg_post<IDX16, V, false>(L, A, st, M, r, t, omega, zout);
+}
+template <bool IDX16, int V>
+__global__ __launch_bounds__(kBlock, 8) void k_mg_post_rz(Launch L, SellView A, const SysState* st,
+                                                       const BlockPre* __restrict__ M, const float* __restrict__ r,
+                                                       const float* __restrict__ t, float omega,
+                                                       float* __restrict__ zout) {
+    mg_post<IDX16, V, true>(L, A, st, M, r, t, omega, zout);
+}
+
+// ---- set-up of the hierarchy ----
+// agg[natural row] = its block (unit) id.
+__global__ void k_unit_of_row(int units, const int* __restrict__ slots, int* __restrict__ agg) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= units * 8) return;
+    const int row = slots[i];
+    if (row >= 0) agg[row] = i >> 3;
+}
+
+// Candidate count of coarse row u: the entries of its fine rows.
+__global__ void k_gal_count(int units, const int* __restrict__ slots, const int* __restrict__ rp,
+                            int* __restrict__ cnt) {
+    const int u = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (u >= units) return;
+    int c = 0;
+    for (int k = 0; k < 8; ++k) {
+        const int row = slots[u * 8 + k];
+        if (row >= 0) c += rp[row + 1] - rp[row];
+    }
+    cnt[u] = c;
+}
+
+// Coarse row u of P^T A P: the entries (agg[col], value) of its fine rows,
+// equal coarse columns summed in the order they come, the distinct columns
+// sorted ascending and written from off[u]; their count in dcnt[u].  The
+// distinct list is kept in LDS (kGalCap per thread, column-major so the 64
+// threads of a workgroup touch 64 banks); a row with more distinct
+// columns sorts all its candidates in place in global memory instead
+// (insertion sort, stable: the same sums in the same order).
+constexpr int kGalThreads = 64;
+constexpr int kGalCap = 96;
+__global__ __launch_bounds__(kGalThreads) void k_gal_fill(int units, const int* __restrict__ slots,
+                                                          const int* __restrict__ rp, const int* __restrict__ ci,
+                                                          const double* __restrict__ va, const int* __restrict__ agg,
+                                                          const int* __restrict__ off, int* __restrict__ ccol,
+                                                          double* __restrict__ cval, int* __restrict__ dcnt) {
+    __shared__ int lcol[kGalCap * kGalThreads];
+    __shared__ double lval[kGalCap * kGalThreads];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int u = static_cast<int>(blockIdx.x) * kGalThreads + tid;
+    if (u >= units) return;
+    int* col = lcol + tid;
+    double* val = lval + tid;
+    int d = 0;
+    bool over = false;
+    for (int k = 0; k < 8 && !over; ++k) {
+        const int row = slots[u * 8 + k];
+        if (row < 0) continue;
+        for (int e = rp[row]; e < rp[row + 1]; ++e) {
+            const int c = agg[ci[e]];
+            int j = 0;
+            while (j < d && col[j * kGalThreads] != c) ++j;
+            if (j < d) {
+                val[j * kGalThreads] += va[e];
+            } else if (d < kGalCap) {
+                col[d * kGalThreads] = c;
+                val[d * kGalThreads] = va[e];
+                ++d;
+            } else {
+                over = true;
+                break;
+            }
+        }
+    }
+    const int o = off[u];
+    if (!over) {
+        for (int i = 1; i < d; ++i) {
+            const int c = col[i * kGalThreads];
+            const double v = val[i * kGalThreads];
+            int j = i;
+            while (j > 0 && col[(j - 1) * kGalThreads] > c) {
+                col[j * kGalThreads] = col[(j - 1) * kGalThreads];
+                val[j * kGalThreads] = val[(j - 1) * kGalThreads];
+                --j;
+            }
+            col[j * kGalThreads] = c;
+            val[j * kGalThreads] = v;
+        }
+        for (int j = 0; j < d; ++j) {
+            ccol[o + j] = col[j * kGalThreads];
+            cval[o + j] = val[j * kGalThreads];
+        }
+        dcnt[u] = d;
+        return;
+    }
+    int m = 0;
+    for (int k = 0; k < 8; ++k) {
+        const int row = slots[u * 8 + k];
+        if (row < 0) continue;
+        for (int e = rp[row]; e < rp[row + 1]; ++e) {
+            const int c = agg[ci[e]];
+            const double v = va[e];
+            int j = m;
+            while (j > 0 && ccol[o + j - 1] > c) {
+                ccol[o + j] = ccol[o + j - 1];
+                cval[o + j] = cval[o + j - 1];
+                --j;
+            }
+            ccol[o + j] = c;
+            cval[o + j] = v;
+            ++m;
+        }
+    }
+    int dd = 0;
+    for (int j = 0; j < m; ++j) {
+        if (dd > 0 && ccol[o + dd - 1] == ccol[o + j]) {
+            cval[o + dd - 1] += cval[o + j];
+        } else {
+            ccol[o + dd] = ccol[o + j];
+            cval[o + dd] = cval[o + j];
+            ++dd;
+        }
+    }
+    dcnt[u] = dd;
+}
+
+__global__ void k_gal_compact(int units, const int* __restrict__ off, const int* __restrict__ crp,
+                              const int* __restrict__ ccol, const double* __restrict__ cval, int* __restrict__ cci,
+                              double* __restrict__ cva) {
+    const int u = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (u >= units) return;
+    const int o = off[u], b = crp[u], n = crp[u + 1] - crp[u];
+    for (int j = 0; j < n; ++j) {
+        cci[b + j] = ccol[o + j];
+        cva[b + j] = cval[o + j];
+    }
+}
+
+// inv[perm[i]] = i for the real solve rows.
+__global__ void k_inv_perm(int n_pad, const int* __restrict__ perm, int* __restrict__ inv) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i < n_pad && perm[i] >= 0) inv[perm[i]] = i;
+}
+
+// down[q] = position of block q's unit in the next level (through inv when
+// the next level has a solve order), -1 for a block of padding rows.
+__global__ void k_down_map(int nq, const int* __restrict__ block_unit, const int* __restrict__ inv,
+                           int* __restrict__ down) {
+    const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (q >= nq) return;
+    const int u = block_unit[q];
+    down[q] = u < 0 ? -1 : (inv != nullptr ? inv[u] : u);
+}
+
+// The exact inverse of every diagonal 8x8 block of a SELL matrix (FP32
+// values, solve order), as full FP32 blocks for k_block_factor: one thread
+// per block, Cholesky in FP64; a block that is not positive definite keeps
+// its diagonal inverse.
+__global__ void k_block_exact_full(SellView A, int nq, float* __restrict__ full) {
+    const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (q >= nq) return;
+    double a[8][8] = {};
+    for (int i = 0; i < 8; ++i) {
+        const int row = q * 8 + i;
+        const int sl = row / kC, lane = row % kC;
+        const int j0 = A.slice_ptr[sl], j1 = A.slice_ptr[sl + 1];
+        const int base = A.slice_base[sl];
+        for (int j = j0; j < j1; ++j) {
+            const size_t e = static_cast<size_t>(j) * kC + static_cast<size_t>(lane);
+            const int c = j == j0 ? row : (A.idx16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]);
+            if (c >= q * 8 && c < q * 8 + 8) a[i][c - q * 8] += static_cast<double>(A.val[e]);
+        }
+    }
+    double l[8][8] = {};
+    bool ok = true;
+    for (int k = 0; k < 8 && ok; ++k) {
+        double d = a[k][k];
+        for (int t = 0; t < k; ++t) d -= l[k][t] * l[k][t];
+        if (!(d > 0.0)) {
+            ok = false;
+            break;
+        }
+        l[k][k] = sqrt(d);
+        for (int i = k + 1; i < 8; ++i) {
+            double v = a[i][k];
+            for (int t = 0; t < k; ++t) v -= l[i][t] * l[k][t];
+            l[i][k] = v / l[k][k];
+        }
+    }
+    float* out = full + static_cast<size_t>(q) * 64;
+    if (!ok) {
+        for (int i = 0; i < 8; ++i)
+            for (int k = 0; k < 8; ++k) out[i * 8 + k] = (i == k && a[i][i] > 0.0) ? static_cast<float>(1.0 / a[i][i]) : (i == k ? 1.0f : 0.0f);
+        return;
+    }
+    // inverse of L (lower), then A^-1 = L^-T L^-1
+    double li[8][8] = {};
+    for (int k = 0; k < 8; ++k) {
+        li[k][k] = 1.0 / l[k][k];
+        for (int i = k + 1; i < 8; ++i) {
+            double v = 0.0;
+            for (int t = k; t < i; ++t) v -= l[i][t] * li[t][k];
+            li[i][k] = v / l[i][i];
+        }
+    }
+    for (int i = 0; i < 8; ++i)
+        for (int k = 0; k < 8; ++k) {
+            double v = 0.0;
+            for (int t = (i > k ? i : k); t < 8; ++t) v += li[t][i] * li[t][k];
+            out[i * 8 + k] = static_cast<float>(v);
+        }
+}
+
+// Dense n x n matrix (row-major FP64) of a CSR.
+__global__ void k_dense_from_csr(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const double* __restrict__ va, double* __restrict__ dense) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) dense[static_cast<size_t>(i) * n + ci[e]] += va[e];
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one step k
+// in two kernels: the update of every entry outside row and column k (which
+// read row k and column k, unchanged here; the pivot's inverse saved in
+// pinv[k], a non-positive pivot flagged in *bad), then the scaling of row
+// and column k and the pivot itself.
+__global__ void k_gj_update(int n, int k, double* __restrict__ a, double* __restrict__ pinv, int* __restrict__ bad) {
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const double akk = a[static_cast<size_t>(k) * n + k];
+    if (idx == 0) {
+        pinv[k] = 1.0 / akk;
+        if (!(akk > 0.0)) *bad = 1;
+    }
+    if (idx >= static_cast<size_t>(n) * n) return;
+    const int i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
+    if (i == k || j == k) return;
+    a[idx] -= a[static_cast<size_t>(i) * n + k] * a[static_cast<size_t>(k) * n + j] / akk;
+}
+
+__global__ void k_gj_scale(int n, int k, double* __restrict__ a, const double* __restrict__ pinv) {
+    const int j = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (j >= n) return;
+    const double p = pinv[k];
+    if (j == k) {
+        a[static_cast<size_t>(k) * n + k] = p;
+    } else {
+        a[static_cast<size_t>(k) * n + j] *= p;
+        a[static_cast<size_t>(j) * n + k] *= -p;
+    }
+}
+
+__global__ void k_f64_to_f32(size_t n, const double* __restrict__ a, float* __restrict__ out) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) out[i] = static_cast<float>(a[i]);
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1848,9 +2242,16 @@ void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_n
 }
 
 void launch_stream(int grid, const void* src, void* dst, size_t bytes, unsigned* sink, hipStream_t s,
-                   hipEvent_t e0, hipEvent_t e1, bool nontemporal) {
+                   hipEvent_t e0, hipEvent_t e1, int nontemporal) {
     const size_t n16 = bytes / 16;
-    if (dst == nullptr && nontemporal)
+    if (dst != nullptr && nontemporal != 0) {
+        if (nontemporal == 2)
+            hipExtLaunchKernelGGL(k_stream_copy_nt<true>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0,
+                                  static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+        else
+            hipExtLaunchKernelGGL(k_stream_copy_nt<false>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0,
+                                  static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+    } else if (dst == nullptr && nontemporal != 0)
         hipExtLaunchKernelGGL(k_stream_read_nt, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0,
                               static_cast<const uint4*>(src), n16, sink);
     else if (dst == nullptr)
@@ -1963,7 +2364,8 @@ void launch_block_features(const SellView& A, double* dis, float* feat, float* s
 }
 
 void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const double* x, const double* b,
-                        const float* minv, const BlockPre* blockM, float* r, float* z, double* out) {
+                        const float* minv, const BlockPre* blockM, float* r, float* z, double* out,
+                        const MgPlan* mg) {
     const dim3 g(L.grid), blk(kBlock);
     if (A.idx16) {
         if (use_lo)
@@ -1976,6 +2378,7 @@ void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const d
         else
             hipLaunchKernelGGL((k_pcg_replace<false, 0>), g, blk, 0, L.stream, L, A, x, b, minv, blockM, r, z);
     }
+    if (mg != nullptr) launch_vcycle(*mg, L, nullptr, r, z);
     hipLaunchKernelGGL(k_replace_sums, dim3(A.nsys), blk, 0, L.stream, L, out);
 }
 
@@ -1995,8 +2398,10 @@ void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const d
     } while (0)
 
 void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
-                         const double* b, const float* minv, const BlockPre* blockM, float* r, float* z) {
+                         const double* b, const float* minv, const BlockPre* blockM, float* r, float* z,
+                         const MgPlan* mg) {
     XP_DISPATCH_ACC(A, use_lo, k_pcg_replace, dim3(L.grid), dim3(kBlock), 0, L.stream, L, A, x, b, minv, blockM, r, z);
+    if (mg != nullptr) launch_vcycle(*mg, L, nullptr, r, z);
     hipLaunchKernelGGL(k_pcg_init_state, dim3(A.nsys), dim3(kBlock), 0, L.stream, L, st);
 }
 
@@ -2124,6 +2529,105 @@ void launch_op_update_xrz(const Launch& L, int nsys, float* x, const float* p, f
 
 void launch_op_update_p(const Launch& L, const float* z, float* p, double beta) {
     XP_LAUNCH(L, k_op_update_p, dim3(L.grid), L, z, p, beta);
+}
+
+
+// ---- multilevel preconditioner ----
+void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const float* r0, float* z0) {
+    const MgLevelView& f = P.lv[0];
+#define XP_MG_SELL(LL, A, K, ...)                                                            \
+    do {                                                                                     \
+        if ((A).idx16) {                                                                     \
+            if ((A).variant >= 1) hipLaunchKernelGGL((K<true, 1>), dim3((LL).grid), dim3(kBlock), 0, (LL).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((K<true, 0>), dim3((LL).grid), dim3(kBlock), 0, (LL).stream, __VA_ARGS__); \
+        } else {                                                                             \
+            if ((A).variant >= 1) hipLaunchKernelGGL((K<false, 1>), dim3((LL).grid), dim3(kBlock), 0, (LL).stream, __VA_ARGS__); \
+            else hipLaunchKernelGGL((K<false, 0>), dim3((LL).grid), dim3(kBlock), 0, (LL).stream, __VA_ARGS__); \
+        }                                                                                    \
+    } while (0)
+    // level 0: z0 = M r from the update kernel; the pre-smoothed approximation is omega z0
+    float* first_dst = P.nlev > 1 ? P.lv[1].r : P.rc;
+    XP_MG_SELL(L0, f.A, k_mg_restrict, L0, f.A, st, r0, z0, P.omega, f.perm, f.down, first_dst);
+    for (int l = 1; l < P.nlev; ++l) {
+        const MgLevelView& v = P.lv[l];
+        hipLaunchKernelGGL(k_mg_smooth, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, st, v.M, v.r, P.omega, v.z);
+        float* dst = l + 1 < P.nlev ? P.lv[l + 1].r : P.rc;
+        XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.perm, v.down, dst);
+    }
+    hipLaunchKernelGGL(k_mg_coarse, dim3((P.nc + kWaves - 1) / kWaves), dim3(kBlock), 0, L0.stream, st, P.nc, P.cinv,
+                       P.rc, P.ec);
+    for (int l = P.nlev - 1; l >= 1; --l) {
+        const MgLevelView& v = P.lv[l];
+        const float* e = l + 1 < P.nlev ? P.lv[l + 1].z : P.ec;
+        hipLaunchKernelGGL(k_mg_prolong, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, st, v.z, 1.0f, e, P.gamma,
+                           v.perm, v.down, v.t);
+        XP_MG_SELL(v.L, v.A, k_mg_post_norz, v.L, v.A, st, v.M, v.r, v.t, P.omega, v.z);
+    }
+    const float* e1 = P.nlev > 1 ? P.lv[1].z : P.ec;
+    hipLaunchKernelGGL(k_mg_prolong, dim3(L0.grid), dim3(kBlock), 0, L0.stream, L0, st, z0, P.omega, e1, P.gamma,
+                       f.perm, f.down, f.t);
+    XP_MG_SELL(L0, f.A, k_mg_post_rz, L0, f.A, st, f.M, r0, f.t, P.omega, z0);
+#undef XP_MG_SELL
+}
+
+void launch_mg_block_apply(const Launch& L, const BlockPre* M, const float* r, float* z) {
+    hipLaunchKernelGGL(k_mg_smooth, dim3(L.grid), dim3(kBlock), 0, L.stream, L, nullptr, M, r, 1.0f, z);
+}
+
+void launch_mg_unit_of_row(int units, const int* slots, int* agg, hipStream_t s) {
+    if (units > 0)
+        hipLaunchKernelGGL(k_unit_of_row, dim3(blocks_for(static_cast<size_t>(units) * 8, 256)), dim3(256), 0, s,
+                           units, slots, agg);
+}
+
+void launch_mg_galerkin_count(int units, const int* slots, const int* rp, int* cnt, hipStream_t s) {
+    if (units > 0)
+        hipLaunchKernelGGL(k_gal_count, dim3(blocks_for(static_cast<size_t>(units), 256)), dim3(256), 0, s, units,
+                           slots, rp, cnt);
+}
+
+void launch_mg_galerkin_fill(int units, const int* slots, const int* rp, const int* ci, const double* va,
+                             const int* agg, const int* off, int* ccol, double* cval, int* dcnt, hipStream_t s) {
+    if (units > 0)
+        hipLaunchKernelGGL(k_gal_fill, dim3(blocks_for(static_cast<size_t>(units), kGalThreads)), dim3(kGalThreads),
+                           0, s, units, slots, rp, ci, va, agg, off, ccol, cval, dcnt);
+}
+
+void launch_mg_galerkin_compact(int units, const int* off, const int* crp, const int* ccol, const double* cval,
+                                int* cci, double* cva, hipStream_t s) {
+    if (units > 0)
+        hipLaunchKernelGGL(k_gal_compact, dim3(blocks_for(static_cast<size_t>(units), 256)), dim3(256), 0, s, units,
+                           off, crp, ccol, cval, cci, cva);
+}
+
+void launch_mg_inv_perm(int n_pad, const int* perm, int* inv, hipStream_t s) {
+    hipLaunchKernelGGL(k_inv_perm, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad, perm,
+                       inv);
+}
+
+void launch_mg_down_map(int nq, const int* block_unit, const int* inv, int* down, hipStream_t s) {
+    hipLaunchKernelGGL(k_down_map, dim3(blocks_for(static_cast<size_t>(nq), 256)), dim3(256), 0, s, nq, block_unit,
+                       inv, down);
+}
+
+void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t s) {
+    hipLaunchKernelGGL(k_block_exact_full, dim3(blocks_for(static_cast<size_t>(nq), 256)), dim3(256), 0, s, A, nq,
+                       full);
+}
+
+void launch_mg_dense_inverse(int n, const int* rp, const int* ci, const double* va, double* work, float* out,
+                             int* bad, hipStream_t s) {
+    const size_t nn = static_cast<size_t>(n) * static_cast<size_t>(n);
+    double* pinv = work + nn;
+    hipLaunchKernelGGL(k_fill<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, 0.0, work);
+    hipLaunchKernelGGL(k_dense_from_csr, dim3(blocks_for(static_cast<size_t>(n), 256)), dim3(256), 0, s, n, rp, ci,
+                       va, work);
+    for (int k = 0; k < n; ++k) {
+        hipLaunchKernelGGL(k_gj_update, dim3(blocks_for(nn, 256)), dim3(256), 0, s, n, k, work, pinv, bad);
+        hipLaunchKernelGGL(k_gj_scale, dim3(blocks_for(static_cast<size_t>(n), 256)), dim3(256), 0, s, n, k, work,
+                           pinv);
+    }
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, work, out);
 }
 
 }  // namespace xp

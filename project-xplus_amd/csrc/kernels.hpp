@@ -10,6 +10,8 @@
 
 namespace xp {
 
+struct MgPlan;  // the multilevel preconditioner's V-cycle (below)
+
 // Shared launch context of the row-streaming kernels: the work map (one
 // WorkItem per logical workgroup), the per-system workgroup ranges and the
 // per-workgroup reduction partials (kNumSlots x grid doubles).
@@ -58,7 +60,8 @@ void launch_pcg_update_p(const Launch& L, const SysState* st_cur, SysState* st_n
 // init: r = fp32(b - A x) (FP64), z = M r, rz / rr partials, then the state
 // after init; the first fused iteration sets p = z, ap = A z (state.fresh).
 void launch_pcg_init_acc(const Launch& L, const SellView& A, bool use_lo, SysState* st, const double* x,
-                         const double* b, const float* minv, const BlockPre* blockM, float* r, float* z);
+                         const double* b, const float* minv, const BlockPre* blockM, float* r, float* z,
+                         const MgPlan* mg = nullptr);
 // vmode: 0 FP64 values, 1 hi + lo (FP32 pair), 2 hi + lo16.  p alternates
 // between two buffers by iteration parity (p_old the previous iteration's).
 void launch_pcg_spmv_fused_acc(const Launch& L, const SellView& A, int vmode, const SysState* st_cur,
@@ -73,12 +76,59 @@ void launch_pcg_update_xrz_acc(const Launch& L, const SysState* st, bool odd, do
 // update), then clears the flag; st: the current state slot.
 void launch_pcg_flush_x_acc(const Launch& L, int nsys, SysState* st, const double* p, double* x);
 
+// ---- multilevel (aggregation) preconditioner ----
+constexpr int kMaxLevels = 16;
+// One SELL level of the hierarchy (level 0: the solver's matrix).
+struct MgLevelView {
+    Launch L;                    // the level's launch context (level 0: unused, the solver's is passed)
+    SellView A;
+    const BlockPre* M = nullptr; // block factors of the smoother
+    const int* perm = nullptr;   // solve row -> natural row, -1 for padding rows
+    const int* down = nullptr;   // [n_pad / 8] block -> its row in the next level, -1 for padding blocks
+    float* r = nullptr;          // level >= 1: restricted residual
+    float* z = nullptr;          // level >= 1: the level's correction
+    float* t = nullptr;          // prolonged approximation (every level)
+};
+struct MgPlan {
+    int nlev = 0;                // SELL levels; the dense coarsest level follows them
+    MgLevelView lv[kMaxLevels];
+    int nc = 0;                  // rows of the coarsest level
+    const float* cinv = nullptr; // its dense inverse, nc x nc row-major FP32
+    float* rc = nullptr;
+    float* ec = nullptr;
+    float omega = 0.7f;          // smoother damping
+    float gamma = 1.5f;          // coarse-correction scale
+};
+// One V-cycle on the solver's residual r0 (solve order of level 0): on entry
+// z0 holds M r0 (the update kernel's block product, the pre-smoother before
+// damping); on exit z0 holds the cycle's result and the solver's kSlotRz
+// partials (L0) hold r0^T z0.  st: the state slot the update kernel read.
+void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const float* r0, float* z0);
+// z = M r (the level-0 block product alone).
+void launch_mg_block_apply(const Launch& L, const BlockPre* M, const float* r, float* z);
+void launch_mg_unit_of_row(int units, const int* slots, int* agg, hipStream_t s);
+void launch_mg_galerkin_count(int units, const int* slots, const int* rp, int* cnt, hipStream_t s);
+void launch_mg_galerkin_fill(int units, const int* slots, const int* rp, const int* ci, const double* va,
+                             const int* agg, const int* off, int* ccol, double* cval, int* dcnt, hipStream_t s);
+void launch_mg_galerkin_compact(int units, const int* off, const int* crp, const int* ccol, const double* cval,
+                                int* cci, double* cva, hipStream_t s);
+void launch_mg_inv_perm(int n_pad, const int* perm, int* inv, hipStream_t s);
+void launch_mg_down_map(int nq, const int* block_unit, const int* inv, int* down, hipStream_t s);
+// Exact inverses of the diagonal 8x8 blocks (solve order) as full FP32 blocks.
+void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t s);
+// out = inverse of the n x n SPD matrix of a CSR (FP64 Gauss-Jordan, two
+// kernels per pivot, rounded to FP32); work: n * n + n doubles; *bad = 1 on
+// a non-positive pivot (bad must be zero on entry).
+void launch_mg_dense_inverse(int n, const int* rp, const int* ci, const double* va, double* work, float* out,
+                             int* bad, hipStream_t s);
+
 // ---- plain SpMV (no reduction) ----
 // Streaming reference: read `bytes` of src once (dst null) or copy them to
-// dst, grid workgroups, between events e0 / e1 on stream s.
+// dst, grid workgroups, between events e0 / e1 on stream s.  nontemporal:
+// reads 1 = non-temporal loads; copies 1 = non-temporal stores, 2 =
+// non-temporal loads and stores.
 void launch_stream(int grid, const void* src, void* dst, size_t bytes, unsigned* sink, hipStream_t s,
-                   hipEvent_t e0, hipEvent_t e1,
-                   bool nontemporal = false);
+                   hipEvent_t e0, hipEvent_t e1, int nontemporal = 0);
 void launch_spmv_f32(const Launch& L, const SellView& A, const float* x, float* y);
 // FP64 SpMV with exact FP64 values (val64) or hi+lo pairs (val + val_lo).
 void launch_spmv_f64(const Launch& L, const SellView& A, bool use_lo, const double* x, double* y);
@@ -90,7 +140,8 @@ void launch_residual_f64(const Launch& L, const SellView& A, bool use_lo, const 
 // rr partials of the next scalar step rewritten from them;
 // out[3 sys + 0..2] = (rz, rr, FP64 true r^T r) of each system.
 void launch_pcg_replace(const Launch& L, const SellView& A, bool use_lo, const double* x, const double* b,
-                        const float* minv, const BlockPre* blockM, float* r, float* z, double* out);
+                        const float* minv, const BlockPre* blockM, float* r, float* z, double* out,
+                        const MgPlan* mg = nullptr);
 // x64 += (double) d; d = 0.
 void launch_fold_f32_into_f64(int n_pad, float* d, double* x, hipStream_t s);
 // x64 += (double) d32 over the solve-order rows.

@@ -368,9 +368,9 @@ std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const i
     m->variant = (o != nullptr && o->spmv_variant >= 1 && o->spmv_variant <= 2) ? o->spmv_variant : 0;
     m->block_rows = (o != nullptr && o->block_rows == 8) ? 8 : 1;
     m->blocking = m->block_rows == 8 && o != nullptr ? o->blocking : 0;
-    if (m->block_rows == 8 && m->blocking != 0 && m->blocking != 3)
-        throw std::runtime_error("the device builder aggregates with rule 3 (blocking 3) or uses consecutive rows "
-                                 "(blocking 0); rules 1 and 2 run on the host (xp_matrix_create)");
+    if (m->block_rows == 8 && m->blocking != 0 && m->blocking != 3 && m->blocking != 4)
+        throw std::runtime_error("the device builder aggregates with rule 3 (blocking 3) or 4 (blocking 4) or uses "
+                                 "consecutive rows (blocking 0); rules 1 and 2 run on the host (xp_matrix_create)");
     std::lock_guard<std::mutex> lock(xp::set_up_mutex());  // the set-up's work buffers are shared
     xp::DevCsr a{n, 0, row_ptr, col_idx, values};
     xp::DevRows rows;
@@ -388,7 +388,7 @@ std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const i
     a.values = m->d_va.p;
     int units = 0;
     if (m->block_rows == 8)
-        units = m->blocking == 3 ? xp::device_aggregate(a, rows, m->d_slots, &m->agg, s)
+        units = m->blocking >= 3 ? xp::device_aggregate(a, rows, m->d_slots, &m->agg, s, m->blocking == 4 ? 1 : 2)
                                  : xp::device_natural_slots(n, m->d_slots, s);
     xp::DevSell out;
     xp::device_sell(a, rows, (o != nullptr && o->sigma > 0) ? o->sigma : 256, m->keep_lo, m->keep64, m->block_rows,
@@ -476,6 +476,26 @@ void ensure_host_mirror(const xp_matrix& cm) {
 }  // namespace
 
 // ---- solver ------------------------------------------------------------------
+// The multilevel preconditioner's hierarchy (xp_pcg_set_multilevel): level
+// 0 is the solver's matrix with its block factors; level l + 1 is P^T A P
+// over the 8-row aggregates of level l, set up on the device like any
+// device-built matrix (its own rule-3 aggregates, SELL layout, exact block
+// factors), down to a dense coarsest level of at most coarse_max rows.
+struct MgHierarchy {
+    std::vector<std::unique_ptr<xp_matrix>> mats;  // levels 1 .. nlev - 1
+    std::vector<xp::DevBuf<xp::BlockPre>> M;      // [nlev]; level 0 uses the solver's blocks
+    std::vector<xp::DevBuf<int>> down;            // [nlev]
+    std::vector<xp::DevBuf<float>> r, z, t;       // [nlev]; level 0 only t
+    xp::DevBuf<float> cinv, rc, ec;               // the dense coarsest level
+    xp::DevBuf<int> c_rp, c_ci, c_slots;          // its CSR (natural) and the aggregates it came from
+    xp::DevBuf<double> c_va;
+    int nc = 0;
+    int sigma0 = 0;                               // level 0's sort window at build time
+    float omega = 0.7f, gamma = 1.5f;
+    double setup_ms = 0.0;
+    std::vector<double> phase_ms;                 // per level: Galerkin product, matrix set-up, block factors
+};
+
 struct xp_pcg {
     xp_matrix* m = nullptr;
     xp_solve_opts opts{};
@@ -519,6 +539,40 @@ struct xp_pcg {
     int last_restarts = 0;            // recurrence restarts of the last xp_pcg_run_reliable
     bool x64_valid = false;           // x64 holds the solution of the last (refined) run
     const float* rhs = nullptr;       // RHS of the init kernel; null = b
+    std::unique_ptr<MgHierarchy> mg;  // multilevel preconditioner (accurate iteration, blocks)
+
+    // The V-cycle replaces z = M r while a block preconditioner is installed
+    // (a fallback to Jacobi turns it off with the blocks).
+    bool mg_on() const { return mg != nullptr && use_block && opts.accurate; }
+    xp::MgPlan mg_plan() const {
+        xp::MgPlan P;
+        const MgHierarchy& H = *mg;
+        P.nlev = 1 + static_cast<int>(H.mats.size());
+        P.lv[0].A = m->view();
+        P.lv[0].M = blockM.p;
+        P.lv[0].perm = m->d_perm.p;
+        P.lv[0].down = H.down[0].p;
+        P.lv[0].t = H.t[0].p;
+        for (int l = 1; l < P.nlev; ++l) {
+            const xp_matrix& c = *H.mats[static_cast<size_t>(l) - 1];
+            xp::MgLevelView& v = P.lv[l];
+            v.L = c.launch(stream);
+            v.A = c.view();
+            v.M = H.M[static_cast<size_t>(l)].p;
+            v.perm = c.d_perm.p;
+            v.down = H.down[static_cast<size_t>(l)].p;
+            v.r = H.r[static_cast<size_t>(l)].p;
+            v.z = H.z[static_cast<size_t>(l)].p;
+            v.t = H.t[static_cast<size_t>(l)].p;
+        }
+        P.nc = H.nc;
+        P.cinv = H.cinv.p;
+        P.rc = H.rc.p;
+        P.ec = H.ec.p;
+        P.omega = H.omega;
+        P.gamma = H.gamma;
+        return P;
+    }
 
     ~xp_pcg() {
         if (m != nullptr) --m->solvers;
@@ -588,6 +642,7 @@ struct xp_pcg {
                                           ap64.p);
             xp::launch_pcg_update_xrz_acc(L, nxt, (k & 1) != 0, x64.p, pb(k), pb(k + 1), r.p, ap64.p, minv.p, bm(),
                                           z.p);
+            if (mg_on()) xp::launch_vcycle(mg_plan(), L, nxt, r.p, z.p);
         } else if (opts.fused) {
             xp::launch_pcg_spmv_fused(L, A, cur, nxt, trace.p, trace_stride, z.p, p.p, ap.p);
             xp::launch_pcg_update_xrz(L, nxt, x.p, p.p, r.p, ap.p, minv.p, bm(), z.p);
@@ -636,8 +691,9 @@ struct xp_pcg {
 
     void enqueue_init() {
         if (opts.accurate) {  // x64 holds the start vector, b64 the right-hand side
+            const xp::MgPlan P = mg_on() ? mg_plan() : xp::MgPlan{};
             xp::launch_pcg_init_acc(m->launch(stream), m->view(), !m->has64, st.p, x64.p, b64.p, minv.p, bm(), r.p,
-                                    z.p);
+                                    z.p, mg_on() ? &P : nullptr);
             return;
         }
         xp::launch_pcg_init(m->launch(stream), m->view(), st.p, x.p, rhs != nullptr ? rhs : b.p, minv.p, bm(), r.p,
@@ -1057,7 +1113,7 @@ int xp_matrix_create(int n, const int* row_ptr, const int* col_idx, const double
         if (out == nullptr) throw std::runtime_error("null output pointer");
         *out = nullptr;
         if (n <= 0) throw std::runtime_error("n must be positive");
-        if (opts != nullptr && (opts->device_build || (opts->block_rows == 8 && opts->blocking == 3))) {
+        if (opts != nullptr && (opts->device_build || (opts->block_rows == 8 && opts->blocking >= 3))) {
             // upload the CSR and set the matrix up on the device
             if (row_ptr == nullptr || col_idx == nullptr || values == nullptr)
                 throw std::runtime_error("build_sell: null CSR array");
@@ -1479,18 +1535,20 @@ int xp_dot_f64_device(const double* a, const double* b, double* out, int n, void
 // (above the 256 MB MALL, every launch streams from HBM).
 int xp_stream_time(unsigned long long bytes, int mode, int grid, int reps, double* avg_us) {
     return guarded([&] {
-        if (avg_us == nullptr || reps <= 0 || bytes < 16 || mode < 0 || mode > 2 || grid <= 0)
+        if (avg_us == nullptr || reps <= 0 || bytes < 16 || mode < 0 || mode > 4 || grid <= 0)
             throw std::runtime_error("bad argument");
         const hipStream_t s = device().stream;
         const size_t b = static_cast<size_t>(bytes) / 16 * 16;
-        DevBuf<uint4> src(b / 16), dst(mode == 1 ? b / 16 : 0);
+        const bool copy = mode == 1 || mode >= 3;
+        const int nt = mode == 2 ? 1 : (mode >= 3 ? mode - 2 : 0);
+        DevBuf<uint4> src(b / 16), dst(copy ? b / 16 : 0);
         DevBuf<unsigned> sink(static_cast<size_t>(grid));
         HIPX(hipMemsetAsync(src.p, 1, b, s));
         std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps + 3));
         for (auto& e : ev) HIPX(hipEventCreate(&e));
         for (int k = 0; k < reps + 3; ++k)
-            xp::launch_stream(grid, src.p, mode == 1 ? dst.p : nullptr, b, sink.p, s, ev[2 * static_cast<size_t>(k)],
-                              ev[2 * static_cast<size_t>(k) + 1], mode == 2);
+            xp::launch_stream(grid, src.p, copy ? dst.p : nullptr, b, sink.p, s, ev[2 * static_cast<size_t>(k)],
+                              ev[2 * static_cast<size_t>(k) + 1], nt);
         HIPX(hipStreamSynchronize(s));
         double total = 0.0;
         for (int k = 3; k < reps + 3; ++k) {
@@ -1661,8 +1719,11 @@ int xp_pcg_get_trace(xp_pcg* s, int sys, double* rows, int cap) {
 // execution time, as a kernel trace reports it.  The iterations start from
 // x0 with the convergence test off (tau = 0), so every kernel does its full
 // work; the solver's vectors and state are left as after them.
-int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels) {
-    return guarded([&] {
+namespace {
+// us[0..2]: average over all iterations; split (6 entries, may be null):
+// the averages over even iterations, then over odd ones.
+void pcg_kernel_times(xp_pcg* s, int iters, double* us, double* split, int* nkernels) {
+    {
         if (s == nullptr || us == nullptr || !s->have_rhs || iters <= 0) throw std::runtime_error("bad argument");
         xp_pcg& S = *s;
         xp_matrix& m = *S.m;
@@ -1706,17 +1767,37 @@ int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels) {
         }
         HIPX(hipStreamSynchronize(S.stream));
         for (int j = 0; j < 3; ++j) us[j] = 0.0;
+        double par[2][3] = {};
+        int cnt[2] = {0, 0};
         for (int k = 0; k < iters; ++k) {
             const hipEvent_t* e = ev.data() + static_cast<size_t>(k) * static_cast<size_t>(2 * nk);
+            ++cnt[k & 1];
             for (int j = 0; j < nk; ++j) {
                 float ms = 0.0f;
                 HIPX(hipEventElapsedTime(&ms, e[2 * j], e[2 * j + 1]));
                 us[j] += static_cast<double>(ms) * 1000.0 / iters;
+                par[k & 1][j] += static_cast<double>(ms) * 1000.0;
             }
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
+        if (split != nullptr)
+            for (int q = 0; q < 2; ++q)
+                for (int j = 0; j < 3; ++j) split[3 * q + j] = cnt[q] > 0 ? par[q][j] / cnt[q] : 0.0;
         S.x64_valid = false;
         if (nkernels != nullptr) *nkernels = nk;
+    }
+}
+}  // namespace
+
+int xp_pcg_kernel_times(xp_pcg* s, int iters, double* us, int* nkernels) {
+    return guarded([&] { pcg_kernel_times(s, iters, us, nullptr, nkernels); });
+}
+
+int xp_pcg_kernel_times_split(xp_pcg* s, int iters, double* us6, int* nkernels) {
+    return guarded([&] {
+        if (us6 == nullptr) throw std::runtime_error("bad argument");
+        double avg[3];
+        pcg_kernel_times(s, iters, avg, us6, nkernels);
     });
 }
 
@@ -2355,8 +2436,11 @@ int xp_pcg_run_reliable(xp_pcg* s, double rel_tol, double delta, double* rel_res
             // fold and replace (all systems: a replacement is valid at any point)
             if (!acc) xp::launch_fold_f32_into_f64(m.n_pad, S.x.p, S.x64.p, S.stream);
             S.flush_x();
-            xp::launch_pcg_replace(L, m.view(), !m.has64, S.x64.p, S.b64.p, S.minv.p, S.bm(), S.r.p, S.z.p,
-                                   S.repl_out.p);
+            {
+                const xp::MgPlan P = S.mg_on() ? S.mg_plan() : xp::MgPlan{};
+                xp::launch_pcg_replace(L, m.view(), !m.has64, S.x64.p, S.b64.p, S.minv.p, S.bm(), S.r.p, S.z.p,
+                                       S.repl_out.p, S.mg_on() ? &P : nullptr);
+            }
             S.repl_out.download(sums.data(), sums.size(), S.stream);
             HIPX(hipStreamSynchronize(S.stream));
             ++repl;
@@ -3297,6 +3381,234 @@ int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale) {
             std::memcpy(feat + static_cast<size_t>(bid) * 48, hf.data() + static_cast<size_t>(q) * 48, 48 * sizeof(float));
             std::memcpy(scale + static_cast<size_t>(bid) * 8, hs.data() + static_cast<size_t>(q) * 8, 8 * sizeof(float));
         }
+    });
+}
+
+
+// ---- multilevel preconditioner ------------------------------------------------
+namespace {
+
+// Galerkin coarse matrix of a device-built block matrix c: row u = block
+// (unit) u, entry (u, v) = sum of a_ij over natural rows i of u and j of v.
+void galerkin(const xp_matrix& c, xp::DevBuf<int>& crp, xp::DevBuf<int>& cci, xp::DevBuf<double>& cva,
+              hipStream_t s) {
+    const int units = c.unit_first.back();
+    const int n = c.n;
+    int nnz = 0;
+    HIPX(hipMemcpyAsync(&nnz, c.d_rp.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    xp::DevBuf<int> agg(static_cast<size_t>(n)), cnt(static_cast<size_t>(units)),
+        off(static_cast<size_t>(units) + 1), dcnt(static_cast<size_t>(units));
+    xp::DevBuf<int> ccol(std::max<size_t>(static_cast<size_t>(nnz), 1));
+    xp::DevBuf<double> cval(std::max<size_t>(static_cast<size_t>(nnz), 1));
+    xp::launch_mg_unit_of_row(units, c.d_slots.p, agg.p, s);
+    xp::launch_mg_galerkin_count(units, c.d_slots.p, c.d_rp.p, cnt.p, s);
+    xp::device_scan_exclusive(cnt.p, off.p, units, off.p + units, s);
+    xp::launch_mg_galerkin_fill(units, c.d_slots.p, c.d_rp.p, c.d_ci.p, c.d_va.p, agg.p, off.p, ccol.p, cval.p,
+                                dcnt.p, s);
+    crp.alloc(static_cast<size_t>(units) + 1);
+    xp::device_scan_exclusive(dcnt.p, crp.p, units, crp.p + units, s);
+    int cnnz = 0;
+    HIPX(hipMemcpyAsync(&cnnz, crp.p + units, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    cci.alloc(static_cast<size_t>(cnnz));
+    cva.alloc(static_cast<size_t>(cnnz));
+    xp::launch_mg_galerkin_compact(units, off.p, crp.p, ccol.p, cval.p, cci.p, cva.p, s);
+    HIPX(hipStreamSynchronize(s));  // before the temporaries go
+}
+
+void build_multilevel(xp_pcg& S, int coarse_max, double omega, double gamma, int coarse_blocking) {
+    xp_matrix& m0 = *S.m;
+    if (!S.opts.accurate) throw std::runtime_error("the multilevel preconditioner runs in the accurate iteration");
+    if (!m0.device_built || m0.block_rows != 8 || m0.blocking < 3 || m0.nsys != 1)
+        throw std::runtime_error("the multilevel preconditioner needs one system set up on the device with "
+                                 "block_rows = 8 and blocking 3 or 4");
+    if (coarse_blocking != 3 && coarse_blocking != 4) throw std::runtime_error("coarse blocking must be 3 or 4");
+    if (!S.use_block)
+        throw std::runtime_error("install the level-0 block preconditioner (xp_pcg_set_preconditioner / "
+                                 "xp_pcg_set_mlp_preconditioner) before xp_pcg_set_multilevel");
+    if (coarse_max < 8 || coarse_max > 4096) throw std::runtime_error("coarse_max must be in [8, 4096]");
+    if (!(omega > 0.0) || !(omega < 2.0) || !(gamma > 0.0) || !(gamma < 4.0))
+        throw std::runtime_error("bad multilevel parameters (0 < omega < 2, 0 < gamma < 4)");
+    const hipStream_t s = device().stream;
+    HIPX(hipStreamSynchronize(S.stream));
+    auto t0 = std::chrono::steady_clock::now();
+    auto H = std::make_unique<MgHierarchy>();
+    H->omega = static_cast<float>(omega);
+    H->gamma = static_cast<float>(gamma);
+    H->sigma0 = m0.sigma;
+    const xp_matrix* cur = &m0;
+    for (int l = 0;; ++l) {
+        if (l + 1 >= xp::kMaxLevels) throw std::runtime_error("multilevel: too many levels");
+        const int units = cur->unit_first.back();
+        const int nq = cur->n_pad / 8;
+        xp::DevBuf<int> crp, cci;
+        xp::DevBuf<double> cva;
+        auto tp = std::chrono::steady_clock::now();
+        auto lap = [&tp]() {
+            const auto now = std::chrono::steady_clock::now();
+            const double ms = std::chrono::duration<double, std::milli>(now - tp).count();
+            tp = now;
+            return ms;
+        };
+        galerkin(*cur, crp, cci, cva, s);
+        H->phase_ms.push_back(lap());
+        H->down.emplace_back(static_cast<size_t>(nq));
+        H->t.emplace_back(static_cast<size_t>(cur->n_pad));
+        if (l == 0) {
+            H->M.emplace_back();
+            H->r.emplace_back();
+            H->z.emplace_back();
+        }
+        if (units <= coarse_max) {
+            xp::launch_mg_down_map(nq, cur->d_block_unit.p, nullptr, H->down.back().p, s);
+            H->nc = units;
+            xp::DevBuf<double> work(static_cast<size_t>(units) * static_cast<size_t>(units + 1));
+            xp::DevBuf<int> bad(1);
+            HIPX(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+            H->cinv.alloc(static_cast<size_t>(units) * static_cast<size_t>(units));
+            H->rc.alloc(static_cast<size_t>(units));
+            H->ec.alloc(static_cast<size_t>(units));
+            xp::launch_mg_dense_inverse(units, crp.p, cci.p, cva.p, work.p, H->cinv.p, bad.p, s);
+            int hb = 0;
+            HIPX(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            H->c_slots.alloc(static_cast<size_t>(units) * 8);
+            HIPX(hipMemcpyAsync(H->c_slots.p, cur->d_slots.p, static_cast<size_t>(units) * 8 * sizeof(int),
+                                hipMemcpyDeviceToDevice, s));
+            HIPX(hipStreamSynchronize(s));
+            if (hb != 0) throw std::runtime_error("multilevel: the coarsest matrix is not positive definite");
+            H->phase_ms.push_back(lap());
+            H->phase_ms.push_back(0.0);
+            H->c_rp = std::move(crp);
+            H->c_ci = std::move(cci);
+            H->c_va = std::move(cva);
+            break;
+        }
+        xp_matrix_opts o{};
+        o.block_rows = 8;
+        o.blocking = coarse_blocking;
+        o.sigma = 64;
+        std::unique_ptr<xp_matrix> next = make_matrix_device(units, crp.p, cci.p, cva.p, &o);
+        if (next->bad_row >= 0) throw std::runtime_error("multilevel: a coarse matrix has a non-positive diagonal");
+        HIPX(hipStreamSynchronize(s));
+        H->phase_ms.push_back(lap());
+        xp::DevBuf<int> inv(static_cast<size_t>(units));
+        xp::launch_mg_inv_perm(next->n_pad, next->d_perm.p, inv.p, s);
+        xp::launch_mg_down_map(nq, cur->d_block_unit.p, inv.p, H->down.back().p, s);
+        const int nqn = next->n_pad / 8;
+        xp::DevBuf<float> full(static_cast<size_t>(nqn) * 64);
+        xp::launch_block_exact_full(next->view(), nqn, full.p, s);
+        H->M.emplace_back(static_cast<size_t>(nqn));
+        xp::launch_block_factor(nqn, full.p, H->M.back().p, s);
+        H->r.emplace_back(static_cast<size_t>(next->n_pad));
+        H->z.emplace_back(static_cast<size_t>(next->n_pad));
+        xp::launch_fill_f32(static_cast<size_t>(next->n_pad), 0.0f, H->r.back().p, s);
+        xp::launch_fill_f32(static_cast<size_t>(next->n_pad), 0.0f, H->z.back().p, s);
+        HIPX(hipStreamSynchronize(s));  // before the temporaries go
+        H->phase_ms.push_back(lap());
+        H->mats.push_back(std::move(next));
+        cur = H->mats.back().get();
+    }
+    H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    S.mg = std::move(H);
+    S.invalidate_graph();
+}
+
+}  // namespace
+
+int xp_pcg_set_multilevel(xp_pcg* s, int coarse_max, double omega, double gamma, int coarse_blocking) {
+    return guarded([&] {
+        if (s == nullptr) throw std::runtime_error("null solver");
+        if (coarse_max <= 0) {
+            s->mg.reset();
+            s->invalidate_graph();
+            return;
+        }
+        s->mg.reset();
+        build_multilevel(*s, coarse_max, omega, gamma, coarse_blocking);
+    });
+}
+
+int xp_pcg_multilevel_phases(const xp_pcg* s, int cap, double* ms, int* count) {
+    return guarded([&] {
+        if (s == nullptr || count == nullptr) throw std::runtime_error("null argument");
+        *count = s->mg == nullptr ? 0 : static_cast<int>(s->mg->phase_ms.size());
+        if (s->mg != nullptr && ms != nullptr)
+            for (int k = 0; k < std::min(cap, *count); ++k) ms[k] = s->mg->phase_ms[static_cast<size_t>(k)];
+    });
+}
+
+int xp_pcg_multilevel_info(const xp_pcg* s, int cap, int* nlev, int* n, long long* nnz, double* setup_ms) {
+    return guarded([&] {
+        if (s == nullptr || nlev == nullptr) throw std::runtime_error("null argument");
+        if (s->mg == nullptr) {
+            *nlev = 0;
+            return;
+        }
+        const MgHierarchy& H = *s->mg;
+        const int total = 2 + static_cast<int>(H.mats.size());
+        *nlev = total;
+        if (setup_ms != nullptr) *setup_ms = H.setup_ms;
+        for (int l = 0; l < std::min(cap, total); ++l) {
+            const xp_matrix* c = l == 0 ? s->m : (l + 1 < total ? H.mats[static_cast<size_t>(l) - 1].get() : nullptr);
+            if (n != nullptr) n[l] = c != nullptr ? c->n : H.nc;
+            if (nnz != nullptr) {
+                if (c != nullptr) {
+                    nnz[l] = c->nnz;
+                } else {
+                    int v = 0;
+                    HIPX(hipMemcpy(&v, H.c_rp.p + H.nc, sizeof(int), hipMemcpyDeviceToHost));
+                    nnz[l] = v;
+                }
+            }
+        }
+    });
+}
+
+// Level l >= 1 of the hierarchy as a natural-order CSR (its rows are the
+// blocks of level l - 1) and the blocks of level l - 1 it came from (units x 8
+// natural rows of level l - 1, -1 padding).  Null outputs are skipped.
+int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_idx, double* values, int* slots) {
+    return guarded([&] {
+        if (s == nullptr || s->mg == nullptr) throw std::runtime_error("no multilevel hierarchy");
+        const MgHierarchy& H = *s->mg;
+        const int total = 2 + static_cast<int>(H.mats.size());
+        if (level < 1 || level >= total) throw std::runtime_error("bad level");
+        const hipStream_t st = device().stream;
+        const bool dense = level == total - 1;
+        const xp_matrix* c = dense ? nullptr : H.mats[static_cast<size_t>(level) - 1].get();
+        const xp_matrix* prev = level == 1 ? s->m : H.mats[static_cast<size_t>(level) - 2].get();
+        const int n = dense ? H.nc : c->n;
+        const int* rp = dense ? H.c_rp.p : c->d_rp.p;
+        int nz = 0;
+        HIPX(hipMemcpyAsync(&nz, rp + n, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+        if (row_ptr != nullptr) HIPX(hipMemcpyAsync(row_ptr, rp, (static_cast<size_t>(n) + 1) * sizeof(int),
+                                                    hipMemcpyDeviceToHost, st));
+        if (col_idx != nullptr && nz > 0)
+            HIPX(hipMemcpyAsync(col_idx, dense ? H.c_ci.p : c->d_ci.p, static_cast<size_t>(nz) * sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+        if (values != nullptr && nz > 0)
+            HIPX(hipMemcpyAsync(values, dense ? H.c_va.p : c->d_va.p, static_cast<size_t>(nz) * sizeof(double),
+                                hipMemcpyDeviceToHost, st));
+        if (slots != nullptr)
+            HIPX(hipMemcpyAsync(slots, prev->d_slots.p, static_cast<size_t>(n) * 8 * sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+    });
+}
+
+// z = V(r) for device vectors r, z of level 0 (solve order, n_pad): the
+// block product M r, then one V-cycle.  For tests and inspection.
+int xp_pcg_multilevel_apply(xp_pcg* s, const float* r, float* z) {
+    return guarded([&] {
+        if (s == nullptr || r == nullptr || z == nullptr) throw std::runtime_error("null argument");
+        if (!s->mg_on()) throw std::runtime_error("no multilevel hierarchy in use");
+        const xp::MgPlan P = s->mg_plan();
+        xp::Launch L = s->m->launch(s->stream);
+        xp::launch_mg_block_apply(L, s->blockM.p, r, z);
+        xp::launch_vcycle(P, L, nullptr, r, z);
+        HIPX(hipStreamSynchronize(s->stream));
     });
 }
 

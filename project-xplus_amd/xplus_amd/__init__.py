@@ -132,6 +132,7 @@ _p_run_refined = _sig("xp_pcg_run_refined", C.c_int,
 _p_run_reliable = _sig("xp_pcg_run_reliable", C.c_int,
                        [_vp, C.c_double, C.c_double, _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)])
 _p_kernel_times = _sig("xp_pcg_kernel_times", C.c_int, [_vp, C.c_int, _f64p, C.POINTER(C.c_int)])
+_p_kernel_times_split = _sig("xp_pcg_kernel_times_split", C.c_int, [_vp, C.c_int, _f64p, C.POINTER(C.c_int)])
 _p_status = _sig("xp_pcg_get_status", C.c_int, [_vp, C.c_int, C.POINTER(Status)])
 _p_get_x = _sig("xp_pcg_get_x", C.c_int, [_vp, _f64p])
 _p_trace = _sig("xp_pcg_get_trace", C.c_int, [_vp, C.c_int, _f64p, C.c_int])
@@ -161,6 +162,13 @@ _j_timing = _sig("xp_jacobi_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), 
 _j_kernel_time = _sig("xp_jacobi_kernel_time", C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)])
 _rhs_ones = _sig("xp_rhs_ones_f32", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _f64p])
 _p_set_mlp = _sig("xp_pcg_set_mlp_preconditioner", C.c_int, [_vp, _vp])
+_p_set_ml = _sig("xp_pcg_set_multilevel", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int])
+_p_ml_info = _sig("xp_pcg_multilevel_info", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _i32p,
+                                                     np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                                     C.POINTER(C.c_double)])
+_p_ml_level = _sig("xp_pcg_multilevel_level", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp])
+_p_ml_phases = _sig("xp_pcg_multilevel_phases", C.c_int, [_vp, C.c_int, _f64p, C.POINTER(C.c_int)])
+_p_ml_apply = _sig("xp_pcg_multilevel_apply", C.c_int, [_vp, _vp, _vp])
 _m_num_blocks = _sig("xp_matrix_num_blocks", C.c_int, [_vp])
 _m_block_features = _sig("xp_matrix_block_features", C.c_int, [_vp, _f32p, _f32p])
 _mlp_create = _sig("xp_mlp_create", C.c_int, [C.c_ulonglong, C.POINTER(_vp)])
@@ -716,8 +724,9 @@ class DeviceArray:
 
 
 def stream_time(nbytes: int, mode: int = 0, grid: int = 2048, reps: int = 20) -> float:
-    """Average us per launch of the streaming read (mode 0) / copy (mode 1)
-    reference kernel over `nbytes`."""
+    """Average us per launch of the streaming reference kernel over `nbytes`:
+    read (mode 0), copy (1), non-temporal read (2), copy with non-temporal
+    stores (3), copy with non-temporal loads and stores (4)."""
     us = C.c_double()
     _check(_stream_time(nbytes, mode, grid, reps, C.byref(us)))
     return us.value
@@ -759,8 +768,9 @@ class Matrix:
                  _device_csr: "DeviceCsr" = None, _device_bitmap: "DeviceBitmapBlocks" = None):
         """blocking (with block_rows = 8): 0 blocks of 8 consecutive rows,
         1 greedy strength aggregates, 2 strength-ordered aggregates, 3 the
-        parallel strength aggregation on the device.  device_build: set the
-        matrix up on the device (one system)."""
+        parallel strength aggregation on the device, 4 the same with
+        distance-1 seeds.  device_build: set the matrix up on the device (one
+        system)."""
         if isinstance(systems, (Csr, _Shape)):
             systems = [systems]
         self.systems = list(systems)
@@ -1085,12 +1095,63 @@ class Pcg:
     def set_mlp_preconditioner(self, net: "Mlp") -> None:
         _check(_p_set_mlp(self.h, net.h))
 
+    def set_multilevel(self, coarse_max: int = 512, omega: float = 0.7, gamma: float = 1.5,
+                       coarse_blocking: int = 4) -> None:
+        """Multilevel preconditioner over the block aggregates, with the
+        installed blocks as the level-0 smoother (coarse_max <= 0: off);
+        coarse_blocking 3 or 4 aggregates the coarse levels."""
+        _check(_p_set_ml(self.h, coarse_max, omega, gamma, coarse_blocking))
+
+    def multilevel_info(self) -> dict:
+        """Levels (rows, stored entries; the last is the dense coarsest) and set-up ms."""
+        cap = 16
+        nl, ms = C.c_int(), C.c_double()
+        n, nnz = np.zeros(cap, np.int32), np.zeros(cap, np.int64)
+        _check(_p_ml_info(self.h, cap, C.byref(nl), n, nnz, C.byref(ms)))
+        k = min(nl.value, cap)
+        ph, cnt = np.zeros(3 * cap), C.c_int()
+        _check(_p_ml_phases(self.h, 3 * cap, ph, C.byref(cnt)))
+        return {"levels": nl.value, "n": n[:k].tolist(), "nnz": nnz[:k].tolist(), "setup_ms": ms.value,
+                "phase_ms": [round(v, 3) for v in ph[:min(cnt.value, 3 * cap)]]}
+
+    def multilevel_level(self, level: int):
+        """Level `level` >= 1 as a natural-order Csr (rows = blocks of the level
+        above) and the blocks of the level above (rows x 8, -1 padding)."""
+        info = self.multilevel_info()
+        n, nnz = info["n"][level], info["nnz"][level]
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        va = np.zeros(max(nnz, 1), np.float64)
+        slots = np.zeros(n * 8, np.int32)
+        _check(_p_ml_level(self.h, level, rp.ctypes.data_as(_vp), ci.ctypes.data_as(_vp), va.ctypes.data_as(_vp),
+                           slots.ctypes.data_as(_vp)))
+        return Csr(rp, ci[:nnz], va[:nnz]), slots.reshape(n, 8)
+
+    def multilevel_apply(self, r_solve) -> np.ndarray:
+        """z = M r then one V-cycle, r in level-0 solve order (n_pad FP32)."""
+        r = np.ascontiguousarray(r_solve, dtype=np.float32)
+        dr = DeviceArray.from_numpy(r)
+        dz = DeviceArray(r.size, np.float32)
+        try:
+            _check(_p_ml_apply(self.h, dr.ptr, dz.ptr))
+            return dz.numpy()
+        finally:
+            dr.close()
+            dz.close()
+
     def kernel_times(self, iters: int = 100):
         """Average device time (us) of each kernel of the iteration (HIP events)."""
         us = np.zeros(3)
         nk = C.c_int()
         _check(_p_kernel_times(self.h, iters, us, C.byref(nk)))
         return us[:nk.value]
+
+    def kernel_times_split(self, iters: int = 100):
+        """kernel_times split by iteration parity: (even, odd) averages."""
+        us = np.zeros(6)
+        nk = C.c_int()
+        _check(_p_kernel_times_split(self.h, iters, us, C.byref(nk)))
+        return us[:nk.value], us[3:3 + nk.value]
 
     def last_timing(self):
         ms, launches = C.c_double(), C.c_int()
