@@ -506,14 +506,22 @@ int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net);
  * aggregates and exact block factors, down to a dense coarsest level of at
  * most coarse_max rows (inverted once); coarse_blocking 3 aggregates the
  * coarse levels like level 0, 4 with distance-1 seeds (far fewer rounds on
- * their denser graphs).  Each PCG step then applies one
+ * their denser graphs), and levels of at most consecutive_rows rows take
+ * consecutive rows (their natural order follows the blocks' smallest rows,
+ * so consecutive rows are neighbours).  Each PCG step then applies one
  * symmetric V-cycle: z = omega M r, coarse correction scaled by gamma, one
  * post-smoothing step; FP32 vectors inside the cycle.  Needs the accurate
  * iteration on one system set up on the device with block_rows = 8 and
- * blocking 3, and a block preconditioner already installed; coarse_max <= 0
- * removes the hierarchy.  A fallback to Jacobi turns it off with the
+ * blocking 3 or 4, and a block preconditioner already installed.  A fallback to Jacobi turns it off with the
  * blocks. */
-int xp_pcg_set_multilevel(xp_pcg* s, int coarse_max, double omega, double gamma, int coarse_blocking);
+typedef struct {
+    int coarse_max;       /* rows of the dense coarsest level, at most (8..4096); <= 0 removes the hierarchy */
+    double omega;         /* smoother damping, 0 < omega < 2 */
+    double gamma;         /* coarse-correction scale, 0 < gamma < 4 */
+    int coarse_blocking;  /* aggregation of the coarse levels: 3 or 4 (see xp_matrix_opts.blocking) */
+    int consecutive_rows; /* coarse levels of at most this many rows take 8 consecutive rows per block */
+} xp_multilevel_opts;
+int xp_pcg_set_multilevel(xp_pcg* s, const xp_multilevel_opts* opts);
 /* Levels of the hierarchy (*nlev, 0 when none): rows and stored entries of
  * the first `cap` levels (level 0 is the solver's matrix, the last the dense
  * coarsest one) and the set-up time in ms. */

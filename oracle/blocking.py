@@ -177,9 +177,10 @@ def _better(a, b, t):
     return b < 0 or t[a] > t[b] or (t[a] == t[b] and a < b)
 
 
-def seeds_of_round(row_ptr, col_idx, t, elig, owner):
+def seeds_of_round(row_ptr, col_idx, t, elig, owner, seed_dist=2):
     """Unassigned eligible rows that precede every unassigned eligible row
-    within distance 2 (neighbours: the columns of a row's entries)."""
+    within distance 2 (neighbours: the columns of a row's entries), or
+    within distance 1 when seed_dist is 1 (rule 4)."""
     n = len(row_ptr) - 1
     free = [elig[i] and owner[i] < 0 for i in range(n)]
     m1 = [-1] * n
@@ -196,6 +197,8 @@ def seeds_of_round(row_ptr, col_idx, t, elig, owner):
             continue
         best = m1[i]
         for o in range(row_ptr[i], row_ptr[i + 1]):
+            if seed_dist == 1:
+                break
             c = col_idx[o]
             if c != i and m1[c] >= 0 and _better(m1[c], best, t):
                 best = m1[c]
@@ -238,10 +241,10 @@ def growth_bids(row_ptr, col_idx, s, t, elig, owner, size):
     return {B: kj[1] for B, kj in best_of_block.items()}
 
 
-def _rounds(row_ptr, col_idx, s, t, elig, owner, size, members, log=None):
+def _rounds(row_ptr, col_idx, s, t, elig, owner, size, members, log=None, seed_dist=2):
     n = len(row_ptr) - 1
     while any(elig[i] and owner[i] < 0 for i in range(n)):
-        for i in seeds_of_round(row_ptr, col_idx, t, elig, owner):
+        for i in seeds_of_round(row_ptr, col_idx, t, elig, owner, seed_dist):
             owner[i] = i
             size[i] = 1
             members[i] = [i]
@@ -255,7 +258,7 @@ def _rounds(row_ptr, col_idx, s, t, elig, owner, size, members, log=None):
                 members[B].append(j)
 
 
-def parallel_blocking(row_ptr, col_idx, values, log=None):
+def parallel_blocking(row_ptr, col_idx, values, log=None, seed_dist=2):
     """Rule 3: blocks as a list of row lists (ascending rows, ordered by
     first row).
       rounds: seeds = unassigned rows that precede every unassigned row
@@ -269,14 +272,16 @@ def parallel_blocking(row_ptr, col_idx, values, log=None):
               in order of their smallest row, cut into consecutive runs of
               PACK_SEGMENT pieces; within a run each piece goes into the most
               recent of the run's last PACK_WINDOW packed blocks with room,
-              else a new one (packed blocks never mix runs)."""
+              else a new one (packed blocks never mix runs).
+    seed_dist=1 is rule 4: seeds need only precede their distance-1
+    neighbours (the multilevel hierarchy's coarse levels)."""
     n = len(row_ptr) - 1
     if n == 0:
         return []
     s = strengths(row_ptr, col_idx, values)
     t = totals(row_ptr, col_idx, s)
     owner, size, members = [-1] * n, [0] * n, [None] * n
-    _rounds(row_ptr, col_idx, s, t, [True] * n, owner, size, members, log)
+    _rounds(row_ptr, col_idx, s, t, [True] * n, owner, size, members, log, seed_dist)
     pool = [size[owner[r]] < 8 for r in range(n)]
     for r in range(n):
         if pool[r]:
@@ -284,7 +289,7 @@ def parallel_blocking(row_ptr, col_idx, values, log=None):
     for B in range(n):
         if size[B] < 8:
             size[B] = 0
-    _rounds(row_ptr, col_idx, s, t, pool, owner, size, members, log)
+    _rounds(row_ptr, col_idx, s, t, pool, owner, size, members, log, seed_dist)
     full = [sorted(members[B]) for B in range(n) if size[B] == 8]
     pieces = sorted((sorted(members[B]) for B in range(n) if 0 < size[B] < 8), key=lambda b: b[0])
     packed = []

@@ -260,42 +260,84 @@ __global__ void k_seed_pick(int n, const int* __restrict__ rp, const int* __rest
 // room by the strength its entries put into the block (summed in CSR order
 // from the block's first entry) over t_j, picks the best (ties: smaller
 // block id) and posts (FP32 score, -j) to that block with an integer max.
-__global__ void k_grow_bid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+// One pass over the row keeps the distinct blocks met so far, with their
+// running sums, in LDS (kBidCap per thread): every block's sum takes its
+// entries in CSR order, as the definition says.  A row that meets more than
+// kBidCap blocks with room scans its entries once per block instead.
+constexpr int kBidCap = 16;
+__global__ __launch_bounds__(kRowThreads) void k_grow_bid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                            const double* __restrict__ va, const double* __restrict__ dis,
                            const double* __restrict__ t, const uint8_t* __restrict__ elig,
                            const int* __restrict__ owner, const int* __restrict__ size,
                            unsigned long long* __restrict__ bid, const int* __restrict__ prev_taken) {
 #pragma clang fp contract(off)
+    __shared__ int lb[kBidCap * kRowThreads];
+    __shared__ double ls[kBidCap * kRowThreads];
     // a step after one that took no row would find the same bids: nothing to do
     if (prev_taken != nullptr && *prev_taken == 0) return;
-    const int j = static_cast<int>(blockIdx.x) * kRowThreads + static_cast<int>(threadIdx.x);
+    const int tid = static_cast<int>(threadIdx.x);
+    const int j = static_cast<int>(blockIdx.x) * kRowThreads + tid;
     if (j >= n || !elig[j] || owner[j] >= 0) return;
     const int b = rp[j], e = rp[j + 1];
     const double dj = dis[j], tj = t[j];
     int best_b = -1;
     double best_f = -1.0;
+    int* blk = lb + tid;
+    double* sum = ls + tid;
+    int d = 0;
+    bool over = false;
     for (int o = b; o < e; ++o) {
         const int c = ci[o];
         if (c == j) continue;
         const int B = owner[c];
         if (B < 0 || size[B] >= 8) continue;
-        bool seen = false;
-        for (int o2 = b; o2 < o && !seen; ++o2) {
-            const int c2 = ci[o2];
-            seen = c2 != j && owner[c2] == B;
+        const double sv = fabs(va[o]) * dj * dis[c];
+        int k = 0;
+        while (k < d && blk[k * kRowThreads] != B) ++k;
+        if (k < d) {
+            sum[k * kRowThreads] += sv;
+        } else if (d < kBidCap) {
+            blk[d * kRowThreads] = B;
+            sum[d * kRowThreads] = 0.0 + sv;
+            ++d;
+        } else {
+            over = true;
+            break;
         }
-        if (seen) continue;
-        double sum = 0.0;
-        for (int o2 = o; o2 < e; ++o2) {
-            const int c2 = ci[o2];
-            if (c2 == j || owner[c2] != B) continue;
-            const double sv = fabs(va[o2]) * dj * dis[c2];
-            sum += sv;
+    }
+    if (!over) {
+        for (int k = 0; k < d; ++k) {
+            const int B = blk[k * kRowThreads];
+            const double f = tj > 0.0 ? sum[k * kRowThreads] / tj : 0.0;
+            if (f > best_f || (f == best_f && B < best_b)) {
+                best_b = B;
+                best_f = f;
+            }
         }
-        const double f = tj > 0.0 ? sum / tj : 0.0;
-        if (f > best_f || (f == best_f && B < best_b)) {
-            best_b = B;
-            best_f = f;
+    } else {
+        for (int o = b; o < e; ++o) {
+            const int c = ci[o];
+            if (c == j) continue;
+            const int B = owner[c];
+            if (B < 0 || size[B] >= 8) continue;
+            bool seen = false;
+            for (int o2 = b; o2 < o && !seen; ++o2) {
+                const int c2 = ci[o2];
+                seen = c2 != j && owner[c2] == B;
+            }
+            if (seen) continue;
+            double acc = 0.0;
+            for (int o2 = o; o2 < e; ++o2) {
+                const int c2 = ci[o2];
+                if (c2 == j || owner[c2] != B) continue;
+                const double sv = fabs(va[o2]) * dj * dis[c2];
+                acc += sv;
+            }
+            const double f = tj > 0.0 ? acc / tj : 0.0;
+            if (f > best_f || (f == best_f && B < best_b)) {
+                best_b = B;
+                best_f = f;
+            }
         }
     }
     if (best_b < 0) return;

@@ -3417,7 +3417,9 @@ void galerkin(const xp_matrix& c, xp::DevBuf<int>& crp, xp::DevBuf<int>& cci, xp
     HIPX(hipStreamSynchronize(s));  // before the temporaries go
 }
 
-void build_multilevel(xp_pcg& S, int coarse_max, double omega, double gamma, int coarse_blocking) {
+void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
+    const int coarse_max = mo.coarse_max, coarse_blocking = mo.coarse_blocking;
+    const double omega = mo.omega, gamma = mo.gamma;
     xp_matrix& m0 = *S.m;
     if (!S.opts.accurate) throw std::runtime_error("the multilevel preconditioner runs in the accurate iteration");
     if (!m0.device_built || m0.block_rows != 8 || m0.blocking < 3 || m0.nsys != 1)
@@ -3486,7 +3488,7 @@ void build_multilevel(xp_pcg& S, int coarse_max, double omega, double gamma, int
         }
         xp_matrix_opts o{};
         o.block_rows = 8;
-        o.blocking = coarse_blocking;
+        o.blocking = units <= mo.consecutive_rows ? 0 : coarse_blocking;
         o.sigma = 64;
         std::unique_ptr<xp_matrix> next = make_matrix_device(units, crp.p, cci.p, cva.p, &o);
         if (next->bad_row >= 0) throw std::runtime_error("multilevel: a coarse matrix has a non-positive diagonal");
@@ -3516,16 +3518,13 @@ void build_multilevel(xp_pcg& S, int coarse_max, double omega, double gamma, int
 
 }  // namespace
 
-int xp_pcg_set_multilevel(xp_pcg* s, int coarse_max, double omega, double gamma, int coarse_blocking) {
+int xp_pcg_set_multilevel(xp_pcg* s, const xp_multilevel_opts* opts) {
     return guarded([&] {
-        if (s == nullptr) throw std::runtime_error("null solver");
-        if (coarse_max <= 0) {
-            s->mg.reset();
-            s->invalidate_graph();
-            return;
-        }
+        if (s == nullptr || opts == nullptr) throw std::runtime_error("null argument");
         s->mg.reset();
-        build_multilevel(*s, coarse_max, omega, gamma, coarse_blocking);
+        s->invalidate_graph();
+        if (opts->coarse_max <= 0) return;
+        build_multilevel(*s, *opts);
     });
 }
 

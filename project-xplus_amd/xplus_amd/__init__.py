@@ -162,7 +162,12 @@ _j_timing = _sig("xp_jacobi_last_timing", C.c_int, [_vp, C.POINTER(C.c_double), 
 _j_kernel_time = _sig("xp_jacobi_kernel_time", C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)])
 _rhs_ones = _sig("xp_rhs_ones_f32", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _f64p])
 _p_set_mlp = _sig("xp_pcg_set_mlp_preconditioner", C.c_int, [_vp, _vp])
-_p_set_ml = _sig("xp_pcg_set_multilevel", C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int])
+class MultilevelOpts(C.Structure):
+    _fields_ = [("coarse_max", C.c_int), ("omega", C.c_double), ("gamma", C.c_double),
+                ("coarse_blocking", C.c_int), ("consecutive_rows", C.c_int)]
+
+
+_p_set_ml = _sig("xp_pcg_set_multilevel", C.c_int, [_vp, C.POINTER(MultilevelOpts)])
 _p_ml_info = _sig("xp_pcg_multilevel_info", C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), _i32p,
                                                      np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                                      C.POINTER(C.c_double)])
@@ -1096,11 +1101,13 @@ class Pcg:
         _check(_p_set_mlp(self.h, net.h))
 
     def set_multilevel(self, coarse_max: int = 512, omega: float = 0.7, gamma: float = 1.5,
-                       coarse_blocking: int = 4) -> None:
+                       coarse_blocking: int = 4, consecutive_rows: int = 0) -> None:
         """Multilevel preconditioner over the block aggregates, with the
         installed blocks as the level-0 smoother (coarse_max <= 0: off);
-        coarse_blocking 3 or 4 aggregates the coarse levels."""
-        _check(_p_set_ml(self.h, coarse_max, omega, gamma, coarse_blocking))
+        coarse_blocking 3 or 4 aggregates the coarse levels, those of at most
+        consecutive_rows rows take consecutive rows."""
+        o = MultilevelOpts(coarse_max, omega, gamma, coarse_blocking, consecutive_rows)
+        _check(_p_set_ml(self.h, C.byref(o)))
 
     def multilevel_info(self) -> dict:
         """Levels (rows, stored entries; the last is the dense coarsest) and set-up ms."""

@@ -274,6 +274,46 @@ def test_parallel_rule_seeds_brute_force():
         assert BK.seeds_of_round(rp, ci, t, elig, owner) == want
 
 
+def test_rule4_seeds_brute_force():
+    """Rule 4 (seed_dist 1): every round's seeds are the free rows that
+    precede every free row they reach in one step (brute force); on the same
+    state they include every rule-3 seed."""
+    A = random_spd(70, 13, density=0.08)
+    a = csr(A)
+    rp, ci = list(a.row_ptr), list(a.col_idx)
+    s = BK.strengths(rp, ci, list(a.values))
+    t = BK.totals(rp, ci, s)
+    rng = np.random.default_rng(5)
+    for trial in range(5):
+        owner = [int(v) for v in np.where(rng.random(70) < 0.3 * trial / 4, 0, -1)]
+        elig = [True] * 70
+        nb = {i: set(ci[rp[i]:rp[i + 1]]) - {i} for i in range(70)}
+        want = []
+        for i in range(70):
+            if owner[i] >= 0:
+                continue
+            rivals = [j for j in nb[i] if owner[j] < 0]
+            if all(t[i] > t[j] or (t[i] == t[j] and i < j) for j in rivals):
+                want.append(i)
+        got = BK.seeds_of_round(rp, ci, t, elig, owner, seed_dist=1)
+        assert got == want
+        assert set(BK.seeds_of_round(rp, ci, t, elig, owner)) <= set(got)
+
+
+def test_rule4_partitions_and_takes_fewer_rounds_on_dense_rows():
+    """Rule 4 still partitions the rows into blocks of at most 8; on a dense
+    matrix its first round seeds more blocks than rule 3's."""
+    for a in (csr(random_spd(90, 11)), X.gen_fem2d(26, 1.0, 3, 1e3, 0.3, 5), csr(random_spd(120, 17, density=0.25))):
+        check_partition(BK.parallel_blocking(list(a.row_ptr), list(a.col_idx), list(a.values), seed_dist=1), a.n)
+    a = csr(random_spd(120, 17, density=0.25))
+    rp, ci = list(a.row_ptr), list(a.col_idx)
+    t = BK.totals(rp, ci, BK.strengths(rp, ci, list(a.values)))
+    free = [-1] * a.n
+    n1 = len(BK.seeds_of_round(rp, ci, t, [True] * a.n, free, seed_dist=1))
+    n2 = len(BK.seeds_of_round(rp, ci, t, [True] * a.n, free))
+    assert n1 > n2
+
+
 def test_parallel_rule_growth_steps_replayed():
     """Every growth step of a full run, replayed from the owner map it chose
     from: each taken row bid for its block with the best score among the

@@ -1,8 +1,11 @@
-"""The matrix set-up on the device (xp_matrix_create_device, blocking 3).
+"""The matrix set-up on the device (xp_matrix_create_device, blocking 3 and 4).
 
 Bars (all exact: the set-up is integer / byte work and fixed-order FP64):
   * the parallel strength aggregation equals oracle/blocking.py's
-    parallel_blocking block for block (rows, order, padding);
+    parallel_blocking block for block (rows, order, padding), with
+    distance-2 seeds (blocking 3) and distance-1 seeds (blocking 4), on
+    sparse and on dense rows (more neighbouring blocks than the growth
+    bids' LDS list holds: the fallback scan);
   * the SELL arrays built on the device equal the host builder's
     (xp_sell_build_host) for the same CSR and blocks, entry for entry:
     permutation, slice pointers and bases, FP32 values, FP32 and 16-bit
@@ -63,17 +66,19 @@ CASES = {
     "diff3d": lambda: X.gen_diffusion3d(12, 1.5, 9),
     "random": lambda: random_spd(600, 3, 0.01),
     "isolated": isolated_rows,
+    "dense": lambda: random_spd(400, 11, 0.08),
 }
 
 
+@pytest.mark.parametrize("blocking", [3, 4])
 @pytest.mark.parametrize("case", list(CASES))
-def test_device_aggregation_matches_oracle(gpu, case):
+def test_device_aggregation_matches_oracle(gpu, case, blocking):
     a = CASES[case]()
-    m = X.Matrix(a, block_rows=8, blocking=3, keep_lo=True)
+    m = X.Matrix(a, block_rows=8, blocking=blocking, keep_lo=True)
     got = [[int(r) for r in row if r >= 0] for row in m.block_rows()]
-    want = BK.parallel_blocking(list(a.row_ptr), list(a.col_idx), list(a.values))
+    want = BK.parallel_blocking(list(a.row_ptr), list(a.col_idx), list(a.values), seed_dist=2 if blocking == 3 else 1)
     st = m.setup_stats()
-    print(f"{case}: n={a.n} blocks {len(got)} oracle {len(want)} stats {st}", flush=True)
+    print(f"{case} blocking {blocking}: n={a.n} blocks {len(got)} oracle {len(want)} stats {st}", flush=True)
     assert st["device_built"] and st["blocks"] == len(want)
     assert got == want
 
