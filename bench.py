@@ -604,6 +604,7 @@ METHODS = {
     "refined_jacobi": ({"block_rows": 1, "keep_lo": True}, {"fused": None}, "refined"),
     "accurate_jacobi": ({"block_rows": 1, "keep_lo": True}, {"accurate": True}, "reliable"),
     "learned": ({"block_rows": 8, "keep_lo": True}, {"accurate": True}, "reliable"),
+    "learned_multilevel": ({"block_rows": 8, "keep_lo": True}, {"accurate": True}, "reliable"),
 }
 
 
@@ -615,12 +616,13 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
     mopts, popts, run = METHODS[name]
     mkw = dict(mopts)
     mkw.update({"sigma": layout["sigma"], "grid": layout["grid"], "spmv_variant": layout["spmv_variant"]})
-    if name == "learned":
+    learned = name.startswith("learned")
+    if learned:
         mkw["blocking"] = args.blocking
     pkw = dict(popts)
     if pkw.get("fused", 0) is None:
         pkw["fused"] = fused_refined
-    if name == "learned":
+    if learned:
         pkw["stall_limit"] = args.stall_limit
     xout = X.DeviceArray(a.n, np.float64)
     info = {}
@@ -633,6 +635,9 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
         s.set_rhs_device(d.b.ptr)
         if net is not None:
             s.set_mlp_preconditioner(net)
+        if name == "learned_multilevel":
+            s.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0)
+            info["multilevel"] = s.multilevel_info()
         X.synchronize()
         t1 = time.perf_counter()
         if run == "run":
@@ -671,9 +676,16 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
         out["replacements"] = info["ret"][1]
         out["iterations"] = info["ret"][2]
         out["fallback"] = st["fallback"]
-    if name == "learned":
+    if learned:
         out["setup_stats"] = info["setup_stats"]
         out["matrix"] = info["matrix"]
+    if "multilevel" in info:
+        ml = info["multilevel"]
+        out["multilevel"] = {"levels": ml["levels"], "n": ml["n"], "nnz": ml["nnz"],
+                             "hierarchy_setup_ms": round(ml["setup_ms"], 3),
+                             "phase_ms": [round(float(v), 3) for v in ml["phase_ms"]],
+                             "coarse_max": args.ml_coarse_max, "omega": args.ml_omega, "gamma": args.ml_gamma,
+                             "coarse_blocking": args.ml_coarse_blocking}
     out["us_per_iteration"] = round(solve * 1000.0 / max(out["iterations"], 1), 3)
     log(f"{name}: {out}")
     return out
@@ -702,13 +714,15 @@ def workload_c2(X, args, cpu_proc) -> int:
     layouts = tune_layouts(X, args)
     res = {}
     fused_refined = layouts["plain_fused"]["us_per_iter"] < layouts["plain_unfused"]["us_per_iter"]
-    for name in ("learned", "accurate_jacobi", "plain_unfused", "plain_fused", "refined_jacobi"):
+    for name in ("learned_multilevel", "learned", "accurate_jacobi", "plain_unfused", "plain_fused",
+                 "refined_jacobi"):
         lay = layouts["plain_fused" if name == "refined_jacobi" and fused_refined else
-                      "plain_unfused" if name == "refined_jacobi" else name]
-        res[name] = time_to_solution(X, d, a, name, lay, args, net=net if name == "learned" else None,
+                      "plain_unfused" if name == "refined_jacobi" else
+                      "learned" if name == "learned_multilevel" else name]
+        res[name] = time_to_solution(X, d, a, name, lay, args, net=net if name.startswith("learned") else None,
                                      fused_refined=fused_refined)
-    learned, acc = res["learned"], res["accurate_jacobi"]
-    for name in ("learned", "accurate_jacobi", "refined_jacobi"):
+    learned, acc = res["learned_multilevel"], res["accurate_jacobi"]
+    for name in ("learned_multilevel", "learned", "accurate_jacobi", "refined_jacobi"):
         if not res[name]["true_rel_residual"] <= 1.01e-6:
             raise RuntimeError(f"{name} missed 1e-6: {res[name]}")
     plain_form = min(("plain_unfused", "plain_fused"), key=lambda k: res[k]["total_ms"])
@@ -769,18 +783,24 @@ def workload_c2(X, args, cpu_proc) -> int:
         "config": {"workload": f"thermal-FEM P1 {args.mesh}x{args.mesh} nodes, n={a.n}, nnz={a.nnz}, "
                                f"time from the CSR and b in HBM to x with true relative residual 1e-6",
                    "n": a.n, "nnz": a.nnz,
-                   "method": "learned 8x8 block-Jacobi PCG: blocks = parallel strength aggregates of <= 8 rows "
+                   "method": "PCG preconditioned by a multilevel V-cycle whose level-0 smoother is the learned "
+                             "8x8 block-Jacobi preconditioner: blocks = parallel strength aggregates of <= 8 rows "
                              "computed on the device, blocks from a BF16 MFMA MLP (which starts as, and on these "
-                             "aggregates stays within 1% of, the exact block inverses: the gain is the block "
-                             "structure over Jacobi, not the network over exact inverses); FP32 r/z and FP32 "
-                             "hi + 16-bit-remainder matrix values, FP64-accumulated SpMV, FP64 p/Ap/x, FP64 "
-                             "reductions, residual replacement on the FP64 true residual, true-residual "
-                             "stagnation fallback to Jacobi; value = device set-up + solve",
+                             "aggregates stays within 1% of, the exact block inverses); coarse levels = Galerkin "
+                             "products over the aggregates, set up on the device with their own aggregates and "
+                             "exact block factors, dense coarsest inverse; one symmetric FP32 V-cycle per "
+                             "iteration; FP32 r/z and FP32 hi + 16-bit-remainder matrix values, FP64-accumulated "
+                             "SpMV, FP64 p/Ap/x, FP64 reductions, residual replacement on the FP64 true residual, "
+                             "true-residual stagnation fallback to Jacobi; value = device set-up (matrix, blocks, "
+                             "hierarchy) + solve; 'learned' is the same without the coarse levels (the learned "
+                             "block-Jacobi PCG alone)",
                    "network_training": tinfo, "layouts_tuned_on": f"training matrix seed {TUNE_SEED}",
                    "layouts": layouts},
         "true_rel_residual": learned["true_rel_residual"],
         "iterations": learned["iterations"],
-        "learned": learned,
+        "learned_multilevel": learned,
+        "learned": res["learned"],
+        "speedup_learned_block_jacobi_vs_accurate_jacobi": round(acc["total_ms"] / res["learned"]["total_ms"], 3),
         "accurate_jacobi": acc,
         "refined_jacobi": res["refined_jacobi"],
         "plain_forms": {k: res[k] for k in ("plain_unfused", "plain_fused")},
@@ -841,6 +861,10 @@ def main() -> int:
     # aggregates, FP32 iteration: 0.3 +2%, 1.0 +10%; parallel aggregates,
     # accurate iteration: 0.05 +7%, 0.5 +14%)
     ap.add_argument("--lam", type=float, default=0.0)
+    ap.add_argument("--ml-coarse-max", type=int, default=512, help="multilevel: rows of the dense coarsest level")
+    ap.add_argument("--ml-omega", type=float, default=0.7, help="multilevel: smoother damping")
+    ap.add_argument("--ml-gamma", type=float, default=1.5, help="multilevel: coarse-correction scale")
+    ap.add_argument("--ml-coarse-blocking", type=int, default=4, help="multilevel: coarse aggregation rule (3, 4)")
     ap.add_argument("--cpu-golden-worker", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_golden_worker:
