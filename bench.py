@@ -428,15 +428,17 @@ def workload_c3(X, args) -> int:
         # replacement when r^T r dropped 1e4-fold (delta 0.01)
         ml, _ = learned_matrix(X, a, args, tune=True, keep64=True)
         blocks = {}
-        for name in ("exact", "learned"):
+        for name in ("exact", "learned", "learned_multilevel"):
             q = X.Pcg(ml, rel_tol=1e-10, accurate=True, stall_limit=0, max_iters=3 * iters)
             q.set_rhs(a.b, a.x0)
             X.synchronize()
             t0 = time.perf_counter()
-            if name == "learned":
+            if name.startswith("learned"):
                 q.set_mlp_preconditioner(net)
             else:
                 q.set_preconditioner(1)
+            if name == "learned_multilevel":
+                q.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0)
             X.synchronize()
             setup_ms = (time.perf_counter() - t0) * 1000.0
             b_ms, (rel3, repl3, iters3) = timed(X, lambda: q.run_reliable(1e-10, 0.01), 0, 1)
@@ -453,7 +455,8 @@ def workload_c3(X, args) -> int:
                             "true_rel_residual": float(rel2[0]), "delta": 0.1},
                "accurate_delta_0.01": {"ms": round(acc01_ms, 1), "iterations": iters4, "replacements": repl4,
                                        "true_rel_residual": float(rel4[0])},
-               "exact_blocks_accurate": blocks["exact"], "learned_accurate": blocks["learned"]}
+               "exact_blocks_accurate": blocks["exact"], "learned_accurate": blocks["learned"],
+               "learned_multilevel_accurate": blocks["learned_multilevel"]}
         log(f"c3: {row}")  # one line per grid: the run prints as it goes
         out.append(row)
     print(json.dumps({"workload": "FP64-refined thermal-FEM to true relative residual 1e-10", "unit": "ms",
@@ -649,6 +652,8 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
             r = s.run_reliable(1e-6, 0.0)
         s.x_device(xout.ptr)
         t2 = time.perf_counter()
+        if name == "learned_multilevel":
+            info["vcycle_us"] = s.multilevel_time(50)    # after the timed region
         info["status"] = s.status()
         info["ret"] = r
         info["matrix"] = m.info()
@@ -685,7 +690,8 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
                              "hierarchy_setup_ms": round(ml["setup_ms"], 3),
                              "phase_ms": [round(float(v), 3) for v in ml["phase_ms"]],
                              "coarse_max": args.ml_coarse_max, "omega": args.ml_omega, "gamma": args.ml_gamma,
-                             "coarse_blocking": args.ml_coarse_blocking}
+                             "coarse_blocking": args.ml_coarse_blocking,
+                             "vcycle_us": round(info["vcycle_us"], 3)}
     out["us_per_iteration"] = round(solve * 1000.0 / max(out["iterations"], 1), 3)
     log(f"{name}: {out}")
     return out

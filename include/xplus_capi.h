@@ -537,6 +537,10 @@ int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_i
 /* z = M r followed by one V-cycle, for device vectors of level 0 (solve
  * order, n_pad entries). */
 int xp_pcg_multilevel_apply(xp_pcg* s, const float* r, float* z);
+/* Average device microseconds of one V-cycle (with the block product) over
+ * `reps` cycles launched back to back on the solver's residual; overwrites
+ * the solver's z (the next run recomputes it). */
+int xp_pcg_multilevel_time(xp_pcg* s, int reps, double* us);
 /* The device-computed features of a block_rows = 8 matrix, in natural block
  * order (xp_matrix_num_blocks of them): feat nblocks x 48, scale nblocks x 8
  * (the same quantities as xp_blockset_copy's feat and scale). */

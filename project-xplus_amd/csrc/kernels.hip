@@ -216,8 +216,28 @@ __device__ __forceinline__ float sell_row_f32(const SellView& A, int slice, int 
     size_t e = e0;
     xdiag = x[row];
     float acc = A.val[e] * xdiag;
+    int j = 1;
+    // wide slices (the dense coarse levels of the multilevel hierarchy):
+    // chunks of kChunk slots whose column loads, gathers and value loads are
+    // all issued before the chunk's FMAs (the same order of summation)
+    constexpr int kChunk = 8;
+    for (; j + kChunk <= w; j += kChunk) {
+        int c[kChunk];
+        float v[kChunk], xv[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) {
+            const size_t ek = e + static_cast<size_t>(k + 1) * kC;
+            c[k] = IDX16 ? base + static_cast<int>(A.col16[ek]) : A.col32[ek];
+            v[k] = A.val[ek];
+        }
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) xv[k] = x[c[k]];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) acc = fmaf(v[k], xv[k], acc);
+        e += static_cast<size_t>(kChunk) * kC;
+    }
 #pragma unroll 4
-    for (int j = 1; j < w; ++j) {
+    for (; j < w; ++j) {
         e += kC;
         const int c = IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e];
         acc = fmaf(A.val[e], x[c], acc);
@@ -1744,12 +1764,13 @@ __global__ __launch_bounds__(kBlock, 1) void k_op_finish(Launch L, int slot0, in
 // Vectors, values and sums are FP32 (a preconditioner needs no more); every
 // kernel returns at once when the system has stopped.
 
-// dst[down[q]] = sum over the real rows of block q of (r - c (A z))_row.
+// dst[down[q]] = sum over the rows of block q of (r - c (A z))_row (padding
+// rows add 0: r and z are 0 there).
 template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_mg_restrict(Launch L, SellView A, const SysState* st,
                                                         const float* __restrict__ r, const float* __restrict__ z,
-                                                        float c, const int* __restrict__ perm,
-                                                        const int* __restrict__ down, float* __restrict__ dst) {
+                                                        float c, const int* __restrict__ down,
+                                                        float* __restrict__ dst) {
     if (st != nullptr && !st[0].active) return;
     const WorkItem w = work_item(L, logical_block());
     if (w.sys < 0) return;
@@ -1758,7 +1779,8 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_restrict(Launch L, SellView A,
         const int row = sl * kC + lane;
         float zd;
         const float az = sell_row_f32<IDX16, V>(A, sl, lane, z, zd);
-        float t = perm[row] >= 0 ? fmaf(-c, az, r[row]) : 0.0f;
+        // padding rows: r = 0 and z = 0 (identity rows, M identity there), so t = 0
+        float t = fmaf(-c, az, r[row]);
         t = dpp_add<0x141>(t);  // the block's 8 lanes: pairs (i, 7 - i),
         t = dpp_add<0x4E>(t);   // then (i, i ^ 2),
         t = dpp_add<0xB1>(t);   // then (i, i ^ 1)
@@ -2547,12 +2569,12 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
     } while (0)
     // level 0: z0 = M r from the update kernel; the pre-smoothed approximation is omega z0
     float* first_dst = P.nlev > 1 ? P.lv[1].r : P.rc;
-    XP_MG_SELL(L0, f.A, k_mg_restrict, L0, f.A, st, r0, z0, P.omega, f.perm, f.down, first_dst);
+    XP_MG_SELL(L0, f.A, k_mg_restrict, L0, f.A, st, r0, z0, P.omega, f.down, first_dst);
     for (int l = 1; l < P.nlev; ++l) {
         const MgLevelView& v = P.lv[l];
         hipLaunchKernelGGL(k_mg_smooth, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, st, v.M, v.r, P.omega, v.z);
         float* dst = l + 1 < P.nlev ? P.lv[l + 1].r : P.rc;
-        XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.perm, v.down, dst);
+        XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.down, dst);
     }
     hipLaunchKernelGGL(k_mg_coarse, dim3((P.nc + kWaves - 1) / kWaves), dim3(kBlock), 0, L0.stream, st, P.nc, P.cinv,
                        P.rc, P.ec);

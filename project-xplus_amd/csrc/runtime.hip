@@ -3597,6 +3597,33 @@ int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_i
     });
 }
 
+// Average device time of one V-cycle (the block product M r included) over
+// `reps` back-to-back cycles on the solver's residual, between HIP events on
+// the solver's stream.  Overwrites z and the rz partials; the next run
+// recomputes both.
+int xp_pcg_multilevel_time(xp_pcg* s, int reps, double* us) {
+    return guarded([&] {
+        if (s == nullptr || us == nullptr || reps <= 0) throw std::runtime_error("bad argument");
+        if (!s->mg_on()) throw std::runtime_error("no multilevel hierarchy in use");
+        const xp::MgPlan P = s->mg_plan();
+        xp::Launch L = s->m->launch(s->stream);
+        for (int k = 0; k < 3; ++k) {
+            xp::launch_mg_block_apply(L, s->blockM.p, s->r.p, s->z.p);
+            xp::launch_vcycle(P, L, nullptr, s->r.p, s->z.p);
+        }
+        HIPX(hipEventRecord(s->ev0, s->stream));
+        for (int k = 0; k < reps; ++k) {
+            xp::launch_mg_block_apply(L, s->blockM.p, s->r.p, s->z.p);
+            xp::launch_vcycle(P, L, nullptr, s->r.p, s->z.p);
+        }
+        HIPX(hipEventRecord(s->ev1, s->stream));
+        HIPX(hipEventSynchronize(s->ev1));
+        float ms = 0.0f;
+        HIPX(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        *us = 1000.0 * static_cast<double>(ms) / reps;
+    });
+}
+
 // z = V(r) for device vectors r, z of level 0 (solve order, n_pad): the
 // block product M r, then one V-cycle.  For tests and inspection.
 int xp_pcg_multilevel_apply(xp_pcg* s, const float* r, float* z) {

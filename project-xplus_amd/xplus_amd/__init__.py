@@ -174,6 +174,7 @@ _p_ml_info = _sig("xp_pcg_multilevel_info", C.c_int, [_vp, C.c_int, C.POINTER(C.
 _p_ml_level = _sig("xp_pcg_multilevel_level", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp])
 _p_ml_phases = _sig("xp_pcg_multilevel_phases", C.c_int, [_vp, C.c_int, _f64p, C.POINTER(C.c_int)])
 _p_ml_apply = _sig("xp_pcg_multilevel_apply", C.c_int, [_vp, _vp, _vp])
+_p_ml_time = _sig("xp_pcg_multilevel_time", C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)])
 _m_num_blocks = _sig("xp_matrix_num_blocks", C.c_int, [_vp])
 _m_block_features = _sig("xp_matrix_block_features", C.c_int, [_vp, _f32p, _f32p])
 _mlp_create = _sig("xp_mlp_create", C.c_int, [C.c_ulonglong, C.POINTER(_vp)])
@@ -1133,6 +1134,12 @@ class Pcg:
         _check(_p_ml_level(self.h, level, rp.ctypes.data_as(_vp), ci.ctypes.data_as(_vp), va.ctypes.data_as(_vp),
                            slots.ctypes.data_as(_vp)))
         return Csr(rp, ci[:nnz], va[:nnz]), slots.reshape(n, 8)
+
+    def multilevel_time(self, reps: int = 50) -> float:
+        """Average device us of one V-cycle (block product included)."""
+        us = C.c_double()
+        _check(_p_ml_time(self.h, reps, C.byref(us)))
+        return us.value
 
     def multilevel_apply(self, r_solve) -> np.ndarray:
         """z = M r then one V-cycle, r in level-0 solve order (n_pad FP32)."""
