@@ -388,6 +388,19 @@ def profile_run(X, a, args) -> int:
     ml, _ = learned_matrix(X, a, args, tune=False)
     out = accurate_solve(X, a, ml, args, net=net, kernels=True)
     out["matrix"] = ml.info()
+    # the value's path: the same blocks as the smoother of the multilevel cycle
+    s = X.Pcg(ml, rel_tol=1e-6, accurate=True, stall_limit=0)
+    s.set_rhs(a.b, a.x0)
+    s.set_mlp_preconditioner(net)
+    s.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0)
+    X.synchronize()
+    t0 = time.perf_counter()
+    rel, repl, iters = s.run_reliable(1e-6, 0.0)
+    X.synchronize()
+    out["multilevel"] = {"solve_ms": round((time.perf_counter() - t0) * 1000.0, 3), "iterations": iters,
+                         "true_rel_residual": float(rel[0]), "hierarchy": s.multilevel_info(),
+                         "vcycle_us": round(s.multilevel_time(50), 3)}
+    s.close()
     out["spmv_roofline"] = spmv
     out["bitmap_block_spmv"] = bitmap
     print(json.dumps({"profile": "learned accurate solve", **out}), flush=True)
