@@ -881,7 +881,7 @@ def main() -> int:
     # accurate iteration: 0.05 +7%, 0.5 +14%)
     ap.add_argument("--lam", type=float, default=0.0)
     ap.add_argument("--ml-coarse-max", type=int, default=512, help="multilevel: rows of the dense coarsest level")
-    ap.add_argument("--ml-omega", type=float, default=0.7, help="multilevel: smoother damping")
+    ap.add_argument("--ml-omega", type=float, default=0.8, help="multilevel: smoother damping")
     ap.add_argument("--ml-gamma", type=float, default=1.5, help="multilevel: coarse-correction scale")
     ap.add_argument("--ml-coarse-blocking", type=int, default=4, help="multilevel: coarse aggregation rule (3, 4)")
     ap.add_argument("--cpu-golden-worker", action="store_true", help=argparse.SUPPRESS)

@@ -1805,19 +1805,25 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_smooth(Launch L, const SysStat
     }
 }
 
-// t = cz z + gamma e[down[q]] on the real rows of block q (padding rows 0).
-__global__ __launch_bounds__(kBlock, 8) void k_mg_prolong(Launch L, const SysState* st,
-                                                       const float* __restrict__ z, float cz,
-                                                       const float* __restrict__ e, float gamma,
-                                                       const int* __restrict__ perm, const int* __restrict__ down,
-                                                       float* __restrict__ t) {
-    if (st != nullptr && !st[0].active) return;
-    const WorkItem w = work_item(L, logical_block());
-    if (w.sys < 0) return;
-    for (int row = w.slice_begin * kC + static_cast<int>(threadIdx.x); row < w.slice_end * kC; row += kBlock) {
-        const int d = down[row >> 3];
-        t[row] = (perm[row] >= 0 && d >= 0) ? fmaf(gamma, e[d], cz * z[row]) : 0.0f;
-    }
+// Prolongation into the finer level, done by the kernel that produces the
+// coarse value e of row u: the 8 rows of its finer block q (up = q << 8 |
+// mask of the block's real rows) get t = cz z + gamma e, padding rows 0.
+__device__ __forceinline__ void prolong_block(const MgUp& pu, int up, float gamma, float e) {
+    const int q = up >> 8, mask = up & 0xFF;
+    const float4* zq = reinterpret_cast<const float4*>(pu.z + static_cast<size_t>(q) * 8);
+    float4* tq = reinterpret_cast<float4*>(pu.t + static_cast<size_t>(q) * 8);
+    const float4 z0 = zq[0], z1 = zq[1];
+    float4 t0, t1;
+    t0.x = (mask & 1) ? fmaf(gamma, e, pu.cz * z0.x) : 0.0f;
+    t0.y = (mask & 2) ? fmaf(gamma, e, pu.cz * z0.y) : 0.0f;
+    t0.z = (mask & 4) ? fmaf(gamma, e, pu.cz * z0.z) : 0.0f;
+    t0.w = (mask & 8) ? fmaf(gamma, e, pu.cz * z0.w) : 0.0f;
+    t1.x = (mask & 16) ? fmaf(gamma, e, pu.cz * z1.x) : 0.0f;
+    t1.y = (mask & 32) ? fmaf(gamma, e, pu.cz * z1.y) : 0.0f;
+    t1.z = (mask & 64) ? fmaf(gamma, e, pu.cz * z1.z) : 0.0f;
+    t1.w = (mask & 128) ? fmaf(gamma, e, pu.cz * z1.w) : 0.0f;
+    tq[0] = t0;
+    tq[1] = t1;
 }
 
 // zout = t + omega M (r - A t); RZ: the r^T zout partials of the PCG step
@@ -1826,7 +1832,7 @@ template <bool IDX16, int V, bool RZ>
 __device__ __forceinline__ void mg_post(const Launch& L, const SellView& A, const SysState* st,
                                                     const BlockPre* __restrict__ M, const float* __restrict__ r,
                                                     const float* __restrict__ t, float omega,
-                                                    float* __restrict__ zout) {
+                                                    float* __restrict__ zout, const MgUp& pu, float gamma) {
     if (st != nullptr && !st[0].active) return;
     const int lb = logical_block();
     const WorkItem w = work_item(L, lb);
@@ -1841,6 +1847,10 @@ __device__ __forceinline__ void mg_post(const Launch& L, const SellView& A, cons
         const float zz = fmaf(omega, block_apply(M, row, rv - at), td);
         zout[row] = zz;
         if (RZ) v[0] += static_cast<double>(rv) * static_cast<double>(zz);
+        if (pu.up != nullptr) {
+            const int u = pu.up[row];
+            if (u >= 0) prolong_block(pu, u, gamma, zz);
+        }
     }
     if (RZ) {
         block_sum<1>(v);
@@ -1849,31 +1859,52 @@ __device__ __forceinline__ void mg_post(const Launch& L, const SellView& A, cons
 }
 
 // e = Cinv rc on the dense coarsest level: one wavefront per row.
+// Then the prolongation into the finer level (pu).
 __global__ __launch_bounds__(kBlock, 8) void k_mg_coarse(const SysState* st, int nc, const float* __restrict__ cinv,
-                                                      const float* __restrict__ rc, float* __restrict__ ec) {
+                                                      const float* __restrict__ rc, float* __restrict__ ec,
+                                                      MgUp pu, float gamma) {
     if (st != nullptr && !st[0].active) return;
     const int row = static_cast<int>(blockIdx.x) * kWaves + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int lane = static_cast<int>(threadIdx.x) & 63;
     if (row >= nc) return;  // wave-uniform
     double s = 0.0;
     for (int j = lane; j < nc; j += kC) s += static_cast<double>(cinv[static_cast<size_t>(row) * nc + j]) * rc[j];
-    s = wave_sum(s);
-    if (lane == 0) ec[row] = static_cast<float>(s);
+    s = wave_sum(s);  // every lane holds the sum
+    const float e = static_cast<float>(s);
+    if (lane == 0) ec[row] = e;
+    const int u = pu.up[row];
+    if (u >= 0 && lane < 8) {
+        const int q = u >> 8;
+        const size_t i = static_cast<size_t>(q) * 8 + static_cast<size_t>(lane);
+        pu.t[i] = ((u >> lane) & 1) ? fmaf(gamma, e, pu.cz * pu.z[i]) : 0.0f;
+    }
 }
 
 template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_mg_post_norz(Launch L, SellView A, const SysState* st,
                                                          const BlockPre* __restrict__ M, const float* __restrict__ r,
                                                          const float* __restrict__ t, float omega,
-                                                         float* __restrict__ zout) {
-    mg_post<IDX16, V, false>(L, A, st, M, r, t, omega, zout);
+                                                         float* __restrict__ zout, MgUp pu, float gamma) {
+    mg_post<IDX16, V, false>(L, A, st, M, r, t, omega, zout, pu, gamma);
 }
 template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_mg_post_rz(Launch L, SellView A, const SysState* st,
                                                        const BlockPre* __restrict__ M, const float* __restrict__ r,
                                                        const float* __restrict__ t, float omega,
                                                        float* __restrict__ zout) {
-    mg_post<IDX16, V, true>(L, A, st, M, r, t, omega, zout);
+    mg_post<IDX16, V, true>(L, A, st, M, r, t, omega, zout, MgUp{}, 0.0f);
+}
+
+// up[down[q]] = q << 8 | the mask of block q's real rows (the inverse of
+// the down map, for the prolongation); up starts at -1.
+__global__ void k_up_map(int nq, const int* __restrict__ down, const int* __restrict__ perm, int* __restrict__ up) {
+    const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (q >= nq) return;
+    const int d = down[q];
+    if (d < 0) return;
+    int mask = 0;
+    for (int i = 0; i < 8; ++i) mask |= (perm[q * 8 + i] >= 0 ? 1 : 0) << i;
+    up[d] = (q << 8) | mask;
 }
 
 // ---- set-up of the hierarchy ----
@@ -2576,18 +2607,27 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
         float* dst = l + 1 < P.nlev ? P.lv[l + 1].r : P.rc;
         XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.down, dst);
     }
-    hipLaunchKernelGGL(k_mg_coarse, dim3((P.nc + kWaves - 1) / kWaves), dim3(kBlock), 0, L0.stream, st, P.nc, P.cinv,
-                       P.rc, P.ec);
+    // the prolongation into level l is done by the kernel that produces
+    // level l + 1's correction (the coarse apply, then each post-smoothing)
+    auto up_into = [&](int l) {
+        MgUp pu;
+        pu.t = P.lv[l].t;
+        pu.z = l == 0 ? z0 : P.lv[l].z;
+        pu.cz = l == 0 ? P.omega : 1.0f;
+        return pu;
+    };
+    {
+        MgUp pu = up_into(P.nlev - 1);
+        pu.up = P.cup;
+        hipLaunchKernelGGL(k_mg_coarse, dim3((P.nc + kWaves - 1) / kWaves), dim3(kBlock), 0, L0.stream, st, P.nc,
+                           P.cinv, P.rc, P.ec, pu, P.gamma);
+    }
     for (int l = P.nlev - 1; l >= 1; --l) {
         const MgLevelView& v = P.lv[l];
-        const float* e = l + 1 < P.nlev ? P.lv[l + 1].z : P.ec;
-        hipLaunchKernelGGL(k_mg_prolong, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, st, v.z, 1.0f, e, P.gamma,
-                           v.perm, v.down, v.t);
-        XP_MG_SELL(v.L, v.A, k_mg_post_norz, v.L, v.A, st, v.M, v.r, v.t, P.omega, v.z);
+        MgUp pu = up_into(l - 1);
+        pu.up = v.up;
+        XP_MG_SELL(v.L, v.A, k_mg_post_norz, v.L, v.A, st, v.M, v.r, v.t, P.omega, v.z, pu, P.gamma);
     }
-    const float* e1 = P.nlev > 1 ? P.lv[1].z : P.ec;
-    hipLaunchKernelGGL(k_mg_prolong, dim3(L0.grid), dim3(kBlock), 0, L0.stream, L0, st, z0, P.omega, e1, P.gamma,
-                       f.perm, f.down, f.t);
     XP_MG_SELL(L0, f.A, k_mg_post_rz, L0, f.A, st, f.M, r0, f.t, P.omega, z0);
 #undef XP_MG_SELL
 }
@@ -2625,6 +2665,12 @@ void launch_mg_galerkin_compact(int units, const int* off, const int* crp, const
 void launch_mg_inv_perm(int n_pad, const int* perm, int* inv, hipStream_t s) {
     hipLaunchKernelGGL(k_inv_perm, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad, perm,
                        inv);
+}
+
+void launch_mg_up_map(int nq, const int* down, const int* perm, int* up, int nup, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill<int>, dim3(blocks_for(static_cast<size_t>(nup), 256)), dim3(256), 0, s,
+                       static_cast<size_t>(nup), -1, up);
+    hipLaunchKernelGGL(k_up_map, dim3(blocks_for(static_cast<size_t>(nq), 256)), dim3(256), 0, s, nq, down, perm, up);
 }
 
 void launch_mg_down_map(int nq, const int* block_unit, const int* inv, int* down, hipStream_t s) {

@@ -88,12 +88,22 @@ struct MgLevelView {
     float* r = nullptr;          // level >= 1: restricted residual
     float* z = nullptr;          // level >= 1: the level's correction
     float* t = nullptr;          // prolonged approximation (every level)
+    const int* up = nullptr;     // level >= 1: row -> q << 8 | real-row mask of level l - 1's block q (-1: none)
+};
+// Where a kernel that produces a coarse correction e prolongs it: the finer
+// level's t = cz z + gamma e on the rows of the block up names.
+struct MgUp {
+    const int* up = nullptr;
+    float* t = nullptr;
+    const float* z = nullptr;
+    float cz = 1.0f;
 };
 struct MgPlan {
     int nlev = 0;                // SELL levels; the dense coarsest level follows them
     MgLevelView lv[kMaxLevels];
     int nc = 0;                  // rows of the coarsest level
     const float* cinv = nullptr; // its dense inverse, nc x nc row-major FP32
+    const int* cup = nullptr;    // its rows -> the last SELL level's blocks (as MgLevelView::up)
     float* rc = nullptr;
     float* ec = nullptr;
     float omega = 0.7f;          // smoother damping
@@ -114,6 +124,9 @@ void launch_mg_galerkin_compact(int units, const int* off, const int* crp, const
                                 int* cci, double* cva, hipStream_t s);
 void launch_mg_inv_perm(int n_pad, const int* perm, int* inv, hipStream_t s);
 void launch_mg_down_map(int nq, const int* block_unit, const int* inv, int* down, hipStream_t s);
+// up (nup entries, -1 first) = the inverse of down with the real-row masks
+// of the finer level's blocks (perm: the finer level's solve -> natural).
+void launch_mg_up_map(int nq, const int* down, const int* perm, int* up, int nup, hipStream_t s);
 // Exact inverses of the diagonal 8x8 blocks (solve order) as full FP32 blocks.
 void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t s);
 // out = inverse of the n x n SPD matrix of a CSR (FP64 Gauss-Jordan, two

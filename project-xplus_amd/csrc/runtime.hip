@@ -486,7 +486,9 @@ struct MgHierarchy {
     std::vector<xp::DevBuf<xp::BlockPre>> M;      // [nlev]; level 0 uses the solver's blocks
     std::vector<xp::DevBuf<int>> down;            // [nlev]
     std::vector<xp::DevBuf<float>> r, z, t;       // [nlev]; level 0 only t
+    std::vector<xp::DevBuf<int>> up;              // [nlev]; level l >= 1: rows -> level l - 1's blocks
     xp::DevBuf<float> cinv, rc, ec;               // the dense coarsest level
+    xp::DevBuf<int> cup;                          // its rows -> the last SELL level's blocks
     xp::DevBuf<int> c_rp, c_ci, c_slots;          // its CSR (natural) and the aggregates it came from
     xp::DevBuf<double> c_va;
     int nc = 0;
@@ -564,9 +566,11 @@ struct xp_pcg {
             v.r = H.r[static_cast<size_t>(l)].p;
             v.z = H.z[static_cast<size_t>(l)].p;
             v.t = H.t[static_cast<size_t>(l)].p;
+            v.up = H.up[static_cast<size_t>(l)].p;
         }
         P.nc = H.nc;
         P.cinv = H.cinv.p;
+        P.cup = H.cup.p;
         P.rc = H.rc.p;
         P.ec = H.ec.p;
         P.omega = H.omega;
@@ -3455,15 +3459,20 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
         };
         galerkin(*cur, crp, cci, cva, s);
         H->phase_ms.push_back(lap());
+        if (nq >= (1 << 23)) throw std::runtime_error("multilevel: a level has too many blocks");
         H->down.emplace_back(static_cast<size_t>(nq));
         H->t.emplace_back(static_cast<size_t>(cur->n_pad));
+        xp::launch_fill_f32(static_cast<size_t>(cur->n_pad), 0.0f, H->t.back().p, s);  // padding blocks stay 0
         if (l == 0) {
             H->M.emplace_back();
             H->r.emplace_back();
             H->z.emplace_back();
+            H->up.emplace_back();
         }
         if (units <= coarse_max) {
             xp::launch_mg_down_map(nq, cur->d_block_unit.p, nullptr, H->down.back().p, s);
+            H->cup.alloc(static_cast<size_t>(units));
+            xp::launch_mg_up_map(nq, H->down.back().p, cur->d_perm.p, H->cup.p, units, s);
             H->nc = units;
             xp::DevBuf<double> work(static_cast<size_t>(units) * static_cast<size_t>(units + 1));
             xp::DevBuf<int> bad(1);
@@ -3497,6 +3506,8 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
         xp::DevBuf<int> inv(static_cast<size_t>(units));
         xp::launch_mg_inv_perm(next->n_pad, next->d_perm.p, inv.p, s);
         xp::launch_mg_down_map(nq, cur->d_block_unit.p, inv.p, H->down.back().p, s);
+        H->up.emplace_back(static_cast<size_t>(next->n_pad));
+        xp::launch_mg_up_map(nq, H->down.back().p, cur->d_perm.p, H->up.back().p, next->n_pad, s);
         const int nqn = next->n_pad / 8;
         xp::DevBuf<float> full(static_cast<size_t>(nqn) * 64);
         xp::launch_block_exact_full(next->view(), nqn, full.p, s);

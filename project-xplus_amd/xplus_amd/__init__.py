@@ -1101,7 +1101,7 @@ class Pcg:
     def set_mlp_preconditioner(self, net: "Mlp") -> None:
         _check(_p_set_mlp(self.h, net.h))
 
-    def set_multilevel(self, coarse_max: int = 512, omega: float = 0.7, gamma: float = 1.5,
+    def set_multilevel(self, coarse_max: int = 512, omega: float = 0.8, gamma: float = 1.5,
                        coarse_blocking: int = 4, consecutive_rows: int = 0) -> None:
         """Multilevel preconditioner over the block aggregates, with the
         installed blocks as the level-0 smoother (coarse_max <= 0: off);

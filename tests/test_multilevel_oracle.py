@@ -98,10 +98,10 @@ def test_two_level_cycle_matches_dense_formula():
         np.testing.assert_allclose(ML.vcycle(H, r), V @ r, rtol=1e-11, atol=1e-11 * np.abs(V @ r).max())
 
 
-@pytest.mark.parametrize("seed_dist", [2, 1])
-def test_cycle_is_symmetric_positive_definite(seed_dist):
+@pytest.mark.parametrize("seed_dist,omega", [(2, 0.7), (1, 0.7), (1, 0.8)])
+def test_cycle_is_symmetric_positive_definite(seed_dist, omega):
     a = X.gen_fem2d(24, 1.0, 3, 1e3, 0.3, 11)
-    H = hierarchy(a, 2, seed_dist=seed_dist)
+    H = hierarchy(a, 2, omega=omega, seed_dist=seed_dist)
     n = a.n
     V = np.column_stack([ML.vcycle(H, e) for e in np.eye(n)])
     assert np.abs(V - V.T).max() <= 1e-10 * np.abs(V).max()
