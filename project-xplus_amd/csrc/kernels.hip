@@ -1791,6 +1791,63 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_restrict(Launch L, SellView A,
     }
 }
 
+// The coarse levels' small matrices give a workgroup at most one slice: its
+// 16 wavefronts then share the slice's slots (wavefront k takes slots k,
+// k + 16, ...), and wavefront 0 adds the 16 partial sums in wavefront order
+// (LDS) and finishes the rows.  Returns the row sum in wavefront 0 (0
+// elsewhere) and xdiag = x[row] there.
+template <bool IDX16>
+__device__ __forceinline__ float sell_row_split(const SellView& A, int slice, int wave, int lane,
+                                                const float* __restrict__ x, float& xdiag, float (*part)[kC]) {
+    const int j0 = A.slice_ptr[slice];
+    const int wdt = A.slice_ptr[slice + 1] - j0;
+    const int row = slice * kC + lane;
+    const int base = A.slice_base[slice];
+    float acc = 0.0f;
+    for (int j = wave; j < wdt; j += kWaves) {
+        const size_t e = (static_cast<size_t>(j0) + static_cast<size_t>(j)) * kC + static_cast<size_t>(lane);
+        const int c = j == 0 ? row : (IDX16 ? base + static_cast<int>(A.col16[e]) : A.col32[e]);
+        acc = fmaf(A.val[e], x[c], acc);
+    }
+    part[wave][lane] = acc;
+    __syncthreads();
+    float sum = 0.0f;
+    if (wave == 0) {
+        for (int k = 0; k < kWaves; ++k) sum += part[k][lane];
+        xdiag = x[row];
+    }
+    __syncthreads();  // part is reused by the next slice
+    return sum;
+}
+
+template <bool IDX16>
+__global__ __launch_bounds__(kBlock, 8) void k_mg_restrict_split(Launch L, SellView A, const SysState* st,
+                                                              const float* __restrict__ r,
+                                                              const float* __restrict__ z, float c,
+                                                              const int* __restrict__ down,
+                                                              float* __restrict__ dst) {
+    __shared__ float part[kWaves][kC];
+    if (st != nullptr && !st[0].active) return;
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    for (int sl = w.slice_begin; sl < w.slice_end; ++sl) {
+        float zd = 0.0f;
+        const float az = sell_row_split<IDX16>(A, sl, wave, lane, z, zd, part);
+        if (wave == 0) {
+            const int row = sl * kC + lane;
+            float t = fmaf(-c, az, r[row]);
+            t = dpp_add<0x141>(t);
+            t = dpp_add<0x4E>(t);
+            t = dpp_add<0xB1>(t);
+            if ((lane & 7) == 0) {
+                const int d = down[row >> 3];
+                if (d >= 0) dst[d] = t;
+            }
+        }
+    }
+}
+
 // z = omega M r (the smoother from a zero start).
 __global__ __launch_bounds__(kBlock, 8) void k_mg_smooth(Launch L, const SysState* st,
                                                       const BlockPre* __restrict__ M, const float* __restrict__ r,
@@ -1855,6 +1912,33 @@ __device__ __forceinline__ void mg_post(const Launch& L, const SellView& A, cons
     if (RZ) {
         block_sum<1>(v);
         if (threadIdx.x == 0) put_partial(L, kSlotRz, lb, v[0]);
+    }
+}
+
+// k_mg_post for one slice per workgroup (sell_row_split), no RZ partials.
+template <bool IDX16>
+__global__ __launch_bounds__(kBlock, 8) void k_mg_post_split(Launch L, SellView A, const SysState* st,
+                                                          const BlockPre* __restrict__ M, const float* __restrict__ r,
+                                                          const float* __restrict__ t, float omega,
+                                                          float* __restrict__ zout, MgUp pu, float gamma) {
+    __shared__ float part[kWaves][kC];
+    if (st != nullptr && !st[0].active) return;
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    for (int sl = w.slice_begin; sl < w.slice_end; ++sl) {
+        float td = 0.0f;
+        const float at = sell_row_split<IDX16>(A, sl, wave, lane, t, td, part);
+        if (wave == 0) {
+            const int row = sl * kC + lane;
+            const float rv = r[row];
+            const float zz = fmaf(omega, block_apply(M, row, rv - at), td);
+            zout[row] = zz;
+            if (pu.up != nullptr) {
+                const int u = pu.up[row];
+                if (u >= 0) prolong_block(pu, u, gamma, zz);
+            }
+        }
     }
 }
 
@@ -2601,11 +2685,22 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
     // level 0: z0 = M r from the update kernel; the pre-smoothed approximation is omega z0
     float* first_dst = P.nlev > 1 ? P.lv[1].r : P.rc;
     XP_MG_SELL(L0, f.A, k_mg_restrict, L0, f.A, st, r0, z0, P.omega, f.down, first_dst);
+    // a workgroup with at most one slice shares it among its 16 wavefronts
+    auto split = [](const MgLevelView& v) { return v.L.grid >= v.A.nslices; };
     for (int l = 1; l < P.nlev; ++l) {
         const MgLevelView& v = P.lv[l];
         hipLaunchKernelGGL(k_mg_smooth, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, st, v.M, v.r, P.omega, v.z);
         float* dst = l + 1 < P.nlev ? P.lv[l + 1].r : P.rc;
-        XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.down, dst);
+        if (split(v)) {
+            if (v.A.idx16)
+                hipLaunchKernelGGL(k_mg_restrict_split<true>, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, v.A,
+                                   st, v.r, v.z, 1.0f, v.down, dst);
+            else
+                hipLaunchKernelGGL(k_mg_restrict_split<false>, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, v.A,
+                                   st, v.r, v.z, 1.0f, v.down, dst);
+        } else {
+            XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.down, dst);
+        }
     }
     // the prolongation into level l is done by the kernel that produces
     // level l + 1's correction (the coarse apply, then each post-smoothing)
@@ -2626,7 +2721,16 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
         const MgLevelView& v = P.lv[l];
         MgUp pu = up_into(l - 1);
         pu.up = v.up;
-        XP_MG_SELL(v.L, v.A, k_mg_post_norz, v.L, v.A, st, v.M, v.r, v.t, P.omega, v.z, pu, P.gamma);
+        if (split(v)) {
+            if (v.A.idx16)
+                hipLaunchKernelGGL(k_mg_post_split<true>, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, v.A, st,
+                                   v.M, v.r, v.t, P.omega, v.z, pu, P.gamma);
+            else
+                hipLaunchKernelGGL(k_mg_post_split<false>, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, v.A, st,
+                                   v.M, v.r, v.t, P.omega, v.z, pu, P.gamma);
+        } else {
+            XP_MG_SELL(v.L, v.A, k_mg_post_norz, v.L, v.A, st, v.M, v.r, v.t, P.omega, v.z, pu, P.gamma);
+        }
     }
     XP_MG_SELL(L0, f.A, k_mg_post_rz, L0, f.A, st, f.M, r0, f.t, P.omega, z0);
 #undef XP_MG_SELL
