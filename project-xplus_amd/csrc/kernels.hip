@@ -2202,34 +2202,22 @@ __global__ void k_dense_from_csr(int n, const int* __restrict__ rp, const int* _
     for (int e = rp[i]; e < rp[i + 1]; ++e) dense[static_cast<size_t>(i) * n + ci[e]] += va[e];
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one step k
-// in two kernels: the update of every entry outside row and column k (which
-// read row k and column k, unchanged here; the pivot's inverse saved in
-// pinv[k], a non-positive pivot flagged in *bad), then the scaling of row
-// and column k and the pivot itself.
-__global__ void k_gj_update(int n, int k, double* __restrict__ a, double* __restrict__ pinv, int* __restrict__ bad) {
+// Gauss-Jordan inverse of an SPD matrix (no pivoting): pivot k in one
+// kernel that reads the matrix of step k - 1 (a) and writes step k's (b), so
+// no entry is read after it changes; a non-positive pivot sets *bad.
+__global__ void k_gj_step(int n, int k, const double* __restrict__ a, double* __restrict__ b, int* __restrict__ bad) {
     const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    const double akk = a[static_cast<size_t>(k) * n + k];
-    if (idx == 0) {
-        pinv[k] = 1.0 / akk;
-        if (!(akk > 0.0)) *bad = 1;
-    }
     if (idx >= static_cast<size_t>(n) * n) return;
+    const double akk = a[static_cast<size_t>(k) * n + k];
+    if (idx == 0 && !(akk > 0.0)) *bad = 1;
+    const double p = 1.0 / akk;
     const int i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
-    if (i == k || j == k) return;
-    a[idx] -= a[static_cast<size_t>(i) * n + k] * a[static_cast<size_t>(k) * n + j] / akk;
-}
-
-__global__ void k_gj_scale(int n, int k, double* __restrict__ a, const double* __restrict__ pinv) {
-    const int j = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
-    if (j >= n) return;
-    const double p = pinv[k];
-    if (j == k) {
-        a[static_cast<size_t>(k) * n + k] = p;
-    } else {
-        a[static_cast<size_t>(k) * n + j] *= p;
-        a[static_cast<size_t>(j) * n + k] *= -p;
-    }
+    double v;
+    if (i == k && j == k) v = p;
+    else if (i == k) v = a[idx] * p;
+    else if (j == k) v = -a[idx] * p;
+    else v = a[idx] - a[static_cast<size_t>(i) * n + k] * a[static_cast<size_t>(k) * n + j] / akk;
+    b[idx] = v;
 }
 
 __global__ void k_f64_to_f32(size_t n, const double* __restrict__ a, float* __restrict__ out) {
@@ -2790,16 +2778,14 @@ void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t
 void launch_mg_dense_inverse(int n, const int* rp, const int* ci, const double* va, double* work, float* out,
                              int* bad, hipStream_t s) {
     const size_t nn = static_cast<size_t>(n) * static_cast<size_t>(n);
-    double* pinv = work + nn;
+    double* buf[2] = {work, work + nn};
     hipLaunchKernelGGL(k_fill<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, 0.0, work);
     hipLaunchKernelGGL(k_dense_from_csr, dim3(blocks_for(static_cast<size_t>(n), 256)), dim3(256), 0, s, n, rp, ci,
                        va, work);
-    for (int k = 0; k < n; ++k) {
-        hipLaunchKernelGGL(k_gj_update, dim3(blocks_for(nn, 256)), dim3(256), 0, s, n, k, work, pinv, bad);
-        hipLaunchKernelGGL(k_gj_scale, dim3(blocks_for(static_cast<size_t>(n), 256)), dim3(256), 0, s, n, k, work,
-                           pinv);
-    }
-    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, work, out);
+    for (int k = 0; k < n; ++k)
+        hipLaunchKernelGGL(k_gj_step, dim3(blocks_for(nn, 256)), dim3(256), 0, s, n, k, buf[k & 1], buf[(k + 1) & 1],
+                           bad);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, buf[n & 1], out);
 }
 
 }  // namespace xp

@@ -129,9 +129,9 @@ void launch_mg_down_map(int nq, const int* block_unit, const int* inv, int* down
 void launch_mg_up_map(int nq, const int* down, const int* perm, int* up, int nup, hipStream_t s);
 // Exact inverses of the diagonal 8x8 blocks (solve order) as full FP32 blocks.
 void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t s);
-// out = inverse of the n x n SPD matrix of a CSR (FP64 Gauss-Jordan, two
-// kernels per pivot, rounded to FP32); work: n * n + n doubles; *bad = 1 on
-// a non-positive pivot (bad must be zero on entry).
+// out = inverse of the n x n SPD matrix of a CSR (FP64 Gauss-Jordan, one
+// kernel per pivot between two buffers, rounded to FP32); work: 2 n * n
+// doubles; *bad = 1 on a non-positive pivot (bad must be zero on entry).
 void launch_mg_dense_inverse(int n, const int* rp, const int* ci, const double* va, double* work, float* out,
                              int* bad, hipStream_t s);
 

@@ -3392,33 +3392,40 @@ int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale) {
 // ---- multilevel preconditioner ------------------------------------------------
 namespace {
 
+// Work buffers of the hierarchy's set-up, sized by the largest level and
+// reused by every level (allocations are the bulk of a small level's cost).
+struct GalerkinWork {
+    xp::DevBuf<int> agg, cnt, off, dcnt, ccol, crp, cci;
+    xp::DevBuf<double> cval, cva;
+};
+
 // Galerkin coarse matrix of a device-built block matrix c: row u = block
 // (unit) u, entry (u, v) = sum of a_ij over natural rows i of u and j of v.
-void galerkin(const xp_matrix& c, xp::DevBuf<int>& crp, xp::DevBuf<int>& cci, xp::DevBuf<double>& cva,
-              hipStream_t s) {
+// The result is left in w.crp / w.cci / w.cva; returns its entry count.
+int galerkin(const xp_matrix& c, GalerkinWork& w, hipStream_t s) {
     const int units = c.unit_first.back();
     const int n = c.n;
-    int nnz = 0;
-    HIPX(hipMemcpyAsync(&nnz, c.d_rp.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPX(hipStreamSynchronize(s));
-    xp::DevBuf<int> agg(static_cast<size_t>(n)), cnt(static_cast<size_t>(units)),
-        off(static_cast<size_t>(units) + 1), dcnt(static_cast<size_t>(units));
-    xp::DevBuf<int> ccol(std::max<size_t>(static_cast<size_t>(nnz), 1));
-    xp::DevBuf<double> cval(std::max<size_t>(static_cast<size_t>(nnz), 1));
-    xp::launch_mg_unit_of_row(units, c.d_slots.p, agg.p, s);
-    xp::launch_mg_galerkin_count(units, c.d_slots.p, c.d_rp.p, cnt.p, s);
-    xp::device_scan_exclusive(cnt.p, off.p, units, off.p + units, s);
-    xp::launch_mg_galerkin_fill(units, c.d_slots.p, c.d_rp.p, c.d_ci.p, c.d_va.p, agg.p, off.p, ccol.p, cval.p,
-                                dcnt.p, s);
-    crp.alloc(static_cast<size_t>(units) + 1);
-    xp::device_scan_exclusive(dcnt.p, crp.p, units, crp.p + units, s);
+    const size_t nnz = static_cast<size_t>(c.nnz);  // the CSR's entries (device set-up)
+    w.agg.ensure(static_cast<size_t>(n));
+    w.cnt.ensure(static_cast<size_t>(units));
+    w.off.ensure(static_cast<size_t>(units) + 1);
+    w.dcnt.ensure(static_cast<size_t>(units));
+    w.ccol.ensure(std::max<size_t>(nnz, 1));
+    w.cval.ensure(std::max<size_t>(nnz, 1));
+    w.crp.ensure(static_cast<size_t>(units) + 1);
+    xp::launch_mg_unit_of_row(units, c.d_slots.p, w.agg.p, s);
+    xp::launch_mg_galerkin_count(units, c.d_slots.p, c.d_rp.p, w.cnt.p, s);
+    xp::device_scan_exclusive(w.cnt.p, w.off.p, units, w.off.p + units, s);
+    xp::launch_mg_galerkin_fill(units, c.d_slots.p, c.d_rp.p, c.d_ci.p, c.d_va.p, w.agg.p, w.off.p, w.ccol.p,
+                                w.cval.p, w.dcnt.p, s);
+    xp::device_scan_exclusive(w.dcnt.p, w.crp.p, units, w.crp.p + units, s);
     int cnnz = 0;
-    HIPX(hipMemcpyAsync(&cnnz, crp.p + units, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&cnnz, w.crp.p + units, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
-    cci.alloc(static_cast<size_t>(cnnz));
-    cva.alloc(static_cast<size_t>(cnnz));
-    xp::launch_mg_galerkin_compact(units, off.p, crp.p, ccol.p, cval.p, cci.p, cva.p, s);
-    HIPX(hipStreamSynchronize(s));  // before the temporaries go
+    w.cci.ensure(std::max<size_t>(static_cast<size_t>(cnnz), 1));
+    w.cva.ensure(std::max<size_t>(static_cast<size_t>(cnnz), 1));
+    xp::launch_mg_galerkin_compact(units, w.off.p, w.crp.p, w.ccol.p, w.cval.p, w.cci.p, w.cva.p, s);
+    return cnnz;
 }
 
 void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
@@ -3444,12 +3451,11 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
     H->gamma = static_cast<float>(gamma);
     H->sigma0 = m0.sigma;
     const xp_matrix* cur = &m0;
+    GalerkinWork work;
     for (int l = 0;; ++l) {
         if (l + 1 >= xp::kMaxLevels) throw std::runtime_error("multilevel: too many levels");
         const int units = cur->unit_first.back();
         const int nq = cur->n_pad / 8;
-        xp::DevBuf<int> crp, cci;
-        xp::DevBuf<double> cva;
         auto tp = std::chrono::steady_clock::now();
         auto lap = [&tp]() {
             const auto now = std::chrono::steady_clock::now();
@@ -3457,7 +3463,7 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
             tp = now;
             return ms;
         };
-        galerkin(*cur, crp, cci, cva, s);
+        const int cnnz = galerkin(*cur, work, s);
         H->phase_ms.push_back(lap());
         if (nq >= (1 << 23)) throw std::runtime_error("multilevel: a level has too many blocks");
         H->down.emplace_back(static_cast<size_t>(nq));
@@ -3474,13 +3480,13 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
             H->cup.alloc(static_cast<size_t>(units));
             xp::launch_mg_up_map(nq, H->down.back().p, cur->d_perm.p, H->cup.p, units, s);
             H->nc = units;
-            xp::DevBuf<double> work(static_cast<size_t>(units) * static_cast<size_t>(units + 1));
+            xp::DevBuf<double> dense(2 * static_cast<size_t>(units) * static_cast<size_t>(units));
             xp::DevBuf<int> bad(1);
             HIPX(hipMemsetAsync(bad.p, 0, sizeof(int), s));
             H->cinv.alloc(static_cast<size_t>(units) * static_cast<size_t>(units));
             H->rc.alloc(static_cast<size_t>(units));
             H->ec.alloc(static_cast<size_t>(units));
-            xp::launch_mg_dense_inverse(units, crp.p, cci.p, cva.p, work.p, H->cinv.p, bad.p, s);
+            xp::launch_mg_dense_inverse(units, work.crp.p, work.cci.p, work.cva.p, dense.p, H->cinv.p, bad.p, s);
             int hb = 0;
             HIPX(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
             H->c_slots.alloc(static_cast<size_t>(units) * 8);
@@ -3490,16 +3496,23 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
             if (hb != 0) throw std::runtime_error("multilevel: the coarsest matrix is not positive definite");
             H->phase_ms.push_back(lap());
             H->phase_ms.push_back(0.0);
-            H->c_rp = std::move(crp);
-            H->c_ci = std::move(cci);
-            H->c_va = std::move(cva);
+            H->c_rp.alloc(static_cast<size_t>(units) + 1);
+            H->c_ci.alloc(std::max<size_t>(static_cast<size_t>(cnnz), 1));
+            H->c_va.alloc(std::max<size_t>(static_cast<size_t>(cnnz), 1));
+            HIPX(hipMemcpyAsync(H->c_rp.p, work.crp.p, (static_cast<size_t>(units) + 1) * sizeof(int),
+                                hipMemcpyDeviceToDevice, s));
+            HIPX(hipMemcpyAsync(H->c_ci.p, work.cci.p, static_cast<size_t>(cnnz) * sizeof(int),
+                                hipMemcpyDeviceToDevice, s));
+            HIPX(hipMemcpyAsync(H->c_va.p, work.cva.p, static_cast<size_t>(cnnz) * sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
+            HIPX(hipStreamSynchronize(s));
             break;
         }
         xp_matrix_opts o{};
         o.block_rows = 8;
         o.blocking = units <= mo.consecutive_rows ? 0 : coarse_blocking;
         o.sigma = 64;
-        std::unique_ptr<xp_matrix> next = make_matrix_device(units, crp.p, cci.p, cva.p, &o);
+        std::unique_ptr<xp_matrix> next = make_matrix_device(units, work.crp.p, work.cci.p, work.cva.p, &o);
         if (next->bad_row >= 0) throw std::runtime_error("multilevel: a coarse matrix has a non-positive diagonal");
         HIPX(hipStreamSynchronize(s));
         H->phase_ms.push_back(lap());
