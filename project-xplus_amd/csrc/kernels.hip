@@ -2672,7 +2672,14 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
     } while (0)
     // level 0: z0 = M r from the update kernel; the pre-smoothed approximation is omega z0
     float* first_dst = P.nlev > 1 ? P.lv[1].r : P.rc;
-    XP_MG_SELL(L0, f.A, k_mg_restrict, L0, f.A, st, r0, z0, P.omega, f.down, first_dst);
+    // the restriction writes no partial sums: it may take its own workgroup
+    // count (one system: the uniform split needs no work map)
+    Launch Lr = L0;
+    if (P.grid0 > 0 && L0.u_blocks > 0) {
+        Lr.grid = P.grid0;
+        Lr.u_blocks = P.grid0;
+    }
+    XP_MG_SELL(Lr, f.A, k_mg_restrict, Lr, f.A, st, r0, z0, P.omega, f.down, first_dst);
     // a workgroup with at most one slice shares it among its 16 wavefronts
     auto split = [](const MgLevelView& v) { return v.L.grid >= v.A.nslices; };
     for (int l = 1; l < P.nlev; ++l) {

@@ -106,6 +106,7 @@ struct MgPlan {
     const int* cup = nullptr;    // its rows -> the last SELL level's blocks (as MgLevelView::up)
     float* rc = nullptr;
     float* ec = nullptr;
+    int grid0 = 0;               // workgroups of level 0's restriction (0: the solver's)
     float omega = 0.7f;          // smoother damping
     float gamma = 1.5f;          // coarse-correction scale
 };

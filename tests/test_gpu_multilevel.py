@@ -20,6 +20,11 @@ Bars (written in each test):
     count is at most 1.3 x + 5 of the oracle's (FP32 r, z and the FP32 cycle
     cost iterations; measured 194 against 189);
   * the graph-launched solve equals the directly launched one bit for bit;
+  * to 1e-10 (exact FP64 values, replacement at delta 0.01): the true
+    residual meets it and x lies within 1e-6 of the FP64 golden's (measured
+    2.7e-11), in under a tenth of the golden's iterations (241 / 2,806);
+  * useless level-0 blocks under the true-residual watch fall back to
+    Jacobi (the cycle turned off with them) and still converge;
   * misuse is refused with a message.
 """
 from __future__ import annotations
@@ -226,3 +231,56 @@ def test_multilevel_misuse_refused(gpu):
         h.set_multilevel()
     for o in (s, f, h, m, mh):
         o.close()
+
+
+def test_multilevel_to_1e10_against_fp64_golden(gpu):
+    """The 1e-10 path (exact FP64 values, residual replacement at delta
+    0.01, restarts from replaced residuals) with the cycle: the true relative
+    residual meets 1e-10 (recomputed on the CPU within 5%: the solver's
+    residual uses the FP64 values too, summed in another order), the
+    solution lies within 1e-6 of the FP64 golden Jacobi-PCG's (both at true
+    1e-10: the golden's own error bound at this tolerance and conditioning),
+    in under a tenth of the golden's iterations."""
+    a = X.gen_fem2d(202, 1.0, 20, 1e3, 0.3, 20263203)
+    tol = 1e-10
+    g = O.golden(a, tau=(tol * np.linalg.norm(a.b)) ** 2, trace_cap=1)
+    m = X.Matrix(a, keep64=True, block_rows=8, blocking=3)
+    s = X.Pcg(m, rel_tol=tol, accurate=True, stall_limit=0)
+    s.set_rhs(a.b, a.x0)
+    s.set_preconditioner(1)
+    s.set_multilevel(512, OMEGA, GAMMA, 4, 0)
+    rel, repl, iters = s.run_reliable(tol, 0.01)
+    x = s.x()
+    A = a.to_scipy().tocsr()
+    res = np.linalg.norm(a.b - A @ x) / np.linalg.norm(a.b)
+    err = np.linalg.norm(x - g["x"]) / np.linalg.norm(g["x"])
+    print(f"multilevel to 1e-10: {iters} iterations (golden {g['iterations']}), {repl} replacements, "
+          f"rel {rel[0]:.3e} (cpu {res:.3e}), error vs golden {err:.3e}", flush=True)
+    assert s.status()["status"] == X.CONVERGED
+    assert rel[0] <= tol and res <= 1.05 * tol
+    assert err <= 1e-6
+    assert iters < 0.1 * g["iterations"]
+    s.close()
+    m.close()
+
+
+def test_useless_blocks_fall_back_with_the_cycle(gpu):
+    """A level-0 smoother that is useless (every block scaled by 1e-6: SPD,
+    but the damped smoothing and the cycle then do almost nothing) under the
+    true-residual stagnation watch: the solve falls back to Jacobi, which
+    turns the cycle off with the blocks, and still meets the tolerance."""
+    a = X.gen_fem2d(60, 1.0, 6, 1e3, 0.3, 20265706)
+    m = X.Matrix(a, block_rows=8, keep_lo=True, blocking=3)
+    s = X.Pcg(m, rel_tol=1e-6, accurate=True, stall_limit=16, max_iters=20000)
+    s.set_rhs(a.b, a.x0)
+    nb = m.num_blocks()
+    blocks = np.tile(np.eye(8, dtype=np.float32) * 1e-6, (nb, 1, 1))
+    s.set_preconditioner(2, blocks)
+    s.set_multilevel(64, OMEGA, GAMMA, 4, 0)
+    rel, repl, iters = s.run_reliable(1e-6, 0.0)
+    st = s.status()
+    print(f"useless blocks with the cycle: {iters} iterations, fallback {st['fallback']}, rel {rel[0]:.3e}", flush=True)
+    assert st["fallback"] == 1
+    assert rel[0] <= 1e-6
+    s.close()
+    m.close()
