@@ -392,7 +392,7 @@ def profile_run(X, a, args) -> int:
     s = X.Pcg(ml, rel_tol=1e-6, accurate=True, stall_limit=0)
     s.set_rhs(a.b, a.x0)
     s.set_mlp_preconditioner(net)
-    s.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0)
+    s.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0, args.ml_theta)
     X.synchronize()
     t0 = time.perf_counter()
     rel, repl, iters = s.run_reliable(1e-6, 0.0)
@@ -451,7 +451,7 @@ def workload_c3(X, args) -> int:
             else:
                 q.set_preconditioner(1)
             if name == "learned_multilevel":
-                q.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0)
+                q.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0, args.ml_theta)
             X.synchronize()
             setup_ms = (time.perf_counter() - t0) * 1000.0
             b_ms, (rel3, repl3, iters3) = timed(X, lambda: q.run_reliable(1e-10, 0.01), 0, 1)
@@ -652,8 +652,9 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
         if net is not None:
             s.set_mlp_preconditioner(net)
         if name == "learned_multilevel":
-            s.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0)
+            s.set_multilevel(args.ml_coarse_max, args.ml_omega, args.ml_gamma, args.ml_coarse_blocking, 0, args.ml_theta)
             info["multilevel"] = s.multilevel_info()
+            info["ml_rounds"] = s.multilevel_rounds()
         X.synchronize()
         t1 = time.perf_counter()
         if run == "run":
@@ -703,7 +704,8 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
                              "hierarchy_setup_ms": round(ml["setup_ms"], 3),
                              "phase_ms": [round(float(v), 3) for v in ml["phase_ms"]],
                              "coarse_max": args.ml_coarse_max, "omega": args.ml_omega, "gamma": args.ml_gamma,
-                             "coarse_blocking": args.ml_coarse_blocking,
+                             "coarse_blocking": args.ml_coarse_blocking, "theta": args.ml_theta,
+                             "aggregation_rounds": info.get("ml_rounds", []),
                              "vcycle_us": round(info["vcycle_us"], 3)}
     out["us_per_iteration"] = round(solve * 1000.0 / max(out["iterations"], 1), 3)
     log(f"{name}: {out}")
@@ -883,6 +885,7 @@ def main() -> int:
     ap.add_argument("--ml-coarse-max", type=int, default=512, help="multilevel: rows of the dense coarsest level")
     ap.add_argument("--ml-omega", type=float, default=0.8, help="multilevel: smoother damping")
     ap.add_argument("--ml-gamma", type=float, default=1.5, help="multilevel: coarse-correction scale")
+    ap.add_argument("--ml-theta", type=float, default=0.08, help="multilevel: strength threshold of the aggregation")
     ap.add_argument("--ml-coarse-blocking", type=int, default=4, help="multilevel: coarse aggregation rule (3, 4)")
     ap.add_argument("--cpu-golden-worker", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()

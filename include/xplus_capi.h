@@ -496,30 +496,32 @@ int xp_mlp_train(xp_mlp* net, const xp_blockset* set, int steps, int batch, floa
  * The matrix must be built with block_rows = 8. */
 int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net);
 
-/* Multilevel preconditioner over the block aggregates (no reference
- * counterpart: it extends the block-Jacobi preconditioner the reference's
- * Jacobi-PCG loop would take in place of its inverse diagonal,
- * src/CgSolverGolden.hpp:33-40).  The installed 8x8 blocks (exact or
- * learned) become the smoother of level 0; level l + 1 is the Galerkin
- * product P^T A P over the 8-row aggregates of level l (P piecewise
- * constant), set up on the device with its own parallel strength
- * aggregates and exact block factors, down to a dense coarsest level of at
- * most coarse_max rows (inverted once); coarse_blocking 3 aggregates the
- * coarse levels like level 0, 4 with distance-1 seeds (far fewer rounds on
- * their denser graphs), and levels of at most consecutive_rows rows take
- * consecutive rows (their natural order follows the blocks' smallest rows,
- * so consecutive rows are neighbours).  Each PCG step then applies one
- * symmetric V-cycle: z = omega M r, coarse correction scaled by gamma, one
- * post-smoothing step; FP32 vectors inside the cycle.  Needs the accurate
- * iteration on one system set up on the device with block_rows = 8 and
- * blocking 3 or 4, and a block preconditioner already installed.  A fallback to Jacobi turns it off with the
- * blocks. */
+/* Multilevel aggregation preconditioner (no reference counterpart: it
+ * extends the block-Jacobi preconditioner the reference's Jacobi-PCG loop
+ * would take in place of its inverse diagonal, src/CgSolverGolden.hpp:33-40).
+ * The installed 8x8 blocks (exact or learned) become the smoother of level
+ * 0.  Each level's coarse space is a plain aggregation of its rows: on the
+ * strength graph (a_ij^2 >= theta^2 |a_ii| |a_jj|), roots form a maximal
+ * distance-2 independent set chosen in parallel rounds, a root's strong
+ * neighbours join it, every other row joins its strongest joined neighbour;
+ * level l + 1 is the Galerkin product P^T A P over those aggregates (P
+ * piecewise constant), set up on the device with its own 8-row smoother
+ * blocks (coarse_blocking 3 or 4: the parallel strength aggregation of
+ * xp_matrix_opts.blocking; levels of at most consecutive_rows rows take 8
+ * consecutive rows) and exact block factors, down to a dense coarsest level
+ * of at most coarse_max rows (inverted once).  Each PCG step then applies
+ * one symmetric V-cycle: z = omega M r, coarse correction scaled by gamma,
+ * one post-smoothing step; FP32 vectors inside the cycle.  Needs the
+ * accurate iteration on one system set up on the device with block_rows =
+ * 8, and a block preconditioner already installed.  A fallback to Jacobi
+ * turns it off with the blocks. */
 typedef struct {
     int coarse_max;       /* rows of the dense coarsest level, at most (8..4096); <= 0 removes the hierarchy */
     double omega;         /* smoother damping, 0 < omega < 2 */
     double gamma;         /* coarse-correction scale, 0 < gamma < 4 */
-    int coarse_blocking;  /* aggregation of the coarse levels: 3 or 4 (see xp_matrix_opts.blocking) */
+    int coarse_blocking;  /* smoother blocks of the coarse levels: 3 or 4 (see xp_matrix_opts.blocking) */
     int consecutive_rows; /* coarse levels of at most this many rows take 8 consecutive rows per block */
+    double theta;         /* strength threshold of the aggregation, 0 <= theta < 1 */
 } xp_multilevel_opts;
 int xp_pcg_set_multilevel(xp_pcg* s, const xp_multilevel_opts* opts);
 /* Levels of the hierarchy (*nlev, 0 when none): rows and stored entries of
@@ -527,13 +529,20 @@ int xp_pcg_set_multilevel(xp_pcg* s, const xp_multilevel_opts* opts);
  * coarsest one) and the set-up time in ms. */
 int xp_pcg_multilevel_info(const xp_pcg* s, int cap, int* nlev, int* n, long long* nnz, double* setup_ms);
 /* The set-up's time per level in ms, three entries per level below the
- * coarsest: Galerkin product, the coarse matrix's device set-up, its block
- * factors (the dense level: product, inverse, 0); *count entries. */
+ * coarsest: aggregation (with its member lists), Galerkin product, the
+ * coarse level's set-up (device matrix set-up and block factors, or the
+ * dense inverse); *count entries. */
 int xp_pcg_multilevel_phases(const xp_pcg* s, int cap, double* ms, int* count);
-/* Level `level` >= 1 as a natural-order CSR (rows = the blocks of level
- * level - 1) and those blocks (rows x 8 natural rows of level level - 1, -1
- * padding); null outputs are skipped (sizes from xp_pcg_multilevel_info). */
-int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_idx, double* values, int* slots);
+/* Root-selection rounds of each level's aggregation; *count entries. */
+int xp_pcg_multilevel_rounds(const xp_pcg* s, int cap, int* rounds, int* count);
+/* Level `level` >= 1 as a natural-order CSR (rows = the aggregates of level
+ * level - 1) and the aggregate of every natural row of level level - 1;
+ * null outputs are skipped (sizes from xp_pcg_multilevel_info). */
+int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_idx, double* values, int* agg);
+/* The smoother blocks of SELL level 1 <= level < levels - 1: *units blocks,
+ * units x 8 natural rows of the level (-1 padding) into slots (may be
+ * null). */
+int xp_pcg_multilevel_slots(const xp_pcg* s, int level, int* slots, int* units);
 /* z = M r followed by one V-cycle, for device vectors of level 0 (solve
  * order, n_pad entries). */
 int xp_pcg_multilevel_apply(xp_pcg* s, const float* r, float* z);

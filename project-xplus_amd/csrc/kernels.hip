@@ -1753,41 +1753,34 @@ __global__ __launch_bounds__(kBlock, 1) void k_op_finish(Launch L, int slot0, in
 
 // ---------------------------------------------------------------------------
 // Multilevel (aggregation) preconditioner.  Every level is a SELL matrix
-// whose 8-row blocks are the strength aggregates of that level; block q of
-// level l is row down[q] of level l + 1 (the next level's solve order, or the
-// natural order of the dense coarsest level).  P is piecewise constant on
-// the blocks (real rows only), the coarse operator is P^T A P, and one
-// V-cycle is
+// with its own 8-row smoother blocks; the coarse space of level l is a set
+// of aggregates of its rows (plain aggregation, below), and row u of level
+// l + 1 is aggregate u.  P is piecewise constant on the aggregates, the
+// coarse operator is P^T A P, and one V-cycle is
 //   z = omega M r;  r' = P^T (r - A z);  z += gamma P V(r');  z += omega M (r - A z)
 // with M the level's block factors and the coarsest level solved by a dense
 // inverse.  The cycle is symmetric (the same smoother before and after).
-// Vectors, values and sums are FP32 (a preconditioner needs no more); every
-// kernel returns at once when the system has stopped.
+// Each coarse row carries the list of its aggregate's rows as positions in
+// the finer level's solve order (mp / mpos): the restriction is a gather
+// over that list, the prolongation a scatter over it.  Vectors, values and
+// sums are FP32 (a preconditioner needs no more); every kernel returns at
+// once when the system has stopped.
 
-// dst[down[q]] = sum over the rows of block q of (r - c (A z))_row (padding
-// rows add 0: r and z are 0 there).
+// w = r - c (A z) for every solve row of the level (padding rows: r and z
+// are 0 there, so w = 0).
 template <bool IDX16, int V>
 __global__ __launch_bounds__(kBlock, 8) void k_mg_restrict(Launch L, SellView A, const SysState* st,
                                                         const float* __restrict__ r, const float* __restrict__ z,
-                                                        float c, const int* __restrict__ down,
-                                                        float* __restrict__ dst) {
+                                                        float c, float* __restrict__ wout) {
     if (st != nullptr && !st[0].active) return;
     const WorkItem w = work_item(L, logical_block());
     if (w.sys < 0) return;
     XP_WAVE_LANE
-    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {  // wave-uniform: DPP sums
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
         const int row = sl * kC + lane;
         float zd;
         const float az = sell_row_f32<IDX16, V>(A, sl, lane, z, zd);
-        // padding rows: r = 0 and z = 0 (identity rows, M identity there), so t = 0
-        float t = fmaf(-c, az, r[row]);
-        t = dpp_add<0x141>(t);  // the block's 8 lanes: pairs (i, 7 - i),
-        t = dpp_add<0x4E>(t);   // then (i, i ^ 2),
-        t = dpp_add<0xB1>(t);   // then (i, i ^ 1)
-        if ((lane & 7) == 0) {
-            const int d = down[row >> 3];
-            if (d >= 0) dst[d] = t;
-        }
+        wout[row] = fmaf(-c, az, r[row]);
     }
 }
 
@@ -1824,8 +1817,7 @@ template <bool IDX16>
 __global__ __launch_bounds__(kBlock, 8) void k_mg_restrict_split(Launch L, SellView A, const SysState* st,
                                                               const float* __restrict__ r,
                                                               const float* __restrict__ z, float c,
-                                                              const int* __restrict__ down,
-                                                              float* __restrict__ dst) {
+                                                              float* __restrict__ wout) {
     __shared__ float part[kWaves][kC];
     if (st != nullptr && !st[0].active) return;
     const WorkItem w = work_item(L, logical_block());
@@ -1836,14 +1828,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_restrict_split(Launch L, SellV
         const float az = sell_row_split<IDX16>(A, sl, wave, lane, z, zd, part);
         if (wave == 0) {
             const int row = sl * kC + lane;
-            float t = fmaf(-c, az, r[row]);
-            t = dpp_add<0x141>(t);
-            t = dpp_add<0x4E>(t);
-            t = dpp_add<0xB1>(t);
-            if ((lane & 7) == 0) {
-                const int d = down[row >> 3];
-                if (d >= 0) dst[d] = t;
-            }
+            wout[row] = fmaf(-c, az, r[row]);
         }
     }
 }
@@ -1862,25 +1847,44 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_smooth(Launch L, const SysStat
     }
 }
 
+// The restriction's sum for coarse row `row`: the finer level's w over the
+// row's aggregate, in list order (padding rows have empty lists: 0).
+__device__ __forceinline__ float gather_members(const int* __restrict__ mp, const int* __restrict__ mpos,
+                                                const float* __restrict__ wf, int row) {
+    float s = 0.0f;
+    const int e = mp[row + 1];
+    for (int k = mp[row]; k < e; ++k) s += wf[mpos[k]];
+    return s;
+}
+
+// Coarse level: r = P^T w (the gather) and z = omega M r.
+__global__ __launch_bounds__(kBlock, 8) void k_mg_smooth_gather(Launch L, const SysState* st,
+                                                             const BlockPre* __restrict__ M,
+                                                             const int* __restrict__ mp,
+                                                             const int* __restrict__ mpos,
+                                                             const float* __restrict__ wf, float* __restrict__ r,
+                                                             float omega, float* __restrict__ z) {
+    if (st != nullptr && !st[0].active) return;
+    const WorkItem w = work_item(L, logical_block());
+    if (w.sys < 0) return;
+    XP_WAVE_LANE
+    for (int sl = w.slice_begin + wave; sl < w.slice_end; sl += kWaves) {
+        const int row = sl * kC + lane;
+        const float rv = gather_members(mp, mpos, wf, row);
+        r[row] = rv;
+        z[row] = omega * block_apply(M, row, rv);
+    }
+}
+
 // Prolongation into the finer level, done by the kernel that produces the
-// coarse value e of row u: the 8 rows of its finer block q (up = q << 8 |
-// mask of the block's real rows) get t = cz z + gamma e, padding rows 0.
-__device__ __forceinline__ void prolong_block(const MgUp& pu, int up, float gamma, float e) {
-    const int q = up >> 8, mask = up & 0xFF;
-    const float4* zq = reinterpret_cast<const float4*>(pu.z + static_cast<size_t>(q) * 8);
-    float4* tq = reinterpret_cast<float4*>(pu.t + static_cast<size_t>(q) * 8);
-    const float4 z0 = zq[0], z1 = zq[1];
-    float4 t0, t1;
-    t0.x = (mask & 1) ? fmaf(gamma, e, pu.cz * z0.x) : 0.0f;
-    t0.y = (mask & 2) ? fmaf(gamma, e, pu.cz * z0.y) : 0.0f;
-    t0.z = (mask & 4) ? fmaf(gamma, e, pu.cz * z0.z) : 0.0f;
-    t0.w = (mask & 8) ? fmaf(gamma, e, pu.cz * z0.w) : 0.0f;
-    t1.x = (mask & 16) ? fmaf(gamma, e, pu.cz * z1.x) : 0.0f;
-    t1.y = (mask & 32) ? fmaf(gamma, e, pu.cz * z1.y) : 0.0f;
-    t1.z = (mask & 64) ? fmaf(gamma, e, pu.cz * z1.z) : 0.0f;
-    t1.w = (mask & 128) ? fmaf(gamma, e, pu.cz * z1.w) : 0.0f;
-    tq[0] = t0;
-    tq[1] = t1;
+// coarse value e of row `row`: every row of its aggregate gets
+// t = cz z + gamma e.
+__device__ __forceinline__ void prolong_members(const MgUp& pu, int row, float gamma, float e) {
+    const int end = pu.mp[row + 1];
+    for (int k = pu.mp[row]; k < end; ++k) {
+        const int i = pu.mpos[k];
+        pu.t[i] = fmaf(gamma, e, pu.cz * pu.z[i]);
+    }
 }
 
 // zout = t + omega M (r - A t); RZ: the r^T zout partials of the PCG step
@@ -1904,10 +1908,7 @@ __device__ __forceinline__ void mg_post(const Launch& L, const SellView& A, cons
         const float zz = fmaf(omega, block_apply(M, row, rv - at), td);
         zout[row] = zz;
         if (RZ) v[0] += static_cast<double>(rv) * static_cast<double>(zz);
-        if (pu.up != nullptr) {
-            const int u = pu.up[row];
-            if (u >= 0) prolong_block(pu, u, gamma, zz);
-        }
+        if (pu.mp != nullptr) prolong_members(pu, row, gamma, zz);
     }
     if (RZ) {
         block_sum<1>(v);
@@ -1934,20 +1935,23 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_post_split(Launch L, SellView 
             const float rv = r[row];
             const float zz = fmaf(omega, block_apply(M, row, rv - at), td);
             zout[row] = zz;
-            if (pu.up != nullptr) {
-                const int u = pu.up[row];
-                if (u >= 0) prolong_block(pu, u, gamma, zz);
-            }
+            if (pu.mp != nullptr) prolong_members(pu, row, gamma, zz);
         }
     }
 }
 
-// e = Cinv rc on the dense coarsest level: one wavefront per row.
-// Then the prolongation into the finer level (pu).
+// The dense coarsest level: every workgroup gathers the whole restricted
+// residual rc = P^T w into LDS (nc <= kMaxCoarse), then e = Cinv rc with one
+// wavefront per row, then the prolongation of e into the finer level (pu:
+// its lanes scatter over the row's aggregate).
+constexpr int kMaxCoarse = 4096;
 __global__ __launch_bounds__(kBlock, 8) void k_mg_coarse(const SysState* st, int nc, const float* __restrict__ cinv,
-                                                      const float* __restrict__ rc, float* __restrict__ ec,
+                                                      const float* __restrict__ wf, float* __restrict__ ec,
                                                       MgUp pu, float gamma) {
+    __shared__ float rc[kMaxCoarse];
     if (st != nullptr && !st[0].active) return;
+    for (int j = static_cast<int>(threadIdx.x); j < nc; j += kBlock) rc[j] = gather_members(pu.mp, pu.mpos, wf, j);
+    __syncthreads();
     const int row = static_cast<int>(blockIdx.x) * kWaves + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int lane = static_cast<int>(threadIdx.x) & 63;
     if (row >= nc) return;  // wave-uniform
@@ -1956,11 +1960,10 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_coarse(const SysState* st, int
     s = wave_sum(s);  // every lane holds the sum
     const float e = static_cast<float>(s);
     if (lane == 0) ec[row] = e;
-    const int u = pu.up[row];
-    if (u >= 0 && lane < 8) {
-        const int q = u >> 8;
-        const size_t i = static_cast<size_t>(q) * 8 + static_cast<size_t>(lane);
-        pu.t[i] = ((u >> lane) & 1) ? fmaf(gamma, e, pu.cz * pu.z[i]) : 0.0f;
+    const int end = pu.mp[row + 1];
+    for (int k = pu.mp[row] + lane; k < end; k += kC) {
+        const int i = pu.mpos[k];
+        pu.t[i] = fmaf(gamma, e, pu.cz * pu.z[i]);
     }
 }
 
@@ -1979,54 +1982,254 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_post_rz(Launch L, SellView A, 
     mg_post<IDX16, V, true>(L, A, st, M, r, t, omega, zout, MgUp{}, 0.0f);
 }
 
-// up[down[q]] = q << 8 | the mask of block q's real rows (the inverse of
-// the down map, for the prolongation); up starts at -1.
-__global__ void k_up_map(int nq, const int* __restrict__ down, const int* __restrict__ perm, int* __restrict__ up) {
-    const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
-    if (q >= nq) return;
-    const int d = down[q];
-    if (d < 0) return;
-    int mask = 0;
-    for (int i = 0; i < 8; ++i) mask |= (perm[q * 8 + i] >= 0 ? 1 : 0) << i;
-    up[d] = (q << 8) | mask;
+// ---- set-up of the hierarchy: plain aggregation ----
+// Strength: j != i is a strong neighbour of i when a_ij^2 >= theta^2 |a_ii|
+// |a_jj| (FP64; a_ii is the sum of the row's diagonal entries in CSR order).
+// Roots are a maximal distance-2 independent set of the strength graph,
+// chosen in Luby rounds: an undecided row becomes a root when its key is
+// the largest among the undecided rows within strong distance 2, then the
+// rows within distance 2 of a root are excluded.  Root k (index order) is
+// aggregate k, its strong neighbours join it, every other row joins the
+// aggregate of its strongest joined neighbour (largest |a_ij|, then the
+// smallest j), rows left over become aggregates of their own in index
+// order.  Every decision of a round depends on the previous round's state
+// only, so the result does not depend on the schedule.
+
+// The key of row i: the high 32 bits of MurmurHash3's 64-bit finaliser of
+// i + salt * 0x9E3779B97F4A7C15, above i + 1 (unique and nonzero).
+__device__ __forceinline__ unsigned long long agg_key(int i, int salt) {
+    unsigned long long x = static_cast<unsigned long long>(i) +
+                           static_cast<unsigned long long>(salt) * 0x9E3779B97F4A7C15ull;
+    x ^= x >> 33;
+    x *= 0xFF51AFD7ED558CCDull;
+    x ^= x >> 33;
+    x *= 0xC4CEB9FE1A85EC53ull;
+    x ^= x >> 33;
+    return ((x >> 32) << 32) | static_cast<unsigned long long>(i + 1);
 }
 
-// ---- set-up of the hierarchy ----
-// agg[natural row] = its block (unit) id.
-__global__ void k_unit_of_row(int units, const int* __restrict__ slots, int* __restrict__ agg) {
+__global__ void k_row_absdiag(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                              const double* __restrict__ va, double* __restrict__ d) {
     const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
-    if (i >= units * 8) return;
-    const int row = slots[i];
-    if (row >= 0) agg[row] = i >> 3;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+        if (ci[e] == i) s += va[e];
+    d[i] = fabs(s);
 }
 
-// Candidate count of coarse row u: the entries of its fine rows.
-__global__ void k_gal_count(int units, const int* __restrict__ slots, const int* __restrict__ rp,
-                            int* __restrict__ cnt) {
+__global__ void k_agg_strength(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const double* __restrict__ va, const double* __restrict__ d, double th2,
+                               unsigned char* __restrict__ strong) {
+#pragma clang fp contract(off)
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    const double di = d[i];
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = ci[e];
+        const double v = va[e];
+        strong[e] = (j != i && v * v >= th2 * (di * d[j])) ? 1 : 0;
+    }
+}
+
+// state: 0 undecided, 1 root, 2 excluded.
+__global__ void k_mis_max1(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const unsigned char* __restrict__ strong, const unsigned char* __restrict__ state,
+                           int salt, unsigned long long* __restrict__ m1) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    unsigned long long best = state[i] == 0 ? agg_key(i, salt) : 0ull;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = ci[e];
+        if (strong[e] && state[j] == 0) {
+            const unsigned long long k = agg_key(j, salt);
+            best = k > best ? k : best;
+        }
+    }
+    m1[i] = best;
+}
+
+__global__ void k_mis_root(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const unsigned char* __restrict__ strong, const unsigned long long* __restrict__ m1,
+                           int salt, unsigned char* __restrict__ state) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n || state[i] != 0) return;
+    unsigned long long best = m1[i];
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+        if (strong[e]) best = m1[ci[e]] > best ? m1[ci[e]] : best;
+    if (best == agg_key(i, salt)) state[i] = 1;
+}
+
+__global__ void k_mis_near(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const unsigned char* __restrict__ strong, const unsigned char* __restrict__ state,
+                           unsigned char* __restrict__ near) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    int f = state[i] == 1;
+    for (int e = rp[i]; e < rp[i + 1] && !f; ++e)
+        if (strong[e] && state[ci[e]] == 1) f = 1;
+    near[i] = static_cast<unsigned char>(f);
+}
+
+// Excludes the undecided rows near a root; *left += the rows still undecided.
+__global__ __launch_bounds__(256) void k_mis_exclude(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                     const unsigned char* __restrict__ strong,
+                                                     const unsigned char* __restrict__ near,
+                                                     unsigned char* __restrict__ state, int* __restrict__ left) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    int und = 0;
+    if (i < n && state[i] == 0) {
+        int f = near[i];
+        for (int e = rp[i]; e < rp[i + 1] && !f; ++e)
+            if (strong[e] && near[ci[e]]) f = 1;
+        if (f) state[i] = 2;
+        else und = 1;
+    }
+    __shared__ int cnt[4];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const unsigned long long b = __ballot(und);
+    if ((threadIdx.x & 63) == 0) cnt[wave] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+        if (t > 0) atomicAdd(left, t);
+    }
+}
+
+__global__ void k_flag_state(int n, const unsigned char* __restrict__ state, int* __restrict__ flag) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i < n) flag[i] = state[i] == 1;
+}
+
+// agg1: roots -> their id, their strong neighbours -> the root's id, other
+// rows -1 (two roots are never neighbours of one row).
+__global__ void k_agg_join_roots(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                 const unsigned char* __restrict__ strong, const unsigned char* __restrict__ state,
+                                 const int* __restrict__ rid, int* __restrict__ agg1) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    int a = -1;
+    if (state[i] == 1) {
+        a = rid[i];
+    } else {
+        for (int e = rp[i]; e < rp[i + 1]; ++e)
+            if (strong[e] && state[ci[e]] == 1) a = rid[ci[e]];
+    }
+    agg1[i] = a;
+}
+
+// agg: agg1, or the aggregate of the strongest neighbour with agg1 >= 0
+// (largest |a_ij|, then the smallest j); -1 when there is none.
+__global__ void k_agg_join_rest(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                const double* __restrict__ va, const unsigned char* __restrict__ strong,
+                                const int* __restrict__ agg1, int* __restrict__ agg, int* __restrict__ flag) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    int a = agg1[i];
+    if (a < 0) {
+        double best = -1.0;
+        int bj = -1;
+        for (int e = rp[i]; e < rp[i + 1]; ++e) {
+            const int j = ci[e];
+            if (!strong[e] || agg1[j] < 0) continue;
+            const double v = fabs(va[e]);
+            if (v > best || (v == best && j < bj)) {
+                best = v;
+                bj = j;
+            }
+        }
+        if (bj >= 0) a = agg1[bj];
+    }
+    agg[i] = a;
+    flag[i] = a < 0;
+}
+
+// Rows left over: aggregates nroots + (their rank among the left-over rows).
+__global__ void k_agg_left(int n, const int* __restrict__ nroots, const int* __restrict__ pos,
+                           int* __restrict__ agg) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i < n && agg[i] < 0) agg[i] = *nroots + pos[i];
+}
+
+// Member lists: cnt[a] = |aggregate a|; after the scan, fill (any order)
+// then sort each list ascending.
+__global__ void k_agg_count(int n, const int* __restrict__ agg, int* __restrict__ cnt) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i < n) atomicAdd(&cnt[agg[i]], 1);
+}
+
+__global__ void k_agg_fill(int n, const int* __restrict__ agg, const int* __restrict__ mptr, int* __restrict__ cur,
+                           int* __restrict__ mem) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i >= n) return;
+    const int a = agg[i];
+    mem[mptr[a] + atomicAdd(&cur[a], 1)] = i;
+}
+
+__global__ void k_agg_sort(int nagg, const int* __restrict__ mptr, int* __restrict__ mem) {
+    const int a = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (a >= nagg) return;
+    const int b = mptr[a], e = mptr[a + 1];
+    for (int k = b + 1; k < e; ++k) {
+        const int v = mem[k];
+        int j = k;
+        while (j > b && mem[j - 1] > v) {
+            mem[j] = mem[j - 1];
+            --j;
+        }
+        mem[j] = v;
+    }
+}
+
+// The coarse rows' member lists in the finer level's solve order: cnt[p] =
+// the size of the aggregate of coarse solve row p (perm: solve -> natural,
+// nullptr for the natural order), then mpos[mp[p] + k] = the solve position
+// of the aggregate's k-th row.
+__global__ void k_mg_member_count(int np, const int* __restrict__ perm, const int* __restrict__ mptr,
+                                  int* __restrict__ cnt) {
+    const int p = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (p >= np) return;
+    const int a = perm != nullptr ? perm[p] : p;
+    cnt[p] = a < 0 ? 0 : mptr[a + 1] - mptr[a];
+}
+
+__global__ void k_mg_member_pos(int np, const int* __restrict__ perm, const int* __restrict__ mptr,
+                                const int* __restrict__ mem, const int* __restrict__ inv_fine,
+                                const int* __restrict__ mp, int* __restrict__ mpos) {
+    const int p = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (p >= np) return;
+    const int a = perm != nullptr ? perm[p] : p;
+    if (a < 0) return;
+    const int b = mptr[a], o = mp[p];
+    for (int k = 0; k < mptr[a + 1] - b; ++k) mpos[o + k] = inv_fine[mem[b + k]];
+}
+
+// Candidate count of coarse row u: the entries of its aggregate's rows.
+__global__ void k_gal_count(int units, const int* __restrict__ mptr, const int* __restrict__ mem,
+                            const int* __restrict__ rp, int* __restrict__ cnt) {
     const int u = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (u >= units) return;
     int c = 0;
-    for (int k = 0; k < 8; ++k) {
-        const int row = slots[u * 8 + k];
-        if (row >= 0) c += rp[row + 1] - rp[row];
-    }
+    for (int k = mptr[u]; k < mptr[u + 1]; ++k) c += rp[mem[k] + 1] - rp[mem[k]];
     cnt[u] = c;
 }
 
-// Coarse row u of P^T A P: the entries (agg[col], value) of its fine rows,
-// equal coarse columns summed in the order they come, the distinct columns
-// sorted ascending and written from off[u]; their count in dcnt[u].  The
-// distinct list is kept in LDS (kGalCap per thread, column-major so the 64
-// threads of a workgroup touch 64 banks); a row with more distinct
-// columns sorts all its candidates in place in global memory instead
-// (insertion sort, stable: the same sums in the same order).
+// Coarse row u of P^T A P: the entries (agg[col], value) of its aggregate's
+// rows (ascending) in CSR order, equal coarse columns summed in the order
+// they come, the distinct columns sorted ascending and written from off[u];
+// their count in dcnt[u].  The distinct list is kept in LDS (kGalCap per
+// thread, column-major so the 64 threads of a workgroup touch 64 banks); a
+// row with more distinct columns sorts all its candidates in place in
+// global memory instead (insertion sort, stable: the same sums in the same
+// order).
 constexpr int kGalThreads = 64;
 constexpr int kGalCap = 96;
-__global__ __launch_bounds__(kGalThreads) void k_gal_fill(int units, const int* __restrict__ slots,
-                                                          const int* __restrict__ rp, const int* __restrict__ ci,
-                                                          const double* __restrict__ va, const int* __restrict__ agg,
-                                                          const int* __restrict__ off, int* __restrict__ ccol,
-                                                          double* __restrict__ cval, int* __restrict__ dcnt) {
+__global__ __launch_bounds__(kGalThreads) void k_gal_fill(int units, const int* __restrict__ mptr,
+                                                          const int* __restrict__ mem, const int* __restrict__ rp,
+                                                          const int* __restrict__ ci, const double* __restrict__ va,
+                                                          const int* __restrict__ agg, const int* __restrict__ off,
+                                                          int* __restrict__ ccol, double* __restrict__ cval,
+                                                          int* __restrict__ dcnt) {
     __shared__ int lcol[kGalCap * kGalThreads];
     __shared__ double lval[kGalCap * kGalThreads];
     const int tid = static_cast<int>(threadIdx.x);
@@ -2036,9 +2239,8 @@ __global__ __launch_bounds__(kGalThreads) void k_gal_fill(int units, const int* 
     double* val = lval + tid;
     int d = 0;
     bool over = false;
-    for (int k = 0; k < 8 && !over; ++k) {
-        const int row = slots[u * 8 + k];
-        if (row < 0) continue;
+    for (int k = mptr[u]; k < mptr[u + 1] && !over; ++k) {
+        const int row = mem[k];
         for (int e = rp[row]; e < rp[row + 1]; ++e) {
             const int c = agg[ci[e]];
             int j = 0;
@@ -2077,9 +2279,8 @@ __global__ __launch_bounds__(kGalThreads) void k_gal_fill(int units, const int* 
         return;
     }
     int m = 0;
-    for (int k = 0; k < 8; ++k) {
-        const int row = slots[u * 8 + k];
-        if (row < 0) continue;
+    for (int k = mptr[u]; k < mptr[u + 1]; ++k) {
+        const int row = mem[k];
         for (int e = rp[row]; e < rp[row + 1]; ++e) {
             const int c = agg[ci[e]];
             const double v = va[e];
@@ -2123,16 +2324,6 @@ __global__ void k_gal_compact(int units, const int* __restrict__ off, const int*
 __global__ void k_inv_perm(int n_pad, const int* __restrict__ perm, int* __restrict__ inv) {
     const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (i < n_pad && perm[i] >= 0) inv[perm[i]] = i;
-}
-
-// down[q] = position of block q's unit in the next level (through inv when
-// the next level has a solve order), -1 for a block of padding rows.
-__global__ void k_down_map(int nq, const int* __restrict__ block_unit, const int* __restrict__ inv,
-                           int* __restrict__ down) {
-    const int q = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
-    if (q >= nq) return;
-    const int u = block_unit[q];
-    down[q] = u < 0 ? -1 : (inv != nullptr ? inv[u] : u);
 }
 
 // The exact inverse of every diagonal 8x8 block of a SELL matrix (FP32
@@ -2671,7 +2862,6 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
         }                                                                                    \
     } while (0)
     // level 0: z0 = M r from the update kernel; the pre-smoothed approximation is omega z0
-    float* first_dst = P.nlev > 1 ? P.lv[1].r : P.rc;
     // the restriction writes no partial sums: it may take its own workgroup
     // count (one system: the uniform split needs no work map)
     Launch Lr = L0;
@@ -2679,22 +2869,22 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
         Lr.grid = P.grid0;
         Lr.u_blocks = P.grid0;
     }
-    XP_MG_SELL(Lr, f.A, k_mg_restrict, Lr, f.A, st, r0, z0, P.omega, f.down, first_dst);
+    XP_MG_SELL(Lr, f.A, k_mg_restrict, Lr, f.A, st, r0, z0, P.omega, f.w);
     // a workgroup with at most one slice shares it among its 16 wavefronts
     auto split = [](const MgLevelView& v) { return v.L.grid >= v.A.nslices; };
     for (int l = 1; l < P.nlev; ++l) {
         const MgLevelView& v = P.lv[l];
-        hipLaunchKernelGGL(k_mg_smooth, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, st, v.M, v.r, P.omega, v.z);
-        float* dst = l + 1 < P.nlev ? P.lv[l + 1].r : P.rc;
+        hipLaunchKernelGGL(k_mg_smooth_gather, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, st, v.M, v.mp,
+                           v.mpos, P.lv[l - 1].w, v.r, P.omega, v.z);
         if (split(v)) {
             if (v.A.idx16)
                 hipLaunchKernelGGL(k_mg_restrict_split<true>, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, v.A,
-                                   st, v.r, v.z, 1.0f, v.down, dst);
+                                   st, v.r, v.z, 1.0f, v.w);
             else
                 hipLaunchKernelGGL(k_mg_restrict_split<false>, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, v.A,
-                                   st, v.r, v.z, 1.0f, v.down, dst);
+                                   st, v.r, v.z, 1.0f, v.w);
         } else {
-            XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.down, dst);
+            XP_MG_SELL(v.L, v.A, k_mg_restrict, v.L, v.A, st, v.r, v.z, 1.0f, v.w);
         }
     }
     // the prolongation into level l is done by the kernel that produces
@@ -2708,14 +2898,16 @@ void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const 
     };
     {
         MgUp pu = up_into(P.nlev - 1);
-        pu.up = P.cup;
+        pu.mp = P.cmp;
+        pu.mpos = P.cmpos;
         hipLaunchKernelGGL(k_mg_coarse, dim3((P.nc + kWaves - 1) / kWaves), dim3(kBlock), 0, L0.stream, st, P.nc,
-                           P.cinv, P.rc, P.ec, pu, P.gamma);
+                           P.cinv, P.lv[P.nlev - 1].w, P.ec, pu, P.gamma);
     }
     for (int l = P.nlev - 1; l >= 1; --l) {
         const MgLevelView& v = P.lv[l];
         MgUp pu = up_into(l - 1);
-        pu.up = v.up;
+        pu.mp = v.mp;
+        pu.mpos = v.mpos;
         if (split(v)) {
             if (v.A.idx16)
                 hipLaunchKernelGGL(k_mg_post_split<true>, dim3(v.L.grid), dim3(kBlock), 0, v.L.stream, v.L, v.A, st,
@@ -2735,46 +2927,86 @@ void launch_mg_block_apply(const Launch& L, const BlockPre* M, const float* r, f
     hipLaunchKernelGGL(k_mg_smooth, dim3(L.grid), dim3(kBlock), 0, L.stream, L, nullptr, M, r, 1.0f, z);
 }
 
-void launch_mg_unit_of_row(int units, const int* slots, int* agg, hipStream_t s) {
-    if (units > 0)
-        hipLaunchKernelGGL(k_unit_of_row, dim3(blocks_for(static_cast<size_t>(units) * 8, 256)), dim3(256), 0, s,
-                           units, slots, agg);
+namespace {
+inline dim3 rows256(int n) { return dim3(blocks_for(static_cast<size_t>(n > 0 ? n : 1), 256)); }
+}  // namespace
+
+void launch_mg_absdiag(int n, const int* rp, const int* ci, const double* va, double* d, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_row_absdiag, rows256(n), dim3(256), 0, s, n, rp, ci, va, d);
 }
 
-void launch_mg_galerkin_count(int units, const int* slots, const int* rp, int* cnt, hipStream_t s) {
-    if (units > 0)
-        hipLaunchKernelGGL(k_gal_count, dim3(blocks_for(static_cast<size_t>(units), 256)), dim3(256), 0, s, units,
-                           slots, rp, cnt);
+void launch_mg_strength(int n, const int* rp, const int* ci, const double* va, const double* d, double theta,
+                        unsigned char* strong, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_agg_strength, rows256(n), dim3(256), 0, s, n, rp, ci, va, d, theta * theta, strong);
 }
 
-void launch_mg_galerkin_fill(int units, const int* slots, const int* rp, const int* ci, const double* va,
-                             const int* agg, const int* off, int* ccol, double* cval, int* dcnt, hipStream_t s) {
+void launch_mg_mis_round(int n, const int* rp, const int* ci, const unsigned char* strong, int salt,
+                         unsigned char* state, unsigned long long* m1, unsigned char* near, int* left, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mis_max1, rows256(n), dim3(256), 0, s, n, rp, ci, strong, state, salt, m1);
+    hipLaunchKernelGGL(k_mis_root, rows256(n), dim3(256), 0, s, n, rp, ci, strong, m1, salt, state);
+    hipLaunchKernelGGL(k_mis_near, rows256(n), dim3(256), 0, s, n, rp, ci, strong, state, near);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, s, static_cast<size_t>(1), 0, left);
+    hipLaunchKernelGGL(k_mis_exclude, rows256(n), dim3(256), 0, s, n, rp, ci, strong, near, state, left);
+}
+
+void launch_mg_flag_roots(int n, const unsigned char* state, int* flag, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_flag_state, rows256(n), dim3(256), 0, s, n, state, flag);
+}
+
+void launch_mg_join(int n, const int* rp, const int* ci, const double* va, const unsigned char* strong,
+                    const unsigned char* state, const int* rid, int* agg1, int* agg, int* flag, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_agg_join_roots, rows256(n), dim3(256), 0, s, n, rp, ci, strong, state, rid, agg1);
+    hipLaunchKernelGGL(k_agg_join_rest, rows256(n), dim3(256), 0, s, n, rp, ci, va, strong, agg1, agg, flag);
+}
+
+void launch_mg_left(int n, const int* nroots, const int* pos, int* agg, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_agg_left, rows256(n), dim3(256), 0, s, n, nroots, pos, agg);
+}
+
+void launch_mg_member_count(int n, const int* agg, int nagg, int* cnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill<int>, rows256(nagg), dim3(256), 0, s, static_cast<size_t>(nagg), 0, cnt);
+    if (n > 0) hipLaunchKernelGGL(k_agg_count, rows256(n), dim3(256), 0, s, n, agg, cnt);
+}
+
+void launch_mg_member_fill(int n, const int* agg, int nagg, const int* mptr, int* cur, int* mem, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill<int>, rows256(nagg), dim3(256), 0, s, static_cast<size_t>(nagg), 0, cur);
+    if (n > 0) hipLaunchKernelGGL(k_agg_fill, rows256(n), dim3(256), 0, s, n, agg, mptr, cur, mem);
+    if (nagg > 0) hipLaunchKernelGGL(k_agg_sort, rows256(nagg), dim3(256), 0, s, nagg, mptr, mem);
+}
+
+void launch_mg_member_solve_count(int np, const int* perm, const int* mptr, int* cnt, hipStream_t s) {
+    if (np > 0) hipLaunchKernelGGL(k_mg_member_count, rows256(np), dim3(256), 0, s, np, perm, mptr, cnt);
+}
+
+void launch_mg_member_solve_pos(int np, const int* perm, const int* mptr, const int* mem, const int* inv_fine,
+                                const int* mp, int* mpos, hipStream_t s) {
+    if (np > 0)
+        hipLaunchKernelGGL(k_mg_member_pos, rows256(np), dim3(256), 0, s, np, perm, mptr, mem, inv_fine, mp, mpos);
+}
+
+void launch_mg_galerkin_count(int units, const int* mptr, const int* mem, const int* rp, int* cnt, hipStream_t s) {
+    if (units > 0)
+        hipLaunchKernelGGL(k_gal_count, rows256(units), dim3(256), 0, s, units, mptr, mem, rp, cnt);
+}
+
+void launch_mg_galerkin_fill(int units, const int* mptr, const int* mem, const int* rp, const int* ci,
+                             const double* va, const int* agg, const int* off, int* ccol, double* cval, int* dcnt,
+                             hipStream_t s) {
     if (units > 0)
         hipLaunchKernelGGL(k_gal_fill, dim3(blocks_for(static_cast<size_t>(units), kGalThreads)), dim3(kGalThreads),
-                           0, s, units, slots, rp, ci, va, agg, off, ccol, cval, dcnt);
+                           0, s, units, mptr, mem, rp, ci, va, agg, off, ccol, cval, dcnt);
 }
 
 void launch_mg_galerkin_compact(int units, const int* off, const int* crp, const int* ccol, const double* cval,
                                 int* cci, double* cva, hipStream_t s) {
     if (units > 0)
-        hipLaunchKernelGGL(k_gal_compact, dim3(blocks_for(static_cast<size_t>(units), 256)), dim3(256), 0, s, units,
-                           off, crp, ccol, cval, cci, cva);
+        hipLaunchKernelGGL(k_gal_compact, rows256(units), dim3(256), 0, s, units, off, crp, ccol, cval, cci, cva);
 }
 
 void launch_mg_inv_perm(int n_pad, const int* perm, int* inv, hipStream_t s) {
-    hipLaunchKernelGGL(k_inv_perm, dim3(blocks_for(static_cast<size_t>(n_pad), 256)), dim3(256), 0, s, n_pad, perm,
-                       inv);
-}
-
-void launch_mg_up_map(int nq, const int* down, const int* perm, int* up, int nup, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill<int>, dim3(blocks_for(static_cast<size_t>(nup), 256)), dim3(256), 0, s,
-                       static_cast<size_t>(nup), -1, up);
-    hipLaunchKernelGGL(k_up_map, dim3(blocks_for(static_cast<size_t>(nq), 256)), dim3(256), 0, s, nq, down, perm, up);
-}
-
-void launch_mg_down_map(int nq, const int* block_unit, const int* inv, int* down, hipStream_t s) {
-    hipLaunchKernelGGL(k_down_map, dim3(blocks_for(static_cast<size_t>(nq), 256)), dim3(256), 0, s, nq, block_unit,
-                       inv, down);
+    hipLaunchKernelGGL(k_inv_perm, rows256(n_pad), dim3(256), 0, s, n_pad, perm, inv);
 }
 
 void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t s) {

@@ -78,22 +78,26 @@ void launch_pcg_flush_x_acc(const Launch& L, int nsys, SysState* st, const doubl
 
 // ---- multilevel (aggregation) preconditioner ----
 constexpr int kMaxLevels = 16;
-// One SELL level of the hierarchy (level 0: the solver's matrix).
+// One SELL level of the hierarchy (level 0: the solver's matrix).  Row u of
+// level l + 1 is aggregate u of level l; mp / mpos list, for every solve row
+// of level l >= 1, its aggregate's rows as solve positions of level l - 1.
 struct MgLevelView {
     Launch L;                    // the level's launch context (level 0: unused, the solver's is passed)
     SellView A;
     const BlockPre* M = nullptr; // block factors of the smoother
     const int* perm = nullptr;   // solve row -> natural row, -1 for padding rows
-    const int* down = nullptr;   // [n_pad / 8] block -> its row in the next level, -1 for padding blocks
     float* r = nullptr;          // level >= 1: restricted residual
     float* z = nullptr;          // level >= 1: the level's correction
+    float* w = nullptr;          // the residual r - A z the restriction sums (every level)
     float* t = nullptr;          // prolonged approximation (every level)
-    const int* up = nullptr;     // level >= 1: row -> q << 8 | real-row mask of level l - 1's block q (-1: none)
+    const int* mp = nullptr;     // level >= 1: [n_pad + 1] list offsets (padding rows: empty)
+    const int* mpos = nullptr;   // level >= 1: the lists
 };
 // Where a kernel that produces a coarse correction e prolongs it: the finer
-// level's t = cz z + gamma e on the rows of the block up names.
+// level's t = cz z + gamma e on the rows of the coarse row's aggregate.
 struct MgUp {
-    const int* up = nullptr;
+    const int* mp = nullptr;
+    const int* mpos = nullptr;
     float* t = nullptr;
     const float* z = nullptr;
     float cz = 1.0f;
@@ -101,11 +105,11 @@ struct MgUp {
 struct MgPlan {
     int nlev = 0;                // SELL levels; the dense coarsest level follows them
     MgLevelView lv[kMaxLevels];
-    int nc = 0;                  // rows of the coarsest level
+    int nc = 0;                  // rows of the coarsest level (at most 4096)
     const float* cinv = nullptr; // its dense inverse, nc x nc row-major FP32
-    const int* cup = nullptr;    // its rows -> the last SELL level's blocks (as MgLevelView::up)
-    float* rc = nullptr;
-    float* ec = nullptr;
+    const int* cmp = nullptr;    // its rows' aggregates in the last SELL level (as MgLevelView::mp / mpos)
+    const int* cmpos = nullptr;
+    float* ec = nullptr;         // its correction
     int grid0 = 0;               // workgroups of level 0's restriction (0: the solver's)
     float omega = 0.7f;          // smoother damping
     float gamma = 1.5f;          // coarse-correction scale
@@ -117,17 +121,41 @@ struct MgPlan {
 void launch_vcycle(const MgPlan& P, const Launch& L0, const SysState* st, const float* r0, float* z0);
 // z = M r (the level-0 block product alone).
 void launch_mg_block_apply(const Launch& L, const BlockPre* M, const float* r, float* z);
-void launch_mg_unit_of_row(int units, const int* slots, int* agg, hipStream_t s);
-void launch_mg_galerkin_count(int units, const int* slots, const int* rp, int* cnt, hipStream_t s);
-void launch_mg_galerkin_fill(int units, const int* slots, const int* rp, const int* ci, const double* va,
-                             const int* agg, const int* off, int* ccol, double* cval, int* dcnt, hipStream_t s);
+// Plain aggregation of a natural-order CSR (n rows): d = |diagonal|,
+// strong[e] = entry e is a strong coupling (theta), then rounds of root
+// selection (state: 0 undecided, 1 root, 2 excluded; m1, near: work;
+// *left = rows still undecided after the round).
+void launch_mg_absdiag(int n, const int* rp, const int* ci, const double* va, double* d, hipStream_t s);
+void launch_mg_strength(int n, const int* rp, const int* ci, const double* va, const double* d, double theta,
+                        unsigned char* strong, hipStream_t s);
+void launch_mg_mis_round(int n, const int* rp, const int* ci, const unsigned char* strong, int salt,
+                         unsigned char* state, unsigned long long* m1, unsigned char* near, int* left, hipStream_t s);
+// flag[i] = row i is a root.
+void launch_mg_flag_roots(int n, const unsigned char* state, int* flag, hipStream_t s);
+// agg1 / agg: the two joins (rid: the roots' ranks); flag[i] = row i is
+// left over.
+void launch_mg_join(int n, const int* rp, const int* ci, const double* va, const unsigned char* strong,
+                    const unsigned char* state, const int* rid, int* agg1, int* agg, int* flag, hipStream_t s);
+// Left-over rows: agg = *nroots + pos.
+void launch_mg_left(int n, const int* nroots, const int* pos, int* agg, hipStream_t s);
+// Member lists (natural order): cnt = aggregate sizes; after the scan to
+// mptr, mem = each aggregate's rows ascending (cur: work, nagg entries).
+void launch_mg_member_count(int n, const int* agg, int nagg, int* cnt, hipStream_t s);
+void launch_mg_member_fill(int n, const int* agg, int nagg, const int* mptr, int* cur, int* mem, hipStream_t s);
+// The same lists per coarse solve row p (perm: coarse solve -> natural, or
+// nullptr), as solve positions of the finer level (inv_fine: natural ->
+// solve): cnt first, then after the scan to mp, mpos.
+void launch_mg_member_solve_count(int np, const int* perm, const int* mptr, int* cnt, hipStream_t s);
+void launch_mg_member_solve_pos(int np, const int* perm, const int* mptr, const int* mem, const int* inv_fine,
+                                const int* mp, int* mpos, hipStream_t s);
+// Galerkin product P^T A P over the member lists (units coarse rows).
+void launch_mg_galerkin_count(int units, const int* mptr, const int* mem, const int* rp, int* cnt, hipStream_t s);
+void launch_mg_galerkin_fill(int units, const int* mptr, const int* mem, const int* rp, const int* ci,
+                             const double* va, const int* agg, const int* off, int* ccol, double* cval, int* dcnt,
+                             hipStream_t s);
 void launch_mg_galerkin_compact(int units, const int* off, const int* crp, const int* ccol, const double* cval,
                                 int* cci, double* cva, hipStream_t s);
 void launch_mg_inv_perm(int n_pad, const int* perm, int* inv, hipStream_t s);
-void launch_mg_down_map(int nq, const int* block_unit, const int* inv, int* down, hipStream_t s);
-// up (nup entries, -1 first) = the inverse of down with the real-row masks
-// of the finer level's blocks (perm: the finer level's solve -> natural).
-void launch_mg_up_map(int nq, const int* down, const int* perm, int* up, int nup, hipStream_t s);
 // Exact inverses of the diagonal 8x8 blocks (solve order) as full FP32 blocks.
 void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t s);
 // out = inverse of the n x n SPD matrix of a CSR (FP64 Gauss-Jordan, one

@@ -476,27 +476,28 @@ void ensure_host_mirror(const xp_matrix& cm) {
 }  // namespace
 
 // ---- solver ------------------------------------------------------------------
-// The multilevel preconditioner's hierarchy (xp_pcg_set_multilevel): level
+// The multilevel preconditioner's hierarchy, all of it on the device: level
 // 0 is the solver's matrix with its block factors; level l + 1 is P^T A P
-// over the 8-row aggregates of level l, set up on the device like any
-// device-built matrix (its own rule-3 aggregates, SELL layout, exact block
-// factors), down to a dense coarsest level of at most coarse_max rows.
+// over the plain aggregates of level l, set up on the device like any
+// device-built matrix (its own 8-row smoother blocks, SELL layout, exact
+// block factors), down to a dense coarsest level of at most coarse_max rows.
 struct MgHierarchy {
     std::vector<std::unique_ptr<xp_matrix>> mats;  // levels 1 .. nlev - 1
     std::vector<xp::DevBuf<xp::BlockPre>> M;      // [nlev]; level 0 uses the solver's blocks
-    std::vector<xp::DevBuf<int>> down;            // [nlev]
-    std::vector<xp::DevBuf<float>> r, z, t;       // [nlev]; level 0 only t
-    std::vector<xp::DevBuf<int>> up;              // [nlev]; level l >= 1: rows -> level l - 1's blocks
-    xp::DevBuf<float> cinv, rc, ec;               // the dense coarsest level
-    xp::DevBuf<int> cup;                          // its rows -> the last SELL level's blocks
-    xp::DevBuf<int> c_rp, c_ci, c_slots;          // its CSR (natural) and the aggregates it came from
+    std::vector<xp::DevBuf<float>> r, z, w, t;    // [nlev]; level 0 only w and t
+    std::vector<xp::DevBuf<int>> mp, mpos;        // [nlev]; level l >= 1: its rows' aggregates in level l - 1
+    std::vector<xp::DevBuf<int>> agg;             // [nlev]: natural row of level l -> its aggregate
+    xp::DevBuf<float> cinv, ec;                   // the dense coarsest level
+    xp::DevBuf<int> cmp, cmpos;                   // its rows' aggregates in the last SELL level
+    xp::DevBuf<int> c_rp, c_ci;                   // its CSR (natural order)
     xp::DevBuf<double> c_va;
     int nc = 0;
     int sigma0 = 0;                               // level 0's sort window at build time
     int grid0 = 0;                                // workgroups of level 0's restriction
     float omega = 0.7f, gamma = 1.5f;
     double setup_ms = 0.0;
-    std::vector<double> phase_ms;                 // per level: Galerkin product, matrix set-up, block factors
+    std::vector<double> phase_ms;                 // per level: aggregation, Galerkin product, coarse set-up
+    std::vector<int> rounds;                      // per level: root-selection rounds of the aggregation
 };
 
 struct xp_pcg {
@@ -554,25 +555,27 @@ struct xp_pcg {
         P.lv[0].A = m->view();
         P.lv[0].M = blockM.p;
         P.lv[0].perm = m->d_perm.p;
-        P.lv[0].down = H.down[0].p;
+        P.lv[0].w = H.w[0].p;
         P.lv[0].t = H.t[0].p;
         for (int l = 1; l < P.nlev; ++l) {
             const xp_matrix& c = *H.mats[static_cast<size_t>(l) - 1];
+            const size_t k = static_cast<size_t>(l);
             xp::MgLevelView& v = P.lv[l];
             v.L = c.launch(stream);
             v.A = c.view();
-            v.M = H.M[static_cast<size_t>(l)].p;
+            v.M = H.M[k].p;
             v.perm = c.d_perm.p;
-            v.down = H.down[static_cast<size_t>(l)].p;
-            v.r = H.r[static_cast<size_t>(l)].p;
-            v.z = H.z[static_cast<size_t>(l)].p;
-            v.t = H.t[static_cast<size_t>(l)].p;
-            v.up = H.up[static_cast<size_t>(l)].p;
+            v.r = H.r[k].p;
+            v.z = H.z[k].p;
+            v.w = H.w[k].p;
+            v.t = H.t[k].p;
+            v.mp = H.mp[k].p;
+            v.mpos = H.mpos[k].p;
         }
         P.nc = H.nc;
         P.cinv = H.cinv.p;
-        P.cup = H.cup.p;
-        P.rc = H.rc.p;
+        P.cmp = H.cmp.p;
+        P.cmpos = H.cmpos.p;
         P.ec = H.ec.p;
         P.omega = H.omega;
         P.gamma = H.gamma;
@@ -3397,47 +3400,111 @@ namespace {
 // Work buffers of the hierarchy's set-up, sized by the largest level and
 // reused by every level (allocations are the bulk of a small level's cost).
 struct GalerkinWork {
-    xp::DevBuf<int> agg, cnt, off, dcnt, ccol, crp, cci;
+    xp::DevBuf<double> d;                               // |diagonal|
+    xp::DevBuf<unsigned char> strong, state, near;      // per entry / per row
+    xp::DevBuf<unsigned long long> m1;
+    xp::DevBuf<int> flag, rid, agg1, lpos, left;        // aggregation
+    xp::DevBuf<int> cnt, mptr, cur, mem;                // member lists (natural order)
+    xp::DevBuf<int> gcnt, off, dcnt, ccol, crp, cci;    // Galerkin product
     xp::DevBuf<double> cval, cva;
 };
 
-// Galerkin coarse matrix of a device-built block matrix c: row u = block
-// (unit) u, entry (u, v) = sum of a_ij over natural rows i of u and j of v.
-// The result is left in w.crp / w.cci / w.cva; returns its entry count.
-int galerkin(const xp_matrix& c, GalerkinWork& w, hipStream_t s) {
-    const int units = c.unit_first.back();
-    const int n = c.n;
-    const size_t nnz = static_cast<size_t>(c.nnz);  // the CSR's entries (device set-up)
-    w.agg.ensure(static_cast<size_t>(n));
-    w.cnt.ensure(static_cast<size_t>(units));
-    w.off.ensure(static_cast<size_t>(units) + 1);
-    w.dcnt.ensure(static_cast<size_t>(units));
+constexpr int kMaxAggRounds = 256;
+
+// Plain aggregation of a natural-order CSR (n rows, nnz entries) on the
+// device into agg; returns the aggregate count.  Leaves the member lists
+// (w.mptr, w.mem) behind.
+int plain_aggregate(int n, const int* rp, const int* ci, const double* va, size_t nnz, double theta, int salt,
+                    GalerkinWork& w, int* agg, int& rounds, hipStream_t s) {
+    const size_t nn = static_cast<size_t>(n);
+    w.d.ensure(nn);
+    w.strong.ensure(std::max<size_t>(nnz, 1));
+    w.state.ensure(nn);
+    w.near.ensure(nn);
+    w.m1.ensure(nn);
+    w.flag.ensure(nn);
+    w.rid.ensure(nn + 1);
+    w.agg1.ensure(nn);
+    w.lpos.ensure(nn + 1);
+    w.left.ensure(1);
+    xp::launch_mg_absdiag(n, rp, ci, va, w.d.p, s);
+    xp::launch_mg_strength(n, rp, ci, va, w.d.p, theta, w.strong.p, s);
+    HIPX(hipMemsetAsync(w.state.p, 0, nn, s));
+    rounds = 0;
+    for (;;) {
+        if (++rounds > kMaxAggRounds) throw std::runtime_error("multilevel: the aggregation did not finish");
+        xp::launch_mg_mis_round(n, rp, ci, w.strong.p, salt, w.state.p, w.m1.p, w.near.p, w.left.p, s);
+        int left = 0;
+        HIPX(hipMemcpyAsync(&left, w.left.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        if (left == 0) break;
+    }
+    xp::launch_mg_flag_roots(n, w.state.p, w.flag.p, s);
+    xp::device_scan_exclusive(w.flag.p, w.rid.p, n, w.rid.p + n, s);
+    xp::launch_mg_join(n, rp, ci, va, w.strong.p, w.state.p, w.rid.p, w.agg1.p, agg, w.flag.p, s);
+    xp::device_scan_exclusive(w.flag.p, w.lpos.p, n, w.lpos.p + n, s);
+    xp::launch_mg_left(n, w.rid.p + n, w.lpos.p, agg, s);
+    int tot[2] = {0, 0};
+    HIPX(hipMemcpyAsync(&tot[0], w.rid.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&tot[1], w.lpos.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    const int nagg = tot[0] + tot[1];
+    w.cnt.ensure(static_cast<size_t>(nagg));
+    w.mptr.ensure(static_cast<size_t>(nagg) + 1);
+    w.cur.ensure(static_cast<size_t>(nagg));
+    w.mem.ensure(nn);
+    xp::launch_mg_member_count(n, agg, nagg, w.cnt.p, s);
+    xp::device_scan_exclusive(w.cnt.p, w.mptr.p, nagg, w.mptr.p + nagg, s);
+    xp::launch_mg_member_fill(n, agg, nagg, w.mptr.p, w.cur.p, w.mem.p, s);
+    return nagg;
+}
+
+// Galerkin coarse matrix P^T A P of a natural-order CSR over the aggregates
+// agg (member lists in w from plain_aggregate): row u = aggregate u.  The
+// result is left in w.crp / w.cci / w.cva; returns its entry count.
+int galerkin(const int* rp, const int* ci, const double* va, size_t nnz, int nagg, const int* agg, GalerkinWork& w,
+             hipStream_t s) {
+    const size_t na = static_cast<size_t>(nagg);
+    w.gcnt.ensure(na);
+    w.off.ensure(na + 1);
+    w.dcnt.ensure(na);
     w.ccol.ensure(std::max<size_t>(nnz, 1));
     w.cval.ensure(std::max<size_t>(nnz, 1));
-    w.crp.ensure(static_cast<size_t>(units) + 1);
-    xp::launch_mg_unit_of_row(units, c.d_slots.p, w.agg.p, s);
-    xp::launch_mg_galerkin_count(units, c.d_slots.p, c.d_rp.p, w.cnt.p, s);
-    xp::device_scan_exclusive(w.cnt.p, w.off.p, units, w.off.p + units, s);
-    xp::launch_mg_galerkin_fill(units, c.d_slots.p, c.d_rp.p, c.d_ci.p, c.d_va.p, w.agg.p, w.off.p, w.ccol.p,
-                                w.cval.p, w.dcnt.p, s);
-    xp::device_scan_exclusive(w.dcnt.p, w.crp.p, units, w.crp.p + units, s);
+    w.crp.ensure(na + 1);
+    xp::launch_mg_galerkin_count(nagg, w.mptr.p, w.mem.p, rp, w.gcnt.p, s);
+    xp::device_scan_exclusive(w.gcnt.p, w.off.p, nagg, w.off.p + nagg, s);
+    xp::launch_mg_galerkin_fill(nagg, w.mptr.p, w.mem.p, rp, ci, va, agg, w.off.p, w.ccol.p, w.cval.p, w.dcnt.p, s);
+    xp::device_scan_exclusive(w.dcnt.p, w.crp.p, nagg, w.crp.p + nagg, s);
     int cnnz = 0;
-    HIPX(hipMemcpyAsync(&cnnz, w.crp.p + units, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&cnnz, w.crp.p + nagg, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     w.cci.ensure(std::max<size_t>(static_cast<size_t>(cnnz), 1));
     w.cva.ensure(std::max<size_t>(static_cast<size_t>(cnnz), 1));
-    xp::launch_mg_galerkin_compact(units, w.off.p, w.crp.p, w.ccol.p, w.cval.p, w.cci.p, w.cva.p, s);
+    xp::launch_mg_galerkin_compact(nagg, w.off.p, w.crp.p, w.ccol.p, w.cval.p, w.cci.p, w.cva.p, s);
     return cnnz;
+}
+
+// The coarse rows' member lists as solve positions of the finer level
+// (perm: coarse solve -> natural, or nullptr for the dense coarsest level).
+void member_lists(int np, const int* perm, const GalerkinWork& w, const int* inv_fine, int n_fine,
+                  xp::DevBuf<int>& mp, xp::DevBuf<int>& mpos, hipStream_t s) {
+    mp.alloc(static_cast<size_t>(np) + 1);
+    mpos.alloc(std::max<size_t>(static_cast<size_t>(n_fine), 1));
+    xp::DevBuf<int> cnt(static_cast<size_t>(std::max(np, 1)));
+    xp::launch_mg_member_solve_count(np, perm, w.mptr.p, cnt.p, s);
+    xp::device_scan_exclusive(cnt.p, mp.p, np, mp.p + np, s);
+    xp::launch_mg_member_solve_pos(np, perm, w.mptr.p, w.mem.p, inv_fine, mp.p, mpos.p, s);
+    HIPX(hipStreamSynchronize(s));  // before cnt goes
 }
 
 void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
     const int coarse_max = mo.coarse_max, coarse_blocking = mo.coarse_blocking;
-    const double omega = mo.omega, gamma = mo.gamma;
+    const double omega = mo.omega, gamma = mo.gamma, theta = mo.theta;
     xp_matrix& m0 = *S.m;
     if (!S.opts.accurate) throw std::runtime_error("the multilevel preconditioner runs in the accurate iteration");
-    if (!m0.device_built || m0.block_rows != 8 || m0.blocking < 3 || m0.nsys != 1)
+    if (!m0.device_built || m0.block_rows != 8 || m0.nsys != 1)
         throw std::runtime_error("the multilevel preconditioner needs one system set up on the device with "
-                                 "block_rows = 8 and blocking 3 or 4");
+                                 "block_rows = 8");
     if (coarse_blocking != 3 && coarse_blocking != 4) throw std::runtime_error("coarse blocking must be 3 or 4");
     if (!S.use_block)
         throw std::runtime_error("install the level-0 block preconditioner (xp_pcg_set_preconditioner / "
@@ -3445,6 +3512,7 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
     if (coarse_max < 8 || coarse_max > 4096) throw std::runtime_error("coarse_max must be in [8, 4096]");
     if (!(omega > 0.0) || !(omega < 2.0) || !(gamma > 0.0) || !(gamma < 4.0))
         throw std::runtime_error("bad multilevel parameters (0 < omega < 2, 0 < gamma < 4)");
+    if (!(theta >= 0.0) || !(theta < 1.0)) throw std::runtime_error("bad strength threshold (0 <= theta < 1)");
     const hipStream_t s = device().stream;
     HIPX(hipStreamSynchronize(S.stream));
     auto t0 = std::chrono::steady_clock::now();
@@ -3457,10 +3525,12 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
     H->grid0 = default_grid(m0);
     const xp_matrix* cur = &m0;
     GalerkinWork work;
+    xp::DevBuf<int> inv_cur(static_cast<size_t>(std::max(m0.n, 1)));  // natural -> solve row of cur
+    xp::launch_mg_inv_perm(m0.n_pad, m0.d_perm.p, inv_cur.p, s);
     for (int l = 0;; ++l) {
         if (l + 1 >= xp::kMaxLevels) throw std::runtime_error("multilevel: too many levels");
-        const int units = cur->unit_first.back();
-        const int nq = cur->n_pad / 8;
+        const int n = cur->n;
+        const size_t nnz = static_cast<size_t>(cur->nnz);  // the CSR's entries (device set-up)
         auto tp = std::chrono::steady_clock::now();
         auto lap = [&tp]() {
             const auto now = std::chrono::steady_clock::now();
@@ -3468,73 +3538,76 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
             tp = now;
             return ms;
         };
-        const int cnnz = galerkin(*cur, work, s);
-        H->phase_ms.push_back(lap());
-        if (nq >= (1 << 23)) throw std::runtime_error("multilevel: a level has too many blocks");
-        H->down.emplace_back(static_cast<size_t>(nq));
-        H->t.emplace_back(static_cast<size_t>(cur->n_pad));
-        xp::launch_fill_f32(static_cast<size_t>(cur->n_pad), 0.0f, H->t.back().p, s);  // padding blocks stay 0
         if (l == 0) {
             H->M.emplace_back();
             H->r.emplace_back();
             H->z.emplace_back();
-            H->up.emplace_back();
+            H->mp.emplace_back();
+            H->mpos.emplace_back();
+            H->w.emplace_back(static_cast<size_t>(cur->n_pad));
+            H->t.emplace_back(static_cast<size_t>(cur->n_pad));
+            xp::launch_fill_f32(static_cast<size_t>(cur->n_pad), 0.0f, H->w.back().p, s);
+            xp::launch_fill_f32(static_cast<size_t>(cur->n_pad), 0.0f, H->t.back().p, s);
         }
-        if (units <= coarse_max) {
-            xp::launch_mg_down_map(nq, cur->d_block_unit.p, nullptr, H->down.back().p, s);
-            H->cup.alloc(static_cast<size_t>(units));
-            xp::launch_mg_up_map(nq, H->down.back().p, cur->d_perm.p, H->cup.p, units, s);
-            H->nc = units;
-            xp::DevBuf<double> dense(2 * static_cast<size_t>(units) * static_cast<size_t>(units));
+        H->agg.emplace_back(static_cast<size_t>(std::max(n, 1)));
+        int rounds = 0;
+        const int nagg = plain_aggregate(n, cur->d_rp.p, cur->d_ci.p, cur->d_va.p, nnz, theta, l, work,
+                                         H->agg.back().p, rounds, s);
+        H->rounds.push_back(rounds);
+        if (nagg >= n) throw std::runtime_error("multilevel: the aggregation does not coarsen a level (no strong "
+                                                "couplings at this threshold)");
+        H->phase_ms.push_back(lap());
+        const int cnnz = galerkin(cur->d_rp.p, cur->d_ci.p, cur->d_va.p, nnz, nagg, H->agg.back().p, work, s);
+        H->phase_ms.push_back(lap());
+        if (nagg <= coarse_max) {
+            H->nc = nagg;
+            member_lists(nagg, nullptr, work, inv_cur.p, n, H->cmp, H->cmpos, s);
+            xp::DevBuf<double> dense(2 * static_cast<size_t>(nagg) * static_cast<size_t>(nagg));
             xp::DevBuf<int> bad(1);
             HIPX(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-            H->cinv.alloc(static_cast<size_t>(units) * static_cast<size_t>(units));
-            H->rc.alloc(static_cast<size_t>(units));
-            H->ec.alloc(static_cast<size_t>(units));
-            xp::launch_mg_dense_inverse(units, work.crp.p, work.cci.p, work.cva.p, dense.p, H->cinv.p, bad.p, s);
+            H->cinv.alloc(static_cast<size_t>(nagg) * static_cast<size_t>(nagg));
+            H->ec.alloc(static_cast<size_t>(nagg));
+            xp::launch_mg_dense_inverse(nagg, work.crp.p, work.cci.p, work.cva.p, dense.p, H->cinv.p, bad.p, s);
             int hb = 0;
             HIPX(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            H->c_slots.alloc(static_cast<size_t>(units) * 8);
-            HIPX(hipMemcpyAsync(H->c_slots.p, cur->d_slots.p, static_cast<size_t>(units) * 8 * sizeof(int),
-                                hipMemcpyDeviceToDevice, s));
             HIPX(hipStreamSynchronize(s));
             if (hb != 0) throw std::runtime_error("multilevel: the coarsest matrix is not positive definite");
-            H->phase_ms.push_back(lap());
-            H->phase_ms.push_back(0.0);
-            H->c_rp.alloc(static_cast<size_t>(units) + 1);
+            H->c_rp.alloc(static_cast<size_t>(nagg) + 1);
             H->c_ci.alloc(std::max<size_t>(static_cast<size_t>(cnnz), 1));
             H->c_va.alloc(std::max<size_t>(static_cast<size_t>(cnnz), 1));
-            HIPX(hipMemcpyAsync(H->c_rp.p, work.crp.p, (static_cast<size_t>(units) + 1) * sizeof(int),
+            HIPX(hipMemcpyAsync(H->c_rp.p, work.crp.p, (static_cast<size_t>(nagg) + 1) * sizeof(int),
                                 hipMemcpyDeviceToDevice, s));
             HIPX(hipMemcpyAsync(H->c_ci.p, work.cci.p, static_cast<size_t>(cnnz) * sizeof(int),
                                 hipMemcpyDeviceToDevice, s));
             HIPX(hipMemcpyAsync(H->c_va.p, work.cva.p, static_cast<size_t>(cnnz) * sizeof(double),
                                 hipMemcpyDeviceToDevice, s));
             HIPX(hipStreamSynchronize(s));
+            H->phase_ms.push_back(lap());
             break;
         }
         xp_matrix_opts o{};
         o.block_rows = 8;
-        o.blocking = units <= mo.consecutive_rows ? 0 : coarse_blocking;
+        o.blocking = nagg <= mo.consecutive_rows ? 0 : coarse_blocking;
         o.sigma = 64;
-        std::unique_ptr<xp_matrix> next = make_matrix_device(units, work.crp.p, work.cci.p, work.cva.p, &o);
+        std::unique_ptr<xp_matrix> next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o);
         if (next->bad_row >= 0) throw std::runtime_error("multilevel: a coarse matrix has a non-positive diagonal");
-        HIPX(hipStreamSynchronize(s));
-        H->phase_ms.push_back(lap());
-        xp::DevBuf<int> inv(static_cast<size_t>(units));
-        xp::launch_mg_inv_perm(next->n_pad, next->d_perm.p, inv.p, s);
-        xp::launch_mg_down_map(nq, cur->d_block_unit.p, inv.p, H->down.back().p, s);
-        H->up.emplace_back(static_cast<size_t>(next->n_pad));
-        xp::launch_mg_up_map(nq, H->down.back().p, cur->d_perm.p, H->up.back().p, next->n_pad, s);
+        H->mp.emplace_back();
+        H->mpos.emplace_back();
+        member_lists(next->n_pad, next->d_perm.p, work, inv_cur.p, n, H->mp.back(), H->mpos.back(), s);
         const int nqn = next->n_pad / 8;
         xp::DevBuf<float> full(static_cast<size_t>(nqn) * 64);
         xp::launch_block_exact_full(next->view(), nqn, full.p, s);
         H->M.emplace_back(static_cast<size_t>(nqn));
         xp::launch_block_factor(nqn, full.p, H->M.back().p, s);
-        H->r.emplace_back(static_cast<size_t>(next->n_pad));
-        H->z.emplace_back(static_cast<size_t>(next->n_pad));
-        xp::launch_fill_f32(static_cast<size_t>(next->n_pad), 0.0f, H->r.back().p, s);
-        xp::launch_fill_f32(static_cast<size_t>(next->n_pad), 0.0f, H->z.back().p, s);
+        const size_t np = static_cast<size_t>(next->n_pad);
+        H->r.emplace_back(np);
+        H->z.emplace_back(np);
+        H->w.emplace_back(np);
+        H->t.emplace_back(np);
+        for (xp::DevBuf<float>* v : {&H->r.back(), &H->z.back(), &H->w.back(), &H->t.back()})
+            xp::launch_fill_f32(np, 0.0f, v->p, s);
+        inv_cur.ensure(static_cast<size_t>(nagg));
+        xp::launch_mg_inv_perm(next->n_pad, next->d_perm.p, inv_cur.p, s);
         HIPX(hipStreamSynchronize(s));  // before the temporaries go
         H->phase_ms.push_back(lap());
         H->mats.push_back(std::move(next));
@@ -3594,9 +3667,9 @@ int xp_pcg_multilevel_info(const xp_pcg* s, int cap, int* nlev, int* n, long lon
 }
 
 // Level l >= 1 of the hierarchy as a natural-order CSR (its rows are the
-// blocks of level l - 1) and the blocks of level l - 1 it came from (units x 8
-// natural rows of level l - 1, -1 padding).  Null outputs are skipped.
-int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_idx, double* values, int* slots) {
+// aggregates of level l - 1) and the aggregate of every natural row of level
+// l - 1.  Null outputs are skipped.
+int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_idx, double* values, int* agg) {
     return guarded([&] {
         if (s == nullptr || s->mg == nullptr) throw std::runtime_error("no multilevel hierarchy");
         const MgHierarchy& H = *s->mg;
@@ -3619,10 +3692,37 @@ int xp_pcg_multilevel_level(const xp_pcg* s, int level, int* row_ptr, int* col_i
         if (values != nullptr && nz > 0)
             HIPX(hipMemcpyAsync(values, dense ? H.c_va.p : c->d_va.p, static_cast<size_t>(nz) * sizeof(double),
                                 hipMemcpyDeviceToHost, st));
-        if (slots != nullptr)
-            HIPX(hipMemcpyAsync(slots, prev->d_slots.p, static_cast<size_t>(n) * 8 * sizeof(int),
+        if (agg != nullptr && prev->n > 0)
+            HIPX(hipMemcpyAsync(agg, H.agg[static_cast<size_t>(level) - 1].p, static_cast<size_t>(prev->n) * sizeof(int),
                                 hipMemcpyDeviceToHost, st));
         HIPX(hipStreamSynchronize(st));
+    });
+}
+
+// The smoother blocks of SELL level 1 <= l < levels - 1 (units x 8 natural
+// rows of the level, -1 padding; slots may be null) and their count.
+int xp_pcg_multilevel_slots(const xp_pcg* s, int level, int* slots, int* units) {
+    return guarded([&] {
+        if (s == nullptr || s->mg == nullptr || units == nullptr) throw std::runtime_error("no multilevel hierarchy");
+        const MgHierarchy& H = *s->mg;
+        if (level < 1 || level > static_cast<int>(H.mats.size())) throw std::runtime_error("bad level");
+        const xp_matrix& c = *H.mats[static_cast<size_t>(level) - 1];
+        *units = c.unit_first.back();
+        const hipStream_t st = device().stream;
+        if (slots != nullptr && *units > 0)
+            HIPX(hipMemcpyAsync(slots, c.d_slots.p, static_cast<size_t>(*units) * 8 * sizeof(int),
+                                hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+    });
+}
+
+// Root-selection rounds of each level's aggregation.
+int xp_pcg_multilevel_rounds(const xp_pcg* s, int cap, int* rounds, int* count) {
+    return guarded([&] {
+        if (s == nullptr || count == nullptr) throw std::runtime_error("null argument");
+        *count = s->mg == nullptr ? 0 : static_cast<int>(s->mg->rounds.size());
+        if (rounds != nullptr)
+            for (int k = 0; k < std::min(cap, *count); ++k) rounds[k] = s->mg->rounds[static_cast<size_t>(k)];
     });
 }
 

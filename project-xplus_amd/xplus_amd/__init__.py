@@ -164,7 +164,7 @@ _rhs_ones = _sig("xp_rhs_ones_f32", C.c_int, [C.c_int, _i32p, _i32p, _f64p, _f64
 _p_set_mlp = _sig("xp_pcg_set_mlp_preconditioner", C.c_int, [_vp, _vp])
 class MultilevelOpts(C.Structure):
     _fields_ = [("coarse_max", C.c_int), ("omega", C.c_double), ("gamma", C.c_double),
-                ("coarse_blocking", C.c_int), ("consecutive_rows", C.c_int)]
+                ("coarse_blocking", C.c_int), ("consecutive_rows", C.c_int), ("theta", C.c_double)]
 
 
 _p_set_ml = _sig("xp_pcg_set_multilevel", C.c_int, [_vp, C.POINTER(MultilevelOpts)])
@@ -172,6 +172,8 @@ _p_ml_info = _sig("xp_pcg_multilevel_info", C.c_int, [_vp, C.c_int, C.POINTER(C.
                                                      np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                                                      C.POINTER(C.c_double)])
 _p_ml_level = _sig("xp_pcg_multilevel_level", C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp])
+_p_ml_slots = _sig("xp_pcg_multilevel_slots", C.c_int, [_vp, C.c_int, _vp, C.POINTER(C.c_int)])
+_p_ml_rounds = _sig("xp_pcg_multilevel_rounds", C.c_int, [_vp, C.c_int, _i32p, C.POINTER(C.c_int)])
 _p_ml_phases = _sig("xp_pcg_multilevel_phases", C.c_int, [_vp, C.c_int, _f64p, C.POINTER(C.c_int)])
 _p_ml_apply = _sig("xp_pcg_multilevel_apply", C.c_int, [_vp, _vp, _vp])
 _p_ml_time = _sig("xp_pcg_multilevel_time", C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)])
@@ -1102,12 +1104,13 @@ class Pcg:
         _check(_p_set_mlp(self.h, net.h))
 
     def set_multilevel(self, coarse_max: int = 512, omega: float = 0.8, gamma: float = 1.5,
-                       coarse_blocking: int = 4, consecutive_rows: int = 0) -> None:
-        """Multilevel preconditioner over the block aggregates, with the
-        installed blocks as the level-0 smoother (coarse_max <= 0: off);
-        coarse_blocking 3 or 4 aggregates the coarse levels, those of at most
-        consecutive_rows rows take consecutive rows."""
-        o = MultilevelOpts(coarse_max, omega, gamma, coarse_blocking, consecutive_rows)
+                       coarse_blocking: int = 4, consecutive_rows: int = 0, theta: float = 0.08) -> None:
+        """Multilevel aggregation preconditioner with the installed blocks as
+        the level-0 smoother (coarse_max <= 0: off): coarse spaces from plain
+        aggregation at strength threshold theta; coarse_blocking 3 or 4 forms
+        the coarse levels' smoother blocks, those of at most consecutive_rows
+        rows take consecutive rows."""
+        o = MultilevelOpts(coarse_max, omega, gamma, coarse_blocking, consecutive_rows, theta)
         _check(_p_set_ml(self.h, C.byref(o)))
 
     def multilevel_info(self) -> dict:
@@ -1123,17 +1126,32 @@ class Pcg:
                 "phase_ms": [round(v, 3) for v in ph[:min(cnt.value, 3 * cap)]]}
 
     def multilevel_level(self, level: int):
-        """Level `level` >= 1 as a natural-order Csr (rows = blocks of the level
-        above) and the blocks of the level above (rows x 8, -1 padding)."""
+        """Level `level` >= 1 as a natural-order Csr (rows = the aggregates of
+        the level above) and the aggregate of every row of the level above."""
         info = self.multilevel_info()
         n, nnz = info["n"][level], info["nnz"][level]
         rp = np.zeros(n + 1, np.int32)
         ci = np.zeros(max(nnz, 1), np.int32)
         va = np.zeros(max(nnz, 1), np.float64)
-        slots = np.zeros(n * 8, np.int32)
+        agg = np.zeros(max(info["n"][level - 1], 1), np.int32)
         _check(_p_ml_level(self.h, level, rp.ctypes.data_as(_vp), ci.ctypes.data_as(_vp), va.ctypes.data_as(_vp),
-                           slots.ctypes.data_as(_vp)))
-        return Csr(rp, ci[:nnz], va[:nnz]), slots.reshape(n, 8)
+                           agg.ctypes.data_as(_vp)))
+        return Csr(rp, ci[:nnz], va[:nnz]), agg[:info["n"][level - 1]]
+
+    def multilevel_slots(self, level: int) -> np.ndarray:
+        """The smoother blocks of coarse SELL level `level` (units x 8 rows of
+        that level, -1 padding)."""
+        units = C.c_int()
+        _check(_p_ml_slots(self.h, level, None, C.byref(units)))
+        slots = np.zeros(max(units.value, 1) * 8, np.int32)
+        _check(_p_ml_slots(self.h, level, slots.ctypes.data_as(_vp), C.byref(units)))
+        return slots[:units.value * 8].reshape(-1, 8)
+
+    def multilevel_rounds(self) -> list:
+        """Root-selection rounds of each level's aggregation."""
+        r, cnt = np.zeros(16, np.int32), C.c_int()
+        _check(_p_ml_rounds(self.h, 16, r, C.byref(cnt)))
+        return r[:min(cnt.value, 16)].tolist()
 
     def multilevel_time(self, reps: int = 50) -> float:
         """Average device us of one V-cycle (block product included)."""

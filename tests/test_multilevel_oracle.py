@@ -1,5 +1,12 @@
 """Pins of the multilevel oracle (oracle/multilevel.py), on the CPU.
 
+  * plain_aggregation meets its definition, checked by breadth-first search
+    on the strength graph: roots pairwise at strong distance >= 3, every
+    row within distance 2 of a root (maximality), a root's strong
+    neighbours in its aggregate, every other row in the aggregate of its
+    strongest joined neighbour, root k = aggregate k in index order; and it
+    equals a brute-force restatement of the rounds (all-pairs distances
+    instead of max propagation) on small graphs, rounds included;
   * galerkin equals scipy's P^T A P (an independent formula: sparse
     products) to FP64 rounding, and a hand-computed 6-row case bit for bit
     (the order of the sums is part of the definition);
@@ -12,6 +19,7 @@
 from __future__ import annotations
 
 import sys
+from collections import deque
 
 import numpy as np
 import pytest
@@ -24,10 +32,13 @@ sys.path.insert(0, str(O.ROOT / "oracle"))
 import blocking as BK  # noqa: E402
 import multilevel as ML  # noqa: E402
 
+THETA = 0.08
 
-def hierarchy(a, nlev, omega=0.7, gamma=1.5, seed_dist=2):
-    """Levels from the rule-3 (then rule seed_dist) aggregation oracle, exact
-    block inverses (FP64, no FP16 rounding), dense coarsest."""
+
+def hierarchy(a, nlev, omega=0.7, gamma=1.5, seed_dist=2, theta=THETA):
+    """Levels: smoother blocks from the rule-3 (then rule seed_dist)
+    aggregation oracle with exact FP64 inverses, coarse spaces from plain
+    aggregation; dense coarsest."""
     A = a.to_scipy().tocsr()
     rp, ci, va = A.indptr, A.indices, A.data
     levels = []
@@ -37,25 +48,160 @@ def hierarchy(a, nlev, omega=0.7, gamma=1.5, seed_dist=2):
         for k, g in enumerate(groups):
             slots[k, :len(g)] = g
         Am = sp.csr_matrix((va, ci, rp), shape=(len(rp) - 1, len(rp) - 1))
-        levels.append(ML.Level(Am, slots, ML.exact_blocks(Am, slots, f32_values=False)))
-        rp, ci, va = ML.galerkin(rp, ci, va, slots)
+        agg, nagg, _ = ML.plain_aggregation(rp, ci, va, theta, lev)
+        levels.append(ML.Level(Am, slots, ML.exact_blocks(Am, slots, f32_values=False), agg, nagg))
+        rp, ci, va = ML.galerkin(rp, ci, va, agg, nagg)
     coarse = ML.to_dense(rp, ci, va, len(rp) - 1)
     return ML.Hierarchy(levels, coarse, omega, gamma)
+
+
+def bfs_dist(adj, src, cap=3):
+    """Strong distances from src up to cap (cap for farther rows)."""
+    d = {src: 0}
+    q = deque([src])
+    while q:
+        u = q.popleft()
+        if d[u] >= cap:
+            continue
+        for v in adj[u]:
+            if v not in d:
+                d[v] = d[u] + 1
+                q.append(v)
+    return d
+
+
+def adjacency(rp, ci, va, theta):
+    sp_, si, sv = ML.strong_graph(rp, ci, va, theta)
+    n = len(rp) - 1
+    adj = [list(si[sp_[i]:sp_[i + 1]]) for i in range(n)]
+    w = [dict(zip(si[sp_[i]:sp_[i + 1]], sv[sp_[i]:sp_[i + 1]])) for i in range(n)]
+    return adj, w
+
+
+def check_aggregation_properties(rp, ci, va, theta, salt):
+    agg, nagg, rounds, roots = ML.plain_aggregation(rp, ci, va, theta, salt, with_roots=True)
+    n = len(rp) - 1
+    adj, w = adjacency(rp, ci, va, theta)
+    assert agg.shape == (n,) and agg.min() >= 0 and agg.max() == nagg - 1
+    assert set(agg.tolist()) == set(range(nagg))
+    roots = [int(r) for r in roots]
+    assert roots == sorted(roots)
+    dist = [bfs_dist(adj, i) for i in range(n)]
+    # root k is aggregate k; its strong neighbours share it; roots pairwise
+    # at distance >= 3
+    for k, r in enumerate(roots):
+        assert agg[r] == k
+        assert all(agg[j] == k for j in adj[r])
+    for x in roots:
+        for y in roots:
+            if x < y:
+                assert dist[x].get(y, 3) >= 3
+    rs = set(roots)
+    # maximality: every row within distance 2 of a root
+    for i in range(n):
+        assert any(dist[i].get(r, 3) <= 2 for r in roots), i
+    # the other rows: the aggregate of the strongest neighbour joined to a
+    # root (largest |a_ij|, then the smallest j); none are left over here
+    joined = {i for i in range(n) if i in rs or any(j in rs for j in adj[i])}
+    for i in range(n):
+        if i in joined:
+            continue
+        cand = [(w[i][j], -j) for j in adj[i] if j in joined]
+        assert cand, i
+        _, mj = max(cand)
+        assert agg[i] == agg[-mj]
+    assert nagg == len(roots)
+    return agg, nagg, rounds, len(roots)
+
+
+def test_priority_keys_distinct():
+    keys = [ML.priority_key(i, s) for s in (0, 1, 5) for i in range(20000)]
+    per_salt = [set(keys[k * 20000:(k + 1) * 20000]) for k in range(3)]
+    assert all(len(p) == 20000 for p in per_salt)
+    assert ML.priority_key(7, 0) & 0xFFFFFFFF == 8
+
+
+@pytest.mark.parametrize("m,theta", [(20, 0.08), (20, 0.0), (14, 0.25)])
+def test_plain_aggregation_properties_fem(m, theta):
+    a = X.gen_fem2d(m, 1.0, 3, 1e3, 0.3, 17 + m)
+    A = a.to_scipy().tocsr()
+    agg, nagg, rounds, nroots = check_aggregation_properties(A.indptr, A.indices, A.data, theta, 0)
+    assert nagg < a.n // 2 and rounds >= 1
+    sizes = np.bincount(agg)
+    print(f"m {m} theta {theta}: {nagg} aggregates ({nroots} roots), sizes {sizes.min()}..{sizes.max()}, {rounds} rounds")
+
+
+def brute_force(rp, ci, va, theta, salt):
+    """The rounds restated with all-pairs strong distances."""
+    n = len(rp) - 1
+    adj, w = adjacency(rp, ci, va, theta)
+    dist = [bfs_dist(adj, i) for i in range(n)]
+    key = [ML.priority_key(i, salt) for i in range(n)]
+    state = [0] * n
+    rounds = 0
+    while 0 in state:
+        rounds += 1
+        und = [i for i in range(n) if state[i] == 0]
+        new = [i for i in und if all(key[i] > key[j] for j in und if j != i and dist[i].get(j, 3) <= 2)]
+        for i in new:
+            state[i] = 1
+        roots = [i for i in range(n) if state[i] == 1]
+        for i in und:
+            if state[i] == 0 and any(dist[i].get(r, 3) <= 2 for r in roots):
+                state[i] = 2
+    roots = [i for i in range(n) if state[i] == 1]
+    agg = [-1] * n
+    for k, r in enumerate(roots):
+        agg[r] = k
+    for i in range(n):
+        if state[i] != 1:
+            for j in adj[i]:
+                if state[j] == 1:
+                    agg[i] = agg[j]
+    first = list(agg)
+    for i in range(n):
+        if first[i] < 0:
+            cand = [(w[i][j], -j) for j in adj[i] if first[j] >= 0]
+            if cand:
+                agg[i] = first[-max(cand)[1]]
+    k = len(roots)
+    for i in range(n):
+        if agg[i] < 0:
+            agg[i] = k
+            k += 1
+    return np.array(agg), k, rounds
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_plain_aggregation_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(5, 60))
+    # a random symmetric matrix: sparse random couplings, some weak, a few
+    # isolated rows, diagonally dominant
+    B = sp.random(n, n, density=float(rng.uniform(0.03, 0.15)), random_state=seed, format="csr")
+    B.data = rng.uniform(-1.0, 1.0, B.nnz) * np.where(rng.random(B.nnz) < 0.3, 0.01, 1.0)
+    S = (B + B.T).tocsr()
+    S.setdiag(0.0)
+    S.eliminate_zeros()
+    A = (S + sp.diags(np.asarray(abs(S).sum(axis=1)).ravel() + 1.0)).tocsr()
+    A.sort_indices()
+    for theta, salt in ((0.0, seed), (0.08, 3), (0.3, 1)):
+        agg, nagg, rounds = ML.plain_aggregation(A.indptr, A.indices, A.data, theta, salt)
+        want, wn, wr = brute_force(A.indptr, A.indices, A.data, theta, salt)
+        np.testing.assert_array_equal(agg, want)
+        assert (nagg, rounds) == (wn, wr)
 
 
 def test_galerkin_matches_scipy_product():
     a = X.gen_fem2d(30, 1.0, 3, 1e3, 0.3, 7)
     A = a.to_scipy().tocsr()
-    groups = BK.parallel_blocking(list(A.indptr), list(A.indices), list(A.data))
-    slots = np.full((len(groups), 8), -1, np.int32)
-    for k, g in enumerate(groups):
-        slots[k, :len(g)] = g
-    rp, ci, va = ML.galerkin(A.indptr, A.indices, A.data, slots)
-    Ac = sp.csr_matrix((va, ci, rp), shape=(len(groups), len(groups))).toarray()
-    P = ML.Level(A, slots, np.zeros((len(groups), 8, 8))).P
+    agg, nagg, _ = ML.plain_aggregation(A.indptr, A.indices, A.data, THETA, 0)
+    rp, ci, va = ML.galerkin(A.indptr, A.indices, A.data, agg, nagg)
+    Ac = sp.csr_matrix((va, ci, rp), shape=(nagg, nagg)).toarray()
+    P = sp.csr_matrix((np.ones(a.n), (np.arange(a.n), agg)), shape=(a.n, nagg))
     want = (P.T @ A @ P).toarray()
     assert np.all(np.diff(rp) > 0)
-    for u in range(len(groups)):
+    for u in range(nagg):
         assert list(ci[rp[u]:rp[u + 1]]) == sorted(ci[rp[u]:rp[u + 1]])
     np.testing.assert_allclose(Ac, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
 
@@ -69,14 +215,16 @@ def test_galerkin_hand_case_bit_for_bit():
                   [0.0, -1.0, 0.0, -1.0, 4.0, -1.0],
                   [0.0, 0.0, -1.0, 0.0, -1.0, 4.0]]) * 0.1
     M = sp.csr_matrix(A)
-    slots = np.array([[0, 3, 4, -1, -1, -1, -1, -1], [1, 2, 5, -1, -1, -1, -1, -1]], np.int32)
-    rp, ci, va = ML.galerkin(M.indptr, M.indices, M.data, slots)
+    agg = np.array([0, 1, 1, 0, 0, 1], np.int32)
+    rp, ci, va = ML.galerkin(M.indptr, M.indices, M.data, agg, 2)
     assert list(rp) == [0, 2, 4] and list(ci) == [0, 1, 0, 1]
     # (0,0): row 0 -> 0.4 (col 0), -0.1 (col 3); row 3 -> -0.1 (col 0), 0.4, -0.1; row 4 -> -0.1 (3), 0.4
     v00 = 0.4 + -0.1 + -0.1 + 0.4 + -0.1 + -0.1 + 0.4
     # (0,1): row 0 -> -0.1 (col 1); row 4 -> -0.1 (col 1), -0.1 (col 5)
     v01 = -0.1 + -0.1 + -0.1
+    # (1,0): row 1 -> -0.1 (col 0), -0.1 (col 4); row 2 -> nothing; row 5 -> -0.1 (col 4)
     v10 = -0.1 + -0.1 + -0.1
+    # (1,1): row 1 -> 0.4, -0.1 (2); row 2 -> -0.1 (1), 0.4, -0.1 (5); row 5 -> -0.1 (2), 0.4
     v11 = 0.4 + -0.1 + -0.1 + 0.4 + -0.1 + -0.1 + 0.4
     assert va.tolist() == [v00, v01, v10, v11]
 
