@@ -507,8 +507,9 @@ int xp_pcg_set_mlp_preconditioner(xp_pcg* s, xp_mlp* net);
  * level l + 1 is the Galerkin product P^T A P over those aggregates (P
  * piecewise constant), set up on the device with its own 8-row smoother
  * blocks (coarse_blocking 3 or 4: the parallel strength aggregation of
- * xp_matrix_opts.blocking; levels of at most consecutive_rows rows take 8
- * consecutive rows) and exact block factors, down to a dense coarsest level
+ * xp_matrix_opts.blocking; 1: pieces of the level's own aggregates; levels
+ * of at most consecutive_rows rows take 8 consecutive rows) and exact block
+ * factors, down to a dense coarsest level
  * of at most coarse_max rows (inverted once).  Each PCG step then applies
  * one symmetric V-cycle: z = omega M r, coarse correction scaled by gamma,
  * one post-smoothing step; FP32 vectors inside the cycle.  Needs the
@@ -519,7 +520,9 @@ typedef struct {
     int coarse_max;       /* rows of the dense coarsest level, at most (8..4096); <= 0 removes the hierarchy */
     double omega;         /* smoother damping, 0 < omega < 2 */
     double gamma;         /* coarse-correction scale, 0 < gamma < 4 */
-    int coarse_blocking;  /* smoother blocks of the coarse levels: 3 or 4 (see xp_matrix_opts.blocking) */
+    int coarse_blocking;  /* smoother blocks of the coarse levels: 3 or 4 (see xp_matrix_opts.blocking), or 1:
+                             pieces of the level's own aggregates (ceil(size / 8) pieces of consecutive rows each,
+                             sizes as equal as possible; no aggregation rounds of their own) */
     int consecutive_rows; /* coarse levels of at most this many rows take 8 consecutive rows per block */
     double theta;         /* strength threshold of the aggregation, 0 <= theta < 1 */
 } xp_multilevel_opts;

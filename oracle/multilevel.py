@@ -25,6 +25,9 @@ and V-cycle are checked against:
                     |a_ij|, then the smallest j); rows left over become
                     aggregates of their own, in index order
   members        -- each aggregate's rows, ascending
+  pieces         -- 8-row smoother blocks cut from aggregates: ceil(size/8)
+                    pieces of consecutive rows per aggregate, sizes as
+                    equal as possible (the first size % pieces one longer)
   galerkin       -- the coarse matrix P^T A P (P piecewise constant on the
                     aggregates): row u = aggregate u, entry (u, v) = sum of
                     a_ij over rows i of u and columns j of v, equal columns
@@ -161,6 +164,25 @@ def members(agg, nagg):
     order = np.argsort(agg, kind="stable")
     ptr = np.concatenate([[0], np.cumsum(np.bincount(agg, minlength=nagg))])
     return ptr.astype(np.int32), order.astype(np.int32)
+
+
+def pieces(agg, nagg):
+    """8-row smoother blocks cut from aggregates: aggregate a (rows
+    ascending) gives ceil(size / 8) pieces of consecutive rows, the first
+    size % pieces of them one row longer; pieces in aggregate order, as
+    units x 8 slots with -1 padding."""
+    mp, mr = members(agg, nagg)
+    out = []
+    for a in range(nagg):
+        rows = mr[mp[a]:mp[a + 1]]
+        k = -(-len(rows) // 8)
+        q, rem = divmod(len(rows), k)
+        o = 0
+        for p in range(k):
+            ln = q + (1 if p < rem else 0)
+            out.append(list(rows[o:o + ln]) + [-1] * (8 - ln))
+            o += ln
+    return np.array(out, np.int32).reshape(-1, 8)
 
 
 def galerkin(row_ptr, col_idx, values, agg, nagg):

@@ -2204,6 +2204,32 @@ __global__ void k_mg_member_pos(int np, const int* __restrict__ perm, const int*
     for (int k = 0; k < mptr[a + 1] - b; ++k) mpos[o + k] = inv_fine[mem[b + k]];
 }
 
+// Smoother blocks cut from aggregates (members ascending in mptr / mem):
+// aggregate a gives ceil(size / 8) pieces of consecutive members, the first
+// size % pieces of them one row longer (sizes as equal as possible), in
+// aggregate order from piece poff[a].
+__global__ void k_piece_count(int nagg, const int* __restrict__ mptr, int* __restrict__ cnt) {
+    const int a = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (a < nagg) cnt[a] = (mptr[a + 1] - mptr[a] + 7) / 8;
+}
+
+__global__ void k_piece_fill(int nagg, const int* __restrict__ mptr, const int* __restrict__ mem,
+                             const int* __restrict__ poff, int* __restrict__ slots) {
+    const int a = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (a >= nagg) return;
+    const int b = mptr[a], size = mptr[a + 1] - b;
+    const int k = (size + 7) / 8;
+    if (k == 0) return;
+    const int q = size / k, rem = size % k;
+    int m = b;
+    for (int p = 0; p < k; ++p) {
+        const int len = q + (p < rem ? 1 : 0);
+        int* out = slots + static_cast<size_t>(poff[a] + p) * 8;
+        for (int j = 0; j < 8; ++j) out[j] = j < len ? mem[m + j] : -1;
+        m += len;
+    }
+}
+
 // Candidate count of coarse row u: the entries of its aggregate's rows.
 __global__ void k_gal_count(int units, const int* __restrict__ mptr, const int* __restrict__ mem,
                             const int* __restrict__ rp, int* __restrict__ cnt) {
@@ -2223,7 +2249,7 @@ __global__ void k_gal_count(int units, const int* __restrict__ mptr, const int* 
 // global memory instead (insertion sort, stable: the same sums in the same
 // order).
 constexpr int kGalThreads = 64;
-constexpr int kGalCap = 96;
+constexpr int kGalCap = 32;
 __global__ __launch_bounds__(kGalThreads) void k_gal_fill(int units, const int* __restrict__ mptr,
                                                           const int* __restrict__ mem, const int* __restrict__ rp,
                                                           const int* __restrict__ ci, const double* __restrict__ va,
@@ -2396,18 +2422,63 @@ __global__ void k_dense_from_csr(int n, const int* __restrict__ rp, const int* _
 // Gauss-Jordan inverse of an SPD matrix (no pivoting): pivot k in one
 // kernel that reads the matrix of step k - 1 (a) and writes step k's (b), so
 // no entry is read after it changes; a non-positive pivot sets *bad.
-__global__ void k_gj_step(int n, int k, const double* __restrict__ a, double* __restrict__ b, int* __restrict__ bad) {
-    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+// One Gauss-Jordan sweep over the pivot block K = [k, k + bk) (bk <= 8) of
+// the n x n matrix a into b: with P = a_KK, b_KK = P^-1, b_Kj = P^-1 a_Kj,
+// b_iK = -a_iK P^-1 and b_ij = a_ij - a_iK P^-1 a_Kj.  Sweeping every block
+// in turn leaves the inverse (the block form of the one-pivot sweep, which
+// it equals in exact arithmetic).  Every workgroup inverts P itself by the
+// same sweep, one pivot at a time, one element per lane of wavefront 0
+// (FP64); *bad = 1 on a non-positive pivot (P of an SPD matrix's sweep is
+// SPD).
+__global__ __launch_bounds__(256) void k_gj_block(int n, int k, int bk, const double* __restrict__ a,
+                                                  double* __restrict__ b, int* __restrict__ bad) {
+    __shared__ double P[8][8];
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid < 64) {
+        const int r = tid >> 3, c = tid & 7;
+        P[r][c] = (r < bk && c < bk) ? a[static_cast<size_t>(k + r) * n + k + c] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int q = 0; q < bk; ++q) {
+        const double pq = P[q][q];
+        double v = 0.0;
+        if (tid < 64) {
+            const int r = tid >> 3, c = tid & 7;
+            if (r == q && c == q) v = 1.0 / pq;
+            else if (r == q) v = P[q][c] / pq;
+            else if (c == q) v = -P[r][q] / pq;
+            else v = P[r][c] - P[r][q] * P[q][c] / pq;
+        }
+        __syncthreads();
+        if (tid < 64) P[tid >> 3][tid & 7] = v;
+        if (tid == 0 && blockIdx.x == 0 && !(pq > 0.0)) *bad = 1;
+        __syncthreads();
+    }
+    const size_t idx = static_cast<size_t>(blockIdx.x) * 256 + static_cast<size_t>(tid);
     if (idx >= static_cast<size_t>(n) * n) return;
-    const double akk = a[static_cast<size_t>(k) * n + k];
-    if (idx == 0 && !(akk > 0.0)) *bad = 1;
-    const double p = 1.0 / akk;
     const int i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
+    const bool iK = i >= k && i < k + bk, jK = j >= k && j < k + bk;
+    const double* ai = a + static_cast<size_t>(i) * n + k;  // a_iK
     double v;
-    if (i == k && j == k) v = p;
-    else if (i == k) v = a[idx] * p;
-    else if (j == k) v = -a[idx] * p;
-    else v = a[idx] - a[static_cast<size_t>(i) * n + k] * a[static_cast<size_t>(k) * n + j] / akk;
+    if (iK && jK) {
+        v = P[i - k][j - k];
+    } else if (iK) {
+        v = 0.0;
+        for (int q = 0; q < bk; ++q) v += P[i - k][q] * a[static_cast<size_t>(k + q) * n + j];
+    } else if (jK) {
+        v = 0.0;
+        for (int q = 0; q < bk; ++q) v -= ai[q] * P[q][j - k];
+    } else {
+        double akj[8];
+        for (int r = 0; r < 8; ++r) akj[r] = r < bk ? a[static_cast<size_t>(k + r) * n + j] : 0.0;
+        double s = 0.0;
+        for (int q = 0; q < bk; ++q) {
+            double t = 0.0;
+            for (int r = 0; r < bk; ++r) t += P[q][r] * akj[r];
+            s += ai[q] * t;
+        }
+        v = a[idx] - s;
+    }
     b[idx] = v;
 }
 
@@ -2986,6 +3057,14 @@ void launch_mg_member_solve_pos(int np, const int* perm, const int* mptr, const 
         hipLaunchKernelGGL(k_mg_member_pos, rows256(np), dim3(256), 0, s, np, perm, mptr, mem, inv_fine, mp, mpos);
 }
 
+void launch_mg_piece_count(int nagg, const int* mptr, int* cnt, hipStream_t s) {
+    if (nagg > 0) hipLaunchKernelGGL(k_piece_count, rows256(nagg), dim3(256), 0, s, nagg, mptr, cnt);
+}
+
+void launch_mg_piece_fill(int nagg, const int* mptr, const int* mem, const int* poff, int* slots, hipStream_t s) {
+    if (nagg > 0) hipLaunchKernelGGL(k_piece_fill, rows256(nagg), dim3(256), 0, s, nagg, mptr, mem, poff, slots);
+}
+
 void launch_mg_galerkin_count(int units, const int* mptr, const int* mem, const int* rp, int* cnt, hipStream_t s) {
     if (units > 0)
         hipLaunchKernelGGL(k_gal_count, rows256(units), dim3(256), 0, s, units, mptr, mem, rp, cnt);
@@ -3021,10 +3100,11 @@ void launch_mg_dense_inverse(int n, const int* rp, const int* ci, const double* 
     hipLaunchKernelGGL(k_fill<double>, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, 0.0, work);
     hipLaunchKernelGGL(k_dense_from_csr, dim3(blocks_for(static_cast<size_t>(n), 256)), dim3(256), 0, s, n, rp, ci,
                        va, work);
-    for (int k = 0; k < n; ++k)
-        hipLaunchKernelGGL(k_gj_step, dim3(blocks_for(nn, 256)), dim3(256), 0, s, n, k, buf[k & 1], buf[(k + 1) & 1],
-                           bad);
-    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, buf[n & 1], out);
+    int steps = 0;
+    for (int k = 0; k < n; k += 8, ++steps)
+        hipLaunchKernelGGL(k_gj_block, dim3(blocks_for(nn, 256)), dim3(256), 0, s, n, k, std::min(8, n - k),
+                           buf[steps & 1], buf[(steps + 1) & 1], bad);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(nn, 256)), dim3(256), 0, s, nn, buf[steps & 1], out);
 }
 
 }  // namespace xp

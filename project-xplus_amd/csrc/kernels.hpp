@@ -148,6 +148,11 @@ void launch_mg_member_fill(int n, const int* agg, int nagg, const int* mptr, int
 void launch_mg_member_solve_count(int np, const int* perm, const int* mptr, int* cnt, hipStream_t s);
 void launch_mg_member_solve_pos(int np, const int* perm, const int* mptr, const int* mem, const int* inv_fine,
                                 const int* mp, int* mpos, hipStream_t s);
+// Smoother blocks cut from aggregates: cnt = pieces per aggregate
+// (ceil(size / 8)); after the scan to poff, slots = the pieces (consecutive
+// members, sizes as equal as possible), units x 8 rows, -1 padding.
+void launch_mg_piece_count(int nagg, const int* mptr, int* cnt, hipStream_t s);
+void launch_mg_piece_fill(int nagg, const int* mptr, const int* mem, const int* poff, int* slots, hipStream_t s);
 // Galerkin product P^T A P over the member lists (units coarse rows).
 void launch_mg_galerkin_count(int units, const int* mptr, const int* mem, const int* rp, int* cnt, hipStream_t s);
 void launch_mg_galerkin_fill(int units, const int* mptr, const int* mem, const int* rp, const int* ci,
@@ -159,8 +164,9 @@ void launch_mg_inv_perm(int n_pad, const int* perm, int* inv, hipStream_t s);
 // Exact inverses of the diagonal 8x8 blocks (solve order) as full FP32 blocks.
 void launch_block_exact_full(const SellView& A, int nq, float* full, hipStream_t s);
 // out = inverse of the n x n SPD matrix of a CSR (FP64 Gauss-Jordan, one
-// kernel per pivot between two buffers, rounded to FP32); work: 2 n * n
-// doubles; *bad = 1 on a non-positive pivot (bad must be zero on entry).
+// kernel per block of 8 pivots between two buffers, rounded to FP32); work:
+// 2 n * n doubles; *bad = 1 on a non-positive pivot (bad must be zero on
+// entry).
 void launch_mg_dense_inverse(int n, const int* rp, const int* ci, const double* va, double* work, float* out,
                              int* bad, hipStream_t s);
 

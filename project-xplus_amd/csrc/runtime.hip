@@ -355,12 +355,17 @@ std::unique_ptr<xp_matrix> make_matrix(const std::vector<xp::CsrRef>& systems, c
     return m;
 }
 
+// The blocking of a device-built matrix whose 8-row blocks were given by
+// the caller (the multilevel hierarchy's coarse levels).
+constexpr int kBlockingGiven = -1;
+
 // One system from a CSR in device memory, set up on the device: row facts,
 // blocks (block_rows = 8: consecutive rows, blocking 0, or the parallel
 // strength aggregation, blocking 3) and the SELL arrays.  Host copies are
 // made only on demand (ensure_host_mirror).
 std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const int* col_idx, const double* values,
-                                              const xp_matrix_opts* o) {
+                                              const xp_matrix_opts* o, const int* given_slots = nullptr,
+                                              int given_units = 0) {
     const hipStream_t s = device().stream;
     auto m = std::make_unique<xp_matrix>();
     m->keep_lo = o != nullptr && o->keep_lo;
@@ -368,7 +373,8 @@ std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const i
     m->variant = (o != nullptr && o->spmv_variant >= 1 && o->spmv_variant <= 2) ? o->spmv_variant : 0;
     m->block_rows = (o != nullptr && o->block_rows == 8) ? 8 : 1;
     m->blocking = m->block_rows == 8 && o != nullptr ? o->blocking : 0;
-    if (m->block_rows == 8 && m->blocking != 0 && m->blocking != 3 && m->blocking != 4)
+    if (given_slots != nullptr) m->blocking = kBlockingGiven;
+    if (m->block_rows == 8 && m->blocking != 0 && m->blocking != 3 && m->blocking != 4 && m->blocking != kBlockingGiven)
         throw std::runtime_error("the device builder aggregates with rule 3 (blocking 3) or 4 (blocking 4) or uses "
                                  "consecutive rows (blocking 0); rules 1 and 2 run on the host (xp_matrix_create)");
     std::lock_guard<std::mutex> lock(xp::set_up_mutex());  // the set-up's work buffers are shared
@@ -387,9 +393,15 @@ std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const i
     a.col_idx = m->d_ci.p;
     a.values = m->d_va.p;
     int units = 0;
-    if (m->block_rows == 8)
+    if (m->block_rows == 8 && given_slots != nullptr) {
+        units = given_units;
+        m->d_slots.alloc(static_cast<size_t>(units) * 8);
+        HIPX(hipMemcpyAsync(m->d_slots.p, given_slots, static_cast<size_t>(units) * 8 * sizeof(int),
+                            hipMemcpyDeviceToDevice, s));
+    } else if (m->block_rows == 8) {
         units = m->blocking >= 3 ? xp::device_aggregate(a, rows, m->d_slots, &m->agg, s, m->blocking == 4 ? 1 : 2)
                                  : xp::device_natural_slots(n, m->d_slots, s);
+    }
     xp::DevSell out;
     xp::device_sell(a, rows, (o != nullptr && o->sigma > 0) ? o->sigma : 256, m->keep_lo, m->keep64, m->block_rows,
                     m->d_slots.p, units, out, s);
@@ -3397,25 +3409,43 @@ int xp_matrix_block_features(const xp_matrix* m, float* feat, float* scale) {
 // ---- multilevel preconditioner ------------------------------------------------
 namespace {
 
-// Work buffers of the hierarchy's set-up, sized by the largest level and
-// reused by every level (allocations are the bulk of a small level's cost).
+// Each aggregate's rows, ascending (natural order of its level).
+struct MemberLists {
+    xp::DevBuf<int> mptr, mem;
+};
+
+// Work buffers of the hierarchy's set-up, sized by the largest level seen
+// and kept between set-ups (allocations and frees are the bulk of a small
+// level's cost); one set, serialised by mg_set_up_mutex().  Two member-list
+// sets: level l + 1 is aggregated before level l's lists are used up.
 struct GalerkinWork {
     xp::DevBuf<double> d;                               // |diagonal|
     xp::DevBuf<unsigned char> strong, state, near;      // per entry / per row
     xp::DevBuf<unsigned long long> m1;
-    xp::DevBuf<int> flag, rid, agg1, lpos, left;        // aggregation
-    xp::DevBuf<int> cnt, mptr, cur, mem;                // member lists (natural order)
+    xp::DevBuf<int> flag, rid, agg1, lpos, left, cnt, cur;  // aggregation
+    MemberLists lists[2];
+    xp::DevBuf<int> pcnt, poff, pslots;                 // smoother blocks cut from aggregates
     xp::DevBuf<int> gcnt, off, dcnt, ccol, crp, cci;    // Galerkin product
     xp::DevBuf<double> cval, cva;
 };
 
 constexpr int kMaxAggRounds = 256;
 
+std::mutex& mg_set_up_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+
+GalerkinWork& mg_work() {
+    static GalerkinWork w;
+    return w;
+}
+
 // Plain aggregation of a natural-order CSR (n rows, nnz entries) on the
-// device into agg; returns the aggregate count.  Leaves the member lists
-// (w.mptr, w.mem) behind.
+// device into agg, with the member lists into L; returns the aggregate
+// count.
 int plain_aggregate(int n, const int* rp, const int* ci, const double* va, size_t nnz, double theta, int salt,
-                    GalerkinWork& w, int* agg, int& rounds, hipStream_t s) {
+                    GalerkinWork& w, MemberLists& L, int* agg, int& rounds, hipStream_t s) {
     const size_t nn = static_cast<size_t>(n);
     w.d.ensure(nn);
     w.strong.ensure(std::max<size_t>(nnz, 1));
@@ -3450,20 +3480,35 @@ int plain_aggregate(int n, const int* rp, const int* ci, const double* va, size_
     HIPX(hipStreamSynchronize(s));
     const int nagg = tot[0] + tot[1];
     w.cnt.ensure(static_cast<size_t>(nagg));
-    w.mptr.ensure(static_cast<size_t>(nagg) + 1);
     w.cur.ensure(static_cast<size_t>(nagg));
-    w.mem.ensure(nn);
+    L.mptr.ensure(static_cast<size_t>(nagg) + 1);
+    L.mem.ensure(nn);
     xp::launch_mg_member_count(n, agg, nagg, w.cnt.p, s);
-    xp::device_scan_exclusive(w.cnt.p, w.mptr.p, nagg, w.mptr.p + nagg, s);
-    xp::launch_mg_member_fill(n, agg, nagg, w.mptr.p, w.cur.p, w.mem.p, s);
+    xp::device_scan_exclusive(w.cnt.p, L.mptr.p, nagg, L.mptr.p + nagg, s);
+    xp::launch_mg_member_fill(n, agg, nagg, L.mptr.p, w.cur.p, L.mem.p, s);
     return nagg;
 }
 
+// 8-row smoother blocks cut from the aggregates of L (w.pslots); returns
+// their count.
+int cut_pieces(int nagg, const MemberLists& L, GalerkinWork& w, hipStream_t s) {
+    w.pcnt.ensure(static_cast<size_t>(nagg));
+    w.poff.ensure(static_cast<size_t>(nagg) + 1);
+    xp::launch_mg_piece_count(nagg, L.mptr.p, w.pcnt.p, s);
+    xp::device_scan_exclusive(w.pcnt.p, w.poff.p, nagg, w.poff.p + nagg, s);
+    int units = 0;
+    HIPX(hipMemcpyAsync(&units, w.poff.p + nagg, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    w.pslots.ensure(static_cast<size_t>(std::max(units, 1)) * 8);
+    xp::launch_mg_piece_fill(nagg, L.mptr.p, L.mem.p, w.poff.p, w.pslots.p, s);
+    return units;
+}
+
 // Galerkin coarse matrix P^T A P of a natural-order CSR over the aggregates
-// agg (member lists in w from plain_aggregate): row u = aggregate u.  The
-// result is left in w.crp / w.cci / w.cva; returns its entry count.
-int galerkin(const int* rp, const int* ci, const double* va, size_t nnz, int nagg, const int* agg, GalerkinWork& w,
-             hipStream_t s) {
+// agg (member lists L): row u = aggregate u.  The result is left in w.crp /
+// w.cci / w.cva; returns its entry count.
+int galerkin(const int* rp, const int* ci, const double* va, size_t nnz, int nagg, const int* agg,
+             const MemberLists& L, GalerkinWork& w, hipStream_t s) {
     const size_t na = static_cast<size_t>(nagg);
     w.gcnt.ensure(na);
     w.off.ensure(na + 1);
@@ -3471,9 +3516,9 @@ int galerkin(const int* rp, const int* ci, const double* va, size_t nnz, int nag
     w.ccol.ensure(std::max<size_t>(nnz, 1));
     w.cval.ensure(std::max<size_t>(nnz, 1));
     w.crp.ensure(na + 1);
-    xp::launch_mg_galerkin_count(nagg, w.mptr.p, w.mem.p, rp, w.gcnt.p, s);
+    xp::launch_mg_galerkin_count(nagg, L.mptr.p, L.mem.p, rp, w.gcnt.p, s);
     xp::device_scan_exclusive(w.gcnt.p, w.off.p, nagg, w.off.p + nagg, s);
-    xp::launch_mg_galerkin_fill(nagg, w.mptr.p, w.mem.p, rp, ci, va, agg, w.off.p, w.ccol.p, w.cval.p, w.dcnt.p, s);
+    xp::launch_mg_galerkin_fill(nagg, L.mptr.p, L.mem.p, rp, ci, va, agg, w.off.p, w.ccol.p, w.cval.p, w.dcnt.p, s);
     xp::device_scan_exclusive(w.dcnt.p, w.crp.p, nagg, w.crp.p + nagg, s);
     int cnnz = 0;
     HIPX(hipMemcpyAsync(&cnnz, w.crp.p + nagg, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -3484,17 +3529,17 @@ int galerkin(const int* rp, const int* ci, const double* va, size_t nnz, int nag
     return cnnz;
 }
 
-// The coarse rows' member lists as solve positions of the finer level
-// (perm: coarse solve -> natural, or nullptr for the dense coarsest level).
-void member_lists(int np, const int* perm, const GalerkinWork& w, const int* inv_fine, int n_fine,
+// The coarse rows' member lists (from L) as solve positions of the finer
+// level (perm: coarse solve -> natural, or nullptr for the dense coarsest
+// level).
+void member_lists(int np, const int* perm, const MemberLists& L, GalerkinWork& w, const int* inv_fine, int n_fine,
                   xp::DevBuf<int>& mp, xp::DevBuf<int>& mpos, hipStream_t s) {
     mp.alloc(static_cast<size_t>(np) + 1);
     mpos.alloc(std::max<size_t>(static_cast<size_t>(n_fine), 1));
-    xp::DevBuf<int> cnt(static_cast<size_t>(std::max(np, 1)));
-    xp::launch_mg_member_solve_count(np, perm, w.mptr.p, cnt.p, s);
-    xp::device_scan_exclusive(cnt.p, mp.p, np, mp.p + np, s);
-    xp::launch_mg_member_solve_pos(np, perm, w.mptr.p, w.mem.p, inv_fine, mp.p, mpos.p, s);
-    HIPX(hipStreamSynchronize(s));  // before cnt goes
+    w.cnt.ensure(static_cast<size_t>(std::max(np, 1)));
+    xp::launch_mg_member_solve_count(np, perm, L.mptr.p, w.cnt.p, s);
+    xp::device_scan_exclusive(w.cnt.p, mp.p, np, mp.p + np, s);
+    xp::launch_mg_member_solve_pos(np, perm, L.mptr.p, L.mem.p, inv_fine, mp.p, mpos.p, s);
 }
 
 void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
@@ -3505,7 +3550,8 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
     if (!m0.device_built || m0.block_rows != 8 || m0.nsys != 1)
         throw std::runtime_error("the multilevel preconditioner needs one system set up on the device with "
                                  "block_rows = 8");
-    if (coarse_blocking != 3 && coarse_blocking != 4) throw std::runtime_error("coarse blocking must be 3 or 4");
+    if (coarse_blocking != 1 && coarse_blocking != 3 && coarse_blocking != 4)
+        throw std::runtime_error("coarse blocking must be 1, 3 or 4");
     if (!S.use_block)
         throw std::runtime_error("install the level-0 block preconditioner (xp_pcg_set_preconditioner / "
                                  "xp_pcg_set_mlp_preconditioner) before xp_pcg_set_multilevel");
@@ -3516,6 +3562,13 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
     const hipStream_t s = device().stream;
     HIPX(hipStreamSynchronize(S.stream));
     auto t0 = std::chrono::steady_clock::now();
+    auto tp = t0;
+    auto lap = [&tp]() {
+        const auto now = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(now - tp).count();
+        tp = now;
+        return ms;
+    };
     auto H = std::make_unique<MgHierarchy>();
     H->omega = static_cast<float>(omega);
     H->gamma = static_cast<float>(gamma);
@@ -3523,55 +3576,48 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
     // level 0's restriction at the lone SpMV's workgroup count (2 per CU:
     // 1% faster than the solver's 256 on config 2)
     H->grid0 = default_grid(m0);
+    std::lock_guard<std::mutex> lock(mg_set_up_mutex());
+    GalerkinWork& work = mg_work();
     const xp_matrix* cur = &m0;
-    GalerkinWork work;
     xp::DevBuf<int> inv_cur(static_cast<size_t>(std::max(m0.n, 1)));  // natural -> solve row of cur
     xp::launch_mg_inv_perm(m0.n_pad, m0.d_perm.p, inv_cur.p, s);
+    H->M.emplace_back();
+    H->r.emplace_back();
+    H->z.emplace_back();
+    H->mp.emplace_back();
+    H->mpos.emplace_back();
+    H->w.emplace_back(static_cast<size_t>(m0.n_pad));
+    H->t.emplace_back(static_cast<size_t>(m0.n_pad));
+    xp::launch_fill_f32(static_cast<size_t>(m0.n_pad), 0.0f, H->w.back().p, s);
+    xp::launch_fill_f32(static_cast<size_t>(m0.n_pad), 0.0f, H->t.back().p, s);
+    // every level is aggregated as soon as its CSR exists (its aggregates
+    // may cut its own smoother blocks); level l's lists are lists[l & 1]
+    H->agg.emplace_back(static_cast<size_t>(std::max(m0.n, 1)));
+    int rounds = 0;
+    int nagg = plain_aggregate(m0.n, m0.d_rp.p, m0.d_ci.p, m0.d_va.p, static_cast<size_t>(m0.nnz), theta, 0, work,
+                               work.lists[0], H->agg.back().p, rounds, s);
+    H->rounds.push_back(rounds);
+    double agg_ms = lap();
     for (int l = 0;; ++l) {
         if (l + 1 >= xp::kMaxLevels) throw std::runtime_error("multilevel: too many levels");
         const int n = cur->n;
         const size_t nnz = static_cast<size_t>(cur->nnz);  // the CSR's entries (device set-up)
-        auto tp = std::chrono::steady_clock::now();
-        auto lap = [&tp]() {
-            const auto now = std::chrono::steady_clock::now();
-            const double ms = std::chrono::duration<double, std::milli>(now - tp).count();
-            tp = now;
-            return ms;
-        };
-        if (l == 0) {
-            H->M.emplace_back();
-            H->r.emplace_back();
-            H->z.emplace_back();
-            H->mp.emplace_back();
-            H->mpos.emplace_back();
-            H->w.emplace_back(static_cast<size_t>(cur->n_pad));
-            H->t.emplace_back(static_cast<size_t>(cur->n_pad));
-            xp::launch_fill_f32(static_cast<size_t>(cur->n_pad), 0.0f, H->w.back().p, s);
-            xp::launch_fill_f32(static_cast<size_t>(cur->n_pad), 0.0f, H->t.back().p, s);
-        }
-        H->agg.emplace_back(static_cast<size_t>(std::max(n, 1)));
-        int rounds = 0;
-        const int nagg = plain_aggregate(n, cur->d_rp.p, cur->d_ci.p, cur->d_va.p, nnz, theta, l, work,
-                                         H->agg.back().p, rounds, s);
-        H->rounds.push_back(rounds);
+        const MemberLists& lists = work.lists[l & 1];
         if (nagg >= n) throw std::runtime_error("multilevel: the aggregation does not coarsen a level (no strong "
                                                 "couplings at this threshold)");
-        H->phase_ms.push_back(lap());
-        const int cnnz = galerkin(cur->d_rp.p, cur->d_ci.p, cur->d_va.p, nnz, nagg, H->agg.back().p, work, s);
+        H->phase_ms.push_back(agg_ms);
+        const int cnnz = galerkin(cur->d_rp.p, cur->d_ci.p, cur->d_va.p, nnz, nagg, H->agg[static_cast<size_t>(l)].p,
+                                  lists, work, s);
         H->phase_ms.push_back(lap());
         if (nagg <= coarse_max) {
             H->nc = nagg;
-            member_lists(nagg, nullptr, work, inv_cur.p, n, H->cmp, H->cmpos, s);
+            member_lists(nagg, nullptr, lists, work, inv_cur.p, n, H->cmp, H->cmpos, s);
             xp::DevBuf<double> dense(2 * static_cast<size_t>(nagg) * static_cast<size_t>(nagg));
             xp::DevBuf<int> bad(1);
             HIPX(hipMemsetAsync(bad.p, 0, sizeof(int), s));
             H->cinv.alloc(static_cast<size_t>(nagg) * static_cast<size_t>(nagg));
             H->ec.alloc(static_cast<size_t>(nagg));
             xp::launch_mg_dense_inverse(nagg, work.crp.p, work.cci.p, work.cva.p, dense.p, H->cinv.p, bad.p, s);
-            int hb = 0;
-            HIPX(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            HIPX(hipStreamSynchronize(s));
-            if (hb != 0) throw std::runtime_error("multilevel: the coarsest matrix is not positive definite");
             H->c_rp.alloc(static_cast<size_t>(nagg) + 1);
             H->c_ci.alloc(std::max<size_t>(static_cast<size_t>(cnnz), 1));
             H->c_va.alloc(std::max<size_t>(static_cast<size_t>(cnnz), 1));
@@ -3581,19 +3627,36 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
                                 hipMemcpyDeviceToDevice, s));
             HIPX(hipMemcpyAsync(H->c_va.p, work.cva.p, static_cast<size_t>(cnnz) * sizeof(double),
                                 hipMemcpyDeviceToDevice, s));
+            int hb = 0;
+            HIPX(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
             HIPX(hipStreamSynchronize(s));
+            if (hb != 0) throw std::runtime_error("multilevel: the coarsest matrix is not positive definite");
             H->phase_ms.push_back(lap());
             break;
         }
+        // level l + 1: its aggregates first (its own coarse space; with
+        // coarse_blocking 1 they also cut its smoother blocks), then its SELL
+        H->agg.emplace_back(static_cast<size_t>(nagg));
+        MemberLists& next_lists = work.lists[(l + 1) & 1];
+        const int nagg_next = plain_aggregate(nagg, work.crp.p, work.cci.p, work.cva.p, static_cast<size_t>(cnnz), theta,
+                                              l + 1, work, next_lists, H->agg.back().p, rounds, s);
+        H->rounds.push_back(rounds);
+        const double agg_next_ms = lap();
         xp_matrix_opts o{};
         o.block_rows = 8;
         o.blocking = nagg <= mo.consecutive_rows ? 0 : coarse_blocking;
         o.sigma = 64;
-        std::unique_ptr<xp_matrix> next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o);
+        std::unique_ptr<xp_matrix> next;
+        if (o.blocking == 1) {
+            const int units = cut_pieces(nagg_next, next_lists, work, s);  // the pieces of level l + 1's aggregates
+            next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o, work.pslots.p, units);
+        } else {
+            next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o);
+        }
         if (next->bad_row >= 0) throw std::runtime_error("multilevel: a coarse matrix has a non-positive diagonal");
         H->mp.emplace_back();
         H->mpos.emplace_back();
-        member_lists(next->n_pad, next->d_perm.p, work, inv_cur.p, n, H->mp.back(), H->mpos.back(), s);
+        member_lists(next->n_pad, next->d_perm.p, lists, work, inv_cur.p, n, H->mp.back(), H->mpos.back(), s);
         const int nqn = next->n_pad / 8;
         xp::DevBuf<float> full(static_cast<size_t>(nqn) * 64);
         xp::launch_block_exact_full(next->view(), nqn, full.p, s);
@@ -3606,12 +3669,13 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
         H->t.emplace_back(np);
         for (xp::DevBuf<float>* v : {&H->r.back(), &H->z.back(), &H->w.back(), &H->t.back()})
             xp::launch_fill_f32(np, 0.0f, v->p, s);
-        inv_cur.ensure(static_cast<size_t>(nagg));
         xp::launch_mg_inv_perm(next->n_pad, next->d_perm.p, inv_cur.p, s);
         HIPX(hipStreamSynchronize(s));  // before the temporaries go
         H->phase_ms.push_back(lap());
         H->mats.push_back(std::move(next));
         cur = H->mats.back().get();
+        nagg = nagg_next;
+        agg_ms = agg_next_ms;
     }
     H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     S.mg = std::move(H);

@@ -1103,13 +1103,14 @@ class Pcg:
     def set_mlp_preconditioner(self, net: "Mlp") -> None:
         _check(_p_set_mlp(self.h, net.h))
 
-    def set_multilevel(self, coarse_max: int = 512, omega: float = 0.8, gamma: float = 1.5,
-                       coarse_blocking: int = 4, consecutive_rows: int = 0, theta: float = 0.08) -> None:
+    def set_multilevel(self, coarse_max: int = 512, omega: float = 0.8, gamma: float = 1.6,
+                       coarse_blocking: int = 1, consecutive_rows: int = 0, theta: float = 0.08) -> None:
         """Multilevel aggregation preconditioner with the installed blocks as
         the level-0 smoother (coarse_max <= 0: off): coarse spaces from plain
-        aggregation at strength threshold theta; coarse_blocking 3 or 4 forms
-        the coarse levels' smoother blocks, those of at most consecutive_rows
-        rows take consecutive rows."""
+        aggregation at strength threshold theta; the coarse levels' smoother
+        blocks are pieces of their own aggregates (coarse_blocking 1) or their
+        strength aggregates (3 or 4), those of at most consecutive_rows rows
+        take consecutive rows."""
         o = MultilevelOpts(coarse_max, omega, gamma, coarse_blocking, consecutive_rows, theta)
         _check(_p_set_ml(self.h, C.byref(o)))
 

@@ -8,7 +8,8 @@ Bars (written in each test):
     product over them, bit for bit (integer work and FP64 sums in a fixed
     order); the coarse levels' smoother blocks equal the aggregation oracle
     (oracle/blocking.py parallel_blocking, distance-1 seeds for
-    coarse_blocking 4) block for block;
+    coarse_blocking 4) or, for coarse_blocking 1, the oracle's pieces of
+    the level's own aggregates, block for block;
   * one V-cycle on the device (FP32 vectors, FP32 values, FP16 block
     factors, FP32 dense coarsest inverse) equals the FP64 oracle cycle over
     the device's hierarchy within 1e-4 relative in the 2-norm (FP32
@@ -88,7 +89,7 @@ def oracle_hierarchy(a, m, s):
     return ML.Hierarchy(levels, coarse, OMEGA, GAMMA)
 
 
-@pytest.mark.parametrize("coarse_blocking,theta", [(3, THETA), (4, THETA), (4, 0.0)])
+@pytest.mark.parametrize("coarse_blocking,theta", [(3, THETA), (4, THETA), (4, 0.0), (1, THETA)])
 def test_hierarchy_matches_oracle(gpu, coarse_blocking, theta):
     a = X.gen_fem2d(130, 1.0, 20, 1e3, 0.3, 20265700)
     m, s = build(a, coarse_max=40, coarse_blocking=coarse_blocking, theta=theta)
@@ -107,9 +108,14 @@ def test_hierarchy_matches_oracle(gpu, coarse_blocking, theta):
         np.testing.assert_array_equal(C.values, va)
         assert C.n == info["n"][lev] and C.nnz == info["nnz"][lev]
         if lev < info["levels"] - 1:
-            groups = BK.parallel_blocking(list(C.row_ptr), list(C.col_idx), list(C.values),
-                                          seed_dist=1 if coarse_blocking == 4 else 2)
-            np.testing.assert_array_equal(s.multilevel_slots(lev), padded(groups))
+            if coarse_blocking == 1:
+                # pieces of the level's own aggregates (the next coarse space)
+                nxt, nn, _ = ML.plain_aggregation(C.row_ptr, C.col_idx, C.values, theta, lev)
+                want_slots = ML.pieces(nxt, nn)
+            else:
+                want_slots = padded(BK.parallel_blocking(list(C.row_ptr), list(C.col_idx), list(C.values),
+                                                         seed_dist=1 if coarse_blocking == 4 else 2))
+            np.testing.assert_array_equal(s.multilevel_slots(lev), want_slots)
         prev = C
     s.close()
     m.close()

@@ -192,6 +192,32 @@ def test_plain_aggregation_brute_force(seed):
         assert (nagg, rounds) == (wn, wr)
 
 
+def test_pieces_partition_each_aggregate():
+    a = X.gen_fem2d(24, 1.0, 3, 1e3, 0.3, 9)
+    A = a.to_scipy().tocsr()
+    agg, nagg, _ = ML.plain_aggregation(A.indptr, A.indices, A.data, 0.0, 2)
+    sl = ML.pieces(agg, nagg)
+    rows = sl[sl >= 0]
+    assert sorted(rows.tolist()) == list(range(a.n))       # a partition
+    sizes = np.bincount(agg)
+    for u, grp in enumerate(sl):
+        g = grp[grp >= 0]
+        assert 1 <= len(g) <= 8 and np.all(np.diff(g) > 0)  # consecutive members, ascending
+        assert np.all(agg[g] == agg[g[0]])                  # within one aggregate
+        assert np.all(grp[len(g):] == -1)
+    # piece sizes of aggregate a: ceil(size / 8) pieces differing by at most one
+    for k in range(nagg):
+        ps = [int((grp >= 0).sum()) for grp in sl if grp[0] >= 0 and agg[grp[0]] == k]
+        assert len(ps) == -(-sizes[k] // 8) and sum(ps) == sizes[k] and max(ps) - min(ps) <= 1
+        assert ps == sorted(ps, reverse=True)
+    # pieces follow aggregate order
+    first = [agg[grp[0]] for grp in sl]
+    assert first == sorted(first)
+    # a hand case: one aggregate of 17 rows -> 6, 6, 5
+    agg17 = np.zeros(17, np.int32)
+    assert [int((g >= 0).sum()) for g in ML.pieces(agg17, 1)] == [6, 6, 5]
+
+
 def test_galerkin_matches_scipy_product():
     a = X.gen_fem2d(30, 1.0, 3, 1e3, 0.3, 7)
     A = a.to_scipy().tocsr()
