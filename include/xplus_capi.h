@@ -295,7 +295,8 @@ typedef struct {
     double rel_tol;    /* if > 0: tau_k = (rel_tol * ||b_k||)^2 per system */
     int max_iters;     /* 0 -> max(4 n_k, 1000) (src/CgSolverGolden.hpp:47) */
     int graph_iters;   /* iterations per graph launch; 0 = 64 */
-    int use_graph;     /* 0: launch kernels directly (debugging) */
+    int use_graph;     /* 0: launch kernels directly, graph_iters per host check (short solves: no graph
+                          to capture and instantiate) */
     int trace_cap;     /* trace rows kept per system */
     int stall_limit;   /* 0 = off; else stop with XP_STAGNATION after this many
                           iterations without halving the best r^T r */
