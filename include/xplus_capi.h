@@ -252,6 +252,14 @@ int xp_bitmap_spmv_time(int n, int window_size, const int* row_ptr, const int* c
  * stream. */
 int xp_device_alloc(unsigned long long bytes, void** out);
 int xp_device_free(void* ptr);
+/* The library's own device buffers (matrices, solvers, hierarchies) come
+ * from a caching allocator (no reference counterpart): released blocks of
+ * at least 256 KB are kept and reused, after one device synchronisation,
+ * instead of going back to hipFree (a hipMalloc / hipFree pair of a few MB
+ * costs 0.2-0.4 ms).  xp_device_pool_trim returns every cached block to the
+ * runtime; xp_device_pool_bytes reports how many bytes are cached. */
+int xp_device_pool_trim(void);
+int xp_device_pool_bytes(unsigned long long* bytes);
 int xp_memcpy_htod(void* dst, const void* src, unsigned long long bytes);
 int xp_memcpy_dtoh(void* dst, const void* src, unsigned long long bytes);
 /* Device-to-device copy (the runtime's copy kernel): a transfer of known

@@ -22,7 +22,17 @@ inline void hip_check(hipError_t e, const char* what) {
 }
 #define HIPX(expr) ::xp::hip_check((expr), #expr)
 
-// Owning device allocation of `n` elements of T (hipMalloc / hipFree).
+// The library's caching device allocator (device_pool.cpp): blocks of at
+// least 256 KB are kept after release and reused (a released block is
+// reused only after a device synchronisation), smaller ones go straight to
+// hipMalloc / hipFree.  pool_trim returns every cached block to the runtime.
+void* pool_alloc(size_t bytes);
+void pool_free(void* p, size_t bytes);
+void pool_trim();
+size_t pool_cached_bytes();
+
+// Owning device allocation of `n` elements of T (through the caching
+// allocator).
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -40,12 +50,12 @@ struct DevBuf {
     void alloc(size_t count) {
         release();
         if (count == 0) return;
-        HIPX(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+        p = static_cast<T*>(pool_alloc(count * sizeof(T)));
         n = count;
     }
     void ensure(size_t count) { if (count > n) alloc(std::max(count, n + n / 2)); }
     void release() {
-        if (p != nullptr) (void)hipFree(p);
+        if (p != nullptr) pool_free(p, n * sizeof(T));
         p = nullptr;
         n = 0;
     }

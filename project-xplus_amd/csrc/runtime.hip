@@ -1372,6 +1372,17 @@ int xp_device_free(void* ptr) {
     });
 }
 
+int xp_device_pool_trim(void) {
+    return guarded([&] { xp::pool_trim(); });
+}
+
+int xp_device_pool_bytes(unsigned long long* bytes) {
+    return guarded([&] {
+        if (bytes == nullptr) throw std::runtime_error("null argument");
+        *bytes = static_cast<unsigned long long>(xp::pool_cached_bytes());
+    });
+}
+
 int xp_memcpy_htod(void* dst, const void* src, unsigned long long bytes) {
     return guarded([&] {
         const hipStream_t s = device().stream;

@@ -114,6 +114,8 @@ _htod = _sig("xp_memcpy_htod", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dtoh = _sig("xp_memcpy_dtoh", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dtod = _sig("xp_memcpy_dtod", C.c_int, [_vp, _vp, C.c_ulonglong])
 _dev_sync = _sig("xp_device_synchronize", C.c_int, [])
+_pool_trim = _sig("xp_device_pool_trim", C.c_int, [])
+_pool_bytes = _sig("xp_device_pool_bytes", C.c_int, [C.POINTER(C.c_ulonglong)])
 _op_init = _sig("xp_init_pcg_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
 _op_dot = _sig("xp_dot_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp])
 _op_xrz = _sig("xp_update_xrz_f32", C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp])
@@ -754,6 +756,18 @@ class DeviceCsr:
     def close(self) -> None:
         for k in ("row_ptr", "col_idx", "values", "b"):
             getattr(self, k).close()
+
+
+def pool_trim() -> None:
+    """Return the library's cached device blocks to the runtime."""
+    _check(_pool_trim())
+
+
+def pool_bytes() -> int:
+    """Bytes of device memory the library's allocator holds for reuse."""
+    v = C.c_ulonglong()
+    _check(_pool_bytes(C.byref(v)))
+    return v.value
 
 
 def synchronize() -> None:

@@ -48,3 +48,28 @@ def test_kernel_times_split_accurate_blocks(gpu):
     assert rel[0] <= 1e-6 and iters > 0
     s.close()
     m.close()
+
+
+def test_device_pool_reuses_and_trims(gpu):
+    """The caching allocator (no reference counterpart): a solve on a
+    matrix whose buffers reuse cached blocks gives the same bits as the first
+    one; closing keeps blocks cached; trimming returns them all."""
+    a = X.gen_fem2d(200, 1.0, 20, 1e3, 0.3, 20261978)
+    outs = []
+    for k in range(3):
+        m = X.Matrix(a, keep_lo=True, block_rows=8, blocking=3)
+        s = X.Pcg(m, rel_tol=1e-6, accurate=True)
+        s.set_rhs(a.b, a.x0)
+        s.set_preconditioner(1)
+        rel, _, iters = s.run_reliable(1e-6, 0.0)
+        outs.append((iters, s.x()))
+        s.close()
+        m.close()
+        X.synchronize()
+        assert X.pool_bytes() > 0
+        if k == 1:
+            X.pool_trim()
+            assert X.pool_bytes() == 0
+    for iters, x in outs[1:]:
+        assert iters == outs[0][0]
+        np.testing.assert_array_equal(x, outs[0][1])
