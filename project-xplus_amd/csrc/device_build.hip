@@ -264,7 +264,7 @@ __global__ void k_seed_pick(int n, const int* __restrict__ rp, const int* __rest
 // running sums, in LDS (kBidCap per thread): every block's sum takes its
 // entries in CSR order, as the definition says.  A row that meets more than
 // kBidCap blocks with room scans its entries once per block instead.
-constexpr int kBidCap = 16;
+constexpr int kBidCap = 8;
 __global__ __launch_bounds__(kRowThreads) void k_grow_bid(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                            const double* __restrict__ va, const double* __restrict__ dis,
                            const double* __restrict__ t, const uint8_t* __restrict__ elig,
