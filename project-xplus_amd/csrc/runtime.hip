@@ -3441,6 +3441,7 @@ struct GalerkinWork {
 };
 
 constexpr int kMaxAggRounds = 256;
+constexpr int kRoundBatch = 4;
 
 std::mutex& mg_set_up_mutex() {
     static std::mutex mu;
@@ -3467,18 +3468,28 @@ int plain_aggregate(int n, const int* rp, const int* ci, const double* va, size_
     w.rid.ensure(nn + 1);
     w.agg1.ensure(nn);
     w.lpos.ensure(nn + 1);
-    w.left.ensure(1);
+    w.left.ensure(kMaxAggRounds);
     xp::launch_mg_absdiag(n, rp, ci, va, w.d.p, s);
     xp::launch_mg_strength(n, rp, ci, va, w.d.p, theta, w.strong.p, s);
     HIPX(hipMemsetAsync(w.state.p, 0, nn, s));
+    // rounds in batches of kRoundBatch per host check: a round after the
+    // last undecided row is gone changes nothing, and each round leaves its
+    // own count (left[k]), so the count of rounds stays exact
     rounds = 0;
-    for (;;) {
-        if (++rounds > kMaxAggRounds) throw std::runtime_error("multilevel: the aggregation did not finish");
-        xp::launch_mg_mis_round(n, rp, ci, w.strong.p, salt, w.state.p, w.m1.p, w.near.p, w.left.p, s);
-        int left = 0;
-        HIPX(hipMemcpyAsync(&left, w.left.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    for (int k0 = 0;; k0 += kRoundBatch) {
+        if (k0 + kRoundBatch > kMaxAggRounds) throw std::runtime_error("multilevel: the aggregation did not finish");
+        for (int k = k0; k < k0 + kRoundBatch; ++k)
+            xp::launch_mg_mis_round(n, rp, ci, w.strong.p, salt, w.state.p, w.m1.p, w.near.p, w.left.p + k, s);
+        int left[kRoundBatch];
+        HIPX(hipMemcpyAsync(left, w.left.p + k0, sizeof left, hipMemcpyDeviceToHost, s));
         HIPX(hipStreamSynchronize(s));
-        if (left == 0) break;
+        int done = -1;
+        for (int k = 0; k < kRoundBatch && done < 0; ++k)
+            if (left[k] == 0) done = k;
+        if (done >= 0) {
+            rounds = k0 + done + 1;
+            break;
+        }
     }
     xp::launch_mg_flag_roots(n, w.state.p, w.flag.p, s);
     xp::device_scan_exclusive(w.flag.p, w.rid.p, n, w.rid.p + n, s);
