@@ -1848,12 +1848,24 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_smooth(Launch L, const SysStat
 }
 
 // The restriction's sum for coarse row `row`: the finer level's w over the
-// row's aggregate, in list order (padding rows have empty lists: 0).
+// row's aggregate, in list order (padding rows have empty lists: 0).  Eight
+// members at a time: their positions, then their values, are loaded before
+// the adds (the adds keep list order).
 __device__ __forceinline__ float gather_members(const int* __restrict__ mp, const int* __restrict__ mpos,
                                                 const float* __restrict__ wf, int row) {
     float s = 0.0f;
     const int e = mp[row + 1];
-    for (int k = mp[row]; k < e; ++k) s += wf[mpos[k]];
+    for (int k = mp[row]; k < e; k += 8) {
+        int idx[8];
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) idx[u] = k + u < e ? mpos[k + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = idx[u] >= 0 ? wf[idx[u]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k + u < e) s += v[u];
+    }
     return s;
 }
 
@@ -1881,9 +1893,16 @@ __global__ __launch_bounds__(kBlock, 8) void k_mg_smooth_gather(Launch L, const 
 // t = cz z + gamma e.
 __device__ __forceinline__ void prolong_members(const MgUp& pu, int row, float gamma, float e) {
     const int end = pu.mp[row + 1];
-    for (int k = pu.mp[row]; k < end; ++k) {
-        const int i = pu.mpos[k];
-        pu.t[i] = fmaf(gamma, e, pu.cz * pu.z[i]);
+    for (int k = pu.mp[row]; k < end; k += 8) {  // eight members' loads in flight at a time
+        int idx[8];
+        float zv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) idx[u] = k + u < end ? pu.mpos[k + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) zv[u] = idx[u] >= 0 ? pu.z[idx[u]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (idx[u] >= 0) pu.t[idx[u]] = fmaf(gamma, e, pu.cz * zv[u]);
     }
 }
 
