@@ -3441,7 +3441,7 @@ struct GalerkinWork {
 };
 
 constexpr int kMaxAggRounds = 256;
-constexpr int kRoundBatch = 4;
+constexpr int kRoundBatch = 3;
 
 std::mutex& mg_set_up_mutex() {
     static std::mutex mu;
