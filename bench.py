@@ -680,8 +680,9 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
         m.close()
         return (t1 - t0) * 1000.0, (t2 - t1) * 1000.0
 
-    for _ in range(args.warmup):
-        one()
+    # warm-up steps run the whole path too; the first of a process also pays
+    # the device allocations the library's caching allocator then reuses
+    warm = [one() for _ in range(args.warmup)]
     tt = [one() for _ in range(args.steps)]
     x = xout.numpy()
     xout.close()
@@ -690,6 +691,7 @@ def time_to_solution(X, d, a, name, layout, args, net=None, fused_refined=True) 
     st = info["status"]
     res = float(np.linalg.norm(a.b - X.csr_spmv_f64(a, x)) / np.linalg.norm(a.b))
     out = {"total_ms": round(setup + solve, 3), "setup_ms": round(setup, 3), "solve_ms": round(solve, 3),
+           "warmup_total_ms": [round(v[0] + v[1], 3) for v in warm],
            "iterations": st["iterations"], "status": X.STATUS_NAMES[st["status"]], "true_rel_residual": res,
            "layout": {k: layout[k] for k in ("sigma", "spmv_variant", "grid")}}
     if run == "refined":
