@@ -783,18 +783,22 @@ std::mutex& set_up_mutex() {
     return mu;
 }
 
-void device_rows(DevCsr& a, DevRows& rows, hipStream_t s) {
+void device_rows(DevCsr& a, DevRows& rows, hipStream_t s, int trusted_nnz) {
     const int n = a.n;
     if (n <= 0) throw std::runtime_error("n must be positive");
     if (a.row_ptr == nullptr || a.col_idx == nullptr || a.values == nullptr)
         throw std::runtime_error("build_sell: null CSR array");
-    int ends[2] = {0, 0};
-    HIPX(hipMemcpyAsync(&ends[0], a.row_ptr, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPX(hipMemcpyAsync(&ends[1], a.row_ptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPX(hipStreamSynchronize(s));
-    if (ends[0] != 0) throw std::runtime_error("row_ptr[0] must be 0");
-    if (ends[1] < 0) throw std::runtime_error("row_ptr must be nondecreasing");
-    a.nnz = ends[1];
+    if (trusted_nnz < 0) {
+        int ends[2] = {0, 0};
+        HIPX(hipMemcpyAsync(&ends[0], a.row_ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(&ends[1], a.row_ptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        if (ends[0] != 0) throw std::runtime_error("row_ptr[0] must be 0");
+        if (ends[1] < 0) throw std::runtime_error("row_ptr must be nondecreasing");
+        a.nnz = ends[1];
+    } else {
+        a.nnz = trusted_nnz;
+    }
     rows.offcnt = take<int>(kWOffcnt, static_cast<size_t>(n));
     rows.diag_pos = take<int>(kWDiagPos, static_cast<size_t>(n));
     rows.dis = take<double>(kWDis, static_cast<size_t>(n));
@@ -804,11 +808,13 @@ void device_rows(DevCsr& a, DevRows& rows, hipStream_t s) {
     const dim3 g(blocks_of(n, kRowThreads)), b(kRowThreads);
     hipLaunchKernelGGL(k_row_info, g, b, 0, s, n, a.nnz, a.row_ptr, a.col_idx, a.values, rows.offcnt,
                        rows.diag_pos, rows.dis, err);
-    int e = 0;
-    HIPX(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPX(hipStreamSynchronize(s));
-    if (e & kErrRowPtr) throw std::runtime_error("row_ptr must be nondecreasing");
-    if (e & kErrCol) throw std::runtime_error("col_idx out of range");
+    if (trusted_nnz < 0) {
+        int e = 0;
+        HIPX(hipMemcpyAsync(&e, err, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        if (e & kErrRowPtr) throw std::runtime_error("row_ptr must be nondecreasing");
+        if (e & kErrCol) throw std::runtime_error("col_idx out of range");
+    }
     hipLaunchKernelGGL(k_totals, g, b, 0, s, n, a.row_ptr, a.col_idx, a.values, rows.dis, rows.total);
     HIPX(hipGetLastError());
 }

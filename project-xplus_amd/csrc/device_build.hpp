@@ -65,7 +65,10 @@ std::mutex& set_up_mutex();
 // Validates the CSR on the device (throws the host builder's messages:
 // "row_ptr[0] must be 0", "row_ptr must be nondecreasing", "col_idx out of
 // range") and fills `rows`; sets a.nnz from row_ptr[n].
-void device_rows(DevCsr& a, DevRows& rows, hipStream_t s);
+// trusted_nnz >= 0: a CSR this library produced with that many entries
+// (the multilevel hierarchy's Galerkin products): the checks and their two
+// host round trips are skipped.
+void device_rows(DevCsr& a, DevRows& rows, hipStream_t s, int trusted_nnz = -1);
 
 struct AggStats {
     int rounds = 0;          // seed rounds over all rows

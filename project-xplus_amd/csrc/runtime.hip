@@ -365,7 +365,7 @@ constexpr int kBlockingGiven = -1;
 // made only on demand (ensure_host_mirror).
 std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const int* col_idx, const double* values,
                                               const xp_matrix_opts* o, const int* given_slots = nullptr,
-                                              int given_units = 0) {
+                                              int given_units = 0, int trusted_nnz = -1) {
     const hipStream_t s = device().stream;
     auto m = std::make_unique<xp_matrix>();
     m->keep_lo = o != nullptr && o->keep_lo;
@@ -380,7 +380,7 @@ std::unique_ptr<xp_matrix> make_matrix_device(int n, const int* row_ptr, const i
     std::lock_guard<std::mutex> lock(xp::set_up_mutex());  // the set-up's work buffers are shared
     xp::DevCsr a{n, 0, row_ptr, col_idx, values};
     xp::DevRows rows;
-    xp::device_rows(a, rows, s);
+    xp::device_rows(a, rows, s, trusted_nnz);
     // the matrix keeps its own copy of the CSR (the caller's may go away)
     const size_t nnz = static_cast<size_t>(a.nnz);
     m->d_rp.alloc(static_cast<size_t>(n) + 1);
@@ -3671,9 +3671,9 @@ void build_multilevel(xp_pcg& S, const xp_multilevel_opts& mo) {
         std::unique_ptr<xp_matrix> next;
         if (o.blocking == 1) {
             const int units = cut_pieces(nagg_next, next_lists, work, s);  // the pieces of level l + 1's aggregates
-            next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o, work.pslots.p, units);
+            next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o, work.pslots.p, units, cnnz);
         } else {
-            next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o);
+            next = make_matrix_device(nagg, work.crp.p, work.cci.p, work.cva.p, &o, nullptr, 0, cnnz);
         }
         if (next->bad_row >= 0) throw std::runtime_error("multilevel: a coarse matrix has a non-positive diagonal");
         H->mp.emplace_back();
